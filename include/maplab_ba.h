@@ -1,0 +1,243 @@
+/*
+ * maplab_ba.h -- C ABI of the MI355X visual bundle-adjustment library
+ * (libmaplab_ba.so).
+ *
+ * The library replaces the part of maplab's `optv` / `optvi` path that runs
+ * inside `ceres::Solve`:
+ *   - VisualReprojectionError::Evaluate for every keypoint observation
+ *     (algorithms/ceres-error-terms/include/ceres-error-terms/
+ *      visual-error-term-inl.h:13-379), pinhole camera with no / radtan /
+ *     equidistant distortion (aslam_cv2/aslam_cv_cameras/src/
+ *     camera-pinhole.cc:65-164, distortion-radtan.cc:14-91,
+ *     distortion-equidistant.cc:14-142);
+ *   - the Levenberg-Marquardt trust-region loop that ceres::Solve runs with
+ *     the options of algorithms/map-optimization/include/map-optimization/
+ *     solver-options.h:26-74, with landmark (Schur) elimination and the
+ *     reduced-camera-system solve;
+ *   - the chunked outlier-rejection driver
+ *     (algorithms/map-optimization/src/outlier-rejection-solver.cc:28-259).
+ *
+ * Conventions (identical to maplab's OptimizationStateBuffer,
+ * algorithms/map-optimization/src/optimization-state-buffer.cc:153-306):
+ *   vertex block    = [qx qy qz qw | px py pz], JPL q_IM and M_p_MI
+ *   baseframe block = [q_GM (JPL) | G_p_GM]
+ *   camera block    = [q_CI (JPL) | C_p_CI]
+ *   landmark        = p_B, expressed in the frame of its store vertex.
+ * Quaternions are unit, w >= 0.
+ *
+ * Ownership: the caller owns every host buffer passed in; the library copies
+ * what it needs into device memory at mlba_create and owns device memory
+ * until mlba_destroy. One handle may be used by one host thread at a time;
+ * different handles are independent. Every entry point returns an MLBA_*
+ * status code and never aborts; the text of the last failure is available
+ * from mlba_last_error.
+ */
+#ifndef MAPLAB_BA_H_
+#define MAPLAB_BA_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MLBA_ABI_VERSION 1
+
+/* Status codes. */
+#define MLBA_OK 0
+#define MLBA_EINVAL -1   /* invalid argument (sigma <= 0, bad index, ...) */
+#define MLBA_ENOMEM -2   /* device or host allocation failed */
+#define MLBA_EDEVICE -3  /* HIP runtime error */
+#define MLBA_ESTATE -4   /* state violates an invariant (non-unit quaternion) */
+
+/* Distortion models (aslam::Distortion::Type subset). */
+#define MLBA_DISTORTION_NONE 0
+#define MLBA_DISTORTION_RADTAN 1
+#define MLBA_DISTORTION_EQUIDISTANT 2
+
+/* ceres_error_terms::LandmarkErrorType (common.h:14). */
+#define MLBA_LOCAL_KEYFRAME 0
+#define MLBA_LOCAL_MISSION 1
+#define MLBA_GLOBAL 2
+
+/* Robust loss applied to the sigma-whitened residual. */
+#define MLBA_LOSS_MAPLAB_HUBER 0 /* ceres::HuberLoss(delta*sigma), delta 3 (local)
+                                    or 10 (global): optimization-terms-addition.cc:189-198 */
+#define MLBA_LOSS_CAUCHY 1       /* ceres::CauchyLoss(scale_px / sigma) */
+#define MLBA_LOSS_TRIVIAL 2      /* no loss */
+
+/* Linear solver for the LM step. All solve (J'J + D'D) y = J'r. */
+#define MLBA_SOLVER_SCHUR_PCG 0 /* landmark Schur elimination + block-Jacobi PCG
+                                   on the reduced camera system (device) */
+
+/* ceres::TerminationType */
+#define MLBA_CONVERGENCE 0
+#define MLBA_NO_CONVERGENCE 1
+#define MLBA_FAILURE 2
+#define MLBA_USER_SUCCESS 3
+#define MLBA_USER_FAILURE 4
+
+/* Evaluation flags for mlba_evaluate. */
+#define MLBA_EVAL_AMBIENT 0 /* Jacobians as VisualReprojectionError::Evaluate writes them */
+#define MLBA_EVAL_TANGENT 1 /* quaternion columns lifted by the JPL local parameterization */
+
+/*
+ * A map in structure-of-arrays form. Replaces the VIMap + OptimizationStateBuffer
+ * pair that constructOptimizationProblem reads
+ * (algorithms/map-optimization/src/vi-optimization-builder.cc:154-374).
+ * Observation i is keypoint obs_keypoint[2i..2i+1] of camera obs_camera[i]
+ * on vertex obs_vertex[i], associated to landmark obs_landmark[i]. The error
+ * type of an observation follows optimization-terms-addition.cc:68-78:
+ * observer == store vertex -> local keyframe; same mission -> local mission;
+ * otherwise global.
+ */
+typedef struct mlba_problem_desc {
+  int32_t num_vertices;
+  int32_t num_landmarks;
+  int32_t num_observations;
+  int32_t num_cameras;
+  int32_t num_missions;
+  int32_t reserved0;
+  const double* vertex_q_IM__M_p_MI;      /* 7 * num_vertices */
+  const double* baseframe_q_GM__G_p_GM;   /* 7 * num_missions */
+  const double* landmark_p_B;             /* 3 * num_landmarks */
+  const double* camera_q_CI__C_p_CI;      /* 7 * num_cameras */
+  const double* camera_intrinsics;        /* 4 * num_cameras: fu fv cu cv */
+  const double* camera_distortion;        /* 4 * num_cameras */
+  const int32_t* camera_distortion_type;  /* num_cameras, MLBA_DISTORTION_* */
+  const int32_t* camera_resolution;       /* 2 * num_cameras: width height */
+  const int32_t* vertex_mission;          /* num_vertices */
+  const int32_t* landmark_store_vertex;   /* num_landmarks */
+  const int32_t* obs_vertex;              /* num_observations */
+  const int32_t* obs_landmark;            /* num_observations */
+  const int32_t* obs_camera;              /* num_observations */
+  const double* obs_keypoint;             /* 2 * num_observations, pixels */
+  const double* obs_sigma;                /* num_observations, pixels, > 0 */
+  const uint8_t* vertex_constant;         /* num_vertices, nullable (all variable) */
+  const uint8_t* landmark_constant;       /* num_landmarks, nullable (all variable) */
+} mlba_problem_desc;
+
+/*
+ * Solver options: the subset of ceres::Solver::Options that
+ * initSolverOptionsFromFlags sets (solver-options.h:26-74) plus the outlier
+ * rejection options (outlier-rejection-solver.h:15-27). mlba_options_init
+ * fills maplab's flag defaults (solver-options.cc:3-52).
+ */
+typedef struct mlba_options {
+  int32_t max_num_iterations;             /* --ba_num_iterations = 30 */
+  int32_t use_jacobi_scaling;             /* --ba_use_jacobi_scaling = 1 */
+  double max_solver_time_in_seconds;      /* --ba_max_time_seconds = 1e6 */
+  double function_tolerance;              /* 1e-12 */
+  double gradient_tolerance;              /* 1e-10 */
+  double parameter_tolerance;             /* 1e-8 */
+  double initial_trust_region_radius;     /* 1e5 */
+  double max_trust_region_radius;         /* 1e20 */
+  double min_trust_region_radius;         /* 1e-32 */
+  double min_relative_decrease;           /* ceres default 1e-3 */
+  double min_lm_diagonal;                 /* ceres default 1e-6 */
+  double max_lm_diagonal;                 /* ceres default 1e32 */
+  int32_t max_num_consecutive_invalid_steps; /* ceres default 5 */
+  int32_t loss_type;                      /* MLBA_LOSS_* (default MAPLAB_HUBER) */
+  double loss_scale_px;                   /* Cauchy scale in pixels (default 1) */
+  int32_t linear_solver_type;             /* MLBA_SOLVER_* */
+  int32_t pcg_max_iterations;             /* default 500 */
+  double pcg_relative_tolerance;          /* ||r_k|| / ||b||, default 1e-10 */
+  /* outlier rejection (outlier-rejection-solver.cc:9-18) */
+  int32_t reject_outliers_every_n_iters;  /* 3 */
+  int32_t use_reprojection_error;         /* 0 */
+  double max_reprojection_error_px;       /* 50 */
+  /* device launch tuning (0 = library default / autotuned) */
+  int32_t obs_tile;                       /* observations per K1 tile */
+  int32_t schur_row_chunk;                /* S blocks per LDS accumulation chunk */
+} mlba_options;
+
+/* One row of the Ceres iteration log (ceres::IterationSummary subset). */
+typedef struct mlba_iteration_summary {
+  int32_t iteration;
+  int32_t step_is_valid;
+  int32_t step_is_successful;
+  int32_t linear_solver_iterations;
+  double cost;
+  double cost_change;
+  double gradient_max_norm;
+  double step_norm;
+  double relative_decrease;
+  double trust_region_radius;
+  double iteration_time_in_seconds;
+  double cumulative_time_in_seconds;
+} mlba_iteration_summary;
+
+typedef struct mlba_summary {
+  int32_t termination_type;               /* MLBA_CONVERGENCE ... */
+  int32_t num_iterations;                 /* rows written (iteration 0 included) */
+  int32_t num_successful_steps;
+  int32_t num_unsuccessful_steps;
+  int32_t num_residual_blocks;            /* active observations */
+  int32_t num_removed_landmarks;          /* outlier rejection */
+  double initial_cost;
+  double final_cost;
+  double fixed_cost;                      /* cost of residuals with no variable block */
+  double lm_loop_seconds;                 /* wall time of the LM loop (no setup) */
+  double evaluate_seconds;                /* residual+Jacobian+normal-equation passes */
+  double linear_solver_seconds;
+  int64_t total_linear_solver_iterations;
+  int32_t iterations_capacity;            /* caller-set capacity of `iterations` */
+  int32_t reserved0;
+  mlba_iteration_summary* iterations;     /* caller-owned, nullable */
+} mlba_summary;
+
+typedef struct mlba_handle mlba_handle;
+
+int32_t mlba_abi_version(void);
+
+/* Fill maplab's flag defaults (solver-options.cc:3-52). */
+void mlba_options_init(mlba_options* options);
+
+/* Copy the map to the device and build the observation / Schur topology.
+ * Replaces constructOptimizationProblem + buildCeresProblemFromProblemInformation
+ * (vi-optimization-builder.cc:154-374, problem-information.cc:22-97). */
+int32_t mlba_create(const mlba_problem_desc* desc, mlba_handle** out_handle);
+int32_t mlba_destroy(mlba_handle* handle);
+
+/* Last error text for `handle` (or for the last failed mlba_create when NULL). */
+const char* mlba_last_error(const mlba_handle* handle);
+
+/* Replace / read back the optimized state (copyAllStatesBackToMap,
+ * optimization-state-buffer.cc:42-151). Either pointer may be NULL. */
+int32_t mlba_upload_state(mlba_handle* handle, const double* vertex_q_IM__M_p_MI,
+                          const double* landmark_p_B);
+int32_t mlba_download_state(const mlba_handle* handle, double* vertex_q_IM__M_p_MI,
+                            double* landmark_p_B);
+
+/* Deactivate / reactivate landmarks (ProblemInformation::deactivateCostFunction
+ * for all observations of a landmark, outlier-rejection-solver.cc:168-177).
+ * mask[l] != 0 means active. */
+int32_t mlba_set_landmark_active(mlba_handle* handle, const uint8_t* mask);
+int32_t mlba_get_landmark_active(const mlba_handle* handle, uint8_t* mask);
+
+/*
+ * Batched VisualReprojectionError::Evaluate over every observation, in the
+ * caller's observation order. residuals: 2N (may be NULL). jacobians: array
+ * of 9 nullable pointers in the reference block order
+ * {p_B, anchor pose, landmark baseframe, observer baseframe, observer pose,
+ *  q_CI, p_CI, intrinsics, distortion}; block k of observation i is written
+ * row-major at jacobians[k] + i * 2 * size_k, with size_k = {3,7,7,7,7,4,3,4,4}
+ * (ambient) or {3,6,6,6,6,3,3,4,4} (tangent). Blocks the error type does not
+ * use are written as zeros. Failed projections give zero residual and
+ * Jacobians (visual-error-term-inl.h:155-178). cost (nullable) receives
+ * 0.5 * sum rho(|r|^2) over active observations with the options' loss.
+ */
+int32_t mlba_evaluate(mlba_handle* handle, int32_t flags, const mlba_options* options,
+                      double* residuals, double* const* jacobians, double* cost);
+
+/* ceres::Solve on the visual problem (map_optimization::solve, solver.cc:14-44). */
+int32_t mlba_solve(mlba_handle* handle, const mlba_options* options, mlba_summary* summary);
+
+/* map_optimization::solveWithOutlierRejection (outlier-rejection-solver.cc:206-259). */
+int32_t mlba_solve_with_outlier_rejection(mlba_handle* handle, const mlba_options* options,
+                                          mlba_summary* summary);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MAPLAB_BA_H_ */

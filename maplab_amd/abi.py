@@ -1,0 +1,178 @@
+"""ctypes view of the C ABI in include/maplab_ba.h and include/maplab_ba_synth.h.
+
+The structures mirror the C declarations field for field. `load_library()`
+loads the HIP library built in-tree (maplab_amd/libmaplab_ba.so) and raises
+if it is missing: there is no CPU fallback for the solver.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmaplab_ba.so")
+SYNTH_PATH = os.path.join(_HERE, "libmaplab_ba_synth.so")
+
+c_i32 = ctypes.c_int32
+c_f64 = ctypes.c_double
+c_u64 = ctypes.c_uint64
+c_vp = ctypes.c_void_p
+
+MLBA_OK = 0
+MLBA_EINVAL = -1
+MLBA_ENOMEM = -2
+MLBA_EDEVICE = -3
+MLBA_ESTATE = -4
+
+DIST_NONE, DIST_RADTAN, DIST_EQUIDISTANT = 0, 1, 2
+LOCAL_KEYFRAME, LOCAL_MISSION, GLOBAL = 0, 1, 2
+LOSS_MAPLAB_HUBER, LOSS_CAUCHY, LOSS_TRIVIAL = 0, 1, 2
+SOLVER_SCHUR_PCG = 0
+CONVERGENCE, NO_CONVERGENCE, FAILURE, USER_SUCCESS, USER_FAILURE = 0, 1, 2, 3, 4
+EVAL_AMBIENT, EVAL_TANGENT = 0, 1
+
+SYNTH_CIRCLE, SYNTH_RANDOM_WALK = 0, 1
+ANCHOR_FIRST_OBSERVER, ANCHOR_GLOBAL = 0, 1
+
+AMBIENT_BLOCK_SIZES = (3, 7, 7, 7, 7, 4, 3, 4, 4)
+TANGENT_BLOCK_SIZES = (3, 6, 6, 6, 6, 3, 3, 4, 4)
+
+
+class ProblemDesc(ctypes.Structure):
+    _fields_ = [
+        ("num_vertices", c_i32), ("num_landmarks", c_i32), ("num_observations", c_i32),
+        ("num_cameras", c_i32), ("num_missions", c_i32), ("reserved0", c_i32),
+        ("vertex_q_IM__M_p_MI", c_vp), ("baseframe_q_GM__G_p_GM", c_vp), ("landmark_p_B", c_vp),
+        ("camera_q_CI__C_p_CI", c_vp), ("camera_intrinsics", c_vp), ("camera_distortion", c_vp),
+        ("camera_distortion_type", c_vp), ("camera_resolution", c_vp), ("vertex_mission", c_vp),
+        ("landmark_store_vertex", c_vp), ("obs_vertex", c_vp), ("obs_landmark", c_vp),
+        ("obs_camera", c_vp), ("obs_keypoint", c_vp), ("obs_sigma", c_vp),
+        ("vertex_constant", c_vp), ("landmark_constant", c_vp),
+    ]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("max_num_iterations", c_i32), ("use_jacobi_scaling", c_i32),
+        ("max_solver_time_in_seconds", c_f64), ("function_tolerance", c_f64),
+        ("gradient_tolerance", c_f64), ("parameter_tolerance", c_f64),
+        ("initial_trust_region_radius", c_f64), ("max_trust_region_radius", c_f64),
+        ("min_trust_region_radius", c_f64), ("min_relative_decrease", c_f64),
+        ("min_lm_diagonal", c_f64), ("max_lm_diagonal", c_f64),
+        ("max_num_consecutive_invalid_steps", c_i32), ("loss_type", c_i32),
+        ("loss_scale_px", c_f64), ("linear_solver_type", c_i32), ("pcg_max_iterations", c_i32),
+        ("pcg_relative_tolerance", c_f64), ("reject_outliers_every_n_iters", c_i32),
+        ("use_reprojection_error", c_i32), ("max_reprojection_error_px", c_f64),
+        ("obs_tile", c_i32), ("schur_row_chunk", c_i32),
+    ]
+
+
+class IterationSummary(ctypes.Structure):
+    _fields_ = [
+        ("iteration", c_i32), ("step_is_valid", c_i32), ("step_is_successful", c_i32),
+        ("linear_solver_iterations", c_i32), ("cost", c_f64), ("cost_change", c_f64),
+        ("gradient_max_norm", c_f64), ("step_norm", c_f64), ("relative_decrease", c_f64),
+        ("trust_region_radius", c_f64), ("iteration_time_in_seconds", c_f64),
+        ("cumulative_time_in_seconds", c_f64),
+    ]
+
+
+class Summary(ctypes.Structure):
+    _fields_ = [
+        ("termination_type", c_i32), ("num_iterations", c_i32), ("num_successful_steps", c_i32),
+        ("num_unsuccessful_steps", c_i32), ("num_residual_blocks", c_i32),
+        ("num_removed_landmarks", c_i32), ("initial_cost", c_f64), ("final_cost", c_f64),
+        ("fixed_cost", c_f64), ("lm_loop_seconds", c_f64), ("evaluate_seconds", c_f64),
+        ("linear_solver_seconds", c_f64), ("total_linear_solver_iterations", ctypes.c_int64),
+        ("iterations_capacity", c_i32), ("reserved0", c_i32),
+        ("iterations", ctypes.POINTER(IterationSummary)),
+    ]
+
+
+class SynthConfig(ctypes.Structure):
+    _fields_ = [
+        ("kind", c_i32), ("num_keyframes", c_i32), ("num_landmarks", c_i32), ("distortion_type", c_i32),
+        ("distortion", c_f64 * 4), ("intrinsics", c_f64 * 4), ("width", c_i32), ("height", c_i32),
+        ("p_C_I", c_f64 * 3), ("keypoint_sigma_px", c_f64), ("outlier_fraction", c_f64),
+        ("rot_noise_rad", c_f64), ("pos_noise_m", c_f64), ("landmark_noise_m", c_f64),
+        ("step_m", c_f64), ("yaw_drift_rad", c_f64), ("min_dist_m", c_f64), ("max_dist_m", c_f64),
+        ("max_range_m", c_f64), ("min_depth_m", c_f64), ("box_half_m", c_f64),
+        ("circle_radius_m", c_f64), ("min_obs_per_landmark", c_i32), ("max_obs_per_landmark", c_i32),
+        ("detection_probability", c_f64), ("anchor_mode", c_i32), ("fix_scale_landmark", c_i32),
+        ("seed", c_u64),
+    ]
+
+
+def _declare(lib, name, restype, argtypes):
+    fn = getattr(lib, name)
+    fn.restype = restype
+    fn.argtypes = argtypes
+    return fn
+
+
+# Every symbol include/maplab_ba.h declares, with its ctypes signature.
+LIB_SYMBOLS = {
+    "mlba_abi_version": (c_i32, []),
+    "mlba_options_init": (None, [ctypes.POINTER(Options)]),
+    "mlba_create": (c_i32, [ctypes.POINTER(ProblemDesc), ctypes.POINTER(c_vp)]),
+    "mlba_destroy": (c_i32, [c_vp]),
+    "mlba_last_error": (ctypes.c_char_p, [c_vp]),
+    "mlba_upload_state": (c_i32, [c_vp, c_vp, c_vp]),
+    "mlba_download_state": (c_i32, [c_vp, c_vp, c_vp]),
+    "mlba_set_landmark_active": (c_i32, [c_vp, c_vp]),
+    "mlba_get_landmark_active": (c_i32, [c_vp, c_vp]),
+    "mlba_evaluate": (c_i32, [c_vp, c_i32, ctypes.POINTER(Options), c_vp, c_vp, c_vp]),
+    "mlba_solve": (c_i32, [c_vp, ctypes.POINTER(Options), ctypes.POINTER(Summary)]),
+    "mlba_solve_with_outlier_rejection": (c_i32, [c_vp, ctypes.POINTER(Options), ctypes.POINTER(Summary)]),
+}
+
+SYNTH_SYMBOLS = {
+    "mlba_synth_config_init": (c_i32, [ctypes.POINTER(SynthConfig), c_i32]),
+    "mlba_synth_generate": (c_i32, [ctypes.POINTER(SynthConfig), ctypes.POINTER(c_vp)]),
+    "mlba_synth_free": (None, [c_vp]),
+    "mlba_synth_describe": (c_i32, [c_vp, ctypes.POINTER(ProblemDesc)]),
+    "mlba_synth_true_vertices": (ctypes.POINTER(c_f64), [c_vp]),
+    "mlba_synth_true_landmarks": (ctypes.POINTER(c_f64), [c_vp]),
+}
+
+_lib = None
+_synth = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libmaplab_ba.so; raises OSError when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise OSError(f"{path} is missing: build it with `make -C maplab_amd/csrc` "
+                          "(the visual BA path has no CPU fallback)")
+        lib = ctypes.CDLL(path)
+        for name, (res, args) in LIB_SYMBOLS.items():
+            _declare(lib, name, res, args)
+        _lib = lib
+    return _lib
+
+
+def load_synth(path: str = SYNTH_PATH):
+    global _synth
+    if _synth is None:
+        if not os.path.exists(path):
+            raise OSError(f"{path} is missing: build it with `make -C maplab_amd/csrc`")
+        lib = ctypes.CDLL(path)
+        for name, (res, args) in SYNTH_SYMBOLS.items():
+            _declare(lib, name, res, args)
+        _synth = lib
+    return _synth
+
+
+def default_options() -> Options:
+    """maplab's flag defaults (solver-options.cc:3-52), filled by the library."""
+    o = Options()
+    load_library().mlba_options_init(ctypes.byref(o))
+    return o
+
+
+def check(rc: int, handle=None) -> None:
+    if rc != MLBA_OK:
+        msg = load_library().mlba_last_error(handle)
+        raise RuntimeError(f"maplab_ba error {rc}: {msg.decode() if msg else ''}")

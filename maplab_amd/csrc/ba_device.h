@@ -1,0 +1,72 @@
+// ba_device.h -- device-side data layout of the visual BA problem in HBM and
+// the kernel entry points (ba_kernels.hip). Everything here is internal to
+// libmaplab_ba.so; the public ABI is include/maplab_ba.h.
+#ifndef MAPLAB_AMD_BA_DEVICE_H_
+#define MAPLAB_AMD_BA_DEVICE_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mlba {
+
+constexpr int kObsTile = 256;        // observations per linearization tile
+constexpr int kObsRecord = 20;       // doubles per observation record: r(2) Jl(2x3) Jo(2x6)
+constexpr int kLmAcc = 18;           // V(6) V_mission(6) g(3) g_mission(3)
+constexpr int kRowThreads = 256;     // threads of a reduced-camera-system row block
+
+// Landmark flags.
+constexpr uint32_t kLmActive = 1u;
+constexpr uint32_t kLmConstant = 2u;
+constexpr uint32_t kLmAnchorVar = 4u;   // anchor vertex variable and landmark has pose-coupled obs
+constexpr uint32_t kLmSplit = 8u;       // observations span several linearization tiles
+
+// Observation meta word: bits 0-1 error type, bit 2 observer variable,
+// bits 8.. camera index.
+__host__ __device__ inline int32_t obs_type(int32_t meta) { return meta & 3; }
+__host__ __device__ inline bool obs_observer_var(int32_t meta) { return (meta >> 2) & 1; }
+__host__ __device__ inline int32_t obs_camera(int32_t meta) { return meta >> 8; }
+
+struct DevCamera {
+  double R_C_I[9];
+  double p_C_I[3];
+  double intr[4];
+  double dist[4];
+  double width, height;
+  int32_t dist_type;
+  int32_t pad;
+};
+
+// Read-only problem topology (observation order = landmark-sorted).
+struct DevTopology {
+  int32_t N, L, P, Pvar, M, C;
+  const double* obs_kp;        // 2N
+  const double* obs_sinv;      // N
+  const int32_t* obs_vertex;   // N
+  const int32_t* obs_meta;     // N
+  const int32_t* obs_landmark; // N
+  const int32_t* lm_begin;     // L+1
+  const int32_t* lm_anchor;    // L
+  const uint32_t* lm_flags;    // L
+  const DevCamera* cams;       // C
+  const double* mission_T;     // 12M: R_G_M (JPL matrix of q_GM) and p_G_M
+  const int32_t* vertex_mission; // P
+  const int32_t* vertex_var;   // P: variable index or -1
+  const int32_t* var_vertex;   // Pvar
+  // reduced camera system
+  const int32_t* row_ptr;      // Pvar+1 block row pointers of S
+  const int32_t* col_var;      // nnzb: column variable index
+  const int32_t* diag_pos;     // Pvar: position of the diagonal block
+  const int32_t* inc_ptr;      // Pvar+1 incidence lists of each row
+  const int32_t* inc;          // >=0: observation index, <0: -1-landmark (anchor)
+  int32_t max_row_blocks;
+};
+
+// Loss of the whitened residual.
+struct DevLoss {
+  int32_t type;      // 0 Huber (maplab), 1 Cauchy, 2 trivial
+  double scale_px;   // Cauchy scale in pixels
+};
+
+}  // namespace mlba
+
+#endif  // MAPLAB_AMD_BA_DEVICE_H_

@@ -1,0 +1,917 @@
+// ba_host.hip -- C ABI of libmaplab_ba.so: problem upload and topology, the
+// Levenberg-Marquardt trust-region loop over the device kernels, the chunked
+// outlier-rejection driver, and the batched Evaluate entry point.
+//
+// The LM loop restates ceres::TrustRegionMinimizer with the
+// LevenbergMarquardtStrategy as configured by maplab
+// (algorithms/map-optimization/include/map-optimization/solver-options.h:26-74):
+// Jacobi column scaling fixed at the first linearization, LM diagonal
+// clamp(diag(J'J)) / radius, step acceptance at relative decrease > 1e-3,
+// radius update radius / max(1/3, 1 - (2 rho - 1)^3) on success and
+// radius / decrease_factor (doubling) on failure, and the function,
+// parameter, gradient and radius tolerances.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/maplab_ba.h"
+#include "ba_device.h"
+#include "ba_math.h"
+
+namespace mlba {
+void launch_vertex_prep(hipStream_t s, int32_t P, const double* vert, double* vR);
+void launch_linearize(hipStream_t s, bool jac, int32_t tiles, const DevTopology& t, const double* vert,
+                      const double* vR, const double* lm, DevLoss loss, const int32_t* tile_begin, double* obs_rec,
+                      double* lm_acc, double* tile_cost);
+void launch_zero_split(hipStream_t s, int32_t n, const int32_t* ids, double* lm_acc);
+void launch_pose_rows(hipStream_t s, const DevTopology& t, const double* vert, const double* vR, const double* lm,
+                      const double* obs_rec, const double* lm_acc, double* pose_grad, double* pose_diag);
+void launch_jacobi_scaling(hipStream_t s, int32_t Pvar, int32_t L, int32_t use, const double* pose_diag,
+                           const double* lm_acc, double* pose_scale, double* lm_scale);
+void launch_landmark_prep(hipStream_t s, const DevTopology& t, const double* lm_acc, const double* lm_scale,
+                          double inv_mu, double min_diag, double max_diag, double* lm_z, int32_t* fail);
+void launch_schur_rows(hipStream_t s, const DevTopology& t, const double* vert, const double* vR, const double* lm,
+                       const double* obs_rec, const double* lm_acc, const double* lm_z, const double* pose_scale,
+                       const double* pose_diag, double inv_mu, double min_diag, double max_diag, double* S,
+                       double* rhs);
+void launch_block_inverse(hipStream_t s, int32_t Pvar, const int32_t* diag_pos, const double* S, double* Minv,
+                          int32_t* fail);
+int launch_pcg(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_t* col_var, const int32_t* diag_pos,
+               const double* S, const double* Minv, const double* b, double* x, double* r, double* z, double* p,
+               double* q, double* parts_a, double* parts_b, void* state_dev, void* state_host, double tol,
+               int32_t max_iters, int32_t poll);
+size_t pcg_state_bytes();
+void launch_back_substitute(hipStream_t s, const DevTopology& t, const double* vR, const double* lm,
+                            const double* obs_rec, const double* lm_acc, const double* lm_z, const double* w,
+                            double* u, double* part);
+void launch_unscale_poses(hipStream_t s, int32_t Pvar, const double* y, const double* pose_scale, double* w,
+                          double* part);
+void launch_candidate(hipStream_t s, const DevTopology& t, const double* vert, const double* w, double* cand_v,
+                      double* part_v, const double* lm, const double* u, double* cand_l, double* part_l);
+void launch_gradient_norm(hipStream_t s, const DevTopology& t, const double* vert, const double* pose_grad,
+                          const double* lm_acc, double* part);
+void launch_reduce(hipStream_t s, int32_t n, int32_t stride, int32_t offset, bool is_max, const double* parts,
+                   double* out);
+void launch_evaluate_full(hipStream_t s, const DevTopology& t, const double* vert, const double* vR, const double* lm,
+                          const double* baseframes, const double* cam_q, bool tangent, const int32_t* perm,
+                          double* res, double* jout, int32_t jstride, uint32_t mask);
+}  // namespace mlba
+
+using namespace mlba;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// A device allocation that frees itself.
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  bool alloc(size_t count) {
+    if (count <= n && p) return true;
+    release();
+    const size_t bytes = std::max<size_t>(1, count) * sizeof(T);
+    if (hipMalloc(&p, bytes) != hipSuccess) { p = nullptr; return false; }
+    n = std::max<size_t>(1, count);
+    return true;
+  }
+};
+
+// Scalar slots of the reduction output array.
+enum Slot { kCost = 0, kFixed, kMcc, kUU, kWW, kXV, kXL, kCandCost, kCandFixed, kGradMax, kNumSlots };
+
+}  // namespace
+
+struct mlba_handle {
+  std::string err;
+  hipStream_t stream = nullptr;
+  int N = 0, L = 0, P = 0, C = 0, M = 0;
+  // Host topology.
+  std::vector<int32_t> perm;            // sorted -> caller observation index
+  std::vector<int32_t> s_landmark, s_vertex, s_camera, s_type;
+  std::vector<int32_t> lm_begin, lm_anchor, vertex_mission;
+  std::vector<uint8_t> lm_active, lm_const, vertex_const;
+  std::vector<double> h_baseframes, h_cam_q;
+  // Derived per topology build.
+  int32_t Pvar = 0, tiles = 0, nnzb = 0, n_inc = 0, n_split = 0, max_row_blocks = 0;
+  // Device memory.
+  DBuf<double> obs_kp, obs_sinv, obs_rec;
+  DBuf<int32_t> obs_vertex, obs_meta, obs_landmark, perm_d;
+  DBuf<int32_t> lm_begin_d, lm_anchor_d, split_ids;
+  DBuf<uint32_t> lm_flags;
+  DBuf<double> lm, lm_cand, lm_acc, lm_scale, lm_z, lm_u;
+  DBuf<double> vert, vert_cand, vR, vR_cand;
+  DBuf<int32_t> vertex_mission_d, vertex_var, var_vertex;
+  DBuf<DevCamera> cams;
+  DBuf<double> mission_T, baseframes, cam_q;
+  DBuf<int32_t> tile_begin;
+  DBuf<double> tile_cost, tile_cost2;
+  DBuf<int32_t> row_ptr, col_var, diag_pos, inc_ptr, inc;
+  DBuf<double> S, rhs, pose_grad, pose_diag, pose_scale, w, Minv;
+  DBuf<double> px, pr, pz, pp, pq, parts_a, parts_b, parts_c, parts_l, parts_v, parts_w, parts_g;
+  DBuf<double> scal;
+  DBuf<int32_t> fail;
+  DBuf<char> pcg_state;
+  void* pcg_host = nullptr;
+  double* scal_host = nullptr;
+  DevTopology topo{};
+
+  ~mlba_handle() {
+    if (pcg_host) (void)hipHostFree(pcg_host);
+    if (scal_host) (void)hipHostFree(scal_host);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int set_err(mlba_handle* h, int code, const std::string& msg) {
+  if (h) h->err = msg; else g_create_error = msg;
+  return code;
+}
+
+#define HIP_OK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return set_err(h, MLBA_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+int upload(mlba_handle* h, DBuf<T>& b, const T* src, size_t count) {
+  if (!b.alloc(count)) return set_err(h, MLBA_ENOMEM, "device allocation failed");
+  if (count) HIP_OK(hipMemcpyAsync(b.p, src, count * sizeof(T), hipMemcpyHostToDevice, h->stream));
+  return MLBA_OK;
+}
+
+bool valid_quaternion(const double* q) {
+  const double n2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+  return std::isfinite(n2) && std::fabs(n2 - 1.0) < 2e-5 && q[3] >= 0.0;
+}
+
+// Build the observation / landmark / vertex topology and the reduced camera
+// system structure for the current active landmark set, and upload it.
+int build_topology(mlba_handle* h) {
+  const int N = h->N, L = h->L, P = h->P;
+  // Error type per sorted observation (optimization-terms-addition.cc:68-78).
+  // Variable vertices: non-constant and in an active pose-coupled residual.
+  std::vector<uint8_t> used(P, 0), lm_coupled(L, 0);
+  for (int o = 0; o < N; ++o) {
+    const int l = h->s_landmark[o];
+    if (!h->lm_active[l] || h->s_type[o] == kTypeKeyframe) continue;
+    used[h->s_vertex[o]] = 1;
+    used[h->lm_anchor[l]] = 1;
+    lm_coupled[l] = 1;
+  }
+  std::vector<int32_t> vertex_var(P, -1), var_vertex;
+  for (int v = 0; v < P; ++v)
+    if (used[v] && !h->vertex_const[v]) { vertex_var[v] = (int32_t)var_vertex.size(); var_vertex.push_back(v); }
+  const int Pvar = (int)var_vertex.size();
+  h->Pvar = Pvar;
+  // Tiles of whole landmarks, <= kObsTile observations each.
+  std::vector<int32_t> tile_begin{0}, split;
+  std::vector<uint32_t> flags(L, 0);
+  {
+    int cur = 0;
+    for (int l = 0; l < L; ++l) {
+      const int n = h->lm_begin[l + 1] - h->lm_begin[l];
+      if (n > kObsTile) {
+        if (h->lm_begin[l] > cur) tile_begin.push_back(h->lm_begin[l]);
+        for (int k = kObsTile; k < n; k += kObsTile) tile_begin.push_back(h->lm_begin[l] + k);
+        tile_begin.push_back(h->lm_begin[l + 1]);
+        cur = h->lm_begin[l + 1];
+        flags[l] |= kLmSplit;
+        split.push_back(l);
+      } else if (h->lm_begin[l + 1] - cur > kObsTile) {
+        tile_begin.push_back(h->lm_begin[l]);
+        cur = h->lm_begin[l];
+      }
+    }
+    if (tile_begin.back() != N) tile_begin.push_back(N);
+    // Remove empty tiles.
+    tile_begin.erase(std::unique(tile_begin.begin(), tile_begin.end()), tile_begin.end());
+  }
+  h->tiles = (int32_t)tile_begin.size() - 1;
+  h->n_split = (int32_t)split.size();
+  for (int l = 0; l < L; ++l) {
+    if (h->lm_active[l]) flags[l] |= kLmActive;
+    if (h->lm_const[l]) flags[l] |= kLmConstant;
+    if (h->lm_active[l] && lm_coupled[l] && vertex_var[h->lm_anchor[l]] >= 0) flags[l] |= kLmAnchorVar;
+  }
+  std::vector<int32_t> meta(N);
+  for (int o = 0; o < N; ++o) {
+    const int type = h->s_type[o];
+    const bool ov = type != kTypeKeyframe && vertex_var[h->s_vertex[o]] >= 0;
+    meta[o] = type | (ov ? 4 : 0) | (h->s_camera[o] << 8);
+  }
+  // Incidences of each variable vertex row.
+  std::vector<int32_t> inc_ptr(Pvar + 1, 0), inc;
+  {
+    std::vector<int32_t> cnt(Pvar, 0);
+    for (int o = 0; o < N; ++o) {
+      const int l = h->s_landmark[o];
+      if (!h->lm_active[l] || !(meta[o] & 4)) continue;
+      cnt[vertex_var[h->s_vertex[o]]]++;
+    }
+    for (int l = 0; l < L; ++l)
+      if (flags[l] & kLmAnchorVar) cnt[vertex_var[h->lm_anchor[l]]]++;
+    for (int i = 0; i < Pvar; ++i) inc_ptr[i + 1] = inc_ptr[i] + cnt[i];
+    inc.resize(inc_ptr[Pvar]);
+    std::vector<int32_t> fill(inc_ptr.begin(), inc_ptr.end() - 1);
+    for (int o = 0; o < N; ++o) {
+      const int l = h->s_landmark[o];
+      if (!h->lm_active[l] || !(meta[o] & 4)) continue;
+      inc[fill[vertex_var[h->s_vertex[o]]]++] = o;
+    }
+    for (int l = 0; l < L; ++l)
+      if (flags[l] & kLmAnchorVar) inc[fill[vertex_var[h->lm_anchor[l]]]++] = -1 - l;
+  }
+  h->n_inc = (int32_t)inc.size();
+  // Block structure of S: row i couples with every variable vertex of every
+  // landmark it touches.
+  std::vector<int32_t> row_ptr(Pvar + 1, 0), col_var, diag_pos(Pvar, 0);
+  {
+    std::vector<int32_t> mark(Pvar, -1), cols;
+    h->max_row_blocks = 0;
+    for (int i = 0; i < Pvar; ++i) {
+      cols.clear();
+      auto touch = [&](int j) {
+        if (mark[j] != i) { mark[j] = i; cols.push_back(j); }
+      };
+      touch(i);
+      int last_l = -1;
+      for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) {
+        const int x = inc[k];
+        const int l = x >= 0 ? h->s_landmark[x] : -1 - x;
+        if (l == last_l) continue;
+        last_l = l;
+        for (int o = h->lm_begin[l]; o < h->lm_begin[l + 1]; ++o)
+          if (meta[o] & 4) touch(vertex_var[h->s_vertex[o]]);
+        if (flags[l] & kLmAnchorVar) touch(vertex_var[h->lm_anchor[l]]);
+      }
+      std::sort(cols.begin(), cols.end());
+      for (size_t k = 0; k < cols.size(); ++k)
+        if (cols[k] == i) diag_pos[i] = (int32_t)(col_var.size() + k);
+      col_var.insert(col_var.end(), cols.begin(), cols.end());
+      row_ptr[i + 1] = (int32_t)col_var.size();
+      h->max_row_blocks = std::max<int32_t>(h->max_row_blocks, (int32_t)cols.size());
+    }
+  }
+  h->nnzb = (int32_t)col_var.size();
+  int rc;
+  if ((rc = upload(h, h->lm_flags, flags.data(), flags.size()))) return rc;
+  if ((rc = upload(h, h->obs_meta, meta.data(), meta.size()))) return rc;
+  if ((rc = upload(h, h->vertex_var, vertex_var.data(), vertex_var.size()))) return rc;
+  if ((rc = upload(h, h->var_vertex, var_vertex.data(), var_vertex.size()))) return rc;
+  if ((rc = upload(h, h->tile_begin, tile_begin.data(), tile_begin.size()))) return rc;
+  if ((rc = upload(h, h->split_ids, split.data(), split.size()))) return rc;
+  if ((rc = upload(h, h->inc_ptr, inc_ptr.data(), inc_ptr.size()))) return rc;
+  if ((rc = upload(h, h->inc, inc.data(), inc.size()))) return rc;
+  if ((rc = upload(h, h->row_ptr, row_ptr.data(), row_ptr.size()))) return rc;
+  if ((rc = upload(h, h->col_var, col_var.data(), col_var.size()))) return rc;
+  if ((rc = upload(h, h->diag_pos, diag_pos.data(), diag_pos.size()))) return rc;
+  const size_t T = h->tiles, V6 = 6 * (size_t)std::max(Pvar, 1);
+  const size_t vb = (Pvar + 255) / 256 + 1;
+  bool ok = h->tile_cost.alloc(2 * T) && h->tile_cost2.alloc(2 * T) && h->S.alloc(36 * (size_t)h->nnzb) &&
+            h->rhs.alloc(V6) && h->pose_grad.alloc(V6) && h->pose_diag.alloc(V6) && h->pose_scale.alloc(V6) &&
+            h->w.alloc(V6) && h->Minv.alloc(36 * (size_t)std::max(Pvar, 1)) && h->px.alloc(V6) &&
+            h->pr.alloc(V6) && h->pz.alloc(V6) && h->pp.alloc(V6) && h->pq.alloc(V6) &&
+            h->parts_a.alloc(std::max<size_t>(Pvar, 2 * vb) + 1) && h->parts_b.alloc(2 * vb + 2) &&
+            h->parts_l.alloc(2 * ((size_t)(h->L + 255) / 256 + 1)) &&
+            h->parts_c.alloc((size_t)(h->L + 255) / 256 + 1) &&
+            h->parts_v.alloc((size_t)(h->P + 255) / 256 + 1) && h->parts_w.alloc((6 * (size_t)Pvar + 255) / 256 + 1) &&
+            h->parts_g.alloc(((size_t)Pvar + h->L + 255) / 256 + 1);
+  if (!ok) return set_err(h, MLBA_ENOMEM, "device allocation failed");
+  DevTopology& t = h->topo;
+  t.N = h->N; t.L = h->L; t.P = h->P; t.Pvar = Pvar; t.M = h->M; t.C = h->C;
+  t.obs_kp = h->obs_kp.p; t.obs_sinv = h->obs_sinv.p; t.obs_vertex = h->obs_vertex.p; t.obs_meta = h->obs_meta.p;
+  t.obs_landmark = h->obs_landmark.p; t.lm_begin = h->lm_begin_d.p; t.lm_anchor = h->lm_anchor_d.p;
+  t.lm_flags = h->lm_flags.p; t.cams = h->cams.p; t.mission_T = h->mission_T.p;
+  t.vertex_mission = h->vertex_mission_d.p; t.vertex_var = h->vertex_var.p; t.var_vertex = h->var_vertex.p;
+  t.row_ptr = h->row_ptr.p; t.col_var = h->col_var.p; t.diag_pos = h->diag_pos.p; t.inc_ptr = h->inc_ptr.p;
+  t.inc = h->inc.p; t.max_row_blocks = h->max_row_blocks;
+  HIP_OK(hipStreamSynchronize(h->stream));
+  return MLBA_OK;
+}
+
+DevLoss make_loss(const mlba_options* o) {
+  DevLoss l;
+  l.type = o ? o->loss_type : MLBA_LOSS_MAPLAB_HUBER;
+  l.scale_px = o ? o->loss_scale_px : 1.0;
+  return l;
+}
+
+// Linearize at the current state: residual records, landmark reductions,
+// vertex-row gradient and diag(F'F). Leaves cost / fixed cost / gradient
+// max norm in the scalar slots (device).
+int linearize(mlba_handle* h, const mlba_options* o) {
+  hipStream_t s = h->stream;
+  launch_vertex_prep(s, h->P, h->vert.p, h->vR.p);
+  launch_zero_split(s, h->n_split, h->split_ids.p, h->lm_acc.p);
+  launch_linearize(s, true, h->tiles, h->topo, h->vert.p, h->vR.p, h->lm.p, make_loss(o), h->tile_begin.p,
+                   h->obs_rec.p, h->lm_acc.p, h->tile_cost.p);
+  launch_pose_rows(s, h->topo, h->vert.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p, h->pose_grad.p, h->pose_diag.p);
+  launch_reduce(s, h->tiles, 2, 0, false, h->tile_cost.p, h->scal.p + kCost);
+  launch_reduce(s, h->tiles, 2, 1, false, h->tile_cost.p, h->scal.p + kFixed);
+  const int gb = (h->Pvar + h->L + 255) / 256;
+  launch_gradient_norm(s, h->topo, h->vert.p, h->pose_grad.p, h->lm_acc.p, h->parts_g.p);
+  launch_reduce(s, gb, 1, 0, true, h->parts_g.p, h->scal.p + kGradMax);
+  HIP_OK(hipGetLastError());
+  return MLBA_OK;
+}
+
+int read_scalars(mlba_handle* h) {
+  HIP_OK(hipMemcpyAsync(h->scal_host, h->scal.p, kNumSlots * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  return MLBA_OK;
+}
+
+void push_iteration(mlba_summary* sm, const mlba_iteration_summary& it) {
+  if (!sm) return;
+  if (sm->iterations && sm->num_iterations < sm->iterations_capacity) sm->iterations[sm->num_iterations] = it;
+  sm->num_iterations++;
+}
+
+// One ceres::Solve (TrustRegionMinimizer + LevenbergMarquardtStrategy).
+int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_summary* sm, double* last_radius,
+             int* termination) {
+  hipStream_t s = h->stream;
+  const double t_start = now_s();
+  double radius = initial_radius, decrease_factor = 2.0;
+  int rc;
+  const double t_ev0 = now_s();
+  if ((rc = linearize(h, o))) return rc;
+  launch_jacobi_scaling(s, h->Pvar, h->L, o->use_jacobi_scaling, h->pose_diag.p, h->lm_acc.p, h->pose_scale.p,
+                        h->lm_scale.p);
+  if ((rc = read_scalars(h))) return rc;
+  if (sm) sm->evaluate_seconds += now_s() - t_ev0;
+  double x_cost = h->scal_host[kCost];
+  const double fixed_cost = h->scal_host[kFixed];
+  double grad_max = h->scal_host[kGradMax];
+  if (sm) { sm->initial_cost = x_cost + fixed_cost; sm->fixed_cost = fixed_cost; }
+  mlba_iteration_summary it{};
+  it.iteration = 0;
+  it.step_is_valid = 1;
+  it.step_is_successful = 1;
+  it.cost = x_cost + fixed_cost;
+  it.gradient_max_norm = grad_max;
+  it.trust_region_radius = radius;
+  it.cumulative_time_in_seconds = it.iteration_time_in_seconds = now_s() - t_start;
+  push_iteration(sm, it);
+  *termination = MLBA_NO_CONVERGENCE;
+  if (grad_max <= o->gradient_tolerance) { *termination = MLBA_CONVERGENCE; *last_radius = radius; goto done; }
+  if (o->max_num_iterations <= 0) { *last_radius = radius; goto done; }
+  {
+    int consecutive_invalid = 0, iteration = 0;
+    const int vb = (h->Pvar + 255) / 256;
+    const int lb = (h->L + 255) / 256;
+    const int pb = (h->P + 255) / 256;
+    const int wb = (6 * h->Pvar + 255) / 256;
+    while (true) {
+      const double t_iter = now_s();
+      ++iteration;
+      mlba_iteration_summary cur{};
+      cur.iteration = iteration;
+      const double inv_mu = 1.0 / radius;
+      // Linear solve: Schur elimination + PCG on the reduced camera system.
+      const double t_ls = now_s();
+      HIP_OK(hipMemsetAsync(h->fail.p, 0, sizeof(int32_t), s));
+      launch_landmark_prep(s, h->topo, h->lm_acc.p, h->lm_scale.p, inv_mu, o->min_lm_diagonal, o->max_lm_diagonal,
+                           h->lm_z.p, h->fail.p);
+      launch_schur_rows(s, h->topo, h->vert.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p, h->lm_z.p,
+                        h->pose_scale.p, h->pose_diag.p, inv_mu, o->min_lm_diagonal, o->max_lm_diagonal, h->S.p,
+                        h->rhs.p);
+      launch_block_inverse(s, h->Pvar, h->diag_pos.p, h->S.p, h->Minv.p, h->fail.p);
+      HIP_OK(hipGetLastError());
+      int pcg_iters = 0;
+      if (h->Pvar > 0) {
+        pcg_iters = launch_pcg(s, h->Pvar, h->row_ptr.p, h->col_var.p, h->diag_pos.p, h->S.p, h->Minv.p, h->rhs.p,
+                               h->px.p, h->pr.p, h->pz.p, h->pp.p, h->pq.p, h->parts_a.p, h->parts_b.p,
+                               h->pcg_state.p, h->pcg_host, o->pcg_relative_tolerance, o->pcg_max_iterations, 8);
+        if (pcg_iters < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
+      }
+      launch_unscale_poses(s, h->Pvar, h->px.p, h->pose_scale.p, h->w.p, h->parts_w.p);
+      launch_back_substitute(s, h->topo, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p, h->lm_z.p, h->w.p, h->lm_u.p,
+                             h->parts_l.p);
+      launch_reduce(s, lb, 2, 0, false, h->parts_l.p, h->scal.p + kMcc);
+      launch_reduce(s, lb, 2, 1, false, h->parts_l.p, h->scal.p + kUU);
+      launch_reduce(s, wb, 1, 0, false, h->parts_w.p, h->scal.p + kWW);
+      // Candidate point and its cost.
+      launch_candidate(s, h->topo, h->vert.p, h->w.p, h->vert_cand.p, h->parts_v.p, h->lm.p, h->lm_u.p,
+                       h->lm_cand.p, h->parts_c.p);
+      launch_reduce(s, pb, 1, 0, false, h->parts_v.p, h->scal.p + kXV);
+      launch_reduce(s, lb, 1, 0, false, h->parts_c.p, h->scal.p + kXL);
+      launch_vertex_prep(s, h->P, h->vert_cand.p, h->vR_cand.p);
+      launch_linearize(s, false, h->tiles, h->topo, h->vert_cand.p, h->vR_cand.p, h->lm_cand.p, make_loss(o),
+                       h->tile_begin.p, nullptr, nullptr, h->tile_cost2.p);
+      launch_reduce(s, h->tiles, 2, 0, false, h->tile_cost2.p, h->scal.p + kCandCost);
+      HIP_OK(hipGetLastError());
+      int32_t fail_h = 0;
+      HIP_OK(hipMemcpyAsync(&fail_h, h->fail.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      if ((rc = read_scalars(h))) return rc;
+      if (sm) sm->linear_solver_seconds += now_s() - t_ls;
+      const double* sc = h->scal_host;
+      const double mcc = sc[kMcc];
+      bool step_valid = fail_h == 0 && std::isfinite(mcc) && mcc > 0.0;
+      cur.linear_solver_iterations = pcg_iters;
+      if (sm) sm->total_linear_solver_iterations += pcg_iters;
+      cur.step_is_valid = step_valid;
+      if (!step_valid) {
+        if (++consecutive_invalid >= o->max_num_consecutive_invalid_steps) {
+          *termination = MLBA_FAILURE;
+          break;
+        }
+        radius /= decrease_factor;
+        decrease_factor *= 2.0;
+        cur.cost = x_cost + fixed_cost;
+        cur.gradient_max_norm = grad_max;
+        cur.trust_region_radius = radius;
+        cur.iteration_time_in_seconds = now_s() - t_iter;
+        cur.cumulative_time_in_seconds = now_s() - t_start;
+        push_iteration(sm, cur);
+        if (sm) sm->num_unsuccessful_steps++;
+      } else {
+        consecutive_invalid = 0;
+        const double candidate_cost = sc[kCandCost];
+        const double x_norm = std::sqrt(sc[kXV] + sc[kXL]);
+        const double step_norm = std::sqrt(sc[kWW] + sc[kUU]);
+        cur.step_norm = step_norm;  // tangent step norm
+        if (step_norm <= o->parameter_tolerance * (x_norm + o->parameter_tolerance)) {
+          *termination = MLBA_CONVERGENCE;
+          break;
+        }
+        cur.cost_change = x_cost - candidate_cost;
+        if (std::fabs(cur.cost_change) <= o->function_tolerance * x_cost) {
+          *termination = MLBA_CONVERGENCE;
+          break;
+        }
+        cur.relative_decrease = (x_cost - candidate_cost) / mcc;
+        if (cur.relative_decrease > o->min_relative_decrease) {
+          std::swap(h->vert.p, h->vert_cand.p);
+          std::swap(h->lm.p, h->lm_cand.p);
+          const double t_ev = now_s();
+          if ((rc = linearize(h, o))) return rc;
+          if ((rc = read_scalars(h))) return rc;
+          if (sm) sm->evaluate_seconds += now_s() - t_ev;
+          x_cost = h->scal_host[kCost];
+          grad_max = h->scal_host[kGradMax];
+          cur.step_is_successful = 1;
+          cur.cost = x_cost + fixed_cost;
+          cur.gradient_max_norm = grad_max;
+          const double q = 2.0 * cur.relative_decrease - 1.0;
+          radius = std::min(o->max_trust_region_radius, radius / std::max(1.0 / 3.0, 1.0 - q * q * q));
+          decrease_factor = 2.0;
+          if (sm) sm->num_successful_steps++;
+        } else {
+          radius /= decrease_factor;
+          decrease_factor *= 2.0;
+          cur.cost = candidate_cost + fixed_cost;
+          cur.gradient_max_norm = grad_max;
+          if (sm) sm->num_unsuccessful_steps++;
+        }
+        cur.trust_region_radius = radius;
+        cur.iteration_time_in_seconds = now_s() - t_iter;
+        cur.cumulative_time_in_seconds = now_s() - t_start;
+        push_iteration(sm, cur);
+      }
+      if (now_s() - t_start > o->max_solver_time_in_seconds) { *termination = MLBA_NO_CONVERGENCE; break; }
+      if (iteration >= o->max_num_iterations) { *termination = MLBA_NO_CONVERGENCE; break; }
+      if (cur.step_is_successful && cur.gradient_max_norm <= o->gradient_tolerance) {
+        *termination = MLBA_CONVERGENCE;
+        break;
+      }
+      if (radius <= o->min_trust_region_radius) { *termination = MLBA_CONVERGENCE; break; }
+    }
+  }
+  *last_radius = radius;
+done:
+  if (sm) {
+    sm->final_cost = x_cost + fixed_cost;
+    sm->lm_loop_seconds += now_s() - t_start;
+  }
+  return MLBA_OK;
+}
+
+void reset_summary(mlba_summary* sm) {
+  if (!sm) return;
+  sm->num_iterations = 0;
+  sm->num_successful_steps = sm->num_unsuccessful_steps = 0;
+  sm->num_removed_landmarks = 0;
+  sm->lm_loop_seconds = sm->evaluate_seconds = sm->linear_solver_seconds = 0.0;
+  sm->total_linear_solver_iterations = 0;
+  sm->initial_cost = sm->final_cost = sm->fixed_cost = 0.0;
+}
+
+int validate_options(mlba_handle* h, const mlba_options* o) {
+  if (!o) return set_err(h, MLBA_EINVAL, "options is NULL");
+  if (o->loss_type < 0 || o->loss_type > 2) return set_err(h, MLBA_EINVAL, "unknown loss_type");
+  if (o->linear_solver_type != MLBA_SOLVER_SCHUR_PCG) return set_err(h, MLBA_EINVAL, "unknown linear_solver_type");
+  if (!(o->initial_trust_region_radius > 0.0)) return set_err(h, MLBA_EINVAL, "initial_trust_region_radius <= 0");
+  if (o->loss_type == MLBA_LOSS_CAUCHY && !(o->loss_scale_px > 0.0)) return set_err(h, MLBA_EINVAL, "loss_scale_px <= 0");
+  return MLBA_OK;
+}
+
+int count_active_obs(const mlba_handle* h) {
+  int n = 0;
+  for (int o = 0; o < h->N; ++o) n += h->lm_active[h->s_landmark[o]] ? 1 : 0;
+  return n;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+
+extern "C" {
+
+int32_t mlba_abi_version(void) { return MLBA_ABI_VERSION; }
+
+void mlba_options_init(mlba_options* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->max_num_iterations = 30;
+  o->use_jacobi_scaling = 1;
+  o->max_solver_time_in_seconds = 1e6;
+  o->function_tolerance = 1e-12;
+  o->gradient_tolerance = 1e-10;
+  o->parameter_tolerance = 1e-8;
+  o->initial_trust_region_radius = 1e5;
+  o->max_trust_region_radius = 1e20;
+  o->min_trust_region_radius = 1e-32;
+  o->min_relative_decrease = 1e-3;
+  o->min_lm_diagonal = 1e-6;
+  o->max_lm_diagonal = 1e32;
+  o->max_num_consecutive_invalid_steps = 5;
+  o->loss_type = MLBA_LOSS_MAPLAB_HUBER;
+  o->loss_scale_px = 1.0;
+  o->linear_solver_type = MLBA_SOLVER_SCHUR_PCG;
+  o->pcg_max_iterations = 500;
+  o->pcg_relative_tolerance = 1e-10;
+  o->reject_outliers_every_n_iters = 3;
+  o->use_reprojection_error = 0;
+  o->max_reprojection_error_px = 50.0;
+}
+
+const char* mlba_last_error(const mlba_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+int32_t mlba_create(const mlba_problem_desc* d, mlba_handle** out) {
+  mlba_handle* h = nullptr;
+  if (!d || !out) return set_err(h, MLBA_EINVAL, "NULL argument");
+  *out = nullptr;
+  const int N = d->num_observations, L = d->num_landmarks, P = d->num_vertices, C = d->num_cameras,
+            M = d->num_missions;
+  if (N < 0 || L < 0 || P < 0 || C < 0 || M < 0) return set_err(h, MLBA_EINVAL, "negative size");
+  if ((N > 0 && (!d->obs_vertex || !d->obs_landmark || !d->obs_camera || !d->obs_keypoint || !d->obs_sigma)) ||
+      (P > 0 && (!d->vertex_q_IM__M_p_MI || !d->vertex_mission)) ||
+      (L > 0 && (!d->landmark_p_B || !d->landmark_store_vertex)) ||
+      (C > 0 && (!d->camera_q_CI__C_p_CI || !d->camera_intrinsics || !d->camera_distortion ||
+                 !d->camera_distortion_type || !d->camera_resolution)) ||
+      (M > 0 && !d->baseframe_q_GM__G_p_GM))
+    return set_err(h, MLBA_EINVAL, "NULL array in problem description");
+  if ((int64_t)C >= (1 << 23)) return set_err(h, MLBA_EINVAL, "too many cameras");
+  for (int o = 0; o < N; ++o) {
+    if (d->obs_vertex[o] < 0 || d->obs_vertex[o] >= P) return set_err(h, MLBA_EINVAL, "obs_vertex out of range");
+    if (d->obs_landmark[o] < 0 || d->obs_landmark[o] >= L) return set_err(h, MLBA_EINVAL, "obs_landmark out of range");
+    if (d->obs_camera[o] < 0 || d->obs_camera[o] >= C) return set_err(h, MLBA_EINVAL, "obs_camera out of range");
+    if (!(d->obs_sigma[o] > 0.0)) return set_err(h, MLBA_EINVAL, "pixel sigma must be > 0");
+    if (!std::isfinite(d->obs_keypoint[2 * o]) || !std::isfinite(d->obs_keypoint[2 * o + 1]))
+      return set_err(h, MLBA_EINVAL, "non-finite keypoint");
+  }
+  for (int l = 0; l < L; ++l)
+    if (d->landmark_store_vertex[l] < 0 || d->landmark_store_vertex[l] >= P)
+      return set_err(h, MLBA_EINVAL, "landmark_store_vertex out of range");
+  for (int v = 0; v < P; ++v) {
+    if (d->vertex_mission[v] < 0 || d->vertex_mission[v] >= M) return set_err(h, MLBA_EINVAL, "vertex_mission out of range");
+    if (!valid_quaternion(d->vertex_q_IM__M_p_MI + 7 * v)) return set_err(h, MLBA_ESTATE, "vertex quaternion not unit or w < 0");
+  }
+  for (int c = 0; c < C; ++c) {
+    const int dt = d->camera_distortion_type[c];
+    if (dt < 0 || dt > 2) return set_err(h, MLBA_EINVAL, "unknown distortion type");
+    if (!valid_quaternion(d->camera_q_CI__C_p_CI + 7 * c)) return set_err(h, MLBA_ESTATE, "camera quaternion invalid");
+  }
+  for (int m = 0; m < M; ++m)
+    if (!valid_quaternion(d->baseframe_q_GM__G_p_GM + 7 * m)) return set_err(h, MLBA_ESTATE, "baseframe quaternion invalid");
+
+  h = new (std::nothrow) mlba_handle();
+  if (!h) return set_err(nullptr, MLBA_ENOMEM, "host allocation failed");
+  auto fail_out = [&](int code) {
+    g_create_error = h->err;
+    delete h;
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    h->err = "hipStreamCreate failed";
+    return fail_out(MLBA_EDEVICE);
+  }
+  h->N = N; h->L = L; h->P = P; h->C = C; h->M = M;
+  // Sort observations by (landmark, vertex, caller index).
+  h->perm.resize(N);
+  for (int o = 0; o < N; ++o) h->perm[o] = o;
+  std::stable_sort(h->perm.begin(), h->perm.end(), [&](int a, int b) {
+    if (d->obs_landmark[a] != d->obs_landmark[b]) return d->obs_landmark[a] < d->obs_landmark[b];
+    return d->obs_vertex[a] < d->obs_vertex[b];
+  });
+  h->s_landmark.resize(N); h->s_vertex.resize(N); h->s_camera.resize(N); h->s_type.resize(N);
+  std::vector<double> kp(2 * (size_t)N), sinv(N);
+  h->lm_begin.assign(L + 1, 0);
+  for (int k = 0; k < N; ++k) {
+    const int o = h->perm[k];
+    const int l = d->obs_landmark[o], v = d->obs_vertex[o];
+    const int s = d->landmark_store_vertex[l];
+    h->s_landmark[k] = l;
+    h->s_vertex[k] = v;
+    h->s_camera[k] = d->obs_camera[o];
+    h->s_type[k] = v == s ? kTypeKeyframe : (d->vertex_mission[v] == d->vertex_mission[s] ? kTypeMission : kTypeGlobal);
+    kp[2 * k] = d->obs_keypoint[2 * o];
+    kp[2 * k + 1] = d->obs_keypoint[2 * o + 1];
+    sinv[k] = 1.0 / d->obs_sigma[o];
+    h->lm_begin[l + 1]++;
+  }
+  for (int l = 0; l < L; ++l) h->lm_begin[l + 1] += h->lm_begin[l];
+  h->lm_anchor.assign(d->landmark_store_vertex, d->landmark_store_vertex + L);
+  h->vertex_mission.assign(d->vertex_mission, d->vertex_mission + P);
+  h->lm_active.assign(L, 1);
+  h->lm_const.assign(L, 0);
+  h->vertex_const.assign(P, 0);
+  if (d->landmark_constant) for (int l = 0; l < L; ++l) h->lm_const[l] = d->landmark_constant[l] ? 1 : 0;
+  if (d->vertex_constant) for (int v = 0; v < P; ++v) h->vertex_const[v] = d->vertex_constant[v] ? 1 : 0;
+  h->h_baseframes.assign(d->baseframe_q_GM__G_p_GM, d->baseframe_q_GM__G_p_GM + 7 * (size_t)M);
+  h->h_cam_q.assign(d->camera_q_CI__C_p_CI, d->camera_q_CI__C_p_CI + 7 * (size_t)C);
+  std::vector<DevCamera> cams(C);
+  for (int c = 0; c < C; ++c) {
+    DevCamera& k = cams[c];
+    std::memset(&k, 0, sizeof(k));
+    rot_jpl(d->camera_q_CI__C_p_CI + 7 * c, k.R_C_I);
+    for (int i = 0; i < 3; ++i) k.p_C_I[i] = d->camera_q_CI__C_p_CI[7 * c + 4 + i];
+    for (int i = 0; i < 4; ++i) { k.intr[i] = d->camera_intrinsics[4 * c + i]; k.dist[i] = d->camera_distortion[4 * c + i]; }
+    k.width = d->camera_resolution[2 * c];
+    k.height = d->camera_resolution[2 * c + 1];
+    k.dist_type = d->camera_distortion_type[c];
+  }
+  std::vector<double> mT(12 * (size_t)M);
+  for (int m = 0; m < M; ++m) {
+    rot_jpl(d->baseframe_q_GM__G_p_GM + 7 * m, &mT[12 * m]);  // R_G_M
+    for (int i = 0; i < 3; ++i) mT[12 * m + 9 + i] = d->baseframe_q_GM__G_p_GM[7 * m + 4 + i];
+  }
+  int rc;
+  if ((rc = upload(h, h->obs_kp, kp.data(), kp.size())) || (rc = upload(h, h->obs_sinv, sinv.data(), sinv.size())) ||
+      (rc = upload(h, h->obs_vertex, h->s_vertex.data(), (size_t)N)) ||
+      (rc = upload(h, h->obs_landmark, h->s_landmark.data(), (size_t)N)) ||
+      (rc = upload(h, h->perm_d, h->perm.data(), (size_t)N)) ||
+      (rc = upload(h, h->lm_begin_d, h->lm_begin.data(), h->lm_begin.size())) ||
+      (rc = upload(h, h->lm_anchor_d, h->lm_anchor.data(), (size_t)L)) ||
+      (rc = upload(h, h->vertex_mission_d, h->vertex_mission.data(), (size_t)P)) ||
+      (rc = upload(h, h->cams, cams.data(), cams.size())) || (rc = upload(h, h->mission_T, mT.data(), mT.size())) ||
+      (rc = upload(h, h->baseframes, h->h_baseframes.data(), h->h_baseframes.size())) ||
+      (rc = upload(h, h->cam_q, h->h_cam_q.data(), h->h_cam_q.size())) ||
+      (rc = upload(h, h->vert, d->vertex_q_IM__M_p_MI, 7 * (size_t)P)) ||
+      (rc = upload(h, h->lm, d->landmark_p_B, 3 * (size_t)L)))
+    return fail_out(rc);
+  const bool ok = h->obs_rec.alloc((size_t)kObsRecord * N) && h->vert_cand.alloc(7 * (size_t)P) &&
+                  h->vR.alloc(12 * (size_t)P) && h->vR_cand.alloc(12 * (size_t)P) && h->lm_cand.alloc(3 * (size_t)L) &&
+                  h->lm_acc.alloc((size_t)kLmAcc * L) && h->lm_scale.alloc(3 * (size_t)L) &&
+                  h->lm_z.alloc(6 * (size_t)L) && h->lm_u.alloc(3 * (size_t)L) && h->scal.alloc(kNumSlots) &&
+                  h->fail.alloc(1) && h->pcg_state.alloc(pcg_state_bytes());
+  if (!ok) { h->err = "device allocation failed"; return fail_out(MLBA_ENOMEM); }
+  if (hipHostMalloc(&h->pcg_host, pcg_state_bytes(), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&h->scal_host, kNumSlots * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    h->err = "pinned host allocation failed";
+    return fail_out(MLBA_ENOMEM);
+  }
+  if ((rc = build_topology(h))) return fail_out(rc);
+  *out = h;
+  return MLBA_OK;
+}
+
+int32_t mlba_destroy(mlba_handle* h) {
+  if (!h) return MLBA_EINVAL;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  delete h;
+  return MLBA_OK;
+}
+
+int32_t mlba_upload_state(mlba_handle* h, const double* vertices, const double* landmarks) {
+  if (!h) return MLBA_EINVAL;
+  if (vertices)
+    for (int v = 0; v < h->P; ++v)
+      if (!valid_quaternion(vertices + 7 * v)) return set_err(h, MLBA_ESTATE, "vertex quaternion not unit or w < 0");
+  if (vertices) HIP_OK(hipMemcpyAsync(h->vert.p, vertices, 7 * sizeof(double) * h->P, hipMemcpyHostToDevice, h->stream));
+  if (landmarks) HIP_OK(hipMemcpyAsync(h->lm.p, landmarks, 3 * sizeof(double) * h->L, hipMemcpyHostToDevice, h->stream));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  return MLBA_OK;
+}
+
+int32_t mlba_download_state(const mlba_handle* hc, double* vertices, double* landmarks) {
+  mlba_handle* h = const_cast<mlba_handle*>(hc);
+  if (!h) return MLBA_EINVAL;
+  if (vertices) HIP_OK(hipMemcpyAsync(vertices, h->vert.p, 7 * sizeof(double) * h->P, hipMemcpyDeviceToHost, h->stream));
+  if (landmarks) HIP_OK(hipMemcpyAsync(landmarks, h->lm.p, 3 * sizeof(double) * h->L, hipMemcpyDeviceToHost, h->stream));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  return MLBA_OK;
+}
+
+int32_t mlba_set_landmark_active(mlba_handle* h, const uint8_t* mask) {
+  if (!h || !mask) return MLBA_EINVAL;
+  for (int l = 0; l < h->L; ++l) h->lm_active[l] = mask[l] ? 1 : 0;
+  return build_topology(h);
+}
+
+int32_t mlba_get_landmark_active(const mlba_handle* h, uint8_t* mask) {
+  if (!h || !mask) return MLBA_EINVAL;
+  for (int l = 0; l < h->L; ++l) mask[l] = h->lm_active[l];
+  return MLBA_OK;
+}
+
+int32_t mlba_evaluate(mlba_handle* h, int32_t flags, const mlba_options* options, double* residuals,
+                      double* const* jacobians, double* cost) {
+  if (!h) return MLBA_EINVAL;
+  if (flags != MLBA_EVAL_AMBIENT && flags != MLBA_EVAL_TANGENT) return set_err(h, MLBA_EINVAL, "unknown flags");
+  const bool tangent = flags == MLBA_EVAL_TANGENT;
+  static const int amb[9] = {3, 7, 7, 7, 7, 4, 3, 4, 4};
+  static const int tng[9] = {3, 6, 6, 6, 6, 3, 3, 4, 4};
+  const int* sz = tangent ? tng : amb;
+  int stride = 0;
+  uint32_t mask = 0;
+  for (int b = 0; b < 9; ++b) {
+    stride += 2 * sz[b];
+    if (jacobians && jacobians[b]) mask |= 1u << b;
+  }
+  hipStream_t s = h->stream;
+  launch_vertex_prep(s, h->P, h->vert.p, h->vR.p);
+  DBuf<double> res_d, jac_d;
+  if (residuals && !res_d.alloc(2 * (size_t)h->N)) return set_err(h, MLBA_ENOMEM, "device allocation failed");
+  if (mask && !jac_d.alloc((size_t)stride * h->N)) return set_err(h, MLBA_ENOMEM, "device allocation failed");
+  launch_evaluate_full(s, h->topo, h->vert.p, h->vR.p, h->lm.p, h->baseframes.p, h->cam_q.p, tangent, h->perm_d.p,
+                       residuals ? res_d.p : nullptr, mask ? jac_d.p : nullptr, stride, mask);
+  if (cost) {
+    mlba_options defaults;
+    mlba_options_init(&defaults);
+    const mlba_options* o = options ? options : &defaults;
+    launch_linearize(s, false, h->tiles, h->topo, h->vert.p, h->vR.p, h->lm.p, make_loss(o), h->tile_begin.p,
+                     nullptr, nullptr, h->tile_cost2.p);
+    launch_reduce(s, h->tiles, 2, 0, false, h->tile_cost2.p, h->scal.p + kCandCost);
+    launch_reduce(s, h->tiles, 2, 1, false, h->tile_cost2.p, h->scal.p + kCandFixed);
+  }
+  HIP_OK(hipGetLastError());
+  if (residuals)
+    HIP_OK(hipMemcpyAsync(residuals, res_d.p, 2 * sizeof(double) * h->N, hipMemcpyDeviceToHost, s));
+  std::vector<double> jh;
+  if (mask) {
+    jh.resize((size_t)stride * h->N);
+    HIP_OK(hipMemcpyAsync(jh.data(), jac_d.p, jh.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  if (cost) {
+    int rc = read_scalars(h);
+    if (rc) return rc;
+    *cost = h->scal_host[kCandCost] + h->scal_host[kCandFixed];
+  }
+  HIP_OK(hipStreamSynchronize(s));
+  if (mask) {
+    int off = 0;
+    for (int b = 0; b < 9; ++b) {
+      const int n = 2 * sz[b];
+      if (jacobians[b])
+        for (int o = 0; o < h->N; ++o)
+          std::memcpy(jacobians[b] + (size_t)o * n, jh.data() + (size_t)o * stride + off, n * sizeof(double));
+      off += n;
+    }
+  }
+  return MLBA_OK;
+}
+
+int32_t mlba_solve(mlba_handle* h, const mlba_options* o, mlba_summary* sm) {
+  if (!h) return MLBA_EINVAL;
+  int rc = validate_options(h, o);
+  if (rc) return rc;
+  reset_summary(sm);
+  if (sm) sm->num_residual_blocks = count_active_obs(h);
+  double radius = 0.0;
+  int term = MLBA_NO_CONVERGENCE;
+  rc = minimize(h, o, o->initial_trust_region_radius, sm, &radius, &term);
+  if (rc) return rc;
+  if (sm) sm->termination_type = term;
+  return MLBA_OK;
+}
+
+// outlier-rejection-solver.cc:28-103 on the host: p_C.z <= 0 for any
+// observation of an active landmark (optionally a pixel reprojection error
+// above the threshold) removes the landmark.
+static int reject_outliers(mlba_handle* h, const mlba_options* o, int* removed) {
+  std::vector<double> vert(7 * (size_t)h->P), lmv(3 * (size_t)h->L);
+  int rc = mlba_download_state(h, vert.data(), lmv.data());
+  if (rc) return rc;
+  std::vector<DevCamera> cams(h->C);
+  HIP_OK(hipMemcpy(cams.data(), h->cams.p, sizeof(DevCamera) * h->C, hipMemcpyDeviceToHost));
+  std::vector<double> mT(12 * (size_t)h->M);
+  HIP_OK(hipMemcpy(mT.data(), h->mission_T.p, sizeof(double) * mT.size(), hipMemcpyDeviceToHost));
+  std::vector<double> kp(2 * (size_t)h->N);
+  HIP_OK(hipMemcpy(kp.data(), h->obs_kp.p, sizeof(double) * kp.size(), hipMemcpyDeviceToHost));
+  std::vector<uint8_t> bad(h->L, 0);
+  const double max_sq = o->max_reprojection_error_px * o->max_reprojection_error_px;
+  for (int k = 0; k < h->N; ++k) {
+    const int l = h->s_landmark[k];
+    if (!h->lm_active[l] || bad[l]) continue;
+    const int v = h->s_vertex[k], a = h->lm_anchor[l];
+    const double* pB = &lmv[3 * l];
+    double pI[3];
+    if (h->s_type[k] == kTypeKeyframe) {
+      for (int i = 0; i < 3; ++i) pI[i] = pB[i];
+    } else {
+      double Ra[9], Rv[9], pM[3], dm[3];
+      rot_jpl(&vert[7 * a], Ra);
+      rot_jpl(&vert[7 * v], Rv);
+      mat3t_vec(Ra, pB, pM);
+      for (int i = 0; i < 3; ++i) pM[i] += vert[7 * a + 4 + i];
+      if (h->s_type[k] == kTypeGlobal) {
+        const double* TL = &mT[12 * h->vertex_mission[a]];
+        const double* TM = &mT[12 * h->vertex_mission[v]];
+        double pG[3], d[3];
+        mat3_vec(TL, pM, pG);
+        for (int i = 0; i < 3; ++i) d[i] = pG[i] + TL[9 + i] - TM[9 + i];
+        mat3t_vec(TM, d, pM);
+      }
+      for (int i = 0; i < 3; ++i) dm[i] = pM[i] - vert[7 * v + 4 + i];
+      mat3_vec(Rv, dm, pI);
+    }
+    const DevCamera& cam = cams[h->s_camera[k]];
+    double pC[3];
+    mat3_vec(cam.R_C_I, pI, pC);
+    for (int i = 0; i < 3; ++i) pC[i] += cam.p_C_I[i];
+    if (pC[2] <= 0.0) { bad[l] = 1; continue; }
+    if (o->use_reprojection_error) {
+      double px[2], jp[6];
+      const int st = project(cam.dist_type, cam.intr, cam.dist, cam.width, cam.height, pC, px, jp);
+      double e2 = 1.7976931348623157e308;
+      if (st == kVisible || st == kOutsideImage) {
+        const double du = px[0] - kp[2 * k], dv = px[1] - kp[2 * k + 1];
+        e2 = du * du + dv * dv;
+      }
+      if (e2 > max_sq) bad[l] = 1;
+    }
+  }
+  int n = 0;
+  for (int l = 0; l < h->L; ++l)
+    if (bad[l]) { h->lm_active[l] = 0; ++n; }
+  *removed = n;
+  if (n) return build_topology(h);
+  return MLBA_OK;
+}
+
+int32_t mlba_solve_with_outlier_rejection(mlba_handle* h, const mlba_options* o, mlba_summary* sm) {
+  if (!h) return MLBA_EINVAL;
+  int rc = validate_options(h, o);
+  if (rc) return rc;
+  reset_summary(sm);
+  if (o->reject_outliers_every_n_iters <= 0 || o->max_num_iterations <= 0) {
+    if (sm) sm->termination_type = MLBA_FAILURE;  // outlier-rejection-solver.cc:214-218
+    return MLBA_OK;
+  }
+  int remaining = o->max_num_iterations;
+  double radius = o->initial_trust_region_radius;
+  double budget = o->max_solver_time_in_seconds;
+  int term = MLBA_NO_CONVERGENCE;
+  bool first = true;
+  double initial_cost = 0.0;
+  while (remaining > 0 && budget > 0) {
+    const double t0 = now_s();
+    const int step_iters = std::min(remaining, o->reject_outliers_every_n_iters);
+    mlba_options local = *o;
+    local.max_num_iterations = step_iters;
+    if ((rc = minimize(h, &local, radius, sm, &radius, &term))) return rc;
+    if (first && sm) initial_cost = sm->initial_cost;
+    first = false;
+    int removed = 0;
+    if ((rc = reject_outliers(h, o, &removed))) return rc;
+    if (sm) sm->num_removed_landmarks += removed;
+    if (term != MLBA_NO_CONVERGENCE) break;
+    budget -= now_s() - t0;
+    remaining -= step_iters;
+  }
+  if (sm) {
+    sm->termination_type = term;
+    sm->initial_cost = initial_cost;
+    sm->num_residual_blocks = count_active_obs(h);
+  }
+  return MLBA_OK;
+}
+
+}  // extern "C"

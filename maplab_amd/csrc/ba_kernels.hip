@@ -1,0 +1,1356 @@
+// ba_kernels.hip -- gfx950 kernels of the visual bundle-adjustment path.
+//
+// One LM iteration on the device (names match the launchers at the bottom):
+//   k_vertex_prep     R_I_M and |q|^2 of every vertex (state or candidate)
+//   k_linearize<1>    per observation: residual, tangent Jacobians, robust
+//                     correction (VisualReprojectionError::Evaluate + ceres
+//                     Corrector); per landmark: V, g reductions in LDS
+//   k_linearize<0>    the same chain, cost only (candidate evaluation)
+//   k_pose_rows       per variable vertex: unscaled gradient and diag(F'F)
+//   k_landmark_prep   per landmark: Z = C (C V C + D^2)^-1 C (Schur pivot)
+//   k_schur_rows      per variable vertex: one block row of the reduced
+//                     camera system S and its rhs, accumulated in LDS
+//   k_block_inverse   block-Jacobi preconditioner (6x6 inverses of diag(S))
+//   k_pcg_*           preconditioned conjugate gradients on S (FP64)
+//   k_back_substitute landmark step and the LM model cost change
+//   k_candidate       x (+) step for vertices (JPL Plus) and landmarks
+//   k_gradient_norm   max |x - Plus(x, -g)|
+//   k_evaluate_full   every reference Jacobian block (Evaluate parity API)
+// All reductions are fixed-order (per-tile partials summed by one block), so
+// the only run-to-run variation comes from LDS atomics in k_schur_rows.
+#include "ba_device.h"
+#include "ba_math.h"
+
+namespace mlba {
+
+// ------------------------------------------------------------ utilities
+
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Block-wide sum of one value for a 256-thread block; result valid in thread 0.
+__device__ inline double block_sum256(double v, double* lds4) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) lds4[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) s = (lds4[0] + lds4[1]) + (lds4[2] + lds4[3]);
+  return s;
+}
+
+// Inverse of a symmetric positive definite 3x3 matrix given as
+// (xx xy xz yy yz zz); returns false when not positive definite.
+__device__ inline bool spd3_inverse(const double* a, double* out) {
+  const double l00 = sqrt(a[0]);
+  if (!(l00 > 0.0)) return false;
+  const double l10 = a[1] / l00, l20 = a[2] / l00;
+  const double d11 = a[3] - l10 * l10;
+  if (!(d11 > 0.0)) return false;
+  const double l11 = sqrt(d11);
+  const double l21 = (a[4] - l20 * l10) / l11;
+  const double d22 = a[5] - l20 * l20 - l21 * l21;
+  if (!(d22 > 0.0)) return false;
+  const double l22 = sqrt(d22);
+  const double i00 = 1.0 / l00, i11 = 1.0 / l11, i22 = 1.0 / l22;
+  const double i10 = -l10 * i00 * i11;
+  const double i21 = -l21 * i11 * i22;
+  const double i20 = -(l20 * i00 + l21 * i10) * i22;
+  // (L^-1)^T L^-1, L^-1 lower with rows (i00), (i10 i11), (i20 i21 i22).
+  out[0] = i00 * i00 + i10 * i10 + i20 * i20;
+  out[1] = i10 * i11 + i20 * i21;
+  out[2] = i20 * i22;
+  out[3] = i11 * i11 + i21 * i21;
+  out[4] = i21 * i22;
+  out[5] = i22 * i22;
+  return true;
+}
+
+__device__ inline void sym3_full(const double* s, double* m) {
+  m[0] = s[0]; m[1] = s[1]; m[2] = s[2];
+  m[3] = s[1]; m[4] = s[3]; m[5] = s[4];
+  m[6] = s[2]; m[7] = s[4]; m[8] = s[5];
+}
+
+// Anchor map M = [-|q_a|^2 [p_B]x , R_B_LM] (3x6): the anchor-vertex
+// tangent Jacobian of an observation is J_landmark * M
+// (visual-error-term-inl.h:199-224 with the JPL lift).
+__device__ inline void anchor_map(const double* pB, const double* Ra, double n2, double* M) {
+  // -n2 [p]x = -n2 [[0,-z,y],[z,0,-x],[-y,x,0]]
+  M[0] = 0.0;         M[1] = n2 * pB[2];  M[2] = -n2 * pB[1];
+  M[6] = -n2 * pB[2]; M[7] = 0.0;         M[8] = n2 * pB[0];
+  M[12] = n2 * pB[1]; M[13] = -n2 * pB[0]; M[14] = 0.0;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) M[r * 6 + 3 + c] = Ra[r * 3 + c];
+}
+
+// ------------------------------------------------------------ vertices
+
+__global__ void k_vertex_prep(int32_t P, const double* __restrict__ vert, double* __restrict__ vR) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= P) return;
+  const double* q = vert + 7 * v;
+  double R[9];
+  rot_jpl(q, R);
+  double* out = vR + 12 * v;
+  for (int i = 0; i < 9; ++i) out[i] = R[i];
+  out[9] = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+  out[10] = 0.0;
+  out[11] = 0.0;
+}
+
+// ------------------------------------------------------------ linearization
+
+struct ObsEval {
+  double r[2];
+  double Jl[6];   // d r / d p_B, row-major 2x3
+  double Jo[12];  // observer tangent [rot 3 | pos 3], row-major 2x6
+  double rho0;
+  bool failed;
+};
+
+// Residual and Jacobians of one observation (visual-error-term-inl.h:13-379,
+// evaluated directly in the tangent space of the observer pose).
+template <bool kJac>
+__device__ inline void eval_observation(const DevTopology& t, int o, const double* __restrict__ vert,
+                                        const double* __restrict__ vR, const double* __restrict__ lm,
+                                        ObsEval& e) {
+  const int32_t meta = t.obs_meta[o];
+  const int32_t type = obs_type(meta);
+  const DevCamera& cam = t.cams[obs_camera(meta)];
+  const int32_t l = t.obs_landmark[o];
+  const int32_t v = t.obs_vertex[o];
+  const double* pB = lm + 3 * l;
+  double pI[3];
+  double G[9];  // d p_I / d p_B
+  const double* Rv = vR + 12 * v;
+  if (type == kTypeKeyframe) {
+    pI[0] = pB[0]; pI[1] = pB[1]; pI[2] = pB[2];
+  } else {
+    const int32_t a = t.lm_anchor[l];
+    const double* Ra = vR + 12 * a;        // R_B_LM
+    const double* pa = vert + 7 * a + 4;   // p_LM_B
+    double pM[3];
+    mat3t_vec(Ra, pB, pM);
+    pM[0] += pa[0]; pM[1] += pa[1]; pM[2] += pa[2];
+    double H[9];  // d p_M / d p_B
+    if (kJac) for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) H[i * 3 + j] = Ra[j * 3 + i];
+    if (type == kTypeGlobal) {
+      const double* TL = t.mission_T + 12 * t.vertex_mission[a];
+      const double* TM = t.mission_T + 12 * t.vertex_mission[v];
+      double pG[3], d[3];
+      mat3_vec(TL, pM, pG);
+      for (int i = 0; i < 3; ++i) d[i] = pG[i] + TL[9 + i] - TM[9 + i];
+      mat3t_vec(TM, d, pM);
+      if (kJac) {
+        double T1[9], T2[9];
+        mat3_mul(TL, H, T1);  // R_G_LM R_LM_B
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j)
+            T2[i * 3 + j] = TM[i] * T1[j] + TM[3 + i] * T1[3 + j] + TM[6 + i] * T1[6 + j];
+        for (int i = 0; i < 9; ++i) H[i] = T2[i];
+      }
+    }
+    const double* pv = vert + 7 * v + 4;
+    const double dm[3] = {pM[0] - pv[0], pM[1] - pv[1], pM[2] - pv[2]};
+    mat3_vec(Rv, dm, pI);
+    if (kJac) mat3_mul(Rv, H, G);
+  }
+  double pC[3];
+  mat3_vec(cam.R_C_I, pI, pC);
+  pC[0] += cam.p_C_I[0]; pC[1] += cam.p_C_I[1]; pC[2] += cam.p_C_I[2];
+  double kp[2], jp[6];
+  const int32_t status = project(cam.dist_type, cam.intr, cam.dist, cam.width, cam.height, pC, kp, jp);
+  e.failed = projection_failed(status, pC, kp);
+  const double sinv = t.obs_sinv[o];
+  const double* meas = t.obs_kp + 2 * o;
+  if (e.failed) {
+    e.r[0] = e.r[1] = 0.0;
+  } else {
+    e.r[0] = (kp[0] - meas[0]) * sinv;
+    e.r[1] = (kp[1] - meas[1]) * sinv;
+  }
+  if (!kJac) return;
+  if (e.failed) {
+    for (int i = 0; i < 6; ++i) e.Jl[i] = 0.0;
+    for (int i = 0; i < 12; ++i) e.Jo[i] = 0.0;
+    return;
+  }
+  // K = (d kp / d p_C) * sigma^-1 * R_C_I : d r / d p_I (2x3)
+  double K[6];
+  for (int r = 0; r < 2; ++r)
+    for (int c = 0; c < 3; ++c)
+      K[r * 3 + c] = sinv * (jp[r * 3] * cam.R_C_I[c] + jp[r * 3 + 1] * cam.R_C_I[3 + c] +
+                             jp[r * 3 + 2] * cam.R_C_I[6 + c]);
+  if (type == kTypeKeyframe) {
+    for (int i = 0; i < 6; ++i) e.Jl[i] = K[i];
+    for (int i = 0; i < 12; ++i) e.Jo[i] = 0.0;
+    return;
+  }
+  for (int r = 0; r < 2; ++r) {
+    const double* k = K + 3 * r;
+    for (int c = 0; c < 3; ++c) e.Jl[r * 3 + c] = k[0] * G[c] + k[1] * G[3 + c] + k[2] * G[6 + c];
+    // rotation: |q|^2 * (K_r x p_I); position: -K_r R_I_M
+    const double n2 = Rv[9];
+    e.Jo[r * 6 + 0] = n2 * (k[1] * pI[2] - k[2] * pI[1]);
+    e.Jo[r * 6 + 1] = n2 * (k[2] * pI[0] - k[0] * pI[2]);
+    e.Jo[r * 6 + 2] = n2 * (k[0] * pI[1] - k[1] * pI[0]);
+    for (int c = 0; c < 3; ++c) e.Jo[r * 6 + 3 + c] = -(k[0] * Rv[c] + k[1] * Rv[3 + c] + k[2] * Rv[6 + c]);
+  }
+}
+
+// Loss scale of one observation: HuberLoss(delta * sigma) with delta 3 / 10
+// (optimization-terms-addition.cc:189-198), Cauchy(scale_px / sigma).
+__device__ inline double loss_scale(const DevLoss& loss, int32_t type, double sinv) {
+  if (loss.type == 0) return (type == kTypeGlobal ? 10.0 : 3.0) / sinv;
+  if (loss.type == 1) return loss.scale_px * sinv;
+  return 1.0;
+}
+
+// Per observation: robust-corrected residual and Jacobians into obs_rec;
+// per landmark: V = sum Jl'Jl, V_mission (pose-coupled observations only),
+// g = sum Jl'r, g_mission; per tile: cost and fixed cost.
+template <bool kJac>
+__global__ __launch_bounds__(kObsTile) void k_linearize(DevTopology t, const double* __restrict__ vert,
+                                                         const double* __restrict__ vR,
+                                                         const double* __restrict__ lm, DevLoss loss,
+                                                         const int32_t* __restrict__ tile_begin,
+                                                         double* __restrict__ obs_rec,
+                                                         double* __restrict__ lm_acc,
+                                                         double* __restrict__ tile_cost) {
+  __shared__ double red[kJac ? kObsTile * kLmAcc : 1];
+  __shared__ int32_t seg[kJac ? kObsTile : 1];
+  __shared__ double lds4[4];
+  const int tid = threadIdx.x;
+  const int b = tile_begin[blockIdx.x], e = tile_begin[blockIdx.x + 1];
+  const int o = b + tid;
+  double acc[kLmAcc];
+  for (int i = 0; i < kLmAcc; ++i) acc[i] = 0.0;
+  double cost = 0.0, fixed = 0.0;
+  int32_t l = -1;
+  if (o < e) {
+    l = t.obs_landmark[o];
+    const uint32_t flags = t.lm_flags[l];
+    ObsEval ev;
+    if (flags & kLmActive) {
+      eval_observation<kJac>(t, o, vert, vR, lm, ev);
+      const int32_t meta = t.obs_meta[o];
+      const int32_t type = obs_type(meta);
+      const double s = ev.r[0] * ev.r[0] + ev.r[1] * ev.r[1];
+      double rho[3];
+      robust_loss(loss.type, loss_scale(loss, type, t.obs_sinv[o]), s, rho);
+      const bool pose_coupled = type != kTypeKeyframe;
+      const bool has_var = !(flags & kLmConstant) ||
+                           (pose_coupled && (obs_observer_var(meta) || t.vertex_var[t.lm_anchor[l]] >= 0));
+      if (has_var) cost = 0.5 * rho[0]; else fixed = 0.5 * rho[0];
+      if (kJac) {
+        // ceres Corrector: J~ = sqrt(rho') (I - alpha r r'/|r|^2) J, r~ = rs r.
+        const double sq1 = sqrt(rho[1]);
+        double rs = sq1, asq = 0.0;
+        if (s > 0.0 && rho[2] > 0.0) {
+          const double alpha = 1.0 - sqrt(1.0 + 2.0 * s * rho[2] / rho[1]);
+          rs = sq1 / (1.0 - alpha);
+          asq = alpha / s;
+        }
+        double rr[2] = {ev.r[0] * rs, ev.r[1] * rs};
+        auto correct = [&](double* J, int cols) {
+          for (int c = 0; c < cols; ++c) {
+            const double rtj = J[c] * ev.r[0] + J[cols + c] * ev.r[1];
+            J[c] = sq1 * (J[c] - asq * ev.r[0] * rtj);
+            J[cols + c] = sq1 * (J[cols + c] - asq * ev.r[1] * rtj);
+          }
+        };
+        correct(ev.Jl, 3);
+        if (pose_coupled) correct(ev.Jo, 6);
+        double* rec = obs_rec + (size_t)kObsRecord * o;
+        rec[0] = rr[0]; rec[1] = rr[1];
+        for (int i = 0; i < 6; ++i) rec[2 + i] = ev.Jl[i];
+        for (int i = 0; i < 12; ++i) rec[8 + i] = ev.Jo[i];
+        const double* J = ev.Jl;
+        double V[6], g[3];
+        V[0] = J[0] * J[0] + J[3] * J[3];
+        V[1] = J[0] * J[1] + J[3] * J[4];
+        V[2] = J[0] * J[2] + J[3] * J[5];
+        V[3] = J[1] * J[1] + J[4] * J[4];
+        V[4] = J[1] * J[2] + J[4] * J[5];
+        V[5] = J[2] * J[2] + J[5] * J[5];
+        for (int c = 0; c < 3; ++c) g[c] = J[c] * rr[0] + J[3 + c] * rr[1];
+        for (int i = 0; i < 6; ++i) acc[i] = V[i];
+        for (int i = 0; i < 3; ++i) acc[12 + i] = g[i];
+        if (pose_coupled) {
+          for (int i = 0; i < 6; ++i) acc[6 + i] = V[i];
+          for (int i = 0; i < 3; ++i) acc[15 + i] = g[i];
+        }
+      }
+    } else if (kJac) {
+      double* rec = obs_rec + (size_t)kObsRecord * o;
+      for (int i = 0; i < kObsRecord; ++i) rec[i] = 0.0;
+    }
+  }
+  if (kJac) {
+    for (int i = 0; i < kLmAcc; ++i) red[tid * kLmAcc + i] = acc[i];
+    seg[tid] = (o < e) ? l : -1;
+    __syncthreads();
+    // Segment heads reduce their landmark's run in a fixed order.
+    if (o < e && (tid == 0 || seg[tid - 1] != l)) {
+      double sum[kLmAcc];
+      for (int i = 0; i < kLmAcc; ++i) sum[i] = 0.0;
+      for (int k = tid; k < kObsTile && seg[k] == l; ++k)
+        for (int i = 0; i < kLmAcc; ++i) sum[i] += red[k * kLmAcc + i];
+      double* dst = lm_acc + (size_t)kLmAcc * l;
+      if (t.lm_flags[l] & kLmSplit) {
+        for (int i = 0; i < kLmAcc; ++i) atomicAdd(dst + i, sum[i]);
+      } else {
+        for (int i = 0; i < kLmAcc; ++i) dst[i] = sum[i];
+      }
+    }
+  }
+  const double c_sum = block_sum256(cost, lds4);
+  const double f_sum = block_sum256(fixed, lds4);
+  if (tid == 0) {
+    tile_cost[2 * blockIdx.x] = c_sum;
+    tile_cost[2 * blockIdx.x + 1] = f_sum;
+  }
+}
+
+// Zero the accumulators of landmarks split over several tiles.
+__global__ void k_zero_split(int32_t n, const int32_t* __restrict__ ids, double* __restrict__ lm_acc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * kLmAcc) return;
+  lm_acc[(size_t)kLmAcc * ids[i / kLmAcc] + i % kLmAcc] = 0.0;
+}
+
+// ------------------------------------------------------------ vertex rows
+
+// Per variable vertex: unscaled gradient g = J'r and diag(J'J) of its six
+// tangent columns, over its incidences (observations it makes and the
+// landmarks anchored in it).
+__global__ __launch_bounds__(kRowThreads) void k_pose_rows(DevTopology t, const double* __restrict__ vert,
+                                                            const double* __restrict__ vR,
+                                                            const double* __restrict__ lm,
+                                                            const double* __restrict__ obs_rec,
+                                                            const double* __restrict__ lm_acc,
+                                                            double* __restrict__ pose_grad,
+                                                            double* __restrict__ pose_diag) {
+  __shared__ double red[kRowThreads / 64][12];
+  const int i = blockIdx.x;
+  const int ib = t.inc_ptr[i], ie = t.inc_ptr[i + 1];
+  double g[6] = {0, 0, 0, 0, 0, 0}, d[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = ib + threadIdx.x; k < ie; k += kRowThreads) {
+    const int32_t x = t.inc[k];
+    if (x >= 0) {
+      const double* rec = obs_rec + (size_t)kObsRecord * x;
+      const double* Jo = rec + 8;
+      for (int c = 0; c < 6; ++c) {
+        g[c] += Jo[c] * rec[0] + Jo[6 + c] * rec[1];
+        d[c] += Jo[c] * Jo[c] + Jo[6 + c] * Jo[6 + c];
+      }
+    } else {
+      const int32_t l = -1 - x;
+      const int32_t a = t.lm_anchor[l];
+      double M[18], VLM[9];
+      anchor_map(lm + 3 * l, vR + 12 * a, vR[12 * a + 9], M);
+      const double* acc = lm_acc + (size_t)kLmAcc * l;
+      sym3_full(acc + 6, VLM);
+      const double* gLM = acc + 15;
+      for (int c = 0; c < 6; ++c) {
+        const double m0 = M[c], m1 = M[6 + c], m2 = M[12 + c];
+        g[c] += m0 * gLM[0] + m1 * gLM[1] + m2 * gLM[2];
+        d[c] += m0 * (VLM[0] * m0 + VLM[1] * m1 + VLM[2] * m2) + m1 * (VLM[3] * m0 + VLM[4] * m1 + VLM[5] * m2) +
+                m2 * (VLM[6] * m0 + VLM[7] * m1 + VLM[8] * m2);
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int c = 0; c < 6; ++c) {
+    g[c] = wave_sum(g[c]);
+    d[c] = wave_sum(d[c]);
+  }
+  if (lane == 0)
+    for (int c = 0; c < 6; ++c) { red[wave][c] = g[c]; red[wave][6 + c] = d[c]; }
+  __syncthreads();
+  if (threadIdx.x < 12) {
+    double s = 0.0;
+    for (int w = 0; w < kRowThreads / 64; ++w) s += red[w][threadIdx.x];
+    if (threadIdx.x < 6) pose_grad[6 * i + threadIdx.x] = s;
+    else pose_diag[6 * i + threadIdx.x - 6] = s;
+  }
+}
+
+// ceres Jacobi scaling: c = 1 / (1 + sqrt(diag(J'J))) of every tangent column.
+__global__ void k_jacobi_scaling(int32_t Pvar, int32_t L, int32_t use, const double* __restrict__ pose_diag,
+                                 const double* __restrict__ lm_acc, double* __restrict__ pose_scale,
+                                 double* __restrict__ lm_scale) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 6 * Pvar) pose_scale[i] = use ? 1.0 / (1.0 + sqrt(pose_diag[i])) : 1.0;
+  if (i < 3 * L) {
+    const int l = i / 3, c = i % 3;
+    const int di = c == 0 ? 0 : (c == 1 ? 3 : 5);
+    lm_scale[i] = use ? 1.0 / (1.0 + sqrt(lm_acc[(size_t)kLmAcc * l + di])) : 1.0;
+  }
+}
+
+// ------------------------------------------------------------ landmark pivot
+
+// Z_l = C (C V C + D^2)^-1 C, D^2 = clamp(diag(C V C)) / mu (ceres
+// LevenbergMarquardtStrategy diagonal). Zero for non-variable landmarks.
+__global__ void k_landmark_prep(DevTopology t, const double* __restrict__ lm_acc,
+                                const double* __restrict__ lm_scale, double inv_mu, double min_diag,
+                                double max_diag, double* __restrict__ lm_z, int32_t* __restrict__ fail) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= t.L) return;
+  double* z = lm_z + 6 * l;
+  const uint32_t flags = t.lm_flags[l];
+  if (!(flags & kLmActive) || (flags & kLmConstant)) {
+    for (int i = 0; i < 6; ++i) z[i] = 0.0;
+    return;
+  }
+  const double* V = lm_acc + (size_t)kLmAcc * l;
+  const double* c = lm_scale + 3 * l;
+  double A[6] = {c[0] * c[0] * V[0], c[0] * c[1] * V[1], c[0] * c[2] * V[2],
+                 c[1] * c[1] * V[3], c[1] * c[2] * V[4], c[2] * c[2] * V[5]};
+  A[0] += fmin(fmax(A[0], min_diag), max_diag) * inv_mu;
+  A[3] += fmin(fmax(A[3], min_diag), max_diag) * inv_mu;
+  A[5] += fmin(fmax(A[5], min_diag), max_diag) * inv_mu;
+  double inv[6];
+  if (!spd3_inverse(A, inv)) {
+    atomicAdd(fail, 1);
+    for (int i = 0; i < 6; ++i) z[i] = 0.0;
+    return;
+  }
+  z[0] = c[0] * c[0] * inv[0]; z[1] = c[0] * c[1] * inv[1]; z[2] = c[0] * c[2] * inv[2];
+  z[3] = c[1] * c[1] * inv[3]; z[4] = c[1] * c[2] * inv[4]; z[5] = c[2] * c[2] * inv[5];
+}
+
+// ------------------------------------------------------------ Schur rows
+
+constexpr int kSchurChunk = 96;  // S blocks accumulated in LDS per pass
+
+// W' = Jl^T (Jo C) (3x6): landmark side unscaled, pose side scaled.
+__device__ inline void wprime_obs(const double* rec, const double* cs, double* W) {
+  const double* Jl = rec + 2;
+  const double* Jo = rec + 8;
+  for (int a = 0; a < 3; ++a)
+    for (int c = 0; c < 6; ++c) W[a * 6 + c] = (Jl[a] * Jo[c] + Jl[3 + a] * Jo[6 + c]) * cs[c];
+}
+
+// One block row of S = F'F + D^2 - sum_l W_l' Z_l W_l and rhs = F'r - sum W' Z g.
+__global__ __launch_bounds__(kRowThreads) void k_schur_rows(
+    DevTopology t, const double* __restrict__ vert, const double* __restrict__ vR,
+    const double* __restrict__ lm, const double* __restrict__ obs_rec, const double* __restrict__ lm_acc,
+    const double* __restrict__ lm_z, const double* __restrict__ pose_scale,
+    const double* __restrict__ pose_diag, double inv_mu, double min_diag, double max_diag,
+    double* __restrict__ S, double* __restrict__ rhs) {
+  __shared__ double acc[kSchurChunk * 36];
+  __shared__ double Q[kRowThreads * 18];
+  __shared__ int32_t xinc[kRowThreads];
+  __shared__ int32_t xlm[kRowThreads];
+  __shared__ int32_t off[kRowThreads + 1];
+  __shared__ int32_t colv[kSchurChunk];
+  __shared__ double rhs_acc[6];
+  __shared__ int32_t wave_tot[kRowThreads / 64];
+  const int tid = threadIdx.x;
+  const int i = blockIdx.x;
+  const int v = t.var_vertex[i];
+  const int rb = t.row_ptr[i], re = t.row_ptr[i + 1];
+  const int ib = t.inc_ptr[i], ie = t.inc_ptr[i + 1];
+  const double* cs_row = pose_scale + 6 * i;
+  for (int c0 = rb; c0 < re; c0 += kSchurChunk) {
+    const int nc = min(kSchurChunk, re - c0);
+    const bool first = c0 == rb;
+    for (int k = tid; k < nc; k += kRowThreads) colv[k] = t.col_var[c0 + k];
+    for (int k = tid; k < nc * 36; k += kRowThreads) acc[k] = 0.0;
+    if (first && tid < 6) rhs_acc[tid] = 0.0;
+    __syncthreads();
+    for (int xb = ib; xb < ie; xb += kRowThreads) {
+      const int nx = min(kRowThreads, ie - xb);
+      int32_t npairs = 0;
+      if (tid < nx) {
+        const int32_t x = t.inc[xb + tid];
+        const int32_t l = x >= 0 ? t.obs_landmark[x] : -1 - x;
+        npairs = t.lm_begin[l + 1] - t.lm_begin[l] + 1;
+        xinc[tid] = x;
+        xlm[tid] = l;
+        // Q_x = [anchor: C_a M^T] - W_x'^T Z_l, rhs_x = direct - W_x'^T Z g
+        double W[18], Z[9];
+        double rx[6];
+        const double* acc_l = lm_acc + (size_t)kLmAcc * l;
+        sym3_full(lm_z + 6 * l, Z);
+        double Qx[18];
+        if (x >= 0) {
+          const double* rec = obs_rec + (size_t)kObsRecord * x;
+          wprime_obs(rec, cs_row, W);
+          const double* Jo = rec + 8;
+          for (int c = 0; c < 6; ++c) rx[c] = cs_row[c] * (Jo[c] * rec[0] + Jo[6 + c] * rec[1]);
+          for (int k = 0; k < 18; ++k) Qx[k] = 0.0;
+        } else {
+          double M[18], VLM[9];
+          anchor_map(lm + 3 * l, vR + 12 * v, vR[12 * v + 9], M);
+          sym3_full(acc_l + 6, VLM);
+          for (int a = 0; a < 3; ++a)
+            for (int c = 0; c < 6; ++c)
+              W[a * 6 + c] = (VLM[a * 3] * M[c] + VLM[a * 3 + 1] * M[6 + c] + VLM[a * 3 + 2] * M[12 + c]) * cs_row[c];
+          const double* gLM = acc_l + 15;
+          for (int c = 0; c < 6; ++c) {
+            rx[c] = cs_row[c] * (M[c] * gLM[0] + M[6 + c] * gLM[1] + M[12 + c] * gLM[2]);
+            for (int a = 0; a < 3; ++a) Qx[c * 3 + a] = cs_row[c] * M[a * 6 + c];
+          }
+        }
+        const double* gl = acc_l + 12;
+        for (int c = 0; c < 6; ++c) {
+          double wz[3];
+          for (int a = 0; a < 3; ++a)
+            wz[a] = W[c] * Z[a] + W[6 + c] * Z[3 + a] + W[12 + c] * Z[6 + a];  // (W'^T Z)[c][a]
+          for (int a = 0; a < 3; ++a) Qx[c * 3 + a] -= wz[a];
+          rx[c] -= wz[0] * gl[0] + wz[1] * gl[1] + wz[2] * gl[2];
+        }
+        for (int k = 0; k < 18; ++k) Q[tid * 18 + k] = Qx[k];
+        if (first)
+          for (int c = 0; c < 6; ++c) atomicAdd(&rhs_acc[c], rx[c]);
+      }
+      // Exclusive scan of npairs over the block.
+      {
+        const int lane = tid & 63, wave = tid >> 6;
+        int32_t incl = npairs;
+        for (int d = 1; d < 64; d <<= 1) {
+          const int32_t y = __shfl_up(incl, d, 64);
+          if (lane >= d) incl += y;
+        }
+        if (lane == 63) wave_tot[wave] = incl;
+        __syncthreads();
+        int32_t base = 0;
+        for (int w = 0; w < wave; ++w) base += wave_tot[w];
+        off[tid + 1] = base + incl;
+        if (tid == 0) off[0] = 0;
+        __syncthreads();
+      }
+      const int total = off[nx];
+      for (int f = tid; f < total; f += kRowThreads) {
+        int lo = 0, hi = nx - 1;  // largest xi with off[xi] <= f
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (off[mid] <= f) lo = mid; else hi = mid - 1;
+        }
+        const int xi = lo;
+        const int32_t x = xinc[xi];
+        const int32_t l = xlm[xi];
+        const int32_t lb = t.lm_begin[l];
+        const int32_t nobs = t.lm_begin[l + 1] - lb;
+        const int32_t y = f - off[xi];
+        const uint32_t flags = t.lm_flags[l];
+        const bool lconst = (flags & kLmConstant) != 0;
+        int32_t col;
+        double B[36];
+        const double* Qx = Q + xi * 18;
+        if (y < nobs) {
+          const int32_t oy = lb + y;
+          const int32_t meta = t.obs_meta[oy];
+          if (obs_type(meta) == kTypeKeyframe || !obs_observer_var(meta)) continue;
+          const bool same = (x == oy);
+          if (lconst && x >= 0 && !same) continue;
+          col = t.vertex_var[t.obs_vertex[oy]];
+          const double* rec = obs_rec + (size_t)kObsRecord * oy;
+          const double* cs = pose_scale + 6 * col;
+          double W[18];
+          wprime_obs(rec, cs, W);
+          for (int a = 0; a < 6; ++a)
+            for (int c = 0; c < 6; ++c)
+              B[a * 6 + c] = Qx[a * 3] * W[c] + Qx[a * 3 + 1] * W[6 + c] + Qx[a * 3 + 2] * W[12 + c];
+          if (same) {
+            const double* Jo = rec + 8;
+            for (int a = 0; a < 6; ++a)
+              for (int c = 0; c < 6; ++c) B[a * 6 + c] += cs[a] * cs[c] * (Jo[a] * Jo[c] + Jo[6 + a] * Jo[6 + c]);
+          }
+        } else {
+          if (!(flags & kLmAnchorVar)) continue;
+          const int32_t a = t.lm_anchor[l];
+          col = t.vertex_var[a];
+          const double* cs = pose_scale + 6 * col;
+          double M[18], VLM[9], W[18];
+          anchor_map(lm + 3 * l, vR + 12 * a, vR[12 * a + 9], M);
+          sym3_full(lm_acc + (size_t)kLmAcc * l + 6, VLM);
+          for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 6; ++c)
+              W[r * 6 + c] = (VLM[r * 3] * M[c] + VLM[r * 3 + 1] * M[6 + c] + VLM[r * 3 + 2] * M[12 + c]) * cs[c];
+          for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c)
+              B[r * 6 + c] = Qx[r * 3] * W[c] + Qx[r * 3 + 1] * W[6 + c] + Qx[r * 3 + 2] * W[12 + c];
+          if (x >= 0) {  // direct observer-anchor coupling: W_x'^T M C_a
+            double Wx[18];
+            wprime_obs(obs_rec + (size_t)kObsRecord * x, cs_row, Wx);
+            for (int r = 0; r < 6; ++r)
+              for (int c = 0; c < 6; ++c)
+                B[r * 6 + c] += cs[c] * (Wx[r] * M[c] + Wx[6 + r] * M[6 + c] + Wx[12 + r] * M[12 + c]);
+          }
+        }
+        // Column position inside this chunk.
+        int lo2 = 0, hi2 = nc - 1, pos = -1;
+        while (lo2 <= hi2) {
+          const int mid = (lo2 + hi2) >> 1;
+          const int cv = colv[mid];
+          if (cv == col) { pos = mid; break; }
+          if (cv < col) lo2 = mid + 1; else hi2 = mid - 1;
+        }
+        if (pos < 0) continue;
+        double* dst = acc + pos * 36;
+        for (int k = 0; k < 36; ++k) atomicAdd(dst + k, B[k]);
+      }
+      __syncthreads();
+    }
+    // LM diagonal of the pose columns on the diagonal block.
+    const int dp = t.diag_pos[i];
+    if (dp >= c0 && dp < c0 + nc && tid < 6) {
+      const double d = cs_row[tid] * cs_row[tid] * pose_diag[6 * i + tid];
+      acc[(dp - c0) * 36 + tid * 7] += fmin(fmax(d, min_diag), max_diag) * inv_mu;
+    }
+    __syncthreads();
+    for (int k = tid; k < nc * 36; k += kRowThreads) S[(size_t)c0 * 36 + k] = acc[k];
+    if (first && tid < 6) rhs[6 * i + tid] = rhs_acc[tid];
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------ PCG
+
+// 6x6 SPD inverse of every diagonal block of S (block-Jacobi preconditioner).
+__global__ void k_block_inverse(int32_t Pvar, const int32_t* __restrict__ diag_pos, const double* __restrict__ S,
+                                double* __restrict__ Minv, int32_t* __restrict__ fail) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Pvar) return;
+  const double* A = S + (size_t)36 * diag_pos[i];
+  double L[36];
+  for (int k = 0; k < 36; ++k) L[k] = 0.0;
+  bool ok = true;
+  for (int j = 0; j < 6; ++j) {
+    double s = A[j * 6 + j];
+    for (int k = 0; k < j; ++k) s -= L[j * 6 + k] * L[j * 6 + k];
+    if (!(s > 0.0)) { ok = false; s = 1.0; }
+    const double ljj = sqrt(s);
+    L[j * 6 + j] = ljj;
+    for (int r = j + 1; r < 6; ++r) {
+      double v = A[r * 6 + j];
+      for (int k = 0; k < j; ++k) v -= L[r * 6 + k] * L[j * 6 + k];
+      L[r * 6 + j] = v / ljj;
+    }
+  }
+  if (!ok) atomicAdd(fail, 1);
+  // Columns of A^-1 by forward / backward substitution.
+  double* out = Minv + (size_t)36 * i;
+  for (int c = 0; c < 6; ++c) {
+    double y[6];
+    for (int r = 0; r < 6; ++r) {
+      double s = (r == c) ? 1.0 : 0.0;
+      for (int k = 0; k < r; ++k) s -= L[r * 6 + k] * y[k];
+      y[r] = s / L[r * 6 + r];
+    }
+    for (int r = 5; r >= 0; --r) {
+      double s = y[r];
+      for (int k = r + 1; k < 6; ++k) s -= L[k * 6 + r] * y[k];
+      y[r] = s / L[r * 6 + r];
+    }
+    for (int r = 0; r < 6; ++r) out[r * 6 + c] = y[r];
+  }
+}
+
+// Scalars of one PCG solve, resident on the device.
+struct PcgState {
+  double rz;        // r'z of the current iterate
+  double bb;        // |b|^2
+  double rr;        // |r|^2
+  double tol2;      // (tol |b|)^2
+  int32_t iters;
+  int32_t done;
+  int32_t max_iters;
+  int32_t pad;
+};
+
+// Fixed-order sum of n partials (every block computes the same value).
+__device__ inline double sum_partials(const double* __restrict__ p, int n, double* lds4) {
+  double s = 0.0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) s += p[k];
+  s = block_sum256(s, lds4);
+  __shared__ double bcast;
+  if (threadIdx.x == 0) bcast = s;
+  __syncthreads();
+  return bcast;
+}
+
+// r = b, x = 0, z = M^-1 r, p = z; partials of r'z and b'b.
+__global__ __launch_bounds__(256) void k_pcg_init(int32_t Pvar, const double* __restrict__ b,
+                                                  const double* __restrict__ Minv, double* __restrict__ x,
+                                                  double* __restrict__ r, double* __restrict__ z,
+                                                  double* __restrict__ p, double* __restrict__ part) {
+  __shared__ double lds4[4];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double rz = 0.0, bb = 0.0;
+  if (i < Pvar) {
+    const double* M = Minv + (size_t)36 * i;
+    double rv[6];
+    for (int a = 0; a < 6; ++a) { rv[a] = b[6 * i + a]; r[6 * i + a] = rv[a]; x[6 * i + a] = 0.0; }
+    for (int a = 0; a < 6; ++a) {
+      double s = 0.0;
+      for (int c = 0; c < 6; ++c) s += M[a * 6 + c] * rv[c];
+      z[6 * i + a] = s;
+      p[6 * i + a] = s;
+      rz += rv[a] * s;
+      bb += rv[a] * rv[a];
+    }
+  }
+  const double s1 = block_sum256(rz, lds4);
+  const double s2 = block_sum256(bb, lds4);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = s1; part[2 * blockIdx.x + 1] = s2; }
+}
+
+__global__ void k_pcg_init_scalars(int32_t nparts, const double* __restrict__ part, double tol, int32_t max_iters,
+                                   PcgState* __restrict__ st) {
+  __shared__ double lds4[4];
+  double a = 0.0, b = 0.0;
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x) { a += part[2 * k]; b += part[2 * k + 1]; }
+  a = block_sum256(a, lds4);
+  b = block_sum256(b, lds4);
+  if (threadIdx.x == 0) {
+    st->rz = a;
+    st->bb = b;
+    st->rr = b;
+    st->tol2 = tol * tol * b;
+    st->iters = 0;
+    st->max_iters = max_iters;
+    st->done = (b <= 0.0 || max_iters <= 0) ? 1 : 0;
+  }
+}
+
+// q = S p (one block row per work-group) and per-row partial p'q.
+__global__ __launch_bounds__(256) void k_pcg_spmv(int32_t Pvar, const int32_t* __restrict__ row_ptr,
+                                                  const int32_t* __restrict__ col_var, const double* __restrict__ S,
+                                                  const double* __restrict__ p, double* __restrict__ q,
+                                                  double* __restrict__ part, const PcgState* __restrict__ st) {
+  __shared__ double red[4][6];
+  if (st->done) return;
+  const int i = blockIdx.x;
+  const int rb = row_ptr[i], re = row_ptr[i + 1];
+  double y[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = rb + threadIdx.x; k < re; k += 256) {
+    const double* blk = S + (size_t)36 * k;
+    const double* xv = p + 6 * col_var[k];
+    double xr[6];
+    for (int c = 0; c < 6; ++c) xr[c] = xv[c];
+    for (int a = 0; a < 6; ++a) {
+      double s = 0.0;
+      for (int c = 0; c < 6; ++c) s += blk[a * 6 + c] * xr[c];
+      y[a] += s;
+    }
+  }
+  for (int a = 0; a < 6; ++a) y[a] = wave_sum(y[a]);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) for (int a = 0; a < 6; ++a) red[wave][a] = y[a];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double pq = 0.0;
+    for (int a = 0; a < 6; ++a) {
+      const double s = (red[0][a] + red[1][a]) + (red[2][a] + red[3][a]);
+      q[6 * i + a] = s;
+      pq += p[6 * i + a] * s;
+    }
+    part[i] = pq;
+  }
+}
+
+// alpha = rz / p'q; x += alpha p; r -= alpha q; z = M^-1 r; partials r'z, r'r.
+__global__ __launch_bounds__(256) void k_pcg_update(int32_t Pvar, int32_t nparts, const double* __restrict__ part_pq,
+                                                    const double* __restrict__ Minv, const double* __restrict__ p,
+                                                    const double* __restrict__ q, double* __restrict__ x,
+                                                    double* __restrict__ r, double* __restrict__ z,
+                                                    double* __restrict__ part, const PcgState* __restrict__ st) {
+  __shared__ double lds4[4];
+  if (st->done) return;
+  const double pq = sum_partials(part_pq, nparts, lds4);
+  const double alpha = st->rz / pq;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double rz = 0.0, rr = 0.0;
+  if (i < Pvar) {
+    double rv[6];
+    for (int a = 0; a < 6; ++a) {
+      x[6 * i + a] += alpha * p[6 * i + a];
+      rv[a] = r[6 * i + a] - alpha * q[6 * i + a];
+      r[6 * i + a] = rv[a];
+      rr += rv[a] * rv[a];
+    }
+    const double* M = Minv + (size_t)36 * i;
+    for (int a = 0; a < 6; ++a) {
+      double s = 0.0;
+      for (int c = 0; c < 6; ++c) s += M[a * 6 + c] * rv[c];
+      z[6 * i + a] = s;
+      rz += rv[a] * s;
+    }
+  }
+  const double s1 = block_sum256(rz, lds4);
+  const double s2 = block_sum256(rr, lds4);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = s1; part[2 * blockIdx.x + 1] = s2; }
+}
+
+// beta = rz_new / rz; p = z + beta p; convergence test on |r|.
+__global__ __launch_bounds__(256) void k_pcg_direction(int32_t Pvar, int32_t nparts, const double* __restrict__ part,
+                                                       const double* __restrict__ z, double* __restrict__ p,
+                                                       PcgState* __restrict__ st) {
+  __shared__ double lds4[4];
+  __shared__ double b_rz, b_rr;
+  if (st->done) return;
+  double a = 0.0, b = 0.0;
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x) { a += part[2 * k]; b += part[2 * k + 1]; }
+  a = block_sum256(a, lds4);
+  b = block_sum256(b, lds4);
+  if (threadIdx.x == 0) { b_rz = a; b_rr = b; }
+  __syncthreads();
+  const double rz_new = b_rz;
+  const double beta = rz_new / st->rz;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < Pvar)
+    for (int k = 0; k < 6; ++k) p[6 * i + k] = z[6 * i + k] + beta * p[6 * i + k];
+}
+
+// Scalar bookkeeping after k_pcg_direction (one thread).
+__global__ void k_pcg_scalars(int32_t nparts, const double* __restrict__ part, PcgState* __restrict__ st) {
+  __shared__ double lds4[4];
+  if (st->done) return;
+  double a = 0.0, b = 0.0;
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x) { a += part[2 * k]; b += part[2 * k + 1]; }
+  a = block_sum256(a, lds4);
+  b = block_sum256(b, lds4);
+  if (threadIdx.x == 0) {
+    st->rz = a;
+    st->rr = b;
+    st->iters += 1;
+    if (b <= st->tol2 || st->iters >= st->max_iters) st->done = 1;
+  }
+}
+
+// ------------------------------------------------------------ step
+
+// Landmark step and model cost change. With y the solution of the scaled
+// system, w = C_p y_p (per vertex) and u = C_l y_l = Z_l (g_l - sum W' y_p):
+// J y = Jl u + Jo w_o + Jl M w_a per observation and
+// model_cost_change = sum (J y)'r - |J y|^2 / 2 (ceres ComputeTrustRegionStep).
+__global__ __launch_bounds__(256) void k_back_substitute(DevTopology t, const double* __restrict__ vR,
+                                                         const double* __restrict__ lm,
+                                                         const double* __restrict__ obs_rec,
+                                                         const double* __restrict__ lm_acc,
+                                                         const double* __restrict__ lm_z,
+                                                         const double* __restrict__ w, double* __restrict__ u_out,
+                                                         double* __restrict__ part) {
+  __shared__ double lds4[4];
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  double mcc = 0.0, uu = 0.0;
+  if (l < t.L && (t.lm_flags[l] & kLmActive)) {
+    const uint32_t flags = t.lm_flags[l];
+    const int lb = t.lm_begin[l], le = t.lm_begin[l + 1];
+    const bool avar = (flags & kLmAnchorVar) != 0;
+    const bool lvar = !(flags & kLmConstant);
+    double M[18], wa[6] = {0, 0, 0, 0, 0, 0}, Mwa[3] = {0, 0, 0};
+    const double* acc = lm_acc + (size_t)kLmAcc * l;
+    if (avar) {
+      const int a = t.lm_anchor[l];
+      anchor_map(lm + 3 * l, vR + 12 * a, vR[12 * a + 9], M);
+      for (int c = 0; c < 6; ++c) wa[c] = w[6 * t.vertex_var[a] + c];
+      for (int r = 0; r < 3; ++r) {
+        double s = 0.0;
+        for (int c = 0; c < 6; ++c) s += M[r * 6 + c] * wa[c];
+        Mwa[r] = s;
+      }
+    }
+    double u[3] = {0, 0, 0};
+    if (lvar) {
+      double tv[3] = {acc[12], acc[13], acc[14]};
+      for (int o = lb; o < le; ++o) {
+        const int32_t meta = t.obs_meta[o];
+        if (obs_type(meta) == kTypeKeyframe || !obs_observer_var(meta)) continue;
+        const double* rec = obs_rec + (size_t)kObsRecord * o;
+        const double* wo = w + 6 * t.vertex_var[t.obs_vertex[o]];
+        const double* Jl = rec + 2;
+        const double* Jo = rec + 8;
+        double jw[2];
+        for (int r = 0; r < 2; ++r) {
+          double s = 0.0;
+          for (int c = 0; c < 6; ++c) s += Jo[r * 6 + c] * wo[c];
+          jw[r] = s;
+        }
+        for (int a = 0; a < 3; ++a) tv[a] -= Jl[a] * jw[0] + Jl[3 + a] * jw[1];
+      }
+      if (avar) {
+        double VLM[9];
+        sym3_full(acc + 6, VLM);
+        for (int a = 0; a < 3; ++a) tv[a] -= VLM[a * 3] * Mwa[0] + VLM[a * 3 + 1] * Mwa[1] + VLM[a * 3 + 2] * Mwa[2];
+      }
+      double Z[9];
+      sym3_full(lm_z + 6 * l, Z);
+      for (int a = 0; a < 3; ++a) u[a] = Z[a * 3] * tv[0] + Z[a * 3 + 1] * tv[1] + Z[a * 3 + 2] * tv[2];
+      uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    }
+    u_out[3 * l] = u[0]; u_out[3 * l + 1] = u[1]; u_out[3 * l + 2] = u[2];
+    for (int o = lb; o < le; ++o) {
+      const int32_t meta = t.obs_meta[o];
+      const double* rec = obs_rec + (size_t)kObsRecord * o;
+      const double* Jl = rec + 2;
+      double lu[3] = {u[0], u[1], u[2]};
+      const bool coupled = obs_type(meta) != kTypeKeyframe;
+      if (coupled && avar) { lu[0] += Mwa[0]; lu[1] += Mwa[1]; lu[2] += Mwa[2]; }
+      double jy[2];
+      for (int r = 0; r < 2; ++r) jy[r] = Jl[r * 3] * lu[0] + Jl[r * 3 + 1] * lu[1] + Jl[r * 3 + 2] * lu[2];
+      if (coupled && obs_observer_var(meta)) {
+        const double* wo = w + 6 * t.vertex_var[t.obs_vertex[o]];
+        const double* Jo = rec + 8;
+        for (int r = 0; r < 2; ++r)
+          for (int c = 0; c < 6; ++c) jy[r] += Jo[r * 6 + c] * wo[c];
+      }
+      mcc += jy[0] * rec[0] + jy[1] * rec[1] - 0.5 * (jy[0] * jy[0] + jy[1] * jy[1]);
+    }
+  } else if (l < t.L) {
+    u_out[3 * l] = u_out[3 * l + 1] = u_out[3 * l + 2] = 0.0;
+  }
+  const double s1 = block_sum256(mcc, lds4);
+  const double s2 = block_sum256(uu, lds4);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = s1; part[2 * blockIdx.x + 1] = s2; }
+}
+
+// w = C_p y_p and |w|^2 partials.
+__global__ __launch_bounds__(256) void k_unscale_poses(int32_t Pvar, const double* __restrict__ y,
+                                                       const double* __restrict__ pose_scale,
+                                                       double* __restrict__ w, double* __restrict__ part) {
+  __shared__ double lds4[4];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double s = 0.0;
+  if (i < 6 * Pvar) {
+    const double v = y[i] * pose_scale[i];
+    w[i] = v;
+    s = v * v;
+  }
+  const double t = block_sum256(s, lds4);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// Candidate state x (+) delta with delta = -w (vertices, JPL Plus) and
+// -u (landmarks); partials of |x|^2 over variable blocks.
+__global__ __launch_bounds__(256) void k_candidate_vertices(DevTopology t, const double* __restrict__ vert,
+                                                            const double* __restrict__ w,
+                                                            double* __restrict__ cand, double* __restrict__ part) {
+  __shared__ double lds4[4];
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  double xx = 0.0;
+  if (v < t.P) {
+    const double* x = vert + 7 * v;
+    double* out = cand + 7 * v;
+    const int32_t j = t.vertex_var[v];
+    if (j >= 0) {
+      const double* wv = w + 6 * j;
+      const double d[3] = {-wv[0], -wv[1], -wv[2]};
+      quat_plus_jpl(x, d, out);
+      out[4] = x[4] - wv[3];
+      out[5] = x[5] - wv[4];
+      out[6] = x[6] - wv[5];
+      for (int k = 0; k < 7; ++k) xx += x[k] * x[k];
+    } else {
+      for (int k = 0; k < 7; ++k) out[k] = x[k];
+    }
+  }
+  const double s = block_sum256(xx, lds4);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_candidate_landmarks(DevTopology t, const double* __restrict__ lm,
+                                                             const double* __restrict__ u,
+                                                             double* __restrict__ cand, double* __restrict__ part) {
+  __shared__ double lds4[4];
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  double xx = 0.0;
+  if (l < t.L) {
+    const uint32_t f = t.lm_flags[l];
+    const bool var = (f & kLmActive) && !(f & kLmConstant);
+    for (int k = 0; k < 3; ++k) {
+      const double x = lm[3 * l + k];
+      cand[3 * l + k] = var ? x - u[3 * l + k] : x;
+      if (var) xx += x * x;
+    }
+  }
+  const double s = block_sum256(xx, lds4);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// max |x - Plus(x, -g)| over variable blocks (ceres gradient_max_norm).
+__global__ __launch_bounds__(256) void k_gradient_norm(DevTopology t, const double* __restrict__ vert,
+                                                       const double* __restrict__ pose_grad,
+                                                       const double* __restrict__ lm_acc,
+                                                       double* __restrict__ part) {
+  __shared__ double red[4];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  double m = 0.0;
+  if (k < t.Pvar) {
+    const double* x = vert + 7 * t.var_vertex[k];
+    const double* g = pose_grad + 6 * k;
+    const double d[3] = {-g[0], -g[1], -g[2]};
+    double q[4];
+    quat_plus_jpl(x, d, q);
+    for (int c = 0; c < 4; ++c) m = fmax(m, fabs(x[c] - q[c]));
+    for (int c = 0; c < 3; ++c) m = fmax(m, fabs(g[3 + c]));
+  } else if (k - t.Pvar < t.L) {
+    const int l = k - t.Pvar;
+    const uint32_t f = t.lm_flags[l];
+    if ((f & kLmActive) && !(f & kLmConstant)) {
+      const double* g = lm_acc + (size_t)kLmAcc * l + 12;
+      for (int c = 0; c < 3; ++c) m = fmax(m, fabs(g[c]));
+    }
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+// Fixed-order final reduction of partial arrays: out[k] = sum (or max) of
+// parts[k] (stride `stride`, n entries). One block per output.
+__global__ void k_reduce(int32_t n, int32_t stride, int32_t offset, int32_t is_max, const double* __restrict__ parts,
+                         double* __restrict__ out) {
+  __shared__ double lds4[4];
+  double s = 0.0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const double v = parts[(size_t)k * stride + offset];
+    s = is_max ? fmax(s, v) : s + v;
+  }
+  if (is_max) {
+    s = wave_max(s);
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  } else {
+    s = block_sum256(s, lds4);
+    if (threadIdx.x == 0) *out = s;
+  }
+}
+
+// ------------------------------------------------------------ Evaluate API
+
+// Every Jacobian block of VisualReprojectionError::Evaluate
+// (visual-error-term-inl.h:184-367), in the caller's order, ambient or
+// tangent. Observation index is in the original (caller) order via perm.
+__global__ __launch_bounds__(256) void k_evaluate_full(DevTopology t, const double* __restrict__ vert,
+                                                       const double* __restrict__ vR,
+                                                       const double* __restrict__ lm,
+                                                       const double* __restrict__ baseframes,
+                                                       const double* __restrict__ cam_q, int32_t tangent,
+                                                       const int32_t* __restrict__ perm, double* __restrict__ res,
+                                                       double* __restrict__ jout, int32_t jstride,
+                                                       uint32_t block_mask) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= t.N) return;
+  const int32_t meta = t.obs_meta[o];
+  const int32_t type = obs_type(meta);
+  const int32_t ci = obs_camera(meta);
+  const DevCamera& cam = t.cams[ci];
+  const int32_t l = t.obs_landmark[o];
+  const int32_t v = t.obs_vertex[o];
+  const int32_t a = t.lm_anchor[l];
+  const double* pB = lm + 3 * l;
+  const double* Rv = vR + 12 * v;
+  const double* Ra = vR + 12 * a;
+  // Frame chain with all intermediate rotations.
+  double pI[3], pG[3] = {0, 0, 0};
+  double D_pI_pB[9], D_pI_pa[9], D_pI_pGLM[9], D_pI_pGM[9], D_pI_pv[9];
+  double D_pI_qa[9], D_pI_qGLM[9], D_pI_qGM[9], D_pI_qv[9];
+  for (int k = 0; k < 9; ++k) {
+    D_pI_pB[k] = D_pI_pa[k] = D_pI_pGLM[k] = D_pI_pGM[k] = D_pI_pv[k] = 0.0;
+    D_pI_qa[k] = D_pI_qGLM[k] = D_pI_qGM[k] = D_pI_qv[k] = 0.0;
+  }
+  auto skew_right = [](const double* A, const double* p, double* out) {  // A [p]x
+    for (int r = 0; r < 3; ++r) {
+      out[r * 3 + 0] = A[r * 3 + 1] * p[2] - A[r * 3 + 2] * p[1];
+      out[r * 3 + 1] = A[r * 3 + 2] * p[0] - A[r * 3 + 0] * p[2];
+      out[r * 3 + 2] = A[r * 3 + 0] * p[1] - A[r * 3 + 1] * p[0];
+    }
+  };
+  if (type == kTypeKeyframe) {
+    for (int k = 0; k < 3; ++k) pI[k] = pB[k];
+    for (int k = 0; k < 9; ++k) D_pI_pB[k] = (k % 4 == 0) ? 1.0 : 0.0;
+  } else {
+    double pM[3];
+    mat3t_vec(Ra, pB, pM);
+    const double* pa = vert + 7 * a + 4;
+    for (int k = 0; k < 3; ++k) pM[k] += pa[k];
+    double A[9];  // d p_I / d p_M
+    for (int k = 0; k < 9; ++k) A[k] = Rv[k];
+    double P_M_LM[9];  // d p_M / d p_LM
+    for (int k = 0; k < 9; ++k) P_M_LM[k] = (k % 4 == 0) ? 1.0 : 0.0;
+    if (type == kTypeGlobal) {
+      const double* TL = t.mission_T + 12 * t.vertex_mission[a];
+      const double* TM = t.mission_T + 12 * t.vertex_mission[v];
+      double pLM[3] = {pM[0], pM[1], pM[2]};
+      mat3_vec(TL, pLM, pG);
+      for (int k = 0; k < 3; ++k) pG[k] += TL[9 + k];
+      double d[3] = {pG[0] - TM[9], pG[1] - TM[10], pG[2] - TM[11]};
+      mat3t_vec(TM, d, pM);
+      // d p_M / d p_LM = R_M_G R_G_LM
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+          P_M_LM[r * 3 + c] = TM[r] * TL[c] + TM[3 + r] * TL[3 + c] + TM[6 + r] * TL[6 + c];
+      double AR[9];  // d p_I / d p_G = R_I_M R_M_G
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) AR[r * 3 + c] = A[r * 3] * TM[c * 3] + A[r * 3 + 1] * TM[c * 3 + 1] + A[r * 3 + 2] * TM[c * 3 + 2];
+      for (int k = 0; k < 9; ++k) { D_pI_pGLM[k] = AR[k]; D_pI_pGM[k] = -AR[k]; }
+      // Reference forms (visual-error-term-inl.h:208-209) use [p_G]x for both baseframes.
+      skew_right(D_pI_pGLM, pG, D_pI_qGLM);
+      skew_right(D_pI_pGM, pG, D_pI_qGM);
+    }
+    const double* pv = vert + 7 * v + 4;
+    const double dm[3] = {pM[0] - pv[0], pM[1] - pv[1], pM[2] - pv[2]};
+    mat3_vec(Rv, dm, pI);
+    for (int k = 0; k < 9; ++k) D_pI_pv[k] = -A[k];
+    double T1[9];
+    mat3_mul(A, P_M_LM, T1);  // d p_I / d p_LM
+    for (int k = 0; k < 9; ++k) D_pI_pa[k] = T1[k];
+    mat3_mul_bt(T1, Ra, D_pI_pB);  // * R_LM_B
+    skew_right(D_pI_pB, pB, D_pI_qa);
+    for (int k = 0; k < 9; ++k) D_pI_qa[k] = -D_pI_qa[k];
+    double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    skew_right(I3, pI, D_pI_qv);  // [p_I]x
+  }
+  double pC[3];
+  mat3_vec(cam.R_C_I, pI, pC);
+  for (int k = 0; k < 3; ++k) pC[k] += cam.p_C_I[k];
+  double kp[2], jp[6];
+  const int32_t status = project(cam.dist_type, cam.intr, cam.dist, cam.width, cam.height, pC, kp, jp);
+  const bool failed = projection_failed(status, pC, kp);
+  const double sinv = t.obs_sinv[o];
+  const int32_t oo = perm[o];
+  if (res) {
+    res[2 * oo] = failed ? 0.0 : (kp[0] - t.obs_kp[2 * o]) * sinv;
+    res[2 * oo + 1] = failed ? 0.0 : (kp[1] - t.obs_kp[2 * o + 1]) * sinv;
+  }
+  if (!jout) return;
+  // K = d r / d p_I = jp sinv R_C_I; Kc = jp sinv (d r / d p_C).
+  double Kc[6], K[6];
+  for (int k = 0; k < 6; ++k) Kc[k] = failed ? 0.0 : jp[k] * sinv;
+  for (int r = 0; r < 2; ++r)
+    for (int c = 0; c < 3; ++c)
+      K[r * 3 + c] = Kc[r * 3] * cam.R_C_I[c] + Kc[r * 3 + 1] * cam.R_C_I[3 + c] + Kc[r * 3 + 2] * cam.R_C_I[6 + c];
+  // Block offsets inside the output record of this observation.
+  const int amb[9] = {3, 7, 7, 7, 7, 4, 3, 4, 4};
+  const int tng[9] = {3, 6, 6, 6, 6, 3, 3, 4, 4};
+  double* rec = jout + (size_t)oo * jstride;
+  int offs[9];
+  int acc_off = 0;
+  for (int b = 0; b < 9; ++b) { offs[b] = acc_off; acc_off += 2 * (tangent ? tng[b] : amb[b]); }
+  auto zero_block = [&](int b) {
+    const int n = 2 * (tangent ? tng[b] : amb[b]);
+    for (int k = 0; k < n; ++k) rec[offs[b] + k] = 0.0;
+  };
+  // Write J = Kx * D (2x3) into columns [c0, c0+3) of a block of width w.
+  auto put3 = [&](int b, int c0, const double* Kx, const double* D, double scale) {
+    const int w = tangent ? tng[b] : amb[b];
+    for (int r = 0; r < 2; ++r)
+      for (int c = 0; c < 3; ++c)
+        rec[offs[b] + r * w + c0 + c] =
+            scale * (Kx[r * 3] * D[c] + Kx[r * 3 + 1] * D[3 + c] + Kx[r * 3 + 2] * D[6 + c]);
+  };
+  // Rotation block: tangent = J_q |q|^2 ; ambient = J_q * 4 Jq(q)^T.
+  auto put_rot = [&](int b, const double* Kx, const double* D, const double* q) {
+    double Jr[6];
+    for (int r = 0; r < 2; ++r)
+      for (int c = 0; c < 3; ++c) Jr[r * 3 + c] = Kx[r * 3] * D[c] + Kx[r * 3 + 1] * D[3 + c] + Kx[r * 3 + 2] * D[6 + c];
+    const int w = tangent ? tng[b] : amb[b];
+    if (tangent) {
+      const double n2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+      for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 3; ++c) rec[offs[b] + r * w + c] = Jr[r * 3 + c] * n2;
+    } else {
+      // 4 * 0.5 * [[w,-z,y],[z,w,-x],[-y,x,w],[-x,-y,-z]]^T (quaternion-param-jpl.cc:8-32)
+      const double x = q[0], y = q[1], z = q[2], w4 = q[3];
+      const double JqT[12] = {w4, z, -y, -x, -z, w4, x, -y, y, -x, w4, -z};  // 3x4
+      for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 4; ++c)
+          rec[offs[b] + r * w + c] =
+              2.0 * (Jr[r * 3] * JqT[c] + Jr[r * 3 + 1] * JqT[4 + c] + Jr[r * 3 + 2] * JqT[8 + c]);
+    }
+  };
+  const double* qa = vert + 7 * a;
+  const double* qv = vert + 7 * v;
+  const double* qGLM = baseframes + 7 * t.vertex_mission[a];
+  const double* qGM = baseframes + 7 * t.vertex_mission[v];
+  const double* qCI = cam_q + 7 * ci;
+  const int rot0 = tangent ? 0 : 0;
+  (void)rot0;
+  for (int b = 0; b < 9; ++b) {
+    if (!((block_mask >> b) & 1u)) continue;
+    zero_block(b);
+    if (failed) continue;
+    const bool used = (b == 0) || (b == 5) || (b == 6) || (b == 7) || (b == 8) ||
+                      ((b == 1 || b == 4) && type != kTypeKeyframe) || ((b == 2 || b == 3) && type == kTypeGlobal);
+    if (!used) continue;
+    const int pc0 = tangent ? 3 : 4;  // column of the position part of a pose block
+    switch (b) {
+      case 0: put3(0, 0, K, D_pI_pB, 1.0); break;
+      case 1: put_rot(1, K, D_pI_qa, qa); put3(1, pc0, K, D_pI_pa, 1.0); break;
+      case 2: put_rot(2, K, D_pI_qGLM, qGLM); put3(2, pc0, K, D_pI_pGLM, 1.0); break;
+      case 3: put_rot(3, K, D_pI_qGM, qGM); put3(3, pc0, K, D_pI_pGM, 1.0); break;
+      case 4: put_rot(4, K, D_pI_qv, qv); put3(4, pc0, K, D_pI_pv, 1.0); break;
+      case 5: {
+        // d p_C / d q_CI = [R_C_I p_I]x (visual-error-term-inl.h:186)
+        double rp[3];
+        mat3_vec(cam.R_C_I, pI, rp);
+        double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, Sk[9];
+        skew_right(I3, rp, Sk);
+        put_rot(5, Kc, Sk, qCI);
+        break;
+      }
+      case 6: {
+        double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+        put3(6, 0, Kc, I3, 1.0);
+        break;
+      }
+      case 7: {
+        // d kp / d (fu fv cu cv) at the distorted normalized point.
+        const double u = (kp[0] - cam.intr[2]) / cam.intr[0], vv = (kp[1] - cam.intr[3]) / cam.intr[1];
+        double* J = rec + offs[7];
+        J[0] = u * sinv; J[1] = 0.0; J[2] = sinv; J[3] = 0.0;
+        J[4] = 0.0; J[5] = vv * sinv; J[6] = 0.0; J[7] = sinv;
+        break;
+      }
+      case 8: {
+        // d kp / d distortion at the undistorted normalized point, rows x f.
+        double* J = rec + offs[8];
+        const double x = pC[0] / pC[2], y = pC[1] / pC[2];
+        double d[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (cam.dist_type == kDistRadtan) {
+          const double r2 = x * x + y * y, r4 = r2 * r2;
+          d[0] = x * r2; d[1] = x * r4; d[2] = 2.0 * x * y; d[3] = r2 + 2.0 * x * x;
+          d[4] = y * r2; d[5] = y * r4; d[6] = r2 + 2.0 * y * y; d[7] = 2.0 * x * y;
+        } else if (cam.dist_type == kDistEquidistant) {
+          const double r = sqrt(x * x + y * y);
+          if (r >= 1e-10) {
+            const double th = atan(r);
+            const double t3 = th * th * th, t5 = t3 * th * th, t7 = t5 * th * th, t9 = t7 * th * th;
+            d[0] = x * t3 / r; d[1] = x * t5 / r; d[2] = x * t7 / r; d[3] = x * t9 / r;
+            d[4] = y * t3 / r; d[5] = y * t5 / r; d[6] = y * t7 / r; d[7] = y * t9 / r;
+          }
+        }
+        for (int k = 0; k < 4; ++k) {
+          J[k] = d[k] * cam.intr[0] * sinv;
+          J[4 + k] = d[4 + k] * cam.intr[1] * sinv;
+        }
+        break;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------ launchers
+
+void launch_vertex_prep(hipStream_t s, int32_t P, const double* vert, double* vR) {
+  if (P > 0) k_vertex_prep<<<(P + 255) / 256, 256, 0, s>>>(P, vert, vR);
+}
+void launch_linearize(hipStream_t s, bool jac, int32_t tiles, const DevTopology& t, const double* vert,
+                      const double* vR, const double* lm, DevLoss loss, const int32_t* tile_begin, double* obs_rec,
+                      double* lm_acc, double* tile_cost) {
+  if (tiles <= 0) return;
+  if (jac)
+    k_linearize<true><<<tiles, kObsTile, 0, s>>>(t, vert, vR, lm, loss, tile_begin, obs_rec, lm_acc, tile_cost);
+  else
+    k_linearize<false><<<tiles, kObsTile, 0, s>>>(t, vert, vR, lm, loss, tile_begin, obs_rec, lm_acc, tile_cost);
+}
+void launch_zero_split(hipStream_t s, int32_t n, const int32_t* ids, double* lm_acc) {
+  if (n > 0) k_zero_split<<<(n * kLmAcc + 255) / 256, 256, 0, s>>>(n, ids, lm_acc);
+}
+void launch_pose_rows(hipStream_t s, const DevTopology& t, const double* vert, const double* vR, const double* lm,
+                      const double* obs_rec, const double* lm_acc, double* pose_grad, double* pose_diag) {
+  if (t.Pvar > 0) k_pose_rows<<<t.Pvar, kRowThreads, 0, s>>>(t, vert, vR, lm, obs_rec, lm_acc, pose_grad, pose_diag);
+}
+void launch_jacobi_scaling(hipStream_t s, int32_t Pvar, int32_t L, int32_t use, const double* pose_diag,
+                           const double* lm_acc, double* pose_scale, double* lm_scale) {
+  const int n = std::max(6 * Pvar, 3 * L);
+  if (n > 0) k_jacobi_scaling<<<(n + 255) / 256, 256, 0, s>>>(Pvar, L, use, pose_diag, lm_acc, pose_scale, lm_scale);
+}
+void launch_landmark_prep(hipStream_t s, const DevTopology& t, const double* lm_acc, const double* lm_scale,
+                          double inv_mu, double min_diag, double max_diag, double* lm_z, int32_t* fail) {
+  if (t.L > 0)
+    k_landmark_prep<<<(t.L + 255) / 256, 256, 0, s>>>(t, lm_acc, lm_scale, inv_mu, min_diag, max_diag, lm_z, fail);
+}
+void launch_schur_rows(hipStream_t s, const DevTopology& t, const double* vert, const double* vR, const double* lm,
+                       const double* obs_rec, const double* lm_acc, const double* lm_z, const double* pose_scale,
+                       const double* pose_diag, double inv_mu, double min_diag, double max_diag, double* S,
+                       double* rhs) {
+  if (t.Pvar > 0)
+    k_schur_rows<<<t.Pvar, kRowThreads, 0, s>>>(t, vert, vR, lm, obs_rec, lm_acc, lm_z, pose_scale, pose_diag, inv_mu,
+                                                min_diag, max_diag, S, rhs);
+}
+void launch_block_inverse(hipStream_t s, int32_t Pvar, const int32_t* diag_pos, const double* S, double* Minv,
+                          int32_t* fail) {
+  if (Pvar > 0) k_block_inverse<<<(Pvar + 63) / 64, 64, 0, s>>>(Pvar, diag_pos, S, Minv, fail);
+}
+
+// PCG on S y = b. Vectors: x (solution), r, z, p, q (6 Pvar each); parts
+// sized for max(Pvar, 2 * blocks). Runs `max_iters` iterations at most,
+// polling the device flag every `poll` iterations.
+int launch_pcg(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_t* col_var, const int32_t* diag_pos,
+               const double* S, const double* Minv, const double* b, double* x, double* r, double* z, double* p,
+               double* q, double* parts_a, double* parts_b, void* state_dev, void* state_host, double tol,
+               int32_t max_iters, int32_t poll) {
+  PcgState* st = static_cast<PcgState*>(state_dev);
+  PcgState* hs = static_cast<PcgState*>(state_host);
+  const int vb = (Pvar + 255) / 256;
+  k_pcg_init<<<vb, 256, 0, s>>>(Pvar, b, Minv, x, r, z, p, parts_a);
+  k_pcg_init_scalars<<<1, 256, 0, s>>>(vb, parts_a, tol, max_iters, st);
+  int launched = 0;
+  while (launched < max_iters) {
+    const int n = std::min(poll, max_iters - launched);
+    for (int k = 0; k < n; ++k) {
+      k_pcg_spmv<<<Pvar, 256, 0, s>>>(Pvar, row_ptr, col_var, S, p, q, parts_a, st);
+      k_pcg_update<<<vb, 256, 0, s>>>(Pvar, Pvar, parts_a, Minv, p, q, x, r, z, parts_b, st);
+      k_pcg_direction<<<vb, 256, 0, s>>>(Pvar, vb, parts_b, z, p, st);
+      k_pcg_scalars<<<1, 256, 0, s>>>(vb, parts_b, st);
+    }
+    launched += n;
+    if (hipMemcpyAsync(hs, st, sizeof(PcgState), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+    if (hipStreamSynchronize(s) != hipSuccess) return -1;
+    if (hs->done) break;
+  }
+  (void)diag_pos;
+  return hs->iters;
+}
+size_t pcg_state_bytes() { return sizeof(PcgState); }
+
+void launch_back_substitute(hipStream_t s, const DevTopology& t, const double* vR, const double* lm,
+                            const double* obs_rec, const double* lm_acc, const double* lm_z, const double* w,
+                            double* u, double* part) {
+  if (t.L > 0) k_back_substitute<<<(t.L + 255) / 256, 256, 0, s>>>(t, vR, lm, obs_rec, lm_acc, lm_z, w, u, part);
+}
+void launch_unscale_poses(hipStream_t s, int32_t Pvar, const double* y, const double* pose_scale, double* w,
+                          double* part) {
+  if (Pvar > 0) k_unscale_poses<<<(6 * Pvar + 255) / 256, 256, 0, s>>>(Pvar, y, pose_scale, w, part);
+}
+void launch_candidate(hipStream_t s, const DevTopology& t, const double* vert, const double* w, double* cand_v,
+                      double* part_v, const double* lm, const double* u, double* cand_l, double* part_l) {
+  if (t.P > 0) k_candidate_vertices<<<(t.P + 255) / 256, 256, 0, s>>>(t, vert, w, cand_v, part_v);
+  if (t.L > 0) k_candidate_landmarks<<<(t.L + 255) / 256, 256, 0, s>>>(t, lm, u, cand_l, part_l);
+}
+void launch_gradient_norm(hipStream_t s, const DevTopology& t, const double* vert, const double* pose_grad,
+                          const double* lm_acc, double* part) {
+  const int n = t.Pvar + t.L;
+  if (n > 0) k_gradient_norm<<<(n + 255) / 256, 256, 0, s>>>(t, vert, pose_grad, lm_acc, part);
+}
+void launch_reduce(hipStream_t s, int32_t n, int32_t stride, int32_t offset, bool is_max, const double* parts,
+                   double* out) {
+  k_reduce<<<1, 256, 0, s>>>(n, stride, offset, is_max ? 1 : 0, parts, out);
+}
+void launch_evaluate_full(hipStream_t s, const DevTopology& t, const double* vert, const double* vR, const double* lm,
+                          const double* baseframes, const double* cam_q, bool tangent, const int32_t* perm,
+                          double* res, double* jout, int32_t jstride, uint32_t mask) {
+  if (t.N > 0)
+    k_evaluate_full<<<(t.N + 255) / 256, 256, 0, s>>>(t, vert, vR, lm, baseframes, cam_q, tangent ? 1 : 0, perm, res,
+                                                      jout, jstride, mask);
+}
+
+}  // namespace mlba

@@ -1,0 +1,93 @@
+"""Device-resident visual BA problem: a thin owner of an mlba_handle.
+
+This is the Python side of the C ABI (include/maplab_ba.h); the maplab-shaped
+driver on top of it is maplab_amd.optimizer.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import abi
+from .problem import VisualMap
+
+
+class DeviceProblem:
+    """The map copied into HBM with its observation and Schur topology."""
+
+    def __init__(self, vmap: VisualMap):
+        self.lib = abi.load_library()
+        self.map = vmap
+        self._desc = vmap.desc()
+        h = ctypes.c_void_p()
+        abi.check(self.lib.mlba_create(ctypes.byref(self._desc), ctypes.byref(h)))
+        self.handle = h
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.mlba_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int) -> None:
+        abi.check(rc, self.handle)
+
+    def upload_state(self, vertices=None, landmarks=None) -> None:
+        v = None if vertices is None else np.ascontiguousarray(vertices, dtype=np.float64)
+        l = None if landmarks is None else np.ascontiguousarray(landmarks, dtype=np.float64)
+        self._check(self.lib.mlba_upload_state(self.handle, None if v is None else v.ctypes.data,
+                                               None if l is None else l.ctypes.data))
+
+    def download_state(self):
+        v = np.empty((self.map.num_vertices, 7))
+        l = np.empty((self.map.num_landmarks, 3))
+        self._check(self.lib.mlba_download_state(self.handle, v.ctypes.data, l.ctypes.data))
+        return v, l
+
+    def set_landmark_active(self, mask) -> None:
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        self._check(self.lib.mlba_set_landmark_active(self.handle, m.ctypes.data))
+
+    def landmark_active(self) -> np.ndarray:
+        m = np.empty(self.map.num_landmarks, dtype=np.uint8)
+        self._check(self.lib.mlba_get_landmark_active(self.handle, m.ctypes.data))
+        return m
+
+    def evaluate(self, tangent: bool = False, blocks=range(9), options: abi.Options | None = None,
+                 want_cost: bool = True):
+        """Batched VisualReprojectionError::Evaluate: residuals (N, 2), a dict
+        block -> (N, 2, size) Jacobians, and the robust cost."""
+        n = self.map.num_observations
+        sizes = abi.TANGENT_BLOCK_SIZES if tangent else abi.AMBIENT_BLOCK_SIZES
+        res = np.empty((n, 2))
+        jac = {b: np.empty((n, 2, sizes[b])) for b in blocks}
+        ptrs = (ctypes.c_void_p * 9)(*[jac[b].ctypes.data if b in jac else None for b in range(9)])
+        cost = ctypes.c_double(0.0)
+        opt = options if options is not None else abi.default_options()
+        self._check(self.lib.mlba_evaluate(self.handle, abi.EVAL_TANGENT if tangent else abi.EVAL_AMBIENT,
+                                           ctypes.byref(opt), res.ctypes.data, ptrs,
+                                           ctypes.byref(cost) if want_cost else None))
+        return res, jac, cost.value
+
+    def solve(self, options: abi.Options, outlier_rejection: bool = False, capacity: int = 256):
+        """ceres::Solve (or solveWithOutlierRejection) on the device state."""
+        its = (abi.IterationSummary * capacity)()
+        sm = abi.Summary()
+        sm.iterations_capacity = capacity
+        sm.iterations = ctypes.cast(its, ctypes.POINTER(abi.IterationSummary))
+        fn = self.lib.mlba_solve_with_outlier_rejection if outlier_rejection else self.lib.mlba_solve
+        self._check(fn(self.handle, ctypes.byref(options), ctypes.byref(sm)))
+        rows = [its[i] for i in range(min(sm.num_iterations, capacity))]
+        return sm, rows

@@ -1,0 +1,89 @@
+/*
+ * ba_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * FP64 CPU restatement of maplab's visual bundle-adjustment path, used as the
+ * parity oracle for the HIP library and as the timed CPU baseline. Only the
+ * tests, the smoke check and bench.py's cpu_baseline leg load it; nothing in
+ * maplab_amd/ links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative
+ * to the maplab repository root). Parts that live in Ceres Solver (absent
+ * from the reference tree; API bounds it to Ceres 1.13-2.1) are restated
+ * from Ceres' published algorithm and marked "ceres:" in the comments.
+ */
+#ifndef MAPLAB_BA_ORACLE_H_
+#define MAPLAB_BA_ORACLE_H_
+
+#include <stdint.h>
+
+#include "../include/maplab_ba.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Linear solvers of the oracle LM. */
+#define ORACLE_SCHUR_DIRECT 0  /* landmark elimination + dense Cholesky of S */
+#define ORACLE_DENSE_NORMAL 1  /* dense Cholesky of J'J + D'D (small problems) */
+#define ORACLE_SCHUR_PCG 2     /* landmark elimination + block-Jacobi PCG on S */
+
+/* PinholeCamera::project3Functional. Returns the ProjectionResult status:
+ * 0 visible, 1 outside image box, 2 behind camera, 3 invalid.
+ * jp (2x3), jint (2x4), jdist (2x4) are row-major and nullable. */
+int32_t oracle_project(int32_t dist_type, const double* intrinsics, const double* dist,
+                       int32_t width, int32_t height, const double* p_C, double* kp,
+                       double* jp, double* jint, double* jdist);
+
+/* Distortion::distortUsingExternalCoefficients: in/out point, jd (2x2) nullable. */
+void oracle_distort(int32_t dist_type, const double* dist, double* point, double* jd);
+
+/* JplQuaternionParameterization::Plus and ::ComputeJacobian (4x3 row-major). */
+void oracle_quat_plus(const double* q, const double* delta, double* out);
+void oracle_quat_jacobian(const double* q, double* j43);
+void oracle_rotation_jpl(const double* q, double* r33);
+
+/* VisualReprojectionError::Evaluate on explicit parameter blocks.
+ * params: 9 block pointers in the reference order; jac: 9 nullable pointers
+ * (ambient, row-major). Returns 1 when the projection failed (r = J = 0). */
+int32_t oracle_evaluate_visual(const double* meas, double sigma, int32_t error_type,
+                               int32_t dist_type, int32_t width, int32_t height,
+                               const double* const* params, double* residual,
+                               double* const* jac);
+
+/* ceres::LossFunction::Evaluate for the supported losses: rho[3]. */
+void oracle_loss(int32_t loss_type, double a, double s, double* rho);
+
+/* Batched evaluate over a problem, same contract as mlba_evaluate. */
+int32_t oracle_problem_evaluate(const mlba_problem_desc* desc, const uint8_t* landmark_active,
+                                int32_t flags, const mlba_options* options,
+                                double* residuals, double* const* jacobians, double* cost);
+
+/* One Levenberg-Marquardt solve on a copy of the problem state; the final
+ * state is written to vertices_out (7P) and landmarks_out (3L) (nullable).
+ * num_threads <= 0 selects max(1, hw/2) (solver-options.h:37-38). */
+int32_t oracle_solve(const mlba_problem_desc* desc, const mlba_options* options,
+                     int32_t linear_solver, int32_t num_threads,
+                     const uint8_t* landmark_active, double* vertices_out,
+                     double* landmarks_out, mlba_summary* summary);
+
+/* solveWithOutlierRejection; landmark_active_out (L, nullable) receives the
+ * final active mask. */
+int32_t oracle_solve_with_outlier_rejection(const mlba_problem_desc* desc,
+                                            const mlba_options* options,
+                                            int32_t linear_solver, int32_t num_threads,
+                                            double* vertices_out, double* landmarks_out,
+                                            uint8_t* landmark_active_out,
+                                            mlba_summary* summary);
+
+/* The damped, Jacobi-scaled reduced camera system at the given state:
+ * S (n x n dense, n = 6 * number of variable vertices that have residuals,
+ * ordered by vertex index) and rhs (n), for LM radius `mu`. Used to check the
+ * device Schur build. Returns n, or a negative error. */
+int32_t oracle_reduced_camera_system(const mlba_problem_desc* desc, const mlba_options* options,
+                                     double mu, int32_t capacity, double* S, double* rhs,
+                                     int32_t* vertex_of_block);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MAPLAB_BA_ORACLE_H_ */

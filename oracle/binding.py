@@ -1,0 +1,181 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of the FP64 CPU oracle
+(oracle/liboracle_ba.so). Imported by tests/, the smoke check and the
+cpu_baseline leg of bench.py; never by maplab_amd/.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_PATH = os.path.join(_HERE, "liboracle_ba.so")
+
+SCHUR_DIRECT, DENSE_NORMAL, SCHUR_PCG = 0, 1, 2
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(ORACLE_PATH):
+        build()
+    from maplab_amd import abi  # structures only: the ABI data layout
+    lib = ctypes.CDLL(ORACLE_PATH)
+    vp, i32, f64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_double
+    P = ctypes.POINTER
+    sig = {
+        "oracle_project": (i32, [i32, vp, vp, i32, i32, vp, vp, vp, vp, vp]),
+        "oracle_distort": (None, [i32, vp, vp, vp]),
+        "oracle_quat_plus": (None, [vp, vp, vp]),
+        "oracle_quat_jacobian": (None, [vp, vp]),
+        "oracle_rotation_jpl": (None, [vp, vp]),
+        "oracle_evaluate_visual": (i32, [vp, f64, i32, i32, i32, i32, vp, vp, vp]),
+        "oracle_loss": (None, [i32, f64, f64, vp]),
+        "oracle_problem_evaluate": (i32, [P(abi.ProblemDesc), vp, i32, P(abi.Options), vp, vp, vp]),
+        "oracle_solve": (i32, [P(abi.ProblemDesc), P(abi.Options), i32, i32, vp, vp, vp, P(abi.Summary)]),
+        "oracle_solve_with_outlier_rejection": (i32, [P(abi.ProblemDesc), P(abi.Options), i32, i32, vp, vp, vp,
+                                                      P(abi.Summary)]),
+        "oracle_reduced_camera_system": (i32, [P(abi.ProblemDesc), P(abi.Options), f64, i32, vp, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def project(dist_type, intrinsics, dist, width, height, p_C, want_jac=True):
+    lib = load()
+    intr = np.ascontiguousarray(intrinsics, dtype=np.float64)
+    d = np.ascontiguousarray(dist if dist is not None else np.zeros(4), dtype=np.float64)
+    p = np.ascontiguousarray(p_C, dtype=np.float64)
+    kp = np.zeros(2)
+    jp, ji, jd = np.zeros((2, 3)), np.zeros((2, 4)), np.zeros((2, 4))
+    st = lib.oracle_project(dist_type, _p(intr), _p(d), width, height, _p(p), _p(kp),
+                            _p(jp) if want_jac else None, _p(ji) if want_jac else None,
+                            _p(jd) if want_jac else None)
+    return st, kp, jp, ji, jd
+
+
+def distort(dist_type, dist, point):
+    lib = load()
+    pt = np.ascontiguousarray(point, dtype=np.float64).copy()
+    jd = np.zeros((2, 2))
+    lib.oracle_distort(dist_type, _p(np.ascontiguousarray(dist, dtype=np.float64)), _p(pt), _p(jd))
+    return pt, jd
+
+
+def quat_plus(q, delta):
+    lib = load()
+    out = np.zeros(4)
+    lib.oracle_quat_plus(_p(np.ascontiguousarray(q, dtype=np.float64)),
+                         _p(np.ascontiguousarray(delta, dtype=np.float64)), _p(out))
+    return out
+
+
+def quat_jacobian(q):
+    lib = load()
+    out = np.zeros((4, 3))
+    lib.oracle_quat_jacobian(_p(np.ascontiguousarray(q, dtype=np.float64)), _p(out))
+    return out
+
+
+def rotation_jpl(q):
+    lib = load()
+    out = np.zeros((3, 3))
+    lib.oracle_rotation_jpl(_p(np.ascontiguousarray(q, dtype=np.float64)), _p(out))
+    return out
+
+
+AMBIENT = (3, 7, 7, 7, 7, 4, 3, 4, 4)
+
+
+def evaluate_visual(meas, sigma, error_type, dist_type, width, height, params):
+    """params: list of 9 arrays in the reference block order. Returns
+    (failed, residual, [9 ambient Jacobians])."""
+    lib = load()
+    arrs = [np.ascontiguousarray(p, dtype=np.float64) for p in params]
+    pp = (ctypes.c_void_p * 9)(*[a.ctypes.data for a in arrs])
+    res = np.zeros(2)
+    jac = [np.zeros((2, AMBIENT[b])) for b in range(9)]
+    jp = (ctypes.c_void_p * 9)(*[j.ctypes.data for j in jac])
+    failed = lib.oracle_evaluate_visual(_p(np.ascontiguousarray(meas, dtype=np.float64)), sigma, error_type,
+                                        dist_type, width, height, pp, _p(res), jp)
+    return bool(failed), res, jac
+
+
+def loss(loss_type, a, s):
+    lib = load()
+    rho = np.zeros(3)
+    lib.oracle_loss(loss_type, a, s, _p(rho))
+    return rho
+
+
+def problem_evaluate(vmap, tangent=False, options=None, active=None):
+    from maplab_amd import abi
+    lib = load()
+    d = vmap.desc()
+    n = vmap.num_observations
+    sizes = abi.TANGENT_BLOCK_SIZES if tangent else abi.AMBIENT_BLOCK_SIZES
+    res = np.zeros((n, 2))
+    jac = {b: np.zeros((n, 2, sizes[b])) for b in range(9)}
+    jp = (ctypes.c_void_p * 9)(*[jac[b].ctypes.data for b in range(9)])
+    cost = ctypes.c_double(0.0)
+    act = None if active is None else np.ascontiguousarray(active, dtype=np.uint8)
+    rc = lib.oracle_problem_evaluate(ctypes.byref(d), _p(act), 1 if tangent else 0,
+                                     ctypes.byref(options) if options is not None else None,
+                                     _p(res), jp, ctypes.byref(cost))
+    assert rc == 0, rc
+    return res, jac, cost.value
+
+
+def solve(vmap, options, linear_solver=SCHUR_DIRECT, threads=0, outlier_rejection=False, capacity=256,
+          active=None):
+    from maplab_amd import abi
+    lib = load()
+    d = vmap.desc()
+    v = np.zeros((vmap.num_vertices, 7))
+    l = np.zeros((vmap.num_landmarks, 3))
+    its = (abi.IterationSummary * capacity)()
+    sm = abi.Summary()
+    sm.iterations_capacity = capacity
+    sm.iterations = ctypes.cast(its, ctypes.POINTER(abi.IterationSummary))
+    if outlier_rejection:
+        act_out = np.zeros(vmap.num_landmarks, dtype=np.uint8)
+        rc = lib.oracle_solve_with_outlier_rejection(ctypes.byref(d), ctypes.byref(options), linear_solver, threads,
+                                                     _p(v), _p(l), _p(act_out), ctypes.byref(sm))
+    else:
+        act_out = None
+        act = None if active is None else np.ascontiguousarray(active, dtype=np.uint8)
+        rc = lib.oracle_solve(ctypes.byref(d), ctypes.byref(options), linear_solver, threads, _p(act), _p(v), _p(l),
+                              ctypes.byref(sm))
+    assert rc == 0, rc
+    rows = [its[i] for i in range(min(sm.num_iterations, capacity))]
+    return sm, rows, v, l, act_out
+
+
+def reduced_camera_system(vmap, options, mu):
+    lib = load()
+    d = vmap.desc()
+    nmax = 6 * vmap.num_vertices
+    S = np.zeros(nmax * nmax)
+    rhs = np.zeros(nmax)
+    vb = np.zeros(vmap.num_vertices, dtype=np.int32)
+    n = lib.oracle_reduced_camera_system(ctypes.byref(d), ctypes.byref(options), mu, nmax * nmax, _p(S), _p(rhs),
+                                         _p(vb))
+    assert n >= 0, n
+    return S[: n * n].reshape(n, n), rhs[:n], vb[: n // 6]
