@@ -1,0 +1,87 @@
+"""The C ABI library loads and exports every declared symbol, the option
+defaults are maplab's flags, and invalid problems are rejected on the host
+before any device work (no GPU needed)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from cases import tiny_map
+from maplab_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    return set(re.findall(r"^\S[^\n(]*?\b(mlba_\w+)\s*\(", text, re.M))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = abi.load_library()
+    syms = declared("maplab_ba.h")
+    assert syms == set(abi.LIB_SYMBOLS), syms ^ set(abi.LIB_SYMBOLS)
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert lib.mlba_abi_version() == 1
+
+
+def test_synth_library_exports_every_declared_symbol():
+    lib = abi.load_synth()
+    syms = declared("maplab_ba_synth.h")
+    assert syms == set(abi.SYNTH_SYMBOLS)
+    for s in syms:
+        assert hasattr(lib, s), s
+
+
+def test_struct_layouts_match_header():
+    # Field offsets are what the C compiler produces for the declarations.
+    assert ctypes.sizeof(abi.ProblemDesc) == 24 + 17 * 8
+    assert ctypes.sizeof(abi.IterationSummary) == 16 + 8 * 8
+    assert abi.Summary.iterations.offset == ctypes.sizeof(abi.Summary) - 8
+
+
+def test_option_defaults_are_maplab_flags():
+    # algorithms/map-optimization/src/solver-options.cc:3-52 and
+    # outlier-rejection-solver.cc:9-18.
+    o = abi.default_options()
+    assert o.max_num_iterations == 30
+    assert o.use_jacobi_scaling == 1
+    assert o.initial_trust_region_radius == 1e5
+    assert o.max_trust_region_radius == 1e20
+    assert o.min_trust_region_radius == 1e-32
+    assert o.function_tolerance == 1e-12
+    assert o.gradient_tolerance == 1e-10
+    assert o.parameter_tolerance == 1e-8
+    assert o.max_solver_time_in_seconds == 1e6
+    assert o.reject_outliers_every_n_iters == 3
+    assert o.use_reprojection_error == 0
+    assert o.max_reprojection_error_px == 50
+    assert o.loss_type == abi.LOSS_MAPLAB_HUBER
+
+
+def _create(m):
+    lib = abi.load_library()
+    d = m.desc()
+    h = ctypes.c_void_p()
+    rc = lib.mlba_create(ctypes.byref(d), ctypes.byref(h))
+    return rc, lib.mlba_last_error(None).decode()
+
+
+@pytest.mark.parametrize("mutate, code, text", [
+    (lambda m: m.obs_sigma.__setitem__(0, 0.0), abi.MLBA_EINVAL, "sigma"),
+    (lambda m: m.obs_landmark.__setitem__(0, 10**6), abi.MLBA_EINVAL, "obs_landmark"),
+    (lambda m: m.obs_vertex.__setitem__(0, -1), abi.MLBA_EINVAL, "obs_vertex"),
+    (lambda m: m.vertices.__setitem__((1, 3), -0.5), abi.MLBA_ESTATE, "quaternion"),
+    (lambda m: m.landmark_store_vertex.__setitem__(0, 99), abi.MLBA_EINVAL, "store_vertex"),
+    (lambda m: m.camera_distortion_type.__setitem__(0, 7), abi.MLBA_EINVAL, "distortion"),
+])
+def test_invalid_problem_is_rejected(mutate, code, text):
+    # The reference CHECK-fails on these (visual-error-term.h:38,
+    # quaternion-param-jpl.cc:44); the library returns a status instead.
+    m = tiny_map(np.random.default_rng(0), n_vertices=4, n_landmarks=10)
+    mutate(m)
+    rc, msg = _create(m)
+    assert rc == code and text in msg, (rc, msg)

@@ -1,0 +1,146 @@
+"""Parity of the HIP library (through the C ABI) with the FP64 CPU oracle.
+
+Bar: residuals and Jacobians (FP64 on both sides) agree to 1e-9 relative per
+block; LM iteration logs agree to 1e-9 relative per iteration on small maps;
+the final cost after the same number of iterations agrees to 1e-6 relative
+on the larger maps.
+"""
+import numpy as np
+import pytest
+
+from cases import tiny_map
+from maplab_amd import abi, problem
+from maplab_amd.solver import DeviceProblem
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_err(a, b):
+    return np.abs(a - b).max() / max(1e-300, np.abs(b).max())
+
+
+def opts(**kw):
+    o = abi.default_options()
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+@pytest.mark.parametrize("dist_type", [0, 1, 2])
+@pytest.mark.parametrize("global_missions", [False, True])
+@pytest.mark.parametrize("tangent", [False, True])
+def test_evaluate_matches_oracle(oracle, dist_type, global_missions, tangent):
+    rng = np.random.default_rng(20 + dist_type)
+    m = tiny_map(rng, dist_type=dist_type, n_vertices=6, n_landmarks=60, global_missions=global_missions)
+    print(f"evaluate parity: dist {dist_type} global {global_missions} tangent {tangent} N {m.num_observations}",
+          flush=True)
+    with DeviceProblem(m) as dev:
+        res, jac, cost = dev.evaluate(tangent=tangent)
+    ref_res, ref_jac, ref_cost = oracle.problem_evaluate(m, tangent=tangent, options=abi.default_options())
+    assert rel_err(res, ref_res) < 1e-12
+    for b in range(9):
+        if np.abs(ref_jac[b]).max() == 0:
+            assert np.abs(jac[b]).max() == 0, b
+            continue
+        assert rel_err(jac[b], ref_jac[b]) < 1e-9, (b, rel_err(jac[b], ref_jac[b]))
+    assert abs(cost - ref_cost) <= 1e-12 * abs(ref_cost)
+
+
+def test_evaluate_projection_failures_are_zero(oracle):
+    rng = np.random.default_rng(31)
+    m = tiny_map(rng, dist_type=1, n_vertices=5, n_landmarks=30)
+    # Push some landmarks behind / onto the camera plane of their store vertex.
+    m.landmarks[:5, 2] = [-1.0, 0.01, 0.049, 0.051, -0.2]
+    m.normalized()
+    with DeviceProblem(m) as dev:
+        res, jac, _ = dev.evaluate(tangent=False)
+    ref_res, ref_jac, _ = oracle.problem_evaluate(m, tangent=False)
+    np.testing.assert_array_equal(res == 0, ref_res == 0)
+    assert rel_err(res, ref_res) < 1e-12
+    for b in range(9):
+        np.testing.assert_allclose(jac[b], ref_jac[b], rtol=1e-9, atol=1e-12 * max(1, np.abs(ref_jac[b]).max()))
+
+
+@pytest.mark.parametrize("dist_type, global_missions, loss", [(1, False, 0), (2, True, 0), (0, False, 1),
+                                                               (1, True, 2)])
+def test_solve_iteration_log_matches_oracle(oracle, dist_type, global_missions, loss):
+    rng = np.random.default_rng(40 + dist_type)
+    m = tiny_map(rng, dist_type=dist_type, n_vertices=7, n_landmarks=80, global_missions=global_missions)
+    o = opts(max_num_iterations=10, loss_type=loss, pcg_relative_tolerance=1e-12)
+    print(f"solve parity: dist {dist_type} global {global_missions} loss {loss} N {m.num_observations}", flush=True)
+    with DeviceProblem(m) as dev:
+        sm, rows = dev.solve(o)
+        v, l = dev.download_state()
+    ref, ref_rows, rv, rl, _ = oracle.solve(m, o, linear_solver=oracle.SCHUR_DIRECT, threads=1)
+    assert sm.num_iterations == ref.num_iterations
+    assert sm.termination_type == ref.termination_type
+    np.testing.assert_allclose([r.cost for r in rows], [r.cost for r in ref_rows], rtol=1e-9)
+    np.testing.assert_array_equal([r.step_is_successful for r in rows], [r.step_is_successful for r in ref_rows])
+    np.testing.assert_allclose(v, rv, atol=1e-8)
+    np.testing.assert_allclose(l, rl, atol=1e-8)
+
+
+def test_workload1_final_cost_matches_oracle(oracle):
+    # 10 keyframes on a 1 m circle, 2000 landmarks, radtan, Cauchy(1 px), 10 iterations.
+    m = problem.workload_map(1, seed=1)
+    o = opts(max_num_iterations=10, loss_type=abi.LOSS_CAUCHY)
+    with DeviceProblem(m) as dev:
+        sm, rows = dev.solve(o)
+    ref, ref_rows, _, _, _ = oracle.solve(m, o, linear_solver=oracle.SCHUR_DIRECT, threads=4)
+    print(f"workload 1: final cost {sm.final_cost:.12e} oracle {ref.final_cost:.12e}", flush=True)
+    assert abs(sm.final_cost - ref.final_cost) <= 1e-6 * ref.final_cost
+    np.testing.assert_allclose([r.cost for r in rows], [r.cost for r in ref_rows], rtol=1e-8)
+
+
+@pytest.mark.parametrize("distortion", [1, 2])
+def test_random_walk_map_final_cost_matches_oracle(oracle, distortion):
+    # The random-walk generator of the 500-keyframe workload at 60 keyframes
+    # and 12k landmarks (radtan, and the equidistant variant), Huber, 2% outliers.
+    kw = {} if distortion == 1 else dict(distortion_type=2, distortion=(-0.01, 0.02, -0.005, 0.001))
+    m = problem.workload_map(2, seed=2, num_keyframes=60, num_landmarks=12000, **kw)
+    o = opts(max_num_iterations=10)
+    with DeviceProblem(m) as dev:
+        sm, rows = dev.solve(o)
+    ref, ref_rows, _, _, _ = oracle.solve(m, o, linear_solver=oracle.SCHUR_DIRECT, threads=4)
+    print(f"random walk ({distortion}): N {m.num_observations} final {sm.final_cost:.12e} "
+          f"oracle {ref.final_cost:.12e}", flush=True)
+    assert abs(sm.final_cost - ref.final_cost) <= 1e-6 * ref.final_cost
+    assert sm.num_iterations == ref.num_iterations
+
+
+def test_outlier_rejection_driver_matches_oracle(oracle):
+    rng = np.random.default_rng(55)
+    m = tiny_map(rng, dist_type=1, n_vertices=6, n_landmarks=60)
+    free = np.where(m.landmark_constant == 0)[0]
+    m.landmarks[free[:4], 2] = -2.0  # behind the store vertex: rejected after the first chunk
+    m.normalized()
+    o = opts(max_num_iterations=7, reject_outliers_every_n_iters=3)
+    with DeviceProblem(m) as dev:
+        sm, rows = dev.solve(o, outlier_rejection=True)
+        v, l = dev.download_state()
+        active = dev.landmark_active()
+    ref, ref_rows, rv, rl, ract = oracle.solve(m, o, outlier_rejection=True, threads=1)
+    assert sm.num_removed_landmarks == ref.num_removed_landmarks >= 4
+    np.testing.assert_array_equal(active, ract)
+    np.testing.assert_allclose([r.cost for r in rows], [r.cost for r in ref_rows], rtol=1e-9)
+    np.testing.assert_allclose(l[active == 1], rl[active == 1], atol=1e-8)
+
+
+def test_full_size_random_walk_properties():
+    # 500 keyframes, 200k landmarks, ~2M observations: size-independent checks.
+    m = problem.workload_map(2, seed=1)
+    o = opts(max_num_iterations=4)
+    print(f"full-size random walk: N {m.num_observations}", flush=True)
+    with DeviceProblem(m) as dev:
+        _, _, c0 = dev.evaluate(blocks=(), options=o)
+        sm, rows = dev.solve(o)
+        _, _, c1 = dev.evaluate(blocks=(), options=o)
+        v, l = dev.download_state()
+    assert abs(c0 - sm.initial_cost) <= 1e-12 * c0
+    assert abs(c1 - sm.final_cost) <= 1e-12 * c1
+    accepted = [r.cost for r in rows if r.step_is_successful]
+    assert all(b <= a for a, b in zip(accepted, accepted[1:]))
+    assert sm.final_cost < 0.5 * sm.initial_cost
+    assert np.isfinite(v).all() and np.isfinite(l).all()
+    np.testing.assert_allclose(np.linalg.norm(v[:, :4], axis=1), 1.0, atol=1e-12)
+    assert (v[:, 3] >= 0).all()

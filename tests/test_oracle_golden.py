@@ -1,0 +1,138 @@
+"""Pins the FP64 CPU oracle against the known answers held by the reference's
+own tests and against mathematics that fixes the result independently.
+
+Reference tests restated here:
+  aslam_cv2/aslam_cv_cameras/test/test-cameras.cc:387-425 (manual projection),
+  :491-520 (projection statuses), :88-93 / :136-146 / :190-199 (Jacobians vs
+  numdiff); test-distortions.cc:145-150, :191-197 (distortion Jacobians);
+  algorithms/ceres-error-terms/test/test_quaternion_parameterization_test.cc:13-31
+  (JPL local-parameterization Jacobian vs numdiff of Plus).
+Test fixtures of the reference: test camera (400, 300, 320, 240, 640x480)
+(camera-pinhole.h:251); radtan (-0.28, 0.08, -0.00026, -0.00024)
+(distortion-radtan.h:97); equidistant (0.2, 0.01, 0.3, 0.05)
+(distortion-equidistant.h:109).
+"""
+import numpy as np
+import pytest
+
+import independent as ind
+
+TEST_INTR = np.array([400.0, 300.0, 320.0, 240.0])
+RADTAN = np.array([-0.28, 0.08, -0.00026, -0.00024])
+EQUI = np.array([0.2, 0.01, 0.3, 0.05])
+NONE, RAD, EQD = 0, 1, 2
+
+
+def test_manual_projection_without_distortion(oracle):
+    # test-cameras.cc:387-425: exact to 1e-15.
+    st, kp, jp, _, _ = oracle.project(NONE, TEST_INTR, None, 640, 480, [1.0, 1.2, 10.0])
+    assert st == 0
+    np.testing.assert_allclose(kp, [400 * 0.1 + 320, 300 * 0.12 + 240], atol=1e-15, rtol=0)
+    # Jacobian of (fu x/z + cu, fv y/z + cv): closed form.
+    np.testing.assert_allclose(jp, [[40.0, 0, -4.0], [0, 30.0, -3.6]], rtol=1e-15, atol=1e-15)
+
+
+@pytest.mark.parametrize("p, status", [((1, 1, 10), 0), ((1, 1, -10), 2), ((1000, 1000, 10), 1), ((1, 1, 0.0), 3)])
+def test_projection_status(oracle, p, status):
+    # test-cameras.cc:491-520
+    st, *_ = oracle.project(NONE, TEST_INTR, None, 640, 480, p)
+    assert st == status
+
+
+@pytest.mark.parametrize("dist_type, dist", [(RAD, RADTAN), (EQD, EQUI)])
+def test_distortion_matches_textbook_model(oracle, dist_type, dist):
+    rng = np.random.default_rng(3)
+    f = ind.distort_radtan if dist_type == RAD else ind.distort_equidistant
+    for _ in range(50):
+        x, y = rng.uniform(-0.8, 0.8, 2)
+        pt, _ = oracle.distort(dist_type, dist, [x, y])
+        np.testing.assert_allclose(pt, f(dist, x, y), rtol=1e-14, atol=1e-15)
+
+
+@pytest.mark.parametrize("dist_type, dist", [(RAD, RADTAN), (EQD, EQUI)])
+def test_distortion_keypoint_jacobian_numdiff(oracle, dist_type, dist):
+    # test-distortions.cc:145-150 at (0.3, -0.2), tolerance 1e-4 there; the
+    # 5-point stencil allows a much tighter bound.
+    for point in ([0.3, -0.2], [0.05, 0.4], [-0.6, 0.1]):
+        _, jd = oracle.distort(dist_type, dist, point)
+        f = lambda d: oracle.distort(dist_type, dist, np.asarray(point) + d)[0]
+        np.testing.assert_allclose(jd, ind.numdiff(f, 2, 1e-5), rtol=1e-8, atol=1e-10)
+
+
+def test_equidistant_optical_centre_special_case(oracle):
+    # distortion-equidistant.cc:38-43: point unchanged, Jacobian zero.
+    pt, jd = oracle.distort(EQD, EQUI, [1e-12, -1e-12])
+    np.testing.assert_array_equal(pt, [1e-12, -1e-12])
+    np.testing.assert_array_equal(jd, np.zeros((2, 2)))
+
+
+@pytest.mark.parametrize("dist_type, dist", [(NONE, np.zeros(4)), (RAD, RADTAN), (EQD, EQUI)])
+def test_projection_jacobians_numdiff(oracle, dist_type, dist):
+    # test-cameras.cc:88-93 (point, 1e-5), :136-146 (intrinsics 1e-6),
+    # :190-199 (distortion 1e-4): random points at depth ~10.
+    rng = np.random.default_rng(7)
+    for _ in range(10):
+        p = np.array([rng.uniform(-3, 3), rng.uniform(-2, 2), 10.0])
+        st, kp, jp, ji, jd = oracle.project(dist_type, TEST_INTR, dist, 640, 480, p)
+        f_p = lambda d: oracle.project(dist_type, TEST_INTR, dist, 640, 480, p + d, False)[1]
+        f_i = lambda d: oracle.project(dist_type, TEST_INTR + d, dist, 640, 480, p, False)[1]
+        np.testing.assert_allclose(jp, ind.numdiff(f_p, 3, 1e-4), rtol=1e-7, atol=1e-8)
+        np.testing.assert_allclose(ji, ind.numdiff(f_i, 4, 1e-4), rtol=1e-7, atol=1e-8)
+        if dist_type != NONE:
+            f_d = lambda d: oracle.project(dist_type, TEST_INTR, dist + d, 640, 480, p, False)[1]
+            np.testing.assert_allclose(jd, ind.numdiff(f_d, 4, 1e-5), rtol=1e-6, atol=1e-7)
+
+
+def test_rotation_is_transposed_hamilton(oracle):
+    # quaternion-math-inl.h:7-27 is the transpose of the Hamilton matrix.
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        np.testing.assert_allclose(oracle.rotation_jpl(q), ind.R_jpl(q), atol=1e-15)
+
+
+def test_quaternion_parameterization_jacobian(oracle):
+    # test_quaternion_parameterization_test.cc:13-31: q = (1, 1, 0, 1) / |.|,
+    # Jacobian of Plus at delta = 0 equal to numdiff (1e-10 in the reference).
+    q = np.array([1.0, 1.0, 0.0, 1.0]) / np.sqrt(3.0)
+    J = oracle.quat_jacobian(q)
+    f = lambda d: oracle.quat_plus(q, d)
+    np.testing.assert_allclose(J, ind.numdiff(f, 3, 1e-5), atol=1e-10)
+
+
+def test_quaternion_plus_properties(oracle):
+    rng = np.random.default_rng(2)
+    for _ in range(200):
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        q *= np.sign(q[3])
+        d = rng.normal(size=3) * rng.choice([1e-3, 0.1, 0.6, 2.0])
+        out = oracle.quat_plus(q, d)
+        assert abs(np.linalg.norm(out) - 1) < 1e-15 and out[3] >= 0
+        np.testing.assert_allclose(out, ind.jpl_plus(q, d), atol=1e-14)
+    # Large-angle branch is the exact rotation by |d|: composing with -d undoes it.
+    q = np.array([0.0, 0.0, 0.0, 1.0])
+    d = np.array([0.3, -0.5, 0.4])
+    back = oracle.quat_plus(oracle.quat_plus(q, d), -d)
+    np.testing.assert_allclose(back, q, atol=1e-14)
+
+
+def test_huber_and_cauchy_closed_forms(oracle):
+    a = 1.5
+    for s in (0.0, 0.5, 2.25, 3.0, 100.0):
+        rho = oracle.loss(0, a, s)
+        if s <= a * a:
+            np.testing.assert_allclose(rho, [s, 1.0, 0.0])
+        else:
+            np.testing.assert_allclose(rho, [2 * a * np.sqrt(s) - a * a, a / np.sqrt(s), -a / np.sqrt(s) / (2 * s)])
+        rc = oracle.loss(1, a, s)
+        b = a * a
+        np.testing.assert_allclose(rc, [b * np.log1p(s / b), 1 / (1 + s / b), -1 / b / (1 + s / b) ** 2], rtol=1e-14)
+        # rho' and rho'' are the derivatives of rho
+        for lt in (0, 1):
+            if lt == 0 and abs(s - b) < 1e-3:
+                continue
+            h = 1e-6 * max(1.0, s)
+            d1 = (oracle.loss(lt, a, s + h)[0] - oracle.loss(lt, a, max(s - h, 0))[0]) / (s + h - max(s - h, 0))
+            assert abs(d1 - oracle.loss(lt, a, s)[1]) < 1e-6
