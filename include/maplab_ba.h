@@ -184,7 +184,20 @@ typedef struct mlba_summary {
   int32_t iterations_capacity;            /* caller-set capacity of `iterations` */
   int32_t reserved0;
   mlba_iteration_summary* iterations;     /* caller-owned, nullable */
+  /* Device time inside the LM loop, from HIP events on the library's stream:
+   * summed milliseconds and launch count per MLBA_KERNEL_* id. */
+  double kernel_ms[8];
+  int32_t kernel_launches[8];
 } mlba_summary;
+
+/* Kernel ids of mlba_summary.kernel_ms. */
+#define MLBA_KERNEL_LINEARIZE 0       /* residual + Jacobian + landmark reductions */
+#define MLBA_KERNEL_COST 1            /* residual-only cost of the candidate */
+#define MLBA_KERNEL_SCHUR_ROWS 2      /* reduced camera system build */
+#define MLBA_KERNEL_PCG 3             /* whole PCG solve (all its launches) */
+#define MLBA_KERNEL_BACK_SUBSTITUTE 4 /* landmark step + model cost change */
+#define MLBA_KERNEL_POSE_ROWS 5       /* vertex gradient and diag(J'J) */
+#define MLBA_KERNEL_LANDMARK_PREP 6   /* damped landmark pivots */
 
 typedef struct mlba_handle mlba_handle;
 
@@ -236,6 +249,34 @@ int32_t mlba_solve(mlba_handle* handle, const mlba_options* options, mlba_summar
 /* map_optimization::solveWithOutlierRejection (outlier-rejection-solver.cc:206-259). */
 int32_t mlba_solve_with_outlier_rejection(mlba_handle* handle, const mlba_options* options,
                                           mlba_summary* summary);
+
+/*
+ * Measurement hook (no counterpart in the reference): average device time of
+ * each kernel of one LM iteration at the current state, timed with HIP
+ * events on the library's stream over `repeats` launches, plus the sizes
+ * that the algorithmic byte and flop counts of those kernels need.
+ */
+typedef struct mlba_kernel_times {
+  double linearize_ms;        /* residual + Jacobian + landmark reductions */
+  double cost_ms;             /* residual-only cost (candidate evaluation) */
+  double pose_rows_ms;        /* vertex gradient and diag(J'J) */
+  double landmark_prep_ms;    /* damped landmark pivots */
+  double schur_rows_ms;       /* reduced camera system build */
+  double block_inverse_ms;    /* block-Jacobi preconditioner */
+  double pcg_spmv_ms;         /* one PCG matrix-vector product */
+  double pcg_vector_ms;       /* PCG vector updates of one iteration */
+  double back_substitute_ms;  /* landmark step + model cost change */
+  int64_t num_observations;
+  int64_t num_landmarks;
+  int64_t num_variable_vertices;
+  int64_t schur_blocks;       /* 6x6 blocks of S */
+  int64_t schur_pairs;        /* incidence pairs the S build visits */
+  int64_t incidences;         /* vertex-observation incidences */
+  int64_t linearize_tiles;
+} mlba_kernel_times;
+
+int32_t mlba_time_kernels(mlba_handle* handle, const mlba_options* options, int32_t repeats,
+                          mlba_kernel_times* times);
 
 #ifdef __cplusplus
 }

@@ -86,7 +86,12 @@ class Summary(ctypes.Structure):
         ("linear_solver_seconds", c_f64), ("total_linear_solver_iterations", ctypes.c_int64),
         ("iterations_capacity", c_i32), ("reserved0", c_i32),
         ("iterations", ctypes.POINTER(IterationSummary)),
+        ("kernel_ms", c_f64 * 8), ("kernel_launches", c_i32 * 8),
     ]
+
+
+KERNEL_LINEARIZE, KERNEL_COST, KERNEL_SCHUR_ROWS, KERNEL_PCG, KERNEL_BACK_SUBSTITUTE = 0, 1, 2, 3, 4
+KERNEL_POSE_ROWS, KERNEL_LANDMARK_PREP = 5, 6
 
 
 class SynthConfig(ctypes.Structure):
@@ -101,6 +106,14 @@ class SynthConfig(ctypes.Structure):
         ("detection_probability", c_f64), ("anchor_mode", c_i32), ("fix_scale_landmark", c_i32),
         ("seed", c_u64),
     ]
+
+
+class KernelTimes(ctypes.Structure):
+    _fields_ = [(n, c_f64) for n in (
+        "linearize_ms", "cost_ms", "pose_rows_ms", "landmark_prep_ms", "schur_rows_ms", "block_inverse_ms",
+        "pcg_spmv_ms", "pcg_vector_ms", "back_substitute_ms")] + [(n, ctypes.c_int64) for n in (
+        "num_observations", "num_landmarks", "num_variable_vertices", "schur_blocks", "schur_pairs", "incidences",
+        "linearize_tiles")]
 
 
 def _declare(lib, name, restype, argtypes):
@@ -124,6 +137,7 @@ LIB_SYMBOLS = {
     "mlba_evaluate": (c_i32, [c_vp, c_i32, ctypes.POINTER(Options), c_vp, c_vp, c_vp]),
     "mlba_solve": (c_i32, [c_vp, ctypes.POINTER(Options), ctypes.POINTER(Summary)]),
     "mlba_solve_with_outlier_rejection": (c_i32, [c_vp, ctypes.POINTER(Options), ctypes.POINTER(Summary)]),
+    "mlba_time_kernels": (c_i32, [c_vp, ctypes.POINTER(Options), c_i32, ctypes.POINTER(KernelTimes)]),
 }
 
 SYNTH_SYMBOLS = {

@@ -47,6 +47,12 @@ int launch_pcg(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_
                double* q, double* parts_a, double* parts_b, void* state_dev, void* state_host, double tol,
                int32_t max_iters, int32_t poll);
 size_t pcg_state_bytes();
+void launch_pcg_setup(hipStream_t s, int32_t Pvar, const double* b, const double* Minv, double* x, double* r, double* z,
+                      double* p, double* parts_a, void* state_dev);
+void launch_pcg_spmv(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_t* col_var, const double* S,
+                     const double* p, double* q, double* parts_a, const void* state_dev);
+void launch_pcg_vectors(hipStream_t s, int32_t Pvar, const double* parts_a, const double* Minv, double* p,
+                        const double* q, double* x, double* r, double* z, double* parts_b, void* state_dev);
 void launch_back_substitute(hipStream_t s, const DevTopology& t, const double* vR, const double* lm,
                             const double* obs_rec, const double* lm_acc, const double* lm_z, const double* w,
                             double* u, double* part);
@@ -114,6 +120,7 @@ struct mlba_handle {
   std::vector<double> h_baseframes, h_cam_q;
   // Derived per topology build.
   int32_t Pvar = 0, tiles = 0, nnzb = 0, n_inc = 0, n_split = 0, max_row_blocks = 0;
+  int64_t schur_pairs = 0;
   // Device memory.
   DBuf<double> obs_kp, obs_sinv, obs_rec;
   DBuf<int32_t> obs_vertex, obs_meta, obs_landmark, perm_d;
@@ -135,8 +142,14 @@ struct mlba_handle {
   void* pcg_host = nullptr;
   double* scal_host = nullptr;
   DevTopology topo{};
+  // Event pairs bracketing kernels inside the LM loop (kernel id, start, end).
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+  size_t ev_next = 0;
+  mlba_summary* ev_summary = nullptr;
 
   ~mlba_handle() {
+    for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     if (pcg_host) (void)hipHostFree(pcg_host);
     if (scal_host) (void)hipHostFree(scal_host);
     if (stream) (void)hipStreamDestroy(stream);
@@ -253,6 +266,7 @@ int build_topology(mlba_handle* h) {
   {
     std::vector<int32_t> mark(Pvar, -1), cols;
     h->max_row_blocks = 0;
+    h->schur_pairs = 0;
     for (int i = 0; i < Pvar; ++i) {
       cols.clear();
       auto touch = [&](int j) {
@@ -263,6 +277,7 @@ int build_topology(mlba_handle* h) {
       for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) {
         const int x = inc[k];
         const int l = x >= 0 ? h->s_landmark[x] : -1 - x;
+        h->schur_pairs += h->lm_begin[l + 1] - h->lm_begin[l] + 1;
         if (l == last_l) continue;
         last_l = l;
         for (int o = h->lm_begin[l]; o < h->lm_begin[l + 1]; ++o)
@@ -314,6 +329,48 @@ int build_topology(mlba_handle* h) {
   return MLBA_OK;
 }
 
+// Kernel timing inside the LM loop: begin/end record an event pair on the
+// library stream; collect_events (after a stream synchronization) adds the
+// elapsed times to the summary.
+hipEvent_t next_event(mlba_handle* h) {
+  if (h->ev_next == h->ev_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    h->ev_pool.push_back(e);
+  }
+  return h->ev_pool[h->ev_next++];
+}
+struct KernelTimer {
+  mlba_handle* h;
+  int id;
+  hipEvent_t a = nullptr;
+  KernelTimer(mlba_handle* hh, int kid) : h(hh), id(kid) {
+    if (!h->ev_summary) return;
+    a = next_event(h);
+    if (a) (void)hipEventRecord(a, h->stream);
+  }
+  ~KernelTimer() {
+    if (!a) return;
+    hipEvent_t b = next_event(h);
+    if (!b) return;
+    (void)hipEventRecord(b, h->stream);
+    h->ev_pending.push_back({id, {a, b}});
+  }
+};
+void collect_events(mlba_handle* h) {
+  if (h->ev_summary) {
+    for (auto& p : h->ev_pending) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
+        h->ev_summary->kernel_ms[p.first] += ms;
+        h->ev_summary->kernel_launches[p.first] += 1;
+      }
+    }
+  }
+  h->ev_pending.clear();
+  h->ev_next = 0;
+}
+
 DevLoss make_loss(const mlba_options* o) {
   DevLoss l;
   l.type = o ? o->loss_type : MLBA_LOSS_MAPLAB_HUBER;
@@ -328,9 +385,16 @@ int linearize(mlba_handle* h, const mlba_options* o) {
   hipStream_t s = h->stream;
   launch_vertex_prep(s, h->P, h->vert.p, h->vR.p);
   launch_zero_split(s, h->n_split, h->split_ids.p, h->lm_acc.p);
-  launch_linearize(s, true, h->tiles, h->topo, h->vert.p, h->vR.p, h->lm.p, make_loss(o), h->tile_begin.p,
-                   h->obs_rec.p, h->lm_acc.p, h->tile_cost.p);
-  launch_pose_rows(s, h->topo, h->vert.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p, h->pose_grad.p, h->pose_diag.p);
+  {
+    KernelTimer kt(h, MLBA_KERNEL_LINEARIZE);
+    launch_linearize(s, true, h->tiles, h->topo, h->vert.p, h->vR.p, h->lm.p, make_loss(o), h->tile_begin.p,
+                     h->obs_rec.p, h->lm_acc.p, h->tile_cost.p);
+  }
+  {
+    KernelTimer kt(h, MLBA_KERNEL_POSE_ROWS);
+    launch_pose_rows(s, h->topo, h->vert.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p, h->pose_grad.p,
+                     h->pose_diag.p);
+  }
   launch_reduce(s, h->tiles, 2, 0, false, h->tile_cost.p, h->scal.p + kCost);
   launch_reduce(s, h->tiles, 2, 1, false, h->tile_cost.p, h->scal.p + kFixed);
   const int gb = (h->Pvar + h->L + 255) / 256;
@@ -343,6 +407,7 @@ int linearize(mlba_handle* h, const mlba_options* o) {
 int read_scalars(mlba_handle* h) {
   HIP_OK(hipMemcpyAsync(h->scal_host, h->scal.p, kNumSlots * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIP_OK(hipStreamSynchronize(h->stream));
+  collect_events(h);
   return MLBA_OK;
 }
 
@@ -396,23 +461,33 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
       // Linear solve: Schur elimination + PCG on the reduced camera system.
       const double t_ls = now_s();
       HIP_OK(hipMemsetAsync(h->fail.p, 0, sizeof(int32_t), s));
-      launch_landmark_prep(s, h->topo, h->lm_acc.p, h->lm_scale.p, inv_mu, o->min_lm_diagonal, o->max_lm_diagonal,
-                           h->lm_z.p, h->fail.p);
-      launch_schur_rows(s, h->topo, h->vert.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p, h->lm_z.p,
-                        h->pose_scale.p, h->pose_diag.p, inv_mu, o->min_lm_diagonal, o->max_lm_diagonal, h->S.p,
-                        h->rhs.p);
+      {
+        KernelTimer kt(h, MLBA_KERNEL_LANDMARK_PREP);
+        launch_landmark_prep(s, h->topo, h->lm_acc.p, h->lm_scale.p, inv_mu, o->min_lm_diagonal,
+                             o->max_lm_diagonal, h->lm_z.p, h->fail.p);
+      }
+      {
+        KernelTimer kt(h, MLBA_KERNEL_SCHUR_ROWS);
+        launch_schur_rows(s, h->topo, h->vert.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p, h->lm_z.p,
+                          h->pose_scale.p, h->pose_diag.p, inv_mu, o->min_lm_diagonal, o->max_lm_diagonal, h->S.p,
+                          h->rhs.p);
+      }
       launch_block_inverse(s, h->Pvar, h->diag_pos.p, h->S.p, h->Minv.p, h->fail.p);
       HIP_OK(hipGetLastError());
       int pcg_iters = 0;
       if (h->Pvar > 0) {
+        KernelTimer kt(h, MLBA_KERNEL_PCG);
         pcg_iters = launch_pcg(s, h->Pvar, h->row_ptr.p, h->col_var.p, h->diag_pos.p, h->S.p, h->Minv.p, h->rhs.p,
                                h->px.p, h->pr.p, h->pz.p, h->pp.p, h->pq.p, h->parts_a.p, h->parts_b.p,
                                h->pcg_state.p, h->pcg_host, o->pcg_relative_tolerance, o->pcg_max_iterations, 8);
         if (pcg_iters < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
       }
       launch_unscale_poses(s, h->Pvar, h->px.p, h->pose_scale.p, h->w.p, h->parts_w.p);
-      launch_back_substitute(s, h->topo, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p, h->lm_z.p, h->w.p, h->lm_u.p,
-                             h->parts_l.p);
+      {
+        KernelTimer kt(h, MLBA_KERNEL_BACK_SUBSTITUTE);
+        launch_back_substitute(s, h->topo, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p, h->lm_z.p, h->w.p,
+                               h->lm_u.p, h->parts_l.p);
+      }
       launch_reduce(s, lb, 2, 0, false, h->parts_l.p, h->scal.p + kMcc);
       launch_reduce(s, lb, 2, 1, false, h->parts_l.p, h->scal.p + kUU);
       launch_reduce(s, wb, 1, 0, false, h->parts_w.p, h->scal.p + kWW);
@@ -422,8 +497,11 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
       launch_reduce(s, pb, 1, 0, false, h->parts_v.p, h->scal.p + kXV);
       launch_reduce(s, lb, 1, 0, false, h->parts_c.p, h->scal.p + kXL);
       launch_vertex_prep(s, h->P, h->vert_cand.p, h->vR_cand.p);
-      launch_linearize(s, false, h->tiles, h->topo, h->vert_cand.p, h->vR_cand.p, h->lm_cand.p, make_loss(o),
-                       h->tile_begin.p, nullptr, nullptr, h->tile_cost2.p);
+      {
+        KernelTimer kt(h, MLBA_KERNEL_COST);
+        launch_linearize(s, false, h->tiles, h->topo, h->vert_cand.p, h->vR_cand.p, h->lm_cand.p, make_loss(o),
+                         h->tile_begin.p, nullptr, nullptr, h->tile_cost2.p);
+      }
       launch_reduce(s, h->tiles, 2, 0, false, h->tile_cost2.p, h->scal.p + kCandCost);
       HIP_OK(hipGetLastError());
       int32_t fail_h = 0;
@@ -514,6 +592,7 @@ done:
 
 void reset_summary(mlba_summary* sm) {
   if (!sm) return;
+  for (int k = 0; k < 8; ++k) { sm->kernel_ms[k] = 0.0; sm->kernel_launches[k] = 0; }
   sm->num_iterations = 0;
   sm->num_successful_steps = sm->num_unsuccessful_steps = 0;
   sm->num_removed_landmarks = 0;
@@ -806,7 +885,10 @@ int32_t mlba_solve(mlba_handle* h, const mlba_options* o, mlba_summary* sm) {
   if (sm) sm->num_residual_blocks = count_active_obs(h);
   double radius = 0.0;
   int term = MLBA_NO_CONVERGENCE;
+  h->ev_summary = sm;
   rc = minimize(h, o, o->initial_trust_region_radius, sm, &radius, &term);
+  collect_events(h);
+  h->ev_summary = nullptr;
   if (rc) return rc;
   if (sm) sm->termination_type = term;
   return MLBA_OK;
@@ -896,7 +978,11 @@ int32_t mlba_solve_with_outlier_rejection(mlba_handle* h, const mlba_options* o,
     const int step_iters = std::min(remaining, o->reject_outliers_every_n_iters);
     mlba_options local = *o;
     local.max_num_iterations = step_iters;
-    if ((rc = minimize(h, &local, radius, sm, &radius, &term))) return rc;
+    h->ev_summary = sm;
+    rc = minimize(h, &local, radius, sm, &radius, &term);
+    collect_events(h);
+    h->ev_summary = nullptr;
+    if (rc) return rc;
     if (first && sm) initial_cost = sm->initial_cost;
     first = false;
     int removed = 0;
@@ -911,6 +997,84 @@ int32_t mlba_solve_with_outlier_rejection(mlba_handle* h, const mlba_options* o,
     sm->initial_cost = initial_cost;
     sm->num_residual_blocks = count_active_obs(h);
   }
+  return MLBA_OK;
+}
+
+int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats, mlba_kernel_times* t) {
+  if (!h || !t || repeats <= 0) return MLBA_EINVAL;
+  int rc = validate_options(h, o);
+  if (rc) return rc;
+  hipStream_t s = h->stream;
+  std::memset(t, 0, sizeof(*t));
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  auto timed = [&](auto&& launch) -> double {
+    launch();  // warm
+    (void)hipEventRecord(e0, s);
+    for (int k = 0; k < repeats; ++k) launch();
+    (void)hipEventRecord(e1, s);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    return (double)ms / repeats;
+  };
+  const DevLoss loss = make_loss(o);
+  const double inv_mu = 1.0 / o->initial_trust_region_radius;
+  if ((rc = linearize(h, o))) return rc;
+  launch_jacobi_scaling(s, h->Pvar, h->L, o->use_jacobi_scaling, h->pose_diag.p, h->lm_acc.p, h->pose_scale.p,
+                        h->lm_scale.p);
+  launch_vertex_prep(s, h->P, h->vert.p, h->vR.p);
+  t->linearize_ms = timed([&] {
+    launch_zero_split(s, h->n_split, h->split_ids.p, h->lm_acc.p);
+    launch_linearize(s, true, h->tiles, h->topo, h->vert.p, h->vR.p, h->lm.p, loss, h->tile_begin.p, h->obs_rec.p,
+                     h->lm_acc.p, h->tile_cost.p);
+  });
+  t->cost_ms = timed([&] {
+    launch_linearize(s, false, h->tiles, h->topo, h->vert.p, h->vR.p, h->lm.p, loss, h->tile_begin.p, nullptr,
+                     nullptr, h->tile_cost2.p);
+  });
+  t->pose_rows_ms = timed([&] {
+    launch_pose_rows(s, h->topo, h->vert.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p, h->pose_grad.p,
+                     h->pose_diag.p);
+  });
+  HIP_OK(hipMemsetAsync(h->fail.p, 0, sizeof(int32_t), s));
+  t->landmark_prep_ms = timed([&] {
+    launch_landmark_prep(s, h->topo, h->lm_acc.p, h->lm_scale.p, inv_mu, o->min_lm_diagonal, o->max_lm_diagonal,
+                         h->lm_z.p, h->fail.p);
+  });
+  t->schur_rows_ms = timed([&] {
+    launch_schur_rows(s, h->topo, h->vert.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p, h->lm_z.p, h->pose_scale.p,
+                      h->pose_diag.p, inv_mu, o->min_lm_diagonal, o->max_lm_diagonal, h->S.p, h->rhs.p);
+  });
+  t->block_inverse_ms = timed([&] { launch_block_inverse(s, h->Pvar, h->diag_pos.p, h->S.p, h->Minv.p, h->fail.p); });
+  if (h->Pvar > 0) {
+    launch_pcg_setup(s, h->Pvar, h->rhs.p, h->Minv.p, h->px.p, h->pr.p, h->pz.p, h->pp.p, h->parts_a.p,
+                     h->pcg_state.p);
+    t->pcg_spmv_ms = timed([&] {
+      launch_pcg_spmv(s, h->Pvar, h->row_ptr.p, h->col_var.p, h->S.p, h->pp.p, h->pq.p, h->parts_a.p, h->pcg_state.p);
+    });
+    t->pcg_vector_ms = timed([&] {
+      launch_pcg_vectors(s, h->Pvar, h->parts_a.p, h->Minv.p, h->pp.p, h->pq.p, h->px.p, h->pr.p, h->pz.p,
+                         h->parts_b.p, h->pcg_state.p);
+    });
+  }
+  launch_unscale_poses(s, h->Pvar, h->px.p, h->pose_scale.p, h->w.p, h->parts_w.p);
+  t->back_substitute_ms = timed([&] {
+    launch_back_substitute(s, h->topo, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p, h->lm_z.p, h->w.p, h->lm_u.p,
+                           h->parts_l.p);
+  });
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(s));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  t->num_observations = h->N;
+  t->num_landmarks = h->L;
+  t->num_variable_vertices = h->Pvar;
+  t->schur_blocks = h->nnzb;
+  t->schur_pairs = h->schur_pairs;
+  t->incidences = h->n_inc;
+  t->linearize_tiles = h->tiles;
   return MLBA_OK;
 }
 

@@ -800,13 +800,12 @@ __global__ __launch_bounds__(256) void k_pcg_direction(int32_t Pvar, int32_t npa
                                                        const double* __restrict__ z, double* __restrict__ p,
                                                        PcgState* __restrict__ st) {
   __shared__ double lds4[4];
-  __shared__ double b_rz, b_rr;
+  __shared__ double b_rz;
   if (st->done) return;
   double a = 0.0, b = 0.0;
   for (int k = threadIdx.x; k < nparts; k += blockDim.x) { a += part[2 * k]; b += part[2 * k + 1]; }
   a = block_sum256(a, lds4);
-  b = block_sum256(b, lds4);
-  if (threadIdx.x == 0) { b_rz = a; b_rr = b; }
+  if (threadIdx.x == 0) b_rz = a;
   __syncthreads();
   const double rz_new = b_rz;
   const double beta = rz_new / st->rz;
@@ -1321,6 +1320,27 @@ int launch_pcg(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_
   return hs->iters;
 }
 size_t pcg_state_bytes() { return sizeof(PcgState); }
+
+// Pieces of one PCG iteration, for kernel timing: initialize the state, then
+// the matrix-vector product alone or the vector updates alone.
+void launch_pcg_setup(hipStream_t s, int32_t Pvar, const double* b, const double* Minv, double* x, double* r, double* z,
+                      double* p, double* parts_a, void* state_dev) {
+  const int vb = (Pvar + 255) / 256;
+  k_pcg_init<<<vb, 256, 0, s>>>(Pvar, b, Minv, x, r, z, p, parts_a);
+  k_pcg_init_scalars<<<1, 256, 0, s>>>(vb, parts_a, 0.0, 1 << 30, static_cast<PcgState*>(state_dev));
+}
+void launch_pcg_spmv(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_t* col_var, const double* S,
+                     const double* p, double* q, double* parts_a, const void* state_dev) {
+  k_pcg_spmv<<<Pvar, 256, 0, s>>>(Pvar, row_ptr, col_var, S, p, q, parts_a, static_cast<const PcgState*>(state_dev));
+}
+void launch_pcg_vectors(hipStream_t s, int32_t Pvar, const double* parts_a, const double* Minv, double* p,
+                        const double* q, double* x, double* r, double* z, double* parts_b, void* state_dev) {
+  const int vb = (Pvar + 255) / 256;
+  PcgState* st = static_cast<PcgState*>(state_dev);
+  k_pcg_update<<<vb, 256, 0, s>>>(Pvar, Pvar, parts_a, Minv, p, q, x, r, z, parts_b, st);
+  k_pcg_direction<<<vb, 256, 0, s>>>(Pvar, vb, parts_b, z, p, st);
+  k_pcg_scalars<<<1, 256, 0, s>>>(vb, parts_b, st);
+}
 
 void launch_back_substitute(hipStream_t s, const DevTopology& t, const double* vR, const double* lm,
                             const double* obs_rec, const double* lm_acc, const double* lm_z, const double* w,

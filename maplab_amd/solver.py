@@ -91,3 +91,10 @@ class DeviceProblem:
         self._check(fn(self.handle, ctypes.byref(options), ctypes.byref(sm)))
         rows = [its[i] for i in range(min(sm.num_iterations, capacity))]
         return sm, rows
+
+    def time_kernels(self, options: abi.Options, repeats: int = 10) -> abi.KernelTimes:
+        """Average device time of each kernel of one LM iteration (HIP events
+        on the library's stream)."""
+        t = abi.KernelTimes()
+        self._check(self.lib.mlba_time_kernels(self.handle, ctypes.byref(options), repeats, ctypes.byref(t)))
+        return t

@@ -40,7 +40,8 @@ def test_struct_layouts_match_header():
     # Field offsets are what the C compiler produces for the declarations.
     assert ctypes.sizeof(abi.ProblemDesc) == 24 + 17 * 8
     assert ctypes.sizeof(abi.IterationSummary) == 16 + 8 * 8
-    assert abi.Summary.iterations.offset == ctypes.sizeof(abi.Summary) - 8
+    assert abi.Summary.kernel_launches.offset == ctypes.sizeof(abi.Summary) - 32
+    assert abi.Summary.kernel_ms.offset == abi.Summary.iterations.offset + 8
 
 
 def test_option_defaults_are_maplab_flags():
