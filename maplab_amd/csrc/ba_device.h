@@ -44,6 +44,7 @@ struct DevTopology {
   const int32_t* obs_vertex;   // N
   const int32_t* obs_meta;     // N
   const int32_t* obs_landmark; // N
+  const int32_t* obs_col;      // N: variable index of the observer when the observation couples it, else -1
   const int32_t* lm_begin;     // L+1
   const int32_t* lm_anchor;    // L
   const uint32_t* lm_flags;    // L

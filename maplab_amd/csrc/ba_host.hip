@@ -36,10 +36,12 @@ void launch_jacobi_scaling(hipStream_t s, int32_t Pvar, int32_t L, int32_t use, 
                            const double* lm_acc, double* pose_scale, double* lm_scale);
 void launch_landmark_prep(hipStream_t s, const DevTopology& t, const double* lm_acc, const double* lm_scale,
                           double inv_mu, double min_diag, double max_diag, double* lm_z, int32_t* fail);
-void launch_schur_rows(hipStream_t s, const DevTopology& t, const double* vert, const double* vR, const double* lm,
-                       const double* obs_rec, const double* lm_acc, const double* lm_z, const double* pose_scale,
-                       const double* pose_diag, double inv_mu, double min_diag, double max_diag, double* S,
-                       double* rhs);
+void launch_schur_build(hipStream_t s, const DevTopology& t, int32_t ntasks, const int4* tasks, int32_t nmirror,
+                        const int2* mirror, int64_t nnzb, const double* vR, const double* lm, const double* obs_rec,
+                        const double* lm_acc, const double* lm_z, const double* pose_scale, const double* pose_diag,
+                        double inv_mu, double min_diag, double max_diag, double* S, double* rhs);
+int32_t schur_chunk_blocks();
+int32_t schur_batch_incidences();
 void launch_block_inverse(hipStream_t s, int32_t Pvar, const int32_t* diag_pos, const double* S, double* Minv,
                           int32_t* fail);
 int launch_pcg(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_t* col_var, const int32_t* diag_pos,
@@ -47,6 +49,10 @@ int launch_pcg(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_
                double* q, double* parts_a, double* parts_b, void* state_dev, void* state_host, double tol,
                int32_t max_iters, int32_t poll);
 size_t pcg_state_bytes();
+int launch_pcg_persistent(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_t* col_var, const double* S,
+                          const double* Minv, const double* b, double* x, double* r, double* z, double* p0,
+                          double* p1, double* q, double* part, void* state_dev, void* state_host, double tol,
+                          int32_t max_iters, int32_t max_groups);
 void launch_pcg_setup(hipStream_t s, int32_t Pvar, const double* b, const double* Minv, double* x, double* r, double* z,
                       double* p, double* parts_a, void* state_dev);
 void launch_pcg_spmv(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_t* col_var, const double* S,
@@ -121,6 +127,8 @@ struct mlba_handle {
   // Derived per topology build.
   int32_t Pvar = 0, tiles = 0, nnzb = 0, n_inc = 0, n_split = 0, max_row_blocks = 0;
   int64_t schur_pairs = 0;
+  int32_t n_tasks = 0, n_mirror = 0;
+  bool pcg_cooperative = true;  // cleared when the device cannot co-schedule the PCG grid
   // Device memory.
   DBuf<double> obs_kp, obs_sinv, obs_rec;
   DBuf<int32_t> obs_vertex, obs_meta, obs_landmark, perm_d;
@@ -133,9 +141,11 @@ struct mlba_handle {
   DBuf<double> mission_T, baseframes, cam_q;
   DBuf<int32_t> tile_begin;
   DBuf<double> tile_cost, tile_cost2;
-  DBuf<int32_t> row_ptr, col_var, diag_pos, inc_ptr, inc;
+  DBuf<int32_t> row_ptr, col_var, diag_pos, inc_ptr, inc, obs_col;
+  DBuf<int4> schur_tasks;
+  DBuf<int2> mirror;
   DBuf<double> S, rhs, pose_grad, pose_diag, pose_scale, w, Minv;
-  DBuf<double> px, pr, pz, pp, pq, parts_a, parts_b, parts_c, parts_l, parts_v, parts_w, parts_g;
+  DBuf<double> px, pr, pz, pp, pp2, pq, pcg_part, parts_a, parts_b, parts_c, parts_l, parts_v, parts_w, parts_g;
   DBuf<double> scal;
   DBuf<int32_t> fail;
   DBuf<char> pcg_state;
@@ -293,6 +303,30 @@ int build_topology(mlba_handle* h) {
     }
   }
   h->nnzb = (int32_t)col_var.size();
+  // S build tasks: (row, incidence batch, chunk of upper column blocks), and
+  // the (upper, lower) block position pairs of the mirror pass.
+  std::vector<int4> tasks;
+  std::vector<int2> mirror;
+  {
+    const int chunk = schur_chunk_blocks(), batch = schur_batch_incidences();
+    for (int i = 0; i < Pvar; ++i) {
+      for (int c0 = diag_pos[i]; c0 < row_ptr[i + 1]; c0 += chunk)
+        for (int x0 = inc_ptr[i]; x0 < inc_ptr[i + 1]; x0 += batch)
+          tasks.push_back(make_int4(i, x0, std::min(x0 + batch, inc_ptr[i + 1]), c0));
+      for (int k = diag_pos[i] + 1; k < row_ptr[i + 1]; ++k) {
+        const int j = col_var[k];
+        const int32_t* b = col_var.data() + row_ptr[j];
+        const int32_t* e = col_var.data() + row_ptr[j + 1];
+        const int32_t* it = std::lower_bound(b, e, i);
+        mirror.push_back(make_int2(k, (int32_t)(it - col_var.data())));
+      }
+    }
+  }
+  h->n_tasks = (int32_t)tasks.size();
+  h->n_mirror = (int32_t)mirror.size();
+  std::vector<int32_t> obs_col(N, -1);
+  for (int o = 0; o < N; ++o)
+    if ((meta[o] & 4) && h->lm_active[h->s_landmark[o]]) obs_col[o] = vertex_var[h->s_vertex[o]];
   int rc;
   if ((rc = upload(h, h->lm_flags, flags.data(), flags.size()))) return rc;
   if ((rc = upload(h, h->obs_meta, meta.data(), meta.size()))) return rc;
@@ -305,12 +339,16 @@ int build_topology(mlba_handle* h) {
   if ((rc = upload(h, h->row_ptr, row_ptr.data(), row_ptr.size()))) return rc;
   if ((rc = upload(h, h->col_var, col_var.data(), col_var.size()))) return rc;
   if ((rc = upload(h, h->diag_pos, diag_pos.data(), diag_pos.size()))) return rc;
+  if ((rc = upload(h, h->schur_tasks, tasks.data(), tasks.size()))) return rc;
+  if ((rc = upload(h, h->mirror, mirror.data(), mirror.size()))) return rc;
+  if ((rc = upload(h, h->obs_col, obs_col.data(), obs_col.size()))) return rc;
   const size_t T = h->tiles, V6 = 6 * (size_t)std::max(Pvar, 1);
   const size_t vb = (Pvar + 255) / 256 + 1;
   bool ok = h->tile_cost.alloc(2 * T) && h->tile_cost2.alloc(2 * T) && h->S.alloc(36 * (size_t)h->nnzb) &&
             h->rhs.alloc(V6) && h->pose_grad.alloc(V6) && h->pose_diag.alloc(V6) && h->pose_scale.alloc(V6) &&
             h->w.alloc(V6) && h->Minv.alloc(36 * (size_t)std::max(Pvar, 1)) && h->px.alloc(V6) &&
-            h->pr.alloc(V6) && h->pz.alloc(V6) && h->pp.alloc(V6) && h->pq.alloc(V6) &&
+            h->pr.alloc(V6) && h->pz.alloc(V6) && h->pp.alloc(V6) && h->pp2.alloc(V6) && h->pq.alloc(V6) &&
+            h->pcg_part.alloc(3 * (size_t)std::max(Pvar, 1) + 3) &&
             h->parts_a.alloc(std::max<size_t>(Pvar, 2 * vb) + 1) && h->parts_b.alloc(2 * vb + 2) &&
             h->parts_l.alloc(2 * ((size_t)(h->L + 255) / 256 + 1)) &&
             h->parts_c.alloc((size_t)(h->L + 255) / 256 + 1) &&
@@ -320,7 +358,7 @@ int build_topology(mlba_handle* h) {
   DevTopology& t = h->topo;
   t.N = h->N; t.L = h->L; t.P = h->P; t.Pvar = Pvar; t.M = h->M; t.C = h->C;
   t.obs_kp = h->obs_kp.p; t.obs_sinv = h->obs_sinv.p; t.obs_vertex = h->obs_vertex.p; t.obs_meta = h->obs_meta.p;
-  t.obs_landmark = h->obs_landmark.p; t.lm_begin = h->lm_begin_d.p; t.lm_anchor = h->lm_anchor_d.p;
+  t.obs_landmark = h->obs_landmark.p; t.obs_col = h->obs_col.p; t.lm_begin = h->lm_begin_d.p; t.lm_anchor = h->lm_anchor_d.p;
   t.lm_flags = h->lm_flags.p; t.cams = h->cams.p; t.mission_T = h->mission_T.p;
   t.vertex_mission = h->vertex_mission_d.p; t.vertex_var = h->vertex_var.p; t.var_vertex = h->var_vertex.p;
   t.row_ptr = h->row_ptr.p; t.col_var = h->col_var.p; t.diag_pos = h->diag_pos.p; t.inc_ptr = h->inc_ptr.p;
@@ -468,16 +506,27 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
       }
       {
         KernelTimer kt(h, MLBA_KERNEL_SCHUR_ROWS);
-        launch_schur_rows(s, h->topo, h->vert.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p, h->lm_z.p,
-                          h->pose_scale.p, h->pose_diag.p, inv_mu, o->min_lm_diagonal, o->max_lm_diagonal, h->S.p,
-                          h->rhs.p);
+        launch_schur_build(s, h->topo, h->n_tasks, h->schur_tasks.p, h->n_mirror, h->mirror.p, h->nnzb, h->vR.p,
+                           h->lm.p, h->obs_rec.p, h->lm_acc.p, h->lm_z.p, h->pose_scale.p, h->pose_diag.p, inv_mu,
+                           o->min_lm_diagonal, o->max_lm_diagonal, h->S.p, h->rhs.p);
       }
       launch_block_inverse(s, h->Pvar, h->diag_pos.p, h->S.p, h->Minv.p, h->fail.p);
       HIP_OK(hipGetLastError());
       int pcg_iters = 0;
       if (h->Pvar > 0) {
         KernelTimer kt(h, MLBA_KERNEL_PCG);
-        pcg_iters = launch_pcg(s, h->Pvar, h->row_ptr.p, h->col_var.p, h->diag_pos.p, h->S.p, h->Minv.p, h->rhs.p,
+        pcg_iters = -1;
+        if (h->pcg_cooperative) {
+          pcg_iters = launch_pcg_persistent(s, h->Pvar, h->row_ptr.p, h->col_var.p, h->S.p, h->Minv.p, h->rhs.p,
+                                            h->px.p, h->pr.p, h->pz.p, h->pp.p, h->pp2.p, h->pq.p, h->pcg_part.p,
+                                            h->pcg_state.p, h->pcg_host, o->pcg_relative_tolerance,
+                                            o->pcg_max_iterations, 1 << 30);
+          if (pcg_iters < 0) {
+            (void)hipGetLastError();
+            h->pcg_cooperative = false;
+          }
+        }
+        if (pcg_iters < 0) pcg_iters = launch_pcg(s, h->Pvar, h->row_ptr.p, h->col_var.p, h->diag_pos.p, h->S.p, h->Minv.p, h->rhs.p,
                                h->px.p, h->pr.p, h->pz.p, h->pp.p, h->pq.p, h->parts_a.p, h->parts_b.p,
                                h->pcg_state.p, h->pcg_host, o->pcg_relative_tolerance, o->pcg_max_iterations, 8);
         if (pcg_iters < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
@@ -1044,8 +1093,9 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
                          h->lm_z.p, h->fail.p);
   });
   t->schur_rows_ms = timed([&] {
-    launch_schur_rows(s, h->topo, h->vert.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p, h->lm_z.p, h->pose_scale.p,
-                      h->pose_diag.p, inv_mu, o->min_lm_diagonal, o->max_lm_diagonal, h->S.p, h->rhs.p);
+    launch_schur_build(s, h->topo, h->n_tasks, h->schur_tasks.p, h->n_mirror, h->mirror.p, h->nnzb, h->vR.p, h->lm.p,
+                       h->obs_rec.p, h->lm_acc.p, h->lm_z.p, h->pose_scale.p, h->pose_diag.p, inv_mu,
+                       o->min_lm_diagonal, o->max_lm_diagonal, h->S.p, h->rhs.p);
   });
   t->block_inverse_ms = timed([&] { launch_block_inverse(s, h->Pvar, h->diag_pos.p, h->S.p, h->Minv.p, h->fail.p); });
   if (h->Pvar > 0) {

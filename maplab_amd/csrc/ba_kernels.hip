@@ -8,16 +8,21 @@
 //   k_linearize<0>    the same chain, cost only (candidate evaluation)
 //   k_pose_rows       per variable vertex: unscaled gradient and diag(F'F)
 //   k_landmark_prep   per landmark: Z = C (C V C + D^2)^-1 C (Schur pivot)
-//   k_schur_rows      per variable vertex: one block row of the reduced
-//                     camera system S and its rhs, accumulated in LDS
+//   k_schur_tasks     upper block triangle of the reduced camera system S
+//                     and its rhs: (row, incidence batch, column chunk)
+//                     tasks accumulated in LDS, added with FP64 atomics;
+//   k_schur_mirror    lower triangle S_ji = S_ij^T
 //   k_block_inverse   block-Jacobi preconditioner (6x6 inverses of diag(S))
 //   k_pcg_*           preconditioned conjugate gradients on S (FP64)
 //   k_back_substitute landmark step and the LM model cost change
 //   k_candidate       x (+) step for vertices (JPL Plus) and landmarks
 //   k_gradient_norm   max |x - Plus(x, -g)|
 //   k_evaluate_full   every reference Jacobian block (Evaluate parity API)
-// All reductions are fixed-order (per-tile partials summed by one block), so
-// the only run-to-run variation comes from LDS atomics in k_schur_rows.
+// All reductions are fixed-order (per-tile partials summed by one block)
+// except the S build, whose FP64 atomic sums vary in order run to run (last
+// bits only).
+#include <hip/hip_cooperative_groups.h>
+
 #include "ba_device.h"
 #include "ba_math.h"
 
@@ -431,7 +436,8 @@ __global__ void k_landmark_prep(DevTopology t, const double* __restrict__ lm_acc
 
 // ------------------------------------------------------------ Schur rows
 
-constexpr int kSchurChunk = 96;  // S blocks accumulated in LDS per pass
+constexpr int kSchurChunk = 64;  // S blocks accumulated in LDS per task
+constexpr int kSchurBatch = 128; // row incidences per task
 
 // W' = Jl^T (Jo C) (3x6): landmark side unscaled, pose side scaled.
 __device__ inline void wprime_obs(const double* rec, const double* cs, double* W) {
@@ -441,180 +447,190 @@ __device__ inline void wprime_obs(const double* rec, const double* cs, double* W
     for (int c = 0; c < 6; ++c) W[a * 6 + c] = (Jl[a] * Jo[c] + Jl[3 + a] * Jo[6 + c]) * cs[c];
 }
 
-// One block row of S = F'F + D^2 - sum_l W_l' Z_l W_l and rhs = F'r - sum W' Z g.
-__global__ __launch_bounds__(kRowThreads) void k_schur_rows(
-    DevTopology t, const double* __restrict__ vert, const double* __restrict__ vR,
+__global__ void k_fill(size_t n, double* __restrict__ p) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+// Upper block triangle of S = F'F + D^2 - sum_l W_l' Z_l W_l and rhs =
+// F'r - sum W' Z g. One work-group per task = (block row i, a batch of the
+// row's incidences, a chunk of its upper column blocks): the row's pairs
+// (x, y) of incidences sharing a landmark are accumulated in LDS and then
+// added to S with FP64 atomics. S and rhs are zero on entry.
+__global__ __launch_bounds__(kRowThreads) void k_schur_tasks(
+    DevTopology t, const int4* __restrict__ tasks, const double* __restrict__ vR,
     const double* __restrict__ lm, const double* __restrict__ obs_rec, const double* __restrict__ lm_acc,
     const double* __restrict__ lm_z, const double* __restrict__ pose_scale,
     const double* __restrict__ pose_diag, double inv_mu, double min_diag, double max_diag,
     double* __restrict__ S, double* __restrict__ rhs) {
   __shared__ double acc[kSchurChunk * 36];
-  __shared__ double Q[kRowThreads * 18];
-  __shared__ int32_t xinc[kRowThreads];
-  __shared__ int32_t xlm[kRowThreads];
+  __shared__ double Q[kSchurBatch * 18];
+  __shared__ int32_t xinc[kSchurBatch];
+  __shared__ int32_t xlm[kSchurBatch];
   __shared__ int32_t off[kRowThreads + 1];
   __shared__ int32_t colv[kSchurChunk];
   __shared__ double rhs_acc[6];
   __shared__ int32_t wave_tot[kRowThreads / 64];
   const int tid = threadIdx.x;
-  const int i = blockIdx.x;
+  const int4 task = tasks[blockIdx.x];
+  const int i = task.x;
   const int v = t.var_vertex[i];
-  const int rb = t.row_ptr[i], re = t.row_ptr[i + 1];
-  const int ib = t.inc_ptr[i], ie = t.inc_ptr[i + 1];
+  const int re = t.row_ptr[i + 1];
+  const int c0 = task.w;
+  const int nc = min(kSchurChunk, re - c0);
+  const int dp = t.diag_pos[i];
+  const bool rhs_task = c0 == dp;                 // first upper chunk owns the rhs
+  const bool diag_task = rhs_task && task.y == t.inc_ptr[i];
   const double* cs_row = pose_scale + 6 * i;
-  for (int c0 = rb; c0 < re; c0 += kSchurChunk) {
-    const int nc = min(kSchurChunk, re - c0);
-    const bool first = c0 == rb;
-    for (int k = tid; k < nc; k += kRowThreads) colv[k] = t.col_var[c0 + k];
-    for (int k = tid; k < nc * 36; k += kRowThreads) acc[k] = 0.0;
-    if (first && tid < 6) rhs_acc[tid] = 0.0;
-    __syncthreads();
-    for (int xb = ib; xb < ie; xb += kRowThreads) {
-      const int nx = min(kRowThreads, ie - xb);
-      int32_t npairs = 0;
-      if (tid < nx) {
-        const int32_t x = t.inc[xb + tid];
-        const int32_t l = x >= 0 ? t.obs_landmark[x] : -1 - x;
-        npairs = t.lm_begin[l + 1] - t.lm_begin[l] + 1;
-        xinc[tid] = x;
-        xlm[tid] = l;
-        // Q_x = [anchor: C_a M^T] - W_x'^T Z_l, rhs_x = direct - W_x'^T Z g
-        double W[18], Z[9];
-        double rx[6];
-        const double* acc_l = lm_acc + (size_t)kLmAcc * l;
-        sym3_full(lm_z + 6 * l, Z);
-        double Qx[18];
-        if (x >= 0) {
-          const double* rec = obs_rec + (size_t)kObsRecord * x;
-          wprime_obs(rec, cs_row, W);
-          const double* Jo = rec + 8;
-          for (int c = 0; c < 6; ++c) rx[c] = cs_row[c] * (Jo[c] * rec[0] + Jo[6 + c] * rec[1]);
-          for (int k = 0; k < 18; ++k) Qx[k] = 0.0;
-        } else {
-          double M[18], VLM[9];
-          anchor_map(lm + 3 * l, vR + 12 * v, vR[12 * v + 9], M);
-          sym3_full(acc_l + 6, VLM);
-          for (int a = 0; a < 3; ++a)
-            for (int c = 0; c < 6; ++c)
-              W[a * 6 + c] = (VLM[a * 3] * M[c] + VLM[a * 3 + 1] * M[6 + c] + VLM[a * 3 + 2] * M[12 + c]) * cs_row[c];
-          const double* gLM = acc_l + 15;
-          for (int c = 0; c < 6; ++c) {
-            rx[c] = cs_row[c] * (M[c] * gLM[0] + M[6 + c] * gLM[1] + M[12 + c] * gLM[2]);
-            for (int a = 0; a < 3; ++a) Qx[c * 3 + a] = cs_row[c] * M[a * 6 + c];
-          }
-        }
-        const double* gl = acc_l + 12;
-        for (int c = 0; c < 6; ++c) {
-          double wz[3];
-          for (int a = 0; a < 3; ++a)
-            wz[a] = W[c] * Z[a] + W[6 + c] * Z[3 + a] + W[12 + c] * Z[6 + a];  // (W'^T Z)[c][a]
-          for (int a = 0; a < 3; ++a) Qx[c * 3 + a] -= wz[a];
-          rx[c] -= wz[0] * gl[0] + wz[1] * gl[1] + wz[2] * gl[2];
-        }
-        for (int k = 0; k < 18; ++k) Q[tid * 18 + k] = Qx[k];
-        if (first)
-          for (int c = 0; c < 6; ++c) atomicAdd(&rhs_acc[c], rx[c]);
+  for (int k = tid; k < nc; k += kRowThreads) colv[k] = t.col_var[c0 + k];
+  for (int k = tid; k < nc * 36; k += kRowThreads) acc[k] = 0.0;
+  if (tid < 6) rhs_acc[tid] = 0.0;
+  const int nx = task.z - task.y;
+  int32_t npairs = 0;
+  if (tid < nx) {
+    const int32_t x = t.inc[task.y + tid];
+    const int32_t l = x >= 0 ? t.obs_landmark[x] : -1 - x;
+    npairs = t.lm_begin[l + 1] - t.lm_begin[l] + 1;
+    xinc[tid] = x;
+    xlm[tid] = l;
+    // Q_x = [anchor: C_a M^T] - W_x'^T Z_l, rhs_x = direct - W_x'^T Z g
+    double W[18], Z[9], rx[6], Qx[18];
+    const double* acc_l = lm_acc + (size_t)kLmAcc * l;
+    sym3_full(lm_z + 6 * l, Z);
+    if (x >= 0) {
+      const double* rec = obs_rec + (size_t)kObsRecord * x;
+      wprime_obs(rec, cs_row, W);
+      const double* Jo = rec + 8;
+      for (int c = 0; c < 6; ++c) rx[c] = cs_row[c] * (Jo[c] * rec[0] + Jo[6 + c] * rec[1]);
+      for (int k = 0; k < 18; ++k) Qx[k] = 0.0;
+    } else {
+      double M[18], VLM[9];
+      anchor_map(lm + 3 * l, vR + 12 * v, vR[12 * v + 9], M);
+      sym3_full(acc_l + 6, VLM);
+      for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 6; ++c)
+          W[a * 6 + c] = (VLM[a * 3] * M[c] + VLM[a * 3 + 1] * M[6 + c] + VLM[a * 3 + 2] * M[12 + c]) * cs_row[c];
+      const double* gLM = acc_l + 15;
+      for (int c = 0; c < 6; ++c) {
+        rx[c] = cs_row[c] * (M[c] * gLM[0] + M[6 + c] * gLM[1] + M[12 + c] * gLM[2]);
+        for (int a = 0; a < 3; ++a) Qx[c * 3 + a] = cs_row[c] * M[a * 6 + c];
       }
-      // Exclusive scan of npairs over the block.
-      {
-        const int lane = tid & 63, wave = tid >> 6;
-        int32_t incl = npairs;
-        for (int d = 1; d < 64; d <<= 1) {
-          const int32_t y = __shfl_up(incl, d, 64);
-          if (lane >= d) incl += y;
-        }
-        if (lane == 63) wave_tot[wave] = incl;
-        __syncthreads();
-        int32_t base = 0;
-        for (int w = 0; w < wave; ++w) base += wave_tot[w];
-        off[tid + 1] = base + incl;
-        if (tid == 0) off[0] = 0;
-        __syncthreads();
-      }
-      const int total = off[nx];
-      for (int f = tid; f < total; f += kRowThreads) {
-        int lo = 0, hi = nx - 1;  // largest xi with off[xi] <= f
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (off[mid] <= f) lo = mid; else hi = mid - 1;
-        }
-        const int xi = lo;
-        const int32_t x = xinc[xi];
-        const int32_t l = xlm[xi];
-        const int32_t lb = t.lm_begin[l];
-        const int32_t nobs = t.lm_begin[l + 1] - lb;
-        const int32_t y = f - off[xi];
-        const uint32_t flags = t.lm_flags[l];
-        const bool lconst = (flags & kLmConstant) != 0;
-        int32_t col;
-        double B[36];
-        const double* Qx = Q + xi * 18;
-        if (y < nobs) {
-          const int32_t oy = lb + y;
-          const int32_t meta = t.obs_meta[oy];
-          if (obs_type(meta) == kTypeKeyframe || !obs_observer_var(meta)) continue;
-          const bool same = (x == oy);
-          if (lconst && x >= 0 && !same) continue;
-          col = t.vertex_var[t.obs_vertex[oy]];
-          const double* rec = obs_rec + (size_t)kObsRecord * oy;
-          const double* cs = pose_scale + 6 * col;
-          double W[18];
-          wprime_obs(rec, cs, W);
-          for (int a = 0; a < 6; ++a)
-            for (int c = 0; c < 6; ++c)
-              B[a * 6 + c] = Qx[a * 3] * W[c] + Qx[a * 3 + 1] * W[6 + c] + Qx[a * 3 + 2] * W[12 + c];
-          if (same) {
-            const double* Jo = rec + 8;
-            for (int a = 0; a < 6; ++a)
-              for (int c = 0; c < 6; ++c) B[a * 6 + c] += cs[a] * cs[c] * (Jo[a] * Jo[c] + Jo[6 + a] * Jo[6 + c]);
-          }
-        } else {
-          if (!(flags & kLmAnchorVar)) continue;
-          const int32_t a = t.lm_anchor[l];
-          col = t.vertex_var[a];
-          const double* cs = pose_scale + 6 * col;
-          double M[18], VLM[9], W[18];
-          anchor_map(lm + 3 * l, vR + 12 * a, vR[12 * a + 9], M);
-          sym3_full(lm_acc + (size_t)kLmAcc * l + 6, VLM);
-          for (int r = 0; r < 3; ++r)
-            for (int c = 0; c < 6; ++c)
-              W[r * 6 + c] = (VLM[r * 3] * M[c] + VLM[r * 3 + 1] * M[6 + c] + VLM[r * 3 + 2] * M[12 + c]) * cs[c];
-          for (int r = 0; r < 6; ++r)
-            for (int c = 0; c < 6; ++c)
-              B[r * 6 + c] = Qx[r * 3] * W[c] + Qx[r * 3 + 1] * W[6 + c] + Qx[r * 3 + 2] * W[12 + c];
-          if (x >= 0) {  // direct observer-anchor coupling: W_x'^T M C_a
-            double Wx[18];
-            wprime_obs(obs_rec + (size_t)kObsRecord * x, cs_row, Wx);
-            for (int r = 0; r < 6; ++r)
-              for (int c = 0; c < 6; ++c)
-                B[r * 6 + c] += cs[c] * (Wx[r] * M[c] + Wx[6 + r] * M[6 + c] + Wx[12 + r] * M[12 + c]);
-          }
-        }
-        // Column position inside this chunk.
-        int lo2 = 0, hi2 = nc - 1, pos = -1;
-        while (lo2 <= hi2) {
-          const int mid = (lo2 + hi2) >> 1;
-          const int cv = colv[mid];
-          if (cv == col) { pos = mid; break; }
-          if (cv < col) lo2 = mid + 1; else hi2 = mid - 1;
-        }
-        if (pos < 0) continue;
-        double* dst = acc + pos * 36;
-        for (int k = 0; k < 36; ++k) atomicAdd(dst + k, B[k]);
-      }
-      __syncthreads();
     }
-    // LM diagonal of the pose columns on the diagonal block.
-    const int dp = t.diag_pos[i];
-    if (dp >= c0 && dp < c0 + nc && tid < 6) {
-      const double d = cs_row[tid] * cs_row[tid] * pose_diag[6 * i + tid];
-      acc[(dp - c0) * 36 + tid * 7] += fmin(fmax(d, min_diag), max_diag) * inv_mu;
+    const double* gl = acc_l + 12;
+    for (int c = 0; c < 6; ++c) {
+      double wz[3];
+      for (int a = 0; a < 3; ++a) wz[a] = W[c] * Z[a] + W[6 + c] * Z[3 + a] + W[12 + c] * Z[6 + a];
+      for (int a = 0; a < 3; ++a) Qx[c * 3 + a] -= wz[a];
+      rx[c] -= wz[0] * gl[0] + wz[1] * gl[1] + wz[2] * gl[2];
     }
+    for (int k = 0; k < 18; ++k) Q[tid * 18 + k] = Qx[k];
+    if (rhs_task)
+      for (int c = 0; c < 6; ++c) atomicAdd(&rhs_acc[c], rx[c]);
+  }
+  // Exclusive scan of npairs over the block.
+  {
+    const int lane = tid & 63, wave = tid >> 6;
+    int32_t incl = npairs;
+    for (int d = 1; d < 64; d <<= 1) {
+      const int32_t y = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += y;
+    }
+    if (lane == 63) wave_tot[wave] = incl;
     __syncthreads();
-    for (int k = tid; k < nc * 36; k += kRowThreads) S[(size_t)c0 * 36 + k] = acc[k];
-    if (first && tid < 6) rhs[6 * i + tid] = rhs_acc[tid];
+    int32_t base = 0;
+    for (int w = 0; w < wave; ++w) base += wave_tot[w];
+    off[tid + 1] = base + incl;
+    if (tid == 0) off[0] = 0;
     __syncthreads();
   }
+  const int total = off[nx];
+  const int col_lo = colv[0], col_hi = colv[nc - 1];
+  for (int f = tid; f < total; f += kRowThreads) {
+    int lo = 0, hi = nx - 1;  // largest xi with off[xi] <= f
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (off[mid] <= f) lo = mid; else hi = mid - 1;
+    }
+    const int xi = lo;
+    const int32_t l = xlm[xi];
+    const int32_t lb = t.lm_begin[l];
+    const int32_t nobs = t.lm_begin[l + 1] - lb;
+    const int32_t y = f - off[xi];
+    const int32_t oy = lb + y;
+    const int32_t col = y < nobs ? t.obs_col[oy]
+                                 : ((t.lm_flags[l] & kLmAnchorVar) ? t.vertex_var[t.lm_anchor[l]] : -1);
+    if (col < col_lo || col > col_hi) continue;  // other chunk, lower triangle, or no incidence
+    int plo = 0, phi = nc - 1, pos = -1;
+    while (plo <= phi) {
+      const int mid = (plo + phi) >> 1;
+      const int cv = colv[mid];
+      if (cv == col) { pos = mid; break; }
+      if (cv < col) plo = mid + 1; else phi = mid - 1;
+    }
+    if (pos < 0) continue;
+    const int32_t x = xinc[xi];
+    const bool lconst = (t.lm_flags[l] & kLmConstant) != 0;
+    const double* Qx = Q + xi * 18;
+    double* dst = acc + pos * 36;
+    const double* cs = pose_scale + 6 * col;
+    if (y < nobs) {
+      const bool same = (x == oy);
+      if (lconst && x >= 0 && !same) continue;
+      const double* rec = obs_rec + (size_t)kObsRecord * oy;
+      double W[18];
+      wprime_obs(rec, cs, W);
+      const double* Jo = rec + 8;
+      for (int r = 0; r < 6; ++r) {
+        const double q0 = Qx[r * 3], q1 = Qx[r * 3 + 1], q2 = Qx[r * 3 + 2];
+        for (int c = 0; c < 6; ++c) {
+          double b = q0 * W[c] + q1 * W[6 + c] + q2 * W[12 + c];
+          if (same) b += cs[r] * cs[c] * (Jo[r] * Jo[c] + Jo[6 + r] * Jo[6 + c]);
+          atomicAdd(dst + r * 6 + c, b);
+        }
+      }
+    } else {
+      const int32_t a = t.lm_anchor[l];
+      double M[18], VLM[9], W[18];
+      anchor_map(lm + 3 * l, vR + 12 * a, vR[12 * a + 9], M);
+      sym3_full(lm_acc + (size_t)kLmAcc * l + 6, VLM);
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 6; ++c)
+          W[r * 6 + c] = (VLM[r * 3] * M[c] + VLM[r * 3 + 1] * M[6 + c] + VLM[r * 3 + 2] * M[12 + c]) * cs[c];
+      double Wx[18];
+      if (x >= 0) wprime_obs(obs_rec + (size_t)kObsRecord * x, cs_row, Wx);
+      for (int r = 0; r < 6; ++r) {
+        const double q0 = Qx[r * 3], q1 = Qx[r * 3 + 1], q2 = Qx[r * 3 + 2];
+        for (int c = 0; c < 6; ++c) {
+          double b = q0 * W[c] + q1 * W[6 + c] + q2 * W[12 + c];
+          // direct observer-anchor coupling: W_x'^T M C_a
+          if (x >= 0) b += cs[c] * (Wx[r] * M[c] + Wx[6 + r] * M[6 + c] + Wx[12 + r] * M[12 + c]);
+          atomicAdd(dst + r * 6 + c, b);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // LM diagonal of the pose columns, added once per row.
+  if (diag_task && tid < 6) {
+    const double d = cs_row[tid] * cs_row[tid] * pose_diag[6 * i + tid];
+    acc[(dp - c0) * 36 + tid * 7] += fmin(fmax(d, min_diag), max_diag) * inv_mu;
+  }
+  __syncthreads();
+  for (int k = tid; k < nc * 36; k += kRowThreads) {
+    const double val = acc[k];
+    if (val != 0.0) atomicAdd(S + (size_t)c0 * 36 + k, val);
+  }
+  if (rhs_task && tid < 6) atomicAdd(rhs + 6 * i + tid, rhs_acc[tid]);
+}
+
+// Lower block triangle from the upper one: S_ji = S_ij^T.
+__global__ void k_schur_mirror(int32_t n, const int2* __restrict__ pairs, double* __restrict__ S) {
+  const int k = blockIdx.x * (blockDim.x / 36) + threadIdx.x / 36;
+  const int e = threadIdx.x % 36;
+  if (k >= n || threadIdx.x >= (blockDim.x / 36) * 36) return;
+  const int2 pr = pairs[k];
+  const int r = e / 6, c = e % 6;
+  S[(size_t)pr.y * 36 + c * 6 + r] = S[(size_t)pr.x * 36 + r * 6 + c];
 }
 
 // ------------------------------------------------------------ PCG
@@ -827,6 +843,162 @@ __global__ void k_pcg_scalars(int32_t nparts, const double* __restrict__ part, P
     st->rr = b;
     st->iters += 1;
     if (b <= st->tol2 || st->iters >= st->max_iters) st->done = 1;
+  }
+}
+
+// Whole PCG solve in one cooperative launch. Work-group g owns block rows
+// [g R, (g+1) R). Per iteration:
+//   phase 1: beta from the r'z partials; p_k = z + beta p_{k-1} for the owned
+//            rows (double-buffered: readers form neighbour entries on the
+//            fly from z and p_{k-1}); q = S p_k; partial p'q.   grid barrier
+//   phase 2: alpha = r'z / p'q; x += alpha p; r -= alpha q; z = M^-1 r;
+//            partials r'z and r'r.                             grid barrier
+// Every work-group sums the partials in the same order, so all take the same
+// exit decision (converged or max_iters) and pass the same barriers.
+__global__ __launch_bounds__(256) void k_pcg_persistent(
+    int32_t Pvar, int32_t rows_per_group, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col_var,
+    const double* __restrict__ S, const double* __restrict__ Minv, const double* __restrict__ b,
+    double* __restrict__ x, double* __restrict__ r, double* __restrict__ z, double* __restrict__ p0,
+    double* __restrict__ p1, double* __restrict__ q, double* __restrict__ part, double tol, int32_t max_iters,
+    PcgState* __restrict__ st) {
+  namespace cg = cooperative_groups;
+  cg::grid_group grid = cg::this_grid();
+  __shared__ double lds4[4];
+  __shared__ double red[4][6];
+  __shared__ double bsum[3];
+  const int G = gridDim.x;
+  const int g = blockIdx.x;
+  const int row0 = g * rows_per_group;
+  const int row1 = min(Pvar, row0 + rows_per_group);
+  double* pq_part = part;
+  double* rz_part = part + G;
+  double* rr_part = part + 2 * G;
+  // r = b, x = 0, z = M^-1 r, p_prev = 0.
+  double rz_l = 0.0, bb_l = 0.0;
+  for (int e = threadIdx.x; e < 6 * (row1 - row0); e += blockDim.x) {
+    const int i = row0 + e / 6, a = e % 6;
+    r[6 * i + a] = b[6 * i + a];
+    x[6 * i + a] = 0.0;
+    p0[6 * i + a] = 0.0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 6 * (row1 - row0); e += blockDim.x) {
+    const int i = row0 + e / 6, a = e % 6;
+    const double* M = Minv + (size_t)36 * i + a * 6;
+    double s = 0.0;
+    for (int c = 0; c < 6; ++c) s += M[c] * r[6 * i + c];
+    z[6 * i + a] = s;
+    rz_l += r[6 * i + a] * s;
+    bb_l += r[6 * i + a] * r[6 * i + a];
+  }
+  {
+    const double s1 = block_sum256(rz_l, lds4);
+    const double s2 = block_sum256(bb_l, lds4);
+    if (threadIdx.x == 0) { rz_part[g] = s1; rr_part[g] = s2; }
+  }
+  grid.sync();
+  double bb, rz_prev = 1.0;
+  {
+    double a1 = 0.0, a2 = 0.0;
+    for (int k = threadIdx.x; k < G; k += blockDim.x) a2 += rr_part[k];
+    a2 = block_sum256(a2, lds4);
+    if (threadIdx.x == 0) bsum[0] = a2;
+    __syncthreads();
+    bb = bsum[0];
+    (void)a1;
+  }
+  const double tol2 = tol * tol * bb;
+  int it = 0;
+  bool done = !(bb > 0.0) || max_iters <= 0;
+  while (!done) {
+    double* p_prev = (it & 1) ? p1 : p0;
+    double* p_cur = (it & 1) ? p0 : p1;
+    // ---- phase 1
+    double rz_sum = 0.0;
+    for (int k = threadIdx.x; k < G; k += blockDim.x) rz_sum += rz_part[k];
+    rz_sum = block_sum256(rz_sum, lds4);
+    if (threadIdx.x == 0) bsum[1] = rz_sum;
+    __syncthreads();
+    const double rz = bsum[1];
+    const double beta = it == 0 ? 0.0 : rz / rz_prev;
+    rz_prev = rz;
+    double pq_l = 0.0;
+    for (int i = row0; i < row1; ++i) {
+      double y[6] = {0, 0, 0, 0, 0, 0};
+      for (int k = row_ptr[i] + threadIdx.x; k < row_ptr[i + 1]; k += blockDim.x) {
+        const double* blk = S + (size_t)36 * k;
+        const int j = col_var[k];
+        double xr[6];
+        for (int c = 0; c < 6; ++c) xr[c] = z[6 * j + c] + beta * p_prev[6 * j + c];
+        for (int a = 0; a < 6; ++a) {
+          double s = 0.0;
+          for (int c = 0; c < 6; ++c) s += blk[a * 6 + c] * xr[c];
+          y[a] += s;
+        }
+      }
+      for (int a = 0; a < 6; ++a) y[a] = wave_sum(y[a]);
+      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+      if (lane == 0) for (int a = 0; a < 6; ++a) red[wave][a] = y[a];
+      __syncthreads();
+      if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        const double qa = (red[0][a] + red[1][a]) + (red[2][a] + red[3][a]);
+        const double pa = z[6 * i + a] + beta * p_prev[6 * i + a];
+        q[6 * i + a] = qa;
+        p_cur[6 * i + a] = pa;
+        pq_l = pa * qa;
+      }
+      __syncthreads();
+      // fold the 6 products of this row into thread 0's running sum
+      pq_l = wave_sum(threadIdx.x < 64 ? pq_l : 0.0);
+      if (threadIdx.x != 0) pq_l = 0.0;
+      if (threadIdx.x == 0) bsum[2] = (i == row0 ? 0.0 : bsum[2]) + pq_l;
+      __syncthreads();
+      pq_l = 0.0;
+    }
+    if (threadIdx.x == 0) pq_part[g] = row1 > row0 ? bsum[2] : 0.0;
+    grid.sync();
+    // ---- phase 2
+    double pq = 0.0;
+    for (int k = threadIdx.x; k < G; k += blockDim.x) pq += pq_part[k];
+    pq = block_sum256(pq, lds4);
+    if (threadIdx.x == 0) bsum[0] = pq;
+    __syncthreads();
+    const double alpha = rz / bsum[0];
+    double rz_l2 = 0.0, rr_l2 = 0.0;
+    for (int e = threadIdx.x; e < 6 * (row1 - row0); e += blockDim.x) {
+      const int i = row0 + e / 6, a = e % 6;
+      x[6 * i + a] += alpha * p_cur[6 * i + a];
+      r[6 * i + a] -= alpha * q[6 * i + a];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 6 * (row1 - row0); e += blockDim.x) {
+      const int i = row0 + e / 6, a = e % 6;
+      const double* M = Minv + (size_t)36 * i + a * 6;
+      double s = 0.0;
+      for (int c = 0; c < 6; ++c) s += M[c] * r[6 * i + c];
+      z[6 * i + a] = s;
+      rz_l2 += r[6 * i + a] * s;
+      rr_l2 += r[6 * i + a] * r[6 * i + a];
+    }
+    {
+      const double s1 = block_sum256(rz_l2, lds4);
+      const double s2 = block_sum256(rr_l2, lds4);
+      if (threadIdx.x == 0) { rz_part[g] = s1; rr_part[g] = s2; }
+    }
+    grid.sync();
+    ++it;
+    double rr = 0.0;
+    for (int k = threadIdx.x; k < G; k += blockDim.x) rr += rr_part[k];
+    rr = block_sum256(rr, lds4);
+    if (threadIdx.x == 0) bsum[0] = rr;
+    __syncthreads();
+    done = (bsum[0] <= tol2) || it >= max_iters;
+  }
+  if (g == 0 && threadIdx.x == 0) {
+    st->iters = it;
+    st->done = 1;
+    st->bb = bb;
   }
 }
 
@@ -1277,14 +1449,21 @@ void launch_landmark_prep(hipStream_t s, const DevTopology& t, const double* lm_
   if (t.L > 0)
     k_landmark_prep<<<(t.L + 255) / 256, 256, 0, s>>>(t, lm_acc, lm_scale, inv_mu, min_diag, max_diag, lm_z, fail);
 }
-void launch_schur_rows(hipStream_t s, const DevTopology& t, const double* vert, const double* vR, const double* lm,
-                       const double* obs_rec, const double* lm_acc, const double* lm_z, const double* pose_scale,
-                       const double* pose_diag, double inv_mu, double min_diag, double max_diag, double* S,
-                       double* rhs) {
-  if (t.Pvar > 0)
-    k_schur_rows<<<t.Pvar, kRowThreads, 0, s>>>(t, vert, vR, lm, obs_rec, lm_acc, lm_z, pose_scale, pose_diag, inv_mu,
-                                                min_diag, max_diag, S, rhs);
+void launch_schur_build(hipStream_t s, const DevTopology& t, int32_t ntasks, const int4* tasks, int32_t nmirror,
+                        const int2* mirror, int64_t nnzb, const double* vR, const double* lm, const double* obs_rec,
+                        const double* lm_acc, const double* lm_z, const double* pose_scale, const double* pose_diag,
+                        double inv_mu, double min_diag, double max_diag, double* S, double* rhs) {
+  if (t.Pvar <= 0) return;
+  const size_t ns = (size_t)nnzb * 36;
+  k_fill<<<(unsigned)((ns + 255) / 256), 256, 0, s>>>(ns, S);
+  k_fill<<<(unsigned)((6 * (size_t)t.Pvar + 255) / 256), 256, 0, s>>>(6 * (size_t)t.Pvar, rhs);
+  if (ntasks > 0)
+    k_schur_tasks<<<ntasks, kRowThreads, 0, s>>>(t, tasks, vR, lm, obs_rec, lm_acc, lm_z, pose_scale, pose_diag,
+                                                 inv_mu, min_diag, max_diag, S, rhs);
+  if (nmirror > 0) k_schur_mirror<<<(nmirror + 6) / 7, 252, 0, s>>>(nmirror, mirror, S);
 }
+int32_t schur_chunk_blocks() { return kSchurChunk; }
+int32_t schur_batch_incidences() { return kSchurBatch; }
 void launch_block_inverse(hipStream_t s, int32_t Pvar, const int32_t* diag_pos, const double* S, double* Minv,
                           int32_t* fail) {
   if (Pvar > 0) k_block_inverse<<<(Pvar + 63) / 64, 64, 0, s>>>(Pvar, diag_pos, S, Minv, fail);
@@ -1320,6 +1499,34 @@ int launch_pcg(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_
   return hs->iters;
 }
 size_t pcg_state_bytes() { return sizeof(PcgState); }
+
+// Cooperative (single-launch) PCG; returns the iteration count, or -1 when the
+// device cannot co-schedule the grid (the caller then uses launch_pcg).
+int launch_pcg_persistent(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_t* col_var, const double* S,
+                          const double* Minv, const double* b, double* x, double* r, double* z, double* p0,
+                          double* p1, double* q, double* part, void* state_dev, void* state_host, double tol,
+                          int32_t max_iters, int32_t max_groups) {
+  static int cached_groups = 0;
+  if (cached_groups == 0) {
+    int dev = 0, cus = 0, per_cu = 0, coop = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop) return -1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_persistent, 256, 0) != hipSuccess) return -1;
+    cached_groups = std::max(1, std::min(cus, cus * per_cu));
+  }
+  const int G = std::max(1, std::min({cached_groups, max_groups, Pvar}));
+  int32_t rows_per_group = (Pvar + G - 1) / G;
+  PcgState* st = static_cast<PcgState*>(state_dev);
+  void* args[] = {&Pvar, &rows_per_group, (void*)&row_ptr, (void*)&col_var, (void*)&S, (void*)&Minv, (void*)&b,
+                  &x, &r, &z, &p0, &p1, &q, &part, &tol, &max_iters, &st};
+  if (hipLaunchCooperativeKernel((const void*)k_pcg_persistent, dim3(G), dim3(256), args, 0, s) != hipSuccess)
+    return -1;
+  PcgState* hs = static_cast<PcgState*>(state_host);
+  if (hipMemcpyAsync(hs, st, sizeof(PcgState), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+  if (hipStreamSynchronize(s) != hipSuccess) return -1;
+  return hs->iters;
+}
 
 // Pieces of one PCG iteration, for kernel timing: initialize the state, then
 // the matrix-vector product alone or the vector updates alone.
