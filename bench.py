@@ -63,23 +63,27 @@ def kernel_table(sm, kt, n_pcg_total):
     add("k_landmark_prep", abi.KERNEL_LANDMARK_PREP, L * (144 + 24 + 48))
     launches = sm.kernel_launches[abi.KERNEL_PCG]
     if launches and n_pcg_total > 0:
-        # One cooperative launch per LM step runs the whole PCG solve; per PCG
-        # iteration it streams S once (6x6 FP64 blocks + column index) and
-        # ~12 FP64 vectors of the reduced system.
+        # The PCG solve of each LM step: two launches per PCG iteration
+        # (S p with the direction update, one-block vector update). Per
+        # iteration it streams S once and ~14 FP64 vectors of the system.
         it_ms = sm.kernel_ms[abi.KERNEL_PCG] / n_pcg_total
-        it_bytes = nnzb * 292 + P * 6 * 8 * 12
-        rows["k_pcg_persistent"] = {
+        it_bytes = nnzb * 292 + P * 6 * 8 * 14
+        rows["pcg_solve"] = {
             "launches": launches, "total_ms": round(sm.kernel_ms[abi.KERNEL_PCG], 4),
             "avg_ms": round(sm.kernel_ms[abi.KERNEL_PCG] / launches, 5), "pcg_iterations": int(n_pcg_total),
-            "per_iteration_ms": round(it_ms, 5), "bound": "hbm", "bytes": int(it_bytes),
-            "achieved_gbs": round(it_bytes / it_ms / 1e6, 1), "frac": round(it_bytes / it_ms / 1e6 / HBM_PEAK_GBS, 4),
-            "note": "bytes and rate per PCG iteration"}
-    if kt.pcg_spmv_ms > 0:
-        spmv_bytes = nnzb * 292 + P * 104
-        rows["k_pcg_spmv_standalone"] = {
-            "avg_ms": round(kt.pcg_spmv_ms, 5), "bytes": int(spmv_bytes),
+            "per_iteration_ms": round(it_ms, 5), "bytes_per_iteration": int(it_bytes),
+            "achieved_gbs": round(it_bytes / it_ms / 1e6, 1),
+            "note": "events around the whole solve: both kernels, launch gaps and the host convergence polls"}
+    if kt.pcg_spmv_ms > 0 and n_pcg_total > 0:
+        spmv_bytes = nnzb * 292 + P * 6 * 8 * 4
+        rows["k_pcg_spmv"] = {
+            "launches": int(n_pcg_total), "avg_ms": round(kt.pcg_spmv_ms, 5),
+            "total_ms": round(kt.pcg_spmv_ms * n_pcg_total, 4), "bound": "hbm", "bytes": int(spmv_bytes),
             "achieved_gbs": round(spmv_bytes / kt.pcg_spmv_ms / 1e6, 1),
-            "note": "one stand-alone S p launch (mlba_time_kernels), not used by the solve"}
+            "frac": round(spmv_bytes / kt.pcg_spmv_ms / 1e6 / HBM_PEAK_GBS, 4),
+            "timing": "avg from mlba_time_kernels events on the library stream, x PCG iterations of the timed region"}
+        rows["k_pcg_update"] = {"launches": int(n_pcg_total), "avg_ms": round(kt.pcg_vector_ms, 5),
+                                "total_ms": round(kt.pcg_vector_ms * n_pcg_total, 4)}
     return rows
 
 

@@ -44,21 +44,18 @@ int32_t schur_chunk_blocks();
 int32_t schur_batch_incidences();
 void launch_block_inverse(hipStream_t s, int32_t Pvar, const int32_t* diag_pos, const double* S, double* Minv,
                           int32_t* fail);
-int launch_pcg(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_t* col_var, const int32_t* diag_pos,
-               const double* S, const double* Minv, const double* b, double* x, double* r, double* z, double* p,
-               double* q, double* parts_a, double* parts_b, void* state_dev, void* state_host, double tol,
-               int32_t max_iters, int32_t poll);
+int launch_pcg(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_t* col_var, const double* S,
+               const double* Minv, const double* b, double* x, double* r, double* z, double* p0, double* p1,
+               double* q, double* part, void* state_dev, void* state_host, double tol, int32_t max_iters,
+               int32_t poll);
 size_t pcg_state_bytes();
-int launch_pcg_persistent(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_t* col_var, const double* S,
-                          const double* Minv, const double* b, double* x, double* r, double* z, double* p0,
-                          double* p1, double* q, double* part, void* state_dev, void* state_host, double tol,
-                          int32_t max_iters, int32_t max_groups);
 void launch_pcg_setup(hipStream_t s, int32_t Pvar, const double* b, const double* Minv, double* x, double* r, double* z,
-                      double* p, double* parts_a, void* state_dev);
+                      double* p, void* state_dev);
 void launch_pcg_spmv(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_t* col_var, const double* S,
-                     const double* p, double* q, double* parts_a, const void* state_dev);
-void launch_pcg_vectors(hipStream_t s, int32_t Pvar, const double* parts_a, const double* Minv, double* p,
-                        const double* q, double* x, double* r, double* z, double* parts_b, void* state_dev);
+                     const double* z, const double* p_prev, double* p_cur, double* q, double* part,
+                     const void* state_dev);
+void launch_pcg_vectors(hipStream_t s, int32_t Pvar, const double* part, const double* Minv, const double* p,
+                        const double* q, double* x, double* r, double* z, void* state_dev);
 void launch_back_substitute(hipStream_t s, const DevTopology& t, const double* vR, const double* lm,
                             const double* obs_rec, const double* lm_acc, const double* lm_z, const double* w,
                             double* u, double* part);
@@ -128,7 +125,6 @@ struct mlba_handle {
   int32_t Pvar = 0, tiles = 0, nnzb = 0, n_inc = 0, n_split = 0, max_row_blocks = 0;
   int64_t schur_pairs = 0;
   int32_t n_tasks = 0, n_mirror = 0;
-  bool pcg_cooperative = true;  // cleared when the device cannot co-schedule the PCG grid
   // Device memory.
   DBuf<double> obs_kp, obs_sinv, obs_rec;
   DBuf<int32_t> obs_vertex, obs_meta, obs_landmark, perm_d;
@@ -515,20 +511,9 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
       int pcg_iters = 0;
       if (h->Pvar > 0) {
         KernelTimer kt(h, MLBA_KERNEL_PCG);
-        pcg_iters = -1;
-        if (h->pcg_cooperative) {
-          pcg_iters = launch_pcg_persistent(s, h->Pvar, h->row_ptr.p, h->col_var.p, h->S.p, h->Minv.p, h->rhs.p,
-                                            h->px.p, h->pr.p, h->pz.p, h->pp.p, h->pp2.p, h->pq.p, h->pcg_part.p,
-                                            h->pcg_state.p, h->pcg_host, o->pcg_relative_tolerance,
-                                            o->pcg_max_iterations, 1 << 30);
-          if (pcg_iters < 0) {
-            (void)hipGetLastError();
-            h->pcg_cooperative = false;
-          }
-        }
-        if (pcg_iters < 0) pcg_iters = launch_pcg(s, h->Pvar, h->row_ptr.p, h->col_var.p, h->diag_pos.p, h->S.p, h->Minv.p, h->rhs.p,
-                               h->px.p, h->pr.p, h->pz.p, h->pp.p, h->pq.p, h->parts_a.p, h->parts_b.p,
-                               h->pcg_state.p, h->pcg_host, o->pcg_relative_tolerance, o->pcg_max_iterations, 8);
+        pcg_iters = launch_pcg(s, h->Pvar, h->row_ptr.p, h->col_var.p, h->S.p, h->Minv.p, h->rhs.p, h->px.p,
+                               h->pr.p, h->pz.p, h->pp.p, h->pp2.p, h->pq.p, h->pcg_part.p, h->pcg_state.p,
+                               h->pcg_host, o->pcg_relative_tolerance, o->pcg_max_iterations, 16);
         if (pcg_iters < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
       }
       launch_unscale_poses(s, h->Pvar, h->px.p, h->pose_scale.p, h->w.p, h->parts_w.p);
@@ -1099,14 +1084,14 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
   });
   t->block_inverse_ms = timed([&] { launch_block_inverse(s, h->Pvar, h->diag_pos.p, h->S.p, h->Minv.p, h->fail.p); });
   if (h->Pvar > 0) {
-    launch_pcg_setup(s, h->Pvar, h->rhs.p, h->Minv.p, h->px.p, h->pr.p, h->pz.p, h->pp.p, h->parts_a.p,
-                     h->pcg_state.p);
+    launch_pcg_setup(s, h->Pvar, h->rhs.p, h->Minv.p, h->px.p, h->pr.p, h->pz.p, h->pp.p, h->pcg_state.p);
     t->pcg_spmv_ms = timed([&] {
-      launch_pcg_spmv(s, h->Pvar, h->row_ptr.p, h->col_var.p, h->S.p, h->pp.p, h->pq.p, h->parts_a.p, h->pcg_state.p);
+      launch_pcg_spmv(s, h->Pvar, h->row_ptr.p, h->col_var.p, h->S.p, h->pz.p, h->pp.p, h->pp2.p, h->pq.p,
+                      h->pcg_part.p, h->pcg_state.p);
     });
     t->pcg_vector_ms = timed([&] {
-      launch_pcg_vectors(s, h->Pvar, h->parts_a.p, h->Minv.p, h->pp.p, h->pq.p, h->px.p, h->pr.p, h->pz.p,
-                         h->parts_b.p, h->pcg_state.p);
+      launch_pcg_vectors(s, h->Pvar, h->pcg_part.p, h->Minv.p, h->pp2.p, h->pq.p, h->px.p, h->pr.p, h->pz.p,
+                         h->pcg_state.p);
     });
   }
   launch_unscale_poses(s, h->Pvar, h->px.p, h->pose_scale.p, h->w.p, h->parts_w.p);

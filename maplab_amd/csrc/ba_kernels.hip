@@ -13,7 +13,8 @@
 //                     tasks accumulated in LDS, added with FP64 atomics;
 //   k_schur_mirror    lower triangle S_ji = S_ij^T
 //   k_block_inverse   block-Jacobi preconditioner (6x6 inverses of diag(S))
-//   k_pcg_*           preconditioned conjugate gradients on S (FP64)
+//   k_pcg_*           preconditioned conjugate gradients on S (FP64), two
+//                     launches per iteration (row-parallel S p, one-block update)
 //   k_back_substitute landmark step and the LM model cost change
 //   k_candidate       x (+) step for vertices (JPL Plus) and landmarks
 //   k_gradient_norm   max |x - Plus(x, -g)|
@@ -21,8 +22,6 @@
 // All reductions are fixed-order (per-tile partials summed by one block)
 // except the S build, whose FP64 atomic sums vary in order run to run (last
 // bits only).
-#include <hip/hip_cooperative_groups.h>
-
 #include "ba_device.h"
 #include "ba_math.h"
 
@@ -681,82 +680,84 @@ struct PcgState {
   double bb;        // |b|^2
   double rr;        // |r|^2
   double tol2;      // (tol |b|)^2
+  double beta;      // direction update factor for the next iteration
   int32_t iters;
   int32_t done;
   int32_t max_iters;
   int32_t pad;
 };
 
-// Fixed-order sum of n partials (every block computes the same value).
-__device__ inline double sum_partials(const double* __restrict__ p, int n, double* lds4) {
-  double s = 0.0;
-  for (int k = threadIdx.x; k < n; k += blockDim.x) s += p[k];
-  s = block_sum256(s, lds4);
-  __shared__ double bcast;
-  if (threadIdx.x == 0) bcast = s;
+constexpr int kPcgVecThreads = 1024;
+
+// Block-wide sum for a kPcgVecThreads block (16 waves); valid in every thread.
+__device__ inline double block_sum_big(double v, double* lds16) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __syncthreads();
-  return bcast;
+  if (lane == 0) lds16[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < kPcgVecThreads / 64; ++w) s += lds16[w];
+  return s;
 }
 
-// r = b, x = 0, z = M^-1 r, p = z; partials of r'z and b'b.
-__global__ __launch_bounds__(256) void k_pcg_init(int32_t Pvar, const double* __restrict__ b,
-                                                  const double* __restrict__ Minv, double* __restrict__ x,
-                                                  double* __restrict__ r, double* __restrict__ z,
-                                                  double* __restrict__ p, double* __restrict__ part) {
-  __shared__ double lds4[4];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// r = b, x = 0, z = M^-1 r, p = 0; r'z, b'b, tolerance (one block).
+__global__ __launch_bounds__(kPcgVecThreads) void k_pcg_init(int32_t Pvar, const double* __restrict__ b,
+                                                             const double* __restrict__ Minv, double* __restrict__ x,
+                                                             double* __restrict__ r, double* __restrict__ z,
+                                                             double* __restrict__ p, double tol, int32_t max_iters,
+                                                             PcgState* __restrict__ st) {
+  __shared__ double lds16[kPcgVecThreads / 64];
   double rz = 0.0, bb = 0.0;
-  if (i < Pvar) {
+  for (int i = threadIdx.x; i < Pvar; i += kPcgVecThreads) {
     const double* M = Minv + (size_t)36 * i;
     double rv[6];
-    for (int a = 0; a < 6; ++a) { rv[a] = b[6 * i + a]; r[6 * i + a] = rv[a]; x[6 * i + a] = 0.0; }
+    for (int a = 0; a < 6; ++a) {
+      rv[a] = b[6 * i + a];
+      r[6 * i + a] = rv[a];
+      x[6 * i + a] = 0.0;
+      p[6 * i + a] = 0.0;
+      bb += rv[a] * rv[a];
+    }
     for (int a = 0; a < 6; ++a) {
       double s = 0.0;
       for (int c = 0; c < 6; ++c) s += M[a * 6 + c] * rv[c];
       z[6 * i + a] = s;
-      p[6 * i + a] = s;
       rz += rv[a] * s;
-      bb += rv[a] * rv[a];
     }
   }
-  const double s1 = block_sum256(rz, lds4);
-  const double s2 = block_sum256(bb, lds4);
-  if (threadIdx.x == 0) { part[2 * blockIdx.x] = s1; part[2 * blockIdx.x + 1] = s2; }
-}
-
-__global__ void k_pcg_init_scalars(int32_t nparts, const double* __restrict__ part, double tol, int32_t max_iters,
-                                   PcgState* __restrict__ st) {
-  __shared__ double lds4[4];
-  double a = 0.0, b = 0.0;
-  for (int k = threadIdx.x; k < nparts; k += blockDim.x) { a += part[2 * k]; b += part[2 * k + 1]; }
-  a = block_sum256(a, lds4);
-  b = block_sum256(b, lds4);
+  rz = block_sum_big(rz, lds16);
+  bb = block_sum_big(bb, lds16);
   if (threadIdx.x == 0) {
-    st->rz = a;
-    st->bb = b;
-    st->rr = b;
-    st->tol2 = tol * tol * b;
+    st->rz = rz;
+    st->bb = bb;
+    st->rr = bb;
+    st->tol2 = tol * tol * bb;
+    st->beta = 0.0;
     st->iters = 0;
     st->max_iters = max_iters;
-    st->done = (b <= 0.0 || max_iters <= 0) ? 1 : 0;
+    st->done = (bb <= 0.0 || max_iters <= 0) ? 1 : 0;
   }
 }
 
-// q = S p (one block row per work-group) and per-row partial p'q.
+// p_k = z + beta p_{k-1} (rows formed on the fly, own row stored) and
+// q = S p_k, one block row per work-group; part[i] = p_k,i . q_i.
 __global__ __launch_bounds__(256) void k_pcg_spmv(int32_t Pvar, const int32_t* __restrict__ row_ptr,
                                                   const int32_t* __restrict__ col_var, const double* __restrict__ S,
-                                                  const double* __restrict__ p, double* __restrict__ q,
+                                                  const double* __restrict__ z, const double* __restrict__ p_prev,
+                                                  double* __restrict__ p_cur, double* __restrict__ q,
                                                   double* __restrict__ part, const PcgState* __restrict__ st) {
   __shared__ double red[4][6];
   if (st->done) return;
+  const double beta = st->beta;
   const int i = blockIdx.x;
   const int rb = row_ptr[i], re = row_ptr[i + 1];
   double y[6] = {0, 0, 0, 0, 0, 0};
   for (int k = rb + threadIdx.x; k < re; k += 256) {
     const double* blk = S + (size_t)36 * k;
-    const double* xv = p + 6 * col_var[k];
+    const int j = col_var[k];
     double xr[6];
-    for (int c = 0; c < 6; ++c) xr[c] = xv[c];
+    for (int c = 0; c < 6; ++c) xr[c] = z[6 * j + c] + beta * p_prev[6 * j + c];
     for (int a = 0; a < 6; ++a) {
       double s = 0.0;
       for (int c = 0; c < 6; ++c) s += blk[a * 6 + c] * xr[c];
@@ -770,27 +771,33 @@ __global__ __launch_bounds__(256) void k_pcg_spmv(int32_t Pvar, const int32_t* _
   if (threadIdx.x == 0) {
     double pq = 0.0;
     for (int a = 0; a < 6; ++a) {
-      const double s = (red[0][a] + red[1][a]) + (red[2][a] + red[3][a]);
-      q[6 * i + a] = s;
-      pq += p[6 * i + a] * s;
+      const double qa = (red[0][a] + red[1][a]) + (red[2][a] + red[3][a]);
+      const double pa = z[6 * i + a] + beta * p_prev[6 * i + a];
+      q[6 * i + a] = qa;
+      p_cur[6 * i + a] = pa;
+      pq += pa * qa;
     }
     part[i] = pq;
   }
 }
 
-// alpha = rz / p'q; x += alpha p; r -= alpha q; z = M^-1 r; partials r'z, r'r.
-__global__ __launch_bounds__(256) void k_pcg_update(int32_t Pvar, int32_t nparts, const double* __restrict__ part_pq,
-                                                    const double* __restrict__ Minv, const double* __restrict__ p,
-                                                    const double* __restrict__ q, double* __restrict__ x,
-                                                    double* __restrict__ r, double* __restrict__ z,
-                                                    double* __restrict__ part, const PcgState* __restrict__ st) {
-  __shared__ double lds4[4];
+// alpha = r'z / p'q; x += alpha p; r -= alpha q; z = M^-1 r; r'z, r'r;
+// beta and the convergence test for the next iteration (one block).
+__global__ __launch_bounds__(kPcgVecThreads) void k_pcg_update(int32_t Pvar, const double* __restrict__ part,
+                                                               const double* __restrict__ Minv,
+                                                               const double* __restrict__ p,
+                                                               const double* __restrict__ q, double* __restrict__ x,
+                                                               double* __restrict__ r, double* __restrict__ z,
+                                                               PcgState* __restrict__ st) {
+  __shared__ double lds16[kPcgVecThreads / 64];
   if (st->done) return;
-  const double pq = sum_partials(part_pq, nparts, lds4);
-  const double alpha = st->rz / pq;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double pq = 0.0;
+  for (int k = threadIdx.x; k < Pvar; k += kPcgVecThreads) pq += part[k];
+  pq = block_sum_big(pq, lds16);
+  const double rz_old = st->rz;
+  const double alpha = rz_old / pq;
   double rz = 0.0, rr = 0.0;
-  if (i < Pvar) {
+  for (int i = threadIdx.x; i < Pvar; i += kPcgVecThreads) {
     double rv[6];
     for (int a = 0; a < 6; ++a) {
       x[6 * i + a] += alpha * p[6 * i + a];
@@ -806,199 +813,15 @@ __global__ __launch_bounds__(256) void k_pcg_update(int32_t Pvar, int32_t nparts
       rz += rv[a] * s;
     }
   }
-  const double s1 = block_sum256(rz, lds4);
-  const double s2 = block_sum256(rr, lds4);
-  if (threadIdx.x == 0) { part[2 * blockIdx.x] = s1; part[2 * blockIdx.x + 1] = s2; }
-}
-
-// beta = rz_new / rz; p = z + beta p; convergence test on |r|.
-__global__ __launch_bounds__(256) void k_pcg_direction(int32_t Pvar, int32_t nparts, const double* __restrict__ part,
-                                                       const double* __restrict__ z, double* __restrict__ p,
-                                                       PcgState* __restrict__ st) {
-  __shared__ double lds4[4];
-  __shared__ double b_rz;
-  if (st->done) return;
-  double a = 0.0, b = 0.0;
-  for (int k = threadIdx.x; k < nparts; k += blockDim.x) { a += part[2 * k]; b += part[2 * k + 1]; }
-  a = block_sum256(a, lds4);
-  if (threadIdx.x == 0) b_rz = a;
+  rz = block_sum_big(rz, lds16);
+  rr = block_sum_big(rr, lds16);
   __syncthreads();
-  const double rz_new = b_rz;
-  const double beta = rz_new / st->rz;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < Pvar)
-    for (int k = 0; k < 6; ++k) p[6 * i + k] = z[6 * i + k] + beta * p[6 * i + k];
-}
-
-// Scalar bookkeeping after k_pcg_direction (one thread).
-__global__ void k_pcg_scalars(int32_t nparts, const double* __restrict__ part, PcgState* __restrict__ st) {
-  __shared__ double lds4[4];
-  if (st->done) return;
-  double a = 0.0, b = 0.0;
-  for (int k = threadIdx.x; k < nparts; k += blockDim.x) { a += part[2 * k]; b += part[2 * k + 1]; }
-  a = block_sum256(a, lds4);
-  b = block_sum256(b, lds4);
   if (threadIdx.x == 0) {
-    st->rz = a;
-    st->rr = b;
+    st->rz = rz;
+    st->rr = rr;
+    st->beta = rz / rz_old;
     st->iters += 1;
-    if (b <= st->tol2 || st->iters >= st->max_iters) st->done = 1;
-  }
-}
-
-// Whole PCG solve in one cooperative launch. Work-group g owns block rows
-// [g R, (g+1) R). Per iteration:
-//   phase 1: beta from the r'z partials; p_k = z + beta p_{k-1} for the owned
-//            rows (double-buffered: readers form neighbour entries on the
-//            fly from z and p_{k-1}); q = S p_k; partial p'q.   grid barrier
-//   phase 2: alpha = r'z / p'q; x += alpha p; r -= alpha q; z = M^-1 r;
-//            partials r'z and r'r.                             grid barrier
-// Every work-group sums the partials in the same order, so all take the same
-// exit decision (converged or max_iters) and pass the same barriers.
-__global__ __launch_bounds__(256) void k_pcg_persistent(
-    int32_t Pvar, int32_t rows_per_group, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col_var,
-    const double* __restrict__ S, const double* __restrict__ Minv, const double* __restrict__ b,
-    double* __restrict__ x, double* __restrict__ r, double* __restrict__ z, double* __restrict__ p0,
-    double* __restrict__ p1, double* __restrict__ q, double* __restrict__ part, double tol, int32_t max_iters,
-    PcgState* __restrict__ st) {
-  namespace cg = cooperative_groups;
-  cg::grid_group grid = cg::this_grid();
-  __shared__ double lds4[4];
-  __shared__ double red[4][6];
-  __shared__ double bsum[3];
-  const int G = gridDim.x;
-  const int g = blockIdx.x;
-  const int row0 = g * rows_per_group;
-  const int row1 = min(Pvar, row0 + rows_per_group);
-  double* pq_part = part;
-  double* rz_part = part + G;
-  double* rr_part = part + 2 * G;
-  // r = b, x = 0, z = M^-1 r, p_prev = 0.
-  double rz_l = 0.0, bb_l = 0.0;
-  for (int e = threadIdx.x; e < 6 * (row1 - row0); e += blockDim.x) {
-    const int i = row0 + e / 6, a = e % 6;
-    r[6 * i + a] = b[6 * i + a];
-    x[6 * i + a] = 0.0;
-    p0[6 * i + a] = 0.0;
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < 6 * (row1 - row0); e += blockDim.x) {
-    const int i = row0 + e / 6, a = e % 6;
-    const double* M = Minv + (size_t)36 * i + a * 6;
-    double s = 0.0;
-    for (int c = 0; c < 6; ++c) s += M[c] * r[6 * i + c];
-    z[6 * i + a] = s;
-    rz_l += r[6 * i + a] * s;
-    bb_l += r[6 * i + a] * r[6 * i + a];
-  }
-  {
-    const double s1 = block_sum256(rz_l, lds4);
-    const double s2 = block_sum256(bb_l, lds4);
-    if (threadIdx.x == 0) { rz_part[g] = s1; rr_part[g] = s2; }
-  }
-  grid.sync();
-  double bb, rz_prev = 1.0;
-  {
-    double a1 = 0.0, a2 = 0.0;
-    for (int k = threadIdx.x; k < G; k += blockDim.x) a2 += rr_part[k];
-    a2 = block_sum256(a2, lds4);
-    if (threadIdx.x == 0) bsum[0] = a2;
-    __syncthreads();
-    bb = bsum[0];
-    (void)a1;
-  }
-  const double tol2 = tol * tol * bb;
-  int it = 0;
-  bool done = !(bb > 0.0) || max_iters <= 0;
-  while (!done) {
-    double* p_prev = (it & 1) ? p1 : p0;
-    double* p_cur = (it & 1) ? p0 : p1;
-    // ---- phase 1
-    double rz_sum = 0.0;
-    for (int k = threadIdx.x; k < G; k += blockDim.x) rz_sum += rz_part[k];
-    rz_sum = block_sum256(rz_sum, lds4);
-    if (threadIdx.x == 0) bsum[1] = rz_sum;
-    __syncthreads();
-    const double rz = bsum[1];
-    const double beta = it == 0 ? 0.0 : rz / rz_prev;
-    rz_prev = rz;
-    double pq_l = 0.0;
-    for (int i = row0; i < row1; ++i) {
-      double y[6] = {0, 0, 0, 0, 0, 0};
-      for (int k = row_ptr[i] + threadIdx.x; k < row_ptr[i + 1]; k += blockDim.x) {
-        const double* blk = S + (size_t)36 * k;
-        const int j = col_var[k];
-        double xr[6];
-        for (int c = 0; c < 6; ++c) xr[c] = z[6 * j + c] + beta * p_prev[6 * j + c];
-        for (int a = 0; a < 6; ++a) {
-          double s = 0.0;
-          for (int c = 0; c < 6; ++c) s += blk[a * 6 + c] * xr[c];
-          y[a] += s;
-        }
-      }
-      for (int a = 0; a < 6; ++a) y[a] = wave_sum(y[a]);
-      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-      if (lane == 0) for (int a = 0; a < 6; ++a) red[wave][a] = y[a];
-      __syncthreads();
-      if (threadIdx.x < 6) {
-        const int a = threadIdx.x;
-        const double qa = (red[0][a] + red[1][a]) + (red[2][a] + red[3][a]);
-        const double pa = z[6 * i + a] + beta * p_prev[6 * i + a];
-        q[6 * i + a] = qa;
-        p_cur[6 * i + a] = pa;
-        pq_l = pa * qa;
-      }
-      __syncthreads();
-      // fold the 6 products of this row into thread 0's running sum
-      pq_l = wave_sum(threadIdx.x < 64 ? pq_l : 0.0);
-      if (threadIdx.x != 0) pq_l = 0.0;
-      if (threadIdx.x == 0) bsum[2] = (i == row0 ? 0.0 : bsum[2]) + pq_l;
-      __syncthreads();
-      pq_l = 0.0;
-    }
-    if (threadIdx.x == 0) pq_part[g] = row1 > row0 ? bsum[2] : 0.0;
-    grid.sync();
-    // ---- phase 2
-    double pq = 0.0;
-    for (int k = threadIdx.x; k < G; k += blockDim.x) pq += pq_part[k];
-    pq = block_sum256(pq, lds4);
-    if (threadIdx.x == 0) bsum[0] = pq;
-    __syncthreads();
-    const double alpha = rz / bsum[0];
-    double rz_l2 = 0.0, rr_l2 = 0.0;
-    for (int e = threadIdx.x; e < 6 * (row1 - row0); e += blockDim.x) {
-      const int i = row0 + e / 6, a = e % 6;
-      x[6 * i + a] += alpha * p_cur[6 * i + a];
-      r[6 * i + a] -= alpha * q[6 * i + a];
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < 6 * (row1 - row0); e += blockDim.x) {
-      const int i = row0 + e / 6, a = e % 6;
-      const double* M = Minv + (size_t)36 * i + a * 6;
-      double s = 0.0;
-      for (int c = 0; c < 6; ++c) s += M[c] * r[6 * i + c];
-      z[6 * i + a] = s;
-      rz_l2 += r[6 * i + a] * s;
-      rr_l2 += r[6 * i + a] * r[6 * i + a];
-    }
-    {
-      const double s1 = block_sum256(rz_l2, lds4);
-      const double s2 = block_sum256(rr_l2, lds4);
-      if (threadIdx.x == 0) { rz_part[g] = s1; rr_part[g] = s2; }
-    }
-    grid.sync();
-    ++it;
-    double rr = 0.0;
-    for (int k = threadIdx.x; k < G; k += blockDim.x) rr += rr_part[k];
-    rr = block_sum256(rr, lds4);
-    if (threadIdx.x == 0) bsum[0] = rr;
-    __syncthreads();
-    done = (bsum[0] <= tol2) || it >= max_iters;
-  }
-  if (g == 0 && threadIdx.x == 0) {
-    st->iters = it;
-    st->done = 1;
-    st->bb = bb;
+    if (rr <= st->tol2 || st->iters >= st->max_iters) st->done = 1;
   }
 }
 
@@ -1469,84 +1292,48 @@ void launch_block_inverse(hipStream_t s, int32_t Pvar, const int32_t* diag_pos, 
   if (Pvar > 0) k_block_inverse<<<(Pvar + 63) / 64, 64, 0, s>>>(Pvar, diag_pos, S, Minv, fail);
 }
 
-// PCG on S y = b. Vectors: x (solution), r, z, p, q (6 Pvar each); parts
-// sized for max(Pvar, 2 * blocks). Runs `max_iters` iterations at most,
-// polling the device flag every `poll` iterations.
-int launch_pcg(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_t* col_var, const int32_t* diag_pos,
-               const double* S, const double* Minv, const double* b, double* x, double* r, double* z, double* p,
-               double* q, double* parts_a, double* parts_b, void* state_dev, void* state_host, double tol,
-               int32_t max_iters, int32_t poll) {
+// PCG on S y = b (block-Jacobi). Vectors: x (solution), r, z, p0/p1 (search
+// directions, alternating), q (6 Pvar each); part (Pvar). Two launches per
+// iteration; the host polls the device flag every `poll` iterations.
+int launch_pcg(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_t* col_var, const double* S,
+               const double* Minv, const double* b, double* x, double* r, double* z, double* p0, double* p1,
+               double* q, double* part, void* state_dev, void* state_host, double tol, int32_t max_iters,
+               int32_t poll) {
   PcgState* st = static_cast<PcgState*>(state_dev);
   PcgState* hs = static_cast<PcgState*>(state_host);
-  const int vb = (Pvar + 255) / 256;
-  k_pcg_init<<<vb, 256, 0, s>>>(Pvar, b, Minv, x, r, z, p, parts_a);
-  k_pcg_init_scalars<<<1, 256, 0, s>>>(vb, parts_a, tol, max_iters, st);
+  k_pcg_init<<<1, kPcgVecThreads, 0, s>>>(Pvar, b, Minv, x, r, z, p0, tol, max_iters, st);
   int launched = 0;
   while (launched < max_iters) {
     const int n = std::min(poll, max_iters - launched);
     for (int k = 0; k < n; ++k) {
-      k_pcg_spmv<<<Pvar, 256, 0, s>>>(Pvar, row_ptr, col_var, S, p, q, parts_a, st);
-      k_pcg_update<<<vb, 256, 0, s>>>(Pvar, Pvar, parts_a, Minv, p, q, x, r, z, parts_b, st);
-      k_pcg_direction<<<vb, 256, 0, s>>>(Pvar, vb, parts_b, z, p, st);
-      k_pcg_scalars<<<1, 256, 0, s>>>(vb, parts_b, st);
+      double* p_prev = ((launched + k) & 1) ? p1 : p0;
+      double* p_cur = ((launched + k) & 1) ? p0 : p1;
+      k_pcg_spmv<<<Pvar, 256, 0, s>>>(Pvar, row_ptr, col_var, S, z, p_prev, p_cur, q, part, st);
+      k_pcg_update<<<1, kPcgVecThreads, 0, s>>>(Pvar, part, Minv, p_cur, q, x, r, z, st);
     }
     launched += n;
     if (hipMemcpyAsync(hs, st, sizeof(PcgState), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
     if (hipStreamSynchronize(s) != hipSuccess) return -1;
     if (hs->done) break;
   }
-  (void)diag_pos;
   return hs->iters;
 }
 size_t pcg_state_bytes() { return sizeof(PcgState); }
 
-// Cooperative (single-launch) PCG; returns the iteration count, or -1 when the
-// device cannot co-schedule the grid (the caller then uses launch_pcg).
-int launch_pcg_persistent(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_t* col_var, const double* S,
-                          const double* Minv, const double* b, double* x, double* r, double* z, double* p0,
-                          double* p1, double* q, double* part, void* state_dev, void* state_host, double tol,
-                          int32_t max_iters, int32_t max_groups) {
-  static int cached_groups = 0;
-  if (cached_groups == 0) {
-    int dev = 0, cus = 0, per_cu = 0, coop = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return -1;
-    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop) return -1;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg_persistent, 256, 0) != hipSuccess) return -1;
-    cached_groups = std::max(1, std::min(cus, cus * per_cu));
-  }
-  const int G = std::max(1, std::min({cached_groups, max_groups, Pvar}));
-  int32_t rows_per_group = (Pvar + G - 1) / G;
-  PcgState* st = static_cast<PcgState*>(state_dev);
-  void* args[] = {&Pvar, &rows_per_group, (void*)&row_ptr, (void*)&col_var, (void*)&S, (void*)&Minv, (void*)&b,
-                  &x, &r, &z, &p0, &p1, &q, &part, &tol, &max_iters, &st};
-  if (hipLaunchCooperativeKernel((const void*)k_pcg_persistent, dim3(G), dim3(256), args, 0, s) != hipSuccess)
-    return -1;
-  PcgState* hs = static_cast<PcgState*>(state_host);
-  if (hipMemcpyAsync(hs, st, sizeof(PcgState), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
-  if (hipStreamSynchronize(s) != hipSuccess) return -1;
-  return hs->iters;
-}
-
-// Pieces of one PCG iteration, for kernel timing: initialize the state, then
-// the matrix-vector product alone or the vector updates alone.
+// Pieces of one PCG iteration, for kernel timing.
 void launch_pcg_setup(hipStream_t s, int32_t Pvar, const double* b, const double* Minv, double* x, double* r, double* z,
-                      double* p, double* parts_a, void* state_dev) {
-  const int vb = (Pvar + 255) / 256;
-  k_pcg_init<<<vb, 256, 0, s>>>(Pvar, b, Minv, x, r, z, p, parts_a);
-  k_pcg_init_scalars<<<1, 256, 0, s>>>(vb, parts_a, 0.0, 1 << 30, static_cast<PcgState*>(state_dev));
+                      double* p, void* state_dev) {
+  k_pcg_init<<<1, kPcgVecThreads, 0, s>>>(Pvar, b, Minv, x, r, z, p, 0.0, 1 << 30, static_cast<PcgState*>(state_dev));
 }
 void launch_pcg_spmv(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_t* col_var, const double* S,
-                     const double* p, double* q, double* parts_a, const void* state_dev) {
-  k_pcg_spmv<<<Pvar, 256, 0, s>>>(Pvar, row_ptr, col_var, S, p, q, parts_a, static_cast<const PcgState*>(state_dev));
+                     const double* z, const double* p_prev, double* p_cur, double* q, double* part,
+                     const void* state_dev) {
+  k_pcg_spmv<<<Pvar, 256, 0, s>>>(Pvar, row_ptr, col_var, S, z, p_prev, p_cur, q, part,
+                                  static_cast<const PcgState*>(state_dev));
 }
-void launch_pcg_vectors(hipStream_t s, int32_t Pvar, const double* parts_a, const double* Minv, double* p,
-                        const double* q, double* x, double* r, double* z, double* parts_b, void* state_dev) {
-  const int vb = (Pvar + 255) / 256;
-  PcgState* st = static_cast<PcgState*>(state_dev);
-  k_pcg_update<<<vb, 256, 0, s>>>(Pvar, Pvar, parts_a, Minv, p, q, x, r, z, parts_b, st);
-  k_pcg_direction<<<vb, 256, 0, s>>>(Pvar, vb, parts_b, z, p, st);
-  k_pcg_scalars<<<1, 256, 0, s>>>(vb, parts_b, st);
+void launch_pcg_vectors(hipStream_t s, int32_t Pvar, const double* part, const double* Minv, const double* p,
+                        const double* q, double* x, double* r, double* z, void* state_dev) {
+  k_pcg_update<<<1, kPcgVecThreads, 0, s>>>(Pvar, part, Minv, p, q, x, r, z, static_cast<PcgState*>(state_dev));
 }
 
 void launch_back_substitute(hipStream_t s, const DevTopology& t, const double* vR, const double* lm,
