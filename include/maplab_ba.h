@@ -67,8 +67,8 @@ extern "C" {
 #define MLBA_LOSS_TRIVIAL 2      /* no loss */
 
 /* Linear solver for the LM step. All solve (J'J + D'D) y = J'r. */
-#define MLBA_SOLVER_SCHUR_PCG 0 /* landmark Schur elimination + block-Jacobi PCG
-                                   on the reduced camera system (device) */
+#define MLBA_SOLVER_SCHUR_PCG 0 /* landmark Schur elimination + deflated block-Jacobi
+                                   PCG on the reduced camera system (device) */
 
 /* ceres::TerminationType */
 #define MLBA_CONVERGENCE 0
@@ -147,8 +147,10 @@ typedef struct mlba_options {
   int32_t use_reprojection_error;         /* 0 */
   double max_reprojection_error_px;       /* 50 */
   /* device launch tuning (0 = library default / autotuned) */
-  int32_t obs_tile;                       /* observations per K1 tile */
-  int32_t schur_row_chunk;                /* S blocks per LDS accumulation chunk */
+  int32_t obs_tile;                       /* reserved */
+  int32_t schur_row_chunk;                /* reserved */
+  int32_t pcg_coarse_nodes;               /* deflation hat nodes: 0 automatic, -1 none, else K (<= 15) */
+  int32_t reserved1;
 } mlba_options;
 
 /* One row of the Ceres iteration log (ceres::IterationSummary subset). */
@@ -263,8 +265,8 @@ typedef struct mlba_kernel_times {
   double landmark_prep_ms;    /* damped landmark pivots */
   double schur_rows_ms;       /* reduced camera system build */
   double block_inverse_ms;    /* block-Jacobi preconditioner */
-  double pcg_spmv_ms;         /* one PCG matrix-vector product */
-  double pcg_vector_ms;       /* PCG vector updates of one iteration */
+  double pcg_spmv_ms;         /* one whole (deflated) PCG solve at the initial radius */
+  double pcg_vector_ms;       /* reserved (0) */
   double back_substitute_ms;  /* landmark step + model cost change */
   int64_t num_observations;
   int64_t num_landmarks;

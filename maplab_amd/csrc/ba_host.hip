@@ -44,18 +44,15 @@ int32_t schur_chunk_blocks();
 int32_t schur_batch_incidences();
 void launch_block_inverse(hipStream_t s, int32_t Pvar, const int32_t* diag_pos, const double* S, double* Minv,
                           int32_t* fail);
-int launch_pcg(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_t* col_var, const double* S,
-               const double* Minv, const double* b, double* x, double* r, double* z, double* p0, double* p1,
-               double* q, double* part, void* state_dev, void* state_host, double tol, int32_t max_iters,
-               int32_t poll);
 size_t pcg_state_bytes();
-void launch_pcg_setup(hipStream_t s, int32_t Pvar, const double* b, const double* Minv, double* x, double* r, double* z,
-                      double* p, void* state_dev);
-void launch_pcg_spmv(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_t* col_var, const double* S,
-                     const double* z, const double* p_prev, double* p_cur, double* q, double* part,
-                     const void* state_dev);
-void launch_pcg_vectors(hipStream_t s, int32_t Pvar, const double* part, const double* Minv, const double* p,
-                        const double* q, double* x, double* r, double* z, void* state_dev);
+int32_t pcg_max_coarse();
+int32_t pcg_update_groups(int32_t Pvar);
+int launch_pcg_deflated(hipStream_t s, int32_t Pvar, int32_t K, const int32_t* row_ptr, const int32_t* col_var,
+                        const double* S, const double* Minv, const double* b, const int32_t* hat_node,
+                        const double* hat_w, const int32_t* node_range, double* SR, double* E, double* Einv,
+                        double* x, double* r, double* z, double* p0, double* p1, double* q, double* mu,
+                        double* part, double* part2, void* state_dev, void* state_host, int32_t* fail, double tol,
+                        int32_t max_iters, int32_t poll);
 void launch_back_substitute(hipStream_t s, const DevTopology& t, const double* vR, const double* lm,
                             const double* obs_rec, const double* lm_acc, const double* lm_z, const double* w,
                             double* u, double* part);
@@ -140,6 +137,9 @@ struct mlba_handle {
   DBuf<int32_t> row_ptr, col_var, diag_pos, inc_ptr, inc, obs_col;
   DBuf<int4> schur_tasks;
   DBuf<int2> mirror;
+  DBuf<int32_t> hat_node, node_range;
+  DBuf<double> hat_w, defl_SR, defl_E, defl_Einv, defl_mu, pcg_part2;
+  int32_t hat_K = -1;  // coarse nodes of the current hat basis
   DBuf<double> S, rhs, pose_grad, pose_diag, pose_scale, w, Minv;
   DBuf<double> px, pr, pz, pp, pp2, pq, pcg_part, parts_a, parts_b, parts_c, parts_l, parts_v, parts_w, parts_g;
   DBuf<double> scal;
@@ -299,6 +299,7 @@ int build_topology(mlba_handle* h) {
     }
   }
   h->nnzb = (int32_t)col_var.size();
+  h->hat_K = -1;
   // S build tasks: (row, incidence batch, chunk of upper column blocks), and
   // the (upper, lower) block position pairs of the mirror pass.
   std::vector<int4> tasks;
@@ -405,6 +406,47 @@ void collect_events(mlba_handle* h) {
   h->ev_next = 0;
 }
 
+// Hat-function coarse basis over the variable-vertex order for the deflated
+// PCG: K nodes spread evenly; vertex i has weights (1 - t, t) on nodes
+// (n0, n0 + 1). K = 0 disables deflation.
+int ensure_coarse_basis(mlba_handle* h, int32_t requested) {
+  const int Pvar = h->Pvar;
+  int K = requested > 0 ? requested : (requested < 0 ? 0 : std::min(pcg_max_coarse() / 6, Pvar / 16));
+  K = std::min(K, pcg_max_coarse() / 6);
+  if (K < 2 || Pvar < 2 * K) K = 0;
+  if (K == h->hat_K && h->hat_node.p) return MLBA_OK;
+  std::vector<int32_t> node(std::max(Pvar, 1), 0), range(2 * std::max(K, 1), 0);
+  std::vector<double> wts(2 * (size_t)std::max(Pvar, 1), 0.0);
+  if (K > 0) {
+    for (int k = 0; k < K; ++k) { range[2 * k] = Pvar; range[2 * k + 1] = -1; }
+    for (int i = 0; i < Pvar; ++i) {
+      const double sp = (double)i * (K - 1) / (Pvar - 1);
+      const int n0 = std::min((int)sp, K - 2);
+      const double t = sp - n0;
+      node[i] = n0;
+      wts[2 * i] = 1.0 - t;
+      wts[2 * i + 1] = t;
+      for (int k : {n0, n0 + 1}) {
+        range[2 * k] = std::min(range[2 * k], i);
+        range[2 * k + 1] = std::max(range[2 * k + 1], i);
+      }
+    }
+  }
+  int rc;
+  if ((rc = upload(h, h->hat_node, node.data(), node.size())) || (rc = upload(h, h->hat_w, wts.data(), wts.size())) ||
+      (rc = upload(h, h->node_range, range.data(), range.size())))
+    return rc;
+  const size_t ne = 6 * (size_t)K;
+  const size_t groups = pcg_update_groups(std::max(Pvar, 1));
+  if (!h->defl_SR.alloc(6 * (size_t)std::max(Pvar, 1) * std::max<size_t>(ne, 1)) ||
+      !h->defl_E.alloc(std::max<size_t>(ne * ne, 1)) || !h->defl_Einv.alloc(std::max<size_t>(ne * ne, 1)) ||
+      !h->defl_mu.alloc(std::max<size_t>(ne, 1)) || !h->pcg_part2.alloc(groups * (2 + ne)))
+    return set_err(h, MLBA_ENOMEM, "device allocation failed");
+  h->hat_K = K;
+  HIP_OK(hipStreamSynchronize(h->stream));
+  return MLBA_OK;
+}
+
 DevLoss make_loss(const mlba_options* o) {
   DevLoss l;
   l.type = o ? o->loss_type : MLBA_LOSS_MAPLAB_HUBER;
@@ -458,6 +500,7 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
   const double t_start = now_s();
   double radius = initial_radius, decrease_factor = 2.0;
   int rc;
+  if ((rc = ensure_coarse_basis(h, o->pcg_coarse_nodes))) return rc;
   const double t_ev0 = now_s();
   if ((rc = linearize(h, o))) return rc;
   launch_jacobi_scaling(s, h->Pvar, h->L, o->use_jacobi_scaling, h->pose_diag.p, h->lm_acc.p, h->pose_scale.p,
@@ -482,7 +525,6 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
   if (o->max_num_iterations <= 0) { *last_radius = radius; goto done; }
   {
     int consecutive_invalid = 0, iteration = 0;
-    const int vb = (h->Pvar + 255) / 256;
     const int lb = (h->L + 255) / 256;
     const int pb = (h->P + 255) / 256;
     const int wb = (6 * h->Pvar + 255) / 256;
@@ -511,9 +553,11 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
       int pcg_iters = 0;
       if (h->Pvar > 0) {
         KernelTimer kt(h, MLBA_KERNEL_PCG);
-        pcg_iters = launch_pcg(s, h->Pvar, h->row_ptr.p, h->col_var.p, h->S.p, h->Minv.p, h->rhs.p, h->px.p,
-                               h->pr.p, h->pz.p, h->pp.p, h->pp2.p, h->pq.p, h->pcg_part.p, h->pcg_state.p,
-                               h->pcg_host, o->pcg_relative_tolerance, o->pcg_max_iterations, 16);
+        pcg_iters = launch_pcg_deflated(
+            s, h->Pvar, h->hat_K, h->row_ptr.p, h->col_var.p, h->S.p, h->Minv.p, h->rhs.p, h->hat_node.p,
+            h->hat_w.p, h->node_range.p, h->defl_SR.p, h->defl_E.p, h->defl_Einv.p, h->px.p, h->pr.p, h->pz.p,
+            h->pp.p, h->pp2.p, h->pq.p, h->defl_mu.p, h->pcg_part.p, h->pcg_part2.p, h->pcg_state.p, h->pcg_host,
+            h->fail.p, o->pcg_relative_tolerance, o->pcg_max_iterations, 16);
         if (pcg_iters < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
       }
       launch_unscale_poses(s, h->Pvar, h->px.p, h->pose_scale.p, h->w.p, h->parts_w.p);
@@ -1084,15 +1128,15 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
   });
   t->block_inverse_ms = timed([&] { launch_block_inverse(s, h->Pvar, h->diag_pos.p, h->S.p, h->Minv.p, h->fail.p); });
   if (h->Pvar > 0) {
-    launch_pcg_setup(s, h->Pvar, h->rhs.p, h->Minv.p, h->px.p, h->pr.p, h->pz.p, h->pp.p, h->pcg_state.p);
+    if ((rc = ensure_coarse_basis(h, o->pcg_coarse_nodes))) return rc;
     t->pcg_spmv_ms = timed([&] {
-      launch_pcg_spmv(s, h->Pvar, h->row_ptr.p, h->col_var.p, h->S.p, h->pz.p, h->pp.p, h->pp2.p, h->pq.p,
-                      h->pcg_part.p, h->pcg_state.p);
+      (void)launch_pcg_deflated(s, h->Pvar, h->hat_K, h->row_ptr.p, h->col_var.p, h->S.p, h->Minv.p, h->rhs.p,
+                                h->hat_node.p, h->hat_w.p, h->node_range.p, h->defl_SR.p, h->defl_E.p,
+                                h->defl_Einv.p, h->px.p, h->pr.p, h->pz.p, h->pp.p, h->pp2.p, h->pq.p, h->defl_mu.p,
+                                h->pcg_part.p, h->pcg_part2.p, h->pcg_state.p, h->pcg_host, h->fail.p,
+                                o->pcg_relative_tolerance, o->pcg_max_iterations, 16);
     });
-    t->pcg_vector_ms = timed([&] {
-      launch_pcg_vectors(s, h->Pvar, h->pcg_part.p, h->Minv.p, h->pp2.p, h->pq.p, h->px.p, h->pr.p, h->pz.p,
-                         h->pcg_state.p);
-    });
+    t->pcg_vector_ms = 0.0;
   }
   launch_unscale_poses(s, h->Pvar, h->px.p, h->pose_scale.p, h->w.p, h->parts_w.p);
   t->back_substitute_ms = timed([&] {

@@ -13,8 +13,7 @@
 //                     tasks accumulated in LDS, added with FP64 atomics;
 //   k_schur_mirror    lower triangle S_ji = S_ij^T
 //   k_block_inverse   block-Jacobi preconditioner (6x6 inverses of diag(S))
-//   k_pcg_*           preconditioned conjugate gradients on S (FP64), two
-//                     launches per iteration (row-parallel S p, one-block update)
+//   (ba_pcg.hip)      deflated block-Jacobi PCG on S
 //   k_back_substitute landmark step and the LM model cost change
 //   k_candidate       x (+) step for vertices (JPL Plus) and landmarks
 //   k_gradient_norm   max |x - Plus(x, -g)|
@@ -24,31 +23,11 @@
 // bits only).
 #include "ba_device.h"
 #include "ba_math.h"
+#include "ba_reduce.h"
 
 namespace mlba {
 
 // ------------------------------------------------------------ utilities
-
-__device__ inline double wave_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-__device__ inline double wave_max(double v) {
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  return v;
-}
-
-// Block-wide sum of one value for a 256-thread block; result valid in thread 0.
-__device__ inline double block_sum256(double v, double* lds4) {
-  v = wave_sum(v);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) lds4[wave] = v;
-  __syncthreads();
-  double s = 0.0;
-  if (threadIdx.x == 0) s = (lds4[0] + lds4[1]) + (lds4[2] + lds4[3]);
-  return s;
-}
 
 // Inverse of a symmetric positive definite 3x3 matrix given as
 // (xx xy xz yy yz zz); returns false when not positive definite.
@@ -674,157 +653,6 @@ __global__ void k_block_inverse(int32_t Pvar, const int32_t* __restrict__ diag_p
   }
 }
 
-// Scalars of one PCG solve, resident on the device.
-struct PcgState {
-  double rz;        // r'z of the current iterate
-  double bb;        // |b|^2
-  double rr;        // |r|^2
-  double tol2;      // (tol |b|)^2
-  double beta;      // direction update factor for the next iteration
-  int32_t iters;
-  int32_t done;
-  int32_t max_iters;
-  int32_t pad;
-};
-
-constexpr int kPcgVecThreads = 1024;
-
-// Block-wide sum for a kPcgVecThreads block (16 waves); valid in every thread.
-__device__ inline double block_sum_big(double v, double* lds16) {
-  v = wave_sum(v);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) lds16[wave] = v;
-  __syncthreads();
-  double s = 0.0;
-  for (int w = 0; w < kPcgVecThreads / 64; ++w) s += lds16[w];
-  return s;
-}
-
-// r = b, x = 0, z = M^-1 r, p = 0; r'z, b'b, tolerance (one block).
-__global__ __launch_bounds__(kPcgVecThreads) void k_pcg_init(int32_t Pvar, const double* __restrict__ b,
-                                                             const double* __restrict__ Minv, double* __restrict__ x,
-                                                             double* __restrict__ r, double* __restrict__ z,
-                                                             double* __restrict__ p, double tol, int32_t max_iters,
-                                                             PcgState* __restrict__ st) {
-  __shared__ double lds16[kPcgVecThreads / 64];
-  double rz = 0.0, bb = 0.0;
-  for (int i = threadIdx.x; i < Pvar; i += kPcgVecThreads) {
-    const double* M = Minv + (size_t)36 * i;
-    double rv[6];
-    for (int a = 0; a < 6; ++a) {
-      rv[a] = b[6 * i + a];
-      r[6 * i + a] = rv[a];
-      x[6 * i + a] = 0.0;
-      p[6 * i + a] = 0.0;
-      bb += rv[a] * rv[a];
-    }
-    for (int a = 0; a < 6; ++a) {
-      double s = 0.0;
-      for (int c = 0; c < 6; ++c) s += M[a * 6 + c] * rv[c];
-      z[6 * i + a] = s;
-      rz += rv[a] * s;
-    }
-  }
-  rz = block_sum_big(rz, lds16);
-  bb = block_sum_big(bb, lds16);
-  if (threadIdx.x == 0) {
-    st->rz = rz;
-    st->bb = bb;
-    st->rr = bb;
-    st->tol2 = tol * tol * bb;
-    st->beta = 0.0;
-    st->iters = 0;
-    st->max_iters = max_iters;
-    st->done = (bb <= 0.0 || max_iters <= 0) ? 1 : 0;
-  }
-}
-
-// p_k = z + beta p_{k-1} (rows formed on the fly, own row stored) and
-// q = S p_k, one block row per work-group; part[i] = p_k,i . q_i.
-__global__ __launch_bounds__(256) void k_pcg_spmv(int32_t Pvar, const int32_t* __restrict__ row_ptr,
-                                                  const int32_t* __restrict__ col_var, const double* __restrict__ S,
-                                                  const double* __restrict__ z, const double* __restrict__ p_prev,
-                                                  double* __restrict__ p_cur, double* __restrict__ q,
-                                                  double* __restrict__ part, const PcgState* __restrict__ st) {
-  __shared__ double red[4][6];
-  if (st->done) return;
-  const double beta = st->beta;
-  const int i = blockIdx.x;
-  const int rb = row_ptr[i], re = row_ptr[i + 1];
-  double y[6] = {0, 0, 0, 0, 0, 0};
-  for (int k = rb + threadIdx.x; k < re; k += 256) {
-    const double* blk = S + (size_t)36 * k;
-    const int j = col_var[k];
-    double xr[6];
-    for (int c = 0; c < 6; ++c) xr[c] = z[6 * j + c] + beta * p_prev[6 * j + c];
-    for (int a = 0; a < 6; ++a) {
-      double s = 0.0;
-      for (int c = 0; c < 6; ++c) s += blk[a * 6 + c] * xr[c];
-      y[a] += s;
-    }
-  }
-  for (int a = 0; a < 6; ++a) y[a] = wave_sum(y[a]);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) for (int a = 0; a < 6; ++a) red[wave][a] = y[a];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double pq = 0.0;
-    for (int a = 0; a < 6; ++a) {
-      const double qa = (red[0][a] + red[1][a]) + (red[2][a] + red[3][a]);
-      const double pa = z[6 * i + a] + beta * p_prev[6 * i + a];
-      q[6 * i + a] = qa;
-      p_cur[6 * i + a] = pa;
-      pq += pa * qa;
-    }
-    part[i] = pq;
-  }
-}
-
-// alpha = r'z / p'q; x += alpha p; r -= alpha q; z = M^-1 r; r'z, r'r;
-// beta and the convergence test for the next iteration (one block).
-__global__ __launch_bounds__(kPcgVecThreads) void k_pcg_update(int32_t Pvar, const double* __restrict__ part,
-                                                               const double* __restrict__ Minv,
-                                                               const double* __restrict__ p,
-                                                               const double* __restrict__ q, double* __restrict__ x,
-                                                               double* __restrict__ r, double* __restrict__ z,
-                                                               PcgState* __restrict__ st) {
-  __shared__ double lds16[kPcgVecThreads / 64];
-  if (st->done) return;
-  double pq = 0.0;
-  for (int k = threadIdx.x; k < Pvar; k += kPcgVecThreads) pq += part[k];
-  pq = block_sum_big(pq, lds16);
-  const double rz_old = st->rz;
-  const double alpha = rz_old / pq;
-  double rz = 0.0, rr = 0.0;
-  for (int i = threadIdx.x; i < Pvar; i += kPcgVecThreads) {
-    double rv[6];
-    for (int a = 0; a < 6; ++a) {
-      x[6 * i + a] += alpha * p[6 * i + a];
-      rv[a] = r[6 * i + a] - alpha * q[6 * i + a];
-      r[6 * i + a] = rv[a];
-      rr += rv[a] * rv[a];
-    }
-    const double* M = Minv + (size_t)36 * i;
-    for (int a = 0; a < 6; ++a) {
-      double s = 0.0;
-      for (int c = 0; c < 6; ++c) s += M[a * 6 + c] * rv[c];
-      z[6 * i + a] = s;
-      rz += rv[a] * s;
-    }
-  }
-  rz = block_sum_big(rz, lds16);
-  rr = block_sum_big(rr, lds16);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    st->rz = rz;
-    st->rr = rr;
-    st->beta = rz / rz_old;
-    st->iters += 1;
-    if (rr <= st->tol2 || st->iters >= st->max_iters) st->done = 1;
-  }
-}
-
 // ------------------------------------------------------------ step
 
 // Landmark step and model cost change. With y the solution of the scaled
@@ -1290,50 +1118,6 @@ int32_t schur_batch_incidences() { return kSchurBatch; }
 void launch_block_inverse(hipStream_t s, int32_t Pvar, const int32_t* diag_pos, const double* S, double* Minv,
                           int32_t* fail) {
   if (Pvar > 0) k_block_inverse<<<(Pvar + 63) / 64, 64, 0, s>>>(Pvar, diag_pos, S, Minv, fail);
-}
-
-// PCG on S y = b (block-Jacobi). Vectors: x (solution), r, z, p0/p1 (search
-// directions, alternating), q (6 Pvar each); part (Pvar). Two launches per
-// iteration; the host polls the device flag every `poll` iterations.
-int launch_pcg(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_t* col_var, const double* S,
-               const double* Minv, const double* b, double* x, double* r, double* z, double* p0, double* p1,
-               double* q, double* part, void* state_dev, void* state_host, double tol, int32_t max_iters,
-               int32_t poll) {
-  PcgState* st = static_cast<PcgState*>(state_dev);
-  PcgState* hs = static_cast<PcgState*>(state_host);
-  k_pcg_init<<<1, kPcgVecThreads, 0, s>>>(Pvar, b, Minv, x, r, z, p0, tol, max_iters, st);
-  int launched = 0;
-  while (launched < max_iters) {
-    const int n = std::min(poll, max_iters - launched);
-    for (int k = 0; k < n; ++k) {
-      double* p_prev = ((launched + k) & 1) ? p1 : p0;
-      double* p_cur = ((launched + k) & 1) ? p0 : p1;
-      k_pcg_spmv<<<Pvar, 256, 0, s>>>(Pvar, row_ptr, col_var, S, z, p_prev, p_cur, q, part, st);
-      k_pcg_update<<<1, kPcgVecThreads, 0, s>>>(Pvar, part, Minv, p_cur, q, x, r, z, st);
-    }
-    launched += n;
-    if (hipMemcpyAsync(hs, st, sizeof(PcgState), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
-    if (hipStreamSynchronize(s) != hipSuccess) return -1;
-    if (hs->done) break;
-  }
-  return hs->iters;
-}
-size_t pcg_state_bytes() { return sizeof(PcgState); }
-
-// Pieces of one PCG iteration, for kernel timing.
-void launch_pcg_setup(hipStream_t s, int32_t Pvar, const double* b, const double* Minv, double* x, double* r, double* z,
-                      double* p, void* state_dev) {
-  k_pcg_init<<<1, kPcgVecThreads, 0, s>>>(Pvar, b, Minv, x, r, z, p, 0.0, 1 << 30, static_cast<PcgState*>(state_dev));
-}
-void launch_pcg_spmv(hipStream_t s, int32_t Pvar, const int32_t* row_ptr, const int32_t* col_var, const double* S,
-                     const double* z, const double* p_prev, double* p_cur, double* q, double* part,
-                     const void* state_dev) {
-  k_pcg_spmv<<<Pvar, 256, 0, s>>>(Pvar, row_ptr, col_var, S, z, p_prev, p_cur, q, part,
-                                  static_cast<const PcgState*>(state_dev));
-}
-void launch_pcg_vectors(hipStream_t s, int32_t Pvar, const double* part, const double* Minv, const double* p,
-                        const double* q, double* x, double* r, double* z, void* state_dev) {
-  k_pcg_update<<<1, kPcgVecThreads, 0, s>>>(Pvar, part, Minv, p, q, x, r, z, static_cast<PcgState*>(state_dev));
 }
 
 void launch_back_substitute(hipStream_t s, const DevTopology& t, const double* vR, const double* lm,
