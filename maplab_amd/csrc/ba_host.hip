@@ -49,7 +49,8 @@ int32_t pcg_max_coarse();
 int32_t pcg_update_groups(int32_t Pvar);
 int launch_pcg_deflated(hipStream_t s, int32_t Pvar, int32_t K, const int32_t* row_ptr, const int32_t* col_var,
                         const double* S, const double* Minv, const double* b, const int32_t* hat_node,
-                        const double* hat_w, const int32_t* node_range, double* SR, double* E, double* Einv,
+                        const double* hat_w, const int32_t* node_range, const int32_t* sr_lo, int32_t sw,
+                        double* SR, double* E, double* Einv,
                         double* x, double* r, double* z, double* p0, double* p1, double* q, double* mu,
                         double* part, double* part2, void* state_dev, void* state_host, int32_t* fail, double tol,
                         int32_t max_iters, int32_t poll);
@@ -137,7 +138,9 @@ struct mlba_handle {
   DBuf<int32_t> row_ptr, col_var, diag_pos, inc_ptr, inc, obs_col;
   DBuf<int4> schur_tasks;
   DBuf<int2> mirror;
-  DBuf<int32_t> hat_node, node_range;
+  DBuf<int32_t> hat_node, node_range, sr_lo;
+  std::vector<int32_t> h_row_ptr, h_col_var;  // host copy of the S block structure
+  int32_t sr_width = 0;
   DBuf<double> hat_w, defl_SR, defl_E, defl_Einv, defl_mu, pcg_part2;
   int32_t hat_K = -1;  // coarse nodes of the current hat basis
   DBuf<double> S, rhs, pose_grad, pose_diag, pose_scale, w, Minv;
@@ -300,6 +303,8 @@ int build_topology(mlba_handle* h) {
   }
   h->nnzb = (int32_t)col_var.size();
   h->hat_K = -1;
+  h->h_row_ptr = row_ptr;
+  h->h_col_var = col_var;
   // S build tasks: (row, incidence batch, chunk of upper column blocks), and
   // the (upper, lower) block position pairs of the mirror pass.
   std::vector<int4> tasks;
@@ -432,13 +437,29 @@ int ensure_coarse_basis(mlba_handle* h, int32_t requested) {
       }
     }
   }
+  // Window of hat nodes that row i of S W touches: nodes of its block columns.
+  std::vector<int32_t> lo(std::max(Pvar, 1), 0);
+  int width = 1;
+  if (K > 0) {
+    for (int i = 0; i < Pvar; ++i) {
+      int a = K, b = -1;
+      for (int k = h->h_row_ptr[i]; k < h->h_row_ptr[i + 1]; ++k) {
+        const int n0 = node[h->h_col_var[k]];
+        a = std::min(a, n0);
+        b = std::max(b, std::min(n0 + 1, K - 1));
+      }
+      lo[i] = a;
+      width = std::max(width, b - a + 1);
+    }
+  }
+  h->sr_width = 6 * width;
   int rc;
   if ((rc = upload(h, h->hat_node, node.data(), node.size())) || (rc = upload(h, h->hat_w, wts.data(), wts.size())) ||
-      (rc = upload(h, h->node_range, range.data(), range.size())))
+      (rc = upload(h, h->node_range, range.data(), range.size())) || (rc = upload(h, h->sr_lo, lo.data(), lo.size())))
     return rc;
   const size_t ne = 6 * (size_t)K;
   const size_t groups = pcg_update_groups(std::max(Pvar, 1));
-  if (!h->defl_SR.alloc(6 * (size_t)std::max(Pvar, 1) * std::max<size_t>(ne, 1)) ||
+  if (!h->defl_SR.alloc(6 * (size_t)std::max(Pvar, 1) * (size_t)h->sr_width) ||
       !h->defl_E.alloc(std::max<size_t>(ne * ne, 1)) || !h->defl_Einv.alloc(std::max<size_t>(ne * ne, 1)) ||
       !h->defl_mu.alloc(std::max<size_t>(ne, 1)) || !h->pcg_part2.alloc(groups * (2 + ne)))
     return set_err(h, MLBA_ENOMEM, "device allocation failed");
@@ -555,7 +576,7 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
         KernelTimer kt(h, MLBA_KERNEL_PCG);
         pcg_iters = launch_pcg_deflated(
             s, h->Pvar, h->hat_K, h->row_ptr.p, h->col_var.p, h->S.p, h->Minv.p, h->rhs.p, h->hat_node.p,
-            h->hat_w.p, h->node_range.p, h->defl_SR.p, h->defl_E.p, h->defl_Einv.p, h->px.p, h->pr.p, h->pz.p,
+            h->hat_w.p, h->node_range.p, h->sr_lo.p, h->sr_width, h->defl_SR.p, h->defl_E.p, h->defl_Einv.p, h->px.p, h->pr.p, h->pz.p,
             h->pp.p, h->pp2.p, h->pq.p, h->defl_mu.p, h->pcg_part.p, h->pcg_part2.p, h->pcg_state.p, h->pcg_host,
             h->fail.p, o->pcg_relative_tolerance, o->pcg_max_iterations, 16);
         if (pcg_iters < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
@@ -1131,7 +1152,7 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
     if ((rc = ensure_coarse_basis(h, o->pcg_coarse_nodes))) return rc;
     t->pcg_spmv_ms = timed([&] {
       (void)launch_pcg_deflated(s, h->Pvar, h->hat_K, h->row_ptr.p, h->col_var.p, h->S.p, h->Minv.p, h->rhs.p,
-                                h->hat_node.p, h->hat_w.p, h->node_range.p, h->defl_SR.p, h->defl_E.p,
+                                h->hat_node.p, h->hat_w.p, h->node_range.p, h->sr_lo.p, h->sr_width, h->defl_SR.p, h->defl_E.p,
                                 h->defl_Einv.p, h->px.p, h->pr.p, h->pz.p, h->pp.p, h->pp2.p, h->pq.p, h->defl_mu.p,
                                 h->pcg_part.p, h->pcg_part2.p, h->pcg_state.p, h->pcg_host, h->fail.p,
                                 o->pcg_relative_tolerance, o->pcg_max_iterations, 16);

@@ -48,7 +48,9 @@ struct PcgArgs {
   const int32_t* hat_node;     // Pvar: first hat node of the vertex (0 without deflation)
   const double* hat_w;         // 2 Pvar: weights of hat_node and hat_node + 1 (0 without deflation)
   const int32_t* node_range;   // 2 K: first and last vertex of each hat's support
-  double* SR;                  // 6 Pvar x ne, row-major
+  const int32_t* sr_lo;        // Pvar: first hat node of the window of row i of SW
+  int32_t sw, pad2;            // window width of SW rows, in columns (6 x nodes)
+  double* SR;                  // 6 Pvar x sw: row e = 6 i + c holds columns [6 sr_lo[i], 6 sr_lo[i] + sw) of SW
   double* E;                   // ne x ne
   double* Einv;                // ne x ne
   double* x;
@@ -86,7 +88,17 @@ __global__ __launch_bounds__(256) void k_defl_sr(PcgArgs a) {
     if (w1 != 0.0) atomicAdd(&acc[row * ne + 6 * (n0 + 1) + c], w1 * v);
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < 6 * ne; k += blockDim.x) a.SR[(size_t)(6 * i) * ne + k] = acc[k];
+  const int c0 = 6 * a.sr_lo[i], sw = a.sw;
+  for (int k = threadIdx.x; k < 6 * sw; k += blockDim.x) {
+    const int row = k / sw, gc = c0 + k % sw;
+    a.SR[(size_t)(6 * i) * sw + k] = gc < ne ? acc[row * ne + gc] : 0.0;
+  }
+}
+
+// Entry (e, col) of SW from the windowed storage.
+__device__ inline double sw_entry(const PcgArgs& a, int e, int col) {
+  const int lc = col - 6 * a.sr_lo[e / 6];
+  return (lc >= 0 && lc < a.sw) ? a.SR[(size_t)e * a.sw + lc] : 0.0;
 }
 
 // E = W'(SW), one entry per thread.
@@ -97,7 +109,7 @@ __global__ void k_defl_e(PcgArgs a) {
   const int row = t / ne, col = t % ne, k = row / 6, comp = row % 6;
   double s = 0.0;
   for (int i = a.node_range[2 * k]; i <= a.node_range[2 * k + 1]; ++i)
-    s += hat_weight(a, i, k) * a.SR[(size_t)(6 * i + comp) * ne + col];
+    s += hat_weight(a, i, k) * sw_entry(a, 6 * i + comp, col);
   a.E[t] = s;
 }
 
@@ -143,7 +155,6 @@ __global__ __launch_bounds__(1024) void k_defl_inverse(PcgArgs a) {
 __global__ __launch_bounds__(1024) void k_defl_init(PcgArgs a, double tol, int32_t max_iters) {
   __shared__ double y[kMaxCoarse];
   __shared__ double cvec[kMaxCoarse];
-  __shared__ double tpart[1024];
   __shared__ double lds16[16];
   const int ne = a.ne, P = a.Pvar, tid = threadIdx.x;
   // c = W'b
@@ -168,8 +179,9 @@ __global__ __launch_bounds__(1024) void k_defl_init(PcgArgs a, double tol, int32
     for (int comp = 0; comp < 6; ++comp) {
       const double bv = a.b[6 * i + comp];
       double sy = 0.0;
-      const double* srow = a.SR + (size_t)(6 * i + comp) * ne;
-      for (int c = 0; c < ne; ++c) sy += srow[c] * y[c];
+      const double* srow = a.SR + (size_t)(6 * i + comp) * a.sw;
+      const int c0 = 6 * a.sr_lo[i];
+      for (int c = 0; c < a.sw && c0 + c < ne; ++c) sy += srow[c] * y[c0 + c];
       rv[comp] = bv - sy;
       a.r[6 * i + comp] = rv[comp];
       a.x[6 * i + comp] = ne ? w0 * y[6 * n0 + comp] + (w1 != 0.0 ? w1 * y[6 * (n0 + 1) + comp] : 0.0) : 0.0;
@@ -186,19 +198,16 @@ __global__ __launch_bounds__(1024) void k_defl_init(PcgArgs a, double tol, int32
     }
   }
   __syncthreads();
-  // t = (SW)'z0 in groups of columns, then mu0 = E^-1 t.
+  // t = (SW)'z0 with LDS atomics over the windowed rows, then mu0 = E^-1 t.
   if (ne > 0) {
-    const int groups = blockDim.x / ne;
-    const int col = tid % ne, g = tid / ne;
-    double s = 0.0;
-    if (g < groups)
-      for (int e = g; e < 6 * P; e += groups) s += a.SR[(size_t)e * ne + col] * a.z[e];
-    tpart[tid] = s;
+    for (int k = tid; k < ne; k += blockDim.x) cvec[k] = 0.0;
     __syncthreads();
-    if (tid < ne) {
-      double t = 0.0;
-      for (int gg = 0; gg < groups; ++gg) t += tpart[gg * ne + tid];
-      cvec[tid] = t;
+    const int sw = a.sw;
+    for (size_t el = tid; el < (size_t)6 * P * sw; el += blockDim.x) {
+      const int e = (int)(el / sw), lc = (int)(el % sw);
+      const int gc = 6 * a.sr_lo[e / 6] + lc;
+      const double v = a.SR[el];
+      if (gc < ne && v != 0.0) atomicAdd(&cvec[gc], v * a.z[e]);
     }
     __syncthreads();
     for (int rrow = tid; rrow < ne; rrow += blockDim.x) {
@@ -330,12 +339,21 @@ __global__ __launch_bounds__(256) void k_pcg_update(PcgArgs a, const double* __r
   const double rr_l = block_sum256(rv * rv, lds4);
   double* out = a.part2 + (size_t)blockIdx.x * (2 + ne);
   if (tid == 0) { out[0] = rz_l; out[1] = rr_l; }
-  for (int col = tid; col < ne; col += blockDim.x) {
-    double s = 0.0;
-    const int row0 = blockIdx.x * kRowsPerGroup;
-    for (int t = 0; t < kGroupThreads && row0 + t / 6 < P; ++t)
-      s += a.SR[(size_t)(6 * row0 + t) * ne + col] * zloc[t];
-    out[2 + col] = s;
+  if (ne > 0) {
+    // (SW)'z over the windowed rows of this group, all threads, LDS atomics.
+    for (int k = tid; k < ne; k += blockDim.x) cs[k] = 0.0;
+    __syncthreads();
+    const int row0 = blockIdx.x * kRowsPerGroup, sw = a.sw;
+    const int nent = 6 * min(kRowsPerGroup, P - row0);
+    const double* base = a.SR + (size_t)(6 * row0) * sw;
+    for (int el = tid; el < nent * sw; el += blockDim.x) {
+      const int t = el / sw, lc = el % sw;
+      const int gc = 6 * a.sr_lo[row0 + t / 6] + lc;
+      const double v = base[el];
+      if (gc < ne && v != 0.0) atomicAdd(&cs[gc], v * zloc[t]);
+    }
+    __syncthreads();
+    for (int col = tid; col < ne; col += blockDim.x) out[2 + col] = cs[col];
   }
   __threadfence();
   __syncthreads();
@@ -382,7 +400,8 @@ int32_t pcg_update_groups(int32_t Pvar) { return (Pvar + kRowsPerGroup - 1) / kR
 // E / Einv (ne x ne). Returns the iteration count or -1 on a launch error.
 int launch_pcg_deflated(hipStream_t s, int32_t Pvar, int32_t K, const int32_t* row_ptr, const int32_t* col_var,
                         const double* S, const double* Minv, const double* b, const int32_t* hat_node,
-                        const double* hat_w, const int32_t* node_range, double* SR, double* E, double* Einv,
+                        const double* hat_w, const int32_t* node_range, const int32_t* sr_lo, int32_t sw,
+                        double* SR, double* E, double* Einv,
                         double* x, double* r, double* z, double* p0, double* p1, double* q, double* mu,
                         double* part, double* part2, void* state_dev, void* state_host, int32_t* fail, double tol,
                         int32_t max_iters, int32_t poll) {
@@ -393,7 +412,7 @@ int launch_pcg_deflated(hipStream_t s, int32_t Pvar, int32_t K, const int32_t* r
   a.ne = 6 * K;
   a.pad = 0;
   a.row_ptr = row_ptr; a.col_var = col_var; a.S = S; a.Minv = Minv; a.b = b;
-  a.hat_node = hat_node; a.hat_w = hat_w; a.node_range = node_range;
+  a.hat_node = hat_node; a.hat_w = hat_w; a.node_range = node_range; a.sr_lo = sr_lo; a.sw = sw; a.pad2 = 0;
   a.SR = SR; a.E = E; a.Einv = Einv; a.x = x; a.r = r; a.z = z; a.p0 = p0; a.p1 = p1; a.q = q; a.mu = mu;
   a.part = part; a.part2 = part2;
   a.st = static_cast<PcgState*>(state_dev);
