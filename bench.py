@@ -24,66 +24,68 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
-FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 vector
 
 
 def kernel_table(sm, kt, n_pcg_total):
     """Per-kernel device time in the timed region and its roofline terms.
-    Algorithmic bytes per launch (FP64 observation records of 20 doubles):
-      linearize: N * 196 B (36 B read + 160 B record written) + L * 176 B
-      cost:      N * 36 B + L * 32 B
-      spmv:      nnzb * 292 B (6x6 FP64 block + column index) + Pvar * 104 B
-      schur:     flops = pairs * 288 (6x3.3x6 product + W' formation)."""
+    Algorithmic HBM bytes per launch (FP64; observation record = r, Jl, Jo =
+    20 doubles; incidence = an observation coupling a variable vertex, or
+    the anchor of a landmark with a variable anchor):
+      linearize:          N * 196 (36 read + 160 record written) + L * 176
+      cost:               N * 36 + L * 32
+      schur_incidences:   n_inc * 360 (160 record read, 144 Q + 48 rhs written,
+                          8 index) + L * 72 (pivot Z and g)
+      schur_blocks:       pairs * 8 (pair list) + n_inc * 336 (Q, Jl/Jo, rhs)
+                          + upper blocks * 584 (S_ij and S_ji written, index)
+      pcg_spmv:           nnzb * 292 (6x6 block + column index) + P * 6 * 8 * 4
+      pcg_update:         P * 6 * 8 * 9 + P * 288 (Minv) + P * 6 * sw * 8 (SW)
+    The Schur block kernel also does pairs * 216 FP64 flops (6x3 . 3x6),
+    below its byte time at MI355X's FP64 peak."""
     from maplab_amd import abi
     N, L, P = kt.num_observations, kt.num_landmarks, kt.num_variable_vertices
-    nnzb, pairs = kt.schur_blocks, kt.schur_pairs
+    nnzb, pairs, n_inc, nub = kt.schur_blocks, kt.schur_pairs, kt.incidences, kt.schur_upper_blocks
     rows = {}
 
-    def add(name, kid, bytes_per_launch=None, flops_per_launch=None):
-        launches = sm.kernel_launches[kid]
-        if launches == 0:
-            return
-        ms = sm.kernel_ms[kid] / launches
-        row = {"launches": launches, "total_ms": round(sm.kernel_ms[kid], 4), "avg_ms": round(ms, 5)}
+    def row_of(launches, total_ms, bytes_per_launch=None):
+        ms = total_ms / launches
+        row = {"launches": int(launches), "total_ms": round(total_ms, 4), "avg_ms": round(ms, 5)}
         if bytes_per_launch is not None:
             row.update(bound="hbm", bytes=int(bytes_per_launch), achieved_gbs=round(bytes_per_launch / ms / 1e6, 1),
                        frac=round(bytes_per_launch / ms / 1e6 / HBM_PEAK_GBS, 4))
-        if flops_per_launch is not None:
-            row.update(bound="fp64", flops=int(flops_per_launch),
-                       achieved_tflops=round(flops_per_launch / ms / 1e9, 3),
-                       frac=round(flops_per_launch / ms / 1e9 / FP64_PEAK_TFLOPS, 4))
-        rows[name] = row
+        return row
+
+    def add(name, kid, bytes_per_launch=None):
+        launches = sm.kernel_launches[kid]
+        if launches:
+            rows[name] = row_of(launches, sm.kernel_ms[kid], bytes_per_launch)
 
     add("k_linearize", abi.KERNEL_LINEARIZE, N * 196 + L * 176)
     add("k_linearize_cost", abi.KERNEL_COST, N * 36 + L * 32)
-    # Upper block triangle only: about half of the incidence pairs.
-    add("k_schur_tasks", abi.KERNEL_SCHUR_ROWS, flops_per_launch=pairs * 144)
+    add("k_schur_incidences", abi.KERNEL_SCHUR_INCIDENCES, n_inc * 360 + L * 72)
+    add("k_schur_blocks", abi.KERNEL_SCHUR_BLOCKS, pairs * 8 + n_inc * 336 + nub * 584)
+    if "k_schur_blocks" in rows:
+        r = rows["k_schur_blocks"]
+        r["flops"] = int(pairs * 216)
+        r["achieved_tflops"] = round(pairs * 216 / r["avg_ms"] / 1e9, 3)
     add("k_back_substitute", abi.KERNEL_BACK_SUBSTITUTE, N * 160 + L * 216)
     add("k_pose_rows", abi.KERNEL_POSE_ROWS)
     add("k_landmark_prep", abi.KERNEL_LANDMARK_PREP, L * (144 + 24 + 48))
     launches = sm.kernel_launches[abi.KERNEL_PCG]
     if launches and n_pcg_total > 0:
-        # The PCG solve of each LM step: two launches per PCG iteration
-        # (S p with the direction update, one-block vector update). Per
-        # iteration it streams S once and ~14 FP64 vectors of the system.
         it_ms = sm.kernel_ms[abi.KERNEL_PCG] / n_pcg_total
-        it_bytes = nnzb * 292 + P * 6 * 8 * 14
         rows["pcg_solve"] = {
-            "launches": launches, "total_ms": round(sm.kernel_ms[abi.KERNEL_PCG], 4),
+            "launches": int(launches), "total_ms": round(sm.kernel_ms[abi.KERNEL_PCG], 4),
             "avg_ms": round(sm.kernel_ms[abi.KERNEL_PCG] / launches, 5), "pcg_iterations": int(n_pcg_total),
-            "per_iteration_ms": round(it_ms, 5), "bytes_per_iteration": int(it_bytes),
-            "achieved_gbs": round(it_bytes / it_ms / 1e6, 1),
-            "note": "events around the whole solve: both kernels, launch gaps and the host convergence polls"}
+            "per_iteration_ms": round(it_ms, 5),
+            "note": "events around each whole solve: coarse-space set-up, both kernels of every iteration, "
+                    "launch gaps and the host convergence polls"}
     if kt.pcg_spmv_ms > 0 and n_pcg_total > 0:
-        spmv_bytes = nnzb * 292 + P * 6 * 8 * 4
-        rows["k_pcg_spmv"] = {
-            "launches": int(n_pcg_total), "avg_ms": round(kt.pcg_spmv_ms, 5),
-            "total_ms": round(kt.pcg_spmv_ms * n_pcg_total, 4), "bound": "hbm", "bytes": int(spmv_bytes),
-            "achieved_gbs": round(spmv_bytes / kt.pcg_spmv_ms / 1e6, 1),
-            "frac": round(spmv_bytes / kt.pcg_spmv_ms / 1e6 / HBM_PEAK_GBS, 4),
-            "timing": "avg from mlba_time_kernels events on the library stream, x PCG iterations of the timed region"}
-        rows["k_pcg_update"] = {"launches": int(n_pcg_total), "avg_ms": round(kt.pcg_vector_ms, 5),
-                                "total_ms": round(kt.pcg_vector_ms * n_pcg_total, 4)}
+        timing = "avg of mlba_time_kernels events around one launch on the library stream, x PCG iterations"
+        sw = 36  # upper bound of the coarse-space window width used in the byte count
+        for name, ms, b in (("k_pcg_spmv", kt.pcg_spmv_ms, nnzb * 292 + P * 6 * 8 * 4),
+                            ("k_pcg_update", kt.pcg_update_ms, P * 6 * 8 * 9 + P * 288 + P * 6 * sw * 8)):
+            rows[name] = row_of(n_pcg_total, ms * n_pcg_total, b)
+            rows[name]["timing"] = timing
     return rows
 
 
@@ -157,12 +159,8 @@ def main(argv=None):
     single = {k: v for k, v in table.items() if "bound" in v and "total_ms" in v}
     dom_name = max(single, key=lambda k: single[k]["total_ms"])
     dom = single[dom_name]
-    if dom["bound"] == "hbm":
-        roof = {"kernel": dom_name, "bound": "hbm", "achieved": dom["achieved_gbs"], "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": dom["frac"], "traffic": None}
-    else:
-        roof = {"kernel": dom_name, "bound": "fp64", "achieved": dom["achieved_tflops"], "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": dom["frac"], "traffic": None}
+    roof = {"kernel": dom_name, "bound": "hbm", "achieved": dom["achieved_gbs"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": dom["frac"], "traffic": None}
 
     result = {
         "metric": "BA LM iters/s + obs Jacobians/s at 1e-6 cost rel err; PCG iters learned vs Jacobi",
@@ -184,10 +182,12 @@ def main(argv=None):
                          f"2% outliers, first vertex + one landmark fixed"),
             "keyframes": vmap.num_vertices, "landmarks": vmap.num_landmarks,
             "observations": vmap.num_observations, "lm_iterations": n_iters, "seed": args.seed,
-            "linear_solver": "landmark Schur + block-Jacobi PCG (FP64, rel tol %g)" % opts.pcg_relative_tolerance,
+            "linear_solver": ("landmark Schur + deflated block-Jacobi PCG (FP64, rel tol %g)"
+                              % opts.pcg_relative_tolerance),
         },
         "obs_jacobians_per_s": round(obs_j_per_s, 1) if obs_j_per_s else None,
-        "pcg_iterations_block_jacobi": int(n_pcg),
+        "pcg_iterations": int(n_pcg),
+        "pcg_preconditioner": "block-Jacobi + piecewise-linear coarse-space deflation",
         "lm_accepted": int(sm.num_successful_steps), "lm_rejected": int(sm.num_unsuccessful_steps),
         "initial_cost": sm.initial_cost, "final_cost": sm.final_cost,
         "kernels": table,

@@ -195,11 +195,12 @@ typedef struct mlba_summary {
 /* Kernel ids of mlba_summary.kernel_ms. */
 #define MLBA_KERNEL_LINEARIZE 0       /* residual + Jacobian + landmark reductions */
 #define MLBA_KERNEL_COST 1            /* residual-only cost of the candidate */
-#define MLBA_KERNEL_SCHUR_ROWS 2      /* reduced camera system build */
+#define MLBA_KERNEL_SCHUR_BLOCKS 2    /* reduced camera system blocks and rhs */
 #define MLBA_KERNEL_PCG 3             /* whole PCG solve (all its launches) */
 #define MLBA_KERNEL_BACK_SUBSTITUTE 4 /* landmark step + model cost change */
 #define MLBA_KERNEL_POSE_ROWS 5       /* vertex gradient and diag(J'J) */
 #define MLBA_KERNEL_LANDMARK_PREP 6   /* damped landmark pivots */
+#define MLBA_KERNEL_SCHUR_INCIDENCES 7 /* Schur factors of every row incidence */
 
 typedef struct mlba_handle mlba_handle;
 
@@ -263,18 +264,22 @@ typedef struct mlba_kernel_times {
   double cost_ms;             /* residual-only cost (candidate evaluation) */
   double pose_rows_ms;        /* vertex gradient and diag(J'J) */
   double landmark_prep_ms;    /* damped landmark pivots */
-  double schur_rows_ms;       /* reduced camera system build */
+  double schur_incidences_ms; /* Schur factors of every row incidence */
+  double schur_blocks_ms;     /* reduced camera system blocks and rhs */
   double block_inverse_ms;    /* block-Jacobi preconditioner */
-  double pcg_spmv_ms;         /* one whole (deflated) PCG solve at the initial radius */
-  double pcg_vector_ms;       /* reserved (0) */
+  double pcg_solve_ms;        /* one whole (deflated) PCG solve at the initial radius */
+  double pcg_spmv_ms;         /* one PCG iteration's S p product */
+  double pcg_update_ms;       /* one PCG iteration's vector update and reductions */
   double back_substitute_ms;  /* landmark step + model cost change */
   int64_t num_observations;
   int64_t num_landmarks;
   int64_t num_variable_vertices;
   int64_t schur_blocks;       /* 6x6 blocks of S */
-  int64_t schur_pairs;        /* incidence pairs the S build visits */
-  int64_t incidences;         /* vertex-observation incidences */
+  int64_t schur_upper_blocks; /* blocks of the upper triangle (diagonal included) */
+  int64_t schur_pairs;        /* incidence pairs summed into the upper blocks */
+  int64_t incidences;         /* vertex-observation and anchor incidences */
   int64_t linearize_tiles;
+  int64_t pcg_iterations;     /* iterations of the timed PCG solve */
 } mlba_kernel_times;
 
 int32_t mlba_time_kernels(mlba_handle* handle, const mlba_options* options, int32_t repeats,

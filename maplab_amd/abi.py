@@ -90,8 +90,8 @@ class Summary(ctypes.Structure):
     ]
 
 
-KERNEL_LINEARIZE, KERNEL_COST, KERNEL_SCHUR_ROWS, KERNEL_PCG, KERNEL_BACK_SUBSTITUTE = 0, 1, 2, 3, 4
-KERNEL_POSE_ROWS, KERNEL_LANDMARK_PREP = 5, 6
+KERNEL_LINEARIZE, KERNEL_COST, KERNEL_SCHUR_BLOCKS, KERNEL_PCG, KERNEL_BACK_SUBSTITUTE = 0, 1, 2, 3, 4
+KERNEL_POSE_ROWS, KERNEL_LANDMARK_PREP, KERNEL_SCHUR_INCIDENCES = 5, 6, 7
 
 
 class SynthConfig(ctypes.Structure):
@@ -110,10 +110,11 @@ class SynthConfig(ctypes.Structure):
 
 class KernelTimes(ctypes.Structure):
     _fields_ = [(n, c_f64) for n in (
-        "linearize_ms", "cost_ms", "pose_rows_ms", "landmark_prep_ms", "schur_rows_ms", "block_inverse_ms",
-        "pcg_spmv_ms", "pcg_vector_ms", "back_substitute_ms")] + [(n, ctypes.c_int64) for n in (
-        "num_observations", "num_landmarks", "num_variable_vertices", "schur_blocks", "schur_pairs", "incidences",
-        "linearize_tiles")]
+        "linearize_ms", "cost_ms", "pose_rows_ms", "landmark_prep_ms", "schur_incidences_ms", "schur_blocks_ms",
+        "block_inverse_ms", "pcg_solve_ms", "pcg_spmv_ms", "pcg_update_ms", "back_substitute_ms")] + [
+        (n, ctypes.c_int64) for n in (
+            "num_observations", "num_landmarks", "num_variable_vertices", "schur_blocks", "schur_upper_blocks",
+            "schur_pairs", "incidences", "linearize_tiles", "pcg_iterations")]
 
 
 def _declare(lib, name, restype, argtypes):

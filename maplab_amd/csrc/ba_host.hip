@@ -36,12 +36,18 @@ void launch_jacobi_scaling(hipStream_t s, int32_t Pvar, int32_t L, int32_t use, 
                            const double* lm_acc, double* pose_scale, double* lm_scale);
 void launch_landmark_prep(hipStream_t s, const DevTopology& t, const double* lm_acc, const double* lm_scale,
                           double inv_mu, double min_diag, double max_diag, double* lm_z, int32_t* fail);
-void launch_schur_build(hipStream_t s, const DevTopology& t, int32_t ntasks, const int4* tasks, int32_t nmirror,
-                        const int2* mirror, int64_t nnzb, const double* vR, const double* lm, const double* obs_rec,
-                        const double* lm_acc, const double* lm_z, const double* pose_scale, const double* pose_diag,
-                        double inv_mu, double min_diag, double max_diag, double* S, double* rhs);
-int32_t schur_chunk_blocks();
-int32_t schur_batch_incidences();
+void launch_schur_incidences(hipStream_t s, const DevTopology& t, int32_t n_inc, const int32_t* inc_row,
+                             const double* vR, const double* lm, const double* obs_rec, const double* lm_acc,
+                             const double* lm_z, const double* pose_scale, double* inc_q, double* inc_rhs,
+                             double* lm_wa);
+void launch_schur_blocks(hipStream_t s, const DevTopology& t, int32_t nub, const int4* ublk, int32_t n_items,
+                         const int4* items, const int32_t* item_ptr, const uint2* pairs, const double* obs_rec,
+                         const double* inc_q, const double* inc_rhs, const double* lm_wa, const double* pose_scale,
+                         const double* pose_diag, double inv_mu, double min_diag, double max_diag, double* partial,
+                         double* S, double* rhs);
+uint32_t schur_pair_anchor_bit();
+uint32_t schur_pair_same_bit();
+int32_t schur_item_pairs();
 void launch_block_inverse(hipStream_t s, int32_t Pvar, const int32_t* diag_pos, const double* S, double* Minv,
                           int32_t* fail);
 size_t pcg_state_bytes();
@@ -49,11 +55,12 @@ int32_t pcg_max_coarse();
 int32_t pcg_update_groups(int32_t Pvar);
 int launch_pcg_deflated(hipStream_t s, int32_t Pvar, int32_t K, const int32_t* row_ptr, const int32_t* col_var,
                         const double* S, const double* Minv, const double* b, const int32_t* hat_node,
-                        const double* hat_w, const int32_t* node_range, const int32_t* sr_lo, int32_t sw,
-                        double* SR, double* E, double* Einv,
-                        double* x, double* r, double* z, double* p0, double* p1, double* q, double* mu,
-                        double* part, double* part2, void* state_dev, void* state_host, int32_t* fail, double tol,
-                        int32_t max_iters, int32_t poll);
+                        const double* hat_w, const int32_t* node_range, const int32_t* grp_lo, int32_t gw,
+                        double* SR, double* E, double* Einv, double* y,
+                        double* x, double* r, double* z, double* p0, double* p1, double* q,
+                        double* part, double* part2, void* state_dev, void* state_host, double tol,
+                        int32_t max_iters, int32_t poll, const hipEvent_t* probe);
+int32_t pcg_group_rows();
 void launch_back_substitute(hipStream_t s, const DevTopology& t, const double* vR, const double* lm,
                             const double* obs_rec, const double* lm_acc, const double* lm_z, const double* w,
                             double* u, double* part);
@@ -122,7 +129,7 @@ struct mlba_handle {
   // Derived per topology build.
   int32_t Pvar = 0, tiles = 0, nnzb = 0, n_inc = 0, n_split = 0, max_row_blocks = 0;
   int64_t schur_pairs = 0;
-  int32_t n_tasks = 0, n_mirror = 0;
+  int32_t n_upper = 0, n_items = 0;     // upper-triangle blocks of S; Schur work items
   // Device memory.
   DBuf<double> obs_kp, obs_sinv, obs_rec;
   DBuf<int32_t> obs_vertex, obs_meta, obs_landmark, perm_d;
@@ -136,12 +143,15 @@ struct mlba_handle {
   DBuf<int32_t> tile_begin;
   DBuf<double> tile_cost, tile_cost2;
   DBuf<int32_t> row_ptr, col_var, diag_pos, inc_ptr, inc, obs_col;
-  DBuf<int4> schur_tasks;
-  DBuf<int2> mirror;
-  DBuf<int32_t> hat_node, node_range, sr_lo;
+  DBuf<int32_t> inc_row, item_ptr;      // row of each incidence; work-item range of each upper block
+  DBuf<int4> ublk, items;               // upper block (position of S_ij, of S_ji, i, j); item (block, pairs)
+  DBuf<uint2> pairs;                    // incidence pairs (x in row i, y in row j) of each upper block
+  DBuf<double> inc_q, inc_rhs, lm_wa;   // Schur factors per incidence / anchored landmark
+  DBuf<double> schur_partial;           // 36 per work item
+  DBuf<int32_t> hat_node, node_range, grp_lo;
   std::vector<int32_t> h_row_ptr, h_col_var;  // host copy of the S block structure
-  int32_t sr_width = 0;
-  DBuf<double> hat_w, defl_SR, defl_E, defl_Einv, defl_mu, pcg_part2;
+  int32_t grp_width = 0;                // columns of the per-group SW window
+  DBuf<double> hat_w, defl_SR, defl_E, defl_Einv, defl_y, pcg_part2;
   int32_t hat_K = -1;  // coarse nodes of the current hat basis
   DBuf<double> S, rhs, pose_grad, pose_diag, pose_scale, w, Minv;
   DBuf<double> px, pr, pz, pp, pp2, pq, pcg_part, parts_a, parts_b, parts_c, parts_l, parts_v, parts_w, parts_g;
@@ -275,7 +285,6 @@ int build_topology(mlba_handle* h) {
   {
     std::vector<int32_t> mark(Pvar, -1), cols;
     h->max_row_blocks = 0;
-    h->schur_pairs = 0;
     for (int i = 0; i < Pvar; ++i) {
       cols.clear();
       auto touch = [&](int j) {
@@ -286,7 +295,6 @@ int build_topology(mlba_handle* h) {
       for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) {
         const int x = inc[k];
         const int l = x >= 0 ? h->s_landmark[x] : -1 - x;
-        h->schur_pairs += h->lm_begin[l + 1] - h->lm_begin[l] + 1;
         if (l == last_l) continue;
         last_l = l;
         for (int o = h->lm_begin[l]; o < h->lm_begin[l + 1]; ++o)
@@ -305,27 +313,87 @@ int build_topology(mlba_handle* h) {
   h->hat_K = -1;
   h->h_row_ptr = row_ptr;
   h->h_col_var = col_var;
-  // S build tasks: (row, incidence batch, chunk of upper column blocks), and
-  // the (upper, lower) block position pairs of the mirror pass.
-  std::vector<int4> tasks;
-  std::vector<int2> mirror;
+  // Upper blocks of S and, for each, the incidence pairs (x in row i, y in
+  // row j >= i, same landmark) whose products sum into it, ordered by x.
+  std::vector<int32_t> inc_row(inc.size());
+  for (int i = 0; i < Pvar; ++i)
+    for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) inc_row[k] = i;
+  auto inc_landmark = [&](int32_t x) { return x >= 0 ? h->s_landmark[x] : -1 - x; };
+  std::vector<int32_t> lm_inc_ptr(L + 1, 0), lm_inc(inc.size());
+  for (size_t k = 0; k < inc.size(); ++k) lm_inc_ptr[inc_landmark(inc[k]) + 1]++;
+  for (int l = 0; l < L; ++l) lm_inc_ptr[l + 1] += lm_inc_ptr[l];
   {
-    const int chunk = schur_chunk_blocks(), batch = schur_batch_incidences();
+    std::vector<int32_t> fill(lm_inc_ptr.begin(), lm_inc_ptr.end() - 1);
+    for (size_t k = 0; k < inc.size(); ++k) lm_inc[fill[inc_landmark(inc[k])]++] = (int32_t)k;
+  }
+  std::vector<int4> ublk;
+  std::vector<int32_t> pair_ptr{0};
+  std::vector<uint2> pairs;
+  {
+    const uint32_t abit = schur_pair_anchor_bit(), sbit = schur_pair_same_bit();
+    if (h->N >= (int)sbit || L >= (int)sbit || (int64_t)inc.size() >= (int64_t)sbit)
+      return set_err(h, MLBA_EINVAL, "problem too large for the Schur pair encoding");
+    std::vector<int32_t> upos(Pvar, -1), cnt;
+    std::vector<int64_t> fill;
+    int64_t total = 0;
     for (int i = 0; i < Pvar; ++i) {
-      for (int c0 = diag_pos[i]; c0 < row_ptr[i + 1]; c0 += chunk)
-        for (int x0 = inc_ptr[i]; x0 < inc_ptr[i + 1]; x0 += batch)
-          tasks.push_back(make_int4(i, x0, std::min(x0 + batch, inc_ptr[i + 1]), c0));
-      for (int k = diag_pos[i] + 1; k < row_ptr[i + 1]; ++k) {
+      const int base = (int)ublk.size();
+      for (int k = diag_pos[i]; k < row_ptr[i + 1]; ++k) {
         const int j = col_var[k];
-        const int32_t* b = col_var.data() + row_ptr[j];
-        const int32_t* e = col_var.data() + row_ptr[j + 1];
-        const int32_t* it = std::lower_bound(b, e, i);
-        mirror.push_back(make_int2(k, (int32_t)(it - col_var.data())));
+        int kt = k;
+        if (j != i) {
+          const int32_t* b = col_var.data() + row_ptr[j];
+          const int32_t* e = col_var.data() + row_ptr[j + 1];
+          kt = (int32_t)(std::lower_bound(b, e, i) - col_var.data());
+        }
+        upos[j] = (int32_t)ublk.size();
+        ublk.push_back(make_int4(k, kt, i, j));
+      }
+      const int nb = (int)ublk.size() - base;
+      cnt.assign(nb, 0);
+      for (int x = inc_ptr[i]; x < inc_ptr[i + 1]; ++x) {
+        const int l = inc_landmark(inc[x]);
+        for (int m = lm_inc_ptr[l]; m < lm_inc_ptr[l + 1]; ++m) {
+          const int j = inc_row[lm_inc[m]];
+          if (j < i) continue;
+          if (upos[j] < base) return set_err(h, MLBA_EINVAL, "internal: S block structure misses a landmark pair");
+          cnt[upos[j] - base]++;
+        }
+      }
+      fill.assign(nb, 0);
+      for (int b = 0; b < nb; ++b) {
+        fill[b] = total;
+        total += cnt[b];
+        if (total > INT32_MAX) return set_err(h, MLBA_ENOMEM, "reduced camera system has too many landmark pairs");
+        pair_ptr.push_back((int32_t)total);
+      }
+      pairs.resize((size_t)total);
+      for (int x = inc_ptr[i]; x < inc_ptr[i + 1]; ++x) {
+        const int l = inc_landmark(inc[x]);
+        const uint32_t xe = (uint32_t)x | (inc[x] < 0 ? abit : 0u);
+        for (int m = lm_inc_ptr[l]; m < lm_inc_ptr[l + 1]; ++m) {
+          const int y = lm_inc[m];
+          if (inc_row[y] < i) continue;
+          const uint32_t ye = inc[y] < 0 ? ((uint32_t)l | abit) : ((uint32_t)inc[y] | (y == x ? sbit : 0u));
+          pairs[(size_t)fill[upos[inc_row[y]] - base]++] = make_uint2(xe, ye);
+        }
       }
     }
   }
-  h->n_tasks = (int32_t)tasks.size();
-  h->n_mirror = (int32_t)mirror.size();
+  h->n_upper = (int32_t)ublk.size();
+  h->schur_pairs = (int64_t)pairs.size();
+  // Work items: each upper block's pairs in ranges of at most schur_item_pairs().
+  std::vector<int4> items;
+  std::vector<int32_t> item_ptr{0};
+  {
+    const int ip = schur_item_pairs();
+    for (int u = 0; u < h->n_upper; ++u) {
+      for (int pb = pair_ptr[u]; pb < pair_ptr[u + 1]; pb += ip)
+        items.push_back(make_int4(u, pb, std::min(pb + ip, pair_ptr[u + 1]), 0));
+      item_ptr.push_back((int32_t)items.size());
+    }
+  }
+  h->n_items = (int32_t)items.size();
   std::vector<int32_t> obs_col(N, -1);
   for (int o = 0; o < N; ++o)
     if ((meta[o] & 4) && h->lm_active[h->s_landmark[o]]) obs_col[o] = vertex_var[h->s_vertex[o]];
@@ -341,12 +409,17 @@ int build_topology(mlba_handle* h) {
   if ((rc = upload(h, h->row_ptr, row_ptr.data(), row_ptr.size()))) return rc;
   if ((rc = upload(h, h->col_var, col_var.data(), col_var.size()))) return rc;
   if ((rc = upload(h, h->diag_pos, diag_pos.data(), diag_pos.size()))) return rc;
-  if ((rc = upload(h, h->schur_tasks, tasks.data(), tasks.size()))) return rc;
-  if ((rc = upload(h, h->mirror, mirror.data(), mirror.size()))) return rc;
+  if ((rc = upload(h, h->inc_row, inc_row.data(), inc_row.size()))) return rc;
+  if ((rc = upload(h, h->ublk, ublk.data(), ublk.size()))) return rc;
+  if ((rc = upload(h, h->items, items.data(), items.size()))) return rc;
+  if ((rc = upload(h, h->item_ptr, item_ptr.data(), item_ptr.size()))) return rc;
+  if ((rc = upload(h, h->pairs, pairs.data(), pairs.size()))) return rc;
   if ((rc = upload(h, h->obs_col, obs_col.data(), obs_col.size()))) return rc;
   const size_t T = h->tiles, V6 = 6 * (size_t)std::max(Pvar, 1);
   const size_t vb = (Pvar + 255) / 256 + 1;
   bool ok = h->tile_cost.alloc(2 * T) && h->tile_cost2.alloc(2 * T) && h->S.alloc(36 * (size_t)h->nnzb) &&
+            h->inc_q.alloc(18 * std::max<size_t>(inc.size(), 1)) && h->inc_rhs.alloc(6 * std::max<size_t>(inc.size(), 1)) &&
+            h->lm_wa.alloc(36 * (size_t)std::max(L, 1)) && h->schur_partial.alloc(36 * std::max<size_t>(items.size(), 1)) &&
             h->rhs.alloc(V6) && h->pose_grad.alloc(V6) && h->pose_diag.alloc(V6) && h->pose_scale.alloc(V6) &&
             h->w.alloc(V6) && h->Minv.alloc(36 * (size_t)std::max(Pvar, 1)) && h->px.alloc(V6) &&
             h->pr.alloc(V6) && h->pz.alloc(V6) && h->pp.alloc(V6) && h->pp2.alloc(V6) && h->pq.alloc(V6) &&
@@ -437,32 +510,39 @@ int ensure_coarse_basis(mlba_handle* h, int32_t requested) {
       }
     }
   }
-  // Window of hat nodes that row i of S W touches: nodes of its block columns.
-  std::vector<int32_t> lo(std::max(Pvar, 1), 0);
+  // Window of hat nodes that the rows of each PCG row group touch in S W:
+  // nodes of their block columns.
+  const int gr = pcg_group_rows();
+  const int ngroups = (std::max(Pvar, 1) + gr - 1) / gr;
+  std::vector<int32_t> lo(ngroups, 0);
   int width = 1;
   if (K > 0) {
-    for (int i = 0; i < Pvar; ++i) {
+    for (int g = 0; g < ngroups; ++g) {
       int a = K, b = -1;
-      for (int k = h->h_row_ptr[i]; k < h->h_row_ptr[i + 1]; ++k) {
-        const int n0 = node[h->h_col_var[k]];
-        a = std::min(a, n0);
-        b = std::max(b, std::min(n0 + 1, K - 1));
-      }
-      lo[i] = a;
+      for (int i = g * gr; i < std::min(Pvar, (g + 1) * gr); ++i)
+        for (int k = h->h_row_ptr[i]; k < h->h_row_ptr[i + 1]; ++k) {
+          const int n0 = node[h->h_col_var[k]];
+          a = std::min(a, n0);
+          b = std::max(b, std::min(n0 + 1, K - 1));
+        }
+      if (b < a) a = b = 0;
+      lo[g] = a;
       width = std::max(width, b - a + 1);
     }
   }
-  h->sr_width = 6 * width;
+  h->grp_width = 6 * width;
   int rc;
   if ((rc = upload(h, h->hat_node, node.data(), node.size())) || (rc = upload(h, h->hat_w, wts.data(), wts.size())) ||
-      (rc = upload(h, h->node_range, range.data(), range.size())) || (rc = upload(h, h->sr_lo, lo.data(), lo.size())))
+      (rc = upload(h, h->node_range, range.data(), range.size())) || (rc = upload(h, h->grp_lo, lo.data(), lo.size())))
     return rc;
   const size_t ne = 6 * (size_t)K;
   const size_t groups = pcg_update_groups(std::max(Pvar, 1));
-  if (!h->defl_SR.alloc(6 * (size_t)std::max(Pvar, 1) * (size_t)h->sr_width) ||
+  if (!h->defl_SR.alloc(groups * (size_t)h->grp_width * (size_t)(6 * gr)) ||
       !h->defl_E.alloc(std::max<size_t>(ne * ne, 1)) || !h->defl_Einv.alloc(std::max<size_t>(ne * ne, 1)) ||
-      !h->defl_mu.alloc(std::max<size_t>(ne, 1)) || !h->pcg_part2.alloc(groups * (2 + ne)))
+      !h->defl_y.alloc(std::max<size_t>(ne, 1)) || !h->pcg_part2.alloc(groups * (3 + ne)))
     return set_err(h, MLBA_ENOMEM, "device allocation failed");
+  // Entries of the last group past the last row stay zero.
+  HIP_OK(hipMemsetAsync(h->defl_SR.p, 0, h->defl_SR.n * sizeof(double), h->stream));
   h->hat_K = K;
   HIP_OK(hipStreamSynchronize(h->stream));
   return MLBA_OK;
@@ -564,10 +644,15 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
                              o->max_lm_diagonal, h->lm_z.p, h->fail.p);
       }
       {
-        KernelTimer kt(h, MLBA_KERNEL_SCHUR_ROWS);
-        launch_schur_build(s, h->topo, h->n_tasks, h->schur_tasks.p, h->n_mirror, h->mirror.p, h->nnzb, h->vR.p,
-                           h->lm.p, h->obs_rec.p, h->lm_acc.p, h->lm_z.p, h->pose_scale.p, h->pose_diag.p, inv_mu,
-                           o->min_lm_diagonal, o->max_lm_diagonal, h->S.p, h->rhs.p);
+        KernelTimer kt(h, MLBA_KERNEL_SCHUR_INCIDENCES);
+        launch_schur_incidences(s, h->topo, h->n_inc, h->inc_row.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p,
+                                h->lm_z.p, h->pose_scale.p, h->inc_q.p, h->inc_rhs.p, h->lm_wa.p);
+      }
+      {
+        KernelTimer kt(h, MLBA_KERNEL_SCHUR_BLOCKS);
+        launch_schur_blocks(s, h->topo, h->n_upper, h->ublk.p, h->n_items, h->items.p, h->item_ptr.p, h->pairs.p,
+                            h->obs_rec.p, h->inc_q.p, h->inc_rhs.p, h->lm_wa.p, h->pose_scale.p, h->pose_diag.p,
+                            inv_mu, o->min_lm_diagonal, o->max_lm_diagonal, h->schur_partial.p, h->S.p, h->rhs.p);
       }
       launch_block_inverse(s, h->Pvar, h->diag_pos.p, h->S.p, h->Minv.p, h->fail.p);
       HIP_OK(hipGetLastError());
@@ -576,9 +661,9 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
         KernelTimer kt(h, MLBA_KERNEL_PCG);
         pcg_iters = launch_pcg_deflated(
             s, h->Pvar, h->hat_K, h->row_ptr.p, h->col_var.p, h->S.p, h->Minv.p, h->rhs.p, h->hat_node.p,
-            h->hat_w.p, h->node_range.p, h->sr_lo.p, h->sr_width, h->defl_SR.p, h->defl_E.p, h->defl_Einv.p, h->px.p, h->pr.p, h->pz.p,
-            h->pp.p, h->pp2.p, h->pq.p, h->defl_mu.p, h->pcg_part.p, h->pcg_part2.p, h->pcg_state.p, h->pcg_host,
-            h->fail.p, o->pcg_relative_tolerance, o->pcg_max_iterations, 16);
+            h->hat_w.p, h->node_range.p, h->grp_lo.p, h->grp_width, h->defl_SR.p, h->defl_E.p, h->defl_Einv.p, h->defl_y.p, h->px.p, h->pr.p, h->pz.p,
+            h->pp.p, h->pp2.p, h->pq.p, h->pcg_part.p, h->pcg_part2.p, h->pcg_state.p, h->pcg_host,
+            o->pcg_relative_tolerance, o->pcg_max_iterations, 16, nullptr);
         if (pcg_iters < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
       }
       launch_unscale_poses(s, h->Pvar, h->px.p, h->pose_scale.p, h->w.p, h->parts_w.p);
@@ -1142,22 +1227,51 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
     launch_landmark_prep(s, h->topo, h->lm_acc.p, h->lm_scale.p, inv_mu, o->min_lm_diagonal, o->max_lm_diagonal,
                          h->lm_z.p, h->fail.p);
   });
-  t->schur_rows_ms = timed([&] {
-    launch_schur_build(s, h->topo, h->n_tasks, h->schur_tasks.p, h->n_mirror, h->mirror.p, h->nnzb, h->vR.p, h->lm.p,
-                       h->obs_rec.p, h->lm_acc.p, h->lm_z.p, h->pose_scale.p, h->pose_diag.p, inv_mu,
-                       o->min_lm_diagonal, o->max_lm_diagonal, h->S.p, h->rhs.p);
+  t->schur_incidences_ms = timed([&] {
+    launch_schur_incidences(s, h->topo, h->n_inc, h->inc_row.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p,
+                            h->lm_z.p, h->pose_scale.p, h->inc_q.p, h->inc_rhs.p, h->lm_wa.p);
+  });
+  t->schur_blocks_ms = timed([&] {
+    launch_schur_blocks(s, h->topo, h->n_upper, h->ublk.p, h->n_items, h->items.p, h->item_ptr.p, h->pairs.p,
+                        h->obs_rec.p, h->inc_q.p, h->inc_rhs.p, h->lm_wa.p, h->pose_scale.p, h->pose_diag.p, inv_mu,
+                        o->min_lm_diagonal, o->max_lm_diagonal, h->schur_partial.p, h->S.p, h->rhs.p);
   });
   t->block_inverse_ms = timed([&] { launch_block_inverse(s, h->Pvar, h->diag_pos.p, h->S.p, h->Minv.p, h->fail.p); });
   if (h->Pvar > 0) {
     if ((rc = ensure_coarse_basis(h, o->pcg_coarse_nodes))) return rc;
-    t->pcg_spmv_ms = timed([&] {
-      (void)launch_pcg_deflated(s, h->Pvar, h->hat_K, h->row_ptr.p, h->col_var.p, h->S.p, h->Minv.p, h->rhs.p,
-                                h->hat_node.p, h->hat_w.p, h->node_range.p, h->sr_lo.p, h->sr_width, h->defl_SR.p, h->defl_E.p,
-                                h->defl_Einv.p, h->px.p, h->pr.p, h->pz.p, h->pp.p, h->pp2.p, h->pq.p, h->defl_mu.p,
-                                h->pcg_part.p, h->pcg_part2.p, h->pcg_state.p, h->pcg_host, h->fail.p,
-                                o->pcg_relative_tolerance, o->pcg_max_iterations, 16);
+    int32_t iters = 0;
+    t->pcg_solve_ms = timed([&] {
+      iters = launch_pcg_deflated(s, h->Pvar, h->hat_K, h->row_ptr.p, h->col_var.p, h->S.p, h->Minv.p, h->rhs.p,
+                                  h->hat_node.p, h->hat_w.p, h->node_range.p, h->grp_lo.p, h->grp_width, h->defl_SR.p,
+                                  h->defl_E.p, h->defl_Einv.p, h->defl_y.p, h->px.p, h->pr.p, h->pz.p, h->pp.p, h->pp2.p, h->pq.p,
+                                  h->pcg_part.p, h->pcg_part2.p, h->pcg_state.p, h->pcg_host,
+                                  o->pcg_relative_tolerance, o->pcg_max_iterations, 16, nullptr);
     });
-    t->pcg_vector_ms = 0.0;
+    t->pcg_iterations = iters;
+    // One more iteration, each kernel bracketed by events, averaged over repeats.
+    hipEvent_t pe[3];
+    for (auto& e : pe) HIP_OK(hipEventCreate(&e));
+    double spmv = 0.0, upd = 0.0;
+    int nrep = 0;
+    for (int r = 0; r < repeats && iters > 0; ++r) {
+      const int it = launch_pcg_deflated(s, h->Pvar, h->hat_K, h->row_ptr.p, h->col_var.p, h->S.p, h->Minv.p,
+                                         h->rhs.p, h->hat_node.p, h->hat_w.p, h->node_range.p, h->grp_lo.p,
+                                         h->grp_width, h->defl_SR.p, h->defl_E.p, h->defl_Einv.p, h->defl_y.p, h->px.p, h->pr.p,
+                                         h->pz.p, h->pp.p, h->pp2.p, h->pq.p, h->pcg_part.p, h->pcg_part2.p,
+                                         h->pcg_state.p, h->pcg_host, o->pcg_relative_tolerance,
+                                         std::min<int32_t>(o->pcg_max_iterations, 4), 16, pe);
+      if (it < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
+      HIP_OK(hipEventSynchronize(pe[2]));
+      float a = 0.f, b = 0.f;
+      HIP_OK(hipEventElapsedTime(&a, pe[0], pe[1]));
+      HIP_OK(hipEventElapsedTime(&b, pe[1], pe[2]));
+      spmv += a;
+      upd += b;
+      ++nrep;
+    }
+    for (auto& e : pe) (void)hipEventDestroy(e);
+    t->pcg_spmv_ms = nrep ? spmv / nrep : 0.0;
+    t->pcg_update_ms = nrep ? upd / nrep : 0.0;
   }
   launch_unscale_poses(s, h->Pvar, h->px.p, h->pose_scale.p, h->w.p, h->parts_w.p);
   t->back_substitute_ms = timed([&] {
@@ -1172,6 +1286,7 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
   t->num_landmarks = h->L;
   t->num_variable_vertices = h->Pvar;
   t->schur_blocks = h->nnzb;
+  t->schur_upper_blocks = h->n_upper;
   t->schur_pairs = h->schur_pairs;
   t->incidences = h->n_inc;
   t->linearize_tiles = h->tiles;

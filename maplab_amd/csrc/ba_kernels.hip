@@ -8,19 +8,20 @@
 //   k_linearize<0>    the same chain, cost only (candidate evaluation)
 //   k_pose_rows       per variable vertex: unscaled gradient and diag(F'F)
 //   k_landmark_prep   per landmark: Z = C (C V C + D^2)^-1 C (Schur pivot)
-//   k_schur_tasks     upper block triangle of the reduced camera system S
-//                     and its rhs: (row, incidence batch, column chunk)
-//                     tasks accumulated in LDS, added with FP64 atomics;
-//   k_schur_mirror    lower triangle S_ji = S_ij^T
+//   k_schur_incidences per row incidence: the Schur factors Q_x and rhs_x
+//   k_schur_items     partial sums of the upper blocks of the reduced camera
+//                     system S over work items of <= 512 incidence pairs
+//   k_schur_finalize  S blocks from the item partials, both triangles
+//   k_schur_rhs       rhs of every S row
 //   k_block_inverse   block-Jacobi preconditioner (6x6 inverses of diag(S))
 //   (ba_pcg.hip)      deflated block-Jacobi PCG on S
 //   k_back_substitute landmark step and the LM model cost change
 //   k_candidate       x (+) step for vertices (JPL Plus) and landmarks
 //   k_gradient_norm   max |x - Plus(x, -g)|
 //   k_evaluate_full   every reference Jacobian block (Evaluate parity API)
-// All reductions are fixed-order (per-tile partials summed by one block)
-// except the S build, whose FP64 atomic sums vary in order run to run (last
-// bits only).
+// All reductions are fixed-order (per-tile partials summed by one block,
+// S blocks by one wave each), so results repeat bit for bit run to run,
+// except landmarks split over linearization tiles (FP64 atomics).
 #include "ba_device.h"
 #include "ba_math.h"
 #include "ba_reduce.h"
@@ -412,10 +413,7 @@ __global__ void k_landmark_prep(DevTopology t, const double* __restrict__ lm_acc
   z[3] = c[1] * c[1] * inv[3]; z[4] = c[1] * c[2] * inv[4]; z[5] = c[2] * c[2] * inv[5];
 }
 
-// ------------------------------------------------------------ Schur rows
-
-constexpr int kSchurChunk = 64;  // S blocks accumulated in LDS per task
-constexpr int kSchurBatch = 128; // row incidences per task
+// ------------------------------------------------------------ Schur complement
 
 // W' = Jl^T (Jo C) (3x6): landmark side unscaled, pose side scaled.
 __device__ inline void wprime_obs(const double* rec, const double* cs, double* W) {
@@ -425,190 +423,205 @@ __device__ inline void wprime_obs(const double* rec, const double* cs, double* W
     for (int c = 0; c < 6; ++c) W[a * 6 + c] = (Jl[a] * Jo[c] + Jl[3 + a] * Jo[6 + c]) * cs[c];
 }
 
-__global__ void k_fill(size_t n, double* __restrict__ p) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0.0;
-}
-
-// Upper block triangle of S = F'F + D^2 - sum_l W_l' Z_l W_l and rhs =
-// F'r - sum W' Z g. One work-group per task = (block row i, a batch of the
-// row's incidences, a chunk of its upper column blocks): the row's pairs
-// (x, y) of incidences sharing a landmark are accumulated in LDS and then
-// added to S with FP64 atomics. S and rhs are zero on entry.
-__global__ __launch_bounds__(kRowThreads) void k_schur_tasks(
-    DevTopology t, const int4* __restrict__ tasks, const double* __restrict__ vR,
+// Incidence factors of the reduced camera system
+//   S = F'F + D^2 - sum_l W_l' Z_l W_l,  rhs = F'r - sum_l W_l' Z_l g_l,
+// with Z_l = C_l (C_l V_l C_l + D_l^2)^-1 C_l from k_landmark_prep. An
+// incidence x of variable vertex i is an observation by i whose residual
+// depends on the pose of i, or the anchor of a landmark l whose anchor
+// vertex i is variable (A_l = M_l C_i, the anchor map of the landmark):
+//   observation: W_x = Jl^T Jo C_i, Q_x = -W_x^T Z_l,
+//                rhs_x = C_i Jo^T r - W_x^T Z_l g_l
+//   anchor:      W_x = V_LM,l A_l,  Q_x = A_l^T - W_x^T Z_l,
+//                rhs_x = A_l^T g_LM,l - W_x^T Z_l g_l
+// (ceres' Schur eliminator, schur_eliminator_impl.h, restated on the
+// anchor-lifted blocks). One thread per incidence; anchors also store
+// [W_x | A_l] per landmark for the block kernel.
+__global__ __launch_bounds__(256) void k_schur_incidences(
+    DevTopology t, int32_t n_inc, const int32_t* __restrict__ inc_row, const double* __restrict__ vR,
     const double* __restrict__ lm, const double* __restrict__ obs_rec, const double* __restrict__ lm_acc,
-    const double* __restrict__ lm_z, const double* __restrict__ pose_scale,
-    const double* __restrict__ pose_diag, double inv_mu, double min_diag, double max_diag,
-    double* __restrict__ S, double* __restrict__ rhs) {
-  __shared__ double acc[kSchurChunk * 36];
-  __shared__ double Q[kSchurBatch * 18];
-  __shared__ int32_t xinc[kSchurBatch];
-  __shared__ int32_t xlm[kSchurBatch];
-  __shared__ int32_t off[kRowThreads + 1];
-  __shared__ int32_t colv[kSchurChunk];
-  __shared__ double rhs_acc[6];
-  __shared__ int32_t wave_tot[kRowThreads / 64];
-  const int tid = threadIdx.x;
-  const int4 task = tasks[blockIdx.x];
-  const int i = task.x;
-  const int v = t.var_vertex[i];
-  const int re = t.row_ptr[i + 1];
-  const int c0 = task.w;
-  const int nc = min(kSchurChunk, re - c0);
-  const int dp = t.diag_pos[i];
-  const bool rhs_task = c0 == dp;                 // first upper chunk owns the rhs
-  const bool diag_task = rhs_task && task.y == t.inc_ptr[i];
-  const double* cs_row = pose_scale + 6 * i;
-  for (int k = tid; k < nc; k += kRowThreads) colv[k] = t.col_var[c0 + k];
-  for (int k = tid; k < nc * 36; k += kRowThreads) acc[k] = 0.0;
-  if (tid < 6) rhs_acc[tid] = 0.0;
-  const int nx = task.z - task.y;
-  int32_t npairs = 0;
-  if (tid < nx) {
-    const int32_t x = t.inc[task.y + tid];
-    const int32_t l = x >= 0 ? t.obs_landmark[x] : -1 - x;
-    npairs = t.lm_begin[l + 1] - t.lm_begin[l] + 1;
-    xinc[tid] = x;
-    xlm[tid] = l;
-    // Q_x = [anchor: C_a M^T] - W_x'^T Z_l, rhs_x = direct - W_x'^T Z g
-    double W[18], Z[9], rx[6], Qx[18];
-    const double* acc_l = lm_acc + (size_t)kLmAcc * l;
-    sym3_full(lm_z + 6 * l, Z);
-    if (x >= 0) {
-      const double* rec = obs_rec + (size_t)kObsRecord * x;
-      wprime_obs(rec, cs_row, W);
-      const double* Jo = rec + 8;
-      for (int c = 0; c < 6; ++c) rx[c] = cs_row[c] * (Jo[c] * rec[0] + Jo[6 + c] * rec[1]);
-      for (int k = 0; k < 18; ++k) Qx[k] = 0.0;
-    } else {
-      double M[18], VLM[9];
-      anchor_map(lm + 3 * l, vR + 12 * v, vR[12 * v + 9], M);
-      sym3_full(acc_l + 6, VLM);
-      for (int a = 0; a < 3; ++a)
-        for (int c = 0; c < 6; ++c)
-          W[a * 6 + c] = (VLM[a * 3] * M[c] + VLM[a * 3 + 1] * M[6 + c] + VLM[a * 3 + 2] * M[12 + c]) * cs_row[c];
-      const double* gLM = acc_l + 15;
-      for (int c = 0; c < 6; ++c) {
-        rx[c] = cs_row[c] * (M[c] * gLM[0] + M[6 + c] * gLM[1] + M[12 + c] * gLM[2]);
-        for (int a = 0; a < 3; ++a) Qx[c * 3 + a] = cs_row[c] * M[a * 6 + c];
-      }
-    }
-    const double* gl = acc_l + 12;
+    const double* __restrict__ lm_z, const double* __restrict__ pose_scale, double* __restrict__ inc_q,
+    double* __restrict__ inc_rhs, double* __restrict__ lm_wa) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_inc) return;
+  const int i = inc_row[p];
+  const int32_t x = t.inc[p];
+  const int32_t l = x >= 0 ? t.obs_landmark[x] : -1 - x;
+  const double* cs = pose_scale + 6 * i;
+  const double* acc_l = lm_acc + (size_t)kLmAcc * l;
+  double W[18], Z[9], rx[6], Qx[18];
+  sym3_full(lm_z + 6 * l, Z);
+  if (x >= 0) {
+    const double* rec = obs_rec + (size_t)kObsRecord * x;
+    wprime_obs(rec, cs, W);
+    const double* Jo = rec + 8;
+    for (int c = 0; c < 6; ++c) rx[c] = cs[c] * (Jo[c] * rec[0] + Jo[6 + c] * rec[1]);
+    for (int k = 0; k < 18; ++k) Qx[k] = 0.0;
+  } else {
+    const int v = t.var_vertex[i];
+    double M[18], A[18], VLM[9];
+    anchor_map(lm + 3 * l, vR + 12 * v, vR[12 * v + 9], M);
+    for (int a = 0; a < 3; ++a)
+      for (int c = 0; c < 6; ++c) A[a * 6 + c] = M[a * 6 + c] * cs[c];
+    sym3_full(acc_l + 6, VLM);
+    for (int a = 0; a < 3; ++a)
+      for (int c = 0; c < 6; ++c)
+        W[a * 6 + c] = VLM[a * 3] * A[c] + VLM[a * 3 + 1] * A[6 + c] + VLM[a * 3 + 2] * A[12 + c];
+    const double* gLM = acc_l + 15;
     for (int c = 0; c < 6; ++c) {
-      double wz[3];
-      for (int a = 0; a < 3; ++a) wz[a] = W[c] * Z[a] + W[6 + c] * Z[3 + a] + W[12 + c] * Z[6 + a];
-      for (int a = 0; a < 3; ++a) Qx[c * 3 + a] -= wz[a];
-      rx[c] -= wz[0] * gl[0] + wz[1] * gl[1] + wz[2] * gl[2];
+      rx[c] = A[c] * gLM[0] + A[6 + c] * gLM[1] + A[12 + c] * gLM[2];
+      for (int a = 0; a < 3; ++a) Qx[c * 3 + a] = A[a * 6 + c];
     }
-    for (int k = 0; k < 18; ++k) Q[tid * 18 + k] = Qx[k];
-    if (rhs_task)
-      for (int c = 0; c < 6; ++c) atomicAdd(&rhs_acc[c], rx[c]);
-  }
-  // Exclusive scan of npairs over the block.
-  {
-    const int lane = tid & 63, wave = tid >> 6;
-    int32_t incl = npairs;
-    for (int d = 1; d < 64; d <<= 1) {
-      const int32_t y = __shfl_up(incl, d, 64);
-      if (lane >= d) incl += y;
-    }
-    if (lane == 63) wave_tot[wave] = incl;
-    __syncthreads();
-    int32_t base = 0;
-    for (int w = 0; w < wave; ++w) base += wave_tot[w];
-    off[tid + 1] = base + incl;
-    if (tid == 0) off[0] = 0;
-    __syncthreads();
-  }
-  const int total = off[nx];
-  const int col_lo = colv[0], col_hi = colv[nc - 1];
-  for (int f = tid; f < total; f += kRowThreads) {
-    int lo = 0, hi = nx - 1;  // largest xi with off[xi] <= f
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (off[mid] <= f) lo = mid; else hi = mid - 1;
-    }
-    const int xi = lo;
-    const int32_t l = xlm[xi];
-    const int32_t lb = t.lm_begin[l];
-    const int32_t nobs = t.lm_begin[l + 1] - lb;
-    const int32_t y = f - off[xi];
-    const int32_t oy = lb + y;
-    const int32_t col = y < nobs ? t.obs_col[oy]
-                                 : ((t.lm_flags[l] & kLmAnchorVar) ? t.vertex_var[t.lm_anchor[l]] : -1);
-    if (col < col_lo || col > col_hi) continue;  // other chunk, lower triangle, or no incidence
-    int plo = 0, phi = nc - 1, pos = -1;
-    while (plo <= phi) {
-      const int mid = (plo + phi) >> 1;
-      const int cv = colv[mid];
-      if (cv == col) { pos = mid; break; }
-      if (cv < col) plo = mid + 1; else phi = mid - 1;
-    }
-    if (pos < 0) continue;
-    const int32_t x = xinc[xi];
-    const bool lconst = (t.lm_flags[l] & kLmConstant) != 0;
-    const double* Qx = Q + xi * 18;
-    double* dst = acc + pos * 36;
-    const double* cs = pose_scale + 6 * col;
-    if (y < nobs) {
-      const bool same = (x == oy);
-      if (lconst && x >= 0 && !same) continue;
-      const double* rec = obs_rec + (size_t)kObsRecord * oy;
-      double W[18];
-      wprime_obs(rec, cs, W);
-      const double* Jo = rec + 8;
-      for (int r = 0; r < 6; ++r) {
-        const double q0 = Qx[r * 3], q1 = Qx[r * 3 + 1], q2 = Qx[r * 3 + 2];
-        for (int c = 0; c < 6; ++c) {
-          double b = q0 * W[c] + q1 * W[6 + c] + q2 * W[12 + c];
-          if (same) b += cs[r] * cs[c] * (Jo[r] * Jo[c] + Jo[6 + r] * Jo[6 + c]);
-          atomicAdd(dst + r * 6 + c, b);
-        }
-      }
-    } else {
-      const int32_t a = t.lm_anchor[l];
-      double M[18], VLM[9], W[18];
-      anchor_map(lm + 3 * l, vR + 12 * a, vR[12 * a + 9], M);
-      sym3_full(lm_acc + (size_t)kLmAcc * l + 6, VLM);
-      for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 6; ++c)
-          W[r * 6 + c] = (VLM[r * 3] * M[c] + VLM[r * 3 + 1] * M[6 + c] + VLM[r * 3 + 2] * M[12 + c]) * cs[c];
-      double Wx[18];
-      if (x >= 0) wprime_obs(obs_rec + (size_t)kObsRecord * x, cs_row, Wx);
-      for (int r = 0; r < 6; ++r) {
-        const double q0 = Qx[r * 3], q1 = Qx[r * 3 + 1], q2 = Qx[r * 3 + 2];
-        for (int c = 0; c < 6; ++c) {
-          double b = q0 * W[c] + q1 * W[6 + c] + q2 * W[12 + c];
-          // direct observer-anchor coupling: W_x'^T M C_a
-          if (x >= 0) b += cs[c] * (Wx[r] * M[c] + Wx[6 + r] * M[6 + c] + Wx[12 + r] * M[12 + c]);
-          atomicAdd(dst + r * 6 + c, b);
-        }
-      }
+    double* wa = lm_wa + (size_t)36 * l;
+    for (int k = 0; k < 18; ++k) {
+      wa[k] = W[k];
+      wa[18 + k] = A[k];
     }
   }
-  __syncthreads();
-  // LM diagonal of the pose columns, added once per row.
-  if (diag_task && tid < 6) {
-    const double d = cs_row[tid] * cs_row[tid] * pose_diag[6 * i + tid];
-    acc[(dp - c0) * 36 + tid * 7] += fmin(fmax(d, min_diag), max_diag) * inv_mu;
+  const double* gl = acc_l + 12;
+  for (int c = 0; c < 6; ++c) {
+    double wz[3];
+    for (int a = 0; a < 3; ++a) wz[a] = W[c] * Z[a] + W[6 + c] * Z[3 + a] + W[12 + c] * Z[6 + a];
+    for (int a = 0; a < 3; ++a) Qx[c * 3 + a] -= wz[a];
+    rx[c] -= wz[0] * gl[0] + wz[1] * gl[1] + wz[2] * gl[2];
   }
-  __syncthreads();
-  for (int k = tid; k < nc * 36; k += kRowThreads) {
-    const double val = acc[k];
-    if (val != 0.0) atomicAdd(S + (size_t)c0 * 36 + k, val);
-  }
-  if (rhs_task && tid < 6) atomicAdd(rhs + 6 * i + tid, rhs_acc[tid]);
+  double* q = inc_q + (size_t)18 * p;
+  for (int k = 0; k < 18; ++k) q[k] = Qx[k];
+  double* rr = inc_rhs + (size_t)6 * p;
+  for (int c = 0; c < 6; ++c) rr[c] = rx[c];
 }
 
-// Lower block triangle from the upper one: S_ji = S_ij^T.
-__global__ void k_schur_mirror(int32_t n, const int2* __restrict__ pairs, double* __restrict__ S) {
-  const int k = blockIdx.x * (blockDim.x / 36) + threadIdx.x / 36;
-  const int e = threadIdx.x % 36;
-  if (k >= n || threadIdx.x >= (blockDim.x / 36) * 36) return;
-  const int2 pr = pairs[k];
+constexpr int kItemWaves = 4;      // work items per work-group (one per wave)
+constexpr int kItemPairs = 512;    // incidence pairs per work item (16 per wave pass)
+constexpr uint32_t kPairAnchor = 0x80000000u;  // x or y is the anchor incidence of its landmark
+constexpr uint32_t kPairSame = 0x40000000u;    // y is the observation of x itself
+
+// Partial sums of S blocks over work items = (upper block u, a range of at
+// most kItemPairs of its incidence pairs), one wave per item:
+//   sum over pairs (x in row i, y in row j, same landmark) of
+//     Q_x W_y + [y = x] C_i Jo^T Jo C_i + [x observation, y anchor] W_x^T A_l.
+// Four lanes share a pair, lane quadrant q owning rows 3 (q >> 1) .. +2 and
+// columns 3 (q & 1) .. +2 of the 6x6 block; the 16 lanes of a quadrant are
+// summed in a fixed butterfly order. Pair encoding: x = row incidence
+// (| kPairAnchor); y = observation index (| kPairSame) or landmark index |
+// kPairAnchor. Work-groups are mapped so that each XCD walks a contiguous
+// range of items (block rows), whose incidence factors stay in its L2.
+__global__ __launch_bounds__(64 * kItemWaves) void k_schur_items(
+    DevTopology t, int32_t n_items, const int4* __restrict__ items, const int4* __restrict__ ublk,
+    const uint2* __restrict__ pairs, const double* __restrict__ obs_rec, const double* __restrict__ inc_q,
+    const double* __restrict__ lm_wa, const double* __restrict__ pose_scale, double* __restrict__ partial) {
+  const int per_xcd = gridDim.x / 8;
+  const int wg = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  const int it = wg * kItemWaves + (threadIdx.x >> 6);
+  if (it >= n_items) return;
+  const int lane = threadIdx.x & 63, slot = lane >> 2, quad = lane & 3;
+  const int r0 = 3 * (quad >> 1), c0 = 3 * (quad & 1);
+  const int4 item = items[it];  // (u, first pair, end pair)
+  const int4 ub = ublk[item.x];
+  const int i = ub.z, j = ub.w;
+  double csr[3], csc[3];
+  for (int d = 0; d < 3; ++d) {
+    csr[d] = pose_scale[6 * i + r0 + d];
+    csc[d] = pose_scale[6 * j + c0 + d];
+  }
+  double acc[9];
+  for (int k = 0; k < 9; ++k) acc[k] = 0.0;
+  for (int p = item.y + slot; p < item.z; p += 16) {
+    const uint2 pr = pairs[p];
+    const int32_t x = (int32_t)(pr.x & ~kPairAnchor);
+    const bool xa = (pr.x & kPairAnchor) != 0, ya = (pr.y & kPairAnchor) != 0;
+    const bool same = (pr.y & kPairSame) != 0 && !ya;
+    const int32_t yv = (int32_t)(pr.y & ~(kPairAnchor | kPairSame));
+    const double* Q = inc_q + (size_t)18 * x + 3 * r0;
+    double q[9], W[9];
+    for (int k = 0; k < 9; ++k) q[k] = Q[k];
+    double jo[6];  // Jo rows 0 and 1 at the quadrant's columns (observation y)
+    if (!ya) {
+      const double* rec = obs_rec + (size_t)kObsRecord * yv;
+      double jl[6];
+      for (int k = 0; k < 6; ++k) jl[k] = rec[2 + k];
+      for (int d = 0; d < 3; ++d) {
+        jo[d] = rec[8 + c0 + d];
+        jo[3 + d] = rec[14 + c0 + d];
+      }
+      for (int aa = 0; aa < 3; ++aa)
+        for (int d = 0; d < 3; ++d) W[aa * 3 + d] = (jl[aa] * jo[d] + jl[3 + aa] * jo[3 + d]) * csc[d];
+    } else {
+      const double* wa = lm_wa + (size_t)36 * yv + c0;
+      for (int aa = 0; aa < 3; ++aa)
+        for (int d = 0; d < 3; ++d) W[aa * 3 + d] = wa[aa * 6 + d];
+    }
+    for (int rr = 0; rr < 3; ++rr)
+      for (int d = 0; d < 3; ++d)
+        acc[rr * 3 + d] += q[rr * 3] * W[d] + q[rr * 3 + 1] * W[3 + d] + q[rr * 3 + 2] * W[6 + d];
+    if (ya && !xa) {
+      // direct observer-anchor coupling W_x^T A_l
+      const double* rec = obs_rec + (size_t)kObsRecord * t.inc[x];
+      const double* A = lm_wa + (size_t)36 * yv + 18 + c0;
+      double wx[9];  // W_x at the quadrant's rows: wx[a][rr]
+      for (int aa = 0; aa < 3; ++aa)
+        for (int rr = 0; rr < 3; ++rr)
+          wx[aa * 3 + rr] = (rec[2 + aa] * rec[8 + r0 + rr] + rec[5 + aa] * rec[14 + r0 + rr]) * csr[rr];
+      for (int rr = 0; rr < 3; ++rr)
+        for (int d = 0; d < 3; ++d)
+          acc[rr * 3 + d] += wx[rr] * A[d] + wx[3 + rr] * A[6 + d] + wx[6 + rr] * A[12 + d];
+    } else if (same) {
+      const double* rec = obs_rec + (size_t)kObsRecord * yv;
+      for (int rr = 0; rr < 3; ++rr) {
+        const double o0 = rec[8 + r0 + rr], o1 = rec[14 + r0 + rr];
+        for (int d = 0; d < 3; ++d) acc[rr * 3 + d] += csr[rr] * csc[d] * (o0 * jo[d] + o1 * jo[3 + d]);
+      }
+    }
+  }
+  for (int k = 0; k < 9; ++k) {
+    double v = acc[k];
+    for (int off = 4; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+    acc[k] = v;
+  }
+  if (lane < 4) {
+    double* out = partial + (size_t)36 * it;
+    for (int rr = 0; rr < 3; ++rr)
+      for (int d = 0; d < 3; ++d) out[(r0 + rr) * 6 + c0 + d] = acc[rr * 3 + d];
+  }
+}
+
+// S_ij = sum of the block's item partials in item order (+ the LM diagonal
+// D^2 / radius on S_ii), written to S_ij and S_ji = S_ij^T. One thread per
+// block entry.
+__global__ __launch_bounds__(252) void k_schur_finalize(int32_t nub, const int4* __restrict__ ublk,
+                                                        const int32_t* __restrict__ item_ptr,
+                                                        const double* __restrict__ partial,
+                                                        const double* __restrict__ pose_scale,
+                                                        const double* __restrict__ pose_diag, double inv_mu,
+                                                        double min_diag, double max_diag, double* __restrict__ S) {
+  const int u = blockIdx.x * 7 + threadIdx.x / 36, e = threadIdx.x % 36;
+  if (u >= nub) return;
+  const int4 ub = ublk[u];
+  double v = 0.0;
+  for (int it = item_ptr[u]; it < item_ptr[u + 1]; ++it) v += partial[(size_t)36 * it + e];
   const int r = e / 6, c = e % 6;
-  S[(size_t)pr.y * 36 + c * 6 + r] = S[(size_t)pr.x * 36 + r * 6 + c];
+  if (ub.z == ub.w && r == c) {
+    const double sc = pose_scale[6 * ub.z + c];
+    v += fmin(fmax(sc * sc * pose_diag[6 * ub.z + c], min_diag), max_diag) * inv_mu;
+  }
+  S[(size_t)36 * ub.x + e] = v;
+  if (ub.y != ub.x) S[(size_t)36 * ub.y + c * 6 + r] = v;
+}
+
+// rhs_i = sum of rhs_x over the row's incidences, one work-group per row.
+__global__ __launch_bounds__(256) void k_schur_rhs(DevTopology t, const double* __restrict__ inc_rhs,
+                                                   double* __restrict__ rhs) {
+  __shared__ double red[4][6];
+  const int i = blockIdx.x;
+  double g[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = t.inc_ptr[i] + threadIdx.x; k < t.inc_ptr[i + 1]; k += 256)
+    for (int c = 0; c < 6; ++c) g[c] += inc_rhs[(size_t)6 * k + c];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int c = 0; c < 6; ++c) g[c] = wave_sum(g[c]);
+  if (lane == 0)
+    for (int c = 0; c < 6; ++c) red[wave][c] = g[c];
+  __syncthreads();
+  if (threadIdx.x < 6)
+    rhs[6 * i + threadIdx.x] = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
 }
 
 // ------------------------------------------------------------ PCG
@@ -1100,21 +1113,33 @@ void launch_landmark_prep(hipStream_t s, const DevTopology& t, const double* lm_
   if (t.L > 0)
     k_landmark_prep<<<(t.L + 255) / 256, 256, 0, s>>>(t, lm_acc, lm_scale, inv_mu, min_diag, max_diag, lm_z, fail);
 }
-void launch_schur_build(hipStream_t s, const DevTopology& t, int32_t ntasks, const int4* tasks, int32_t nmirror,
-                        const int2* mirror, int64_t nnzb, const double* vR, const double* lm, const double* obs_rec,
-                        const double* lm_acc, const double* lm_z, const double* pose_scale, const double* pose_diag,
-                        double inv_mu, double min_diag, double max_diag, double* S, double* rhs) {
-  if (t.Pvar <= 0) return;
-  const size_t ns = (size_t)nnzb * 36;
-  k_fill<<<(unsigned)((ns + 255) / 256), 256, 0, s>>>(ns, S);
-  k_fill<<<(unsigned)((6 * (size_t)t.Pvar + 255) / 256), 256, 0, s>>>(6 * (size_t)t.Pvar, rhs);
-  if (ntasks > 0)
-    k_schur_tasks<<<ntasks, kRowThreads, 0, s>>>(t, tasks, vR, lm, obs_rec, lm_acc, lm_z, pose_scale, pose_diag,
-                                                 inv_mu, min_diag, max_diag, S, rhs);
-  if (nmirror > 0) k_schur_mirror<<<(nmirror + 6) / 7, 252, 0, s>>>(nmirror, mirror, S);
+void launch_schur_incidences(hipStream_t s, const DevTopology& t, int32_t n_inc, const int32_t* inc_row,
+                             const double* vR, const double* lm, const double* obs_rec, const double* lm_acc,
+                             const double* lm_z, const double* pose_scale, double* inc_q, double* inc_rhs,
+                             double* lm_wa) {
+  if (t.Pvar > 0 && n_inc > 0)
+    k_schur_incidences<<<(n_inc + 255) / 256, 256, 0, s>>>(t, n_inc, inc_row, vR, lm, obs_rec, lm_acc, lm_z,
+                                                           pose_scale, inc_q, inc_rhs, lm_wa);
 }
-int32_t schur_chunk_blocks() { return kSchurChunk; }
-int32_t schur_batch_incidences() { return kSchurBatch; }
+void launch_schur_blocks(hipStream_t s, const DevTopology& t, int32_t nub, const int4* ublk, int32_t n_items,
+                         const int4* items, const int32_t* item_ptr, const uint2* pairs, const double* obs_rec,
+                         const double* inc_q, const double* inc_rhs, const double* lm_wa, const double* pose_scale,
+                         const double* pose_diag, double inv_mu, double min_diag, double max_diag, double* partial,
+                         double* S, double* rhs) {
+  if (t.Pvar <= 0 || nub <= 0) return;
+  k_schur_rhs<<<t.Pvar, 256, 0, s>>>(t, inc_rhs, rhs);
+  if (n_items > 0) {
+    const int wgs = (n_items + kItemWaves - 1) / kItemWaves;
+    const int per_xcd = (wgs + 7) / 8;
+    k_schur_items<<<8 * per_xcd, 64 * kItemWaves, 0, s>>>(t, n_items, items, ublk, pairs, obs_rec, inc_q, lm_wa,
+                                                          pose_scale, partial);
+  }
+  k_schur_finalize<<<(nub + 6) / 7, 252, 0, s>>>(nub, ublk, item_ptr, partial, pose_scale, pose_diag, inv_mu,
+                                                 min_diag, max_diag, S);
+}
+int32_t schur_item_pairs() { return kItemPairs; }
+uint32_t schur_pair_same_bit() { return kPairSame; }
+uint32_t schur_pair_anchor_bit() { return kPairAnchor; }
 void launch_block_inverse(hipStream_t s, int32_t Pvar, const int32_t* diag_pos, const double* S, double* Minv,
                           int32_t* fail) {
   if (Pvar > 0) k_block_inverse<<<(Pvar + 63) / 64, 64, 0, s>>>(Pvar, diag_pos, S, Minv, fail);

@@ -5,41 +5,49 @@
 // a handful of very small eigenvalues on long trajectories: smooth, global
 // modes of the vertex chain. They are removed by deflation (Saad, Yeung,
 // Erhel, Guyomarc'h, "A deflated version of the conjugate gradient
-// algorithm", SIAM J. Sci. Comput. 21, 2000) with a coarse space W = R of
+// algorithm", SIAM J. Sci. Comput. 21, 2000) with a coarse space W of
 // piecewise-linear "hat" functions over the vertex order (K nodes, 6 K
 // columns): x0 = W E^-1 W'b, E = W'SW, and every search direction is made
 // S-orthogonal to W with mu = E^-1 (SW)'z. The iteration converges to the
 // same solution as plain PCG (the tolerance is on the true residual
-// recursion), in 3-4x fewer iterations on the benchmark map.
+// recursion), in 3-4x fewer iterations on long random-walk maps.
 //
-// Per LM step: k_defl_sr (SW), k_defl_e (E), k_defl_inverse (E^-1, Gauss-
-// Jordan in LDS), k_defl_init; per PCG iteration: k_pcg_spmv (S p with the
-// direction update formed on the fly) and k_pcg_update (vector update; the
-// last work-group to finish completes the reductions, no inter-group waits).
+// Rows of S are handled in groups of 42 vertices (252 entries); SW is stored
+// per group, column-major over the group's window of coarse columns, so that
+// (SW)'z of a group is a set of coalesced column dot products.
+//
+// Per LM step: k_defl_sr (SW), k_defl_e (E), k_defl_prep (state, E^-1 by
+// Gauss-Jordan, y = E^-1 W'b), k_defl_init (x0, r0, z0 and the first
+// per-group partials). Per PCG iteration: k_pcg_spmv (sums the previous
+// per-group partials, forms p = z - W mu + beta p_prev on the fly, q = S p)
+// and k_pcg_update (x, r, z and the per-group partials r'z, r'r,
+// E^-1 (SW)'z). Every work-group sums partials in the same fixed order: the
+// iteration is deterministic and uses no atomics.
 #include "ba_device.h"
 #include "ba_reduce.h"
 
 namespace mlba {
 
-constexpr int kMaxCoarse = 90;   // 6 x 15 hat nodes; E^-1 by Gauss-Jordan in LDS
+constexpr int kMaxCoarse = 90;   // 6 x 15 hat nodes
 constexpr int kRowsPerGroup = 42;
-constexpr int kGroupThreads = 6 * kRowsPerGroup;  // 252: one thread per (row, component)
+constexpr int kGroupThreads = 6 * kRowsPerGroup;  // 252 entries (row, component) per group
+constexpr int kGroupBlock = 1024;                 // threads of the per-group kernels
+constexpr int kPartChunk = 2048;                  // doubles of partials summed per LDS pass
 
 struct PcgState {
-  double rz;        // r'z of the current iterate
+  double rz[2];     // r'z of iteration k in rz[k & 1] (written by k_pcg_spmv)
   double bb;        // |b|^2
-  double rr;        // |r|^2
+  double tol_rel2;  // tol^2
   double tol2;      // (tol |b|)^2
-  double beta;      // direction update factor for the next iteration
   int32_t iters;
   int32_t done;
   int32_t max_iters;
-  uint32_t counter; // work-groups of k_pcg_update that finished this iteration
+  int32_t pad;
 };
 
 struct PcgArgs {
   int32_t Pvar, ne;            // ne = 6 K coarse columns (0: no deflation)
-  int32_t K, pad;
+  int32_t K, groups;
   const int32_t* row_ptr;
   const int32_t* col_var;
   const double* S;
@@ -48,9 +56,9 @@ struct PcgArgs {
   const int32_t* hat_node;     // Pvar: first hat node of the vertex (0 without deflation)
   const double* hat_w;         // 2 Pvar: weights of hat_node and hat_node + 1 (0 without deflation)
   const int32_t* node_range;   // 2 K: first and last vertex of each hat's support
-  const int32_t* sr_lo;        // Pvar: first hat node of the window of row i of SW
-  int32_t sw, pad2;            // window width of SW rows, in columns (6 x nodes)
-  double* SR;                  // 6 Pvar x sw: row e = 6 i + c holds columns [6 sr_lo[i], 6 sr_lo[i] + sw) of SW
+  const int32_t* grp_lo;       // groups: first hat node of the group's window of SW columns
+  int32_t gw, pad2;            // window width in columns (6 x nodes)
+  double* SR;                  // groups x gw x 252: SW entry (group row t, window column lc)
   double* E;                   // ne x ne
   double* Einv;                // ne x ne
   double* x;
@@ -59,11 +67,10 @@ struct PcgArgs {
   double* p0;
   double* p1;
   double* q;
-  double* mu;                  // ne
   double* part;                // Pvar: p'q of each row
-  double* part2;               // groups x (2 + ne)
+  double* part2;               // groups x (3 + ne): r'z, r'r, b'b and E^-1 (SW)'z of the group's rows
+  double* y;                   // ne: E^-1 W'b
   PcgState* st;
-  int32_t* fail;
 };
 
 __device__ inline double hat_weight(const PcgArgs& a, int i, int k) {
@@ -71,7 +78,37 @@ __device__ inline double hat_weight(const PcgArgs& a, int i, int k) {
   return n0 == k ? a.hat_w[2 * i] : (n0 + 1 == k ? a.hat_w[2 * i + 1] : 0.0);
 }
 
-// SW, one block row of S per work-group: (SW)_i = sum_j S_ij W_j, W_j = w0 e_n0 + w1 e_n0+1.
+__device__ inline double* sr_col(const PcgArgs& a, int g, int lc) {
+  return a.SR + ((size_t)g * a.gw + lc) * kGroupThreads;
+}
+
+// Block sums of three values (blockDim.x = 64 x waves, at most 16 waves), in
+// every thread.
+__device__ inline void block_sum3(double& v0, double& v1, double& v2, double* lds) {
+  const int nw = blockDim.x >> 6, w = threadIdx.x >> 6;
+  v0 = wave_sum(v0);
+  v1 = wave_sum(v1);
+  v2 = wave_sum(v2);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    lds[w] = v0;
+    lds[16 + w] = v1;
+    lds[32 + w] = v2;
+  }
+  __syncthreads();
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int k = 0; k < nw; ++k) {
+    s0 += lds[k];
+    s1 += lds[16 + k];
+    s2 += lds[32 + k];
+  }
+  v0 = s0;
+  v1 = s1;
+  v2 = s2;
+}
+
+// SW, one block row of S per work-group: (SW)_i = sum_j S_ij W_j with
+// W_j = w0 e_n0 + w1 e_n0+1, written into the group window.
 __global__ __launch_bounds__(256) void k_defl_sr(PcgArgs a) {
   __shared__ double acc[6 * kMaxCoarse];
   const int i = blockIdx.x, ne = a.ne;
@@ -88,17 +125,18 @@ __global__ __launch_bounds__(256) void k_defl_sr(PcgArgs a) {
     if (w1 != 0.0) atomicAdd(&acc[row * ne + 6 * (n0 + 1) + c], w1 * v);
   }
   __syncthreads();
-  const int c0 = 6 * a.sr_lo[i], sw = a.sw;
-  for (int k = threadIdx.x; k < 6 * sw; k += blockDim.x) {
-    const int row = k / sw, gc = c0 + k % sw;
-    a.SR[(size_t)(6 * i) * sw + k] = gc < ne ? acc[row * ne + gc] : 0.0;
+  const int g = i / kRowsPerGroup, t0 = 6 * (i - g * kRowsPerGroup), c0 = 6 * a.grp_lo[g];
+  for (int k = threadIdx.x; k < 6 * a.gw; k += blockDim.x) {
+    const int comp = k % 6, lc = k / 6, gc = c0 + lc;
+    sr_col(a, g, lc)[t0 + comp] = gc < ne ? acc[comp * ne + gc] : 0.0;
   }
 }
 
-// Entry (e, col) of SW from the windowed storage.
+// Entry (e, col) of SW from the group windows.
 __device__ inline double sw_entry(const PcgArgs& a, int e, int col) {
-  const int lc = col - 6 * a.sr_lo[e / 6];
-  return (lc >= 0 && lc < a.sw) ? a.SR[(size_t)e * a.sw + lc] : 0.0;
+  const int g = e / kGroupThreads;
+  const int lc = col - 6 * a.grp_lo[g];
+  return (lc >= 0 && lc < a.gw) ? sr_col(a, g, lc)[e - g * kGroupThreads] : 0.0;
 }
 
 // E = W'(SW), one entry per thread.
@@ -113,157 +151,261 @@ __global__ void k_defl_e(PcgArgs a) {
   a.E[t] = s;
 }
 
-// E^-1 by Gauss-Jordan elimination without pivoting (E is SPD), in LDS.
-// A non-positive pivot disables deflation for this solve (E^-1 = 0).
-__global__ __launch_bounds__(1024) void k_defl_inverse(PcgArgs a) {
-  __shared__ double M[kMaxCoarse * 2 * kMaxCoarse];
-  __shared__ double fcol[kMaxCoarse];
-  __shared__ int bad;
-  const int ne = a.ne, w = 2 * ne;
-  for (int t = threadIdx.x; t < ne * w; t += blockDim.x) {
-    const int r = t / w, c = t % w;
-    // symmetrize: lower triangle of E mirrored
-    M[t] = c < ne ? (c <= r ? a.E[r * ne + c] : a.E[c * ne + r]) : (c - ne == r ? 1.0 : 0.0);
-  }
-  if (threadIdx.x == 0) bad = 0;
-  __syncthreads();
-  for (int j = 0; j < ne; ++j) {
-    const double piv = M[j * w + j];
-    if (!(piv > 0.0)) {
-      if (threadIdx.x == 0) bad = 1;
-      break;
-    }
-    for (int r = threadIdx.x; r < ne; r += blockDim.x) fcol[r] = M[r * w + j];
-    __syncthreads();
-    for (int c = threadIdx.x; c < w; c += blockDim.x) M[j * w + c] /= piv;
-    __syncthreads();
-    for (int t = threadIdx.x; t < ne * w; t += blockDim.x) {
-      const int r = t / w, c = t % w;
-      if (r != j) M[t] -= fcol[r] * M[j * w + c];
-    }
-    __syncthreads();
-  }
-  __syncthreads();
-  for (int t = threadIdx.x; t < ne * ne; t += blockDim.x) {
-    const int r = t / ne, c = t % ne;
-    a.Einv[t] = bad ? 0.0 : M[r * w + ne + c];
-  }
-}
+constexpr int kPrepThreads = 1024;
+constexpr int kInvPerThread = (kMaxCoarse * kMaxCoarse + kPrepThreads - 1) / kPrepThreads;
 
-// x0 = W E^-1 W'b, r0 = b - (SW) E^-1 W'b, z0 = M^-1 r0, mu0 = E^-1 (SW)'z0,
-// p = 0 and the scalars (one work-group).
-__global__ __launch_bounds__(1024) void k_defl_init(PcgArgs a, double tol, int32_t max_iters) {
-  __shared__ double y[kMaxCoarse];
+// Solve set-up (one work-group): the PCG state, c = W'b, E^-1 by in-place
+// Gauss-Jordan elimination without pivoting (E is SPD; each thread keeps its
+// entries in registers, the pivot row and column go through LDS), and
+// y = E^-1 c. A non-positive pivot disables deflation for this solve
+// (E^-1 = 0, y = 0).
+__global__ __launch_bounds__(kPrepThreads) void k_defl_prep(PcgArgs a, double tol, int32_t max_iters) {
+  __shared__ double Ms[kMaxCoarse * kMaxCoarse];
+  __shared__ double cpart[kPrepThreads];
   __shared__ double cvec[kMaxCoarse];
-  __shared__ double lds16[16];
-  const int ne = a.ne, P = a.Pvar, tid = threadIdx.x;
-  // c = W'b
-  for (int rr = tid; rr < ne; rr += blockDim.x) {
-    const int k = rr / 6, comp = rr % 6;
-    double s = 0.0;
-    for (int i = a.node_range[2 * k]; i <= a.node_range[2 * k + 1]; ++i) s += hat_weight(a, i, k) * a.b[6 * i + comp];
-    cvec[rr] = s;
-  }
-  __syncthreads();
-  for (int rr = tid; rr < ne; rr += blockDim.x) {
-    double s = 0.0;
-    for (int c = 0; c < ne; ++c) s += a.Einv[rr * ne + c] * cvec[c];
-    y[rr] = s;
-  }
-  __syncthreads();
-  double bb = 0.0, rz = 0.0, rr2 = 0.0;
-  for (int i = tid; i < P; i += blockDim.x) {
-    double rv[6];
-    const int n0 = a.hat_node[i];
-    const double w0 = a.hat_w[2 * i], w1 = a.hat_w[2 * i + 1];
-    for (int comp = 0; comp < 6; ++comp) {
-      const double bv = a.b[6 * i + comp];
-      double sy = 0.0;
-      const double* srow = a.SR + (size_t)(6 * i + comp) * a.sw;
-      const int c0 = 6 * a.sr_lo[i];
-      for (int c = 0; c < a.sw && c0 + c < ne; ++c) sy += srow[c] * y[c0 + c];
-      rv[comp] = bv - sy;
-      a.r[6 * i + comp] = rv[comp];
-      a.x[6 * i + comp] = ne ? w0 * y[6 * n0 + comp] + (w1 != 0.0 ? w1 * y[6 * (n0 + 1) + comp] : 0.0) : 0.0;
-      a.p0[6 * i + comp] = 0.0;
-      bb += bv * bv;
-      rr2 += rv[comp] * rv[comp];
-    }
-    const double* M = a.Minv + (size_t)36 * i;
-    for (int comp = 0; comp < 6; ++comp) {
-      double s = 0.0;
-      for (int c = 0; c < 6; ++c) s += M[comp * 6 + c] * rv[c];
-      a.z[6 * i + comp] = s;
-      rz += rv[comp] * s;
-    }
-  }
-  __syncthreads();
-  // t = (SW)'z0 with LDS atomics over the windowed rows, then mu0 = E^-1 t.
-  if (ne > 0) {
-    for (int k = tid; k < ne; k += blockDim.x) cvec[k] = 0.0;
-    __syncthreads();
-    const int sw = a.sw;
-    for (size_t el = tid; el < (size_t)6 * P * sw; el += blockDim.x) {
-      const int e = (int)(el / sw), lc = (int)(el % sw);
-      const int gc = 6 * a.sr_lo[e / 6] + lc;
-      const double v = a.SR[el];
-      if (gc < ne && v != 0.0) atomicAdd(&cvec[gc], v * a.z[e]);
-    }
-    __syncthreads();
-    for (int rrow = tid; rrow < ne; rrow += blockDim.x) {
-      double m = 0.0;
-      for (int c = 0; c < ne; ++c) m += a.Einv[rrow * ne + c] * cvec[c];
-      a.mu[rrow] = m;
-    }
-  }
-  // block sums (1024 threads = 16 waves)
-  auto bsum = [&](double v) {
-    v = wave_sum(v);
-    __syncthreads();
-    if ((tid & 63) == 0) lds16[tid >> 6] = v;
-    __syncthreads();
-    double s = 0.0;
-    for (int k = 0; k < 16; ++k) s += lds16[k];
-    return s;
-  };
-  bb = bsum(bb);
-  rz = bsum(rz);
-  rr2 = bsum(rr2);
+  __shared__ double rowk[kMaxCoarse];
+  __shared__ double colk[kMaxCoarse];
+  const int ne = a.ne, tid = threadIdx.x;
   if (tid == 0) {
     PcgState* st = a.st;
-    st->rz = rz;
-    st->bb = bb;
-    st->rr = rr2;
-    st->tol2 = tol * tol * bb;
-    st->beta = 0.0;
+    st->rz[0] = st->rz[1] = 0.0;
+    st->bb = 0.0;
+    st->tol_rel2 = tol * tol;
+    st->tol2 = 0.0;
     st->iters = 0;
+    st->done = 0;
     st->max_iters = max_iters;
-    st->counter = 0;
-    st->done = (bb <= 0.0 || max_iters <= 0 || rr2 <= st->tol2) ? 1 : 0;
+  }
+  if (ne == 0) return;
+  // c = W'b: thread (column, g) sums the support rows lo + g, lo + g + G, ...
+  {
+    const int G = kPrepThreads / ne, col = tid % ne, g = tid / ne, k = col / 6, comp = col % 6;
+    double acc = 0.0;
+    if (g < G)
+      for (int i = a.node_range[2 * k] + g; i <= a.node_range[2 * k + 1]; i += G)
+        acc += hat_weight(a, i, k) * a.b[6 * i + comp];
+    cpart[tid] = acc;
+    __syncthreads();
+    if (tid < ne) {
+      double c = 0.0;
+      for (int gg = 0; gg < G; ++gg) c += cpart[gg * ne + tid];
+      cvec[tid] = c;
+    }
+  }
+  // In-place Gauss-Jordan: for pivot k, a_rc -= a_rk a_kc / p, a_kc /= p,
+  // a_rk = -a_rk / p, a_kk = 1 / p.
+  const int n2 = ne * ne;
+  double v[kInvPerThread];
+  int er[kInvPerThread], ec[kInvPerThread];  // entry (row, column); -1 past the matrix
+#pragma unroll
+  for (int m = 0; m < kInvPerThread; ++m) {
+    const int e = tid + m * kPrepThreads;
+    er[m] = e < n2 ? e / ne : -1;
+    ec[m] = e < n2 ? e - (e / ne) * ne : -1;
+    v[m] = e < n2 ? (ec[m] <= er[m] ? a.E[er[m] * ne + ec[m]] : a.E[ec[m] * ne + er[m]]) : 0.0;
+  }
+  bool ok = true;
+  for (int k = 0; k < ne; ++k) {
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kInvPerThread; ++m) {
+      if (er[m] == k) rowk[ec[m]] = v[m];
+      if (ec[m] == k) colk[er[m]] = v[m];
+    }
+    __syncthreads();
+    const double p = rowk[k];
+    if (!(p > 0.0)) {
+      ok = false;
+      break;
+    }
+    const double ip = 1.0 / p;
+    // Branch-free update of the owned entries.
+#pragma unroll
+    for (int m = 0; m < kInvPerThread; ++m) {
+      const bool isr = er[m] == k, isc = ec[m] == k;
+      const double ck = colk[max(er[m], 0)], rk = rowk[max(ec[m], 0)];
+      const double vs = v[m] * ip;
+      double nv = fma(-ck * ip, rk, v[m]);
+      nv = isc ? -vs : nv;
+      nv = isr ? vs : nv;
+      v[m] = (isr && isc) ? ip : nv;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < kInvPerThread; ++m)
+    if (er[m] >= 0) Ms[er[m] * ne + ec[m]] = ok ? v[m] : 0.0;
+  __syncthreads();
+  for (int e = tid; e < n2; e += kPrepThreads) a.Einv[e] = Ms[e];
+  // y = E^-1 c: thread (row, part) sums a contiguous part of the columns.
+  const int nparts = kPrepThreads / ne, row = tid % ne, part = tid / ne, len = (ne + nparts - 1) / nparts;
+  double yv = 0.0;
+  if (part < nparts)
+    for (int c = part * len; c < min(ne, (part + 1) * len); ++c) yv += Ms[row * ne + c] * cvec[c];
+  cpart[tid] = yv;
+  __syncthreads();
+  if (tid < ne) {
+    double y = 0.0;
+    for (int pp = 0; pp < nparts; ++pp) y += cpart[pp * ne + tid];
+    a.y[tid] = y;
   }
 }
 
-// q = S p_k with p_k = z - W mu + beta p_{k-1} formed on the fly (the own
-// row is stored as the new direction); part[i] = p_k,i . q_i.
-// Thread t < 252 handles component t % 6 of every 42nd block of the row.
-__global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a, const double* __restrict__ p_prev,
-                                                  double* __restrict__ p_cur) {
-  __shared__ double mu_s[kMaxCoarse];
-  __shared__ double ysum[kGroupThreads];
-  const PcgState* st = a.st;
-  if (st->done) return;
+// Per-group partials of group g: out = {rz, rr, bb, E^-1 (SW)'z}; zloc holds
+// the group's z (0 past the last row), cs (ne) is zero on entry.
+__device__ inline void group_partials(const PcgArgs& a, int g, const double* zloc, double rz, double rr, double bb,
+                                      double* cs, double* mpart, double* out) {
   const int ne = a.ne, tid = threadIdx.x;
-  for (int k = tid; k < ne; k += blockDim.x) mu_s[k] = a.mu[k];
+  if (tid == 0) {
+    out[0] = rz;
+    out[1] = rr;
+    out[2] = bb;
+  }
+  if (ne == 0) return;
+  // (SW)'z: wave w takes the window columns w, w + waves, ...; its lanes sum
+  // the group entries t = lane + 64 m.
+  const int nw = blockDim.x >> 6, wave = tid >> 6, lane = tid & 63;
+  const int c0 = 6 * a.grp_lo[g];
+  for (int lc = wave; lc < a.gw; lc += nw) {
+    const double* col = sr_col(a, g, lc);
+    double s = 0.0;
+#pragma unroll
+    for (int m = 0; m < (kGroupThreads + 63) / 64; ++m) {
+      const int t = lane + 64 * m;
+      if (t < kGroupThreads) s += col[t] * zloc[t];
+    }
+    s = wave_sum(s);
+    if (lane == 0 && c0 + lc < ne) cs[c0 + lc] = s;
+  }
   __syncthreads();
-  const double beta = st->beta;
+  // E^-1 cs: thread (row, part) sums a contiguous part of the columns.
+  const int nparts = blockDim.x / ne, rrow = tid % ne, part = tid / ne;
+  const int len = (ne + nparts - 1) / nparts;
+  double m = 0.0;
+  if (part < nparts) {
+    const int c_end = min(ne, (part + 1) * len);
+    for (int c = part * len; c < c_end; ++c) m += a.Einv[c * ne + rrow] * cs[c];  // E^-1 is symmetric
+  }
+  mpart[tid] = m;
+  __syncthreads();
+  if (tid < ne) {
+    double mu = 0.0;
+    for (int pp = 0; pp < nparts; ++pp) mu += mpart[pp * ne + tid];
+    out[3 + tid] = mu;
+  }
+}
+
+// x0 = W y, r0 = b - (SW) y, z0 = M^-1 r0, p = 0 on one group of rows per
+// work-group, and the group's partials.
+__global__ __launch_bounds__(kGroupBlock) void k_defl_init(PcgArgs a) {
+  __shared__ double rloc[kGroupThreads];
+  __shared__ double zloc[kGroupThreads];
+  __shared__ double lds[48];
+  __shared__ double cs[kMaxCoarse];
+  __shared__ double mpart[kGroupBlock];
+  __shared__ double ys[kMaxCoarse];
+  const int ne = a.ne, tid = threadIdx.x, P = a.Pvar, g = blockIdx.x;
+  for (int c = tid; c < ne; c += blockDim.x) {
+    ys[c] = a.y[c];
+    cs[c] = 0.0;
+  }
+  __syncthreads();
+  const int row = g * kRowsPerGroup + tid / 6, comp = tid % 6;
+  const bool active = tid < kGroupThreads && row < P;
+  double rv = 0.0, bv = 0.0;
+  if (active) {
+    const int e = 6 * row + comp;
+    bv = a.b[e];
+    double sy = 0.0, xv = 0.0;
+    if (ne) {
+      const int c0 = 6 * a.grp_lo[g];
+      for (int lc = 0; lc < a.gw && c0 + lc < ne; ++lc) sy += sr_col(a, g, lc)[tid] * ys[c0 + lc];
+      const int n0 = a.hat_node[row];
+      const double w0 = a.hat_w[2 * row], w1 = a.hat_w[2 * row + 1];
+      xv = w0 * ys[6 * n0 + comp] + (w1 != 0.0 ? w1 * ys[6 * (n0 + 1) + comp] : 0.0);
+    }
+    rv = bv - sy;
+    a.x[e] = xv;
+    a.r[e] = rv;
+    a.p0[e] = 0.0;
+  }
+  if (tid < kGroupThreads) rloc[tid] = rv;
+  __syncthreads();
+  double zv = 0.0;
+  if (active) {
+    const double* M = a.Minv + (size_t)36 * row + 6 * comp;
+    const int base = tid - comp;
+    for (int c = 0; c < 6; ++c) zv += M[c] * rloc[base + c];
+    a.z[6 * row + comp] = zv;
+  }
+  if (tid < kGroupThreads) zloc[tid] = zv;
+  double rz = rv * zv, rr = rv * rv, bb = bv * bv;
+  block_sum3(rz, rr, bb, lds);
+  group_partials(a, g, zloc, rz, rr, bb, cs, mpart, a.part2 + (size_t)g * (3 + ne));
+}
+
+// Iteration k: sums the per-group partials of the previous update (or of
+// k_defl_init), stops when |r|^2 <= tol2 or k = max_iters, else forms
+// p_k = z - W mu + beta p_{k-1} on the fly, q = S p_k (the own row's p_k is
+// stored) and part[i] = p_k,i . q_i. Thread t < 252 handles component t % 6
+// of every 42nd block of the row.
+__global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a, int32_t k, const double* __restrict__ p_prev,
+                                                  double* __restrict__ p_cur) {
+  __shared__ double chunk[kPartChunk];
+  __shared__ double sums[3 + kMaxCoarse];
+  __shared__ double ysum[kGroupThreads];
+  PcgState* st = a.st;
+  if (st->done) return;
+  const int ne = a.ne, tid = threadIdx.x, stride = 3 + ne;
+  // Fixed-order sum of the group partials, a chunk of whole groups per pass.
+  const int per_chunk = kPartChunk / stride;
+  double acc = 0.0;
+  for (int g0 = 0; g0 < a.groups; g0 += per_chunk) {
+    const int ng = min(per_chunk, a.groups - g0);
+    __syncthreads();
+    for (int q = tid; q < ng * stride; q += blockDim.x) chunk[q] = a.part2[(size_t)g0 * stride + q];
+    __syncthreads();
+    if (tid < stride)
+      for (int g = 0; g < ng; ++g) acc += chunk[g * stride + tid];
+  }
+  if (tid < stride) sums[tid] = acc;
+  __syncthreads();
+  const double* mu_s = sums + 3;
+  const double rz = sums[0], rr = sums[1];
+  double tol2;
+  bool stop;
+  if (k == 0) {
+    const double bb = sums[2];
+    tol2 = st->tol_rel2 * bb;
+    stop = bb <= 0.0 || st->max_iters <= 0 || rr <= tol2;
+    if (blockIdx.x == 0 && tid == 0) {
+      st->bb = bb;
+      st->tol2 = tol2;
+    }
+  } else {
+    tol2 = st->tol2;
+    stop = rr <= tol2 || k >= st->max_iters;
+  }
+  if (stop) {
+    if (blockIdx.x == 0 && tid == 0) {
+      st->iters = k;
+      st->done = 1;
+    }
+    return;
+  }
+  const double beta = k > 0 ? rz / st->rz[(k - 1) & 1] : 0.0;
+  if (blockIdx.x == 0 && tid == 0) {
+    st->rz[k & 1] = rz;
+    st->iters = k;
+  }
   const int i = blockIdx.x;
   const int rb = a.row_ptr[i], re = a.row_ptr[i + 1];
   double y = 0.0;
   if (tid < kGroupThreads) {
     const int comp = tid % 6, g = tid / 6;
-    for (int k = rb + g; k < re; k += kRowsPerGroup) {
-      const int j = a.col_var[k];
-      const double* srow = a.S + (size_t)36 * k + 6 * comp;
+    for (int kb = rb + g; kb < re; kb += kRowsPerGroup) {
+      const int j = a.col_var[kb];
+      const double* srow = a.S + (size_t)36 * kb + 6 * comp;
       const int n0 = a.hat_node[j];
       const double w0 = a.hat_w[2 * j], w1 = a.hat_w[2 * j + 1];
       double s = 0.0;
@@ -294,29 +436,24 @@ __global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a, const double* __res
   if (tid == 0) a.part[i] = ((ysum[0] + ysum[1]) + (ysum[2] + ysum[3])) + (ysum[4] + ysum[5]);
 }
 
-// alpha = r'z / p'q; x += alpha p; r -= alpha q; z = M^-1 r on 42 rows per
-// work-group; partial r'z, r'r and (SW)'z. The last work-group to finish
-// (atomic ticket) sums the partials in group order and sets mu = E^-1 (SW)'z,
-// beta, the iteration count and the convergence flag.
-__global__ __launch_bounds__(256) void k_pcg_update(PcgArgs a, const double* __restrict__ p_cur) {
+// alpha = r'z / p'q; x += alpha p; r -= alpha q; z = M^-1 r on one group of
+// rows per work-group, and the group's partials r'z, r'r, E^-1 (SW)'z.
+__global__ __launch_bounds__(kGroupBlock) void k_pcg_update(PcgArgs a, int32_t k, const double* __restrict__ p_cur) {
   __shared__ double rloc[kGroupThreads];
   __shared__ double zloc[kGroupThreads];
-  __shared__ double lds4[4];
+  __shared__ double lds[48];
   __shared__ double cs[kMaxCoarse];
-  __shared__ double bc[2];
-  __shared__ int is_last;
-  PcgState* st = a.st;
+  __shared__ double mpart[kGroupBlock];
+  const PcgState* st = a.st;
   if (st->done) return;
-  const int ne = a.ne, tid = threadIdx.x, P = a.Pvar;
+  const int ne = a.ne, tid = threadIdx.x, P = a.Pvar, g = blockIdx.x;
   // p'q, summed by every group in the same order.
-  double pq = 0.0;
-  for (int k = tid; k < P; k += blockDim.x) pq += a.part[k];
-  pq = block_sum256(pq, lds4);
-  if (tid == 0) bc[0] = pq;
-  __syncthreads();
-  const double rz_old = st->rz;
-  const double alpha = rz_old / bc[0];
-  const int row = blockIdx.x * kRowsPerGroup + tid / 6, comp = tid % 6;
+  double pq = 0.0, unused0 = 0.0, unused1 = 0.0;
+  for (int r = tid; r < P; r += blockDim.x) pq += a.part[r];
+  for (int c = tid; c < ne; c += blockDim.x) cs[c] = 0.0;
+  block_sum3(pq, unused0, unused1, lds);
+  const double alpha = st->rz[k & 1] / pq;
+  const int row = g * kRowsPerGroup + tid / 6, comp = tid % 6;
   const bool active = tid < kGroupThreads && row < P;
   double rv = 0.0;
   if (active) {
@@ -335,112 +472,85 @@ __global__ __launch_bounds__(256) void k_pcg_update(PcgArgs a, const double* __r
     a.z[6 * row + comp] = zv;
   }
   if (tid < kGroupThreads) zloc[tid] = zv;
-  const double rz_l = block_sum256(rv * zv, lds4);
-  const double rr_l = block_sum256(rv * rv, lds4);
-  double* out = a.part2 + (size_t)blockIdx.x * (2 + ne);
-  if (tid == 0) { out[0] = rz_l; out[1] = rr_l; }
-  if (ne > 0) {
-    // (SW)'z over the windowed rows of this group, all threads, LDS atomics.
-    for (int k = tid; k < ne; k += blockDim.x) cs[k] = 0.0;
-    __syncthreads();
-    const int row0 = blockIdx.x * kRowsPerGroup, sw = a.sw;
-    const int nent = 6 * min(kRowsPerGroup, P - row0);
-    const double* base = a.SR + (size_t)(6 * row0) * sw;
-    for (int el = tid; el < nent * sw; el += blockDim.x) {
-      const int t = el / sw, lc = el % sw;
-      const int gc = 6 * a.sr_lo[row0 + t / 6] + lc;
-      const double v = base[el];
-      if (gc < ne && v != 0.0) atomicAdd(&cs[gc], v * zloc[t]);
-    }
-    __syncthreads();
-    for (int col = tid; col < ne; col += blockDim.x) out[2 + col] = cs[col];
-  }
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) is_last = atomicAdd(&st->counter, 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  const int G = gridDim.x;
-  double rz = 0.0, rr = 0.0;
-  if (tid == 0)
-    for (int g = 0; g < G; ++g) {
-      rz += a.part2[(size_t)g * (2 + ne)];
-      rr += a.part2[(size_t)g * (2 + ne) + 1];
-    }
-  for (int col = tid; col < ne; col += blockDim.x) {
-    double s = 0.0;
-    for (int g = 0; g < G; ++g) s += a.part2[(size_t)g * (2 + ne) + 2 + col];
-    cs[col] = s;
-  }
-  __syncthreads();
-  for (int rrow = tid; rrow < ne; rrow += blockDim.x) {
-    double m = 0.0;
-    for (int c = 0; c < ne; ++c) m += a.Einv[rrow * ne + c] * cs[c];
-    a.mu[rrow] = m;
-  }
-  if (tid == 0) {
-    st->rz = rz;
-    st->rr = rr;
-    st->beta = rz / rz_old;
-    st->iters += 1;
-    st->counter = 0;
-    if (rr <= st->tol2 || st->iters >= st->max_iters) st->done = 1;
-  }
+  double rz = rv * zv, rr = rv * rv, zero = 0.0;
+  block_sum3(rz, rr, zero, lds);
+  group_partials(a, g, zloc, rz, rr, 0.0, cs, mpart, a.part2 + (size_t)g * (3 + ne));
+}
+
+// Clears the convergence flag and lifts the iteration limit so that one more
+// iteration runs (kernel timing only).
+__global__ void k_pcg_reopen(PcgState* st) {
+  st->done = 0;
+  st->max_iters = st->iters + 2;
+  st->tol2 = -1.0;
 }
 
 // ------------------------------------------------------------ launchers
 
 size_t pcg_state_bytes() { return sizeof(PcgState); }
 int32_t pcg_max_coarse() { return kMaxCoarse; }
+int32_t pcg_group_rows() { return kRowsPerGroup; }
 int32_t pcg_update_groups(int32_t Pvar) { return (Pvar + kRowsPerGroup - 1) / kRowsPerGroup; }
 
-// Deflated PCG solve of S y = b. Buffers (device): x, r, z, p0, p1, q, mu
-// (6 Pvar / ne), part (Pvar), part2 (groups x (2 + ne)), SR (6 Pvar x ne),
-// E / Einv (ne x ne). Returns the iteration count or -1 on a launch error.
+// Deflated PCG solve of S y = b. Buffers (device): x, r, z, p0, p1, q
+// (6 Pvar), part (Pvar), part2 (groups x (3 + ne)), SR (groups x gw x 252),
+// E / Einv (ne x ne), y (ne). The host polls the state every `poll`
+// iterations. With `probe` (three events), one more iteration's k_pcg_spmv
+// and k_pcg_update are timed after the solve (x and r then hold one extra
+// step). Returns the iteration count or -1 on a launch error.
 int launch_pcg_deflated(hipStream_t s, int32_t Pvar, int32_t K, const int32_t* row_ptr, const int32_t* col_var,
                         const double* S, const double* Minv, const double* b, const int32_t* hat_node,
-                        const double* hat_w, const int32_t* node_range, const int32_t* sr_lo, int32_t sw,
-                        double* SR, double* E, double* Einv,
-                        double* x, double* r, double* z, double* p0, double* p1, double* q, double* mu,
-                        double* part, double* part2, void* state_dev, void* state_host, int32_t* fail, double tol,
-                        int32_t max_iters, int32_t poll) {
+                        const double* hat_w, const int32_t* node_range, const int32_t* grp_lo, int32_t gw,
+                        double* SR, double* E, double* Einv, double* y,
+                        double* x, double* r, double* z, double* p0, double* p1, double* q,
+                        double* part, double* part2, void* state_dev, void* state_host, double tol,
+                        int32_t max_iters, int32_t poll, const hipEvent_t* probe) {
   if (Pvar <= 0) return 0;
   PcgArgs a;
   a.Pvar = Pvar;
   a.K = K;
   a.ne = 6 * K;
-  a.pad = 0;
+  a.groups = pcg_update_groups(Pvar);
   a.row_ptr = row_ptr; a.col_var = col_var; a.S = S; a.Minv = Minv; a.b = b;
-  a.hat_node = hat_node; a.hat_w = hat_w; a.node_range = node_range; a.sr_lo = sr_lo; a.sw = sw; a.pad2 = 0;
-  a.SR = SR; a.E = E; a.Einv = Einv; a.x = x; a.r = r; a.z = z; a.p0 = p0; a.p1 = p1; a.q = q; a.mu = mu;
-  a.part = part; a.part2 = part2;
+  a.hat_node = hat_node; a.hat_w = hat_w; a.node_range = node_range; a.grp_lo = grp_lo; a.gw = gw; a.pad2 = 0;
+  a.SR = SR; a.E = E; a.Einv = Einv; a.x = x; a.r = r; a.z = z; a.p0 = p0; a.p1 = p1; a.q = q;
+  a.part = part; a.part2 = part2; a.y = y;
   a.st = static_cast<PcgState*>(state_dev);
-  a.fail = fail;
+  const int groups = a.groups;
   if (a.ne > 0) {
     k_defl_sr<<<Pvar, 256, 0, s>>>(a);
     k_defl_e<<<(a.ne * a.ne + 255) / 256, 256, 0, s>>>(a);
-    k_defl_inverse<<<1, 1024, 0, s>>>(a);
   }
-  k_defl_init<<<1, 1024, 0, s>>>(a, tol, max_iters);
-  const int groups = pcg_update_groups(Pvar);
+  k_defl_prep<<<1, kPrepThreads, 0, s>>>(a, tol, max_iters);
+  k_defl_init<<<groups, kGroupBlock, 0, s>>>(a);
   PcgState* hs = static_cast<PcgState*>(state_host);
-  int launched = 0;
-  while (launched < max_iters) {
-    const int n = std::min(poll, max_iters - launched);
-    for (int k = 0; k < n; ++k) {
-      double* p_prev = ((launched + k) & 1) ? p1 : p0;
-      double* p_cur = ((launched + k) & 1) ? p0 : p1;
-      k_pcg_spmv<<<Pvar, 256, 0, s>>>(a, p_prev, p_cur);
-      k_pcg_update<<<groups, 256, 0, s>>>(a, p_cur);
+  int k = 0;
+  while (true) {
+    const int n = std::min(poll, max_iters - k);
+    for (int j = 0; j < n; ++j, ++k) {
+      k_pcg_spmv<<<Pvar, 256, 0, s>>>(a, k, (k & 1) ? p1 : p0, (k & 1) ? p0 : p1);
+      k_pcg_update<<<groups, kGroupBlock, 0, s>>>(a, k, (k & 1) ? p0 : p1);
     }
-    launched += n;
+    // At k = max_iters this launch only records the final state.
+    if (k >= max_iters) k_pcg_spmv<<<Pvar, 256, 0, s>>>(a, k, p0, p1);
     if (hipMemcpyAsync(hs, a.st, sizeof(PcgState), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
     if (hipStreamSynchronize(s) != hipSuccess) return -1;
-    if (hs->done) break;
+    if (hs->done || k >= max_iters) break;
   }
   if (hipGetLastError() != hipSuccess) return -1;
-  return hs->iters;
+  const int iters = hs->iters;
+  if (probe && iters > 0) {
+    // Re-open the state for one timed iteration k = iters.
+    k_pcg_reopen<<<1, 1, 0, s>>>(a.st);
+    const int kk = iters;
+    if (hipEventRecord(probe[0], s) != hipSuccess) return -1;
+    k_pcg_spmv<<<Pvar, 256, 0, s>>>(a, kk, (kk & 1) ? p1 : p0, (kk & 1) ? p0 : p1);
+    if (hipEventRecord(probe[1], s) != hipSuccess) return -1;
+    k_pcg_update<<<groups, kGroupBlock, 0, s>>>(a, kk, (kk & 1) ? p0 : p1);
+    if (hipEventRecord(probe[2], s) != hipSuccess) return -1;
+    if (hipGetLastError() != hipSuccess) return -1;
+  }
+  return iters;
 }
 
 }  // namespace mlba

@@ -44,6 +44,31 @@ def test_struct_layouts_match_header():
     assert abi.Summary.kernel_ms.offset == abi.Summary.iterations.offset + 8
 
 
+@pytest.mark.parametrize("struct, cname, header", [
+    (abi.ProblemDesc, "mlba_problem_desc", "maplab_ba.h"),
+    (abi.Options, "mlba_options", "maplab_ba.h"),
+    (abi.IterationSummary, "mlba_iteration_summary", "maplab_ba.h"),
+    (abi.Summary, "mlba_summary", "maplab_ba.h"),
+    (abi.KernelTimes, "mlba_kernel_times", "maplab_ba.h"),
+    (abi.SynthConfig, "mlba_synth_config", "maplab_ba_synth.h"),
+])
+def test_struct_layout_matches_compiled_header(tmp_path, struct, cname, header):
+    # The C compiler's sizeof / offsetof of every field equals the ctypes mirror.
+    fields = [f[0] for f in struct._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text(
+        "#include <stdio.h>\n#include <stddef.h>\n#include \"%s\"\nint main(void) {\n"
+        "  printf(\"%%zu\\n\", sizeof(%s));\n" % (header, cname)
+        + "".join("  printf(\"%%zu\\n\", offsetof(%s, %s));\n" % (cname, f) for f in fields)
+        + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert out[0] == ctypes.sizeof(struct)
+    assert out[1:] == [getattr(struct, f).offset for f in fields]
+
+
 def test_option_defaults_are_maplab_flags():
     # algorithms/map-optimization/src/solver-options.cc:3-52 and
     # outlier-rejection-solver.cc:9-18.
