@@ -33,9 +33,11 @@ def kernel_table(sm, kt, n_pcg_total):
     the anchor of a landmark with a variable anchor):
       linearize:          N * 196 (36 read + 160 record written) + L * 176
       cost:               N * 36 + L * 32
-      schur_incidences:   n_inc * 360 (160 record read, 144 Q + 48 rhs written,
-                          8 index) + L * 72 (pivot Z and g)
-      schur_blocks:       pairs * 8 (pair list) + n_inc * 336 (Q, Jl/Jo, rhs)
+      schur_factors:      n_inc * 352 (160 record read, 144 W + 48 r0 written)
+                          (per linearization)
+      schur_incidences:   n_inc * 388 (144 W + 48 r0 read, 144 Q + 48 rhs
+                          written, 4 index) + L * 72 (pivot Z and g)
+      schur_blocks:       pairs * 8 (pair list) + n_inc * 336 (Q, W, rhs)
                           + upper blocks * 584 (S_ij and S_ji written, index)
       pcg_spmv:           nnzb * 292 (6x6 block + column index) + P * 6 * 8 * 4
       pcg_update:         P * 6 * 8 * 9 + P * 288 (Minv) + P * 6 * sw * 8 (SW)
@@ -61,7 +63,8 @@ def kernel_table(sm, kt, n_pcg_total):
 
     add("k_linearize", abi.KERNEL_LINEARIZE, N * 196 + L * 176)
     add("k_linearize_cost", abi.KERNEL_COST, N * 36 + L * 32)
-    add("k_schur_incidences", abi.KERNEL_SCHUR_INCIDENCES, n_inc * 360 + L * 72)
+    add("k_schur_factors", abi.KERNEL_SCHUR_FACTORS, n_inc * 352)
+    add("k_schur_incidences", abi.KERNEL_SCHUR_INCIDENCES, n_inc * 388 + L * 72)
     add("k_schur_blocks", abi.KERNEL_SCHUR_BLOCKS, pairs * 8 + n_inc * 336 + nub * 584)
     if "k_schur_blocks" in rows:
         r = rows["k_schur_blocks"]

@@ -169,6 +169,8 @@ typedef struct mlba_iteration_summary {
   double cumulative_time_in_seconds;
 } mlba_iteration_summary;
 
+#define MLBA_NUM_KERNELS 12          /* slots of mlba_summary.kernel_ms / kernel_launches */
+
 typedef struct mlba_summary {
   int32_t termination_type;               /* MLBA_CONVERGENCE ... */
   int32_t num_iterations;                 /* rows written (iteration 0 included) */
@@ -188,8 +190,8 @@ typedef struct mlba_summary {
   mlba_iteration_summary* iterations;     /* caller-owned, nullable */
   /* Device time inside the LM loop, from HIP events on the library's stream:
    * summed milliseconds and launch count per MLBA_KERNEL_* id. */
-  double kernel_ms[8];
-  int32_t kernel_launches[8];
+  double kernel_ms[MLBA_NUM_KERNELS];
+  int32_t kernel_launches[MLBA_NUM_KERNELS];
 } mlba_summary;
 
 /* Kernel ids of mlba_summary.kernel_ms. */
@@ -200,7 +202,8 @@ typedef struct mlba_summary {
 #define MLBA_KERNEL_BACK_SUBSTITUTE 4 /* landmark step + model cost change */
 #define MLBA_KERNEL_POSE_ROWS 5       /* vertex gradient and diag(J'J) */
 #define MLBA_KERNEL_LANDMARK_PREP 6   /* damped landmark pivots */
-#define MLBA_KERNEL_SCHUR_INCIDENCES 7 /* Schur factors of every row incidence */
+#define MLBA_KERNEL_SCHUR_INCIDENCES 7 /* per-iteration Schur factors of every row incidence */
+#define MLBA_KERNEL_SCHUR_FACTORS 8   /* per-linearization Schur factors of every row incidence */
 
 typedef struct mlba_handle mlba_handle;
 
@@ -253,6 +256,28 @@ int32_t mlba_solve(mlba_handle* handle, const mlba_options* options, mlba_summar
 int32_t mlba_solve_with_outlier_rejection(mlba_handle* handle, const mlba_options* options,
                                           mlba_summary* summary);
 
+/* Landmark admission thresholds: maplab's elq_* flags
+ * (map-structure/vi-map/src/landmark-quality-metrics.cc:11-33). */
+typedef struct mlba_quality_options {
+  int32_t min_observers;                     /* elq_min_observers (4) */
+  int32_t reserved0;
+  double min_observation_angle_deg;          /* elq_min_observation_angle_deg (5) */
+  double max_distance_from_closest_observer; /* elq_max_distance_from_closest_observer (40 m) */
+  double min_distance_from_closest_observer; /* elq_min_distance_from_closest_observer (0.05 m) */
+  double max_reprojection_error_px;          /* elq_max_reprojection_error_px (-1: check off) */
+} mlba_quality_options;
+
+void mlba_quality_options_init(mlba_quality_options* options);
+
+/* vi_map::isVisualLandmarkWellConstrained (landmark-quality-metrics.cc:103-171,
+ * with common::getMaxDisparityRadAngleOfUnitVectorBundle, geometry.cc:30-44)
+ * of every landmark at the current device state, over all of its
+ * observations in the problem; good[l] = 1 when the landmark is well
+ * constrained (the admission test of addLandmarkTermsForVertices,
+ * optimization-terms-addition.cc:304-312). Evaluated on the device; the
+ * active mask is not changed (pass the result to mlba_set_landmark_active). */
+int32_t mlba_landmark_quality(mlba_handle* handle, const mlba_quality_options* options, uint8_t* good);
+
 /*
  * Measurement hook (no counterpart in the reference): average device time of
  * each kernel of one LM iteration at the current state, timed with HIP
@@ -264,7 +289,8 @@ typedef struct mlba_kernel_times {
   double cost_ms;             /* residual-only cost (candidate evaluation) */
   double pose_rows_ms;        /* vertex gradient and diag(J'J) */
   double landmark_prep_ms;    /* damped landmark pivots */
-  double schur_incidences_ms; /* Schur factors of every row incidence */
+  double schur_factors_ms;    /* per-linearization Schur factors of every row incidence */
+  double schur_incidences_ms; /* per-iteration Schur factors of every row incidence */
   double schur_blocks_ms;     /* reduced camera system blocks and rhs */
   double block_inverse_ms;    /* block-Jacobi preconditioner */
   double pcg_solve_ms;        /* one whole (deflated) PCG solve at the initial radius */

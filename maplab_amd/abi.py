@@ -77,6 +77,9 @@ class IterationSummary(ctypes.Structure):
     ]
 
 
+NUM_KERNELS = 12  # MLBA_NUM_KERNELS
+
+
 class Summary(ctypes.Structure):
     _fields_ = [
         ("termination_type", c_i32), ("num_iterations", c_i32), ("num_successful_steps", c_i32),
@@ -86,12 +89,12 @@ class Summary(ctypes.Structure):
         ("linear_solver_seconds", c_f64), ("total_linear_solver_iterations", ctypes.c_int64),
         ("iterations_capacity", c_i32), ("reserved0", c_i32),
         ("iterations", ctypes.POINTER(IterationSummary)),
-        ("kernel_ms", c_f64 * 8), ("kernel_launches", c_i32 * 8),
+        ("kernel_ms", c_f64 * NUM_KERNELS), ("kernel_launches", c_i32 * NUM_KERNELS),
     ]
 
 
 KERNEL_LINEARIZE, KERNEL_COST, KERNEL_SCHUR_BLOCKS, KERNEL_PCG, KERNEL_BACK_SUBSTITUTE = 0, 1, 2, 3, 4
-KERNEL_POSE_ROWS, KERNEL_LANDMARK_PREP, KERNEL_SCHUR_INCIDENCES = 5, 6, 7
+KERNEL_POSE_ROWS, KERNEL_LANDMARK_PREP, KERNEL_SCHUR_INCIDENCES, KERNEL_SCHUR_FACTORS = 5, 6, 7, 8
 
 
 class SynthConfig(ctypes.Structure):
@@ -110,11 +113,34 @@ class SynthConfig(ctypes.Structure):
 
 class KernelTimes(ctypes.Structure):
     _fields_ = [(n, c_f64) for n in (
-        "linearize_ms", "cost_ms", "pose_rows_ms", "landmark_prep_ms", "schur_incidences_ms", "schur_blocks_ms",
+        "linearize_ms", "cost_ms", "pose_rows_ms", "landmark_prep_ms", "schur_factors_ms", "schur_incidences_ms",
+        "schur_blocks_ms",
         "block_inverse_ms", "pcg_solve_ms", "pcg_spmv_ms", "pcg_update_ms", "back_substitute_ms")] + [
         (n, ctypes.c_int64) for n in (
             "num_observations", "num_landmarks", "num_variable_vertices", "schur_blocks", "schur_upper_blocks",
             "schur_pairs", "incidences", "linearize_tiles", "pcg_iterations")]
+
+
+class QualityOptions(ctypes.Structure):
+    _fields_ = [("min_observers", c_i32), ("reserved0", c_i32), ("min_observation_angle_deg", c_f64),
+                ("max_distance_from_closest_observer", c_f64), ("min_distance_from_closest_observer", c_f64),
+                ("max_reprojection_error_px", c_f64)]
+
+
+class ViProblemOptions(ctypes.Structure):
+    _fields_ = [("add_visual_constraints", c_i32), ("fix_landmark_positions", c_i32), ("fix_vertices", c_i32),
+                ("min_landmarks_per_frame", c_i32), ("enable_visual_outlier_rejection", c_i32),
+                ("reserved0", c_i32), ("solver", Options), ("landmark_quality", QualityOptions)]
+
+
+class ProblemSelection(ctypes.Structure):
+    _fields_ = [("num_observations", c_i32), ("num_vertices", c_i32), ("num_landmarks", c_i32),
+                ("num_clusters", c_i32), ("obs_in_problem", c_vp), ("vertex_in_problem", c_vp),
+                ("landmark_in_problem", c_vp), ("vertex_constant", c_vp), ("landmark_constant", c_vp),
+                ("gauge_vertex", c_vp), ("gauge_landmark", c_vp)]
+
+
+QUALITY_UNKNOWN, QUALITY_BAD, QUALITY_GOOD, QUALITY_LOCALIZATION_SUMMARY = 0, 1, 2, 3
 
 
 def _declare(lib, name, restype, argtypes):
@@ -139,6 +165,18 @@ LIB_SYMBOLS = {
     "mlba_solve": (c_i32, [c_vp, ctypes.POINTER(Options), ctypes.POINTER(Summary)]),
     "mlba_solve_with_outlier_rejection": (c_i32, [c_vp, ctypes.POINTER(Options), ctypes.POINTER(Summary)]),
     "mlba_time_kernels": (c_i32, [c_vp, ctypes.POINTER(Options), c_i32, ctypes.POINTER(KernelTimes)]),
+    "mlba_quality_options_init": (None, [ctypes.POINTER(QualityOptions)]),
+    "mlba_landmark_quality": (c_i32, [c_vp, ctypes.POINTER(QualityOptions), c_vp]),
+}
+
+# Every symbol include/maplab_ba_optimizer.h declares (exported by the same library).
+OPTIMIZER_SYMBOLS = {
+    "mlba_vi_problem_options_init": (None, [ctypes.POINTER(ViProblemOptions)]),
+    "mlba_select_problem": (c_i32, [ctypes.POINTER(ProblemDesc), ctypes.POINTER(ViProblemOptions), c_vp, c_vp,
+                                    ctypes.POINTER(ProblemSelection)]),
+    "mlba_vi_map_optimize": (c_i32, [ctypes.POINTER(ProblemDesc), ctypes.POINTER(ViProblemOptions), c_vp, c_vp,
+                                     c_vp, c_vp, ctypes.POINTER(Summary)]),
+    "mlba_optimizer_last_error": (ctypes.c_char_p, []),
 }
 
 SYNTH_SYMBOLS = {
@@ -162,7 +200,7 @@ def load_library(path: str = LIB_PATH):
             raise OSError(f"{path} is missing: build it with `make -C maplab_amd/csrc` "
                           "(the visual BA path has no CPU fallback)")
         lib = ctypes.CDLL(path)
-        for name, (res, args) in LIB_SYMBOLS.items():
+        for name, (res, args) in {**LIB_SYMBOLS, **OPTIMIZER_SYMBOLS}.items():
             _declare(lib, name, res, args)
         _lib = lib
     return _lib
@@ -178,6 +216,13 @@ def load_synth(path: str = SYNTH_PATH):
             _declare(lib, name, res, args)
         _synth = lib
     return _synth
+
+
+def default_quality_options() -> QualityOptions:
+    """maplab's elq_* flag defaults (mlba_quality_options_init)."""
+    o = QualityOptions()
+    load_library().mlba_quality_options_init(ctypes.byref(o))
+    return o
 
 
 def default_options() -> Options:

@@ -31,22 +31,27 @@ void launch_linearize(hipStream_t s, bool jac, int32_t tiles, const DevTopology&
                       double* lm_acc, double* tile_cost);
 void launch_zero_split(hipStream_t s, int32_t n, const int32_t* ids, double* lm_acc);
 void launch_pose_rows(hipStream_t s, const DevTopology& t, const double* vert, const double* vR, const double* lm,
-                      const double* obs_rec, const double* lm_acc, double* pose_grad, double* pose_diag);
+                      const double* obs_rec, const double* lm_acc, double* pose_grad, double* pose_diag,
+                      double* pose_jtj);
 void launch_jacobi_scaling(hipStream_t s, int32_t Pvar, int32_t L, int32_t use, const double* pose_diag,
                            const double* lm_acc, double* pose_scale, double* lm_scale);
 void launch_landmark_prep(hipStream_t s, const DevTopology& t, const double* lm_acc, const double* lm_scale,
                           double inv_mu, double min_diag, double max_diag, double* lm_z, int32_t* fail);
-void launch_schur_incidences(hipStream_t s, const DevTopology& t, int32_t n_inc, const int32_t* inc_row,
-                             const double* vR, const double* lm, const double* obs_rec, const double* lm_acc,
-                             const double* lm_z, const double* pose_scale, double* inc_q, double* inc_rhs,
-                             double* lm_wa);
+void launch_schur_factors(hipStream_t s, const DevTopology& t, int32_t n_inc, const int32_t* inc_row,
+                          const double* vR, const double* lm, const double* obs_rec, const double* lm_acc,
+                          const double* pose_scale, double* inc_w, double* inc_r0, double* lm_a);
+void launch_schur_incidences(hipStream_t s, const DevTopology& t, int32_t n_inc, const double* inc_w,
+                             const double* inc_r0, const double* lm_a, const double* lm_acc, const double* lm_z,
+                             double* inc_q, double* inc_rhs);
 void launch_schur_blocks(hipStream_t s, const DevTopology& t, int32_t nub, const int4* ublk, int32_t n_items,
-                         const int4* items, const int32_t* item_ptr, const uint2* pairs, const double* obs_rec,
-                         const double* inc_q, const double* inc_rhs, const double* lm_wa, const double* pose_scale,
-                         const double* pose_diag, double inv_mu, double min_diag, double max_diag, double* partial,
-                         double* S, double* rhs);
+                         const int4* items, const int32_t* item_ptr, const uint2* pairs, const double* inc_q,
+                         const double* inc_w, const double* inc_rhs, const double* lm_a, const double* pose_scale,
+                         const double* pose_diag, const double* pose_jtj, double inv_mu, double min_diag,
+                         double max_diag, double* partial, double* S, double* rhs);
 uint32_t schur_pair_anchor_bit();
-uint32_t schur_pair_same_bit();
+void launch_landmark_quality(hipStream_t s, const DevTopology& t, const double* vert, const double* vR,
+                             const double* lm, int32_t min_observers, double min_angle_deg, double max_dist,
+                             double min_dist, double max_reproj_px, uint8_t* good);
 int32_t schur_item_pairs();
 void launch_block_inverse(hipStream_t s, int32_t Pvar, const int32_t* diag_pos, const double* S, double* Minv,
                           int32_t* fail);
@@ -146,14 +151,15 @@ struct mlba_handle {
   DBuf<int32_t> inc_row, item_ptr;      // row of each incidence; work-item range of each upper block
   DBuf<int4> ublk, items;               // upper block (position of S_ij, of S_ji, i, j); item (block, pairs)
   DBuf<uint2> pairs;                    // incidence pairs (x in row i, y in row j) of each upper block
-  DBuf<double> inc_q, inc_rhs, lm_wa;   // Schur factors per incidence / anchored landmark
+  DBuf<double> inc_w, inc_r0, lm_a;     // per-linearization Schur factors (incidence order) and anchor maps
+  DBuf<double> inc_q, inc_rhs;          // per-iteration Schur factors
   DBuf<double> schur_partial;           // 36 per work item
   DBuf<int32_t> hat_node, node_range, grp_lo;
   std::vector<int32_t> h_row_ptr, h_col_var;  // host copy of the S block structure
   int32_t grp_width = 0;                // columns of the per-group SW window
   DBuf<double> hat_w, defl_SR, defl_E, defl_Einv, defl_y, pcg_part2;
   int32_t hat_K = -1;  // coarse nodes of the current hat basis
-  DBuf<double> S, rhs, pose_grad, pose_diag, pose_scale, w, Minv;
+  DBuf<double> S, rhs, pose_grad, pose_diag, pose_jtj, pose_scale, w, Minv;
   DBuf<double> px, pr, pz, pp, pp2, pq, pcg_part, parts_a, parts_b, parts_c, parts_l, parts_v, parts_w, parts_g;
   DBuf<double> scal;
   DBuf<int32_t> fail;
@@ -330,9 +336,7 @@ int build_topology(mlba_handle* h) {
   std::vector<int32_t> pair_ptr{0};
   std::vector<uint2> pairs;
   {
-    const uint32_t abit = schur_pair_anchor_bit(), sbit = schur_pair_same_bit();
-    if (h->N >= (int)sbit || L >= (int)sbit || (int64_t)inc.size() >= (int64_t)sbit)
-      return set_err(h, MLBA_EINVAL, "problem too large for the Schur pair encoding");
+    const uint32_t abit = schur_pair_anchor_bit();
     std::vector<int32_t> upos(Pvar, -1), cnt;
     std::vector<int64_t> fill;
     int64_t total = 0;
@@ -374,8 +378,7 @@ int build_topology(mlba_handle* h) {
         for (int m = lm_inc_ptr[l]; m < lm_inc_ptr[l + 1]; ++m) {
           const int y = lm_inc[m];
           if (inc_row[y] < i) continue;
-          const uint32_t ye = inc[y] < 0 ? ((uint32_t)l | abit) : ((uint32_t)inc[y] | (y == x ? sbit : 0u));
-          pairs[(size_t)fill[upos[inc_row[y]] - base]++] = make_uint2(xe, ye);
+          pairs[(size_t)fill[upos[inc_row[y]] - base]++] = make_uint2(xe, (uint32_t)y | (inc[y] < 0 ? abit : 0u));
         }
       }
     }
@@ -419,7 +422,9 @@ int build_topology(mlba_handle* h) {
   const size_t vb = (Pvar + 255) / 256 + 1;
   bool ok = h->tile_cost.alloc(2 * T) && h->tile_cost2.alloc(2 * T) && h->S.alloc(36 * (size_t)h->nnzb) &&
             h->inc_q.alloc(18 * std::max<size_t>(inc.size(), 1)) && h->inc_rhs.alloc(6 * std::max<size_t>(inc.size(), 1)) &&
-            h->lm_wa.alloc(36 * (size_t)std::max(L, 1)) && h->schur_partial.alloc(36 * std::max<size_t>(items.size(), 1)) &&
+            h->inc_w.alloc(18 * std::max<size_t>(inc.size(), 1)) && h->inc_r0.alloc(6 * std::max<size_t>(inc.size(), 1)) &&
+            h->lm_a.alloc(18 * (size_t)std::max(L, 1)) && h->schur_partial.alloc(36 * std::max<size_t>(items.size(), 1)) &&
+            h->pose_jtj.alloc(36 * (size_t)std::max(Pvar, 1)) &&
             h->rhs.alloc(V6) && h->pose_grad.alloc(V6) && h->pose_diag.alloc(V6) && h->pose_scale.alloc(V6) &&
             h->w.alloc(V6) && h->Minv.alloc(36 * (size_t)std::max(Pvar, 1)) && h->px.alloc(V6) &&
             h->pr.alloc(V6) && h->pz.alloc(V6) && h->pp.alloc(V6) && h->pp2.alloc(V6) && h->pq.alloc(V6) &&
@@ -558,6 +563,14 @@ DevLoss make_loss(const mlba_options* o) {
 // Linearize at the current state: residual records, landmark reductions,
 // vertex-row gradient and diag(F'F). Leaves cost / fixed cost / gradient
 // max norm in the scalar slots (device).
+// Per-linearization Schur factors (W, r0 of every incidence, anchor maps);
+// needs the Jacobi scaling of the solve.
+void schur_factors(mlba_handle* h) {
+  KernelTimer kt(h, MLBA_KERNEL_SCHUR_FACTORS);
+  launch_schur_factors(h->stream, h->topo, h->n_inc, h->inc_row.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p,
+                       h->pose_scale.p, h->inc_w.p, h->inc_r0.p, h->lm_a.p);
+}
+
 int linearize(mlba_handle* h, const mlba_options* o) {
   hipStream_t s = h->stream;
   launch_vertex_prep(s, h->P, h->vert.p, h->vR.p);
@@ -570,7 +583,7 @@ int linearize(mlba_handle* h, const mlba_options* o) {
   {
     KernelTimer kt(h, MLBA_KERNEL_POSE_ROWS);
     launch_pose_rows(s, h->topo, h->vert.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p, h->pose_grad.p,
-                     h->pose_diag.p);
+                     h->pose_diag.p, h->pose_jtj.p);
   }
   launch_reduce(s, h->tiles, 2, 0, false, h->tile_cost.p, h->scal.p + kCost);
   launch_reduce(s, h->tiles, 2, 1, false, h->tile_cost.p, h->scal.p + kFixed);
@@ -606,6 +619,7 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
   if ((rc = linearize(h, o))) return rc;
   launch_jacobi_scaling(s, h->Pvar, h->L, o->use_jacobi_scaling, h->pose_diag.p, h->lm_acc.p, h->pose_scale.p,
                         h->lm_scale.p);
+  schur_factors(h);
   if ((rc = read_scalars(h))) return rc;
   if (sm) sm->evaluate_seconds += now_s() - t_ev0;
   double x_cost = h->scal_host[kCost];
@@ -645,14 +659,15 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
       }
       {
         KernelTimer kt(h, MLBA_KERNEL_SCHUR_INCIDENCES);
-        launch_schur_incidences(s, h->topo, h->n_inc, h->inc_row.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p,
-                                h->lm_z.p, h->pose_scale.p, h->inc_q.p, h->inc_rhs.p, h->lm_wa.p);
+        launch_schur_incidences(s, h->topo, h->n_inc, h->inc_w.p, h->inc_r0.p, h->lm_a.p, h->lm_acc.p, h->lm_z.p,
+                                h->inc_q.p, h->inc_rhs.p);
       }
       {
         KernelTimer kt(h, MLBA_KERNEL_SCHUR_BLOCKS);
         launch_schur_blocks(s, h->topo, h->n_upper, h->ublk.p, h->n_items, h->items.p, h->item_ptr.p, h->pairs.p,
-                            h->obs_rec.p, h->inc_q.p, h->inc_rhs.p, h->lm_wa.p, h->pose_scale.p, h->pose_diag.p,
-                            inv_mu, o->min_lm_diagonal, o->max_lm_diagonal, h->schur_partial.p, h->S.p, h->rhs.p);
+                            h->inc_q.p, h->inc_w.p, h->inc_rhs.p, h->lm_a.p, h->pose_scale.p, h->pose_diag.p,
+                            h->pose_jtj.p, inv_mu, o->min_lm_diagonal, o->max_lm_diagonal, h->schur_partial.p,
+                            h->S.p, h->rhs.p);
       }
       launch_block_inverse(s, h->Pvar, h->diag_pos.p, h->S.p, h->Minv.p, h->fail.p);
       HIP_OK(hipGetLastError());
@@ -733,6 +748,7 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
           std::swap(h->lm.p, h->lm_cand.p);
           const double t_ev = now_s();
           if ((rc = linearize(h, o))) return rc;
+          schur_factors(h);
           if ((rc = read_scalars(h))) return rc;
           if (sm) sm->evaluate_seconds += now_s() - t_ev;
           x_cost = h->scal_host[kCost];
@@ -776,7 +792,7 @@ done:
 
 void reset_summary(mlba_summary* sm) {
   if (!sm) return;
-  for (int k = 0; k < 8; ++k) { sm->kernel_ms[k] = 0.0; sm->kernel_launches[k] = 0; }
+  for (int k = 0; k < MLBA_NUM_KERNELS; ++k) { sm->kernel_ms[k] = 0.0; sm->kernel_launches[k] = 0; }
   sm->num_iterations = 0;
   sm->num_successful_steps = sm->num_unsuccessful_steps = 0;
   sm->num_removed_landmarks = 0;
@@ -1004,6 +1020,33 @@ int32_t mlba_get_landmark_active(const mlba_handle* h, uint8_t* mask) {
   return MLBA_OK;
 }
 
+void mlba_quality_options_init(mlba_quality_options* o) {
+  if (!o) return;
+  o->min_observers = 4;
+  o->reserved0 = 0;
+  o->min_observation_angle_deg = 5.0;
+  o->max_distance_from_closest_observer = 40.0;
+  o->min_distance_from_closest_observer = 0.05;
+  o->max_reprojection_error_px = -1.0;
+}
+
+int32_t mlba_landmark_quality(mlba_handle* h, const mlba_quality_options* o, uint8_t* good) {
+  if (!h || !o || !good) return h ? set_err(h, MLBA_EINVAL, "null argument") : MLBA_EINVAL;
+  if (o->min_observers < 0) return set_err(h, MLBA_EINVAL, "min_observers must be >= 0");
+  if (h->L == 0) return MLBA_OK;
+  hipStream_t s = h->stream;
+  DBuf<uint8_t> d;
+  if (!d.alloc(h->L)) return set_err(h, MLBA_ENOMEM, "device allocation failed");
+  launch_vertex_prep(s, h->P, h->vert.p, h->vR.p);
+  launch_landmark_quality(s, h->topo, h->vert.p, h->vR.p, h->lm.p, o->min_observers, o->min_observation_angle_deg,
+                          o->max_distance_from_closest_observer, o->min_distance_from_closest_observer,
+                          o->max_reprojection_error_px, d.p);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(good, d.p, h->L, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  return MLBA_OK;
+}
+
 int32_t mlba_evaluate(mlba_handle* h, int32_t flags, const mlba_options* options, double* residuals,
                       double* const* jacobians, double* cost) {
   if (!h) return MLBA_EINVAL;
@@ -1208,6 +1251,7 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
   if ((rc = linearize(h, o))) return rc;
   launch_jacobi_scaling(s, h->Pvar, h->L, o->use_jacobi_scaling, h->pose_diag.p, h->lm_acc.p, h->pose_scale.p,
                         h->lm_scale.p);
+  t->schur_factors_ms = timed([&] { schur_factors(h); });
   launch_vertex_prep(s, h->P, h->vert.p, h->vR.p);
   t->linearize_ms = timed([&] {
     launch_zero_split(s, h->n_split, h->split_ids.p, h->lm_acc.p);
@@ -1220,7 +1264,7 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
   });
   t->pose_rows_ms = timed([&] {
     launch_pose_rows(s, h->topo, h->vert.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p, h->pose_grad.p,
-                     h->pose_diag.p);
+                     h->pose_diag.p, h->pose_jtj.p);
   });
   HIP_OK(hipMemsetAsync(h->fail.p, 0, sizeof(int32_t), s));
   t->landmark_prep_ms = timed([&] {
@@ -1228,13 +1272,14 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
                          h->lm_z.p, h->fail.p);
   });
   t->schur_incidences_ms = timed([&] {
-    launch_schur_incidences(s, h->topo, h->n_inc, h->inc_row.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p,
-                            h->lm_z.p, h->pose_scale.p, h->inc_q.p, h->inc_rhs.p, h->lm_wa.p);
+    launch_schur_incidences(s, h->topo, h->n_inc, h->inc_w.p, h->inc_r0.p, h->lm_a.p, h->lm_acc.p, h->lm_z.p,
+                            h->inc_q.p, h->inc_rhs.p);
   });
   t->schur_blocks_ms = timed([&] {
     launch_schur_blocks(s, h->topo, h->n_upper, h->ublk.p, h->n_items, h->items.p, h->item_ptr.p, h->pairs.p,
-                        h->obs_rec.p, h->inc_q.p, h->inc_rhs.p, h->lm_wa.p, h->pose_scale.p, h->pose_diag.p, inv_mu,
-                        o->min_lm_diagonal, o->max_lm_diagonal, h->schur_partial.p, h->S.p, h->rhs.p);
+                        h->inc_q.p, h->inc_w.p, h->inc_rhs.p, h->lm_a.p, h->pose_scale.p, h->pose_diag.p,
+                        h->pose_jtj.p, inv_mu, o->min_lm_diagonal, o->max_lm_diagonal, h->schur_partial.p, h->S.p,
+                        h->rhs.p);
   });
   t->block_inverse_ms = timed([&] { launch_block_inverse(s, h->Pvar, h->diag_pos.p, h->S.p, h->Minv.p, h->fail.p); });
   if (h->Pvar > 0) {

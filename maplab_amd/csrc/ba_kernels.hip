@@ -6,9 +6,12 @@
 //                     correction (VisualReprojectionError::Evaluate + ceres
 //                     Corrector); per landmark: V, g reductions in LDS
 //   k_linearize<0>    the same chain, cost only (candidate evaluation)
-//   k_pose_rows       per variable vertex: unscaled gradient and diag(F'F)
+//   k_pose_rows       per variable vertex: unscaled gradient, diag(F'F) and
+//                     the direct observation block sum Jo^T Jo
 //   k_landmark_prep   per landmark: Z = C (C V C + D^2)^-1 C (Schur pivot)
-//   k_schur_incidences per row incidence: the Schur factors Q_x and rhs_x
+//   k_schur_factors   per row incidence, per linearization: W_x, r0_x (and
+//                     the anchor maps A_l)
+//   k_schur_incidences per row incidence, per LM iteration: Q_x and rhs_x
 //   k_schur_items     partial sums of the upper blocks of the reduced camera
 //                     system S over work items of <= 512 incidence pairs
 //   k_schur_finalize  S blocks from the item partials, both triangles
@@ -322,11 +325,14 @@ __global__ __launch_bounds__(kRowThreads) void k_pose_rows(DevTopology t, const 
                                                             const double* __restrict__ obs_rec,
                                                             const double* __restrict__ lm_acc,
                                                             double* __restrict__ pose_grad,
-                                                            double* __restrict__ pose_diag) {
-  __shared__ double red[kRowThreads / 64][12];
+                                                            double* __restrict__ pose_diag,
+                                                            double* __restrict__ pose_jtj) {
+  constexpr int kVals = 12 + 21;  // gradient, diagonal, upper triangle of sum Jo^T Jo
+  __shared__ double red[kRowThreads / 64][kVals];
   const int i = blockIdx.x;
   const int ib = t.inc_ptr[i], ie = t.inc_ptr[i + 1];
-  double g[6] = {0, 0, 0, 0, 0, 0}, d[6] = {0, 0, 0, 0, 0, 0};
+  double g[6] = {0, 0, 0, 0, 0, 0}, d[6] = {0, 0, 0, 0, 0, 0}, jj[21];
+  for (int k = 0; k < 21; ++k) jj[k] = 0.0;
   for (int k = ib + threadIdx.x; k < ie; k += kRowThreads) {
     const int32_t x = t.inc[k];
     if (x >= 0) {
@@ -336,6 +342,9 @@ __global__ __launch_bounds__(kRowThreads) void k_pose_rows(DevTopology t, const 
         g[c] += Jo[c] * rec[0] + Jo[6 + c] * rec[1];
         d[c] += Jo[c] * Jo[c] + Jo[6 + c] * Jo[6 + c];
       }
+      int u = 0;
+      for (int r = 0; r < 6; ++r)
+        for (int c = r; c < 6; ++c, ++u) jj[u] += Jo[r] * Jo[c] + Jo[6 + r] * Jo[6 + c];
     } else {
       const int32_t l = -1 - x;
       const int32_t a = t.lm_anchor[l];
@@ -357,14 +366,23 @@ __global__ __launch_bounds__(kRowThreads) void k_pose_rows(DevTopology t, const 
     g[c] = wave_sum(g[c]);
     d[c] = wave_sum(d[c]);
   }
-  if (lane == 0)
+  for (int k = 0; k < 21; ++k) jj[k] = wave_sum(jj[k]);
+  if (lane == 0) {
     for (int c = 0; c < 6; ++c) { red[wave][c] = g[c]; red[wave][6 + c] = d[c]; }
+    for (int k = 0; k < 21; ++k) red[wave][12 + k] = jj[k];
+  }
   __syncthreads();
   if (threadIdx.x < 12) {
     double s = 0.0;
     for (int w = 0; w < kRowThreads / 64; ++w) s += red[w][threadIdx.x];
     if (threadIdx.x < 6) pose_grad[6 * i + threadIdx.x] = s;
     else pose_diag[6 * i + threadIdx.x - 6] = s;
+  } else if (threadIdx.x >= 64 && threadIdx.x < 100) {
+    const int e = threadIdx.x - 64, r = min(e / 6, e % 6), c = max(e / 6, e % 6);
+    const int u = r * 6 - r * (r - 1) / 2 + (c - r);  // index of (r, c), r <= c, in the upper triangle
+    double s = 0.0;
+    for (int w = 0; w < kRowThreads / 64; ++w) s += red[w][12 + u];
+    pose_jtj[36 * i + e] = s;
   }
 }
 
@@ -429,35 +447,34 @@ __device__ inline void wprime_obs(const double* rec, const double* cs, double* W
 // incidence x of variable vertex i is an observation by i whose residual
 // depends on the pose of i, or the anchor of a landmark l whose anchor
 // vertex i is variable (A_l = M_l C_i, the anchor map of the landmark):
-//   observation: W_x = Jl^T Jo C_i, Q_x = -W_x^T Z_l,
-//                rhs_x = C_i Jo^T r - W_x^T Z_l g_l
-//   anchor:      W_x = V_LM,l A_l,  Q_x = A_l^T - W_x^T Z_l,
-//                rhs_x = A_l^T g_LM,l - W_x^T Z_l g_l
+//   observation: W_x = Jl^T Jo C_i,  r0_x = C_i Jo^T r
+//   anchor:      W_x = V_LM,l A_l,   r0_x = A_l^T g_LM,l
+// and per LM iteration (Z_l depends on the damping)
+//   Q_x = [anchor: A_l^T] - W_x^T Z_l,  rhs_x = r0_x - W_x^T Z_l g_l
 // (ceres' Schur eliminator, schur_eliminator_impl.h, restated on the
-// anchor-lifted blocks). One thread per incidence; anchors also store
-// [W_x | A_l] per landmark for the block kernel.
-__global__ __launch_bounds__(256) void k_schur_incidences(
+// anchor-lifted blocks). W and r0 change only with the linearization and
+// are stored in incidence order (rows of S, then landmarks), so the block
+// kernel streams them; A_l is stored per landmark.
+__global__ __launch_bounds__(256) void k_schur_factors(
     DevTopology t, int32_t n_inc, const int32_t* __restrict__ inc_row, const double* __restrict__ vR,
     const double* __restrict__ lm, const double* __restrict__ obs_rec, const double* __restrict__ lm_acc,
-    const double* __restrict__ lm_z, const double* __restrict__ pose_scale, double* __restrict__ inc_q,
-    double* __restrict__ inc_rhs, double* __restrict__ lm_wa) {
+    const double* __restrict__ pose_scale, double* __restrict__ inc_w, double* __restrict__ inc_r0,
+    double* __restrict__ lm_a) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n_inc) return;
   const int i = inc_row[p];
   const int32_t x = t.inc[p];
-  const int32_t l = x >= 0 ? t.obs_landmark[x] : -1 - x;
   const double* cs = pose_scale + 6 * i;
-  const double* acc_l = lm_acc + (size_t)kLmAcc * l;
-  double W[18], Z[9], rx[6], Qx[18];
-  sym3_full(lm_z + 6 * l, Z);
+  double W[18], r0[6];
   if (x >= 0) {
     const double* rec = obs_rec + (size_t)kObsRecord * x;
     wprime_obs(rec, cs, W);
     const double* Jo = rec + 8;
-    for (int c = 0; c < 6; ++c) rx[c] = cs[c] * (Jo[c] * rec[0] + Jo[6 + c] * rec[1]);
-    for (int k = 0; k < 18; ++k) Qx[k] = 0.0;
+    for (int c = 0; c < 6; ++c) r0[c] = cs[c] * (Jo[c] * rec[0] + Jo[6 + c] * rec[1]);
   } else {
+    const int32_t l = -1 - x;
     const int v = t.var_vertex[i];
+    const double* acc_l = lm_acc + (size_t)kLmAcc * l;
     double M[18], A[18], VLM[9];
     anchor_map(lm + 3 * l, vR + 12 * v, vR[12 * v + 9], M);
     for (int a = 0; a < 3; ++a)
@@ -467,17 +484,37 @@ __global__ __launch_bounds__(256) void k_schur_incidences(
       for (int c = 0; c < 6; ++c)
         W[a * 6 + c] = VLM[a * 3] * A[c] + VLM[a * 3 + 1] * A[6 + c] + VLM[a * 3 + 2] * A[12 + c];
     const double* gLM = acc_l + 15;
-    for (int c = 0; c < 6; ++c) {
-      rx[c] = A[c] * gLM[0] + A[6 + c] * gLM[1] + A[12 + c] * gLM[2];
-      for (int a = 0; a < 3; ++a) Qx[c * 3 + a] = A[a * 6 + c];
-    }
-    double* wa = lm_wa + (size_t)36 * l;
-    for (int k = 0; k < 18; ++k) {
-      wa[k] = W[k];
-      wa[18 + k] = A[k];
-    }
+    for (int c = 0; c < 6; ++c) r0[c] = A[c] * gLM[0] + A[6 + c] * gLM[1] + A[12 + c] * gLM[2];
+    double* out = lm_a + (size_t)18 * l;
+    for (int k = 0; k < 18; ++k) out[k] = A[k];
   }
-  const double* gl = acc_l + 12;
+  double* w = inc_w + (size_t)18 * p;
+  for (int k = 0; k < 18; ++k) w[k] = W[k];
+  double* rr = inc_r0 + (size_t)6 * p;
+  for (int c = 0; c < 6; ++c) rr[c] = r0[c];
+}
+
+// Per LM iteration: Q_x and rhs_x of every incidence (one thread each).
+__global__ __launch_bounds__(256) void k_schur_incidences(
+    DevTopology t, int32_t n_inc, const double* __restrict__ inc_w, const double* __restrict__ inc_r0,
+    const double* __restrict__ lm_a, const double* __restrict__ lm_acc, const double* __restrict__ lm_z,
+    double* __restrict__ inc_q, double* __restrict__ inc_rhs) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_inc) return;
+  const int32_t x = t.inc[p];
+  const int32_t l = x >= 0 ? t.obs_landmark[x] : -1 - x;
+  const double* W = inc_w + (size_t)18 * p;
+  double Z[9], Qx[18], rx[6];
+  sym3_full(lm_z + 6 * l, Z);
+  for (int c = 0; c < 6; ++c) rx[c] = inc_r0[(size_t)6 * p + c];
+  if (x >= 0) {
+    for (int k = 0; k < 18; ++k) Qx[k] = 0.0;
+  } else {
+    const double* A = lm_a + (size_t)18 * l;
+    for (int c = 0; c < 6; ++c)
+      for (int a = 0; a < 3; ++a) Qx[c * 3 + a] = A[a * 6 + c];
+  }
+  const double* gl = lm_acc + (size_t)kLmAcc * l + 12;
   for (int c = 0; c < 6; ++c) {
     double wz[3];
     for (int a = 0; a < 3; ++a) wz[a] = W[c] * Z[a] + W[6 + c] * Z[3 + a] + W[12 + c] * Z[6 + a];
@@ -492,23 +529,21 @@ __global__ __launch_bounds__(256) void k_schur_incidences(
 
 constexpr int kItemWaves = 4;      // work items per work-group (one per wave)
 constexpr int kItemPairs = 512;    // incidence pairs per work item (16 per wave pass)
-constexpr uint32_t kPairAnchor = 0x80000000u;  // x or y is the anchor incidence of its landmark
-constexpr uint32_t kPairSame = 0x40000000u;    // y is the observation of x itself
+constexpr uint32_t kPairAnchor = 0x80000000u;  // the incidence is the anchor of its landmark
 
 // Partial sums of S blocks over work items = (upper block u, a range of at
 // most kItemPairs of its incidence pairs), one wave per item:
 //   sum over pairs (x in row i, y in row j, same landmark) of
-//     Q_x W_y + [y = x] C_i Jo^T Jo C_i + [x observation, y anchor] W_x^T A_l.
+//     Q_x W_y + [x observation, y anchor] W_x^T A_l.
 // Four lanes share a pair, lane quadrant q owning rows 3 (q >> 1) .. +2 and
 // columns 3 (q & 1) .. +2 of the 6x6 block; the 16 lanes of a quadrant are
-// summed in a fixed butterfly order. Pair encoding: x = row incidence
-// (| kPairAnchor); y = observation index (| kPairSame) or landmark index |
-// kPairAnchor. Work-groups are mapped so that each XCD walks a contiguous
-// range of items (block rows), whose incidence factors stay in its L2.
+// summed in a fixed butterfly order. Pair entries are incidence indices
+// (| kPairAnchor). Work-groups are mapped so that each XCD walks a
+// contiguous range of items (block rows), whose factors stay in its L2.
 __global__ __launch_bounds__(64 * kItemWaves) void k_schur_items(
-    DevTopology t, int32_t n_items, const int4* __restrict__ items, const int4* __restrict__ ublk,
-    const uint2* __restrict__ pairs, const double* __restrict__ obs_rec, const double* __restrict__ inc_q,
-    const double* __restrict__ lm_wa, const double* __restrict__ pose_scale, double* __restrict__ partial) {
+    DevTopology t, int32_t n_items, const int4* __restrict__ items, const uint2* __restrict__ pairs,
+    const double* __restrict__ inc_q, const double* __restrict__ inc_w, const double* __restrict__ lm_a,
+    double* __restrict__ partial) {
   const int per_xcd = gridDim.x / 8;
   const int wg = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
   const int it = wg * kItemWaves + (threadIdx.x >> 6);
@@ -516,60 +551,27 @@ __global__ __launch_bounds__(64 * kItemWaves) void k_schur_items(
   const int lane = threadIdx.x & 63, slot = lane >> 2, quad = lane & 3;
   const int r0 = 3 * (quad >> 1), c0 = 3 * (quad & 1);
   const int4 item = items[it];  // (u, first pair, end pair)
-  const int4 ub = ublk[item.x];
-  const int i = ub.z, j = ub.w;
-  double csr[3], csc[3];
-  for (int d = 0; d < 3; ++d) {
-    csr[d] = pose_scale[6 * i + r0 + d];
-    csc[d] = pose_scale[6 * j + c0 + d];
-  }
   double acc[9];
   for (int k = 0; k < 9; ++k) acc[k] = 0.0;
   for (int p = item.y + slot; p < item.z; p += 16) {
     const uint2 pr = pairs[p];
-    const int32_t x = (int32_t)(pr.x & ~kPairAnchor);
-    const bool xa = (pr.x & kPairAnchor) != 0, ya = (pr.y & kPairAnchor) != 0;
-    const bool same = (pr.y & kPairSame) != 0 && !ya;
-    const int32_t yv = (int32_t)(pr.y & ~(kPairAnchor | kPairSame));
+    const int32_t x = (int32_t)(pr.x & ~kPairAnchor), y = (int32_t)(pr.y & ~kPairAnchor);
     const double* Q = inc_q + (size_t)18 * x + 3 * r0;
+    const double* Wy = inc_w + (size_t)18 * y + c0;
     double q[9], W[9];
     for (int k = 0; k < 9; ++k) q[k] = Q[k];
-    double jo[6];  // Jo rows 0 and 1 at the quadrant's columns (observation y)
-    if (!ya) {
-      const double* rec = obs_rec + (size_t)kObsRecord * yv;
-      double jl[6];
-      for (int k = 0; k < 6; ++k) jl[k] = rec[2 + k];
-      for (int d = 0; d < 3; ++d) {
-        jo[d] = rec[8 + c0 + d];
-        jo[3 + d] = rec[14 + c0 + d];
-      }
-      for (int aa = 0; aa < 3; ++aa)
-        for (int d = 0; d < 3; ++d) W[aa * 3 + d] = (jl[aa] * jo[d] + jl[3 + aa] * jo[3 + d]) * csc[d];
-    } else {
-      const double* wa = lm_wa + (size_t)36 * yv + c0;
-      for (int aa = 0; aa < 3; ++aa)
-        for (int d = 0; d < 3; ++d) W[aa * 3 + d] = wa[aa * 6 + d];
-    }
+    for (int aa = 0; aa < 3; ++aa)
+      for (int d = 0; d < 3; ++d) W[aa * 3 + d] = Wy[aa * 6 + d];
     for (int rr = 0; rr < 3; ++rr)
       for (int d = 0; d < 3; ++d)
         acc[rr * 3 + d] += q[rr * 3] * W[d] + q[rr * 3 + 1] * W[3 + d] + q[rr * 3 + 2] * W[6 + d];
-    if (ya && !xa) {
+    if ((pr.y & kPairAnchor) && !(pr.x & kPairAnchor)) {
       // direct observer-anchor coupling W_x^T A_l
-      const double* rec = obs_rec + (size_t)kObsRecord * t.inc[x];
-      const double* A = lm_wa + (size_t)36 * yv + 18 + c0;
-      double wx[9];  // W_x at the quadrant's rows: wx[a][rr]
-      for (int aa = 0; aa < 3; ++aa)
-        for (int rr = 0; rr < 3; ++rr)
-          wx[aa * 3 + rr] = (rec[2 + aa] * rec[8 + r0 + rr] + rec[5 + aa] * rec[14 + r0 + rr]) * csr[rr];
+      const double* A = lm_a + (size_t)18 * (-1 - t.inc[y]) + c0;
+      const double* Wx = inc_w + (size_t)18 * x + r0;
       for (int rr = 0; rr < 3; ++rr)
         for (int d = 0; d < 3; ++d)
-          acc[rr * 3 + d] += wx[rr] * A[d] + wx[3 + rr] * A[6 + d] + wx[6 + rr] * A[12 + d];
-    } else if (same) {
-      const double* rec = obs_rec + (size_t)kObsRecord * yv;
-      for (int rr = 0; rr < 3; ++rr) {
-        const double o0 = rec[8 + r0 + rr], o1 = rec[14 + r0 + rr];
-        for (int d = 0; d < 3; ++d) acc[rr * 3 + d] += csr[rr] * csc[d] * (o0 * jo[d] + o1 * jo[3 + d]);
-      }
+          acc[rr * 3 + d] += Wx[rr] * A[d] + Wx[6 + rr] * A[6 + d] + Wx[12 + rr] * A[12 + d];
     }
   }
   for (int k = 0; k < 9; ++k) {
@@ -584,14 +586,15 @@ __global__ __launch_bounds__(64 * kItemWaves) void k_schur_items(
   }
 }
 
-// S_ij = sum of the block's item partials in item order (+ the LM diagonal
-// D^2 / radius on S_ii), written to S_ij and S_ji = S_ij^T. One thread per
-// block entry.
+// S_ij = sum of the block's item partials in item order, plus on S_ii the
+// direct observation term C_i (sum Jo^T Jo) C_i and the LM diagonal
+// D^2 / radius; written to S_ij and S_ji = S_ij^T. One thread per entry.
 __global__ __launch_bounds__(252) void k_schur_finalize(int32_t nub, const int4* __restrict__ ublk,
                                                         const int32_t* __restrict__ item_ptr,
                                                         const double* __restrict__ partial,
                                                         const double* __restrict__ pose_scale,
-                                                        const double* __restrict__ pose_diag, double inv_mu,
+                                                        const double* __restrict__ pose_diag,
+                                                        const double* __restrict__ pose_jtj, double inv_mu,
                                                         double min_diag, double max_diag, double* __restrict__ S) {
   const int u = blockIdx.x * 7 + threadIdx.x / 36, e = threadIdx.x % 36;
   if (u >= nub) return;
@@ -599,9 +602,13 @@ __global__ __launch_bounds__(252) void k_schur_finalize(int32_t nub, const int4*
   double v = 0.0;
   for (int it = item_ptr[u]; it < item_ptr[u + 1]; ++it) v += partial[(size_t)36 * it + e];
   const int r = e / 6, c = e % 6;
-  if (ub.z == ub.w && r == c) {
-    const double sc = pose_scale[6 * ub.z + c];
-    v += fmin(fmax(sc * sc * pose_diag[6 * ub.z + c], min_diag), max_diag) * inv_mu;
+  if (ub.z == ub.w) {
+    const int i = ub.z;
+    v += pose_scale[6 * i + r] * pose_scale[6 * i + c] * pose_jtj[36 * i + e];
+    if (r == c) {
+      const double sc = pose_scale[6 * i + c];
+      v += fmin(fmax(sc * sc * pose_diag[6 * i + c], min_diag), max_diag) * inv_mu;
+    }
   }
   S[(size_t)36 * ub.x + e] = v;
   if (ub.y != ub.x) S[(size_t)36 * ub.y + c * 6 + r] = v;
@@ -622,6 +629,138 @@ __global__ __launch_bounds__(256) void k_schur_rhs(DevTopology t, const double* 
   __syncthreads();
   if (threadIdx.x < 6)
     rhs[6 * i + threadIdx.x] = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+}
+
+// ------------------------------------------------------------ landmark admission
+
+struct QualityArgs {
+  int32_t min_observers, pad;
+  double min_angle_deg;  // smallest accepted max disparity angle of the incidence rays
+  double max_dist, min_dist;
+  double max_reproj2;    // squared; <= 0: check off
+};
+
+constexpr int kQualityRays = 256;  // incidence rays kept in LDS per landmark
+
+// Camera centre and landmark of observation o: ray = c_G - p_G (unit, with
+// its length) and the landmark's depth in the camera. The frame chain is
+// T_G_C = T_G_M T_M_I T_C_I^-1 and p_C = T_C_I T_M_I^-1 T_G_M^-1 p_G
+// (landmark-quality-metrics.cc:118-127).
+__device__ inline void quality_ray(const DevTopology& t, const double* __restrict__ vert,
+                                   const double* __restrict__ vR, int o, const double* pG, double* ray,
+                                   double* dist, double* pC) {
+  const int v = t.obs_vertex[o];
+  const DevCamera& cam = t.cams[obs_camera(t.obs_meta[o])];
+  const double* Rv = vR + 12 * v;           // R_I_M
+  const double* pv = vert + 7 * v + 4;      // p_M_I
+  const double* TM = t.mission_T + 12 * t.vertex_mission[v];
+  double cI[3], cM[3], cG[3];
+  mat3t_vec(cam.R_C_I, cam.p_C_I, cI);
+  for (int a = 0; a < 3; ++a) cI[a] = -cI[a];
+  mat3t_vec(Rv, cI, cM);
+  for (int a = 0; a < 3; ++a) cM[a] += pv[a];
+  mat3_vec(TM, cM, cG);
+  for (int a = 0; a < 3; ++a) cG[a] += TM[9 + a];
+  double d[3], pM[3], pI[3];
+  for (int a = 0; a < 3; ++a) d[a] = pG[a] - TM[9 + a];
+  mat3t_vec(TM, d, pM);
+  for (int a = 0; a < 3; ++a) pM[a] -= pv[a];
+  mat3_vec(Rv, pM, pI);
+  mat3_vec(cam.R_C_I, pI, pC);
+  for (int a = 0; a < 3; ++a) pC[a] += cam.p_C_I[a];
+  for (int a = 0; a < 3; ++a) ray[a] = cG[a] - pG[a];
+  *dist = sqrt(ray[0] * ray[0] + ray[1] * ray[1] + ray[2] * ray[2]);
+}
+
+// isVisualLandmarkWellConstrained of every landmark, one 64-thread
+// work-group per landmark: lanes take the observations (signed distance to
+// the closest observer, optional reprojection check, unit rays kept in LDS)
+// and then the pairs of rays (smallest |cos| of the bundle).
+__global__ __launch_bounds__(64) void k_landmark_quality(DevTopology t, const double* __restrict__ vert,
+                                                         const double* __restrict__ vR,
+                                                         const double* __restrict__ lm, QualityArgs q,
+                                                         uint8_t* __restrict__ good) {
+  __shared__ double rays[kQualityRays][3];
+  __shared__ int valid[kQualityRays];
+  const int l = blockIdx.x, lane = threadIdx.x;
+  const int ob = t.lm_begin[l], n = t.lm_begin[l + 1] - ob;
+  if (n < q.min_observers) {
+    if (lane == 0) good[l] = 0;
+    return;
+  }
+  // Landmark in G from its store vertex and mission.
+  const int a = t.lm_anchor[l];
+  const double* TA = t.mission_T + 12 * t.vertex_mission[a];
+  double pM[3], pG[3];
+  mat3t_vec(vR + 12 * a, lm + 3 * l, pM);
+  for (int k = 0; k < 3; ++k) pM[k] += vert[7 * a + 4 + k];
+  mat3_vec(TA, pM, pG);
+  for (int k = 0; k < 3; ++k) pG[k] += TA[9 + k];
+  double smin = 1.7976931348623157e308;  // DBL_MAX
+  int bad = 0;
+  for (int oi = lane; oi < n; oi += 64) {
+    const int o = ob + oi;
+    double ray[3], dist, pC[3];
+    quality_ray(t, vert, vR, o, pG, ray, &dist, pC);
+    smin = fmin(smin, pC[2] < 0.0 ? -dist : dist);
+    if (q.max_reproj2 > 0.0) {
+      const DevCamera& cam = t.cams[obs_camera(t.obs_meta[o])];
+      double kp[2], jp[6];
+      const int st = project(cam.dist_type, cam.intr, cam.dist, cam.width, cam.height, pC, kp, jp);
+      double e2 = 1.7976931348623157e308;
+      if (st == kVisible || st == kOutsideImage) {
+        const double du = kp[0] - t.obs_kp[2 * o], dv = kp[1] - t.obs_kp[2 * o + 1];
+        e2 = du * du + dv * dv;
+      }
+      if (e2 > q.max_reproj2) bad = 1;
+    }
+    if (oi < kQualityRays) {
+      const double inv = dist > 0.0 ? 1.0 / dist : 0.0;
+      for (int k = 0; k < 3; ++k) rays[oi][k] = ray[k] * inv;
+      valid[oi] = dist > 0.0;
+    }
+  }
+  smin = wave_min(smin);
+  bad = __any(bad);
+  __syncthreads();
+  // Smallest |cos| over the pairs i < j of valid rays (pairs in LDS, or
+  // recomputed beyond kQualityRays).
+  double mincos = 1.0;
+  if (!bad && !(smin > q.max_dist || smin < q.min_dist))
+    for (int i = 0; i + 1 < n; ++i) {
+      double ri[3];
+      bool vi;
+      if (i < kQualityRays) {
+        for (int k = 0; k < 3; ++k) ri[k] = rays[i][k];
+        vi = valid[i];
+      } else {
+        double d, pC[3];
+        quality_ray(t, vert, vR, ob + i, pG, ri, &d, pC);
+        vi = d > 0.0;
+        for (int k = 0; k < 3; ++k) ri[k] = vi ? ri[k] / d : 0.0;
+      }
+      if (!vi) continue;
+      for (int j = i + 1 + lane; j < n; j += 64) {
+        double rj[3];
+        bool vj;
+        if (j < kQualityRays) {
+          for (int k = 0; k < 3; ++k) rj[k] = rays[j][k];
+          vj = valid[j];
+        } else {
+          double d, pC[3];
+          quality_ray(t, vert, vR, ob + j, pG, rj, &d, pC);
+          vj = d > 0.0;
+          for (int k = 0; k < 3; ++k) rj[k] = vj ? rj[k] / d : 0.0;
+        }
+        if (vj) mincos = fmin(mincos, fabs(ri[0] * rj[0] + ri[1] * rj[1] + ri[2] * rj[2]));
+      }
+    }
+  mincos = -wave_max(-mincos);
+  if (lane == 0) {
+    // Fewer than two valid rays gives angle 0.
+    const double angle_deg = acos(fmin(mincos, 1.0)) * (180.0 / 3.14159265358979323846);
+    good[l] = (!bad && !(smin > q.max_dist || smin < q.min_dist) && !(angle_deg < q.min_angle_deg)) ? 1 : 0;
+  }
 }
 
 // ------------------------------------------------------------ PCG
@@ -1100,8 +1239,10 @@ void launch_zero_split(hipStream_t s, int32_t n, const int32_t* ids, double* lm_
   if (n > 0) k_zero_split<<<(n * kLmAcc + 255) / 256, 256, 0, s>>>(n, ids, lm_acc);
 }
 void launch_pose_rows(hipStream_t s, const DevTopology& t, const double* vert, const double* vR, const double* lm,
-                      const double* obs_rec, const double* lm_acc, double* pose_grad, double* pose_diag) {
-  if (t.Pvar > 0) k_pose_rows<<<t.Pvar, kRowThreads, 0, s>>>(t, vert, vR, lm, obs_rec, lm_acc, pose_grad, pose_diag);
+                      const double* obs_rec, const double* lm_acc, double* pose_grad, double* pose_diag,
+                      double* pose_jtj) {
+  if (t.Pvar > 0)
+    k_pose_rows<<<t.Pvar, kRowThreads, 0, s>>>(t, vert, vR, lm, obs_rec, lm_acc, pose_grad, pose_diag, pose_jtj);
 }
 void launch_jacobi_scaling(hipStream_t s, int32_t Pvar, int32_t L, int32_t use, const double* pose_diag,
                            const double* lm_acc, double* pose_scale, double* lm_scale) {
@@ -1113,33 +1254,50 @@ void launch_landmark_prep(hipStream_t s, const DevTopology& t, const double* lm_
   if (t.L > 0)
     k_landmark_prep<<<(t.L + 255) / 256, 256, 0, s>>>(t, lm_acc, lm_scale, inv_mu, min_diag, max_diag, lm_z, fail);
 }
-void launch_schur_incidences(hipStream_t s, const DevTopology& t, int32_t n_inc, const int32_t* inc_row,
-                             const double* vR, const double* lm, const double* obs_rec, const double* lm_acc,
-                             const double* lm_z, const double* pose_scale, double* inc_q, double* inc_rhs,
-                             double* lm_wa) {
+void launch_schur_factors(hipStream_t s, const DevTopology& t, int32_t n_inc, const int32_t* inc_row,
+                          const double* vR, const double* lm, const double* obs_rec, const double* lm_acc,
+                          const double* pose_scale, double* inc_w, double* inc_r0, double* lm_a) {
   if (t.Pvar > 0 && n_inc > 0)
-    k_schur_incidences<<<(n_inc + 255) / 256, 256, 0, s>>>(t, n_inc, inc_row, vR, lm, obs_rec, lm_acc, lm_z,
-                                                           pose_scale, inc_q, inc_rhs, lm_wa);
+    k_schur_factors<<<(n_inc + 255) / 256, 256, 0, s>>>(t, n_inc, inc_row, vR, lm, obs_rec, lm_acc, pose_scale,
+                                                        inc_w, inc_r0, lm_a);
+}
+void launch_schur_incidences(hipStream_t s, const DevTopology& t, int32_t n_inc, const double* inc_w,
+                             const double* inc_r0, const double* lm_a, const double* lm_acc, const double* lm_z,
+                             double* inc_q, double* inc_rhs) {
+  if (t.Pvar > 0 && n_inc > 0)
+    k_schur_incidences<<<(n_inc + 255) / 256, 256, 0, s>>>(t, n_inc, inc_w, inc_r0, lm_a, lm_acc, lm_z, inc_q,
+                                                           inc_rhs);
 }
 void launch_schur_blocks(hipStream_t s, const DevTopology& t, int32_t nub, const int4* ublk, int32_t n_items,
-                         const int4* items, const int32_t* item_ptr, const uint2* pairs, const double* obs_rec,
-                         const double* inc_q, const double* inc_rhs, const double* lm_wa, const double* pose_scale,
-                         const double* pose_diag, double inv_mu, double min_diag, double max_diag, double* partial,
-                         double* S, double* rhs) {
+                         const int4* items, const int32_t* item_ptr, const uint2* pairs, const double* inc_q,
+                         const double* inc_w, const double* inc_rhs, const double* lm_a, const double* pose_scale,
+                         const double* pose_diag, const double* pose_jtj, double inv_mu, double min_diag,
+                         double max_diag, double* partial, double* S, double* rhs) {
   if (t.Pvar <= 0 || nub <= 0) return;
   k_schur_rhs<<<t.Pvar, 256, 0, s>>>(t, inc_rhs, rhs);
   if (n_items > 0) {
     const int wgs = (n_items + kItemWaves - 1) / kItemWaves;
     const int per_xcd = (wgs + 7) / 8;
-    k_schur_items<<<8 * per_xcd, 64 * kItemWaves, 0, s>>>(t, n_items, items, ublk, pairs, obs_rec, inc_q, lm_wa,
-                                                          pose_scale, partial);
+    k_schur_items<<<8 * per_xcd, 64 * kItemWaves, 0, s>>>(t, n_items, items, pairs, inc_q, inc_w, lm_a, partial);
   }
-  k_schur_finalize<<<(nub + 6) / 7, 252, 0, s>>>(nub, ublk, item_ptr, partial, pose_scale, pose_diag, inv_mu,
-                                                 min_diag, max_diag, S);
+  k_schur_finalize<<<(nub + 6) / 7, 252, 0, s>>>(nub, ublk, item_ptr, partial, pose_scale, pose_diag, pose_jtj,
+                                                 inv_mu, min_diag, max_diag, S);
 }
 int32_t schur_item_pairs() { return kItemPairs; }
-uint32_t schur_pair_same_bit() { return kPairSame; }
 uint32_t schur_pair_anchor_bit() { return kPairAnchor; }
+void launch_landmark_quality(hipStream_t s, const DevTopology& t, const double* vert, const double* vR,
+                             const double* lm, int32_t min_observers, double min_angle_deg, double max_dist,
+                             double min_dist, double max_reproj_px, uint8_t* good) {
+  if (t.L <= 0) return;
+  QualityArgs q;
+  q.min_observers = min_observers;
+  q.pad = 0;
+  q.min_angle_deg = min_angle_deg;
+  q.max_dist = max_dist;
+  q.min_dist = min_dist;
+  q.max_reproj2 = max_reproj_px > 0.0 ? max_reproj_px * max_reproj_px : 0.0;
+  k_landmark_quality<<<t.L, 64, 0, s>>>(t, vert, vR, lm, q, good);
+}
 void launch_block_inverse(hipStream_t s, int32_t Pvar, const int32_t* diag_pos, const double* S, double* Minv,
                           int32_t* fail) {
   if (Pvar > 0) k_block_inverse<<<(Pvar + 63) / 64, 64, 0, s>>>(Pvar, diag_pos, S, Minv, fail);

@@ -65,6 +65,14 @@ class DeviceProblem:
         self._check(self.lib.mlba_get_landmark_active(self.handle, m.ctypes.data))
         return m
 
+    def landmark_quality(self, options: abi.QualityOptions | None = None) -> np.ndarray:
+        """isVisualLandmarkWellConstrained of every landmark at the current
+        device state (bool array, caller landmark order)."""
+        o = options if options is not None else abi.default_quality_options()
+        good = np.empty(self.map.num_landmarks, dtype=np.uint8)
+        self._check(self.lib.mlba_landmark_quality(self.handle, ctypes.byref(o), good.ctypes.data))
+        return good.astype(bool)
+
     def evaluate(self, tangent: bool = False, blocks=range(9), options: abi.Options | None = None,
                  want_cost: bool = True):
         """Batched VisualReprojectionError::Evaluate: residuals (N, 2), a dict

@@ -28,6 +28,7 @@
 #include "ba_oracle.h"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -1335,6 +1336,54 @@ std::vector<int> findOutlierLandmarks(const Problem& pb, const mlba_options& o) 
   return out;
 }
 
+// ------------------------------------------------------------ landmark admission
+
+// Rigid transform T (R, p): x -> R x + p (kindr / minkindr Transformation).
+struct Tf {
+  double R[9];
+  double p[3];
+};
+
+Tf tfCompose(const Tf& a, const Tf& b) {  // a * b
+  Tf c;
+  matmul(a.R, b.R, c.R, 3, 3, 3);
+  matmul(a.R, b.p, c.p, 3, 3, 1);
+  for (int k = 0; k < 3; ++k) c.p[k] += a.p[k];
+  return c;
+}
+
+Tf tfInverse(const Tf& a) {
+  Tf c;
+  transpose33(a.R, c.R);
+  matmul(c.R, a.p, c.p, 3, 3, 1);
+  for (int k = 0; k < 3; ++k) c.p[k] = -c.p[k];
+  return c;
+}
+
+void tfApply(const Tf& a, const double* x, double* y) {
+  matmul(a.R, x, y, 3, 3, 1);
+  for (int k = 0; k < 3; ++k) y[k] += a.p[k];
+}
+
+// T_M_I of a vertex block [q_IM | M_p_MI] (rotJPL(q) = R_I_M), T_G_M of a
+// baseframe block (rotJPL(q) = R_G_M), T_C_I of a camera block (rotJPL(q) = R_C_I)
+// (optimization-state-buffer.cc:161-263).
+Tf vertexT_M_I(const double* b) {
+  Tf t;
+  double R_I_M[9];
+  rotJPL(b, R_I_M);
+  transpose33(R_I_M, t.R);
+  for (int k = 0; k < 3; ++k) t.p[k] = b[4 + k];
+  return t;
+}
+
+Tf blockT(const double* b) {
+  Tf t;
+  rotJPL(b, t.R);
+  for (int k = 0; k < 3; ++k) t.p[k] = b[4 + k];
+  return t;
+}
+
 int resolveThreads(int n) {
   if (n > 0) return n;
   const unsigned hw = std::thread::hardware_concurrency();
@@ -1498,6 +1547,72 @@ int32_t oracle_solve_with_outlier_rejection(const mlba_problem_desc* desc, const
   if (vertices_out) std::memcpy(vertices_out, pb.vertices.data(), sizeof(double) * 7 * pb.P);
   if (landmarks_out) std::memcpy(landmarks_out, pb.landmarks.data(), sizeof(double) * 3 * pb.L);
   if (landmark_active_out) std::memcpy(landmark_active_out, pb.lm_active.data(), pb.L);
+  return MLBA_OK;
+}
+
+int32_t oracle_landmark_quality(const mlba_problem_desc* d, const mlba_quality_options* o, uint8_t* good) {
+  if (!d || !o || !good) return MLBA_EINVAL;
+  const int L = d->num_landmarks, N = d->num_observations;
+  std::vector<std::vector<int>> obs(L);
+  for (int i = 0; i < N; ++i) obs[d->obs_landmark[i]].push_back(i);
+  const double max_e2 = o->max_reprojection_error_px * o->max_reprojection_error_px;
+  for (int l = 0; l < L; ++l) {
+    good[l] = 0;
+    // landmark-quality-metrics.cc:105-108
+    if ((int)obs[l].size() < o->min_observers) continue;
+    // map.getLandmark_G_p_fi: T_G_M (store mission) * T_M_I (store vertex) * p_B.
+    const int a = d->landmark_store_vertex[l];
+    const Tf T_G_Ia = tfCompose(blockT(d->baseframe_q_GM__G_p_GM + 7 * d->vertex_mission[a]),
+                                vertexT_M_I(d->vertex_q_IM__M_p_MI + 7 * a));
+    double p_G_fi[3];
+    tfApply(T_G_Ia, d->landmark_p_B + 3 * l, p_G_fi);
+    std::vector<std::array<double, 3>> rays;
+    double signed_min = std::numeric_limits<double>::max();
+    bool reproj_bad = false;
+    for (int i : obs[l]) {
+      const int v = d->obs_vertex[i], c = d->obs_camera[i];
+      // T_G_C = T_G_I * T_C_B^-1 (:118-121); p_C_fi = T_C_B * T_G_I^-1 * p_G_fi (:123-124).
+      const Tf T_G_I = tfCompose(blockT(d->baseframe_q_GM__G_p_GM + 7 * d->vertex_mission[v]),
+                                 vertexT_M_I(d->vertex_q_IM__M_p_MI + 7 * v));
+      const Tf T_C_B = blockT(d->camera_q_CI__C_p_CI + 7 * c);
+      const Tf T_G_C = tfCompose(T_G_I, tfInverse(T_C_B));
+      double p_C[3];
+      tfApply(tfCompose(T_C_B, tfInverse(T_G_I)), p_G_fi, p_C);
+      double ray[3];
+      for (int k = 0; k < 3; ++k) ray[k] = T_G_C.p[k] - p_G_fi[k];
+      const double dist = std::sqrt(ray[0] * ray[0] + ray[1] * ray[1] + ray[2] * ray[2]);
+      signed_min = std::min(signed_min, dist * (p_C[2] < 0.0 ? -1.0 : 1.0));
+      if (dist > 0) rays.push_back({ray[0] / dist, ray[1] / dist, ray[2] / dist});
+      if (o->max_reprojection_error_px > 0.0) {
+        // computeSquaredReprojectionError (:57-73)
+        double kp[2];
+        const int st = project3Functional(d->camera_distortion_type[c], d->camera_intrinsics + 4 * c,
+                                          d->camera_distortion + 4 * c, d->camera_resolution[2 * c],
+                                          d->camera_resolution[2 * c + 1], p_C, kp, nullptr, nullptr, nullptr);
+        double e2 = std::numeric_limits<double>::max();
+        if (st == kVisible || st == kOutside) {
+          const double du = kp[0] - d->obs_keypoint[2 * i], dv = kp[1] - d->obs_keypoint[2 * i + 1];
+          e2 = du * du + dv * dv;
+        }
+        if (e2 > max_e2) reproj_bad = true;
+      }
+    }
+    if (reproj_bad) continue;
+    if (signed_min > o->max_distance_from_closest_observer || signed_min < o->min_distance_from_closest_observer)
+      continue;
+    // common::getMaxDisparityRadAngleOfUnitVectorBundle (geometry.cc:30-44)
+    double angle = 0.0;
+    if (rays.size() >= 2) {
+      double min_cos = 1.0;
+      for (size_t x = 0; x < rays.size(); ++x)
+        for (size_t y = x + 1; y < rays.size(); ++y)
+          min_cos = std::min(min_cos, std::abs(rays[x][0] * rays[y][0] + rays[x][1] * rays[y][1] +
+                                               rays[x][2] * rays[y][2]));
+      angle = std::acos(min_cos);
+    }
+    if (angle * (180.0 / M_PI) < o->min_observation_angle_deg) continue;
+    good[l] = 1;
+  }
   return MLBA_OK;
 }
 

@@ -75,6 +75,12 @@ int32_t oracle_solve_with_outlier_rejection(const mlba_problem_desc* desc,
                                             uint8_t* landmark_active_out,
                                             mlba_summary* summary);
 
+/* vi_map::isVisualLandmarkWellConstrained (landmark-quality-metrics.cc:103-171)
+ * of every landmark over all of its observations in desc; good (L) receives
+ * 1 for well-constrained landmarks. */
+int32_t oracle_landmark_quality(const mlba_problem_desc* desc, const mlba_quality_options* options,
+                                uint8_t* good);
+
 /* The damped, Jacobi-scaled reduced camera system at the given state:
  * S (n x n dense, n = 6 * number of variable vertices that have residuals,
  * ordered by vertex index) and rhs (n), for LM radius `mu`. Used to check the

@@ -45,6 +45,7 @@ def load():
         "oracle_solve_with_outlier_rejection": (i32, [P(abi.ProblemDesc), P(abi.Options), i32, i32, vp, vp, vp,
                                                       P(abi.Summary)]),
         "oracle_reduced_camera_system": (i32, [P(abi.ProblemDesc), P(abi.Options), f64, i32, vp, vp, vp]),
+        "oracle_landmark_quality": (i32, [P(abi.ProblemDesc), P(abi.QualityOptions), vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -179,3 +180,13 @@ def reduced_camera_system(vmap, options, mu):
                                          _p(vb))
     assert n >= 0, n
     return S[: n * n].reshape(n, n), rhs[:n], vb[: n // 6]
+
+
+def landmark_quality(vmap, options):
+    """isVisualLandmarkWellConstrained of every landmark (bool array)."""
+    lib = load()
+    d = vmap.desc()
+    good = np.empty(vmap.num_landmarks, dtype=np.uint8)
+    rc = lib.oracle_landmark_quality(ctypes.byref(d), ctypes.byref(options), _p(good))
+    assert rc == 0, rc
+    return good.astype(bool)

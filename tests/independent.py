@@ -131,3 +131,56 @@ def tangent_jacobians(meas, sigma, error_type, dist_type, params, h=1e-5):
         out[1] = numdiff(r_with(1, pose_plus), 6, h)
         out[4] = numdiff(r_with(4, pose_plus), 6, h)
     return out
+
+
+def landmark_quality(vmap, min_observers=4, min_angle_deg=5.0, max_dist=40.0, min_dist=0.05, max_reproj_px=-1.0):
+    """isVisualLandmarkWellConstrained (landmark-quality-metrics.cc:103-171)
+    restated with Hamilton rotations: camera centres c_G = R_G_M (R_M_I c_I +
+    p_M_I) + p_G_M with c_I = -R_C_I^T p_C_I, the landmark in G from its store
+    vertex, and the disparity angle as the largest pairwise angle between
+    incidence lines (antiparallel rays count as parallel)."""
+    v, bf = vmap.vertices, vmap.baseframes
+    out = np.zeros(vmap.num_landmarks, dtype=bool)
+    by_lm = [[] for _ in range(vmap.num_landmarks)]
+    for i, l in enumerate(vmap.obs_landmark):
+        by_lm[l].append(i)
+
+    def frame(k):
+        m = vmap.vertex_mission[k]
+        R_M_I = Rotation.from_quat(v[k, :4]).as_matrix()
+        return R_jpl(bf[m, :4]), bf[m, 4:], R_M_I, v[k, 4:]
+
+    for l, obs in enumerate(by_lm):
+        if len(obs) < min_observers:
+            continue
+        R_G_M, p_G_M, R_M_I, p_M_I = frame(vmap.landmark_store_vertex[l])
+        p_G = R_G_M @ (R_M_I @ vmap.landmarks[l] + p_M_I) + p_G_M
+        sd, dirs, bad = [], [], False
+        for i in obs:
+            R_G_M, p_G_M, R_M_I, p_M_I = frame(vmap.obs_vertex[i])
+            c = vmap.obs_camera[i]
+            R_C_I = R_jpl(vmap.camera_q_CI__C_p_CI[c, :4])
+            p_C_I = vmap.camera_q_CI__C_p_CI[c, 4:]
+            c_G = R_G_M @ (R_M_I @ (-R_C_I.T @ p_C_I) + p_M_I) + p_G_M
+            p_C = R_C_I @ (R_M_I.T @ (R_G_M.T @ (p_G - p_G_M) - p_M_I)) + p_C_I
+            ray = c_G - p_G
+            d = np.linalg.norm(ray)
+            sd.append(-d if p_C[2] < 0 else d)
+            if d > 0:
+                dirs.append(ray / d)
+            if max_reproj_px > 0:
+                if p_C[2] <= 1e-10:
+                    bad = True
+                    continue
+                kp = project(vmap.camera_distortion_type[c], vmap.camera_intrinsics[c], vmap.camera_distortion[c], p_C)
+                if np.sum((kp - vmap.obs_keypoint[i]) ** 2) > max_reproj_px ** 2:
+                    bad = True
+        if bad or min(sd) > max_dist or min(sd) < min_dist:
+            continue
+        angle = 0.0
+        if len(dirs) >= 2:
+            D = np.array(dirs)
+            cosines = np.abs(D @ D.T)[np.triu_indices(len(dirs), 1)]
+            angle = np.degrees(np.arccos(min(1.0, cosines.min())))
+        out[l] = angle >= min_angle_deg
+    return out

@@ -28,6 +28,14 @@ def test_library_exports_every_declared_symbol():
     assert lib.mlba_abi_version() == 1
 
 
+def test_library_exports_every_optimizer_symbol():
+    lib = abi.load_library()
+    syms = declared("maplab_ba_optimizer.h")
+    assert syms == set(abi.OPTIMIZER_SYMBOLS), syms ^ set(abi.OPTIMIZER_SYMBOLS)
+    for s in syms:
+        assert hasattr(lib, s), s
+
+
 def test_synth_library_exports_every_declared_symbol():
     lib = abi.load_synth()
     syms = declared("maplab_ba_synth.h")
@@ -40,7 +48,7 @@ def test_struct_layouts_match_header():
     # Field offsets are what the C compiler produces for the declarations.
     assert ctypes.sizeof(abi.ProblemDesc) == 24 + 17 * 8
     assert ctypes.sizeof(abi.IterationSummary) == 16 + 8 * 8
-    assert abi.Summary.kernel_launches.offset == ctypes.sizeof(abi.Summary) - 32
+    assert abi.Summary.kernel_launches.offset == ctypes.sizeof(abi.Summary) - 4 * abi.NUM_KERNELS
     assert abi.Summary.kernel_ms.offset == abi.Summary.iterations.offset + 8
 
 
@@ -51,6 +59,9 @@ def test_struct_layouts_match_header():
     (abi.Summary, "mlba_summary", "maplab_ba.h"),
     (abi.KernelTimes, "mlba_kernel_times", "maplab_ba.h"),
     (abi.SynthConfig, "mlba_synth_config", "maplab_ba_synth.h"),
+    (abi.QualityOptions, "mlba_quality_options", "maplab_ba.h"),
+    (abi.ViProblemOptions, "mlba_vi_problem_options", "maplab_ba_optimizer.h"),
+    (abi.ProblemSelection, "mlba_problem_selection", "maplab_ba_optimizer.h"),
 ])
 def test_struct_layout_matches_compiled_header(tmp_path, struct, cname, header):
     # The C compiler's sizeof / offsetof of every field equals the ctypes mirror.
@@ -67,6 +78,19 @@ def test_struct_layout_matches_compiled_header(tmp_path, struct, cname, header):
     out = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
     assert out[0] == ctypes.sizeof(struct)
     assert out[1:] == [getattr(struct, f).offset for f in fields]
+
+
+def test_quality_and_problem_defaults_are_maplab_flags():
+    # landmark-quality-metrics.cc:11-33; vi-optimization-builder.cc:10-60.
+    q = abi.default_quality_options()
+    assert (q.min_observers, q.min_observation_angle_deg) == (4, 5.0)
+    assert (q.max_distance_from_closest_observer, q.min_distance_from_closest_observer) == (40.0, 0.05)
+    assert q.max_reprojection_error_px == -1.0
+    from maplab_amd.optimizer import default_vi_problem_options
+    o = default_vi_problem_options()
+    assert (o.add_visual_constraints, o.fix_landmark_positions, o.fix_vertices) == (1, 0, 0)
+    assert (o.min_landmarks_per_frame, o.enable_visual_outlier_rejection) == (0, 1)
+    assert o.solver.max_num_iterations == 30 and o.landmark_quality.min_observers == 4
 
 
 def test_option_defaults_are_maplab_flags():
