@@ -140,6 +140,16 @@ class ProblemSelection(ctypes.Structure):
                 ("gauge_vertex", c_vp), ("gauge_landmark", c_vp)]
 
 
+class InertialProblem(ctypes.Structure):
+    _fields_ = [("num_vertices", c_i32), ("num_edges", c_i32), ("num_samples", c_i32), ("reserved0", c_i32),
+                ("vertex_q_IM__M_p_MI", c_vp), ("vertex_gyro_bias", c_vp), ("vertex_velocity_M", c_vp),
+                ("vertex_accel_bias", c_vp), ("edge_from", c_vp), ("edge_to", c_vp), ("edge_sample_begin", c_vp),
+                ("imu", c_vp), ("timestamps_ns", c_vp), ("gyro_noise_sigma", c_f64), ("gyro_bias_sigma", c_f64),
+                ("acc_noise_sigma", c_f64), ("acc_bias_sigma", c_f64), ("gravity_magnitude", c_f64)]
+
+
+INERTIAL_BLOCK_SIZES = (7, 3, 3, 3, 7, 3, 3, 3)
+
 QUALITY_UNKNOWN, QUALITY_BAD, QUALITY_GOOD, QUALITY_LOCALIZATION_SUMMARY = 0, 1, 2, 3
 
 
@@ -179,6 +189,12 @@ OPTIMIZER_SYMBOLS = {
     "mlba_optimizer_last_error": (ctypes.c_char_p, []),
 }
 
+# Every symbol include/maplab_ba_inertial.h declares (exported by the same library).
+INERTIAL_SYMBOLS = {
+    "mlba_inertial_evaluate": (c_i32, [ctypes.POINTER(InertialProblem), c_vp, c_vp]),
+    "mlba_inertial_last_error": (ctypes.c_char_p, []),
+}
+
 SYNTH_SYMBOLS = {
     "mlba_synth_config_init": (c_i32, [ctypes.POINTER(SynthConfig), c_i32]),
     "mlba_synth_generate": (c_i32, [ctypes.POINTER(SynthConfig), ctypes.POINTER(c_vp)]),
@@ -200,7 +216,7 @@ def load_library(path: str = LIB_PATH):
             raise OSError(f"{path} is missing: build it with `make -C maplab_amd/csrc` "
                           "(the visual BA path has no CPU fallback)")
         lib = ctypes.CDLL(path)
-        for name, (res, args) in {**LIB_SYMBOLS, **OPTIMIZER_SYMBOLS}.items():
+        for name, (res, args) in {**LIB_SYMBOLS, **OPTIMIZER_SYMBOLS, **INERTIAL_SYMBOLS}.items():
             _declare(lib, name, res, args)
         _lib = lib
     return _lib

@@ -66,9 +66,10 @@ int launch_pcg_deflated(hipStream_t s, int32_t Pvar, int32_t K, const int32_t* r
                         double* part, double* part2, void* state_dev, void* state_host, double tol,
                         int32_t max_iters, int32_t poll, const hipEvent_t* probe);
 int32_t pcg_group_rows();
-void launch_back_substitute(hipStream_t s, const DevTopology& t, const double* vR, const double* lm,
-                            const double* obs_rec, const double* lm_acc, const double* lm_z, const double* w,
-                            double* u, double* part);
+int32_t launch_back_substitute(hipStream_t s, const DevTopology& t, int32_t tiles, const int32_t* tile_begin,
+                               int32_t n_split, const int32_t* split_ids, const double* vR, const double* lm,
+                               const double* obs_rec, const double* lm_acc, const double* lm_z, const double* w,
+                               double* u, double* part);
 void launch_unscale_poses(hipStream_t s, int32_t Pvar, const double* y, const double* pose_scale, double* w,
                           double* part);
 void launch_candidate(hipStream_t s, const DevTopology& t, const double* vert, const double* w, double* cand_v,
@@ -430,11 +431,13 @@ int build_topology(mlba_handle* h) {
             h->pr.alloc(V6) && h->pz.alloc(V6) && h->pp.alloc(V6) && h->pp2.alloc(V6) && h->pq.alloc(V6) &&
             h->pcg_part.alloc(3 * (size_t)std::max(Pvar, 1) + 3) &&
             h->parts_a.alloc(std::max<size_t>(Pvar, 2 * vb) + 1) && h->parts_b.alloc(2 * vb + 2) &&
-            h->parts_l.alloc(2 * ((size_t)(h->L + 255) / 256 + 1)) &&
+            h->parts_l.alloc(2 * ((size_t)h->tiles + (size_t)(h->n_split + 255) / 256 + 1)) &&
             h->parts_c.alloc((size_t)(h->L + 255) / 256 + 1) &&
             h->parts_v.alloc((size_t)(h->P + 255) / 256 + 1) && h->parts_w.alloc((6 * (size_t)Pvar + 255) / 256 + 1) &&
             h->parts_g.alloc(((size_t)Pvar + h->L + 255) / 256 + 1);
   if (!ok) return set_err(h, MLBA_ENOMEM, "device allocation failed");
+  // The landmark step of inactive or observation-less landmarks stays zero.
+  if (h->L > 0) HIP_OK(hipMemsetAsync(h->lm_u.p, 0, 3 * (size_t)h->L * sizeof(double), h->stream));
   DevTopology& t = h->topo;
   t.N = h->N; t.L = h->L; t.P = h->P; t.Pvar = Pvar; t.M = h->M; t.C = h->C;
   t.obs_kp = h->obs_kp.p; t.obs_sinv = h->obs_sinv.p; t.obs_vertex = h->obs_vertex.p; t.obs_meta = h->obs_meta.p;
@@ -682,13 +685,15 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
         if (pcg_iters < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
       }
       launch_unscale_poses(s, h->Pvar, h->px.p, h->pose_scale.p, h->w.p, h->parts_w.p);
+      int32_t nbs = 0;
       {
         KernelTimer kt(h, MLBA_KERNEL_BACK_SUBSTITUTE);
-        launch_back_substitute(s, h->topo, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p, h->lm_z.p, h->w.p,
-                               h->lm_u.p, h->parts_l.p);
+        nbs = launch_back_substitute(s, h->topo, h->tiles, h->tile_begin.p, h->n_split, h->split_ids.p, h->vR.p,
+                                     h->lm.p, h->obs_rec.p, h->lm_acc.p, h->lm_z.p, h->w.p, h->lm_u.p,
+                                     h->parts_l.p);
       }
-      launch_reduce(s, lb, 2, 0, false, h->parts_l.p, h->scal.p + kMcc);
-      launch_reduce(s, lb, 2, 1, false, h->parts_l.p, h->scal.p + kUU);
+      launch_reduce(s, nbs, 2, 0, false, h->parts_l.p, h->scal.p + kMcc);
+      launch_reduce(s, nbs, 2, 1, false, h->parts_l.p, h->scal.p + kUU);
       launch_reduce(s, wb, 1, 0, false, h->parts_w.p, h->scal.p + kWW);
       // Candidate point and its cost.
       launch_candidate(s, h->topo, h->vert.p, h->w.p, h->vert_cand.p, h->parts_v.p, h->lm.p, h->lm_u.p,
@@ -1320,8 +1325,8 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
   }
   launch_unscale_poses(s, h->Pvar, h->px.p, h->pose_scale.p, h->w.p, h->parts_w.p);
   t->back_substitute_ms = timed([&] {
-    launch_back_substitute(s, h->topo, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p, h->lm_z.p, h->w.p, h->lm_u.p,
-                           h->parts_l.p);
+    (void)launch_back_substitute(s, h->topo, h->tiles, h->tile_begin.p, h->n_split, h->split_ids.p, h->vR.p,
+                                 h->lm.p, h->obs_rec.p, h->lm_acc.p, h->lm_z.p, h->w.p, h->lm_u.p, h->parts_l.p);
   });
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(s));

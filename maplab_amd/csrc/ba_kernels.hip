@@ -811,81 +811,174 @@ __global__ void k_block_inverse(int32_t Pvar, const int32_t* __restrict__ diag_p
 // system, w = C_p y_p (per vertex) and u = C_l y_l = Z_l (g_l - sum W' y_p):
 // J y = Jl u + Jo w_o + Jl M w_a per observation and
 // model_cost_change = sum (J y)'r - |J y|^2 / 2 (ceres ComputeTrustRegionStep).
-__global__ __launch_bounds__(256) void k_back_substitute(DevTopology t, const double* __restrict__ vR,
-                                                         const double* __restrict__ lm,
-                                                         const double* __restrict__ obs_rec,
-                                                         const double* __restrict__ lm_acc,
-                                                         const double* __restrict__ lm_z,
-                                                         const double* __restrict__ w, double* __restrict__ u_out,
-                                                         double* __restrict__ part) {
+// One work-group per linearization tile of whole landmarks: each observation
+// record is read once; the landmark sums are taken by the segment heads in a
+// fixed order. Tiles of landmarks split over several tiles are left to
+// k_back_substitute_split (their part entries are zero here). u of inactive
+// or observation-less landmarks is not written (kept zero).
+__device__ inline void landmark_step(const DevTopology& t, int l, const double* __restrict__ vR,
+                                     const double* __restrict__ lm, const double* __restrict__ lm_acc,
+                                     const double* __restrict__ lm_z, const double* __restrict__ w,
+                                     double* tv, double* u, double* Mwa) {
+  const uint32_t flags = t.lm_flags[l];
+  const double* acc = lm_acc + (size_t)kLmAcc * l;
+  Mwa[0] = Mwa[1] = Mwa[2] = 0.0;
+  if (flags & kLmAnchorVar) {
+    const int a = t.lm_anchor[l];
+    double M[18];
+    anchor_map(lm + 3 * l, vR + 12 * a, vR[12 * a + 9], M);
+    const double* wa = w + 6 * t.vertex_var[a];
+    for (int r = 0; r < 3; ++r) {
+      double s = 0.0;
+      for (int c = 0; c < 6; ++c) s += M[r * 6 + c] * wa[c];
+      Mwa[r] = s;
+    }
+  }
+  u[0] = u[1] = u[2] = 0.0;
+  if (!(flags & kLmConstant)) {
+    if (flags & kLmAnchorVar) {
+      double VLM[9];
+      sym3_full(acc + 6, VLM);
+      for (int a = 0; a < 3; ++a) tv[a] -= VLM[a * 3] * Mwa[0] + VLM[a * 3 + 1] * Mwa[1] + VLM[a * 3 + 2] * Mwa[2];
+    }
+    double Z[9];
+    sym3_full(lm_z + 6 * l, Z);
+    for (int a = 0; a < 3; ++a) u[a] = Z[a * 3] * tv[0] + Z[a * 3 + 1] * tv[1] + Z[a * 3 + 2] * tv[2];
+  }
+}
+
+__global__ __launch_bounds__(kObsTile) void k_back_substitute(DevTopology t, const int32_t* __restrict__ tile_begin,
+                                                              const double* __restrict__ vR,
+                                                              const double* __restrict__ lm,
+                                                              const double* __restrict__ obs_rec,
+                                                              const double* __restrict__ lm_acc,
+                                                              const double* __restrict__ lm_z,
+                                                              const double* __restrict__ w, double* __restrict__ u_out,
+                                                              double* __restrict__ part) {
+  __shared__ double red[kObsTile * 3];
+  __shared__ double ush[kObsTile * 6];  // u and M w_a at each segment head
+  __shared__ int32_t seg[kObsTile];
   __shared__ double lds4[4];
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
-  double mcc = 0.0, uu = 0.0;
-  if (l < t.L && (t.lm_flags[l] & kLmActive)) {
-    const uint32_t flags = t.lm_flags[l];
-    const int lb = t.lm_begin[l], le = t.lm_begin[l + 1];
-    const bool avar = (flags & kLmAnchorVar) != 0;
-    const bool lvar = !(flags & kLmConstant);
-    double M[18], wa[6] = {0, 0, 0, 0, 0, 0}, Mwa[3] = {0, 0, 0};
-    const double* acc = lm_acc + (size_t)kLmAcc * l;
-    if (avar) {
-      const int a = t.lm_anchor[l];
-      anchor_map(lm + 3 * l, vR + 12 * a, vR[12 * a + 9], M);
-      for (int c = 0; c < 6; ++c) wa[c] = w[6 * t.vertex_var[a] + c];
-      for (int r = 0; r < 3; ++r) {
+  const int tid = threadIdx.x;
+  const int b = tile_begin[blockIdx.x], e = tile_begin[blockIdx.x + 1];
+  if (t.lm_flags[t.obs_landmark[b]] & kLmSplit) {  // whole tile belongs to a split landmark
+    if (tid == 0) part[2 * blockIdx.x] = part[2 * blockIdx.x + 1] = 0.0;
+    return;
+  }
+  const int o = b + tid;
+  int32_t l = -1;
+  uint32_t flags = 0;
+  double r[2] = {0, 0}, jl[6], jw[2] = {0, 0}, c[3] = {0, 0, 0};
+  bool coupled = false, ovar = false;
+  if (o < e) {
+    l = t.obs_landmark[o];
+    flags = t.lm_flags[l];
+  }
+  const bool active = o < e && (flags & kLmActive);
+  if (active) {
+    const double* rec = obs_rec + (size_t)kObsRecord * o;
+    const int32_t meta = t.obs_meta[o];
+    coupled = obs_type(meta) != kTypeKeyframe;
+    ovar = coupled && obs_observer_var(meta);
+    r[0] = rec[0];
+    r[1] = rec[1];
+    for (int k = 0; k < 6; ++k) jl[k] = rec[2 + k];
+    if (ovar) {
+      const double* wo = w + 6 * t.vertex_var[t.obs_vertex[o]];
+      for (int rr = 0; rr < 2; ++rr) {
         double s = 0.0;
-        for (int c = 0; c < 6; ++c) s += M[r * 6 + c] * wa[c];
-        Mwa[r] = s;
+        for (int k = 0; k < 6; ++k) s += rec[8 + rr * 6 + k] * wo[k];
+        jw[rr] = s;
       }
+      for (int a = 0; a < 3; ++a) c[a] = -(jl[a] * jw[0] + jl[3 + a] * jw[1]);
     }
-    double u[3] = {0, 0, 0};
-    if (lvar) {
-      double tv[3] = {acc[12], acc[13], acc[14]};
-      for (int o = lb; o < le; ++o) {
-        const int32_t meta = t.obs_meta[o];
-        if (obs_type(meta) == kTypeKeyframe || !obs_observer_var(meta)) continue;
-        const double* rec = obs_rec + (size_t)kObsRecord * o;
-        const double* wo = w + 6 * t.vertex_var[t.obs_vertex[o]];
-        const double* Jl = rec + 2;
-        const double* Jo = rec + 8;
-        double jw[2];
-        for (int r = 0; r < 2; ++r) {
-          double s = 0.0;
-          for (int c = 0; c < 6; ++c) s += Jo[r * 6 + c] * wo[c];
-          jw[r] = s;
-        }
-        for (int a = 0; a < 3; ++a) tv[a] -= Jl[a] * jw[0] + Jl[3 + a] * jw[1];
-      }
-      if (avar) {
-        double VLM[9];
-        sym3_full(acc + 6, VLM);
-        for (int a = 0; a < 3; ++a) tv[a] -= VLM[a * 3] * Mwa[0] + VLM[a * 3 + 1] * Mwa[1] + VLM[a * 3 + 2] * Mwa[2];
-      }
-      double Z[9];
-      sym3_full(lm_z + 6 * l, Z);
-      for (int a = 0; a < 3; ++a) u[a] = Z[a * 3] * tv[0] + Z[a * 3 + 1] * tv[1] + Z[a * 3 + 2] * tv[2];
-      uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  }
+  for (int k = 0; k < 3; ++k) red[tid * 3 + k] = c[k];
+  seg[tid] = active ? l : -1;
+  __syncthreads();
+  double uu = 0.0;
+  if (active && (tid == 0 || seg[tid - 1] != l)) {
+    const double* acc = lm_acc + (size_t)kLmAcc * l;
+    double tv[3] = {acc[12], acc[13], acc[14]};
+    for (int k = tid; k < kObsTile && seg[k] == l; ++k)
+      for (int a = 0; a < 3; ++a) tv[a] += red[k * 3 + a];
+    double u[3], Mwa[3];
+    landmark_step(t, l, vR, lm, lm_acc, lm_z, w, tv, u, Mwa);
+    uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    for (int a = 0; a < 3; ++a) {
+      u_out[3 * l + a] = u[a];
+      ush[tid * 6 + a] = u[a];
+      ush[tid * 6 + 3 + a] = Mwa[a];
     }
-    u_out[3 * l] = u[0]; u_out[3 * l + 1] = u[1]; u_out[3 * l + 2] = u[2];
+  }
+  __syncthreads();
+  double mcc = 0.0;
+  if (active) {
+    const int h = t.lm_begin[l] - b;
+    double lu[3];
+    for (int a = 0; a < 3; ++a) lu[a] = ush[h * 6 + a] + (coupled ? ush[h * 6 + 3 + a] : 0.0);
+    double jy[2];
+    for (int rr = 0; rr < 2; ++rr) jy[rr] = jl[rr * 3] * lu[0] + jl[rr * 3 + 1] * lu[1] + jl[rr * 3 + 2] * lu[2] + jw[rr];
+    mcc = jy[0] * r[0] + jy[1] * r[1] - 0.5 * (jy[0] * jy[0] + jy[1] * jy[1]);
+  }
+  const double s1 = block_sum256(mcc, lds4);
+  const double s2 = block_sum256(uu, lds4);
+  if (tid == 0) {
+    part[2 * blockIdx.x] = s1;
+    part[2 * blockIdx.x + 1] = s2;
+  }
+}
+
+// The same for the landmarks split over several tiles, one thread each.
+__global__ __launch_bounds__(256) void k_back_substitute_split(DevTopology t, int32_t n_split,
+                                                               const int32_t* __restrict__ split_ids,
+                                                               const double* __restrict__ vR,
+                                                               const double* __restrict__ lm,
+                                                               const double* __restrict__ obs_rec,
+                                                               const double* __restrict__ lm_acc,
+                                                               const double* __restrict__ lm_z,
+                                                               const double* __restrict__ w,
+                                                               double* __restrict__ u_out, double* __restrict__ part) {
+  __shared__ double lds4[4];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  double mcc = 0.0, uu = 0.0;
+  if (k < n_split && (t.lm_flags[split_ids[k]] & kLmActive)) {
+    const int l = split_ids[k];
+    const int lb = t.lm_begin[l], le = t.lm_begin[l + 1];
+    const double* acc = lm_acc + (size_t)kLmAcc * l;
+    double tv[3] = {acc[12], acc[13], acc[14]};
+    for (int o = lb; o < le; ++o) {
+      const int32_t meta = t.obs_meta[o];
+      if (obs_type(meta) == kTypeKeyframe || !obs_observer_var(meta)) continue;
+      const double* rec = obs_rec + (size_t)kObsRecord * o;
+      const double* wo = w + 6 * t.vertex_var[t.obs_vertex[o]];
+      double jw[2];
+      for (int rr = 0; rr < 2; ++rr) {
+        double s = 0.0;
+        for (int c = 0; c < 6; ++c) s += rec[8 + rr * 6 + c] * wo[c];
+        jw[rr] = s;
+      }
+      for (int a = 0; a < 3; ++a) tv[a] -= rec[2 + a] * jw[0] + rec[5 + a] * jw[1];
+    }
+    double u[3], Mwa[3];
+    landmark_step(t, l, vR, lm, lm_acc, lm_z, w, tv, u, Mwa);
+    uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    for (int a = 0; a < 3; ++a) u_out[3 * l + a] = u[a];
     for (int o = lb; o < le; ++o) {
       const int32_t meta = t.obs_meta[o];
       const double* rec = obs_rec + (size_t)kObsRecord * o;
-      const double* Jl = rec + 2;
-      double lu[3] = {u[0], u[1], u[2]};
       const bool coupled = obs_type(meta) != kTypeKeyframe;
-      if (coupled && avar) { lu[0] += Mwa[0]; lu[1] += Mwa[1]; lu[2] += Mwa[2]; }
+      double lu[3];
+      for (int a = 0; a < 3; ++a) lu[a] = u[a] + (coupled ? Mwa[a] : 0.0);
       double jy[2];
-      for (int r = 0; r < 2; ++r) jy[r] = Jl[r * 3] * lu[0] + Jl[r * 3 + 1] * lu[1] + Jl[r * 3 + 2] * lu[2];
+      for (int rr = 0; rr < 2; ++rr) jy[rr] = rec[2 + rr * 3] * lu[0] + rec[3 + rr * 3] * lu[1] + rec[4 + rr * 3] * lu[2];
       if (coupled && obs_observer_var(meta)) {
         const double* wo = w + 6 * t.vertex_var[t.obs_vertex[o]];
-        const double* Jo = rec + 8;
-        for (int r = 0; r < 2; ++r)
-          for (int c = 0; c < 6; ++c) jy[r] += Jo[r * 6 + c] * wo[c];
+        for (int rr = 0; rr < 2; ++rr)
+          for (int c = 0; c < 6; ++c) jy[rr] += rec[8 + rr * 6 + c] * wo[c];
       }
       mcc += jy[0] * rec[0] + jy[1] * rec[1] - 0.5 * (jy[0] * jy[0] + jy[1] * jy[1]);
     }
-  } else if (l < t.L) {
-    u_out[3 * l] = u_out[3 * l + 1] = u_out[3 * l + 2] = 0.0;
   }
   const double s1 = block_sum256(mcc, lds4);
   const double s2 = block_sum256(uu, lds4);
@@ -1303,10 +1396,19 @@ void launch_block_inverse(hipStream_t s, int32_t Pvar, const int32_t* diag_pos, 
   if (Pvar > 0) k_block_inverse<<<(Pvar + 63) / 64, 64, 0, s>>>(Pvar, diag_pos, S, Minv, fail);
 }
 
-void launch_back_substitute(hipStream_t s, const DevTopology& t, const double* vR, const double* lm,
-                            const double* obs_rec, const double* lm_acc, const double* lm_z, const double* w,
-                            double* u, double* part) {
-  if (t.L > 0) k_back_substitute<<<(t.L + 255) / 256, 256, 0, s>>>(t, vR, lm, obs_rec, lm_acc, lm_z, w, u, part);
+// Part entries: 2 per tile, then 2 per block of 256 split landmarks; returns
+// their count.
+int32_t launch_back_substitute(hipStream_t s, const DevTopology& t, int32_t tiles, const int32_t* tile_begin,
+                               int32_t n_split, const int32_t* split_ids, const double* vR, const double* lm,
+                               const double* obs_rec, const double* lm_acc, const double* lm_z, const double* w,
+                               double* u, double* part) {
+  if (tiles > 0)
+    k_back_substitute<<<tiles, kObsTile, 0, s>>>(t, tile_begin, vR, lm, obs_rec, lm_acc, lm_z, w, u, part);
+  const int sb = (n_split + 255) / 256;
+  if (sb > 0)
+    k_back_substitute_split<<<sb, 256, 0, s>>>(t, n_split, split_ids, vR, lm, obs_rec, lm_acc, lm_z, w, u,
+                                               part + 2 * (size_t)std::max(tiles, 0));
+  return std::max(tiles, 0) + sb;
 }
 void launch_unscale_poses(hipStream_t s, int32_t Pvar, const double* y, const double* pose_scale, double* w,
                           double* part) {

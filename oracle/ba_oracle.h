@@ -81,6 +81,17 @@ int32_t oracle_solve_with_outlier_rejection(const mlba_problem_desc* desc,
 int32_t oracle_landmark_quality(const mlba_problem_desc* desc, const mlba_quality_options* options,
                                 uint8_t* good);
 
+/* InertialErrorTerm::Evaluate (inertial-error-term.cc:73-286) for one edge:
+ * n IMU samples imu (n x 6: accelerometer, gyroscope) at t_ns (n), noise =
+ * {gyro noise, gyro bias, acc noise, acc bias sigmas, gravity}, params =
+ * {pose_from (7), b_g_from, v_from, b_a_from, pose_to (7), b_g_to, v_to,
+ * b_a_to}. residual (15, nullable), jacobians[8] (nullable each, row-major
+ * 15 x {7,3,3,3,7,3,3,3}, ambient as the reference), and optionally the
+ * accumulated transition (15x15), covariance (15x15) and end state (16). */
+int32_t oracle_inertial_evaluate(int32_t n, const double* imu, const int64_t* t_ns, const double* noise,
+                                 const double* const* params, double* residual, double* const* jacobians,
+                                 double* phi_out, double* Q_out, double* end_state_out);
+
 /* The damped, Jacobi-scaled reduced camera system at the given state:
  * S (n x n dense, n = 6 * number of variable vertices that have residuals,
  * ordered by vertex index) and rhs (n), for LM radius `mu`. Used to check the

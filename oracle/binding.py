@@ -190,3 +190,29 @@ def landmark_quality(vmap, options):
     rc = lib.oracle_landmark_quality(ctypes.byref(d), ctypes.byref(options), _p(good))
     assert rc == 0, rc
     return good.astype(bool)
+
+
+def inertial_evaluate(imu, t_ns, noise, params, want_jac=True):
+    """InertialErrorTerm::Evaluate for one edge: returns (residual (15),
+    jacobians [8] ambient 15 x {7,3,3,3,7,3,3,3}, phi_accum, Q_accum,
+    end_state (16))."""
+    lib = load()
+    fn = lib.oracle_inertial_evaluate
+    fn.restype = ctypes.c_int32
+    fn.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                   ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
+                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    imu = np.ascontiguousarray(imu, dtype=np.float64).reshape(-1, 6)
+    t = np.ascontiguousarray(t_ns, dtype=np.int64)
+    nz = np.ascontiguousarray(noise, dtype=np.float64)
+    ps = [np.ascontiguousarray(p, dtype=np.float64) for p in params]
+    pp = (ctypes.c_void_p * 8)(*[p.ctypes.data for p in ps])
+    r = np.zeros(15)
+    sizes = (7, 3, 3, 3, 7, 3, 3, 3)
+    jac = [np.zeros((15, k)) for k in sizes]
+    jp = (ctypes.c_void_p * 8)(*[j.ctypes.data for j in jac])
+    phi, Q, xe = np.zeros((15, 15)), np.zeros((15, 15)), np.zeros(16)
+    rc = fn(imu.shape[0], imu.ctypes.data, t.ctypes.data, nz.ctypes.data, pp, r.ctypes.data,
+            jp if want_jac else None, phi.ctypes.data, Q.ctypes.data, xe.ctypes.data)
+    assert rc == 0, rc
+    return r, jac, phi, Q, xe

@@ -36,6 +36,14 @@ def test_library_exports_every_optimizer_symbol():
         assert hasattr(lib, s), s
 
 
+def test_library_exports_every_inertial_symbol():
+    lib = abi.load_library()
+    syms = declared("maplab_ba_inertial.h")
+    assert syms == set(abi.INERTIAL_SYMBOLS), syms ^ set(abi.INERTIAL_SYMBOLS)
+    for s in syms:
+        assert hasattr(lib, s), s
+
+
 def test_synth_library_exports_every_declared_symbol():
     lib = abi.load_synth()
     syms = declared("maplab_ba_synth.h")
@@ -62,6 +70,7 @@ def test_struct_layouts_match_header():
     (abi.QualityOptions, "mlba_quality_options", "maplab_ba.h"),
     (abi.ViProblemOptions, "mlba_vi_problem_options", "maplab_ba_optimizer.h"),
     (abi.ProblemSelection, "mlba_problem_selection", "maplab_ba_optimizer.h"),
+    (abi.InertialProblem, "mlba_inertial_problem", "maplab_ba_inertial.h"),
 ])
 def test_struct_layout_matches_compiled_header(tmp_path, struct, cname, header):
     # The C compiler's sizeof / offsetof of every field equals the ctypes mirror.
@@ -135,3 +144,28 @@ def test_invalid_problem_is_rejected(mutate, code, text):
     mutate(m)
     rc, msg = _create(m)
     assert rc == code and text in msg, (rc, msg)
+
+
+def _edges(**kw):
+    from maplab_amd.inertial import InertialEdges
+    base = dict(vertices=np.tile([0, 0, 0, 1.0, 0, 0, 0], (2, 1)), gyro_bias=np.zeros((2, 3)),
+                velocity=np.zeros((2, 3)), accel_bias=np.zeros((2, 3)), edge_from=[0], edge_to=[1],
+                edge_sample_begin=[0, 3], imu=np.tile([0, 0, 9.81, 0, 0, 0], (3, 1)),
+                timestamps_ns=np.array([0, 5000000, 10000000], dtype=np.int64))
+    base.update(kw)
+    return InertialEdges(**base)
+
+
+@pytest.mark.parametrize("kw, text", [
+    (dict(noise=(0.0, 1e-4, 1e-2, 1e-3)), "sigmas"),
+    (dict(edge_sample_begin=[0, 0]), "without IMU data"),
+    (dict(timestamps_ns=np.array([0, 5000000, 4000000], dtype=np.int64)), "not properly ordered"),
+    (dict(edge_to=[5]), "out of range"),
+])
+def test_inertial_rejects_invalid_edges(kw, text):
+    # The reference CHECK-fails on these (inertial-error-term.h:84-89,
+    # inertial-error-term.cc:36-37); the library returns EINVAL before any
+    # device work.
+    from maplab_amd import inertial
+    with pytest.raises(RuntimeError, match=text):
+        inertial.evaluate(_edges(**kw))
