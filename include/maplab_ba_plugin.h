@@ -1,0 +1,64 @@
+/*
+ * maplab_ba_plugin.h -- console-plugin boundary of the map-optimization
+ * plugin (console-plugins/map-optimization-plugin/src/optimizer-plugin.cc:
+ * 27-55, 190-235). libmaplab_ba_plugin.so exports the two symbols maplab's
+ * console resolves with dlsym (console-plugin-base.h:54-58,
+ * console-plugin-base-with-plotter.h:43-56), createConsolePlugin and
+ * destroyConsolePlugin, under the plugin id "map-optimization", with the
+ * commands optimize_visual / optv and optimize_visual_inertial / optvi run
+ * synchronously. Their options come from the reference's flag names
+ * (--ba_*, --elq_*, --map_mission, --map_mission_list) given on the command
+ * line. The console is the small host structure below: it holds the selected
+ * map (the VIMapManager map under write access in the reference).
+ */
+#ifndef MAPLAB_BA_PLUGIN_H_
+#define MAPLAB_BA_PLUGIN_H_
+
+#include <stdint.h>
+
+#include "maplab_ba.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* common::CommandStatus (console-common/command-registerer.h:19-24). */
+#define MLBA_CMD_SUCCESS 0
+#define MLBA_CMD_UNKNOWN_ERROR 1
+#define MLBA_CMD_STUPID_USER_ERROR 2
+
+/* The selected map: its description, and the state buffers the command
+ * writes back into (7 P vertices, 3 L landmarks, L landmark qualities,
+ * MLBA_QUALITY_* of maplab_ba_optimizer.h; qualities nullable). */
+typedef struct mlba_console_map {
+  mlba_problem_desc desc;
+  double* vertices;
+  double* landmarks;
+  uint8_t* landmark_quality;
+} mlba_console_map;
+
+/* The console a plugin is created for. selected_map NULL = no map selected. */
+typedef struct mlba_console {
+  mlba_console_map* selected_map;
+  mlba_summary* last_summary;     /* nullable: receives the summary of the last optimization */
+} mlba_console;
+
+/* Plugin lifetime (the console's dlsym names; plotter unused, may be NULL). */
+void* createConsolePlugin(mlba_console* console, void* plotter);
+void destroyConsolePlugin(void* plugin);
+
+/* The plugin id ("map-optimization") and its command names. */
+const char* mlba_plugin_id(const void* plugin);
+int32_t mlba_plugin_num_commands(const void* plugin);
+const char* mlba_plugin_command_name(const void* plugin, int32_t index);
+
+/* Runs one console line, "<command> [--flag=value | --flag value | --flag |
+ * --noflag]...", and returns its CommandStatus; unknown commands and flags
+ * are user errors. The reason of a failure is in mlba_plugin_last_message. */
+int32_t mlba_plugin_process_command(void* plugin, const char* line);
+const char* mlba_plugin_last_message(const void* plugin);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MAPLAB_BA_PLUGIN_H_ */

@@ -1,0 +1,98 @@
+"""ctypes host of the map-optimization console plugin
+(libmaplab_ba_plugin.so, include/maplab_ba_plugin.h): what maplab's console
+does with a plugin -- dlopen, createConsolePlugin, run a command line on the
+selected map, destroyConsolePlugin."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import abi
+from .problem import VisualMap
+
+PLUGIN_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmaplab_ba_plugin.so")
+CMD_SUCCESS, CMD_UNKNOWN_ERROR, CMD_STUPID_USER_ERROR = 0, 1, 2
+
+
+class ConsoleMap(ctypes.Structure):
+    _fields_ = [("desc", abi.ProblemDesc), ("vertices", ctypes.c_void_p), ("landmarks", ctypes.c_void_p),
+                ("landmark_quality", ctypes.c_void_p)]
+
+
+class Console(ctypes.Structure):
+    _fields_ = [("selected_map", ctypes.POINTER(ConsoleMap)), ("last_summary", ctypes.POINTER(abi.Summary))]
+
+
+PLUGIN_SYMBOLS = {
+    "createConsolePlugin": (ctypes.c_void_p, [ctypes.POINTER(Console), ctypes.c_void_p]),
+    "destroyConsolePlugin": (None, [ctypes.c_void_p]),
+    "mlba_plugin_id": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "mlba_plugin_num_commands": (ctypes.c_int32, [ctypes.c_void_p]),
+    "mlba_plugin_command_name": (ctypes.c_char_p, [ctypes.c_void_p, ctypes.c_int32]),
+    "mlba_plugin_process_command": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_char_p]),
+    "mlba_plugin_last_message": (ctypes.c_char_p, [ctypes.c_void_p]),
+}
+
+
+def load_plugin_library(path: str = PLUGIN_PATH):
+    if not os.path.exists(path):
+        raise OSError(f"{path} is missing: build it with `make -C maplab_amd/csrc`")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in PLUGIN_SYMBOLS.items():
+        fn = getattr(lib, name)
+        fn.restype, fn.argtypes = res, args
+    return lib
+
+
+class MapConsole:
+    """A console with one selected map (VisualMap state buffers are written back in place)."""
+
+    def __init__(self, vmap: VisualMap | None = None, capacity: int = 1024):
+        self.lib = load_plugin_library()
+        self._its = (abi.IterationSummary * capacity)()
+        self.summary = abi.Summary()
+        self.summary.iterations_capacity = capacity
+        self.summary.iterations = ctypes.cast(self._its, ctypes.POINTER(abi.IterationSummary))
+        self.console = Console()
+        self.console.last_summary = ctypes.pointer(self.summary)
+        self.map = None
+        if vmap is not None:
+            self.select(vmap)
+        self.plugin = self.lib.createConsolePlugin(ctypes.byref(self.console), None)
+
+    def select(self, vmap: VisualMap) -> None:
+        self.vmap = vmap.normalized()
+        self.quality = np.zeros(vmap.num_landmarks, dtype=np.uint8)
+        self._state_v = self.vmap.vertices.copy()
+        self._state_l = self.vmap.landmarks.copy()
+        self.map = ConsoleMap(desc=self.vmap.desc(), vertices=self._state_v.ctypes.data,
+                              landmarks=self._state_l.ctypes.data, landmark_quality=self.quality.ctypes.data)
+        self.console.selected_map = ctypes.pointer(self.map)
+
+    @property
+    def state(self):
+        return self._state_v, self._state_l
+
+    def plugin_id(self) -> str:
+        return self.lib.mlba_plugin_id(self.plugin).decode()
+
+    def commands(self) -> list[str]:
+        return [self.lib.mlba_plugin_command_name(self.plugin, i).decode()
+                for i in range(self.lib.mlba_plugin_num_commands(self.plugin))]
+
+    def run(self, line: str) -> tuple[int, str]:
+        rc = self.lib.mlba_plugin_process_command(self.plugin, line.encode())
+        return rc, self.lib.mlba_plugin_last_message(self.plugin).decode()
+
+    def close(self) -> None:
+        if self.plugin:
+            self.lib.destroyConsolePlugin(self.plugin)
+            self.plugin = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
