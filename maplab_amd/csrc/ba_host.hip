@@ -64,7 +64,7 @@ int launch_pcg_deflated(hipStream_t s, int32_t Pvar, int32_t K, const int32_t* r
                         double* SR, double* E, double* Einv, double* y,
                         double* x, double* r, double* z, double* p0, double* p1, double* q,
                         double* part, double* part2, void* state_dev, void* state_host, double tol,
-                        int32_t max_iters, int32_t poll, const hipEvent_t* probe);
+                        int32_t max_iters, int32_t first, int32_t poll, const hipEvent_t* probe);
 int32_t pcg_group_rows();
 int32_t launch_back_substitute(hipStream_t s, const DevTopology& t, int32_t tiles, const int32_t* tile_begin,
                                int32_t n_split, const int32_t* split_ids, const double* vR, const double* lm,
@@ -88,6 +88,12 @@ using namespace mlba;
 namespace {
 
 thread_local std::string g_create_error;
+
+// PCG iterations between host convergence polls. A solve's first poll comes
+// after the previous solve's iteration count plus half of this: successive
+// LM solves take similar counts, a poll costs a host round trip (tens of
+// microseconds) and an iteration past convergence only two early-exit launches.
+constexpr int32_t kPcgPoll = 16;
 
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -160,6 +166,7 @@ struct mlba_handle {
   int32_t grp_width = 0;                // columns of the per-group SW window
   DBuf<double> hat_w, defl_SR, defl_E, defl_Einv, defl_y, pcg_part2;
   int32_t hat_K = -1;  // coarse nodes of the current hat basis
+  int32_t pcg_last_iters = 0;  // iterations of the previous solve: length of the next solve's first batch
   DBuf<double> S, rhs, pose_grad, pose_diag, pose_jtj, pose_scale, w, Minv;
   DBuf<double> px, pr, pz, pp, pp2, pq, pcg_part, parts_a, parts_b, parts_c, parts_l, parts_v, parts_w, parts_g;
   DBuf<double> scal;
@@ -318,6 +325,7 @@ int build_topology(mlba_handle* h) {
   }
   h->nnzb = (int32_t)col_var.size();
   h->hat_K = -1;
+  h->pcg_last_iters = 0;
   h->h_row_ptr = row_ptr;
   h->h_col_var = col_var;
   // Upper blocks of S and, for each, the incidence pairs (x in row i, y in
@@ -681,8 +689,10 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
             s, h->Pvar, h->hat_K, h->row_ptr.p, h->col_var.p, h->S.p, h->Minv.p, h->rhs.p, h->hat_node.p,
             h->hat_w.p, h->node_range.p, h->grp_lo.p, h->grp_width, h->defl_SR.p, h->defl_E.p, h->defl_Einv.p, h->defl_y.p, h->px.p, h->pr.p, h->pz.p,
             h->pp.p, h->pp2.p, h->pq.p, h->pcg_part.p, h->pcg_part2.p, h->pcg_state.p, h->pcg_host,
-            o->pcg_relative_tolerance, o->pcg_max_iterations, 16, nullptr);
+            o->pcg_relative_tolerance, o->pcg_max_iterations,
+            h->pcg_last_iters > 0 ? h->pcg_last_iters + kPcgPoll / 2 : kPcgPoll, kPcgPoll, nullptr);
         if (pcg_iters < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
+        h->pcg_last_iters = pcg_iters;
       }
       launch_unscale_poses(s, h->Pvar, h->px.p, h->pose_scale.p, h->w.p, h->parts_w.p);
       int32_t nbs = 0;
@@ -1295,7 +1305,7 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
                                   h->hat_node.p, h->hat_w.p, h->node_range.p, h->grp_lo.p, h->grp_width, h->defl_SR.p,
                                   h->defl_E.p, h->defl_Einv.p, h->defl_y.p, h->px.p, h->pr.p, h->pz.p, h->pp.p, h->pp2.p, h->pq.p,
                                   h->pcg_part.p, h->pcg_part2.p, h->pcg_state.p, h->pcg_host,
-                                  o->pcg_relative_tolerance, o->pcg_max_iterations, 16, nullptr);
+                                  o->pcg_relative_tolerance, o->pcg_max_iterations, kPcgPoll, kPcgPoll, nullptr);
     });
     t->pcg_iterations = iters;
     // One more iteration, each kernel bracketed by events, averaged over repeats.
@@ -1309,7 +1319,7 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
                                          h->grp_width, h->defl_SR.p, h->defl_E.p, h->defl_Einv.p, h->defl_y.p, h->px.p, h->pr.p,
                                          h->pz.p, h->pp.p, h->pp2.p, h->pq.p, h->pcg_part.p, h->pcg_part2.p,
                                          h->pcg_state.p, h->pcg_host, o->pcg_relative_tolerance,
-                                         std::min<int32_t>(o->pcg_max_iterations, 4), 16, pe);
+                                         std::min<int32_t>(o->pcg_max_iterations, 4), kPcgPoll, kPcgPoll, pe);
       if (it < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
       HIP_OK(hipEventSynchronize(pe[2]));
       float a = 0.f, b = 0.f;

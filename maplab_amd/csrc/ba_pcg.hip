@@ -19,9 +19,9 @@
 // Per LM step: k_defl_sr (SW), k_defl_e (E), k_defl_prep (state, E^-1 by
 // Gauss-Jordan, y = E^-1 W'b), k_defl_init (x0, r0, z0 and the first
 // per-group partials). Per PCG iteration: k_pcg_spmv (sums the previous
-// per-group partials, forms p = z - W mu + beta p_prev on the fly, q = S p)
-// and k_pcg_update (x, r, z and the per-group partials r'z, r'r,
-// E^-1 (SW)'z). Every work-group sums partials in the same fixed order: the
+// per-group partials; p = z - W mu + beta p_prev and
+// q = S z + beta S p_prev - (SW) mu) and k_pcg_update (x, r, z and the
+// per-group partials r'z, r'r, E^-1 (SW)'z). Every work-group sums partials in the same fixed order: the
 // iteration is deterministic and uses no atomics.
 #include "ba_device.h"
 #include "ba_reduce.h"
@@ -251,10 +251,59 @@ __global__ __launch_bounds__(kPrepThreads) void k_defl_prep(PcgArgs a, double to
   }
 }
 
+// Operands of a group's partials that do not change during the solve, loaded
+// into registers at the start of the per-group kernels (kGroupBlock threads)
+// so that their latency overlaps the kernel's reductions: wave w holds the
+// SW window columns w + 16 m, its lanes the group entries lane + 64 j; thread
+// (row, part) of E^-1 holds the entries of its part of the window columns.
+constexpr int kGroupWaves = kGroupBlock / 64;
+constexpr int kSrCols = (kMaxCoarse + kGroupWaves - 1) / kGroupWaves;
+constexpr int kSrRows = (kGroupThreads + 63) / 64;
+constexpr int einv_per_thread() {
+  int m = 0;
+  for (int ne = 6; ne <= kMaxCoarse; ne += 6) {
+    const int parts = kGroupBlock / ne, len = (ne + parts - 1) / parts;
+    m = len > m ? len : m;
+  }
+  return m;
+}
+constexpr int kEinvPer = einv_per_thread();
+
+struct GroupLoads {
+  double sr[kSrCols][kSrRows];
+  double einv[kEinvPer];
+};
+
+__device__ inline void load_group(const PcgArgs& a, int g, GroupLoads& L) {
+  const int ne = a.ne, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+#pragma unroll
+  for (int m = 0; m < kSrCols; ++m) {
+    const int lc = wave + kGroupWaves * m;
+#pragma unroll
+    for (int j = 0; j < kSrRows; ++j) {
+      const int t = lane + 64 * j;
+      L.sr[m][j] = (ne && lc < a.gw && t < kGroupThreads) ? sr_col(a, g, lc)[t] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < kEinvPer; ++m) L.einv[m] = 0.0;
+  if (ne == 0) return;
+  const int nparts = kGroupBlock / ne, rrow = tid % ne, part = tid / ne;
+  const int c0 = 6 * a.grp_lo[g], w = min(a.gw, ne - c0), len = (w + nparts - 1) / nparts;
+  if (part < nparts) {
+#pragma unroll
+    for (int m = 0; m < kEinvPer; ++m) {
+      const int lc = part * len + m;
+      if (m < len && lc < w) L.einv[m] = a.Einv[(size_t)(c0 + lc) * ne + rrow];  // E^-1 is symmetric
+    }
+  }
+}
+
 // Per-group partials of group g: out = {rz, rr, bb, E^-1 (SW)'z}; zloc holds
-// the group's z (0 past the last row), cs (ne) is zero on entry.
-__device__ inline void group_partials(const PcgArgs& a, int g, const double* zloc, double rz, double rr, double bb,
-                                      double* cs, double* mpart, double* out) {
+// the group's z (0 past the last row). (SW)'z is non-zero only on the group's
+// window of columns, so E^-1 (SW)'z reads only those columns of E^-1.
+__device__ inline void group_partials(const PcgArgs& a, int g, const GroupLoads& L, const double* zloc, double rz,
+                                      double rr, double bb, double* cs, double* mpart, double* out) {
   const int ne = a.ne, tid = threadIdx.x;
   if (tid == 0) {
     out[0] = rz;
@@ -262,29 +311,31 @@ __device__ inline void group_partials(const PcgArgs& a, int g, const double* zlo
     out[2] = bb;
   }
   if (ne == 0) return;
-  // (SW)'z: wave w takes the window columns w, w + waves, ...; its lanes sum
-  // the group entries t = lane + 64 m.
-  const int nw = blockDim.x >> 6, wave = tid >> 6, lane = tid & 63;
-  const int c0 = 6 * a.grp_lo[g];
-  for (int lc = wave; lc < a.gw; lc += nw) {
-    const double* col = sr_col(a, g, lc);
-    double s = 0.0;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int c0 = 6 * a.grp_lo[g], w = min(a.gw, ne - c0);
 #pragma unroll
-    for (int m = 0; m < (kGroupThreads + 63) / 64; ++m) {
-      const int t = lane + 64 * m;
-      if (t < kGroupThreads) s += col[t] * zloc[t];
+  for (int m = 0; m < kSrCols; ++m) {
+    const int lc = wave + kGroupWaves * m;
+    if (lc < w) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < kSrRows; ++j) {
+        const int t = lane + 64 * j;
+        if (t < kGroupThreads) s += L.sr[m][j] * zloc[t];
+      }
+      s = wave_sum(s);
+      if (lane == 0) cs[lc] = s;
     }
-    s = wave_sum(s);
-    if (lane == 0 && c0 + lc < ne) cs[c0 + lc] = s;
   }
   __syncthreads();
-  // E^-1 cs: thread (row, part) sums a contiguous part of the columns.
-  const int nparts = blockDim.x / ne, rrow = tid % ne, part = tid / ne;
-  const int len = (ne + nparts - 1) / nparts;
+  const int nparts = kGroupBlock / ne, part = tid / ne, len = (w + nparts - 1) / nparts;
   double m = 0.0;
   if (part < nparts) {
-    const int c_end = min(ne, (part + 1) * len);
-    for (int c = part * len; c < c_end; ++c) m += a.Einv[c * ne + rrow] * cs[c];  // E^-1 is symmetric
+#pragma unroll
+    for (int j = 0; j < kEinvPer; ++j) {
+      const int lc = part * len + j;
+      if (j < len && lc < w) m += L.einv[j] * cs[lc];
+    }
   }
   mpart[tid] = m;
   __syncthreads();
@@ -305,10 +356,9 @@ __global__ __launch_bounds__(kGroupBlock) void k_defl_init(PcgArgs a) {
   __shared__ double mpart[kGroupBlock];
   __shared__ double ys[kMaxCoarse];
   const int ne = a.ne, tid = threadIdx.x, P = a.Pvar, g = blockIdx.x;
-  for (int c = tid; c < ne; c += blockDim.x) {
-    ys[c] = a.y[c];
-    cs[c] = 0.0;
-  }
+  GroupLoads L;
+  load_group(a, g, L);
+  for (int c = tid; c < ne; c += blockDim.x) ys[c] = a.y[c];
   __syncthreads();
   const int row = g * kRowsPerGroup + tid / 6, comp = tid % 6;
   const bool active = tid < kGroupThreads && row < P;
@@ -341,22 +391,52 @@ __global__ __launch_bounds__(kGroupBlock) void k_defl_init(PcgArgs a) {
   if (tid < kGroupThreads) zloc[tid] = zv;
   double rz = rv * zv, rr = rv * rv, bb = bv * bv;
   block_sum3(rz, rr, bb, lds);
-  group_partials(a, g, zloc, rz, rr, bb, cs, mpart, a.part2 + (size_t)g * (3 + ne));
+  group_partials(a, g, L, zloc, rz, rr, bb, cs, mpart, a.part2 + (size_t)g * (3 + ne));
 }
 
-// Iteration k: sums the per-group partials of the previous update (or of
-// k_defl_init), stops when |r|^2 <= tol2 or k = max_iters, else forms
-// p_k = z - W mu + beta p_{k-1} on the fly, q = S p_k (the own row's p_k is
-// stored) and part[i] = p_k,i . q_i. Thread t < 252 handles component t % 6
-// of every 42nd block of the row.
+// Iteration k: stops when |r|^2 <= tol2 or k = max_iters, else
+// q = S p_k with p_k = z - W mu + beta p_{k-1}, evaluated as
+// S z + beta S p_{k-1} - (SW) mu so that the row's block products (the bulk
+// of the loads) do not wait for the sums of the previous update's per-group
+// partials (r'z, r'r and mu); the own row's p_k is stored and
+// part[i] = p_k,i . q_i. Thread t < 252 handles component t % 6 of every 42nd
+// block of the row and window columns t / 6 + 42 m of (SW)_i.
 __global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a, int32_t k, const double* __restrict__ p_prev,
                                                   double* __restrict__ p_cur) {
   __shared__ double chunk[kPartChunk];
   __shared__ double sums[3 + kMaxCoarse];
-  __shared__ double ysum[kGroupThreads];
+  __shared__ double ysz[kGroupThreads], ysp[kGroupThreads], ysw[kGroupThreads];
   PcgState* st = a.st;
-  if (st->done) return;
+  const int32_t done = st->done, max_iters = st->max_iters;
+  const double rz_prev = st->rz[(k + 1) & 1], tol2_prev = st->tol2, tol_rel2 = st->tol_rel2;
   const int ne = a.ne, tid = threadIdx.x, stride = 3 + ne;
+  const int i = blockIdx.x, comp = tid % 6, g = tid / 6;
+  // Block products, independent of the reductions.
+  constexpr int kSwCols = (kMaxCoarse + kRowsPerGroup - 1) / kRowsPerGroup;
+  double sz = 0.0, sp = 0.0, sw[kSwCols] = {};
+  const int gi = i / kRowsPerGroup, c0 = 6 * (ne ? a.grp_lo[gi] : 0);
+  if (tid < kGroupThreads) {
+    const int rb = a.row_ptr[i], re = a.row_ptr[i + 1];
+#pragma unroll 1
+    for (int kb = rb + g; kb < re; kb += kRowsPerGroup) {
+      const int j = a.col_var[kb];
+      const double* srow = a.S + (size_t)36 * kb + 6 * comp;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        sz += srow[c] * a.z[6 * j + c];
+        sp += srow[c] * p_prev[6 * j + c];
+      }
+    }
+    if (ne) {
+      const int t = 6 * (i - gi * kRowsPerGroup) + comp;
+#pragma unroll
+      for (int m = 0; m < kSwCols; ++m) {
+        const int lc = g + m * kRowsPerGroup;
+        if (lc < a.gw) sw[m] = sr_col(a, gi, lc)[t];
+      }
+    }
+  }
+  if (done) return;  // the same value in every thread of the work-group
   // Fixed-order sum of the group partials, a chunk of whole groups per pass.
   const int per_chunk = kPartChunk / stride;
   double acc = 0.0;
@@ -366,7 +446,7 @@ __global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a, int32_t k, const do
     for (int q = tid; q < ng * stride; q += blockDim.x) chunk[q] = a.part2[(size_t)g0 * stride + q];
     __syncthreads();
     if (tid < stride)
-      for (int g = 0; g < ng; ++g) acc += chunk[g * stride + tid];
+      for (int gg = 0; gg < ng; ++gg) acc += chunk[gg * stride + tid];
   }
   if (tid < stride) sums[tid] = acc;
   __syncthreads();
@@ -376,15 +456,15 @@ __global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a, int32_t k, const do
   bool stop;
   if (k == 0) {
     const double bb = sums[2];
-    tol2 = st->tol_rel2 * bb;
-    stop = bb <= 0.0 || st->max_iters <= 0 || rr <= tol2;
+    tol2 = tol_rel2 * bb;
+    stop = bb <= 0.0 || max_iters <= 0 || rr <= tol2;
     if (blockIdx.x == 0 && tid == 0) {
       st->bb = bb;
       st->tol2 = tol2;
     }
   } else {
-    tol2 = st->tol2;
-    stop = rr <= tol2 || k >= st->max_iters;
+    tol2 = tol2_prev;
+    stop = rr <= tol2 || k >= max_iters;
   }
   if (stop) {
     if (blockIdx.x == 0 && tid == 0) {
@@ -393,35 +473,34 @@ __global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a, int32_t k, const do
     }
     return;
   }
-  const double beta = k > 0 ? rz / st->rz[(k - 1) & 1] : 0.0;
+  const double beta = k > 0 ? rz / rz_prev : 0.0;
   if (blockIdx.x == 0 && tid == 0) {
     st->rz[k & 1] = rz;
     st->iters = k;
   }
-  const int i = blockIdx.x;
-  const int rb = a.row_ptr[i], re = a.row_ptr[i + 1];
-  double y = 0.0;
   if (tid < kGroupThreads) {
-    const int comp = tid % 6, g = tid / 6;
-    for (int kb = rb + g; kb < re; kb += kRowsPerGroup) {
-      const int j = a.col_var[kb];
-      const double* srow = a.S + (size_t)36 * kb + 6 * comp;
-      const int n0 = a.hat_node[j];
-      const double w0 = a.hat_w[2 * j], w1 = a.hat_w[2 * j + 1];
-      double s = 0.0;
-      for (int c = 0; c < 6; ++c) {
-        double v = a.z[6 * j + c] + beta * p_prev[6 * j + c];
-        if (ne) v -= w0 * mu_s[6 * n0 + c] + (w1 != 0.0 ? w1 * mu_s[6 * (n0 + 1) + c] : 0.0);
-        s += srow[c] * v;
+    double w = 0.0;
+    if (ne) {
+#pragma unroll
+      for (int m = 0; m < kSwCols; ++m) {
+        const int lc = g + m * kRowsPerGroup;
+        if (lc < a.gw && c0 + lc < ne) w += sw[m] * mu_s[c0 + lc];
       }
-      y += s;
     }
-    ysum[tid] = y;
+    ysz[tid] = sz;
+    ysp[tid] = sp;
+    ysw[tid] = w;
   }
   __syncthreads();
   if (tid < 6) {
-    double qa = 0.0;
-    for (int g = 0; g < kRowsPerGroup; ++g) qa += ysum[6 * g + tid];
+    double qz = 0.0, qp = 0.0, qw = 0.0;
+#pragma unroll 6
+    for (int gg = 0; gg < kRowsPerGroup; ++gg) {
+      qz += ysz[6 * gg + tid];
+      qp += ysp[6 * gg + tid];
+      qw += ysw[6 * gg + tid];
+    }
+    const double qa = qz + beta * qp - qw;
     double pa = a.z[6 * i + tid] + beta * p_prev[6 * i + tid];
     if (ne) {
       const int n0 = a.hat_node[i];
@@ -430,10 +509,10 @@ __global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a, int32_t k, const do
     }
     a.q[6 * i + tid] = qa;
     p_cur[6 * i + tid] = pa;
-    ysum[tid] = pa * qa;
+    ysz[tid] = pa * qa;
   }
   __syncthreads();
-  if (tid == 0) a.part[i] = ((ysum[0] + ysum[1]) + (ysum[2] + ysum[3])) + (ysum[4] + ysum[5]);
+  if (tid == 0) a.part[i] = ((ysz[0] + ysz[1]) + (ysz[2] + ysz[3])) + (ysz[4] + ysz[5]);
 }
 
 // alpha = r'z / p'q; x += alpha p; r -= alpha q; z = M^-1 r on one group of
@@ -445,36 +524,50 @@ __global__ __launch_bounds__(kGroupBlock) void k_pcg_update(PcgArgs a, int32_t k
   __shared__ double cs[kMaxCoarse];
   __shared__ double mpart[kGroupBlock];
   const PcgState* st = a.st;
-  if (st->done) return;
+  const int32_t done = st->done;
+  const double rz = st->rz[k & 1];
   const int ne = a.ne, tid = threadIdx.x, P = a.Pvar, g = blockIdx.x;
+  const int row = g * kRowsPerGroup + tid / 6, comp = tid % 6;
+  const bool active = tid < kGroupThreads && row < P;
+  GroupLoads L;
+  load_group(a, g, L);
+  // The group's vectors and preconditioner rows, loaded before the reduction.
+  double xv = 0.0, pv = 0.0, rv = 0.0, qv = 0.0, M[6] = {};
+  if (active) {
+    const int e = 6 * row + comp;
+    xv = a.x[e];
+    pv = p_cur[e];
+    rv = a.r[e];
+    qv = a.q[e];
+    const double* Mr = a.Minv + (size_t)36 * row + 6 * comp;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) M[c] = Mr[c];
+  }
   // p'q, summed by every group in the same order.
   double pq = 0.0, unused0 = 0.0, unused1 = 0.0;
   for (int r = tid; r < P; r += blockDim.x) pq += a.part[r];
-  for (int c = tid; c < ne; c += blockDim.x) cs[c] = 0.0;
+  if (done) return;  // the same value in every thread
   block_sum3(pq, unused0, unused1, lds);
-  const double alpha = st->rz[k & 1] / pq;
-  const int row = g * kRowsPerGroup + tid / 6, comp = tid % 6;
-  const bool active = tid < kGroupThreads && row < P;
-  double rv = 0.0;
+  const double alpha = rz / pq;
   if (active) {
     const int e = 6 * row + comp;
-    a.x[e] += alpha * p_cur[e];
-    rv = a.r[e] - alpha * a.q[e];
+    a.x[e] = xv + alpha * pv;
+    rv -= alpha * qv;
     a.r[e] = rv;
   }
   if (tid < kGroupThreads) rloc[tid] = rv;
   __syncthreads();
   double zv = 0.0;
   if (active) {
-    const double* M = a.Minv + (size_t)36 * row + 6 * comp;
     const int base = tid - comp;
+#pragma unroll
     for (int c = 0; c < 6; ++c) zv += M[c] * rloc[base + c];
     a.z[6 * row + comp] = zv;
   }
   if (tid < kGroupThreads) zloc[tid] = zv;
-  double rz = rv * zv, rr = rv * rv, zero = 0.0;
-  block_sum3(rz, rr, zero, lds);
-  group_partials(a, g, zloc, rz, rr, 0.0, cs, mpart, a.part2 + (size_t)g * (3 + ne));
+  double rzg = rv * zv, rr = rv * rv, zero = 0.0;
+  block_sum3(rzg, rr, zero, lds);
+  group_partials(a, g, L, zloc, rzg, rr, 0.0, cs, mpart, a.part2 + (size_t)g * (3 + ne));
 }
 
 // Clears the convergence flag and lifts the iteration limit so that one more
@@ -494,17 +587,17 @@ int32_t pcg_update_groups(int32_t Pvar) { return (Pvar + kRowsPerGroup - 1) / kR
 
 // Deflated PCG solve of S y = b. Buffers (device): x, r, z, p0, p1, q
 // (6 Pvar), part (Pvar), part2 (groups x (3 + ne)), SR (groups x gw x 252),
-// E / Einv (ne x ne), y (ne). The host polls the state every `poll`
-// iterations. With `probe` (three events), one more iteration's k_pcg_spmv
-// and k_pcg_update are timed after the solve (x and r then hold one extra
-// step). Returns the iteration count or -1 on a launch error.
+// E / Einv (ne x ne), y (ne). The host polls the state after `first`
+// iterations, then every `poll` iterations. With `probe` (three events), one
+// more iteration's k_pcg_spmv and k_pcg_update are timed after the solve (x
+// and r then hold one extra step). Returns the iteration count or -1 on a launch error.
 int launch_pcg_deflated(hipStream_t s, int32_t Pvar, int32_t K, const int32_t* row_ptr, const int32_t* col_var,
                         const double* S, const double* Minv, const double* b, const int32_t* hat_node,
                         const double* hat_w, const int32_t* node_range, const int32_t* grp_lo, int32_t gw,
                         double* SR, double* E, double* Einv, double* y,
                         double* x, double* r, double* z, double* p0, double* p1, double* q,
                         double* part, double* part2, void* state_dev, void* state_host, double tol,
-                        int32_t max_iters, int32_t poll, const hipEvent_t* probe) {
+                        int32_t max_iters, int32_t first, int32_t poll, const hipEvent_t* probe) {
   if (Pvar <= 0) return 0;
   PcgArgs a;
   a.Pvar = Pvar;
@@ -526,7 +619,7 @@ int launch_pcg_deflated(hipStream_t s, int32_t Pvar, int32_t K, const int32_t* r
   PcgState* hs = static_cast<PcgState*>(state_host);
   int k = 0;
   while (true) {
-    const int n = std::min(poll, max_iters - k);
+    const int n = std::min(k == 0 ? std::max(first, 1) : poll, max_iters - k);
     for (int j = 0; j < n; ++j, ++k) {
       k_pcg_spmv<<<Pvar, 256, 0, s>>>(a, k, (k & 1) ? p1 : p0, (k & 1) ? p0 : p1);
       k_pcg_update<<<groups, kGroupBlock, 0, s>>>(a, k, (k & 1) ? p0 : p1);
