@@ -554,7 +554,7 @@ int ensure_coarse_basis(mlba_handle* h, int32_t requested) {
   const size_t ne = 6 * (size_t)K;
   const size_t groups = pcg_update_groups(std::max(Pvar, 1));
   if (!h->defl_SR.alloc(groups * (size_t)h->grp_width * (size_t)(6 * gr)) ||
-      !h->defl_E.alloc(std::max<size_t>(ne * ne, 1)) || !h->defl_Einv.alloc(std::max<size_t>(ne * ne, 1)) ||
+      !h->defl_E.alloc(std::max<size_t>(ne * ne, 1)) || !h->defl_Einv.alloc(std::max<size_t>(ne * ne + groups * (size_t)h->grp_width * ne, 1)) ||
       !h->defl_y.alloc(std::max<size_t>(ne, 1)) || !h->pcg_part2.alloc(groups * (3 + ne)))
     return set_err(h, MLBA_ENOMEM, "device allocation failed");
   // Entries of the last group past the last row stay zero.

@@ -528,7 +528,7 @@ __global__ __launch_bounds__(256) void k_schur_incidences(
 }
 
 constexpr int kItemWaves = 4;      // work items per work-group (one per wave)
-constexpr int kItemPairs = 512;    // incidence pairs per work item (16 per wave pass)
+constexpr int kItemPairs = 512;    // incidence pairs per work item (32 per wave pass)
 constexpr uint32_t kPairAnchor = 0x80000000u;  // the incidence is the anchor of its landmark
 
 // Partial sums of S blocks over work items = (upper block u, a range of at
@@ -553,25 +553,61 @@ __global__ __launch_bounds__(64 * kItemWaves) void k_schur_items(
   const int4 item = items[it];  // (u, first pair, end pair)
   double acc[9];
   for (int k = 0; k < 9; ++k) acc[k] = 0.0;
-  for (int p = item.y + slot; p < item.z; p += 16) {
-    const uint2 pr = pairs[p];
-    const int32_t x = (int32_t)(pr.x & ~kPairAnchor), y = (int32_t)(pr.y & ~kPairAnchor);
-    const double* Q = inc_q + (size_t)18 * x + 3 * r0;
-    const double* Wy = inc_w + (size_t)18 * y + c0;
-    double q[9], W[9];
-    for (int k = 0; k < 9; ++k) q[k] = Q[k];
+  // Two pairs per lane and pass (slots p and p + 16), and the pair indices of
+  // the next pass requested before this pass's products: each pass waits for
+  // one round of loads instead of four dependent ones.
+  const uint2 none = make_uint2(0u, 0u);
+  int p = item.y + slot;
+  uint2 n0 = p < item.z ? pairs[p] : none, n1 = p + 16 < item.z ? pairs[p + 16] : none;
+  for (; p < item.z; p += 32) {
+    const uint2 pr0 = n0, pr1 = n1;
+    const bool has1 = p + 16 < item.z;
+    const int32_t x0 = (int32_t)(pr0.x & ~kPairAnchor), y0 = (int32_t)(pr0.y & ~kPairAnchor);
+    const int32_t x1 = (int32_t)(pr1.x & ~kPairAnchor), y1 = (int32_t)(pr1.y & ~kPairAnchor);
+    const double* Q0 = inc_q + (size_t)18 * x0 + 3 * r0;
+    const double* W0 = inc_w + (size_t)18 * y0 + c0;
+    const double* Q1 = inc_q + (size_t)18 * x1 + 3 * r0;
+    const double* W1 = inc_w + (size_t)18 * y1 + c0;
+    double q[18], W[18];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) q[k] = Q0[k];
+#pragma unroll
     for (int aa = 0; aa < 3; ++aa)
-      for (int d = 0; d < 3; ++d) W[aa * 3 + d] = Wy[aa * 6 + d];
-    for (int rr = 0; rr < 3; ++rr)
-      for (int d = 0; d < 3; ++d)
-        acc[rr * 3 + d] += q[rr * 3] * W[d] + q[rr * 3 + 1] * W[3 + d] + q[rr * 3 + 2] * W[6 + d];
-    if ((pr.y & kPairAnchor) && !(pr.x & kPairAnchor)) {
-      // direct observer-anchor coupling W_x^T A_l
-      const double* A = lm_a + (size_t)18 * (-1 - t.inc[y]) + c0;
-      const double* Wx = inc_w + (size_t)18 * x + r0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) W[aa * 3 + d] = W0[aa * 6 + d];
+    if (has1) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) q[9 + k] = Q1[k];
+#pragma unroll
+      for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) W[9 + aa * 3 + d] = W1[aa * 6 + d];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) q[9 + k] = W[9 + k] = 0.0;
+    }
+    n0 = p + 32 < item.z ? pairs[p + 32] : none;
+    n1 = p + 48 < item.z ? pairs[p + 48] : none;
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
       for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
         for (int d = 0; d < 3; ++d)
-          acc[rr * 3 + d] += Wx[rr] * A[d] + Wx[6 + rr] * A[6 + d] + Wx[12 + rr] * A[12 + d];
+          acc[rr * 3 + d] += q[9 * h + rr * 3] * W[9 * h + d] + q[9 * h + rr * 3 + 1] * W[9 * h + 3 + d] +
+                             q[9 * h + rr * 3 + 2] * W[9 * h + 6 + d];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint2 pr = h ? pr1 : pr0;
+      if ((h == 0 || has1) && (pr.y & kPairAnchor) && !(pr.x & kPairAnchor)) {
+        // direct observer-anchor coupling W_x^T A_l
+        const int32_t x = h ? x1 : x0, y = h ? y1 : y0;
+        const double* A = lm_a + (size_t)18 * (-1 - t.inc[y]) + c0;
+        const double* Wx = inc_w + (size_t)18 * x + r0;
+        for (int rr = 0; rr < 3; ++rr)
+          for (int d = 0; d < 3; ++d)
+            acc[rr * 3 + d] += Wx[rr] * A[d] + Wx[6 + rr] * A[6 + d] + Wx[12 + rr] * A[12 + d];
+      }
     }
   }
   for (int k = 0; k < 9; ++k) {

@@ -61,6 +61,7 @@ struct PcgArgs {
   double* SR;                  // groups x gw x 252: SW entry (group row t, window column lc)
   double* E;                   // ne x ne
   double* Einv;                // ne x ne
+  double* EinvW;               // groups x gw x ne: the rows of E^-1 of each group's window columns
   double* x;
   double* r;
   double* z;
@@ -274,6 +275,9 @@ struct GroupLoads {
   double einv[kEinvPer];
 };
 
+// The E^-1 entries come from the per-group copy EinvW (addresses that do not
+// depend on a prior load) or, with kFromEinv, from E^-1 itself.
+template <bool kFromEinv>
 __device__ inline void load_group(const PcgArgs& a, int g, GroupLoads& L) {
   const int ne = a.ne, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 #pragma unroll
@@ -288,20 +292,27 @@ __device__ inline void load_group(const PcgArgs& a, int g, GroupLoads& L) {
 #pragma unroll
   for (int m = 0; m < kEinvPer; ++m) L.einv[m] = 0.0;
   if (ne == 0) return;
-  const int nparts = kGroupBlock / ne, rrow = tid % ne, part = tid / ne;
-  const int c0 = 6 * a.grp_lo[g], w = min(a.gw, ne - c0), len = (w + nparts - 1) / nparts;
+  const int nparts = kGroupBlock / ne, rrow = tid % ne, part = tid / ne, gw = a.gw;
+  const int len = (gw + nparts - 1) / nparts;
+  const int c0 = kFromEinv ? 6 * a.grp_lo[g] : 0;
   if (part < nparts) {
 #pragma unroll
     for (int m = 0; m < kEinvPer; ++m) {
       const int lc = part * len + m;
-      if (m < len && lc < w) L.einv[m] = a.Einv[(size_t)(c0 + lc) * ne + rrow];  // E^-1 is symmetric
+      if (m < len && lc < gw) {
+        if (kFromEinv)
+          L.einv[m] = c0 + lc < ne ? a.Einv[(size_t)(c0 + lc) * ne + rrow] : 0.0;  // E^-1 is symmetric
+        else
+          L.einv[m] = a.EinvW[((size_t)g * gw + lc) * ne + rrow];
+      }
     }
   }
 }
 
 // Per-group partials of group g: out = {rz, rr, bb, E^-1 (SW)'z}; zloc holds
 // the group's z (0 past the last row). (SW)'z is non-zero only on the group's
-// window of columns, so E^-1 (SW)'z reads only those columns of E^-1.
+// window of columns, so E^-1 (SW)'z needs only those columns of E^-1 (window
+// columns past the last coarse column have zero SW and E^-1 entries).
 __device__ inline void group_partials(const PcgArgs& a, int g, const GroupLoads& L, const double* zloc, double rz,
                                       double rr, double bb, double* cs, double* mpart, double* out) {
   const int ne = a.ne, tid = threadIdx.x;
@@ -311,8 +322,7 @@ __device__ inline void group_partials(const PcgArgs& a, int g, const GroupLoads&
     out[2] = bb;
   }
   if (ne == 0) return;
-  const int wave = tid >> 6, lane = tid & 63;
-  const int c0 = 6 * a.grp_lo[g], w = min(a.gw, ne - c0);
+  const int wave = tid >> 6, lane = tid & 63, w = a.gw;
 #pragma unroll
   for (int m = 0; m < kSrCols; ++m) {
     const int lc = wave + kGroupWaves * m;
@@ -357,8 +367,16 @@ __global__ __launch_bounds__(kGroupBlock) void k_defl_init(PcgArgs a) {
   __shared__ double ys[kMaxCoarse];
   const int ne = a.ne, tid = threadIdx.x, P = a.Pvar, g = blockIdx.x;
   GroupLoads L;
-  load_group(a, g, L);
+  load_group<true>(a, g, L);
   for (int c = tid; c < ne; c += blockDim.x) ys[c] = a.y[c];
+  if (ne) {
+    // The group's window rows of E^-1 for k_pcg_update.
+    const int c0 = 6 * a.grp_lo[g];
+    for (int e = tid; e < a.gw * ne; e += blockDim.x) {
+      const int lc = e / ne, col = e - lc * ne;
+      a.EinvW[(size_t)g * a.gw * ne + e] = c0 + lc < ne ? a.Einv[(size_t)(c0 + lc) * ne + col] : 0.0;
+    }
+  }
   __syncthreads();
   const int row = g * kRowsPerGroup + tid / 6, comp = tid % 6;
   const bool active = tid < kGroupThreads && row < P;
@@ -411,13 +429,35 @@ __global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a, int32_t k, const do
   const double rz_prev = st->rz[(k + 1) & 1], tol2_prev = st->tol2, tol_rel2 = st->tol_rel2;
   const int ne = a.ne, tid = threadIdx.x, stride = 3 + ne;
   const int i = blockIdx.x, comp = tid % 6, g = tid / 6;
-  // Block products, independent of the reductions.
+  // Every read of the work-group is issued before its first wait (each
+  // dependent round of global loads costs microseconds): the state, the first
+  // chunk of group partials, the row's own z, p_{k-1} and hat weights, (SW)_i
+  // and the block products.
+  constexpr int kChunkPer = kPartChunk / 256;
+  const int per_chunk = kPartChunk / stride, ng0 = min(per_chunk, a.groups);
+  double pre[kChunkPer];
+#pragma unroll
+  for (int m = 0; m < kChunkPer; ++m) {
+    const int q = tid + 256 * m;
+    pre[m] = q < ng0 * stride ? a.part2[q] : 0.0;
+  }
+  double z_own = 0.0, p_own = 0.0, hw0 = 0.0, hw1 = 0.0;
+  int hn0 = 0;
+  if (tid < 6) {
+    z_own = a.z[6 * i + tid];
+    p_own = p_prev[6 * i + tid];
+    if (ne) {
+      hn0 = a.hat_node[i];
+      hw0 = a.hat_w[2 * i];
+      hw1 = a.hat_w[2 * i + 1];
+    }
+  }
   constexpr int kSwCols = (kMaxCoarse + kRowsPerGroup - 1) / kRowsPerGroup;
   double sz = 0.0, sp = 0.0, sw[kSwCols] = {};
   const int gi = i / kRowsPerGroup, c0 = 6 * (ne ? a.grp_lo[gi] : 0);
   if (tid < kGroupThreads) {
     const int rb = a.row_ptr[i], re = a.row_ptr[i + 1];
-#pragma unroll 1
+#pragma unroll 4
     for (int kb = rb + g; kb < re; kb += kRowsPerGroup) {
       const int j = a.col_var[kb];
       const double* srow = a.S + (size_t)36 * kb + 6 * comp;
@@ -438,9 +478,13 @@ __global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a, int32_t k, const do
   }
   if (done) return;  // the same value in every thread of the work-group
   // Fixed-order sum of the group partials, a chunk of whole groups per pass.
-  const int per_chunk = kPartChunk / stride;
+#pragma unroll
+  for (int m = 0; m < kChunkPer; ++m) chunk[tid + 256 * m] = pre[m];
+  __syncthreads();
   double acc = 0.0;
-  for (int g0 = 0; g0 < a.groups; g0 += per_chunk) {
+  if (tid < stride)
+    for (int gg = 0; gg < ng0; ++gg) acc += chunk[gg * stride + tid];
+  for (int g0 = ng0; g0 < a.groups; g0 += per_chunk) {
     const int ng = min(per_chunk, a.groups - g0);
     __syncthreads();
     for (int q = tid; q < ng * stride; q += blockDim.x) chunk[q] = a.part2[(size_t)g0 * stride + q];
@@ -501,12 +545,8 @@ __global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a, int32_t k, const do
       qw += ysw[6 * gg + tid];
     }
     const double qa = qz + beta * qp - qw;
-    double pa = a.z[6 * i + tid] + beta * p_prev[6 * i + tid];
-    if (ne) {
-      const int n0 = a.hat_node[i];
-      const double w0 = a.hat_w[2 * i], w1 = a.hat_w[2 * i + 1];
-      pa -= w0 * mu_s[6 * n0 + tid] + (w1 != 0.0 ? w1 * mu_s[6 * (n0 + 1) + tid] : 0.0);
-    }
+    double pa = z_own + beta * p_own;
+    if (ne) pa -= hw0 * mu_s[6 * hn0 + tid] + (hw1 != 0.0 ? hw1 * mu_s[6 * (hn0 + 1) + tid] : 0.0);
     a.q[6 * i + tid] = qa;
     p_cur[6 * i + tid] = pa;
     ysz[tid] = pa * qa;
@@ -530,7 +570,7 @@ __global__ __launch_bounds__(kGroupBlock) void k_pcg_update(PcgArgs a, int32_t k
   const int row = g * kRowsPerGroup + tid / 6, comp = tid % 6;
   const bool active = tid < kGroupThreads && row < P;
   GroupLoads L;
-  load_group(a, g, L);
+  load_group<false>(a, g, L);
   // The group's vectors and preconditioner rows, loaded before the reduction.
   double xv = 0.0, pv = 0.0, rv = 0.0, qv = 0.0, M[6] = {};
   if (active) {
@@ -587,7 +627,7 @@ int32_t pcg_update_groups(int32_t Pvar) { return (Pvar + kRowsPerGroup - 1) / kR
 
 // Deflated PCG solve of S y = b. Buffers (device): x, r, z, p0, p1, q
 // (6 Pvar), part (Pvar), part2 (groups x (3 + ne)), SR (groups x gw x 252),
-// E / Einv (ne x ne), y (ne). The host polls the state after `first`
+// E (ne x ne), Einv (ne x ne + groups x gw x ne), y (ne). The host polls the state after `first`
 // iterations, then every `poll` iterations. With `probe` (three events), one
 // more iteration's k_pcg_spmv and k_pcg_update are timed after the solve (x
 // and r then hold one extra step). Returns the iteration count or -1 on a launch error.
@@ -606,7 +646,7 @@ int launch_pcg_deflated(hipStream_t s, int32_t Pvar, int32_t K, const int32_t* r
   a.groups = pcg_update_groups(Pvar);
   a.row_ptr = row_ptr; a.col_var = col_var; a.S = S; a.Minv = Minv; a.b = b;
   a.hat_node = hat_node; a.hat_w = hat_w; a.node_range = node_range; a.grp_lo = grp_lo; a.gw = gw; a.pad2 = 0;
-  a.SR = SR; a.E = E; a.Einv = Einv; a.x = x; a.r = r; a.z = z; a.p0 = p0; a.p1 = p1; a.q = q;
+  a.SR = SR; a.E = E; a.Einv = Einv; a.EinvW = Einv + (size_t)a.ne * a.ne; a.x = x; a.r = r; a.z = z; a.p0 = p0; a.p1 = p1; a.q = q;
   a.part = part; a.part2 = part2; a.y = y;
   a.st = static_cast<PcgState*>(state_dev);
   const int groups = a.groups;
