@@ -39,8 +39,14 @@ def kernel_table(sm, kt, n_pcg_total):
                           written, 4 index) + L * 72 (pivot Z and g)
       schur_blocks:       pairs * 8 (pair list) + n_inc * 336 (Q, W, rhs)
                           + upper blocks * 584 (S_ij and S_ji written, index)
-      pcg_spmv:           nnzb * 292 (6x6 block + column index) + P * 6 * 8 * 4
-      pcg_update:         P * 6 * 8 * 9 + P * 288 (Minv) + P * 6 * sw * 8 (SW)
+      pcg_spmv:           nnzb * 292 (6x6 block + column index) + P * 200 (z, p
+                          read; q, p written; p'q) + P * 6 * gw * 8 (row of SW)
+      pcg_update:         P * 344 (x, r read and written, p, q read, z
+                          written, p'q) + P * 288 (Minv) + P * 6 * gw * 8 (SW)
+                          + groups * gw * ne * 8 (E^-1 window rows)
+                          + groups * (3 + ne) * 8 (partials written)
+    with gw the coarse columns a row group couples to, ne the coarse columns
+    and groups = ceil(P / 42).
     The Schur block kernel also does pairs * 216 FP64 flops (6x3 . 3x6),
     below its byte time at MI355X's FP64 peak."""
     from maplab_amd import abi
@@ -83,10 +89,13 @@ def kernel_table(sm, kt, n_pcg_total):
             "note": "events around each whole solve: coarse-space set-up, both kernels of every iteration, "
                     "launch gaps and the host convergence polls"}
     if kt.pcg_spmv_ms > 0 and n_pcg_total > 0:
-        timing = "avg of mlba_time_kernels events around one launch on the library stream, x PCG iterations"
-        sw = 36  # upper bound of the coarse-space window width used in the byte count
-        for name, ms, b in (("k_pcg_spmv", kt.pcg_spmv_ms, nnzb * 292 + P * 6 * 8 * 4),
-                            ("k_pcg_update", kt.pcg_update_ms, P * 6 * 8 * 9 + P * 288 + P * 6 * sw * 8)):
+        timing = ("mlba_time_kernels: events around a batch of back-to-back launches on the library stream "
+                  "(launch gaps included), per launch, x PCG iterations")
+        gw, ne = kt.pcg_window_columns, kt.pcg_coarse_columns
+        groups = (P + 41) // 42
+        spmv_b = nnzb * 292 + P * 200 + P * 6 * gw * 8
+        upd_b = P * 344 + P * 288 + P * 6 * gw * 8 + groups * gw * ne * 8 + groups * (3 + ne) * 8
+        for name, ms, b in (("k_pcg_spmv", kt.pcg_spmv_ms, spmv_b), ("k_pcg_update", kt.pcg_update_ms, upd_b)):
             rows[name] = row_of(n_pcg_total, ms * n_pcg_total, b)
             rows[name]["timing"] = timing
     return rows

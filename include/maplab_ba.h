@@ -306,6 +306,8 @@ typedef struct mlba_kernel_times {
   int64_t incidences;         /* vertex-observation and anchor incidences */
   int64_t linearize_tiles;
   int64_t pcg_iterations;     /* iterations of the timed PCG solve */
+  int64_t pcg_coarse_columns; /* columns of the deflation space (6 x hat nodes) */
+  int64_t pcg_window_columns; /* coarse columns a PCG row group couples to */
 } mlba_kernel_times;
 
 int32_t mlba_time_kernels(mlba_handle* handle, const mlba_options* options, int32_t repeats,

@@ -118,7 +118,8 @@ class KernelTimes(ctypes.Structure):
         "block_inverse_ms", "pcg_solve_ms", "pcg_spmv_ms", "pcg_update_ms", "back_substitute_ms")] + [
         (n, ctypes.c_int64) for n in (
             "num_observations", "num_landmarks", "num_variable_vertices", "schur_blocks", "schur_upper_blocks",
-            "schur_pairs", "incidences", "linearize_tiles", "pcg_iterations")]
+            "schur_pairs", "incidences", "linearize_tiles", "pcg_iterations", "pcg_coarse_columns",
+            "pcg_window_columns")]
 
 
 class QualityOptions(ctypes.Structure):

@@ -64,7 +64,8 @@ int launch_pcg_deflated(hipStream_t s, int32_t Pvar, int32_t K, const int32_t* r
                         double* SR, double* E, double* Einv, double* y,
                         double* x, double* r, double* z, double* p0, double* p1, double* q,
                         double* part, double* part2, void* state_dev, void* state_host, double tol,
-                        int32_t max_iters, int32_t first, int32_t poll, const hipEvent_t* probe);
+                        int32_t max_iters, int32_t first, int32_t poll, const hipEvent_t* probe,
+                        int32_t probe_reps);
 int32_t pcg_group_rows();
 int32_t launch_back_substitute(hipStream_t s, const DevTopology& t, int32_t tiles, const int32_t* tile_begin,
                                int32_t n_split, const int32_t* split_ids, const double* vR, const double* lm,
@@ -690,7 +691,7 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
             h->hat_w.p, h->node_range.p, h->grp_lo.p, h->grp_width, h->defl_SR.p, h->defl_E.p, h->defl_Einv.p, h->defl_y.p, h->px.p, h->pr.p, h->pz.p,
             h->pp.p, h->pp2.p, h->pq.p, h->pcg_part.p, h->pcg_part2.p, h->pcg_state.p, h->pcg_host,
             o->pcg_relative_tolerance, o->pcg_max_iterations,
-            h->pcg_last_iters > 0 ? h->pcg_last_iters + kPcgPoll / 2 : kPcgPoll, kPcgPoll, nullptr);
+            h->pcg_last_iters > 0 ? h->pcg_last_iters + kPcgPoll / 2 : kPcgPoll, kPcgPoll, nullptr, 0);
         if (pcg_iters < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
         h->pcg_last_iters = pcg_iters;
       }
@@ -1305,33 +1306,32 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
                                   h->hat_node.p, h->hat_w.p, h->node_range.p, h->grp_lo.p, h->grp_width, h->defl_SR.p,
                                   h->defl_E.p, h->defl_Einv.p, h->defl_y.p, h->px.p, h->pr.p, h->pz.p, h->pp.p, h->pp2.p, h->pq.p,
                                   h->pcg_part.p, h->pcg_part2.p, h->pcg_state.p, h->pcg_host,
-                                  o->pcg_relative_tolerance, o->pcg_max_iterations, kPcgPoll, kPcgPoll, nullptr);
+                                  o->pcg_relative_tolerance, o->pcg_max_iterations, kPcgPoll, kPcgPoll, nullptr, 0);
     });
     t->pcg_iterations = iters;
-    // One more iteration, each kernel bracketed by events, averaged over repeats.
+    // One more iteration: each of its two kernels launched repeats x 8 times
+    // back to back between events (launch gaps included), averaged.
     hipEvent_t pe[3];
     for (auto& e : pe) HIP_OK(hipEventCreate(&e));
-    double spmv = 0.0, upd = 0.0;
-    int nrep = 0;
-    for (int r = 0; r < repeats && iters > 0; ++r) {
+    const int reps = std::max(1, repeats) * 8;
+    if (iters > 0) {
       const int it = launch_pcg_deflated(s, h->Pvar, h->hat_K, h->row_ptr.p, h->col_var.p, h->S.p, h->Minv.p,
                                          h->rhs.p, h->hat_node.p, h->hat_w.p, h->node_range.p, h->grp_lo.p,
                                          h->grp_width, h->defl_SR.p, h->defl_E.p, h->defl_Einv.p, h->defl_y.p, h->px.p, h->pr.p,
                                          h->pz.p, h->pp.p, h->pp2.p, h->pq.p, h->pcg_part.p, h->pcg_part2.p,
                                          h->pcg_state.p, h->pcg_host, o->pcg_relative_tolerance,
-                                         std::min<int32_t>(o->pcg_max_iterations, 4), kPcgPoll, kPcgPoll, pe);
+                                         std::min<int32_t>(o->pcg_max_iterations, 4), kPcgPoll, kPcgPoll, pe, reps);
       if (it < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
       HIP_OK(hipEventSynchronize(pe[2]));
       float a = 0.f, b = 0.f;
       HIP_OK(hipEventElapsedTime(&a, pe[0], pe[1]));
       HIP_OK(hipEventElapsedTime(&b, pe[1], pe[2]));
-      spmv += a;
-      upd += b;
-      ++nrep;
+      t->pcg_spmv_ms = a / reps;
+      t->pcg_update_ms = b / reps;
     }
     for (auto& e : pe) (void)hipEventDestroy(e);
-    t->pcg_spmv_ms = nrep ? spmv / nrep : 0.0;
-    t->pcg_update_ms = nrep ? upd / nrep : 0.0;
+    t->pcg_coarse_columns = 6 * (int64_t)h->hat_K;
+    t->pcg_window_columns = h->grp_width;
   }
   launch_unscale_poses(s, h->Pvar, h->px.p, h->pose_scale.p, h->w.p, h->parts_w.p);
   t->back_substitute_ms = timed([&] {

@@ -628,16 +628,18 @@ int32_t pcg_update_groups(int32_t Pvar) { return (Pvar + kRowsPerGroup - 1) / kR
 // Deflated PCG solve of S y = b. Buffers (device): x, r, z, p0, p1, q
 // (6 Pvar), part (Pvar), part2 (groups x (3 + ne)), SR (groups x gw x 252),
 // E (ne x ne), Einv (ne x ne + groups x gw x ne), y (ne). The host polls the state after `first`
-// iterations, then every `poll` iterations. With `probe` (three events), one
-// more iteration's k_pcg_spmv and k_pcg_update are timed after the solve (x
-// and r then hold one extra step). Returns the iteration count or -1 on a launch error.
+// iterations, then every `poll` iterations. With `probe` (three events),
+// k_pcg_spmv and then k_pcg_update of one more iteration are each launched
+// `probe_reps` times back to back after the solve and timed as two batches
+// (k_pcg_spmv is idempotent; x, r and z then hold extra steps). Returns the iteration count or -1 on a launch error.
 int launch_pcg_deflated(hipStream_t s, int32_t Pvar, int32_t K, const int32_t* row_ptr, const int32_t* col_var,
                         const double* S, const double* Minv, const double* b, const int32_t* hat_node,
                         const double* hat_w, const int32_t* node_range, const int32_t* grp_lo, int32_t gw,
                         double* SR, double* E, double* Einv, double* y,
                         double* x, double* r, double* z, double* p0, double* p1, double* q,
                         double* part, double* part2, void* state_dev, void* state_host, double tol,
-                        int32_t max_iters, int32_t first, int32_t poll, const hipEvent_t* probe) {
+                        int32_t max_iters, int32_t first, int32_t poll, const hipEvent_t* probe,
+                        int32_t probe_reps) {
   if (Pvar <= 0) return 0;
   PcgArgs a;
   a.Pvar = Pvar;
@@ -677,9 +679,9 @@ int launch_pcg_deflated(hipStream_t s, int32_t Pvar, int32_t K, const int32_t* r
     k_pcg_reopen<<<1, 1, 0, s>>>(a.st);
     const int kk = iters;
     if (hipEventRecord(probe[0], s) != hipSuccess) return -1;
-    k_pcg_spmv<<<Pvar, 256, 0, s>>>(a, kk, (kk & 1) ? p1 : p0, (kk & 1) ? p0 : p1);
+    for (int j = 0; j < probe_reps; ++j) k_pcg_spmv<<<Pvar, 256, 0, s>>>(a, kk, (kk & 1) ? p1 : p0, (kk & 1) ? p0 : p1);
     if (hipEventRecord(probe[1], s) != hipSuccess) return -1;
-    k_pcg_update<<<groups, kGroupBlock, 0, s>>>(a, kk, (kk & 1) ? p0 : p1);
+    for (int j = 0; j < probe_reps; ++j) k_pcg_update<<<groups, kGroupBlock, 0, s>>>(a, kk, (kk & 1) ? p0 : p1);
     if (hipEventRecord(probe[2], s) != hipSuccess) return -1;
     if (hipGetLastError() != hipSuccess) return -1;
   }
