@@ -101,6 +101,30 @@ def kernel_table(sm, kt, n_pcg_total):
     return rows
 
 
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "lm_loop_random_walk_500kf_hbm_traffic.md")
+# Bench kernel row -> the kernel whose rocprofv3 FETCH_SIZE / WRITE_SIZE row
+# gives its HBM traffic (the Schur block build is dominated by its item pass).
+TRAFFIC_KERNEL = {"k_schur_blocks": "mlba::k_schur_items", "k_pcg_spmv": "mlba::k_pcg_spmv",
+                  "k_pcg_update": "mlba::k_pcg_update", "k_schur_incidences": "mlba::k_schur_incidences",
+                  "k_schur_factors": "mlba::k_schur_factors", "k_linearize": "mlba::k_linearize<true>",
+                  "k_back_substitute": "mlba::k_back_substitute"}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on the default workload, from the
+    committed rocprofv3 counter summary (calibrated FETCH_SIZE + WRITE_SIZE);
+    None when the summary has no row for it."""
+    name = TRAFFIC_KERNEL.get(kernel)
+    if name is None or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 7 and cells[0] == f"`{name}`":
+            return int(float(cells[4]) * 1e6)
+    return None
+
+
 def cpu_baseline(vmap, opts, iters, threads):
     from oracle import binding as oracle
     import ctypes
@@ -171,8 +195,11 @@ def main(argv=None):
     single = {k: v for k, v in table.items() if "bound" in v and "total_ms" in v}
     dom_name = max(single, key=lambda k: single[k]["total_ms"])
     dom = single[dom_name]
+    default_workload = (args.keyframes, args.landmarks, args.seed, args.equidistant) == (500, 200000, 1, False)
     roof = {"kernel": dom_name, "bound": "hbm", "achieved": dom["achieved_gbs"], "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": dom["frac"], "traffic": None}
+            "unit": "GB/s", "frac": dom["frac"],
+            "traffic": measured_traffic(dom_name) if default_workload else None,
+            "algorithmic_bytes": dom["bytes"]}
 
     result = {
         "metric": "BA LM iters/s + obs Jacobians/s at 1e-6 cost rel err; PCG iters learned vs Jacobi",
