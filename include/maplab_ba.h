@@ -150,7 +150,8 @@ typedef struct mlba_options {
   int32_t obs_tile;                       /* reserved */
   int32_t schur_row_chunk;                /* reserved */
   int32_t pcg_coarse_nodes;               /* deflation hat nodes: 0 automatic, -1 none, else K (<= 15) */
-  int32_t reserved1;
+  int32_t pcg_ritz_vectors;               /* recycled Ritz vectors added to the deflation space:
+                                             0 automatic (12), -1 none, else r (<= 12) */
 } mlba_options;
 
 /* One row of the Ceres iteration log (ceres::IterationSummary subset). */

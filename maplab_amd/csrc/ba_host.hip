@@ -21,6 +21,8 @@
 #include <vector>
 
 #include "../../include/maplab_ba.h"
+#include "ba_pcg.h"
+#include "ba_ritz.h"
 #include "ba_device.h"
 #include "ba_math.h"
 
@@ -55,18 +57,7 @@ void launch_landmark_quality(hipStream_t s, const DevTopology& t, const double* 
 int32_t schur_item_pairs();
 void launch_block_inverse(hipStream_t s, int32_t Pvar, const int32_t* diag_pos, const double* S, double* Minv,
                           int32_t* fail);
-size_t pcg_state_bytes();
-int32_t pcg_max_coarse();
-int32_t pcg_update_groups(int32_t Pvar);
-int launch_pcg_deflated(hipStream_t s, int32_t Pvar, int32_t K, const int32_t* row_ptr, const int32_t* col_var,
-                        const double* S, const double* Minv, const double* b, const int32_t* hat_node,
-                        const double* hat_w, const int32_t* node_range, const int32_t* grp_lo, int32_t gw,
-                        double* SR, double* E, double* Einv, double* y,
-                        double* x, double* r, double* z, double* p0, double* p1, double* q,
-                        double* part, double* part2, void* state_dev, void* state_host, double tol,
-                        int32_t max_iters, int32_t first, int32_t poll, const hipEvent_t* probe,
-                        int32_t probe_reps);
-int32_t pcg_group_rows();
+int32_t pcg_max_lanczos();
 int32_t launch_back_substitute(hipStream_t s, const DevTopology& t, int32_t tiles, const int32_t* tile_begin,
                                int32_t n_split, const int32_t* split_ids, const double* vR, const double* lm,
                                const double* obs_rec, const double* lm_acc, const double* lm_z, const double* w,
@@ -95,6 +86,12 @@ thread_local std::string g_create_error;
 // LM solves take similar counts, a poll costs a host round trip (tens of
 // microseconds) and an iteration past convergence only two early-exit launches.
 constexpr int32_t kPcgPoll = 16;
+
+// Pinned host buffer of the recycling: the solve's coefficients (2 x 64),
+// the tridiagonal's eigenvectors (64 x 12), the Gram matrices (2 x 24 x 24)
+// and the merge coefficients (24 x 12).
+constexpr size_t kRitzCoefOff = 2 * 64, kRitzGramOff = kRitzCoefOff + 64 * 12;
+constexpr size_t kRitzMergeOff = kRitzGramOff + 2 * 24 * 24, kRitzPinned = kRitzMergeOff + 24 * 12;
 
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -164,17 +161,31 @@ struct mlba_handle {
   DBuf<double> schur_partial;           // 36 per work item
   DBuf<int32_t> hat_node, node_range, grp_lo;
   std::vector<int32_t> h_row_ptr, h_col_var;  // host copy of the S block structure
-  int32_t grp_width = 0;                // columns of the per-group SW window
-  DBuf<double> hat_w, defl_SR, defl_E, defl_Einv, defl_y, pcg_part2;
+  int32_t grp_width = 0;                // hat columns of the per-group SW window
+  DBuf<double> hat_w, defl_SR, defl_E, defl_Einv, defl_y, defl_cden, pcg_part2;
   int32_t hat_K = -1;  // coarse nodes of the current hat basis
   int32_t pcg_last_iters = 0;  // iterations of the previous solve: length of the next solve's first batch
+  // Recycled deflation columns: ritz[ritz_cur] holds ritz_nr of them; the
+  // other buffer receives the merged set. ritz_Zh / ritz_coef keep the
+  // Lanczos vectors and coefficients of the last solve.
+  DBuf<double> ritz[2], ritz_new, ritz_snew, ritz_Zh, ritz_coef, ritz_small;
+  int32_t ritz_cur = 0, ritz_nr = 0, ritz_zcap = 0;
+  // Recycling runs in three phases that overlap device work: after a solve
+  // the coefficients are copied (stage 1); while the step's back-substitution
+  // and cost run, the host takes the tridiagonal's eigenvectors and queues
+  // the Ritz kernels and the Gram download (stage 2); before the next solve
+  // the host solves the small merge problem and queues the merge (stage 0).
+  int32_t ritz_stage = 0, ritz_m = 0, ritz_r = 0;
+  double* ritz_pinned = nullptr;  // pinned: coefficients, Y, Gram matrices, merge coefficients
+  hipEvent_t ritz_ev_coef = nullptr, ritz_ev_ab = nullptr;
   DBuf<double> S, rhs, pose_grad, pose_diag, pose_jtj, pose_scale, w, Minv;
   DBuf<double> px, pr, pz, pp, pp2, pq, pcg_part, parts_a, parts_b, parts_c, parts_l, parts_v, parts_w, parts_g;
   DBuf<double> scal;
   DBuf<int32_t> fail;
   DBuf<char> pcg_state;
   void* pcg_host = nullptr;
-  double* scal_host = nullptr;
+  double* scal_host = nullptr;  // pinned: kNumSlots scalars, then the step's failure flag
+  hipEvent_t scal_ev = nullptr;
   DevTopology topo{};
   // Event pairs bracketing kernels inside the LM loop (kernel id, start, end).
   std::vector<hipEvent_t> ev_pool;
@@ -183,9 +194,13 @@ struct mlba_handle {
   mlba_summary* ev_summary = nullptr;
 
   ~mlba_handle() {
+    if (ritz_pinned) (void)hipHostFree(ritz_pinned);
+    if (ritz_ev_coef) (void)hipEventDestroy(ritz_ev_coef);
+    if (ritz_ev_ab) (void)hipEventDestroy(ritz_ev_ab);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     if (pcg_host) (void)hipHostFree(pcg_host);
     if (scal_host) (void)hipHostFree(scal_host);
+    if (scal_ev) (void)hipEventDestroy(scal_ev);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -327,6 +342,8 @@ int build_topology(mlba_handle* h) {
   h->nnzb = (int32_t)col_var.size();
   h->hat_K = -1;
   h->pcg_last_iters = 0;
+  h->ritz_nr = 0;
+  h->ritz_stage = 0;
   h->h_row_ptr = row_ptr;
   h->h_col_var = col_var;
   // Upper blocks of S and, for each, the incidence pairs (x in row i, y in
@@ -506,8 +523,8 @@ void collect_events(mlba_handle* h) {
 // (n0, n0 + 1). K = 0 disables deflation.
 int ensure_coarse_basis(mlba_handle* h, int32_t requested) {
   const int Pvar = h->Pvar;
-  int K = requested > 0 ? requested : (requested < 0 ? 0 : std::min(pcg_max_coarse() / 6, Pvar / 16));
-  K = std::min(K, pcg_max_coarse() / 6);
+  int K = requested > 0 ? requested : (requested < 0 ? 0 : std::min(pcg_max_hat_nodes(), Pvar / 16));
+  K = std::min(K, pcg_max_hat_nodes());
   if (K < 2 || Pvar < 2 * K) K = 0;
   if (K == h->hat_K && h->hat_node.p) return MLBA_OK;
   std::vector<int32_t> node(std::max(Pvar, 1), 0), range(2 * std::max(K, 1), 0);
@@ -552,16 +569,171 @@ int ensure_coarse_basis(mlba_handle* h, int32_t requested) {
   if ((rc = upload(h, h->hat_node, node.data(), node.size())) || (rc = upload(h, h->hat_w, wts.data(), wts.size())) ||
       (rc = upload(h, h->node_range, range.data(), range.size())) || (rc = upload(h, h->grp_lo, lo.data(), lo.size())))
     return rc;
-  const size_t ne = 6 * (size_t)K;
+  // Room for the recycled dense columns beside the hat columns.
+  const size_t nr = K > 0 ? (size_t)pcg_max_ritz() : 0;
+  const size_t ne = 6 * (size_t)K + nr, gw = (size_t)h->grp_width + nr;
   const size_t groups = pcg_update_groups(std::max(Pvar, 1));
-  if (!h->defl_SR.alloc(groups * (size_t)h->grp_width * (size_t)(6 * gr)) ||
-      !h->defl_E.alloc(std::max<size_t>(ne * ne, 1)) || !h->defl_Einv.alloc(std::max<size_t>(ne * ne + groups * (size_t)h->grp_width * ne, 1)) ||
-      !h->defl_y.alloc(std::max<size_t>(ne, 1)) || !h->pcg_part2.alloc(groups * (3 + ne)))
+  if (!h->defl_SR.alloc(groups * gw * (size_t)(6 * gr)) ||
+      !h->defl_E.alloc(std::max<size_t>(ne * ne, 1)) || !h->defl_Einv.alloc(std::max<size_t>(ne * ne + groups * gw * ne, 1)) ||
+      !h->defl_y.alloc(std::max<size_t>(ne, 1)) || !h->defl_cden.alloc(std::max<size_t>(nr, 1)) ||
+      !h->pcg_part2.alloc(groups * (3 + ne)))
     return set_err(h, MLBA_ENOMEM, "device allocation failed");
+  if (nr > 0) {
+    const size_t n6 = 6 * (size_t)Pvar, zc = (size_t)pcg_max_lanczos();
+    if (!h->ritz[0].alloc(nr * n6) || !h->ritz[1].alloc(nr * n6) || !h->ritz_new.alloc(nr * n6) ||
+        !h->ritz_snew.alloc(nr * n6) || !h->ritz_Zh.alloc(zc * n6) || !h->ritz_coef.alloc(2 * zc) ||
+        !h->ritz_small.alloc(2 * (2 * nr) * (2 * nr) + zc * nr + 2 * nr * nr))
+      return set_err(h, MLBA_ENOMEM, "device allocation failed");
+    h->ritz_zcap = (int32_t)zc;
+    if (!h->ritz_pinned) {
+      HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&h->ritz_pinned), kRitzPinned * sizeof(double), hipHostMallocDefault));
+      HIP_OK(hipEventCreateWithFlags(&h->ritz_ev_coef, hipEventDisableTiming));
+      HIP_OK(hipEventCreateWithFlags(&h->ritz_ev_ab, hipEventDisableTiming));
+    }
+  } else {
+    h->ritz_zcap = 0;
+  }
+  h->ritz_nr = 0;
+  h->ritz_stage = 0;
   // Entries of the last group past the last row stay zero.
   HIP_OK(hipMemsetAsync(h->defl_SR.p, 0, h->defl_SR.n * sizeof(double), h->stream));
   h->hat_K = K;
   HIP_OK(hipStreamSynchronize(h->stream));
+  return MLBA_OK;
+}
+
+// Recycled Ritz columns requested by the options (0 = automatic).
+int32_t ritz_count(const mlba_options* o) {
+  if (o->pcg_ritz_vectors < 0) return 0;
+  return o->pcg_ritz_vectors == 0 ? pcg_max_ritz() : std::min(o->pcg_ritz_vectors, pcg_max_ritz());
+}
+
+// Device buffers of a PCG solve with the handle's coarse basis and nr
+// recycled columns; Lanczos vectors are kept when `keep_lanczos`.
+PcgBuffers pcg_buffers(mlba_handle* h, int32_t nr, bool keep_lanczos) {
+  PcgBuffers B{};
+  B.Pvar = h->Pvar;
+  B.K = std::max(h->hat_K, 0);
+  B.nr = B.K > 0 ? nr : 0;
+  B.gwh = h->grp_width;
+  B.zcap = keep_lanczos && B.K > 0 ? h->ritz_zcap : 0;
+  B.row_ptr = h->row_ptr.p; B.col_var = h->col_var.p; B.S = h->S.p; B.Minv = h->Minv.p; B.b = h->rhs.p;
+  B.hat_node = h->hat_node.p; B.hat_w = h->hat_w.p; B.node_range = h->node_range.p; B.grp_lo = h->grp_lo.p;
+  B.R = h->ritz[h->ritz_cur].p;
+  B.SR = h->defl_SR.p; B.E = h->defl_E.p; B.Einv = h->defl_Einv.p; B.y = h->defl_y.p; B.cden = h->defl_cden.p;
+  B.x = h->px.p; B.r = h->pr.p; B.z = h->pz.p; B.p0 = h->pp.p; B.p1 = h->pp2.p; B.q = h->pq.p;
+  B.part = h->pcg_part.p; B.part2 = h->pcg_part2.p; B.Zh = h->ritz_Zh.p; B.coef = h->ritz_coef.p;
+  B.state_dev = h->pcg_state.p; B.state_host = h->pcg_host;
+  return B;
+}
+
+// Recycled deflation columns from the solve just finished (its Lanczos
+// vectors and coefficients, its S and SW): the r Ritz vectors of the solve's
+// tridiagonal for its smallest eigenvalues, merged with the current columns
+// by a Rayleigh-Ritz step on span[R, Rn] w.r.t. (S, M) that keeps the r best.
+// A solve whose deflation was not applied (E not positive definite) clears
+// the columns; one too short to resolve r Ritz vectors, or whose
+// coefficients are not usable, keeps them.
+//
+// Stage 1, right after the solve: queue the coefficient download.
+void ritz_after_solve(mlba_handle* h, int32_t r, int32_t iters) {
+  h->ritz_stage = 0;
+  if (r <= 0 || h->hat_K <= 0 || h->ritz_zcap <= 0 || !pcg_deflation_applied(h->pcg_host)) {
+    h->ritz_nr = 0;
+    return;
+  }
+  const int m = std::min<int>(iters, h->ritz_zcap);
+  if (m < r + 2) return;
+  if (hipMemcpyAsync(h->ritz_pinned, h->ritz_coef.p, 2 * (size_t)m * sizeof(double), hipMemcpyDeviceToHost,
+                     h->stream) != hipSuccess ||
+      hipEventRecord(h->ritz_ev_coef, h->stream) != hipSuccess)
+    return;
+  h->ritz_m = m;
+  h->ritz_r = r;
+  h->ritz_stage = 1;
+}
+
+// Stage 2, while the step's back-substitution and cost run: the
+// tridiagonal's eigenvectors on the host, then Rn, S Rn and the Gram
+// matrices queued on the device, and their download.
+int ritz_queue_vectors(mlba_handle* h) {
+  if (h->ritz_stage != 1) return MLBA_OK;
+  h->ritz_stage = 0;
+  hipStream_t s = h->stream;
+  HIP_OK(hipEventSynchronize(h->ritz_ev_coef));
+  const int m = h->ritz_m, r = h->ritz_r;
+  const double* coef = h->ritz_pinned;
+  // Lanczos tridiagonal of the deflated preconditioned operator:
+  // T_jj = 1 / alpha_j + beta_{j-1} / alpha_{j-1}, T_j,j+1 = -sqrt(beta_j) / alpha_j,
+  // beta_j = rz_{j+1} / rz_j.
+  std::vector<double> d(m), e(std::max(m - 1, 1)), vals(r), Y((size_t)m * r);
+  for (int j = 0; j < m; ++j) {
+    const double rz = coef[2 * j], al = coef[2 * j + 1];
+    if (!(rz > 0.0) || !(al > 0.0) || !std::isfinite(rz) || !std::isfinite(al)) return MLBA_OK;
+    d[j] = 1.0 / al + (j > 0 ? (rz / coef[2 * (j - 1)]) / coef[2 * (j - 1) + 1] : 0.0);
+    if (j + 1 < m) e[j] = -std::sqrt(coef[2 * (j + 1)] / rz) / al;
+  }
+  if (tridiag_smallest_eigenpairs(m, d.data(), e.data(), r, vals.data(), Y.data())) return MLBA_OK;
+  const int rmax = pcg_max_ritz();
+  double* hY = h->ritz_pinned + kRitzCoefOff;
+  for (int j = 0; j < m; ++j)
+    for (int c = 0; c < r; ++c) hY[(size_t)j * r + c] = Y[(size_t)j * r + c] / std::sqrt(coef[2 * j]);
+  double* dAB = h->ritz_small.p;
+  double* dY = dAB + 2 * (2 * rmax) * (2 * rmax);
+  const size_t n6 = 6 * (size_t)h->Pvar;
+  HIP_OK(hipMemcpyAsync(dY, hY, (size_t)m * r * sizeof(double), hipMemcpyHostToDevice, s));
+  launch_ritz_form(s, h->Pvar, m, r, h->ritz_Zh.p, dY, h->ritz_new.p);
+  const int nr_old = h->ritz_nr;
+  if (nr_old == 0) {
+    HIP_OK(hipMemcpyAsync(h->ritz[h->ritz_cur].p, h->ritz_new.p, (size_t)r * n6 * sizeof(double),
+                          hipMemcpyDeviceToDevice, s));
+    HIP_OK(hipGetLastError());
+    h->ritz_nr = r;
+    return MLBA_OK;
+  }
+  launch_ritz_spmv(s, h->Pvar, r, h->row_ptr.p, h->col_var.p, h->S.p, h->ritz_new.p, h->ritz_snew.p);
+  launch_ritz_gram(s, pcg_buffers(h, nr_old, false), r, h->ritz_new.p, h->ritz_snew.p, h->diag_pos.p, dAB);
+  const int nb = nr_old + r;
+  HIP_OK(hipMemcpyAsync(h->ritz_pinned + kRitzGramOff, dAB, 2 * (size_t)nb * nb * sizeof(double),
+                        hipMemcpyDeviceToHost, s));
+  HIP_OK(hipEventRecord(h->ritz_ev_ab, s));
+  HIP_OK(hipGetLastError());
+  h->ritz_stage = 2;
+  return MLBA_OK;
+}
+
+// Stage 3, before the next solve: the small generalized eigenproblem of the
+// merge on the host and the merged columns queued on the device.
+int ritz_merge(mlba_handle* h) {
+  if (h->ritz_stage != 2) return MLBA_OK;
+  h->ritz_stage = 0;
+  hipStream_t s = h->stream;
+  HIP_OK(hipEventSynchronize(h->ritz_ev_ab));
+  const int nr_old = h->ritz_nr, r = h->ritz_r, nb = nr_old + r, rmax = pcg_max_ritz();
+  const double* AB = h->ritz_pinned + kRitzGramOff;
+  std::vector<double> A((size_t)nb * nb), B((size_t)nb * nb), C((size_t)nb * r), vals(r);
+  for (int i = 0; i < nb; ++i)
+    for (int j = 0; j < nb; ++j) {
+      A[(size_t)i * nb + j] = 0.5 * (AB[(size_t)i * nb + j] + AB[(size_t)j * nb + i]);
+      B[(size_t)i * nb + j] = 0.5 * (AB[(size_t)nb * nb + (size_t)i * nb + j] + AB[(size_t)nb * nb + (size_t)j * nb + i]);
+    }
+  const size_t n6 = 6 * (size_t)h->Pvar;
+  if (generalized_smallest_eigenpairs(nb, A.data(), B.data(), r, vals.data(), C.data())) {
+    // The merged Gram matrix is singular: start over from the new vectors.
+    HIP_OK(hipMemcpyAsync(h->ritz[h->ritz_cur].p, h->ritz_new.p, (size_t)r * n6 * sizeof(double),
+                          hipMemcpyDeviceToDevice, s));
+    h->ritz_nr = r;
+    return MLBA_OK;
+  }
+  double* hC = h->ritz_pinned + kRitzMergeOff;
+  std::copy(C.begin(), C.end(), hC);
+  double* dC = h->ritz_small.p + 2 * (2 * rmax) * (2 * rmax) + (size_t)h->ritz_zcap * rmax;
+  HIP_OK(hipMemcpyAsync(dC, hC, C.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  launch_ritz_merge(s, h->Pvar, h->ritz[h->ritz_cur].p, nr_old, h->ritz_new.p, r, dC, r,
+                    h->ritz[h->ritz_cur ^ 1].p);
+  HIP_OK(hipGetLastError());
+  h->ritz_cur ^= 1;
+  h->ritz_nr = r;
   return MLBA_OK;
 }
 
@@ -686,14 +858,15 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
       int pcg_iters = 0;
       if (h->Pvar > 0) {
         KernelTimer kt(h, MLBA_KERNEL_PCG);
-        pcg_iters = launch_pcg_deflated(
-            s, h->Pvar, h->hat_K, h->row_ptr.p, h->col_var.p, h->S.p, h->Minv.p, h->rhs.p, h->hat_node.p,
-            h->hat_w.p, h->node_range.p, h->grp_lo.p, h->grp_width, h->defl_SR.p, h->defl_E.p, h->defl_Einv.p, h->defl_y.p, h->px.p, h->pr.p, h->pz.p,
-            h->pp.p, h->pp2.p, h->pq.p, h->pcg_part.p, h->pcg_part2.p, h->pcg_state.p, h->pcg_host,
-            o->pcg_relative_tolerance, o->pcg_max_iterations,
-            h->pcg_last_iters > 0 ? h->pcg_last_iters + kPcgPoll / 2 : kPcgPoll, kPcgPoll, nullptr, 0);
+        const int32_t nritz = ritz_count(o);
+        if ((rc = ritz_merge(h))) return rc;
+        pcg_iters = launch_pcg_deflated(s, pcg_buffers(h, nritz > 0 ? h->ritz_nr : 0, nritz > 0),
+                                        o->pcg_relative_tolerance, o->pcg_max_iterations,
+                                        h->pcg_last_iters > 0 ? h->pcg_last_iters + kPcgPoll / 2 : kPcgPoll,
+                                        kPcgPoll, nullptr, 0);
         if (pcg_iters < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
         h->pcg_last_iters = pcg_iters;
+        ritz_after_solve(h, nritz, pcg_iters);
       }
       launch_unscale_poses(s, h->Pvar, h->px.p, h->pose_scale.p, h->w.p, h->parts_w.p);
       int32_t nbs = 0;
@@ -719,9 +892,15 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
       }
       launch_reduce(s, h->tiles, 2, 0, false, h->tile_cost2.p, h->scal.p + kCandCost);
       HIP_OK(hipGetLastError());
-      int32_t fail_h = 0;
-      HIP_OK(hipMemcpyAsync(&fail_h, h->fail.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-      if ((rc = read_scalars(h))) return rc;
+      int32_t* fail_pinned = reinterpret_cast<int32_t*>(h->scal_host + kNumSlots);
+      HIP_OK(hipMemcpyAsync(fail_pinned, h->fail.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      HIP_OK(hipMemcpyAsync(h->scal_host, h->scal.p, kNumSlots * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIP_OK(hipEventRecord(h->scal_ev, s));
+      // The recycling's host work and kernels go behind the step's cost.
+      if ((rc = ritz_queue_vectors(h))) return rc;
+      HIP_OK(hipEventSynchronize(h->scal_ev));
+      collect_events(h);
+      const int32_t fail_h = *fail_pinned;
       if (sm) sm->linear_solver_seconds += now_s() - t_ls;
       const double* sc = h->scal_host;
       const double mcc = sc[kMcc];
@@ -988,7 +1167,8 @@ int32_t mlba_create(const mlba_problem_desc* d, mlba_handle** out) {
                   h->fail.alloc(1) && h->pcg_state.alloc(pcg_state_bytes());
   if (!ok) { h->err = "device allocation failed"; return fail_out(MLBA_ENOMEM); }
   if (hipHostMalloc(&h->pcg_host, pcg_state_bytes(), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&h->scal_host, kNumSlots * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&h->scal_host, (kNumSlots + 1) * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&h->scal_ev, hipEventDisableTiming) != hipSuccess) {
     h->err = "pinned host allocation failed";
     return fail_out(MLBA_ENOMEM);
   }
@@ -1302,10 +1482,7 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
     if ((rc = ensure_coarse_basis(h, o->pcg_coarse_nodes))) return rc;
     int32_t iters = 0;
     t->pcg_solve_ms = timed([&] {
-      iters = launch_pcg_deflated(s, h->Pvar, h->hat_K, h->row_ptr.p, h->col_var.p, h->S.p, h->Minv.p, h->rhs.p,
-                                  h->hat_node.p, h->hat_w.p, h->node_range.p, h->grp_lo.p, h->grp_width, h->defl_SR.p,
-                                  h->defl_E.p, h->defl_Einv.p, h->defl_y.p, h->px.p, h->pr.p, h->pz.p, h->pp.p, h->pp2.p, h->pq.p,
-                                  h->pcg_part.p, h->pcg_part2.p, h->pcg_state.p, h->pcg_host,
+      iters = launch_pcg_deflated(s, pcg_buffers(h, ritz_count(o) > 0 ? h->ritz_nr : 0, false),
                                   o->pcg_relative_tolerance, o->pcg_max_iterations, kPcgPoll, kPcgPoll, nullptr, 0);
     });
     t->pcg_iterations = iters;
@@ -1315,12 +1492,9 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
     for (auto& e : pe) HIP_OK(hipEventCreate(&e));
     const int reps = std::max(1, repeats) * 8;
     if (iters > 0) {
-      const int it = launch_pcg_deflated(s, h->Pvar, h->hat_K, h->row_ptr.p, h->col_var.p, h->S.p, h->Minv.p,
-                                         h->rhs.p, h->hat_node.p, h->hat_w.p, h->node_range.p, h->grp_lo.p,
-                                         h->grp_width, h->defl_SR.p, h->defl_E.p, h->defl_Einv.p, h->defl_y.p, h->px.p, h->pr.p,
-                                         h->pz.p, h->pp.p, h->pp2.p, h->pq.p, h->pcg_part.p, h->pcg_part2.p,
-                                         h->pcg_state.p, h->pcg_host, o->pcg_relative_tolerance,
-                                         std::min<int32_t>(o->pcg_max_iterations, 4), kPcgPoll, kPcgPoll, pe, reps);
+      const int it = launch_pcg_deflated(s, pcg_buffers(h, ritz_count(o) > 0 ? h->ritz_nr : 0, false),
+                                         o->pcg_relative_tolerance, std::min<int32_t>(o->pcg_max_iterations, 4),
+                                         kPcgPoll, kPcgPoll, pe, reps);
       if (it < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
       HIP_OK(hipEventSynchronize(pe[2]));
       float a = 0.f, b = 0.f;
@@ -1330,8 +1504,9 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
       t->pcg_update_ms = b / reps;
     }
     for (auto& e : pe) (void)hipEventDestroy(e);
-    t->pcg_coarse_columns = 6 * (int64_t)h->hat_K;
-    t->pcg_window_columns = h->grp_width;
+    const int32_t nr_used = h->hat_K > 0 && ritz_count(o) > 0 ? h->ritz_nr : 0;
+    t->pcg_coarse_columns = 6 * (int64_t)std::max(h->hat_K, 0) + nr_used;
+    t->pcg_window_columns = h->grp_width + nr_used;
   }
   launch_unscale_poses(s, h->Pvar, h->px.p, h->pose_scale.p, h->w.p, h->parts_w.p);
   t->back_substitute_ms = timed([&] {

@@ -5,30 +5,43 @@
 // a handful of very small eigenvalues on long trajectories: smooth, global
 // modes of the vertex chain. They are removed by deflation (Saad, Yeung,
 // Erhel, Guyomarc'h, "A deflated version of the conjugate gradient
-// algorithm", SIAM J. Sci. Comput. 21, 2000) with a coarse space W of
-// piecewise-linear "hat" functions over the vertex order (K nodes, 6 K
-// columns): x0 = W E^-1 W'b, E = W'SW, and every search direction is made
+// algorithm", SIAM J. Sci. Comput. 21, 2000) with a coarse space
+// W = [H | R]: H the piecewise-linear "hat" functions over the vertex order
+// (K nodes, 6 K columns), R up to 12 dense columns recycled from earlier
+// solves (Ritz vectors of the preconditioned operator for its smallest
+// eigenvalues, from each solve's Lanczos coefficients; see the ritz kernels
+// below). x0 = W E^-1 W'b, E = W'SW, and every search direction is made
 // S-orthogonal to W with mu = E^-1 (SW)'z. The iteration converges to the
 // same solution as plain PCG (the tolerance is on the true residual
-// recursion), in 3-4x fewer iterations on long random-walk maps.
+// recursion): the hat space cuts iterations 3-4x on long random-walk maps,
+// the recycled vectors another 2-3x on the solves that follow a rejected
+// LM step (same linearization, new damping).
 //
 // Rows of S are handled in groups of 42 vertices (252 entries); SW is stored
-// per group, column-major over the group's window of coarse columns, so that
-// (SW)'z of a group is a set of coalesced column dot products.
+// per group, column-major over the group's window of coarse columns (the hat
+// columns its rows couple to, then the dense ones), so that (SW)'z of a group
+// is a set of coalesced column dot products.
 //
-// Per LM step: k_defl_sr (SW), k_defl_e (E), k_defl_prep (state, E^-1 by
-// Gauss-Jordan, y = E^-1 W'b), k_defl_init (x0, r0, z0 and the first
-// per-group partials). Per PCG iteration: k_pcg_spmv (sums the previous
+// Per LM step: k_defl_sr (SW), k_defl_e and k_defl_dense (E, R'b),
+// k_defl_prep (state, E^-1 by Gauss-Jordan, y = E^-1 W'b), k_defl_init (x0,
+// r0, z0 and the first per-group partials). Per PCG iteration: k_pcg_spmv (sums the previous
 // per-group partials; p = z - W mu + beta p_prev and
 // q = S z + beta S p_prev - (SW) mu) and k_pcg_update (x, r, z and the
-// per-group partials r'z, r'r, E^-1 (SW)'z). Every work-group sums partials in the same fixed order: the
-// iteration is deterministic and uses no atomics.
+// per-group partials r'z, r'r, E^-1 (SW)'z). Every work-group sums partials
+// in the same fixed order: the set-up and the iteration are deterministic and
+// use no atomics.
+#include "ba_pcg.h"
+
+#include <algorithm>
+
 #include "ba_device.h"
 #include "ba_reduce.h"
 
 namespace mlba {
 
-constexpr int kMaxCoarse = 90;   // 6 x 15 hat nodes
+constexpr int kMaxHatNodes = 15;
+constexpr int kMaxRitz = 12;
+constexpr int kMaxCoarse = 6 * kMaxHatNodes + kMaxRitz;  // 102 columns
 constexpr int kRowsPerGroup = 42;
 constexpr int kGroupThreads = 6 * kRowsPerGroup;  // 252 entries (row, component) per group
 constexpr int kGroupBlock = 1024;                 // threads of the per-group kernels
@@ -42,12 +55,15 @@ struct PcgState {
   int32_t iters;
   int32_t done;
   int32_t max_iters;
-  int32_t pad;
+  int32_t defl_ok;  // E was positive definite: the deflation is applied
 };
 
 struct PcgArgs {
-  int32_t Pvar, ne;            // ne = 6 K coarse columns (0: no deflation)
+  int32_t Pvar, ne;            // ne = neh + nr coarse columns (0: no deflation)
+  int32_t neh, nr;             // hat (6 K) and dense columns
   int32_t K, groups;
+  int32_t gwh, n6;             // hat columns of a group window; 6 Pvar
+  int32_t zcap, pad3;
   const int32_t* row_ptr;
   const int32_t* col_var;
   const double* S;
@@ -57,7 +73,8 @@ struct PcgArgs {
   const double* hat_w;         // 2 Pvar: weights of hat_node and hat_node + 1 (0 without deflation)
   const int32_t* node_range;   // 2 K: first and last vertex of each hat's support
   const int32_t* grp_lo;       // groups: first hat node of the group's window of SW columns
-  int32_t gw, pad2;            // window width in columns (6 x nodes)
+  int32_t gw, pad2;            // window width in columns (gwh hat + nr dense)
+  const double* R;             // nr x 6 Pvar dense columns
   double* SR;                  // groups x gw x 252: SW entry (group row t, window column lc)
   double* E;                   // ne x ne
   double* Einv;                // ne x ne
@@ -71,6 +88,9 @@ struct PcgArgs {
   double* part;                // Pvar: p'q of each row
   double* part2;               // groups x (3 + ne): r'z, r'r, b'b and E^-1 (SW)'z of the group's rows
   double* y;                   // ne: E^-1 W'b
+  double* cden;                // nr: R'b
+  double* Zh;                  // zcap x 6 Pvar: deflated preconditioned residuals
+  double* coef;                // 2 zcap: r'z and alpha per iteration
   PcgState* st;
 };
 
@@ -81,6 +101,13 @@ __device__ inline double hat_weight(const PcgArgs& a, int i, int k) {
 
 __device__ inline double* sr_col(const PcgArgs& a, int g, int lc) {
   return a.SR + ((size_t)g * a.gw + lc) * kGroupThreads;
+}
+
+// Coarse column of window column lc of a group whose hat window starts at
+// column c0 (6 x its first node); -1 past the last hat column.
+__device__ inline int win_col(const PcgArgs& a, int c0, int lc) {
+  if (lc >= a.gwh) return a.neh + (lc - a.gwh);
+  return c0 + lc < a.neh ? c0 + lc : -1;
 }
 
 // Block sums of three values (blockDim.x = 64 x waves, at most 16 waves), in
@@ -108,48 +135,115 @@ __device__ inline void block_sum3(double& v0, double& v1, double& v2, double* ld
   v2 = s2;
 }
 
-// SW, one block row of S per work-group: (SW)_i = sum_j S_ij W_j with
-// W_j = w0 e_n0 + w1 e_n0+1, written into the group window.
+constexpr int kRowChunk = 256;  // blocks of a row staged in LDS per pass
+
+// SW, one block row of S per work-group: (SW)_i = sum_j S_ij W_j for the
+// window columns of the row's group, written into the group window. The
+// row's blocks are taken in chunks whose column vertex, hat node and hat
+// weights are staged in LDS first (so that the sums' loads do not wait on
+// each other); thread o owns entry (component o % 6, window column o / 6)
+// and sums the blocks in order: for a hat column those whose vertex lies in
+// the hat's support (W_j = w e_node), for a dense column all (W_j = R_j).
 __global__ __launch_bounds__(256) void k_defl_sr(PcgArgs a) {
-  __shared__ double acc[6 * kMaxCoarse];
-  const int i = blockIdx.x, ne = a.ne;
-  for (int k = threadIdx.x; k < 6 * ne; k += blockDim.x) acc[k] = 0.0;
-  __syncthreads();
+  __shared__ int32_t cj[kRowChunk], cn[kRowChunk];
+  __shared__ double cw0[kRowChunk], cw1[kRowChunk];
+  const int i = blockIdx.x, g = i / kRowsPerGroup, t0 = 6 * (i - g * kRowsPerGroup);
+  const int c0 = 6 * a.grp_lo[g];
   const int rb = a.row_ptr[i], re = a.row_ptr[i + 1];
-  for (int e = rb * 36 + threadIdx.x; e < re * 36; e += blockDim.x) {
-    const int k = e / 36, rem = e % 36, row = rem / 6, c = rem % 6;
-    const int j = a.col_var[k];
-    const int n0 = a.hat_node[j];
-    const double v = a.S[e];
-    atomicAdd(&acc[row * ne + 6 * n0 + c], a.hat_w[2 * j] * v);
-    const double w1 = a.hat_w[2 * j + 1];
-    if (w1 != 0.0) atomicAdd(&acc[row * ne + 6 * (n0 + 1) + c], w1 * v);
+  constexpr int kOutPer = (6 * kMaxCoarse + 255) / 256;
+  double acc[kOutPer];
+#pragma unroll
+  for (int u = 0; u < kOutPer; ++u) acc[u] = 0.0;
+  for (int cb = rb; cb < re; cb += kRowChunk) {
+    const int nb = min(kRowChunk, re - cb);
+    __syncthreads();
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+      const int j = a.col_var[cb + t];
+      cj[t] = j;
+      cn[t] = a.hat_node[j];
+      cw0[t] = a.hat_w[2 * j];
+      cw1[t] = a.hat_w[2 * j + 1];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kOutPer; ++u) {
+      const int o = threadIdx.x + 256 * u;
+      if (o >= 6 * a.gw) continue;
+      const int comp = o % 6, lc = o / 6, col = win_col(a, c0, lc);
+      double s = acc[u];
+      if (col >= 0 && col < a.neh) {
+        const int k = col / 6, cc = col % 6;
+        const double* sv = a.S + (size_t)36 * cb + 6 * comp + cc;
+#pragma unroll 4
+        for (int t = 0; t < nb; ++t) {
+          const int n0 = cn[t];
+          const double w = n0 == k ? cw0[t] : (n0 + 1 == k ? cw1[t] : 0.0);
+          if (w != 0.0) s += w * sv[36 * t];
+        }
+      } else if (col >= a.neh) {
+        const double* Rc = a.R + (size_t)(col - a.neh) * a.n6;
+        const double* sv = a.S + (size_t)36 * cb + 6 * comp;
+#pragma unroll 2
+        for (int t = 0; t < nb; ++t) {
+          const double* st = sv + 36 * t;
+          const double* rj = Rc + 6 * cj[t];
+          s += ((st[0] * rj[0] + st[1] * rj[1]) + (st[2] * rj[2] + st[3] * rj[3])) + (st[4] * rj[4] + st[5] * rj[5]);
+        }
+      }
+      acc[u] = s;
+    }
   }
-  __syncthreads();
-  const int g = i / kRowsPerGroup, t0 = 6 * (i - g * kRowsPerGroup), c0 = 6 * a.grp_lo[g];
-  for (int k = threadIdx.x; k < 6 * a.gw; k += blockDim.x) {
-    const int comp = k % 6, lc = k / 6, gc = c0 + lc;
-    sr_col(a, g, lc)[t0 + comp] = gc < ne ? acc[comp * ne + gc] : 0.0;
+#pragma unroll
+  for (int u = 0; u < kOutPer; ++u) {
+    const int o = threadIdx.x + 256 * u;
+    if (o < 6 * a.gw) sr_col(a, g, o / 6)[t0 + o % 6] = acc[u];
   }
 }
 
 // Entry (e, col) of SW from the group windows.
 __device__ inline double sw_entry(const PcgArgs& a, int e, int col) {
   const int g = e / kGroupThreads;
-  const int lc = col - 6 * a.grp_lo[g];
+  const int lc = col >= a.neh ? a.gwh + (col - a.neh) : col - 6 * a.grp_lo[g];
   return (lc >= 0 && lc < a.gw) ? sr_col(a, g, lc)[e - g * kGroupThreads] : 0.0;
 }
 
-// E = W'(SW), one entry per thread.
+// E = W'(SW), one entry per thread, over the support of a hat column: rows
+// (hat, any) directly, (dense, hat) as the transposed (hat, dense) sum. The
+// dense-dense block is k_defl_dense's. Only the lower triangle is read.
 __global__ void k_defl_e(PcgArgs a) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int ne = a.ne;
   if (t >= ne * ne) return;
-  const int row = t / ne, col = t % ne, k = row / 6, comp = row % 6;
+  int hr = t / ne, oc = t % ne;
+  if (hr >= a.neh) {
+    if (oc >= a.neh) return;
+    const int tmp = hr;
+    hr = oc;
+    oc = tmp;
+  }
+  const int k = hr / 6, comp = hr % 6;
   double s = 0.0;
   for (int i = a.node_range[2 * k]; i <= a.node_range[2 * k + 1]; ++i)
-    s += hat_weight(a, i, k) * sw_entry(a, 6 * i + comp, col);
+    s += hat_weight(a, i, k) * sw_entry(a, 6 * i + comp, oc);
   a.E[t] = s;
+}
+
+// Dense block of E and R'b, one work-group per entry: (a, b < nr) is
+// R_a'(SR)_b over all rows, (a, nr) is R_a'b.
+__global__ __launch_bounds__(256) void k_defl_dense(PcgArgs a) {
+  __shared__ double lds[4];
+  const int ra = blockIdx.x, cb = blockIdx.y;
+  const double* Ra = a.R + (size_t)ra * a.n6;
+  double s = 0.0;
+  for (int e = threadIdx.x; e < a.n6; e += blockDim.x)
+    s += Ra[e] * (cb < a.nr ? sw_entry(a, e, a.neh + cb) : a.b[e]);
+  s = block_sum256(s, lds);
+  if (threadIdx.x == 0) {
+    if (cb < a.nr)
+      a.E[(size_t)(a.neh + ra) * a.ne + a.neh + cb] = s;
+    else
+      a.cden[ra] = s;
+  }
 }
 
 constexpr int kPrepThreads = 1024;
@@ -159,7 +253,7 @@ constexpr int kInvPerThread = (kMaxCoarse * kMaxCoarse + kPrepThreads - 1) / kPr
 // Gauss-Jordan elimination without pivoting (E is SPD; each thread keeps its
 // entries in registers, the pivot row and column go through LDS), and
 // y = E^-1 c. A non-positive pivot disables deflation for this solve
-// (E^-1 = 0, y = 0).
+// (E^-1 = 0, y = 0, defl_ok = 0).
 __global__ __launch_bounds__(kPrepThreads) void k_defl_prep(PcgArgs a, double tol, int32_t max_iters) {
   __shared__ double Ms[kMaxCoarse * kMaxCoarse];
   __shared__ double cpart[kPrepThreads];
@@ -176,15 +270,22 @@ __global__ __launch_bounds__(kPrepThreads) void k_defl_prep(PcgArgs a, double to
     st->iters = 0;
     st->done = 0;
     st->max_iters = max_iters;
+    st->defl_ok = 0;
   }
   if (ne == 0) return;
   // c = W'b: thread (column, g) sums the support rows lo + g, lo + g + G, ...
+  // of a hat column; dense columns come from k_defl_dense.
   {
     const int G = kPrepThreads / ne, col = tid % ne, g = tid / ne, k = col / 6, comp = col % 6;
     double acc = 0.0;
-    if (g < G)
-      for (int i = a.node_range[2 * k] + g; i <= a.node_range[2 * k + 1]; i += G)
-        acc += hat_weight(a, i, k) * a.b[6 * i + comp];
+    if (g < G) {
+      if (col < a.neh) {
+        for (int i = a.node_range[2 * k] + g; i <= a.node_range[2 * k + 1]; i += G)
+          acc += hat_weight(a, i, k) * a.b[6 * i + comp];
+      } else if (g == 0) {
+        acc = a.cden[col - a.neh];
+      }
+    }
     cpart[tid] = acc;
     __syncthreads();
     if (tid < ne) {
@@ -238,6 +339,7 @@ __global__ __launch_bounds__(kPrepThreads) void k_defl_prep(PcgArgs a, double to
     if (er[m] >= 0) Ms[er[m] * ne + ec[m]] = ok ? v[m] : 0.0;
   __syncthreads();
   for (int e = tid; e < n2; e += kPrepThreads) a.Einv[e] = Ms[e];
+  if (tid == 0) a.st->defl_ok = ok ? 1 : 0;
   // y = E^-1 c: thread (row, part) sums a contiguous part of the columns.
   const int nparts = kPrepThreads / ne, row = tid % ne, part = tid / ne, len = (ne + nparts - 1) / nparts;
   double yv = 0.0;
@@ -262,7 +364,7 @@ constexpr int kSrCols = (kMaxCoarse + kGroupWaves - 1) / kGroupWaves;
 constexpr int kSrRows = (kGroupThreads + 63) / 64;
 constexpr int einv_per_thread() {
   int m = 0;
-  for (int ne = 6; ne <= kMaxCoarse; ne += 6) {
+  for (int ne = 1; ne <= kMaxCoarse; ++ne) {
     const int parts = kGroupBlock / ne, len = (ne + parts - 1) / parts;
     m = len > m ? len : m;
   }
@@ -300,10 +402,12 @@ __device__ inline void load_group(const PcgArgs& a, int g, GroupLoads& L) {
     for (int m = 0; m < kEinvPer; ++m) {
       const int lc = part * len + m;
       if (m < len && lc < gw) {
-        if (kFromEinv)
-          L.einv[m] = c0 + lc < ne ? a.Einv[(size_t)(c0 + lc) * ne + rrow] : 0.0;  // E^-1 is symmetric
-        else
+        if (kFromEinv) {
+          const int gc = win_col(a, c0, lc);
+          L.einv[m] = gc >= 0 ? a.Einv[(size_t)gc * ne + rrow] : 0.0;  // E^-1 is symmetric
+        } else {
           L.einv[m] = a.EinvW[((size_t)g * gw + lc) * ne + rrow];
+        }
       }
     }
   }
@@ -373,8 +477,8 @@ __global__ __launch_bounds__(kGroupBlock) void k_defl_init(PcgArgs a) {
     // The group's window rows of E^-1 for k_pcg_update.
     const int c0 = 6 * a.grp_lo[g];
     for (int e = tid; e < a.gw * ne; e += blockDim.x) {
-      const int lc = e / ne, col = e - lc * ne;
-      a.EinvW[(size_t)g * a.gw * ne + e] = c0 + lc < ne ? a.Einv[(size_t)(c0 + lc) * ne + col] : 0.0;
+      const int lc = e / ne, col = e - lc * ne, gc = win_col(a, c0, lc);
+      a.EinvW[(size_t)g * a.gw * ne + e] = gc >= 0 ? a.Einv[(size_t)gc * ne + col] : 0.0;
     }
   }
   __syncthreads();
@@ -387,10 +491,14 @@ __global__ __launch_bounds__(kGroupBlock) void k_defl_init(PcgArgs a) {
     double sy = 0.0, xv = 0.0;
     if (ne) {
       const int c0 = 6 * a.grp_lo[g];
-      for (int lc = 0; lc < a.gw && c0 + lc < ne; ++lc) sy += sr_col(a, g, lc)[tid] * ys[c0 + lc];
+      for (int lc = 0; lc < a.gw; ++lc) {
+        const int gc = win_col(a, c0, lc);
+        if (gc >= 0) sy += sr_col(a, g, lc)[tid] * ys[gc];
+      }
       const int n0 = a.hat_node[row];
       const double w0 = a.hat_w[2 * row], w1 = a.hat_w[2 * row + 1];
       xv = w0 * ys[6 * n0 + comp] + (w1 != 0.0 ? w1 * ys[6 * (n0 + 1) + comp] : 0.0);
+      for (int c = 0; c < a.nr; ++c) xv += a.R[(size_t)c * a.n6 + e] * ys[a.neh + c];
     }
     rv = bv - sy;
     a.x[e] = xv;
@@ -441,8 +549,10 @@ __global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a, int32_t k, const do
     const int q = tid + 256 * m;
     pre[m] = q < ng0 * stride ? a.part2[q] : 0.0;
   }
-  double z_own = 0.0, p_own = 0.0, hw0 = 0.0, hw1 = 0.0;
+  double z_own = 0.0, p_own = 0.0, hw0 = 0.0, hw1 = 0.0, r_own[kMaxRitz];
   int hn0 = 0;
+#pragma unroll
+  for (int c = 0; c < kMaxRitz; ++c) r_own[c] = 0.0;
   if (tid < 6) {
     z_own = a.z[6 * i + tid];
     p_own = p_prev[6 * i + tid];
@@ -450,6 +560,9 @@ __global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a, int32_t k, const do
       hn0 = a.hat_node[i];
       hw0 = a.hat_w[2 * i];
       hw1 = a.hat_w[2 * i + 1];
+#pragma unroll
+      for (int c = 0; c < kMaxRitz; ++c)
+        if (c < a.nr) r_own[c] = a.R[(size_t)c * a.n6 + 6 * i + tid];
     }
   }
   constexpr int kSwCols = (kMaxCoarse + kRowsPerGroup - 1) / kRowsPerGroup;
@@ -521,14 +634,15 @@ __global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a, int32_t k, const do
   if (blockIdx.x == 0 && tid == 0) {
     st->rz[k & 1] = rz;
     st->iters = k;
+    if (k < a.zcap) a.coef[2 * k] = rz;
   }
   if (tid < kGroupThreads) {
     double w = 0.0;
     if (ne) {
 #pragma unroll
       for (int m = 0; m < kSwCols; ++m) {
-        const int lc = g + m * kRowsPerGroup;
-        if (lc < a.gw && c0 + lc < ne) w += sw[m] * mu_s[c0 + lc];
+        const int lc = g + m * kRowsPerGroup, gc = lc < a.gw ? win_col(a, c0, lc) : -1;
+        if (gc >= 0) w += sw[m] * mu_s[gc];
       }
     }
     ysz[tid] = sz;
@@ -545,10 +659,18 @@ __global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a, int32_t k, const do
       qw += ysw[6 * gg + tid];
     }
     const double qa = qz + beta * qp - qw;
-    double pa = z_own + beta * p_own;
-    if (ne) pa -= hw0 * mu_s[6 * hn0 + tid] + (hw1 != 0.0 ? hw1 * mu_s[6 * (hn0 + 1) + tid] : 0.0);
+    double zh = z_own;  // deflated preconditioned residual z - W mu
+    if (ne) {
+      double wm = hw0 * mu_s[6 * hn0 + tid] + (hw1 != 0.0 ? hw1 * mu_s[6 * (hn0 + 1) + tid] : 0.0);
+#pragma unroll
+      for (int c = 0; c < kMaxRitz; ++c)
+        if (c < a.nr) wm += r_own[c] * mu_s[a.neh + c];
+      zh -= wm;
+    }
+    const double pa = zh + beta * p_own;
     a.q[6 * i + tid] = qa;
     p_cur[6 * i + tid] = pa;
+    if (k < a.zcap) a.Zh[(size_t)k * a.n6 + 6 * i + tid] = zh;
     ysz[tid] = pa * qa;
   }
   __syncthreads();
@@ -589,6 +711,7 @@ __global__ __launch_bounds__(kGroupBlock) void k_pcg_update(PcgArgs a, int32_t k
   if (done) return;  // the same value in every thread
   block_sum3(pq, unused0, unused1, lds);
   const double alpha = rz / pq;
+  if (g == 0 && tid == 0 && k < a.zcap) a.coef[2 * k + 1] = alpha;
   if (active) {
     const int e = 6 * row + comp;
     a.x[e] = xv + alpha * pv;
@@ -618,47 +741,172 @@ __global__ void k_pcg_reopen(PcgState* st) {
   st->tol2 = -1.0;
 }
 
+// ------------------------------------------------------------ recycling
+// After a solve, Ritz vectors of its Lanczos process (Saad, Yeung, Erhel,
+// Guyomarc'h 2000, sec. 4; Stathopoulos & Orginos, "Computing and deflating
+// eigenvalues while solving multiple right-hand side linear systems", SIAM
+// J. Sci. Comput. 32, 2010): the stored deflated preconditioned residuals
+// z_k - W mu_k, scaled by 1 / sqrt(r_k'z_k), are the Lanczos vectors of the
+// deflated preconditioned operator, and the CG coefficients give its
+// tridiagonal matrix. The host takes the eigenvectors of that tridiagonal
+// for its smallest eigenvalues (Y), k_ritz_form forms Rn = Zh' Y, and a
+// Rayleigh-Ritz step on span[R, Rn] with respect to (S, M) (k_ritz_spmv,
+// k_ritz_gram, a small generalized eigenproblem on the host, k_ritz_merge)
+// keeps the best r vectors for the next solve.
+
+constexpr int kRitzFormThreads = 256;
+constexpr int kMaxLanczos = 64;
+
+// Rn[c] = sum_j Zh[j] Y[j][c], one thread per (entry, column).
+__global__ __launch_bounds__(kRitzFormThreads) void k_ritz_form(int32_t n6, int32_t m, int32_t r,
+                                                                const double* __restrict__ Zh,
+                                                                const double* __restrict__ Y, double* __restrict__ Rn) {
+  __shared__ double Ys[kMaxLanczos];
+  const int c = blockIdx.y;
+  for (int t = threadIdx.x; t < m; t += blockDim.x) Ys[t] = Y[t * r + c];
+  __syncthreads();
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n6) return;
+  double acc = 0.0;
+#pragma unroll 8
+  for (int j = 0; j < m; ++j) acc += Zh[(size_t)j * n6 + e] * Ys[j];
+  Rn[(size_t)c * n6 + e] = acc;
+}
+
+// SRn = S Rn, one block row per work-group; the row's column vertices are
+// staged in LDS per chunk, thread o < 6 r owns entry (component o % 6,
+// column o / 6) and sums the row's blocks in order.
+__global__ __launch_bounds__(128) void k_ritz_spmv(int32_t n6, int32_t r, const int32_t* __restrict__ row_ptr,
+                                                   const int32_t* __restrict__ col_var, const double* __restrict__ S,
+                                                   const double* __restrict__ Rn, double* __restrict__ SRn) {
+  __shared__ int32_t cj[kRowChunk];
+  const int i = blockIdx.x, o = threadIdx.x;
+  const int comp = o % 6, c = o / 6;
+  const int rb = row_ptr[i], re = row_ptr[i + 1];
+  const double* Rc = Rn + (size_t)min(c, kMaxRitz - 1) * n6;
+  double acc = 0.0;
+  for (int cb = rb; cb < re; cb += kRowChunk) {
+    const int nb = min(kRowChunk, re - cb);
+    __syncthreads();
+    for (int t = o; t < nb; t += blockDim.x) cj[t] = col_var[cb + t];
+    __syncthreads();
+    if (o < 6 * r) {
+      const double* sv = S + (size_t)36 * cb + 6 * comp;
+#pragma unroll 2
+      for (int t = 0; t < nb; ++t) {
+        const double* st = sv + 36 * t;
+        const double* rj = Rc + 6 * cj[t];
+        acc += ((st[0] * rj[0] + st[1] * rj[1]) + (st[2] * rj[2] + st[3] * rj[3])) + (st[4] * rj[4] + st[5] * rj[5]);
+      }
+    }
+  }
+  if (o < 6 * r) SRn[(size_t)c * n6 + 6 * i + comp] = acc;
+}
+
+// Gram matrices of Y = [R | Rn], one work-group per entry (a, b):
+// A_ab = Y_a'(S Y_b) (S R from the group windows of this solve's SW),
+// B_ab = Y_a'(M Y_b) with M the 6x6 diagonal blocks of S.
+__global__ __launch_bounds__(256) void k_ritz_gram(PcgArgs a, int32_t r, const double* __restrict__ Rn,
+                                                   const double* __restrict__ SRn, const int32_t* __restrict__ diag_pos,
+                                                   double* __restrict__ AB) {
+  __shared__ double lds[4];
+  const int ia = blockIdx.x, ib = blockIdx.y, nb = a.nr + r;
+  const double* Ya = ia < a.nr ? a.R + (size_t)ia * a.n6 : Rn + (size_t)(ia - a.nr) * a.n6;
+  const double* Yb = ib < a.nr ? a.R + (size_t)ib * a.n6 : Rn + (size_t)(ib - a.nr) * a.n6;
+  double sa = 0.0, sb = 0.0;
+  for (int e = threadIdx.x; e < a.n6; e += blockDim.x) {
+    const double sy = ib < a.nr ? sw_entry(a, e, a.neh + ib) : SRn[(size_t)(ib - a.nr) * a.n6 + e];
+    const int i = e / 6, comp = e % 6;
+    const double* Mi = a.S + (size_t)36 * diag_pos[i] + 6 * comp;
+    const double* yb = Yb + 6 * i;
+    const double my = ((Mi[0] * yb[0] + Mi[1] * yb[1]) + (Mi[2] * yb[2] + Mi[3] * yb[3])) + (Mi[4] * yb[4] + Mi[5] * yb[5]);
+    sa += Ya[e] * sy;
+    sb += Ya[e] * my;
+  }
+  sa = block_sum256(sa, lds);
+  __syncthreads();
+  sb = block_sum256(sb, lds);
+  if (threadIdx.x == 0) {
+    AB[(size_t)ia * nb + ib] = sa;
+    AB[(size_t)nb * nb + (size_t)ia * nb + ib] = sb;
+  }
+}
+
+// Rnext[c] = sum_a R[a] Cm[a][c] + sum_a Rn[a] Cm[nr_old + a][c], one
+// thread per (entry, column).
+__global__ __launch_bounds__(256) void k_ritz_merge(int32_t n6, const double* __restrict__ R, int32_t nr_old,
+                                                    const double* __restrict__ Rn, int32_t r,
+                                                    const double* __restrict__ Cm, int32_t r_next,
+                                                    double* __restrict__ Rnext) {
+  __shared__ double Cs[2 * kMaxRitz];
+  const int nb = nr_old + r, c = blockIdx.y;
+  for (int t = threadIdx.x; t < nb; t += blockDim.x) Cs[t] = Cm[t * r_next + c];
+  __syncthreads();
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n6) return;
+  double acc = 0.0;
+  for (int ia = 0; ia < nb; ++ia)
+    acc += (ia < nr_old ? R[(size_t)ia * n6 + e] : Rn[(size_t)(ia - nr_old) * n6 + e]) * Cs[ia];
+  Rnext[(size_t)c * n6 + e] = acc;
+}
+
 // ------------------------------------------------------------ launchers
 
 size_t pcg_state_bytes() { return sizeof(PcgState); }
-int32_t pcg_max_coarse() { return kMaxCoarse; }
+int32_t pcg_max_hat_nodes() { return kMaxHatNodes; }
+int32_t pcg_max_ritz() { return kMaxRitz; }
+int32_t pcg_max_lanczos() { return kMaxLanczos; }
 int32_t pcg_group_rows() { return kRowsPerGroup; }
 int32_t pcg_update_groups(int32_t Pvar) { return (Pvar + kRowsPerGroup - 1) / kRowsPerGroup; }
+bool pcg_deflation_applied(const void* state_host) {
+  return static_cast<const PcgState*>(state_host)->defl_ok != 0;
+}
 
-// Deflated PCG solve of S y = b. Buffers (device): x, r, z, p0, p1, q
-// (6 Pvar), part (Pvar), part2 (groups x (3 + ne)), SR (groups x gw x 252),
-// E (ne x ne), Einv (ne x ne + groups x gw x ne), y (ne). The host polls the state after `first`
+static PcgArgs make_args(const PcgBuffers& B) {
+  PcgArgs a;
+  a.Pvar = B.Pvar;
+  a.K = B.K;
+  a.neh = 6 * B.K;
+  a.nr = B.K > 0 ? B.nr : 0;
+  a.ne = a.neh + a.nr;
+  a.groups = pcg_update_groups(B.Pvar);
+  a.gwh = B.gwh;
+  a.gw = B.gwh + a.nr;
+  a.n6 = 6 * B.Pvar;
+  a.zcap = std::min(B.zcap, kMaxLanczos);
+  a.pad2 = a.pad3 = 0;
+  a.row_ptr = B.row_ptr; a.col_var = B.col_var; a.S = B.S; a.Minv = B.Minv; a.b = B.b;
+  a.hat_node = B.hat_node; a.hat_w = B.hat_w; a.node_range = B.node_range; a.grp_lo = B.grp_lo; a.R = B.R;
+  a.SR = B.SR; a.E = B.E; a.Einv = B.Einv; a.EinvW = B.Einv + (size_t)a.ne * a.ne;
+  a.x = B.x; a.r = B.r; a.z = B.z; a.p0 = B.p0; a.p1 = B.p1; a.q = B.q;
+  a.part = B.part; a.part2 = B.part2; a.y = B.y; a.cden = B.cden; a.Zh = B.Zh; a.coef = B.coef;
+  a.st = static_cast<PcgState*>(B.state_dev);
+  return a;
+}
+
+// Deflated PCG solve of S y = b. The host polls the state after `first`
 // iterations, then every `poll` iterations. With `probe` (three events),
 // k_pcg_spmv and then k_pcg_update of one more iteration are each launched
 // `probe_reps` times back to back after the solve and timed as two batches
-// (k_pcg_spmv is idempotent; x, r and z then hold extra steps). Returns the iteration count or -1 on a launch error.
-int launch_pcg_deflated(hipStream_t s, int32_t Pvar, int32_t K, const int32_t* row_ptr, const int32_t* col_var,
-                        const double* S, const double* Minv, const double* b, const int32_t* hat_node,
-                        const double* hat_w, const int32_t* node_range, const int32_t* grp_lo, int32_t gw,
-                        double* SR, double* E, double* Einv, double* y,
-                        double* x, double* r, double* z, double* p0, double* p1, double* q,
-                        double* part, double* part2, void* state_dev, void* state_host, double tol,
-                        int32_t max_iters, int32_t first, int32_t poll, const hipEvent_t* probe,
-                        int32_t probe_reps) {
+// (k_pcg_spmv is idempotent; x, r and z then hold extra steps). Returns the
+// iteration count or -1 on a launch error.
+int launch_pcg_deflated(hipStream_t s, const PcgBuffers& B, double tol, int32_t max_iters, int32_t first,
+                        int32_t poll, const hipEvent_t* probe, int32_t probe_reps) {
+  const int32_t Pvar = B.Pvar;
   if (Pvar <= 0) return 0;
-  PcgArgs a;
-  a.Pvar = Pvar;
-  a.K = K;
-  a.ne = 6 * K;
-  a.groups = pcg_update_groups(Pvar);
-  a.row_ptr = row_ptr; a.col_var = col_var; a.S = S; a.Minv = Minv; a.b = b;
-  a.hat_node = hat_node; a.hat_w = hat_w; a.node_range = node_range; a.grp_lo = grp_lo; a.gw = gw; a.pad2 = 0;
-  a.SR = SR; a.E = E; a.Einv = Einv; a.EinvW = Einv + (size_t)a.ne * a.ne; a.x = x; a.r = r; a.z = z; a.p0 = p0; a.p1 = p1; a.q = q;
-  a.part = part; a.part2 = part2; a.y = y;
-  a.st = static_cast<PcgState*>(state_dev);
+  if (B.K > kMaxHatNodes || B.nr > kMaxRitz || B.nr < 0) return -1;
+  const PcgArgs a = make_args(B);
+  double* p0 = B.p0;
+  double* p1 = B.p1;
   const int groups = a.groups;
   if (a.ne > 0) {
     k_defl_sr<<<Pvar, 256, 0, s>>>(a);
     k_defl_e<<<(a.ne * a.ne + 255) / 256, 256, 0, s>>>(a);
+    if (a.nr > 0) k_defl_dense<<<dim3(a.nr, a.nr + 1), 256, 0, s>>>(a);
   }
   k_defl_prep<<<1, kPrepThreads, 0, s>>>(a, tol, max_iters);
   k_defl_init<<<groups, kGroupBlock, 0, s>>>(a);
-  PcgState* hs = static_cast<PcgState*>(state_host);
+  PcgState* hs = static_cast<PcgState*>(B.state_host);
   int k = 0;
   while (true) {
     const int n = std::min(k == 0 ? std::max(first, 1) : poll, max_iters - k);
@@ -686,6 +934,35 @@ int launch_pcg_deflated(hipStream_t s, int32_t Pvar, int32_t K, const int32_t* r
     if (hipGetLastError() != hipSuccess) return -1;
   }
   return iters;
+}
+
+void launch_ritz_form(hipStream_t s, int32_t Pvar, int32_t m, int32_t r, const double* Zh, const double* Y,
+                      double* Rn) {
+  const int n6 = 6 * Pvar;
+  if (n6 <= 0 || m <= 0 || m > kMaxLanczos || r <= 0 || r > kMaxRitz) return;
+  k_ritz_form<<<dim3((n6 + kRitzFormThreads - 1) / kRitzFormThreads, r), kRitzFormThreads, 0, s>>>(n6, m, r, Zh, Y,
+                                                                                                 Rn);
+}
+
+void launch_ritz_spmv(hipStream_t s, int32_t Pvar, int32_t r, const int32_t* row_ptr, const int32_t* col_var,
+                      const double* S, const double* Rn, double* SRn) {
+  if (Pvar <= 0 || r <= 0 || r > kMaxRitz) return;
+  k_ritz_spmv<<<Pvar, 128, 0, s>>>(6 * Pvar, r, row_ptr, col_var, S, Rn, SRn);
+}
+
+void launch_ritz_gram(hipStream_t s, const PcgBuffers& B, int32_t r, const double* Rn, const double* SRn,
+                      const int32_t* diag_pos, double* AB) {
+  const PcgArgs a = make_args(B);
+  const int nb = a.nr + r;
+  if (B.Pvar <= 0 || r <= 0 || r > kMaxRitz || a.K <= 0) return;
+  k_ritz_gram<<<dim3(nb, nb), 256, 0, s>>>(a, r, Rn, SRn, diag_pos, AB);
+}
+
+void launch_ritz_merge(hipStream_t s, int32_t Pvar, const double* R, int32_t nr_old, const double* Rn, int32_t r,
+                       const double* Cm, int32_t r_next, double* Rnext) {
+  const int n6 = 6 * Pvar;
+  if (n6 <= 0 || r_next <= 0 || r_next > kMaxRitz || nr_old + r > 2 * kMaxRitz) return;
+  k_ritz_merge<<<dim3((n6 + 255) / 256, r_next), 256, 0, s>>>(n6, R, nr_old, Rn, r, Cm, r_next, Rnext);
 }
 
 }  // namespace mlba
