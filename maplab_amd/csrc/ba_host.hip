@@ -159,10 +159,10 @@ struct mlba_handle {
   DBuf<double> inc_w, inc_r0, lm_a;     // per-linearization Schur factors (incidence order) and anchor maps
   DBuf<double> inc_q, inc_rhs;          // per-iteration Schur factors
   DBuf<double> schur_partial;           // 36 per work item
-  DBuf<int32_t> hat_node, node_range, grp_lo;
+  DBuf<int32_t> hat_node, node_range, grp_lo, blk_hat, blk_rng;
   std::vector<int32_t> h_row_ptr, h_col_var;  // host copy of the S block structure
   int32_t grp_width = 0;                // hat columns of the per-group SW window
-  DBuf<double> hat_w, defl_SR, defl_E, defl_Einv, defl_y, defl_cden, pcg_part2;
+  DBuf<double> hat_w, blk_w, defl_SR, defl_E, defl_Einv, defl_y, defl_cden, pcg_part2;
   int32_t hat_K = -1;  // coarse nodes of the current hat basis
   int32_t pcg_last_iters = 0;  // iterations of the previous solve: length of the next solve's first batch
   // Recycled deflation columns: ritz[ritz_cur] holds ritz_nr of them; the
@@ -565,7 +565,30 @@ int ensure_coarse_basis(mlba_handle* h, int32_t requested) {
     }
   }
   h->grp_width = 6 * width;
+  // Hat node and weights of each block's column vertex.
+  std::vector<int32_t> bh(std::max<size_t>(h->h_col_var.size(), 1), 0);
+  std::vector<double> bw(2 * bh.size(), 0.0);
+  for (size_t kb = 0; kb < h->h_col_var.size(); ++kb) {
+    const int j = h->h_col_var[kb];
+    bh[kb] = node[j];
+    bw[2 * kb] = wts[2 * j];
+    bw[2 * kb + 1] = wts[2 * j + 1];
+  }
+  // Blocks of each row in the support of each hat (columns are sorted, the
+  // support is a vertex range, so they are contiguous).
+  std::vector<int32_t> br(2 * (size_t)std::max(Pvar, 1) * std::max(K, 1), 0);
+  for (int i = 0; i < Pvar && K > 0; ++i) {
+    const int32_t* b = h->h_col_var.data() + h->h_row_ptr[i];
+    const int32_t* e = h->h_col_var.data() + h->h_row_ptr[i + 1];
+    for (int k = 0; k < K; ++k) {
+      br[2 * ((size_t)i * K + k)] = (int32_t)(std::lower_bound(b, e, range[2 * k]) - h->h_col_var.data());
+      br[2 * ((size_t)i * K + k) + 1] = (int32_t)(std::upper_bound(b, e, range[2 * k + 1]) - h->h_col_var.data());
+    }
+  }
   int rc;
+  if ((rc = upload(h, h->blk_hat, bh.data(), bh.size())) || (rc = upload(h, h->blk_w, bw.data(), bw.size())) ||
+      (rc = upload(h, h->blk_rng, br.data(), br.size())))
+    return rc;
   if ((rc = upload(h, h->hat_node, node.data(), node.size())) || (rc = upload(h, h->hat_w, wts.data(), wts.size())) ||
       (rc = upload(h, h->node_range, range.data(), range.size())) || (rc = upload(h, h->grp_lo, lo.data(), lo.size())))
     return rc;
@@ -620,6 +643,7 @@ PcgBuffers pcg_buffers(mlba_handle* h, int32_t nr, bool keep_lanczos) {
   B.row_ptr = h->row_ptr.p; B.col_var = h->col_var.p; B.S = h->S.p; B.Minv = h->Minv.p; B.b = h->rhs.p;
   B.hat_node = h->hat_node.p; B.hat_w = h->hat_w.p; B.node_range = h->node_range.p; B.grp_lo = h->grp_lo.p;
   B.R = h->ritz[h->ritz_cur].p;
+  B.blk_hat = h->blk_hat.p; B.blk_w = h->blk_w.p; B.blk_rng = h->blk_rng.p;
   B.SR = h->defl_SR.p; B.E = h->defl_E.p; B.Einv = h->defl_Einv.p; B.y = h->defl_y.p; B.cden = h->defl_cden.p;
   B.x = h->px.p; B.r = h->pr.p; B.z = h->pz.p; B.p0 = h->pp.p; B.p1 = h->pp2.p; B.q = h->pq.p;
   B.part = h->pcg_part.p; B.part2 = h->pcg_part2.p; B.Zh = h->ritz_Zh.p; B.coef = h->ritz_coef.p;

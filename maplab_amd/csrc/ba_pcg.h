@@ -28,6 +28,9 @@ struct PcgBuffers {
   const int32_t* node_range;   // 2 K: first and last vertex of each hat's support
   const int32_t* grp_lo;       // groups: first hat node of the group's window
   const double* R;             // nr x 6 Pvar, column-major
+  const int32_t* blk_hat;      // per block of S: hat node of its column vertex
+  const double* blk_w;         // per block of S: the two hat weights of its column vertex
+  const int32_t* blk_rng;      // Pvar x K x 2: first and end block of row i in the support of hat k
   double* SR;                  // groups x (gwh + nr) x 252
   double* E;                   // ne x ne (ne = 6 K + nr)
   double* Einv;                // ne x ne + groups x (gwh + nr) x ne

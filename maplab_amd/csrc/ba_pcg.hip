@@ -75,6 +75,9 @@ struct PcgArgs {
   const int32_t* grp_lo;       // groups: first hat node of the group's window of SW columns
   int32_t gw, pad2;            // window width in columns (gwh hat + nr dense)
   const double* R;             // nr x 6 Pvar dense columns
+  const int32_t* blk_hat;      // per block of S: hat node of its column vertex
+  const double* blk_w;         // per block of S: the two hat weights of its column vertex
+  const int32_t* blk_rng;      // Pvar x K x 2: blocks of row i in the support of hat k
   double* SR;                  // groups x gw x 252: SW entry (group row t, window column lc)
   double* E;                   // ne x ne
   double* Einv;                // ne x ne
@@ -135,68 +138,73 @@ __device__ inline void block_sum3(double& v0, double& v1, double& v2, double* ld
   v2 = s2;
 }
 
-constexpr int kRowChunk = 256;  // blocks of a row staged in LDS per pass
-
-// SW, one block row of S per work-group: (SW)_i = sum_j S_ij W_j for the
-// window columns of the row's group, written into the group window. The
-// row's blocks are taken in chunks whose column vertex, hat node and hat
-// weights are staged in LDS first (so that the sums' loads do not wait on
-// each other); thread o owns entry (component o % 6, window column o / 6)
-// and sums the blocks in order: for a hat column those whose vertex lies in
-// the hat's support (W_j = w e_node), for a dense column all (W_j = R_j).
+// SW for the hat columns, one block row of S per work-group:
+// (SH)_i = sum_j S_ij H_j over the group window's hat columns, written into
+// the group window. Thread o owns entry (component o % 6, window column
+// o / 6) of hat node k and sums, in order, the row's blocks whose vertex lies
+// in the hat's support: a contiguous range of the row (blk_rng), each with
+// weight w0 or w1 of its vertex (its first hat node is k or k - 1).
 __global__ __launch_bounds__(256) void k_defl_sr(PcgArgs a) {
-  __shared__ int32_t cj[kRowChunk], cn[kRowChunk];
-  __shared__ double cw0[kRowChunk], cw1[kRowChunk];
   const int i = blockIdx.x, g = i / kRowsPerGroup, t0 = 6 * (i - g * kRowsPerGroup);
   const int c0 = 6 * a.grp_lo[g];
-  const int rb = a.row_ptr[i], re = a.row_ptr[i + 1];
-  constexpr int kOutPer = (6 * kMaxCoarse + 255) / 256;
-  double acc[kOutPer];
-#pragma unroll
-  for (int u = 0; u < kOutPer; ++u) acc[u] = 0.0;
-  for (int cb = rb; cb < re; cb += kRowChunk) {
-    const int nb = min(kRowChunk, re - cb);
-    __syncthreads();
-    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
-      const int j = a.col_var[cb + t];
-      cj[t] = j;
-      cn[t] = a.hat_node[j];
-      cw0[t] = a.hat_w[2 * j];
-      cw1[t] = a.hat_w[2 * j + 1];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < kOutPer; ++u) {
-      const int o = threadIdx.x + 256 * u;
-      if (o >= 6 * a.gw) continue;
-      const int comp = o % 6, lc = o / 6, col = win_col(a, c0, lc);
-      double s = acc[u];
-      if (col >= 0 && col < a.neh) {
-        const int k = col / 6, cc = col % 6;
-        const double* sv = a.S + (size_t)36 * cb + 6 * comp + cc;
+  for (int o = threadIdx.x; o < 6 * a.gwh; o += blockDim.x) {
+    const int comp = o % 6, lc = o / 6, col = win_col(a, c0, lc);
+    double acc = 0.0;
+    if (col >= 0) {
+      const int k = col / 6, cc = col % 6;
+      const int lo = a.blk_rng[2 * (i * a.K + k)], hi = a.blk_rng[2 * (i * a.K + k) + 1];
 #pragma unroll 4
-        for (int t = 0; t < nb; ++t) {
-          const int n0 = cn[t];
-          const double w = n0 == k ? cw0[t] : (n0 + 1 == k ? cw1[t] : 0.0);
-          if (w != 0.0) s += w * sv[36 * t];
-        }
-      } else if (col >= a.neh) {
-        const double* Rc = a.R + (size_t)(col - a.neh) * a.n6;
-        const double* sv = a.S + (size_t)36 * cb + 6 * comp;
-#pragma unroll 2
-        for (int t = 0; t < nb; ++t) {
-          const double* st = sv + 36 * t;
-          const double* rj = Rc + 6 * cj[t];
-          s += ((st[0] * rj[0] + st[1] * rj[1]) + (st[2] * rj[2] + st[3] * rj[3])) + (st[4] * rj[4] + st[5] * rj[5]);
-        }
+      for (int kb = lo; kb < hi; ++kb) {
+        const int n0 = a.blk_hat[kb];
+        const double w0 = a.blk_w[2 * kb], w1 = a.blk_w[2 * kb + 1];
+        acc += (n0 == k ? w0 : w1) * a.S[(size_t)36 * kb + 6 * comp + cc];
       }
-      acc[u] = s;
+    }
+    sr_col(a, g, lc)[t0 + comp] = acc;
+  }
+}
+
+// SW for the dense columns (or S Rn for the recycling), one block row per
+// work-group of six waves: wave comp, lane l sums over the row's blocks
+// l, l + 64, ... the products S_ij[comp][:] . R_c[6 j ..] for every dense
+// column c, then a fixed-order wave sum per column. Writes the group window
+// (dense part) or a plain column-major buffer.
+constexpr int kDenseThreads = 384;
+template <bool kWindow>
+__global__ __launch_bounds__(kDenseThreads) void k_row_dense(PcgArgs a, const int32_t* __restrict__ row_ptr,
+                                                             const int32_t* __restrict__ col_var,
+                                                             const double* __restrict__ S,
+                                                             const double* __restrict__ R, int32_t nr, int32_t n6,
+                                                             double* __restrict__ out) {
+  const int i = blockIdx.x, comp = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int rb = row_ptr[i], re = row_ptr[i + 1];
+  double acc[kMaxRitz];
+#pragma unroll
+  for (int c = 0; c < kMaxRitz; ++c) acc[c] = 0.0;
+  for (int kb = rb + lane; kb < re; kb += 64) {
+    const double* sv = S + (size_t)36 * kb + 6 * comp;
+    const double s0 = sv[0], s1 = sv[1], s2 = sv[2], s3 = sv[3], s4 = sv[4], s5 = sv[5];
+    const double* rj = R + 6 * col_var[kb];
+#pragma unroll
+    for (int c = 0; c < kMaxRitz; ++c) {
+      if (c < nr) {
+        const double* r = rj + (size_t)c * n6;
+        acc[c] += ((s0 * r[0] + s1 * r[1]) + (s2 * r[2] + s3 * r[3])) + (s4 * r[4] + s5 * r[5]);
+      }
     }
   }
 #pragma unroll
-  for (int u = 0; u < kOutPer; ++u) {
-    const int o = threadIdx.x + 256 * u;
-    if (o < 6 * a.gw) sr_col(a, g, o / 6)[t0 + o % 6] = acc[u];
+  for (int c = 0; c < kMaxRitz; ++c) {
+    if (c >= nr) break;
+    const double v = wave_sum(acc[c]);
+    if (lane == 0) {
+      if (kWindow) {
+        const int g = i / kRowsPerGroup;
+        sr_col(a, g, a.gwh + c)[6 * (i - g * kRowsPerGroup) + comp] = v;
+      } else {
+        out[(size_t)c * n6 + 6 * i + comp] = v;
+      }
+    }
   }
 }
 
@@ -207,25 +215,26 @@ __device__ inline double sw_entry(const PcgArgs& a, int e, int col) {
   return (lc >= 0 && lc < a.gw) ? sr_col(a, g, lc)[e - g * kGroupThreads] : 0.0;
 }
 
-// E = W'(SW), one entry per thread, over the support of a hat column: rows
-// (hat, any) directly, (dense, hat) as the transposed (hat, dense) sum. The
-// dense-dense block is k_defl_dense's. Only the lower triangle is read.
-__global__ void k_defl_e(PcgArgs a) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ne = a.ne;
-  if (t >= ne * ne) return;
-  int hr = t / ne, oc = t % ne;
-  if (hr >= a.neh) {
-    if (oc >= a.neh) return;
-    const int tmp = hr;
-    hr = oc;
-    oc = tmp;
+// E = W'(SW) for the hat rows: one wave per (hat node k, column oc) sums the
+// six rows (k, comp) over the hat's support (lanes take every 64th support
+// row, then a fixed-order wave sum). A dense column's entries are also the
+// transposed (dense, hat) entries. The dense-dense block is k_defl_dense's.
+__global__ __launch_bounds__(64) void k_defl_e(PcgArgs a) {
+  const int k = blockIdx.x, oc = blockIdx.y, lane = threadIdx.x;
+  double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int i = a.node_range[2 * k] + lane; i <= a.node_range[2 * k + 1]; i += 64) {
+    const double w = hat_weight(a, i, k);
+#pragma unroll
+    for (int comp = 0; comp < 6; ++comp) s[comp] += w * sw_entry(a, 6 * i + comp, oc);
   }
-  const int k = hr / 6, comp = hr % 6;
-  double s = 0.0;
-  for (int i = a.node_range[2 * k]; i <= a.node_range[2 * k + 1]; ++i)
-    s += hat_weight(a, i, k) * sw_entry(a, 6 * i + comp, oc);
-  a.E[t] = s;
+#pragma unroll
+  for (int comp = 0; comp < 6; ++comp) {
+    const double v = wave_sum(s[comp]);
+    if (lane == 0) {
+      a.E[(size_t)(6 * k + comp) * a.ne + oc] = v;
+      if (oc >= a.neh) a.E[(size_t)oc * a.ne + 6 * k + comp] = v;
+    }
+  }
 }
 
 // Dense block of E and R'b, one work-group per entry: (a, b < nr) is
@@ -247,18 +256,22 @@ __global__ __launch_bounds__(256) void k_defl_dense(PcgArgs a) {
 }
 
 constexpr int kPrepThreads = 1024;
-constexpr int kInvPerThread = (kMaxCoarse * kMaxCoarse + kPrepThreads - 1) / kPrepThreads;
+constexpr int kTri = kMaxCoarse * (kMaxCoarse + 1) / 2;  // entries of the lower triangle
+constexpr int kInvPerThread = (kTri + kPrepThreads - 1) / kPrepThreads;
 
-// Solve set-up (one work-group): the PCG state, c = W'b, E^-1 by in-place
-// Gauss-Jordan elimination without pivoting (E is SPD; each thread keeps its
-// entries in registers, the pivot row and column go through LDS), and
-// y = E^-1 c. A non-positive pivot disables deflation for this solve
-// (E^-1 = 0, y = 0, defl_ok = 0).
+// Solve set-up (one work-group): the PCG state, c = W'b, E^-1 by the sweep
+// operator on the lower triangle (Goodnight, "A tutorial on the SWEEP
+// operator", Amer. Statistician 33, 1979: sweeping pivot k of a symmetric A
+// maps a_kk -> -1/a_kk, a_ik -> a_ik / a_kk, a_ij -> a_ij - a_ik a_kj / a_kk,
+// keeps A symmetric, and sweeping every pivot gives -A^-1; E is SPD, so the
+// pivots are the positive Cholesky pivots and no pivoting is needed). Each
+// thread keeps its lower-triangle entries in registers, pivot column k goes
+// through LDS. Then y = E^-1 c. A non-positive pivot disables deflation for
+// this solve (E^-1 = 0, y = 0, defl_ok = 0).
 __global__ __launch_bounds__(kPrepThreads) void k_defl_prep(PcgArgs a, double tol, int32_t max_iters) {
   __shared__ double Ms[kMaxCoarse * kMaxCoarse];
   __shared__ double cpart[kPrepThreads];
   __shared__ double cvec[kMaxCoarse];
-  __shared__ double rowk[kMaxCoarse];
   __shared__ double colk[kMaxCoarse];
   const int ne = a.ne, tid = threadIdx.x;
   if (tid == 0) {
@@ -294,50 +307,58 @@ __global__ __launch_bounds__(kPrepThreads) void k_defl_prep(PcgArgs a, double to
       cvec[tid] = c;
     }
   }
-  // In-place Gauss-Jordan: for pivot k, a_rc -= a_rk a_kc / p, a_kc /= p,
-  // a_rk = -a_rk / p, a_kk = 1 / p.
-  const int n2 = ne * ne;
+  const int nt = ne * (ne + 1) / 2;
   double v[kInvPerThread];
-  int er[kInvPerThread], ec[kInvPerThread];  // entry (row, column); -1 past the matrix
+  int er[kInvPerThread], ec[kInvPerThread];  // entry (row >= column); -1 past the triangle
 #pragma unroll
   for (int m = 0; m < kInvPerThread; ++m) {
     const int e = tid + m * kPrepThreads;
-    er[m] = e < n2 ? e / ne : -1;
-    ec[m] = e < n2 ? e - (e / ne) * ne : -1;
-    v[m] = e < n2 ? (ec[m] <= er[m] ? a.E[er[m] * ne + ec[m]] : a.E[ec[m] * ne + er[m]]) : 0.0;
+    int i = -1, j = -1;
+    if (e < nt) {
+      i = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+      while (i * (i + 1) / 2 > e) --i;
+      while ((i + 1) * (i + 2) / 2 <= e) ++i;
+      j = e - i * (i + 1) / 2;
+    }
+    er[m] = i;
+    ec[m] = j;
+    v[m] = e < nt ? a.E[(size_t)i * ne + j] : 0.0;
   }
   bool ok = true;
   for (int k = 0; k < ne; ++k) {
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < kInvPerThread; ++m) {
-      if (er[m] == k) rowk[ec[m]] = v[m];
-      if (ec[m] == k) colk[er[m]] = v[m];
+      if (er[m] == k) colk[ec[m]] = v[m];       // (k, j <= k)
+      else if (ec[m] == k) colk[er[m]] = v[m];  // (i > k, k)
     }
     __syncthreads();
-    const double p = rowk[k];
+    const double p = colk[k];
     if (!(p > 0.0)) {
       ok = false;
       break;
     }
     const double ip = 1.0 / p;
-    // Branch-free update of the owned entries.
 #pragma unroll
     for (int m = 0; m < kInvPerThread; ++m) {
-      const bool isr = er[m] == k, isc = ec[m] == k;
-      const double ck = colk[max(er[m], 0)], rk = rowk[max(ec[m], 0)];
-      const double vs = v[m] * ip;
-      double nv = fma(-ck * ip, rk, v[m]);
-      nv = isc ? -vs : nv;
-      nv = isr ? vs : nv;
-      v[m] = (isr && isc) ? ip : nv;
+      const int i = max(er[m], 0), j = max(ec[m], 0);
+      const bool ri = er[m] == k, rj = ec[m] == k;
+      double nv = fma(-colk[i] * ip, colk[j], v[m]);
+      nv = (ri || rj) ? v[m] * ip : nv;
+      v[m] = (ri && rj) ? -ip : nv;
     }
   }
   __syncthreads();
+  // E^-1 = -(swept matrix), both triangles.
 #pragma unroll
   for (int m = 0; m < kInvPerThread; ++m)
-    if (er[m] >= 0) Ms[er[m] * ne + ec[m]] = ok ? v[m] : 0.0;
+    if (er[m] >= 0) {
+      const double x = ok ? -v[m] : 0.0;
+      Ms[er[m] * ne + ec[m]] = x;
+      Ms[ec[m] * ne + er[m]] = x;
+    }
   __syncthreads();
+  const int n2 = ne * ne;
   for (int e = tid; e < n2; e += kPrepThreads) a.Einv[e] = Ms[e];
   if (tid == 0) a.st->defl_ok = ok ? 1 : 0;
   // y = E^-1 c: thread (row, part) sums a contiguous part of the columns.
@@ -750,7 +771,7 @@ __global__ void k_pcg_reopen(PcgState* st) {
 // deflated preconditioned operator, and the CG coefficients give its
 // tridiagonal matrix. The host takes the eigenvectors of that tridiagonal
 // for its smallest eigenvalues (Y), k_ritz_form forms Rn = Zh' Y, and a
-// Rayleigh-Ritz step on span[R, Rn] with respect to (S, M) (k_ritz_spmv,
+// Rayleigh-Ritz step on span[R, Rn] with respect to (S, M) (k_row_dense for S Rn,
 // k_ritz_gram, a small generalized eigenproblem on the host, k_ritz_merge)
 // keeps the best r vectors for the next solve.
 
@@ -771,36 +792,6 @@ __global__ __launch_bounds__(kRitzFormThreads) void k_ritz_form(int32_t n6, int3
 #pragma unroll 8
   for (int j = 0; j < m; ++j) acc += Zh[(size_t)j * n6 + e] * Ys[j];
   Rn[(size_t)c * n6 + e] = acc;
-}
-
-// SRn = S Rn, one block row per work-group; the row's column vertices are
-// staged in LDS per chunk, thread o < 6 r owns entry (component o % 6,
-// column o / 6) and sums the row's blocks in order.
-__global__ __launch_bounds__(128) void k_ritz_spmv(int32_t n6, int32_t r, const int32_t* __restrict__ row_ptr,
-                                                   const int32_t* __restrict__ col_var, const double* __restrict__ S,
-                                                   const double* __restrict__ Rn, double* __restrict__ SRn) {
-  __shared__ int32_t cj[kRowChunk];
-  const int i = blockIdx.x, o = threadIdx.x;
-  const int comp = o % 6, c = o / 6;
-  const int rb = row_ptr[i], re = row_ptr[i + 1];
-  const double* Rc = Rn + (size_t)min(c, kMaxRitz - 1) * n6;
-  double acc = 0.0;
-  for (int cb = rb; cb < re; cb += kRowChunk) {
-    const int nb = min(kRowChunk, re - cb);
-    __syncthreads();
-    for (int t = o; t < nb; t += blockDim.x) cj[t] = col_var[cb + t];
-    __syncthreads();
-    if (o < 6 * r) {
-      const double* sv = S + (size_t)36 * cb + 6 * comp;
-#pragma unroll 2
-      for (int t = 0; t < nb; ++t) {
-        const double* st = sv + 36 * t;
-        const double* rj = Rc + 6 * cj[t];
-        acc += ((st[0] * rj[0] + st[1] * rj[1]) + (st[2] * rj[2] + st[3] * rj[3])) + (st[4] * rj[4] + st[5] * rj[5]);
-      }
-    }
-  }
-  if (o < 6 * r) SRn[(size_t)c * n6 + 6 * i + comp] = acc;
 }
 
 // Gram matrices of Y = [R | Rn], one work-group per entry (a, b):
@@ -877,6 +868,7 @@ static PcgArgs make_args(const PcgBuffers& B) {
   a.pad2 = a.pad3 = 0;
   a.row_ptr = B.row_ptr; a.col_var = B.col_var; a.S = B.S; a.Minv = B.Minv; a.b = B.b;
   a.hat_node = B.hat_node; a.hat_w = B.hat_w; a.node_range = B.node_range; a.grp_lo = B.grp_lo; a.R = B.R;
+  a.blk_hat = B.blk_hat; a.blk_w = B.blk_w; a.blk_rng = B.blk_rng;
   a.SR = B.SR; a.E = B.E; a.Einv = B.Einv; a.EinvW = B.Einv + (size_t)a.ne * a.ne;
   a.x = B.x; a.r = B.r; a.z = B.z; a.p0 = B.p0; a.p1 = B.p1; a.q = B.q;
   a.part = B.part; a.part2 = B.part2; a.y = B.y; a.cden = B.cden; a.Zh = B.Zh; a.coef = B.coef;
@@ -901,7 +893,9 @@ int launch_pcg_deflated(hipStream_t s, const PcgBuffers& B, double tol, int32_t 
   const int groups = a.groups;
   if (a.ne > 0) {
     k_defl_sr<<<Pvar, 256, 0, s>>>(a);
-    k_defl_e<<<(a.ne * a.ne + 255) / 256, 256, 0, s>>>(a);
+    if (a.nr > 0)
+      k_row_dense<true><<<Pvar, kDenseThreads, 0, s>>>(a, a.row_ptr, a.col_var, a.S, a.R, a.nr, a.n6, nullptr);
+    k_defl_e<<<dim3(a.K, a.ne), 64, 0, s>>>(a);
     if (a.nr > 0) k_defl_dense<<<dim3(a.nr, a.nr + 1), 256, 0, s>>>(a);
   }
   k_defl_prep<<<1, kPrepThreads, 0, s>>>(a, tol, max_iters);
@@ -947,7 +941,7 @@ void launch_ritz_form(hipStream_t s, int32_t Pvar, int32_t m, int32_t r, const d
 void launch_ritz_spmv(hipStream_t s, int32_t Pvar, int32_t r, const int32_t* row_ptr, const int32_t* col_var,
                       const double* S, const double* Rn, double* SRn) {
   if (Pvar <= 0 || r <= 0 || r > kMaxRitz) return;
-  k_ritz_spmv<<<Pvar, 128, 0, s>>>(6 * Pvar, r, row_ptr, col_var, S, Rn, SRn);
+  k_row_dense<false><<<Pvar, kDenseThreads, 0, s>>>(PcgArgs{}, row_ptr, col_var, S, Rn, r, 6 * Pvar, SRn);
 }
 
 void launch_ritz_gram(hipStream_t s, const PcgBuffers& B, int32_t r, const double* Rn, const double* SRn,
