@@ -87,6 +87,9 @@ thread_local std::string g_create_error;
 // microseconds) and an iteration past convergence only two early-exit launches.
 constexpr int32_t kPcgPoll = 16;
 
+// Block rows per band of the Schur item dispatch order.
+constexpr int kItemBandRows = 4;
+
 // Pinned host buffer of the recycling: the solve's coefficients (2 x 64),
 // the tridiagonal's eigenvectors (64 x 12), the Gram matrices (2 x 24 x 24)
 // and the merge coefficients (24 x 12).
@@ -419,9 +422,19 @@ int build_topology(mlba_handle* h) {
     const int ip = schur_item_pairs();
     for (int u = 0; u < h->n_upper; ++u) {
       for (int pb = pair_ptr[u]; pb < pair_ptr[u + 1]; pb += ip)
-        items.push_back(make_int4(u, pb, std::min(pb + ip, pair_ptr[u + 1]), 0));
+        items.push_back(make_int4(u, pb, std::min(pb + ip, pair_ptr[u + 1]), (int32_t)items.size()));
       item_ptr.push_back((int32_t)items.size());
     }
+    // Dispatch order: bands of kItemBandRows block rows, each band swept by
+    // column j, so that the band's Q factors and the current column's W
+    // factors are reused from L2. Partial slots keep the block order.
+    std::stable_sort(items.begin(), items.end(), [&](const int4& a, const int4& b) {
+      const int4 ua = ublk[a.x], ub = ublk[b.x];
+      const int ba = ua.z / kItemBandRows, bb = ub.z / kItemBandRows;
+      if (ba != bb) return ba < bb;
+      if (ua.w != ub.w) return ua.w < ub.w;
+      return ua.z < ub.z;
+    });
   }
   h->n_items = (int32_t)items.size();
   std::vector<int32_t> obs_col(N, -1);

@@ -538,8 +538,11 @@ constexpr uint32_t kPairAnchor = 0x80000000u;  // the incidence is the anchor of
 // Four lanes share a pair, lane quadrant q owning rows 3 (q >> 1) .. +2 and
 // columns 3 (q & 1) .. +2 of the 6x6 block; the 16 lanes of a quadrant are
 // summed in a fixed butterfly order. Pair entries are incidence indices
-// (| kPairAnchor). Work-groups are mapped so that each XCD walks a
-// contiguous range of items (block rows), whose factors stay in its L2.
+// (| kPairAnchor). Items come in dispatch order (bands of a few block rows
+// swept together column by column, so that the band's Q factors and one
+// column's W factors are reused from L2) and carry the slot of their partial
+// sum; work-groups are mapped so that each XCD walks a contiguous range of
+// the dispatch order.
 __global__ __launch_bounds__(64 * kItemWaves) void k_schur_items(
     DevTopology t, int32_t n_items, const int4* __restrict__ items, const uint2* __restrict__ pairs,
     const double* __restrict__ inc_q, const double* __restrict__ inc_w, const double* __restrict__ lm_a,
@@ -550,7 +553,7 @@ __global__ __launch_bounds__(64 * kItemWaves) void k_schur_items(
   if (it >= n_items) return;
   const int lane = threadIdx.x & 63, slot = lane >> 2, quad = lane & 3;
   const int r0 = 3 * (quad >> 1), c0 = 3 * (quad & 1);
-  const int4 item = items[it];  // (u, first pair, end pair)
+  const int4 item = items[it];  // (u, first pair, end pair, partial slot)
   double acc[9];
   for (int k = 0; k < 9; ++k) acc[k] = 0.0;
   // Two pairs per lane and pass (slots p and p + 16), and the pair indices of
@@ -616,7 +619,7 @@ __global__ __launch_bounds__(64 * kItemWaves) void k_schur_items(
     acc[k] = v;
   }
   if (lane < 4) {
-    double* out = partial + (size_t)36 * it;
+    double* out = partial + (size_t)36 * item.w;
     for (int rr = 0; rr < 3; ++rr)
       for (int d = 0; d < 3; ++d) out[(r0 + rr) * 6 + c0 + d] = acc[rr * 3 + d];
   }
