@@ -15,6 +15,7 @@ sample of the same workload on this host.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -188,6 +189,15 @@ def main(argv=None):
     n_iters = sm.num_iterations - 1
     its_per_s = n_iters / wall
     n_pcg = sm.total_linear_solver_iterations
+    # The same LM iterations with plain block-Jacobi PCG (no coarse space, no
+    # recycled vectors), untimed: PCG iterations of the two preconditioners.
+    dev.upload_state(init_v, init_l)
+    jo = abi.default_options()
+    ctypes.pointer(jo)[0] = opts
+    jo.pcg_coarse_nodes, jo.pcg_ritz_vectors = -1, -1
+    jsm, jrows = dev.solve(jo)
+    n_pcg_jacobi = jsm.total_linear_solver_iterations
+    same_log = [r.step_is_successful for r in rows] == [r.step_is_successful for r in jrows]
     table = kernel_table(sm, kt, n_pcg)
     lin = table.get("k_linearize", {})
     obs_j_per_s = vmap.num_observations / (lin["avg_ms"] / 1e3) if lin else None
@@ -226,7 +236,12 @@ def main(argv=None):
         },
         "obs_jacobians_per_s": round(obs_j_per_s, 1) if obs_j_per_s else None,
         "pcg_iterations": int(n_pcg),
-        "pcg_preconditioner": "block-Jacobi + piecewise-linear coarse-space deflation",
+        "pcg_preconditioner": ("block-Jacobi + deflation by piecewise-linear hat functions and up to 12 Ritz "
+                               "vectors recycled from the previous solves"),
+        "pcg_iterations_vs_block_jacobi": {
+            "deflated": int(n_pcg), "block_jacobi": int(n_pcg_jacobi),
+            "ratio": round(n_pcg_jacobi / max(1, n_pcg), 2), "same_lm_log": bool(same_log),
+            "tolerance": opts.pcg_relative_tolerance},
         "lm_accepted": int(sm.num_successful_steps), "lm_rejected": int(sm.num_unsuccessful_steps),
         "initial_cost": sm.initial_cost, "final_cost": sm.final_cost,
         "kernels": table,
