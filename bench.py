@@ -126,6 +126,60 @@ def measured_traffic(kernel):
     return None
 
 
+def multimap(args):
+    """Aggregate LM iterations/s and observations/s of many independent maps,
+    solved one after the other and concurrently (one JSON line)."""
+    import numpy as np
+    from maplab_amd import abi, problem
+    from maplab_amd.solver import DeviceProblem
+    rng = np.random.default_rng(args.seed)
+    maps = []
+    for i in range(args.multimap):
+        kf = int(round(np.exp(rng.uniform(np.log(50), np.log(800)))))
+        lm = int(round(np.exp(rng.uniform(np.log(10000), np.log(150000)))))
+        kw = {} if i % 2 == 0 else dict(distortion_type=2, distortion=(-0.01, 0.02, -0.005, 0.001))
+        maps.append(problem.workload_map(2, seed=args.seed + 1 + i, num_keyframes=kf, num_landmarks=lm, **kw))
+    n_obs = sum(m.num_observations for m in maps)
+    print(f"bench: {len(maps)} maps, {sum(m.num_vertices for m in maps)} keyframes, {n_obs} observations",
+          file=sys.stderr, flush=True)
+    opts = abi.default_options()
+    opts.max_num_iterations = args.steps
+    devs = [DeviceProblem(m) for m in maps]
+    init = [(m.vertices.copy(), m.landmarks.copy()) for m in maps]
+    if args.warmup > 0:
+        w = abi.default_options()
+        w.max_num_iterations = args.warmup
+        DeviceProblem.solve_many(devs, w, threads=args.multimap_threads)
+    for d, (v, l) in zip(devs, init):
+        d.upload_state(v, l)
+    t0 = time.perf_counter()
+    seq = [d.solve(opts)[0] for d in devs]
+    t_seq = time.perf_counter() - t0
+    for d, (v, l) in zip(devs, init):
+        d.upload_state(v, l)
+    t0 = time.perf_counter()
+    con = [sm for sm, _ in DeviceProblem.solve_many(devs, opts, threads=args.multimap_threads)]
+    t_con = time.perf_counter() - t0
+    iters = sum(sm.num_iterations - 1 for sm in con)
+    same = all(a.final_cost == b.final_cost for a, b in zip(seq, con))
+    for d in devs:
+        d.close()
+    print(json.dumps({
+        "metric": "aggregate LM iterations/s over independent maps solved concurrently",
+        "value": round(iters / t_con, 2), "unit": "LM iterations/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "higher_is_better": True, "dtype": "f64", "data": "synthetic (seeded random-walk maps)",
+        "config": {"workload": f"{len(maps)} random-walk maps, 50-800 keyframes, 10k-150k landmarks "
+                               f"(log-uniform), {n_obs} observations, radtan and equidistant",
+                   "threads": args.multimap_threads},
+        "sequential": {"wall_s": round(t_seq, 4), "lm_iterations_per_s": round(iters / t_seq, 2)},
+        "concurrent": {"wall_s": round(t_con, 4), "lm_iterations_per_s": round(iters / t_con, 2),
+                       "observation_lm_iterations_per_s": round(
+                           sum(m.num_observations * (sm.num_iterations - 1) for m, sm in zip(maps, con)) / t_con,
+                           1)},
+        "speedup": round(t_seq / t_con, 3), "same_results": bool(same)}), flush=True)
+    return 0
+
+
 def cpu_baseline(vmap, opts, iters, threads):
     from oracle import binding as oracle
     import ctypes
@@ -152,6 +206,11 @@ def main(argv=None):
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, nproc)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--repeats", type=int, default=20, help="launches per kernel in mlba_time_kernels")
+    ap.add_argument("--multimap", type=int, default=0,
+                    help="instead of the headline line: this many independent random-walk maps (sizes "
+                         "log-uniform in 50-800 keyframes and 10k-150k landmarks) solved one after the other "
+                         "and concurrently (mlba_solve_many)")
+    ap.add_argument("--multimap-threads", type=int, default=8)
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -160,6 +219,9 @@ def main(argv=None):
 
     from maplab_amd import abi, problem
     from maplab_amd.solver import DeviceProblem
+
+    if args.multimap > 0:
+        return multimap(args)
 
     kw = dict(num_keyframes=args.keyframes, num_landmarks=args.landmarks)
     if args.equidistant:

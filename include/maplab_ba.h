@@ -223,7 +223,10 @@ int32_t mlba_destroy(mlba_handle* handle);
 const char* mlba_last_error(const mlba_handle* handle);
 
 /* Replace / read back the optimized state (copyAllStatesBackToMap,
- * optimization-state-buffer.cc:42-151). Either pointer may be NULL. */
+ * optimization-state-buffer.cc:42-151). Either pointer may be NULL. An
+ * upload also clears what the linear solver carries from solve to solve
+ * (recycled deflation vectors), so a solve from an uploaded state repeats
+ * bit for bit. */
 int32_t mlba_upload_state(mlba_handle* handle, const double* vertex_q_IM__M_p_MI,
                           const double* landmark_p_B);
 int32_t mlba_download_state(const mlba_handle* handle, double* vertex_q_IM__M_p_MI,
@@ -252,6 +255,20 @@ int32_t mlba_evaluate(mlba_handle* handle, int32_t flags, const mlba_options* op
 
 /* ceres::Solve on the visual problem (map_optimization::solve, solver.cc:14-44). */
 int32_t mlba_solve(mlba_handle* handle, const mlba_options* options, mlba_summary* summary);
+
+/* Several independent problems solved concurrently: handle i runs
+ * mlba_solve (or, with outlier_rejection, mlba_solve_with_outlier_rejection)
+ * with options[i] and summaries[i] (nullable), each on its own stream; up to
+ * `threads` host threads drive the handles (0: min(n, 8)), so the device
+ * overlaps one problem's host decisions and small launches with another's
+ * kernels. statuses[i] (nullable array) receives each status; the return
+ * value is the first non-OK status in handle order (every handle is still
+ * solved). The handles must be distinct. No counterpart in the reference
+ * (maplab solves one map per VIMapOptimizer::optimize call); for callers
+ * that optimize many maps, e.g. a batch of missions or sessions. */
+int32_t mlba_solve_many(mlba_handle* const* handles, const mlba_options* const* options,
+                        mlba_summary* const* summaries, int32_t* statuses, int32_t n, int32_t threads,
+                        int32_t outlier_rejection);
 
 /* map_optimization::solveWithOutlierRejection (outlier-rejection-solver.cc:206-259). */
 int32_t mlba_solve_with_outlier_rejection(mlba_handle* handle, const mlba_options* options,

@@ -175,6 +175,9 @@ LIB_SYMBOLS = {
     "mlba_evaluate": (c_i32, [c_vp, c_i32, ctypes.POINTER(Options), c_vp, c_vp, c_vp]),
     "mlba_solve": (c_i32, [c_vp, ctypes.POINTER(Options), ctypes.POINTER(Summary)]),
     "mlba_solve_with_outlier_rejection": (c_i32, [c_vp, ctypes.POINTER(Options), ctypes.POINTER(Summary)]),
+    "mlba_solve_many": (c_i32, [ctypes.POINTER(c_vp), ctypes.POINTER(ctypes.POINTER(Options)),
+                                ctypes.POINTER(ctypes.POINTER(Summary)), ctypes.POINTER(c_i32), c_i32, c_i32,
+                                c_i32]),
     "mlba_time_kernels": (c_i32, [c_vp, ctypes.POINTER(Options), c_i32, ctypes.POINTER(KernelTimes)]),
     "mlba_quality_options_init": (None, [ctypes.POINTER(QualityOptions)]),
     "mlba_landmark_quality": (c_i32, [c_vp, ctypes.POINTER(QualityOptions), c_vp]),

@@ -17,7 +17,9 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <atomic>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/maplab_ba.h"
@@ -1229,6 +1231,12 @@ int32_t mlba_upload_state(mlba_handle* h, const double* vertices, const double* 
   if (vertices) HIP_OK(hipMemcpyAsync(h->vert.p, vertices, 7 * sizeof(double) * h->P, hipMemcpyHostToDevice, h->stream));
   if (landmarks) HIP_OK(hipMemcpyAsync(h->lm.p, landmarks, 3 * sizeof(double) * h->L, hipMemcpyHostToDevice, h->stream));
   HIP_OK(hipStreamSynchronize(h->stream));
+  // A new state starts the linear solver afresh: no recycled deflation
+  // columns or iteration estimate carried over from earlier solves, so that
+  // a solve from an uploaded state repeats bit for bit.
+  h->ritz_nr = 0;
+  h->ritz_stage = 0;
+  h->pcg_last_iters = 0;
   return MLBA_OK;
 }
 
@@ -1416,6 +1424,38 @@ static int reject_outliers(mlba_handle* h, const mlba_options* o, int* removed) 
   *removed = n;
   if (n) return build_topology(h);
   return MLBA_OK;
+}
+
+int32_t mlba_solve_many(mlba_handle* const* handles, const mlba_options* const* options,
+                        mlba_summary* const* summaries, int32_t* statuses, int32_t n, int32_t threads,
+                        int32_t outlier_rejection) {
+  if (n < 0 || (n > 0 && (!handles || !options))) return MLBA_EINVAL;
+  for (int i = 0; i < n; ++i) {
+    if (!handles[i] || !options[i]) return MLBA_EINVAL;
+    for (int j = 0; j < i; ++j)
+      if (handles[j] == handles[i]) return MLBA_EINVAL;
+  }
+  std::vector<int32_t> st(n, MLBA_OK);
+  std::atomic<int> next{0};
+  auto worker = [&]() {
+    for (int i = next.fetch_add(1); i < n; i = next.fetch_add(1)) {
+      mlba_summary* sm = summaries ? summaries[i] : nullptr;
+      st[i] = outlier_rejection ? mlba_solve_with_outlier_rejection(handles[i], options[i], sm)
+                                : mlba_solve(handles[i], options[i], sm);
+    }
+  };
+  const int nt = std::max(1, std::min(n, threads > 0 ? threads : 8));
+  std::vector<std::thread> pool;
+  pool.reserve(nt - 1);
+  for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+  worker();
+  for (std::thread& t : pool) t.join();
+  int32_t rc = MLBA_OK;
+  for (int i = 0; i < n; ++i) {
+    if (statuses) statuses[i] = st[i];
+    if (rc == MLBA_OK && st[i] != MLBA_OK) rc = st[i];
+  }
+  return rc;
 }
 
 int32_t mlba_solve_with_outlier_rejection(mlba_handle* h, const mlba_options* o, mlba_summary* sm) {

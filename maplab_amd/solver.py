@@ -100,6 +100,31 @@ class DeviceProblem:
         rows = [its[i] for i in range(min(sm.num_iterations, capacity))]
         return sm, rows
 
+    @staticmethod
+    def solve_many(problems, options, threads: int = 0, outlier_rejection: bool = False, capacity: int = 256):
+        """mlba_solve_many: the problems (DeviceProblem, distinct) solved
+        concurrently, problem i with options[i] (or one Options for all).
+        Returns [(summary, rows)] in problem order; raises on a failed solve."""
+        n = len(problems)
+        if isinstance(options, abi.Options):
+            options = [options] * n
+        lib = abi.load_library()
+        its = [(abi.IterationSummary * capacity)() for _ in range(n)]
+        sms = [abi.Summary() for _ in range(n)]
+        for sm, it in zip(sms, its):
+            sm.iterations_capacity = capacity
+            sm.iterations = ctypes.cast(it, ctypes.POINTER(abi.IterationSummary))
+        handles = (ctypes.c_void_p * n)(*[p.handle for p in problems])
+        opts = (ctypes.POINTER(abi.Options) * n)(*[ctypes.pointer(o) for o in options])
+        sums = (ctypes.POINTER(abi.Summary) * n)(*[ctypes.pointer(sm) for sm in sms])
+        statuses = (ctypes.c_int32 * n)()
+        rc = lib.mlba_solve_many(handles, opts, sums, statuses, n, threads, 1 if outlier_rejection else 0)
+        for p, st in zip(problems, statuses):
+            p._check(st)
+        if rc:
+            raise RuntimeError(f"mlba_solve_many failed: {rc}")
+        return [(sm, [it[i] for i in range(min(sm.num_iterations, capacity))]) for sm, it in zip(sms, its)]
+
     def time_kernels(self, options: abi.Options, repeats: int = 10) -> abi.KernelTimes:
         """Average device time of each kernel of one LM iteration (HIP events
         on the library's stream)."""

@@ -169,3 +169,14 @@ def test_inertial_rejects_invalid_edges(kw, text):
     from maplab_amd import inertial
     with pytest.raises(RuntimeError, match=text):
         inertial.evaluate(_edges(**kw))
+
+
+def test_solve_many_rejects_bad_arguments_without_touching_the_device():
+    lib = abi.load_library()
+    assert lib.mlba_solve_many(None, None, None, None, 0, 0, 0) == abi.MLBA_OK
+    assert lib.mlba_solve_many(None, None, None, None, -1, 0, 0) == abi.MLBA_EINVAL
+    assert lib.mlba_solve_many(None, None, None, None, 2, 0, 0) == abi.MLBA_EINVAL
+    handles = (ctypes.c_void_p * 2)(None, None)
+    o = abi.default_options()
+    opts = (ctypes.POINTER(abi.Options) * 2)(ctypes.pointer(o), ctypes.pointer(o))
+    assert lib.mlba_solve_many(handles, opts, None, None, 2, 0, 0) == abi.MLBA_EINVAL
