@@ -84,10 +84,11 @@ namespace {
 thread_local std::string g_create_error;
 
 // PCG iterations between host convergence polls. A solve's first poll comes
-// after the previous solve's iteration count plus half of this: successive
-// LM solves take similar counts, a poll costs a host round trip (tens of
-// microseconds) and an iteration past convergence only two early-exit launches.
-constexpr int32_t kPcgPoll = 16;
+// after the last similar solve's iteration count (with or without recycled
+// columns), then every kPcgPollAfter iterations: successive LM solves take
+// similar counts, a poll costs a host round trip (≈20 µs) and an iteration
+// past convergence two early-exit launches (≈10 µs).
+constexpr int32_t kPcgPoll = 16, kPcgPollAfter = 4;
 
 // Block rows per band of the Schur item dispatch order.
 constexpr int kItemBandRows = 4;
@@ -169,7 +170,9 @@ struct mlba_handle {
   int32_t grp_width = 0;                // hat columns of the per-group SW window
   DBuf<double> hat_w, blk_w, defl_SR, defl_E, defl_Einv, defl_y, defl_cden, pcg_part2;
   int32_t hat_K = -1;  // coarse nodes of the current hat basis
-  int32_t pcg_last_iters = 0;  // iterations of the previous solve: length of the next solve's first batch
+  // Iterations of the last solve without and with recycled columns: the
+  // length of the next such solve's first batch.
+  int32_t pcg_last_iters[2] = {0, 0};
   // Recycled deflation columns: ritz[ritz_cur] holds ritz_nr of them; the
   // other buffer receives the merged set. ritz_Zh / ritz_coef keep the
   // Lanczos vectors and coefficients of the last solve.
@@ -346,7 +349,7 @@ int build_topology(mlba_handle* h) {
   }
   h->nnzb = (int32_t)col_var.size();
   h->hat_K = -1;
-  h->pcg_last_iters = 0;
+  h->pcg_last_iters[0] = h->pcg_last_iters[1] = 0;
   h->ritz_nr = 0;
   h->ritz_stage = 0;
   h->h_row_ptr = row_ptr;
@@ -899,12 +902,13 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
         KernelTimer kt(h, MLBA_KERNEL_PCG);
         const int32_t nritz = ritz_count(o);
         if ((rc = ritz_merge(h))) return rc;
-        pcg_iters = launch_pcg_deflated(s, pcg_buffers(h, nritz > 0 ? h->ritz_nr : 0, nritz > 0),
-                                        o->pcg_relative_tolerance, o->pcg_max_iterations,
-                                        h->pcg_last_iters > 0 ? h->pcg_last_iters + kPcgPoll / 2 : kPcgPoll,
-                                        kPcgPoll, nullptr, 0);
+        const int32_t nr_used = nritz > 0 ? h->ritz_nr : 0;
+        int32_t& est = h->pcg_last_iters[nr_used > 0 ? 1 : 0];
+        pcg_iters = launch_pcg_deflated(s, pcg_buffers(h, nr_used, nritz > 0), o->pcg_relative_tolerance,
+                                        o->pcg_max_iterations, est > 0 ? est : kPcgPoll,
+                                        est > 0 ? kPcgPollAfter : kPcgPoll, nullptr, 0);
         if (pcg_iters < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
-        h->pcg_last_iters = pcg_iters;
+        est = pcg_iters;
         ritz_after_solve(h, nritz, pcg_iters);
       }
       launch_unscale_poses(s, h->Pvar, h->px.p, h->pose_scale.p, h->w.p, h->parts_w.p);
@@ -1236,7 +1240,7 @@ int32_t mlba_upload_state(mlba_handle* h, const double* vertices, const double* 
   // a solve from an uploaded state repeats bit for bit.
   h->ritz_nr = 0;
   h->ritz_stage = 0;
-  h->pcg_last_iters = 0;
+  h->pcg_last_iters[0] = h->pcg_last_iters[1] = 0;
   return MLBA_OK;
 }
 
