@@ -1,0 +1,144 @@
+"""maplab VI-map folders (protobuf + sensors.yaml) <-> the library's SoA
+problem (maplab_amd/vimap.py).
+
+No maplab map file exists in the reference tree, so the format is pinned by
+its schema (field numbers of vi_map.proto / visual-frame.proto / id.proto,
+file names of vi-map-serialization.h) and by round trips: a map written
+from SoA arrays reads back to the same problem (same blocks, same
+observations up to order, same cost under the oracle), optimized states
+written back read back exactly, undeclared protobuf fields survive, and the
+frame conventions (Hamilton T_M_I / T_G_M / T_B_C vs the JPL state blocks)
+agree with closed forms. The reference's own loader is parity unpinned.
+"""
+import gzip
+import os
+
+import numpy as np
+import pytest
+
+from maplab_amd import abi, problem, vimap
+from oracle import binding as oracle
+
+
+def sorted_obs(m, lm_map=None):
+    ol = m.obs_landmark if lm_map is None else lm_map[m.obs_landmark]
+    key = np.lexsort((m.obs_camera, ol, m.obs_vertex))
+    return (m.obs_vertex[key], ol[key], m.obs_camera[key], m.obs_keypoint[key], m.obs_sigma[key])
+
+
+def landmark_permutation(read, orig):
+    """Index into `orig` of each landmark of `read` (landmarks come back grouped
+    by store vertex; positions identify them)."""
+    where = {tuple(p): i for i, p in enumerate(orig.landmarks)}
+    return np.array([where[tuple(p)] for p in read.landmarks])
+
+
+@pytest.fixture(scope="module", params=[False, True], ids=["radtan", "equidistant"])
+def written(request, tmp_path_factory):
+    kw = dict(distortion_type=2, distortion=(-0.01, 0.02, -0.005, 0.001)) if request.param else {}
+    m = problem.workload_map(2, seed=3, num_keyframes=30, num_landmarks=3000, **kw)
+    folder = str(tmp_path_factory.mktemp("map"))
+    vimap.write_vimap(m, folder)
+    return m, folder
+
+
+def test_written_map_reads_back_to_the_same_problem(written):
+    m, folder = written
+    r, _ = vimap.load_vimap(folder)
+    np.testing.assert_array_equal(r.vertices, np.where(m.vertices[:, 3:4] < 0, -1, 1) * m.vertices)
+    np.testing.assert_allclose(r.baseframes, m.baseframes, atol=1e-15)
+    perm = landmark_permutation(r, m)
+    assert sorted(perm) == list(range(m.num_landmarks))
+    np.testing.assert_array_equal(r.landmarks, m.landmarks[perm])
+    np.testing.assert_array_equal(r.landmark_store_vertex, m.landmark_store_vertex[perm])
+    np.testing.assert_array_equal(r.vertex_mission, m.vertex_mission)
+    np.testing.assert_allclose(r.camera_q_CI__C_p_CI, m.camera_q_CI__C_p_CI, atol=1e-14)
+    np.testing.assert_array_equal(r.camera_intrinsics, m.camera_intrinsics)
+    np.testing.assert_array_equal(r.camera_distortion_type, m.camera_distortion_type)
+    np.testing.assert_array_equal(r.camera_distortion, m.camera_distortion)
+    np.testing.assert_array_equal(r.camera_resolution, m.camera_resolution)
+    for a, b in zip(sorted_obs(r, perm), sorted_obs(m)):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_read_map_has_the_same_cost_under_the_oracle(written):
+    m, folder = written
+    r, _ = vimap.load_vimap(folder)
+    o = abi.default_options()
+    _, _, c0 = oracle.problem_evaluate(m, options=o)
+    _, _, c1 = oracle.problem_evaluate(r, options=o)
+    assert abs(c1 - c0) <= 1e-12 * c0
+
+
+def test_optimized_state_written_back_reads_back(written, tmp_path):
+    m, folder = written
+    r, rec = vimap.load_vimap(folder)
+    rng = np.random.default_rng(0)
+    v = r.vertices.copy()
+    v[:, 4:] += rng.normal(0, 0.01, (v.shape[0], 3))
+    l = r.landmarks + rng.normal(0, 0.01, r.landmarks.shape)
+    q = np.full(r.num_landmarks, 2, np.uint8)
+    q[:5] = 1
+    out = vimap.save_vimap(rec, v, l, out_folder=str(tmp_path / "out"), landmark_quality=q)
+    r2, rec2 = vimap.load_vimap(out)
+    np.testing.assert_array_equal(r2.vertices, v)
+    np.testing.assert_array_equal(r2.landmarks, l)
+    assert [lm.quality for lm in rec2.landmark_protos[:6]] == [1, 1, 1, 1, 1, 2]
+    for a, b in zip(sorted_obs(r2), sorted_obs(r)):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_files_are_gzip_and_unknown_fields_survive(written, tmp_path):
+    m, folder = written
+    base = os.path.join(folder, vimap.FOLDER)
+    with open(os.path.join(base, "vertices0"), "rb") as f:
+        assert f.read(2) == b"\x1f\x8b"  # proto_use_compression = true
+    M = vimap.messages()
+    # Add a field the schema here does not declare (ViwlsVertex.resource_map = 9).
+    r, rec = vimap.load_vimap(folder)
+    extra = b"\x4a\x03\x0a\x01\x2a"  # field 9, length 3: a nested message with field 1 = b"*"
+    for ft, path, proto, zipped in rec.files:
+        if ft == vimap.FILE_VERTICES:
+            v = proto.vertices[0]
+            v.MergeFromString(extra)
+    out = vimap.save_vimap(rec, r.vertices, r.landmarks, out_folder=str(tmp_path / "out"))
+    with open(os.path.join(out, vimap.FOLDER, "vertices0"), "rb") as f:
+        data = gzip.decompress(f.read())
+    vm = M["VIMap"]()
+    vm.ParseFromString(data)
+    assert extra in vm.vertices[0].SerializeToString()
+
+
+def test_ids_and_hex_strings_follow_hash_id():
+    key = (0x0123456789ABCDEF, 0x1122334455667788)
+    # HashId::hexString prints the 16 bytes in memory order (little-endian words).
+    assert vimap.id_hex(key) == "efcdab89674523018877665544332211"
+    assert vimap.hex_key(vimap.id_hex(key)) == key
+
+
+def test_frame_conventions_match_the_closed_forms():
+    # A camera with T_B_C = (R_B_C, p_B_C): the JPL state block q_CI has
+    # R_JPL(q_CI) = R_C_I = R_B_C' and C_p_CI = -R_B_C' p_B_C.
+    ang = 0.3
+    R_B_C = np.array([[np.cos(ang), -np.sin(ang), 0], [np.sin(ang), np.cos(ang), 0], [0, 0, 1.0]])
+    p_B_C = np.array([0.1, -0.2, 0.05])
+    q = vimap._matrix_to_quat(R_B_C)
+    np.testing.assert_allclose(oracle.rotation_jpl(q), R_B_C.T, atol=1e-15)
+    np.testing.assert_allclose(vimap._quat_to_matrix(q), R_B_C, atol=1e-15)
+    np.testing.assert_allclose(-R_B_C.T @ p_B_C, [np.cos(ang) * -0.1 + np.sin(ang) * 0.2,
+                                                  np.sin(ang) * 0.1 + np.cos(ang) * 0.2, -0.05], atol=1e-15)
+
+
+def test_solving_the_read_map_matches_the_original(written):
+    # Same problem up to landmark and observation order: the oracle's LM log agrees.
+    m, folder = written
+    r, _ = vimap.load_vimap(folder)
+    perm = landmark_permutation(r, m)
+    r.landmark_constant = m.landmark_constant[perm].copy()
+    r.vertex_constant = m.vertex_constant.copy()
+    o = abi.default_options()
+    o.max_num_iterations = 4
+    a, arows, _, _, _ = oracle.solve(m, o, threads=4)
+    b, brows, _, _, _ = oracle.solve(r, o, threads=4)
+    assert [x.step_is_successful for x in arows] == [x.step_is_successful for x in brows]
+    np.testing.assert_allclose([x.cost for x in brows], [x.cost for x in arows], rtol=1e-9)
