@@ -7,9 +7,9 @@
  *   - VisualReprojectionError::Evaluate for every keypoint observation
  *     (algorithms/ceres-error-terms/include/ceres-error-terms/
  *      visual-error-term-inl.h:13-379), pinhole camera with no / radtan /
- *     equidistant distortion (aslam_cv2/aslam_cv_cameras/src/
+ *     equidistant / fisheye distortion (aslam_cv2/aslam_cv_cameras/src/
  *     camera-pinhole.cc:65-164, distortion-radtan.cc:14-91,
- *     distortion-equidistant.cc:14-142);
+ *     distortion-equidistant.cc:14-142, distortion-fisheye.cc:14-120);
  *   - the Levenberg-Marquardt trust-region loop that ceres::Solve runs with
  *     the options of algorithms/map-optimization/include/map-optimization/
  *     solver-options.h:26-74, with landmark (Schur) elimination and the
@@ -54,6 +54,9 @@ extern "C" {
 #define MLBA_DISTORTION_NONE 0
 #define MLBA_DISTORTION_RADTAN 1
 #define MLBA_DISTORTION_EQUIDISTANT 2
+#define MLBA_DISTORTION_FISHEYE 3  /* FOV model, one parameter w (distortion[0]), |w| < 1e-16 or
+                                      0.5 <= w <= 1.5 (distortion-fisheye.cc:145-156); its
+                                      distortion Jacobian block has w in column 0, zeros after */
 
 /* ceres_error_terms::LandmarkErrorType (common.h:14). */
 #define MLBA_LOCAL_KEYFRAME 0

@@ -24,7 +24,7 @@ MLBA_ENOMEM = -2
 MLBA_EDEVICE = -3
 MLBA_ESTATE = -4
 
-DIST_NONE, DIST_RADTAN, DIST_EQUIDISTANT = 0, 1, 2
+DIST_NONE, DIST_RADTAN, DIST_EQUIDISTANT, DIST_FISHEYE = 0, 1, 2, 3
 LOCAL_KEYFRAME, LOCAL_MISSION, GLOBAL = 0, 1, 2
 LOSS_MAPLAB_HUBER, LOSS_CAUCHY, LOSS_TRIVIAL = 0, 1, 2
 SOLVER_SCHUR_PCG = 0

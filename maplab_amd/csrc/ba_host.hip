@@ -1122,7 +1122,13 @@ int32_t mlba_create(const mlba_problem_desc* d, mlba_handle** out) {
   }
   for (int c = 0; c < C; ++c) {
     const int dt = d->camera_distortion_type[c];
-    if (dt < 0 || dt > 2) return set_err(h, MLBA_EINVAL, "unknown distortion type");
+    if (dt < 0 || dt > 3) return set_err(h, MLBA_EINVAL, "unknown distortion type");
+    if (dt == kDistFisheye) {
+      // FisheyeDistortion::areParametersValid (distortion-fisheye.cc:145-156).
+      const double w = d->camera_distortion[4 * c];
+      if (!(std::fabs(w) < 1e-16 || (w >= 0.5 && w <= 1.5)))
+        return set_err(h, MLBA_EINVAL, "fisheye distortion w outside [0.5, 1.5]");
+    }
     if (!valid_quaternion(d->camera_q_CI__C_p_CI + 7 * c)) return set_err(h, MLBA_ESTATE, "camera quaternion invalid");
   }
   for (int m = 0; m < M; ++m)

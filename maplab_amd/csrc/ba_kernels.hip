@@ -1329,20 +1329,8 @@ __global__ __launch_bounds__(256) void k_evaluate_full(DevTopology t, const doub
         // d kp / d distortion at the undistorted normalized point, rows x f.
         double* J = rec + offs[8];
         const double x = pC[0] / pC[2], y = pC[1] / pC[2];
-        double d[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        if (cam.dist_type == kDistRadtan) {
-          const double r2 = x * x + y * y, r4 = r2 * r2;
-          d[0] = x * r2; d[1] = x * r4; d[2] = 2.0 * x * y; d[3] = r2 + 2.0 * x * x;
-          d[4] = y * r2; d[5] = y * r4; d[6] = r2 + 2.0 * y * y; d[7] = 2.0 * x * y;
-        } else if (cam.dist_type == kDistEquidistant) {
-          const double r = sqrt(x * x + y * y);
-          if (r >= 1e-10) {
-            const double th = atan(r);
-            const double t3 = th * th * th, t5 = t3 * th * th, t7 = t5 * th * th, t9 = t7 * th * th;
-            d[0] = x * t3 / r; d[1] = x * t5 / r; d[2] = x * t7 / r; d[3] = x * t9 / r;
-            d[4] = y * t3 / r; d[5] = y * t5 / r; d[6] = y * t7 / r; d[7] = y * t9 / r;
-          }
-        }
+        double d[8];
+        distortion_parameter_jacobian(cam.dist_type, cam.dist, x, y, d);
         for (int k = 0; k < 4; ++k) {
           J[k] = d[k] * cam.intr[0] * sinv;
           J[4 + k] = d[4 + k] * cam.intr[1] * sinv;

@@ -8,7 +8,7 @@
 //  * JPL Plus (algorithms/ceres-error-terms/src/parameterization/
 //    quaternion-param-jpl.cc:35-68) left-multiplies the small rotation, so
 //    its tangent derivative of R(q) p is [R(q) p]x (see below);
-//  * pinhole projection with radtan / equidistant distortion and the
+//  * pinhole projection with radtan / equidistant / fisheye (FOV) distortion and the
 //    projection status of aslam_cv2/aslam_cv_cameras/src/camera-pinhole.cc:65-164.
 //
 // Unlike the reference, Jacobians are produced directly in the tangent space
@@ -30,7 +30,7 @@
 
 namespace mlba {
 
-enum : int32_t { kDistNone = 0, kDistRadtan = 1, kDistEquidistant = 2 };
+enum : int32_t { kDistNone = 0, kDistRadtan = 1, kDistEquidistant = 2, kDistFisheye = 3 };
 enum : int32_t { kTypeKeyframe = 0, kTypeMission = 1, kTypeGlobal = 2 };
 enum : int32_t { kVisible = 0, kOutsideImage = 1, kBehind = 2, kInvalid = 3 };
 
@@ -126,8 +126,74 @@ MLBA_HD void distort(int32_t type, const double* k, double& x, double& y, double
     const double scale = (r > 1e-8) ? s : 1.0;
     x *= scale;
     y *= scale;
+  } else if (type == kDistFisheye) {
+    // FOV model (distortion-fisheye.cc:14-82): the point is scaled by
+    // f(r) = atan(2 tan(w / 2) r) / (w r), with the reference's two limits:
+    // w^2 < 1e-5 -> f = 1, J = I; r^2 < 1e-5 -> f = 2 tan(w / 2) / w, J = f I.
+    const double w = k[0];
+    const double r2 = x * x + y * y;
+    double f;
+    if (w * w < 1e-5) {
+      f = 1.0;
+      jd[0] = 1.0; jd[1] = 0.0; jd[2] = 0.0; jd[3] = 1.0;
+    } else {
+      const double t = tan(0.5 * w);
+      if (r2 < 1e-5) {
+        f = 2.0 * t / w;
+        jd[0] = f; jd[1] = 0.0; jd[2] = 0.0; jd[3] = f;
+      } else {
+        const double r = sqrt(r2);
+        const double at = atan(2.0 * t * r);
+        f = at / (w * r);
+        // J = f I + (f'(r) / r) [x y]' [x y].
+        const double c = 2.0 * t / (w * r2 * (4.0 * t * t * r2 + 1.0)) - at / (w * r2 * r);
+        jd[0] = f + c * x * x;
+        jd[1] = c * x * y;
+        jd[2] = jd[1];
+        jd[3] = f + c * y * y;
+      }
+    }
+    x *= f;
+    y *= f;
   } else {
     jd[0] = 1.0; jd[1] = 0.0; jd[2] = 0.0; jd[3] = 1.0;
+  }
+}
+
+// d(distorted point) / d(distortion parameters) at the normalized point
+// (x, y), 2 x 4 row-major, the parameters a model does not have left zero:
+// distortParameterJacobian of distortion-radtan.cc:67-91,
+// distortion-equidistant.cc:109-142 and distortion-fisheye.cc:84-120. For
+// the fisheye limit r^2 < 1e-5 the reference fills both rows with df/dw
+// itself (not x df/dw, y df/dw); that is kept.
+MLBA_HD void distortion_parameter_jacobian(int32_t type, const double* k, double x, double y, double* d) {
+  for (int i = 0; i < 8; ++i) d[i] = 0.0;
+  if (type == kDistRadtan) {
+    const double r2 = x * x + y * y, r4 = r2 * r2;
+    d[0] = x * r2; d[1] = x * r4; d[2] = 2.0 * x * y; d[3] = r2 + 2.0 * x * x;
+    d[4] = y * r2; d[5] = y * r4; d[6] = r2 + 2.0 * y * y; d[7] = 2.0 * x * y;
+  } else if (type == kDistEquidistant) {
+    const double r = sqrt(x * x + y * y);
+    if (r >= 1e-10) {
+      const double th = atan(r);
+      const double t3 = th * th * th, t5 = t3 * th * th, t7 = t5 * th * th, t9 = t7 * th * th;
+      d[0] = x * t3 / r; d[1] = x * t5 / r; d[2] = x * t7 / r; d[3] = x * t9 / r;
+      d[4] = y * t3 / r; d[5] = y * t5 / r; d[6] = y * t7 / r; d[7] = y * t9 / r;
+    }
+  } else if (type == kDistFisheye) {
+    const double w = k[0];
+    if (w * w < 1e-5) return;
+    const double r2 = x * x + y * y;
+    if (r2 < 1e-5) {
+      const double cw = cos(0.5 * w);
+      d[0] = d[4] = (w - sin(w)) / (w * w * cw * cw);
+    } else {
+      // df/dw = (1 + t^2) / (w (1 + 4 t^2 r^2)) - atan(2 t r) / (w^2 r).
+      const double t = tan(0.5 * w), r = sqrt(r2);
+      const double g = (1.0 + t * t) / (w * (4.0 * t * t * r2 + 1.0)) - atan(2.0 * t * r) / (w * w * r);
+      d[0] = x * g;
+      d[4] = y * g;
+    }
   }
 }
 

@@ -192,7 +192,8 @@ def _matrix_yaml(M) -> dict:
     return {"rows": int(M.shape[0]), "cols": int(M.shape[1]), "data": [float(v) for v in M.reshape(-1)]}
 
 
-_DIST_NAMES = {"none": abi.DIST_NONE, "radial-tangential": abi.DIST_RADTAN, "equidistant": abi.DIST_EQUIDISTANT}
+_DIST_NAMES = {"none": abi.DIST_NONE, "radial-tangential": abi.DIST_RADTAN, "equidistant": abi.DIST_EQUIDISTANT,
+               "fisheye": abi.DIST_FISHEYE}  # camera.cc:128-160, 281-287
 
 
 # ------------------------------------------------------------------ loading
@@ -509,7 +510,8 @@ def write_vimap(vmap: VisualMap, folder: str, compress: bool = True, seed: int =
         dt = int(vmap.camera_distortion_type[c])
         cam["distortion"] = {"type": names[dt]}
         if dt != abi.DIST_NONE:
-            cam["distortion"]["parameters"] = _matrix_yaml(vmap.camera_distortion[c].reshape(4, 1))
+            n = 1 if dt == abi.DIST_FISHEYE else 4
+            cam["distortion"]["parameters"] = _matrix_yaml(vmap.camera_distortion[c, :n].reshape(n, 1))
         cams.append({"camera": cam, "T_B_C": _matrix_yaml(T)})
     sensors = {"sensors": [{"id": id_hex(ncam_id), "sensor_type": "NCAMERA", "cameras": cams}], "extrinsics": []}
     with open(os.path.join(base, SENSORS), "w") as f:

@@ -12,6 +12,7 @@
 //       include/aslam/cameras/camera-pinhole-inl.h:48-65, camera-inl.h:17-25
 //   RadTanDistortion  aslam_cv2/aslam_cv_cameras/src/distortion-radtan.cc:14-91
 //   EquidistantDistortion aslam_cv2/aslam_cv_cameras/src/distortion-equidistant.cc:14-142
+//   FisheyeDistortion aslam_cv2/aslam_cv_cameras/src/distortion-fisheye.cc:14-120
 //   JPL quaternion math common/maplab-common/include/maplab-common/quaternion-math-inl.h:7-27,113-154
 //   JplQuaternionParameterization
 //       algorithms/ceres-error-terms/src/parameterization/quaternion-param-jpl.cc:8-76
@@ -201,11 +202,64 @@ void equidistantParamJac(const double* pt, double* J) {
   J[4] = y * t3 / r; J[5] = y * t5 / r; J[6] = y * t7 / r; J[7] = y * t9 / r;
 }
 
+// distortion-fisheye.cc:14-82 (FOV model, one parameter w). The 2x2
+// Jacobian is written entry by entry as the reference does.
+void distortFisheye(const double* k, double* pt, double* J) {
+  const double w = k[0];
+  const double u = pt[0], v = pt[1];
+  const double r_u = std::sqrt(u * u + v * v);
+  const double r_u3 = r_u * r_u * r_u;
+  const double tw = std::tan(w / 2.0);
+  const double tw2 = tw * tw;
+  const double at = std::atan(2.0 * tw * r_u);
+  double r_rd;
+  if (w * w < 1e-5) r_rd = 1.0;
+  else if (r_u * r_u < 1e-5) r_rd = 2.0 * tw / w;
+  else r_rd = at / (r_u * w);
+  if (J) {
+    if (w * w < 1e-5) {
+      J[0] = 1.0; J[1] = 0.0; J[2] = 0.0; J[3] = 1.0;
+    } else if (r_u * r_u < 1e-5) {
+      const double f = 2.0 * tw / w;
+      J[0] = f; J[1] = 0.0; J[2] = 0.0; J[3] = f;
+    } else {
+      const double n2 = u * u + v * v;
+      const double den = w * n2 * (4.0 * tw2 * n2 + 1.0);
+      J[0] = at / (w * r_u) - (u * u * at) / (w * r_u3) + (2.0 * u * u * tw) / den;
+      J[1] = (2.0 * u * v * tw) / den - (u * v * at) / (w * r_u3);
+      J[2] = J[1];
+      J[3] = at / (w * r_u) - (v * v * at) / (w * r_u3) + (2.0 * v * v * tw) / den;
+    }
+  }
+  pt[0] = u * r_rd;
+  pt[1] = v * r_rd;
+}
+// distortion-fisheye.cc:84-120, one column (w) written into a 2x4 block.
+// In the limit r_u^2 < 1e-5 the reference sets both rows to the constant
+// (w - sin w) / (w^2 cos^2(w / 2)); that is kept.
+void fisheyeParamJac(const double* k, const double* pt, double* J) {
+  for (int i = 0; i < 8; ++i) J[i] = 0.0;
+  const double w = k[0];
+  const double tw = std::tan(w / 2.0), tw2 = tw * tw;
+  const double u = pt[0], v = pt[1];
+  const double r_u = std::sqrt(u * u + v * v);
+  const double at = std::atan(2.0 * tw * r_u);
+  if (w * w < 1e-5) return;
+  if (r_u * r_u < 1e-5) {
+    J[0] = J[4] = (w - std::sin(w)) / (w * w * std::cos(w / 2) * std::cos(w / 2));
+    return;
+  }
+  J[0] = (2.0 * u * (tw2 / 2.0 + 0.5)) / (w * (4.0 * tw2 * r_u * r_u + 1.0)) - (u * at) / (w * w * r_u);
+  J[4] = (2.0 * v * (tw2 / 2.0 + 0.5)) / (w * (4.0 * tw2 * r_u * r_u + 1.0)) - (v * at) / (w * w * r_u);
+}
+
 void distort(int type, const double* k, double* pt, double* J) {
   if (type == MLBA_DISTORTION_RADTAN) {
     distortRadtan(k, pt, J);
   } else if (type == MLBA_DISTORTION_EQUIDISTANT) {
     distortEquidistant(k, pt, J);
+  } else if (type == MLBA_DISTORTION_FISHEYE) {
+    distortFisheye(k, pt, J);
   } else if (J) {  // NullDistortion: identity
     J[0] = 1.0; J[1] = 0.0; J[2] = 0.0; J[3] = 1.0;
   }
@@ -226,6 +280,7 @@ int project3Functional(int dist_type, const double* intr, const double* dist, in
   if (Jdist) {
     if (dist_type == MLBA_DISTORTION_RADTAN) radtanParamJac(kp, Jdist);
     else if (dist_type == MLBA_DISTORTION_EQUIDISTANT) equidistantParamJac(kp, Jdist);
+    else if (dist_type == MLBA_DISTORTION_FISHEYE) fisheyeParamJac(dist, kp, Jdist);
     else for (int i = 0; i < 8; ++i) Jdist[i] = 0.0;
     for (int i = 0; i < 4; ++i) { Jdist[i] *= fu; Jdist[4 + i] *= fv; }
   }

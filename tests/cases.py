@@ -11,6 +11,7 @@ CAMERAS = {
     0: np.zeros(4),
     1: np.array([-0.28, 0.07, 2e-4, 2e-4]),
     2: np.array([-0.01, 0.02, -0.005, 0.001]),
+    3: np.array([0.9, 0.0, 0.0, 0.0]),  # fisheye (FOV) w
 }
 INTR = np.array([460.0, 460.0, 376.0, 240.0])
 WIDTH, HEIGHT = 752, 480

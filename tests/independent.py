@@ -61,12 +61,30 @@ def distort_equidistant(k, x, y):
     return x * s, y * s
 
 
+def distort_fov(k, x, y):
+    """FOV model (Devernay & Faugeras 2001): r_d = atan(2 r tan(w / 2)) / w,
+    with the reference's limits (w^2 < 1e-5: identity; r^2 < 1e-5: the
+    factor's limit 2 tan(w / 2) / w)."""
+    w = k[0]
+    r2 = x * x + y * y
+    if w * w < 1e-5:
+        f = 1.0
+    elif r2 < 1e-5:
+        f = 2 * np.tan(w / 2) / w
+    else:
+        r = np.sqrt(r2)
+        f = np.arctan(2 * r * np.tan(w / 2)) / (w * r)
+    return x * f, y * f
+
+
 def project(dist_type, intr, dist, p_C):
     x, y = p_C[0] / p_C[2], p_C[1] / p_C[2]
     if dist_type == 1:
         x, y = distort_radtan(dist, x, y)
     elif dist_type == 2:
         x, y = distort_equidistant(dist, x, y)
+    elif dist_type == 3:
+        x, y = distort_fov(dist, x, y)
     return np.array([intr[0] * x + intr[2], intr[1] * y + intr[3]])
 
 

@@ -26,7 +26,7 @@ def opts(**kw):
     return o
 
 
-@pytest.mark.parametrize("dist_type", [0, 1, 2])
+@pytest.mark.parametrize("dist_type", [0, 1, 2, 3])
 @pytest.mark.parametrize("global_missions", [False, True])
 @pytest.mark.parametrize("tangent", [False, True])
 def test_evaluate_matches_oracle(oracle, dist_type, global_missions, tangent):
@@ -62,7 +62,7 @@ def test_evaluate_projection_failures_are_zero(oracle):
 
 
 @pytest.mark.parametrize("dist_type, global_missions, loss", [(1, False, 0), (2, True, 0), (0, False, 1),
-                                                               (1, True, 2)])
+                                                               (1, True, 2), (3, False, 0)])
 def test_solve_iteration_log_matches_oracle(oracle, dist_type, global_missions, loss):
     rng = np.random.default_rng(40 + dist_type)
     m = tiny_map(rng, dist_type=dist_type, n_vertices=7, n_landmarks=80, global_missions=global_missions)
@@ -92,11 +92,13 @@ def test_workload1_final_cost_matches_oracle(oracle):
     np.testing.assert_allclose([r.cost for r in rows], [r.cost for r in ref_rows], rtol=1e-8)
 
 
-@pytest.mark.parametrize("distortion", [1, 2])
+@pytest.mark.parametrize("distortion", [1, 2, 3])
 def test_random_walk_map_final_cost_matches_oracle(oracle, distortion):
     # The random-walk generator of the 500-keyframe workload at 60 keyframes
-    # and 12k landmarks (radtan, and the equidistant variant), Huber, 2% outliers.
-    kw = {} if distortion == 1 else dict(distortion_type=2, distortion=(-0.01, 0.02, -0.005, 0.001))
+    # and 12k landmarks (radtan, the equidistant variant and a fisheye
+    # camera), Huber, 2% outliers.
+    kw = {1: {}, 2: dict(distortion_type=2, distortion=(-0.01, 0.02, -0.005, 0.001)),
+          3: dict(distortion_type=3, distortion=(0.9, 0.0, 0.0, 0.0))}[distortion]
     m = problem.workload_map(2, seed=2, num_keyframes=60, num_landmarks=12000, **kw)
     o = opts(max_num_iterations=10)
     with DeviceProblem(m) as dev:

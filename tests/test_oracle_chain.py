@@ -21,7 +21,7 @@ def lift(oracle, b, J, params):
 
 
 @pytest.mark.parametrize("error_type", [ind.LOCAL_KEYFRAME, ind.LOCAL_MISSION, ind.GLOBAL])
-@pytest.mark.parametrize("dist_type", [0, 1, 2])
+@pytest.mark.parametrize("dist_type", [0, 1, 2, 3])
 def test_residual_and_tangent_jacobians(oracle, error_type, dist_type):
     rng = np.random.default_rng(100 + 10 * error_type + dist_type)
     for _ in range(8):

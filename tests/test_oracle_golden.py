@@ -10,7 +10,8 @@ Reference tests restated here:
 Test fixtures of the reference: test camera (400, 300, 320, 240, 640x480)
 (camera-pinhole.h:251); radtan (-0.28, 0.08, -0.00026, -0.00024)
 (distortion-radtan.h:97); equidistant (0.2, 0.01, 0.3, 0.05)
-(distortion-equidistant.h:109).
+(distortion-equidistant.h:109); fisheye keypoint (399.139364932153,
+200.430317533923) at w = 1 (test_visual_term_test.cc:300-301).
 """
 import numpy as np
 import pytest
@@ -20,7 +21,8 @@ import independent as ind
 TEST_INTR = np.array([400.0, 300.0, 320.0, 240.0])
 RADTAN = np.array([-0.28, 0.08, -0.00026, -0.00024])
 EQUI = np.array([0.2, 0.01, 0.3, 0.05])
-NONE, RAD, EQD = 0, 1, 2
+NONE, RAD, EQD, FOV = 0, 1, 2, 3
+FISHEYE = np.array([0.95, 0.0, 0.0, 0.0])
 
 
 def test_manual_projection_without_distortion(oracle):
@@ -39,17 +41,17 @@ def test_projection_status(oracle, p, status):
     assert st == status
 
 
-@pytest.mark.parametrize("dist_type, dist", [(RAD, RADTAN), (EQD, EQUI)])
+@pytest.mark.parametrize("dist_type, dist", [(RAD, RADTAN), (EQD, EQUI), (FOV, FISHEYE)])
 def test_distortion_matches_textbook_model(oracle, dist_type, dist):
     rng = np.random.default_rng(3)
-    f = ind.distort_radtan if dist_type == RAD else ind.distort_equidistant
+    f = {RAD: ind.distort_radtan, EQD: ind.distort_equidistant, FOV: ind.distort_fov}[dist_type]
     for _ in range(50):
         x, y = rng.uniform(-0.8, 0.8, 2)
         pt, _ = oracle.distort(dist_type, dist, [x, y])
         np.testing.assert_allclose(pt, f(dist, x, y), rtol=1e-14, atol=1e-15)
 
 
-@pytest.mark.parametrize("dist_type, dist", [(RAD, RADTAN), (EQD, EQUI)])
+@pytest.mark.parametrize("dist_type, dist", [(RAD, RADTAN), (EQD, EQUI), (FOV, FISHEYE)])
 def test_distortion_keypoint_jacobian_numdiff(oracle, dist_type, dist):
     # test-distortions.cc:145-150 at (0.3, -0.2), tolerance 1e-4 there; the
     # 5-point stencil allows a much tighter bound.
@@ -66,7 +68,7 @@ def test_equidistant_optical_centre_special_case(oracle):
     np.testing.assert_array_equal(jd, np.zeros((2, 2)))
 
 
-@pytest.mark.parametrize("dist_type, dist", [(NONE, np.zeros(4)), (RAD, RADTAN), (EQD, EQUI)])
+@pytest.mark.parametrize("dist_type, dist", [(NONE, np.zeros(4)), (RAD, RADTAN), (EQD, EQUI), (FOV, FISHEYE)])
 def test_projection_jacobians_numdiff(oracle, dist_type, dist):
     # test-cameras.cc:88-93 (point, 1e-5), :136-146 (intrinsics 1e-6),
     # :190-199 (distortion 1e-4): random points at depth ~10.
@@ -81,6 +83,49 @@ def test_projection_jacobians_numdiff(oracle, dist_type, dist):
         if dist_type != NONE:
             f_d = lambda d: oracle.project(dist_type, TEST_INTR, dist + d, 640, 480, p, False)[1]
             np.testing.assert_allclose(jd, ind.numdiff(f_d, 4, 1e-5), rtol=1e-6, atol=1e-7)
+
+
+def test_fisheye_keypoint_of_the_visual_term_test(oracle):
+    # test_visual_term_test.cc:300-301 (VisualErrorTermNonZeroDistortionOptimization):
+    # keypoint (399.139364932153, 200.430317533923) generated in Matlab for
+    # w = 1 with intrinsics (100, 100, 320, 240), 640 x 480, landmark at the
+    # origin of an identity base pose, observer at M_p_MI = (-1, 0.5, -1)
+    # with identity orientation and extrinsics.
+    intr = np.array([100.0, 100.0, 320.0, 240.0])
+    st, kp, *_ = oracle.project(FOV, intr, [1.0, 0, 0, 0], 640, 480, [1.0, -0.5, 1.0])
+    assert st == 0
+    np.testing.assert_allclose(kp, [399.139364932153, 200.430317533923], rtol=0, atol=1e-9)
+    ident = np.array([0.0, 0, 0, 1, 0, 0, 0])
+    observer = np.array([0.0, 0, 0, 1, -1.0, 0.5, -1.0])
+    params = [np.zeros(3), ident, ident, ident, observer, ident[:4], np.zeros(3), intr, np.array([1.0, 0, 0, 0])]
+    failed, r, _ = oracle.evaluate_visual([399.139364932153, 200.430317533923], 0.7, 1, FOV, 640, 480, params)
+    assert not failed
+    np.testing.assert_allclose(r, [0.0, 0.0], atol=2e-9)
+
+
+def test_fisheye_limits_follow_the_reference(oracle):
+    # distortion-fisheye.cc:31-42 / :50-56 / :99-104: w^2 < 1e-5 leaves the
+    # point and an identity Jacobian; r^2 < 1e-5 scales by 2 tan(w/2) / w and
+    # the parameter Jacobian's two rows are that factor's w-derivative.
+    pt, jd = oracle.distort(FOV, [1e-3, 0, 0, 0], [0.3, -0.2])
+    np.testing.assert_array_equal(pt, [0.3, -0.2])
+    np.testing.assert_array_equal(jd, np.eye(2))
+    w = 0.95
+    f = 2 * np.tan(w / 2) / w
+    pt, jd = oracle.distort(FOV, [w, 0, 0, 0], [1e-3, 2e-3])
+    np.testing.assert_allclose(pt, [1e-3 * f, 2e-3 * f], rtol=1e-15)
+    np.testing.assert_allclose(jd, f * np.eye(2), rtol=1e-15)
+    # Parameter Jacobian outside the limit: numdiff in w; inside: the
+    # closed-form derivative of 2 tan(w/2)/w, the same in both rows.
+    # (At r = 1/2 the factor is 1 for every w, so the point has r != 1/2.)
+    p = np.array([0.6, -0.2, 1.0])
+    _, _, _, _, jdist = oracle.project(FOV, TEST_INTR, [w, 0, 0, 0], 640, 480, p)
+    f_w = lambda d: oracle.project(FOV, TEST_INTR, [w + d[0], 0, 0, 0], 640, 480, p, False)[1]
+    np.testing.assert_allclose(jdist[:, :1], ind.numdiff(f_w, 1, 1e-5), rtol=1e-7)
+    np.testing.assert_array_equal(jdist[:, 1:], 0.0)
+    _, _, _, _, jdist = oracle.project(FOV, TEST_INTR, [w, 0, 0, 0], 640, 480, [1e-3, 1e-3, 1.0])
+    df = ((1 + np.tan(w / 2) ** 2) * w - 2 * np.tan(w / 2)) / (w * w)
+    np.testing.assert_allclose(jdist[:, 0], [TEST_INTR[0] * df, TEST_INTR[1] * df], rtol=1e-12)
 
 
 def test_rotation_is_transposed_hamilton(oracle):

@@ -3,8 +3,8 @@
 * Reference known answers (algorithms/ceres-error-terms/test/
   test_visual_term_test.cc:142-226): ground-truth start stops after one
   iteration with zero cost; a landmark seen by two fixed cameras converges to
-  the origin. The reference fixture uses FOV-fisheye distortion with w = 0,
-  which is the identity (distortion-fisheye.cc:32-34); here NullDistortion.
+  the origin. The reference fixture's camera, pinhole with FOV-fisheye
+  distortion w = 0 (test_visual_term_test.cc:26-59), is used as it is.
 * Three linear solvers of the oracle (Schur + Cholesky, dense normal
   equations, Schur + PCG) agree on the LM trajectory.
 * A separate numpy restatement of the Ceres LM rules (Jacobi scaling, LM
@@ -44,7 +44,7 @@ def fixture_map(landmark, cameras, keypoints, sigma=0.7, intr=(1.0, 1.0, 320.0, 
     return VisualMap(
         vertices=verts, baseframes=np.array([[0, 0, 0, 1.0, 0, 0, 0]]), landmarks=np.array([landmark], float),
         camera_q_CI__C_p_CI=np.array([[0, 0, 0, 1.0, 0, 0, 0]]), camera_intrinsics=np.array([intr]),
-        camera_distortion=np.zeros((1, 4)), camera_distortion_type=np.array([0], np.int32),
+        camera_distortion=np.zeros((1, 4)), camera_distortion_type=np.array([3], np.int32),
         camera_resolution=np.array([[640, 480]], np.int32), vertex_mission=np.zeros(P, np.int32),
         landmark_store_vertex=np.array([0], np.int32), obs_vertex=np.arange(1, P, dtype=np.int32),
         obs_landmark=np.zeros(n, np.int32), obs_camera=np.zeros(n, np.int32), obs_keypoint=np.array(keypoints, float),
@@ -59,7 +59,7 @@ def test_ground_truth_start_stops_at_iteration_zero(oracle):
         vertices=np.array([[0, 0, 0, 1.0, 0, 0, -1], [0, 0, 0, 1.0, 0, 0, -1]]),
         baseframes=np.array([[0, 0, 0, 1.0, 0, 0, 0]]), landmarks=np.array([[0, 0, 1.0]]),
         camera_q_CI__C_p_CI=np.array([[0, 0, 0, 1.0, 0, 0, 0]]), camera_intrinsics=np.array([[1.0, 1, 0, 0]]),
-        camera_distortion=np.zeros((1, 4)), camera_distortion_type=np.array([0], np.int32),
+        camera_distortion=np.zeros((1, 4)), camera_distortion_type=np.array([3], np.int32),
         camera_resolution=np.array([[640, 480]], np.int32), vertex_mission=np.zeros(2, np.int32),
         landmark_store_vertex=np.array([0], np.int32), obs_vertex=np.array([1], np.int32),
         obs_landmark=np.zeros(1, np.int32), obs_camera=np.zeros(1, np.int32), obs_keypoint=np.zeros((1, 2)),

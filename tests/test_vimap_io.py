@@ -33,9 +33,10 @@ def landmark_permutation(read, orig):
     return np.array([where[tuple(p)] for p in read.landmarks])
 
 
-@pytest.fixture(scope="module", params=[False, True], ids=["radtan", "equidistant"])
+@pytest.fixture(scope="module", params=[1, 2, 3], ids=["radtan", "equidistant", "fisheye"])
 def written(request, tmp_path_factory):
-    kw = dict(distortion_type=2, distortion=(-0.01, 0.02, -0.005, 0.001)) if request.param else {}
+    kw = {1: {}, 2: dict(distortion_type=2, distortion=(-0.01, 0.02, -0.005, 0.001)),
+          3: dict(distortion_type=3, distortion=(0.9, 0.0, 0.0, 0.0))}[request.param]
     m = problem.workload_map(2, seed=3, num_keyframes=30, num_landmarks=3000, **kw)
     folder = str(tmp_path_factory.mktemp("map"))
     vimap.write_vimap(m, folder)
