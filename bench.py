@@ -162,6 +162,20 @@ def multimap(args):
     t_con = time.perf_counter() - t0
     iters = sum(sm.num_iterations - 1 for sm in con)
     same = all(a.final_cost == b.final_cost for a, b in zip(seq, con))
+    # The same concurrent solve after each map's Schur block schedule is
+    # autotuned (set-up, untimed), against the fixed default schedule above.
+    tuned = None
+    if not args.no_autotune:
+        for d, (v, l) in zip(devs, init):
+            d.upload_state(v, l)
+            d.autotune(opts, repeats=3)
+            d.upload_state(v, l)
+        t0 = time.perf_counter()
+        con_t = [sm for sm, _ in DeviceProblem.solve_many(devs, opts, threads=args.multimap_threads)]
+        t_tuned = time.perf_counter() - t0
+        tuned = {"wall_s": round(t_tuned, 4),
+                 "lm_iterations_per_s": round(sum(sm.num_iterations - 1 for sm in con_t) / t_tuned, 2),
+                 "same_results_as_fixed": all(a.final_cost == b.final_cost for a, b in zip(con, con_t))}
     for d in devs:
         d.close()
     print(json.dumps({
@@ -176,7 +190,8 @@ def multimap(args):
                        "observation_lm_iterations_per_s": round(
                            sum(m.num_observations * (sm.num_iterations - 1) for m, sm in zip(maps, con)) / t_con,
                            1)},
-        "speedup": round(t_seq / t_con, 3), "same_results": bool(same)}), flush=True)
+        "speedup": round(t_seq / t_con, 3), "same_results": bool(same),
+        "concurrent_autotuned_schedule": tuned}), flush=True)
     return 0
 
 
@@ -211,6 +226,7 @@ def main(argv=None):
                          "log-uniform in 50-800 keyframes and 10k-150k landmarks) solved one after the other "
                          "and concurrently (mlba_solve_many)")
     ap.add_argument("--multimap-threads", type=int, default=8)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default Schur block schedule")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -240,6 +256,14 @@ def main(argv=None):
         opts.max_num_iterations = args.warmup
         dev.solve(opts)
         dev.upload_state(init_v, init_l)
+    sched = None
+    if not args.no_autotune:
+        # Launch schedule of the Schur block kernel (set-up, untimed; changes
+        # no result bit).
+        sched = dev.autotune(opts, repeats=3)
+        print(f"bench: autotuned schedule (band {sched.item_band_rows}, waves {sched.item_waves}, "
+              f"xcd {sched.xcd_mapping}): {sched.tuned_ms:.4f} ms vs default {sched.default_ms:.4f} ms",
+              file=sys.stderr, flush=True)
     kt = dev.time_kernels(opts, repeats=args.repeats)
     dev.upload_state(init_v, init_l)
     # Timed region: exactly `steps` LM iterations (fewer only if Ceres'
@@ -309,6 +333,11 @@ def main(argv=None):
         "kernels": table,
         "roofline": roof,
     }
+    if sched is not None:
+        result["schur_schedule"] = {
+            "item_band_rows": sched.item_band_rows, "item_waves": sched.item_waves,
+            "xcd_mapping": sched.xcd_mapping, "schedules_tried": sched.schedules_tried,
+            "k_schur_items_default_ms": round(sched.default_ms, 4), "k_schur_items_tuned_ms": round(sched.tuned_ms, 4)}
 
     if not args.no_cpu_baseline:
         nproc = os.cpu_count() or 1

@@ -334,6 +334,37 @@ typedef struct mlba_kernel_times {
 int32_t mlba_time_kernels(mlba_handle* handle, const mlba_options* options, int32_t repeats,
                           mlba_kernel_times* times);
 
+/*
+ * Launch-schedule autotuner (no counterpart in the reference: Ceres has no
+ * device schedules). The reduced-camera-system block kernel sums fixed
+ * incidence-pair ranges (work items) into fixed partial slots; what can be
+ * chosen is only the order the items are dispatched in (bands of
+ * item_band_rows block rows swept column by column, 0 = block order), how
+ * many items a work-group runs (item_waves: 1, 2, 4, 8) and whether
+ * work-groups are mapped XCD-contiguously. None of these changes a result
+ * bit: a solve after mlba_autotune or mlba_set_schedule repeats the same
+ * iteration log and state.
+ *
+ * mlba_autotune times every schedule (item_band_rows in {0,1,2,4,8,16,32} x
+ * item_waves x mapping, `repeats` launches each) on the Schur factors of
+ * the current state and keeps the fastest for this handle; it leaves the
+ * state unchanged. The schedule survives topology rebuilds (outlier
+ * rejection, landmark masks).
+ */
+typedef struct mlba_schedule {
+  int32_t item_band_rows;     /* 0 or 1..4096 */
+  int32_t item_waves;         /* 1, 2, 4 or 8 */
+  int32_t xcd_mapping;        /* 0 or 1 */
+  int32_t schedules_tried;    /* mlba_autotune: schedules timed */
+  double default_ms;          /* mlba_autotune: block kernel with the library default (4, 4, 1) */
+  double tuned_ms;            /* mlba_autotune: block kernel with the chosen schedule */
+} mlba_schedule;
+
+int32_t mlba_autotune(mlba_handle* handle, const mlba_options* options, int32_t repeats,
+                      mlba_schedule* chosen);
+int32_t mlba_set_schedule(mlba_handle* handle, const mlba_schedule* schedule);
+int32_t mlba_get_schedule(const mlba_handle* handle, mlba_schedule* schedule);
+
 #ifdef __cplusplus
 }
 #endif

@@ -122,6 +122,11 @@ class KernelTimes(ctypes.Structure):
             "pcg_window_columns")]
 
 
+class Schedule(ctypes.Structure):
+    _fields_ = [("item_band_rows", c_i32), ("item_waves", c_i32), ("xcd_mapping", c_i32),
+                ("schedules_tried", c_i32), ("default_ms", c_f64), ("tuned_ms", c_f64)]
+
+
 class QualityOptions(ctypes.Structure):
     _fields_ = [("min_observers", c_i32), ("reserved0", c_i32), ("min_observation_angle_deg", c_f64),
                 ("max_distance_from_closest_observer", c_f64), ("min_distance_from_closest_observer", c_f64),
@@ -179,6 +184,9 @@ LIB_SYMBOLS = {
                                 ctypes.POINTER(ctypes.POINTER(Summary)), ctypes.POINTER(c_i32), c_i32, c_i32,
                                 c_i32]),
     "mlba_time_kernels": (c_i32, [c_vp, ctypes.POINTER(Options), c_i32, ctypes.POINTER(KernelTimes)]),
+    "mlba_autotune": (c_i32, [c_vp, ctypes.POINTER(Options), c_i32, ctypes.POINTER(Schedule)]),
+    "mlba_set_schedule": (c_i32, [c_vp, ctypes.POINTER(Schedule)]),
+    "mlba_get_schedule": (c_i32, [c_vp, ctypes.POINTER(Schedule)]),
     "mlba_quality_options_init": (None, [ctypes.POINTER(QualityOptions)]),
     "mlba_landmark_quality": (c_i32, [c_vp, ctypes.POINTER(QualityOptions), c_vp]),
 }

@@ -51,7 +51,11 @@ void launch_schur_blocks(hipStream_t s, const DevTopology& t, int32_t nub, const
                          const int4* items, const int32_t* item_ptr, const uint2* pairs, const double* inc_q,
                          const double* inc_w, const double* inc_rhs, const double* lm_a, const double* pose_scale,
                          const double* pose_diag, const double* pose_jtj, double inv_mu, double min_diag,
-                         double max_diag, double* partial, double* S, double* rhs);
+                         double max_diag, double* partial, double* S, double* rhs, int32_t waves, int32_t xcd_map);
+void launch_schur_items(hipStream_t s, const DevTopology& t, int32_t n_items, const int4* items, const uint2* pairs,
+                        const double* inc_q, const double* inc_w, const double* lm_a, double* partial,
+                        int32_t waves, int32_t xcd_map);
+bool schur_item_waves_supported(int32_t waves);
 uint32_t schur_pair_anchor_bit();
 void launch_landmark_quality(hipStream_t s, const DevTopology& t, const double* vert, const double* vR,
                              const double* lm, int32_t min_observers, double min_angle_deg, double max_dist,
@@ -165,6 +169,14 @@ struct mlba_handle {
   DBuf<double> inc_w, inc_r0, lm_a;     // per-linearization Schur factors (incidence order) and anchor maps
   DBuf<double> inc_q, inc_rhs;          // per-iteration Schur factors
   DBuf<double> schur_partial;           // 36 per work item
+  // Schur work items in block order (item.w = partial slot) with the row and
+  // column of each item's block; the device copy is in dispatch order.
+  std::vector<int4> h_items;
+  std::vector<int2> h_item_rc;
+  // Dispatch schedule of k_schur_items (mlba_autotune): block rows per band
+  // of the dispatch order (0: block order), items (waves) per work-group,
+  // XCD-contiguous work-group mapping. It changes no result.
+  int32_t sched_band = kItemBandRows, sched_waves = 4, sched_xcd = 1;
   DBuf<int32_t> hat_node, node_range, grp_lo, blk_hat, blk_rng;
   std::vector<int32_t> h_row_ptr, h_col_var;  // host copy of the S block structure
   int32_t grp_width = 0;                // hat columns of the per-group SW window
@@ -237,6 +249,31 @@ int upload(mlba_handle* h, DBuf<T>& b, const T* src, size_t count) {
 bool valid_quaternion(const double* q) {
   const double n2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
   return std::isfinite(n2) && std::fabs(n2 - 1.0) < 2e-5 && q[3] >= 0.0;
+}
+
+// Upload the Schur work items in the dispatch order of the current schedule:
+// bands of sched_band block rows, each band swept by column j, so that the
+// band's Q factors and the current column's W factors are reused from L2
+// (sched_band 0: block order). Partial slots keep the block order.
+int upload_item_order(mlba_handle* h) {
+  std::vector<int32_t> ord(h->h_items.size());
+  for (size_t k = 0; k < ord.size(); ++k) ord[k] = (int32_t)k;
+  const int band = h->sched_band;
+  if (band > 0)
+    std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) {
+      const int2 ra = h->h_item_rc[a], rb = h->h_item_rc[b];
+      const int ba = ra.x / band, bb = rb.x / band;
+      if (ba != bb) return ba < bb;
+      if (ra.y != rb.y) return ra.y < rb.y;
+      return ra.x < rb.x;
+    });
+  std::vector<int4> items(ord.size());
+  for (size_t k = 0; k < ord.size(); ++k) items[k] = h->h_items[ord[k]];
+  int rc = upload(h, h->items, items.data(), items.size());
+  if (rc) return rc;
+  // The host vector is released when the call returns: wait for the copy.
+  HIP_OK(hipStreamSynchronize(h->stream));
+  return MLBA_OK;
 }
 
 // Build the observation / landmark / vertex topology and the reduced camera
@@ -430,18 +467,11 @@ int build_topology(mlba_handle* h) {
         items.push_back(make_int4(u, pb, std::min(pb + ip, pair_ptr[u + 1]), (int32_t)items.size()));
       item_ptr.push_back((int32_t)items.size());
     }
-    // Dispatch order: bands of kItemBandRows block rows, each band swept by
-    // column j, so that the band's Q factors and the current column's W
-    // factors are reused from L2. Partial slots keep the block order.
-    std::stable_sort(items.begin(), items.end(), [&](const int4& a, const int4& b) {
-      const int4 ua = ublk[a.x], ub = ublk[b.x];
-      const int ba = ua.z / kItemBandRows, bb = ub.z / kItemBandRows;
-      if (ba != bb) return ba < bb;
-      if (ua.w != ub.w) return ua.w < ub.w;
-      return ua.z < ub.z;
-    });
   }
   h->n_items = (int32_t)items.size();
+  h->h_item_rc.resize(items.size());
+  for (size_t k = 0; k < items.size(); ++k) h->h_item_rc[k] = make_int2(ublk[items[k].x].z, ublk[items[k].x].w);
+  h->h_items = std::move(items);
   std::vector<int32_t> obs_col(N, -1);
   for (int o = 0; o < N; ++o)
     if ((meta[o] & 4) && h->lm_active[h->s_landmark[o]]) obs_col[o] = vertex_var[h->s_vertex[o]];
@@ -459,7 +489,7 @@ int build_topology(mlba_handle* h) {
   if ((rc = upload(h, h->diag_pos, diag_pos.data(), diag_pos.size()))) return rc;
   if ((rc = upload(h, h->inc_row, inc_row.data(), inc_row.size()))) return rc;
   if ((rc = upload(h, h->ublk, ublk.data(), ublk.size()))) return rc;
-  if ((rc = upload(h, h->items, items.data(), items.size()))) return rc;
+  if ((rc = upload_item_order(h))) return rc;
   if ((rc = upload(h, h->item_ptr, item_ptr.data(), item_ptr.size()))) return rc;
   if ((rc = upload(h, h->pairs, pairs.data(), pairs.size()))) return rc;
   if ((rc = upload(h, h->obs_col, obs_col.data(), obs_col.size()))) return rc;
@@ -468,7 +498,8 @@ int build_topology(mlba_handle* h) {
   bool ok = h->tile_cost.alloc(2 * T) && h->tile_cost2.alloc(2 * T) && h->S.alloc(36 * (size_t)h->nnzb) &&
             h->inc_q.alloc(18 * std::max<size_t>(inc.size(), 1)) && h->inc_rhs.alloc(6 * std::max<size_t>(inc.size(), 1)) &&
             h->inc_w.alloc(18 * std::max<size_t>(inc.size(), 1)) && h->inc_r0.alloc(6 * std::max<size_t>(inc.size(), 1)) &&
-            h->lm_a.alloc(18 * (size_t)std::max(L, 1)) && h->schur_partial.alloc(36 * std::max<size_t>(items.size(), 1)) &&
+            h->lm_a.alloc(18 * (size_t)std::max(L, 1)) &&
+            h->schur_partial.alloc(36 * std::max<size_t>(h->h_items.size(), 1)) &&
             h->pose_jtj.alloc(36 * (size_t)std::max(Pvar, 1)) &&
             h->rhs.alloc(V6) && h->pose_grad.alloc(V6) && h->pose_diag.alloc(V6) && h->pose_scale.alloc(V6) &&
             h->w.alloc(V6) && h->Minv.alloc(36 * (size_t)std::max(Pvar, 1)) && h->px.alloc(V6) &&
@@ -893,7 +924,7 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
         launch_schur_blocks(s, h->topo, h->n_upper, h->ublk.p, h->n_items, h->items.p, h->item_ptr.p, h->pairs.p,
                             h->inc_q.p, h->inc_w.p, h->inc_rhs.p, h->lm_a.p, h->pose_scale.p, h->pose_diag.p,
                             h->pose_jtj.p, inv_mu, o->min_lm_diagonal, o->max_lm_diagonal, h->schur_partial.p,
-                            h->S.p, h->rhs.p);
+                            h->S.p, h->rhs.p, h->sched_waves, h->sched_xcd);
       }
       launch_block_inverse(s, h->Pvar, h->diag_pos.p, h->S.p, h->Minv.p, h->fail.p);
       HIP_OK(hipGetLastError());
@@ -1510,6 +1541,102 @@ int32_t mlba_solve_with_outlier_rejection(mlba_handle* h, const mlba_options* o,
   return MLBA_OK;
 }
 
+int32_t mlba_get_schedule(const mlba_handle* h, mlba_schedule* sc) {
+  if (!h || !sc) return MLBA_EINVAL;
+  std::memset(sc, 0, sizeof(*sc));
+  sc->item_band_rows = h->sched_band;
+  sc->item_waves = h->sched_waves;
+  sc->xcd_mapping = h->sched_xcd;
+  return MLBA_OK;
+}
+
+int32_t mlba_set_schedule(mlba_handle* h, const mlba_schedule* sc) {
+  if (!h) return MLBA_EINVAL;
+  if (!sc) return set_err(h, MLBA_EINVAL, "schedule is NULL");
+  if (sc->item_band_rows < 0 || sc->item_band_rows > 4096) return set_err(h, MLBA_EINVAL, "item_band_rows out of range");
+  if (!schur_item_waves_supported(sc->item_waves)) return set_err(h, MLBA_EINVAL, "item_waves must be 1, 2, 4 or 8");
+  if (sc->xcd_mapping != 0 && sc->xcd_mapping != 1) return set_err(h, MLBA_EINVAL, "xcd_mapping must be 0 or 1");
+  const bool reorder = sc->item_band_rows != h->sched_band;
+  h->sched_band = sc->item_band_rows;
+  h->sched_waves = sc->item_waves;
+  h->sched_xcd = sc->xcd_mapping;
+  return reorder ? upload_item_order(h) : MLBA_OK;
+}
+
+int32_t mlba_autotune(mlba_handle* h, const mlba_options* o, int32_t repeats, mlba_schedule* out) {
+  if (!h) return MLBA_EINVAL;
+  if (!out || repeats <= 0) return set_err(h, MLBA_EINVAL, "autotune needs a schedule output and repeats > 0");
+  int rc = validate_options(h, o);
+  if (rc) return rc;
+  std::memset(out, 0, sizeof(*out));
+  hipStream_t s = h->stream;
+  if (h->n_items > 0) {
+    // The Schur factors of the first LM iteration at the current state.
+    if ((rc = linearize(h, o))) return rc;
+    launch_jacobi_scaling(s, h->Pvar, h->L, o->use_jacobi_scaling, h->pose_diag.p, h->lm_acc.p, h->pose_scale.p,
+                          h->lm_scale.p);
+    schur_factors(h);
+    HIP_OK(hipMemsetAsync(h->fail.p, 0, sizeof(int32_t), s));
+    launch_landmark_prep(s, h->topo, h->lm_acc.p, h->lm_scale.p, 1.0 / o->initial_trust_region_radius,
+                         o->min_lm_diagonal, o->max_lm_diagonal, h->lm_z.p, h->fail.p);
+    launch_schur_incidences(s, h->topo, h->n_inc, h->inc_w.p, h->inc_r0.p, h->lm_a.p, h->lm_acc.p, h->lm_z.p,
+                            h->inc_q.p, h->inc_rhs.p);
+    HIP_OK(hipGetLastError());
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+      (void)hipEventDestroy(e0);
+      return set_err(h, MLBA_EDEVICE, "hipEventCreate failed");
+    }
+    auto time_items = [&]() -> double {
+      launch_schur_items(s, h->topo, h->n_items, h->items.p, h->pairs.p, h->inc_q.p, h->inc_w.p, h->lm_a.p,
+                         h->schur_partial.p, h->sched_waves, h->sched_xcd);  // warm
+      (void)hipEventRecord(e0, s);
+      for (int k = 0; k < repeats; ++k)
+        launch_schur_items(s, h->topo, h->n_items, h->items.p, h->pairs.p, h->inc_q.p, h->inc_w.p, h->lm_a.p,
+                           h->schur_partial.p, h->sched_waves, h->sched_xcd);
+      (void)hipEventRecord(e1, s);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      return (double)ms / repeats;
+    };
+    static const int32_t kBands[] = {0, 1, 2, 4, 8, 16, 32}, kWaves[] = {1, 2, 4, 8};
+    mlba_schedule best{kItemBandRows, 4, 1, 0, 0.0, 0.0};
+    double best_ms = -1.0, default_ms = -1.0;
+    for (int32_t band : kBands) {
+      h->sched_band = band;
+      if ((rc = upload_item_order(h))) break;
+      for (int32_t waves : kWaves)
+        for (int32_t xcd = 0; xcd < 2; ++xcd) {
+          h->sched_waves = waves;
+          h->sched_xcd = xcd;
+          const double ms = time_items();
+          ++out->schedules_tried;
+          if (band == kItemBandRows && waves == 4 && xcd == 1) default_ms = ms;
+          if (best_ms < 0.0 || ms < best_ms) {
+            best_ms = ms;
+            best.item_band_rows = band;
+            best.item_waves = waves;
+            best.xcd_mapping = xcd;
+          }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIP_OK(hipGetLastError());
+    if (rc) return rc;
+    h->sched_band = -1;  // force the reorder
+    if ((rc = mlba_set_schedule(h, &best))) return rc;
+    out->default_ms = default_ms;
+    out->tuned_ms = best_ms;
+  }
+  out->item_band_rows = h->sched_band;
+  out->item_waves = h->sched_waves;
+  out->xcd_mapping = h->sched_xcd;
+  return MLBA_OK;
+}
+
 int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats, mlba_kernel_times* t) {
   if (!h || !t || repeats <= 0) return MLBA_EINVAL;
   int rc = validate_options(h, o);
@@ -1562,7 +1689,7 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
     launch_schur_blocks(s, h->topo, h->n_upper, h->ublk.p, h->n_items, h->items.p, h->item_ptr.p, h->pairs.p,
                         h->inc_q.p, h->inc_w.p, h->inc_rhs.p, h->lm_a.p, h->pose_scale.p, h->pose_diag.p,
                         h->pose_jtj.p, inv_mu, o->min_lm_diagonal, o->max_lm_diagonal, h->schur_partial.p, h->S.p,
-                        h->rhs.p);
+                        h->rhs.p, h->sched_waves, h->sched_xcd);
   });
   t->block_inverse_ms = timed([&] { launch_block_inverse(s, h->Pvar, h->diag_pos.p, h->S.p, h->Minv.p, h->fail.p); });
   if (h->Pvar > 0) {

@@ -527,7 +527,6 @@ __global__ __launch_bounds__(256) void k_schur_incidences(
   for (int c = 0; c < 6; ++c) rr[c] = rx[c];
 }
 
-constexpr int kItemWaves = 4;      // work items per work-group (one per wave)
 constexpr int kItemPairs = 512;    // incidence pairs per work item (32 per wave pass)
 constexpr uint32_t kPairAnchor = 0x80000000u;  // the incidence is the anchor of its landmark
 
@@ -541,15 +540,18 @@ constexpr uint32_t kPairAnchor = 0x80000000u;  // the incidence is the anchor of
 // (| kPairAnchor). Items come in dispatch order (bands of a few block rows
 // swept together column by column, so that the band's Q factors and one
 // column's W factors are reused from L2) and carry the slot of their partial
-// sum; work-groups are mapped so that each XCD walks a contiguous range of
-// the dispatch order.
-__global__ __launch_bounds__(64 * kItemWaves) void k_schur_items(
-    DevTopology t, int32_t n_items, const int4* __restrict__ items, const uint2* __restrict__ pairs,
-    const double* __restrict__ inc_q, const double* __restrict__ inc_w, const double* __restrict__ lm_a,
-    double* __restrict__ partial) {
+// sum; with xcd_map, work-groups are mapped so that each XCD walks a
+// contiguous range of the dispatch order (work-group b runs on XCD b mod 8).
+// The dispatch order, kWaves (items per work-group) and xcd_map are a
+// schedule only: every item's partial is the same whichever is chosen.
+template <int kWaves>
+__global__ __launch_bounds__(64 * kWaves) void k_schur_items(
+    DevTopology t, int32_t n_items, int32_t xcd_map, const int4* __restrict__ items,
+    const uint2* __restrict__ pairs, const double* __restrict__ inc_q, const double* __restrict__ inc_w,
+    const double* __restrict__ lm_a, double* __restrict__ partial) {
   const int per_xcd = gridDim.x / 8;
-  const int wg = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
-  const int it = wg * kItemWaves + (threadIdx.x >> 6);
+  const int wg = xcd_map ? (blockIdx.x % 8) * per_xcd + blockIdx.x / 8 : (int)blockIdx.x;
+  const int it = wg * kWaves + (threadIdx.x >> 6);
   if (it >= n_items) return;
   const int lane = threadIdx.x & 63, slot = lane >> 2, quad = lane & 3;
   const int r0 = 3 * (quad >> 1), c0 = 3 * (quad & 1);
@@ -1388,18 +1390,35 @@ void launch_schur_incidences(hipStream_t s, const DevTopology& t, int32_t n_inc,
     k_schur_incidences<<<(n_inc + 255) / 256, 256, 0, s>>>(t, n_inc, inc_w, inc_r0, lm_a, lm_acc, lm_z, inc_q,
                                                            inc_rhs);
 }
+template <int kWaves>
+void launch_items_w(hipStream_t s, const DevTopology& t, int32_t n_items, int32_t xcd_map, const int4* items,
+                    const uint2* pairs, const double* inc_q, const double* inc_w, const double* lm_a,
+                    double* partial) {
+  const int wgs = (n_items + kWaves - 1) / kWaves;
+  const int per_xcd = (wgs + 7) / 8;
+  k_schur_items<kWaves><<<8 * per_xcd, 64 * kWaves, 0, s>>>(t, n_items, xcd_map, items, pairs, inc_q, inc_w, lm_a,
+                                                           partial);
+}
+bool schur_item_waves_supported(int32_t waves) { return waves == 1 || waves == 2 || waves == 4 || waves == 8; }
+void launch_schur_items(hipStream_t s, const DevTopology& t, int32_t n_items, const int4* items, const uint2* pairs,
+                        const double* inc_q, const double* inc_w, const double* lm_a, double* partial,
+                        int32_t waves, int32_t xcd_map) {
+  if (n_items <= 0) return;
+  switch (waves) {
+    case 1: launch_items_w<1>(s, t, n_items, xcd_map, items, pairs, inc_q, inc_w, lm_a, partial); break;
+    case 2: launch_items_w<2>(s, t, n_items, xcd_map, items, pairs, inc_q, inc_w, lm_a, partial); break;
+    case 8: launch_items_w<8>(s, t, n_items, xcd_map, items, pairs, inc_q, inc_w, lm_a, partial); break;
+    default: launch_items_w<4>(s, t, n_items, xcd_map, items, pairs, inc_q, inc_w, lm_a, partial); break;
+  }
+}
 void launch_schur_blocks(hipStream_t s, const DevTopology& t, int32_t nub, const int4* ublk, int32_t n_items,
                          const int4* items, const int32_t* item_ptr, const uint2* pairs, const double* inc_q,
                          const double* inc_w, const double* inc_rhs, const double* lm_a, const double* pose_scale,
                          const double* pose_diag, const double* pose_jtj, double inv_mu, double min_diag,
-                         double max_diag, double* partial, double* S, double* rhs) {
+                         double max_diag, double* partial, double* S, double* rhs, int32_t waves, int32_t xcd_map) {
   if (t.Pvar <= 0 || nub <= 0) return;
   k_schur_rhs<<<t.Pvar, 256, 0, s>>>(t, inc_rhs, rhs);
-  if (n_items > 0) {
-    const int wgs = (n_items + kItemWaves - 1) / kItemWaves;
-    const int per_xcd = (wgs + 7) / 8;
-    k_schur_items<<<8 * per_xcd, 64 * kItemWaves, 0, s>>>(t, n_items, items, pairs, inc_q, inc_w, lm_a, partial);
-  }
+  launch_schur_items(s, t, n_items, items, pairs, inc_q, inc_w, lm_a, partial, waves, xcd_map);
   k_schur_finalize<<<(nub + 6) / 7, 252, 0, s>>>(nub, ublk, item_ptr, partial, pose_scale, pose_diag, pose_jtj,
                                                  inv_mu, min_diag, max_diag, S);
 }

@@ -125,6 +125,22 @@ class DeviceProblem:
             raise RuntimeError(f"mlba_solve_many failed: {rc}")
         return [(sm, [it[i] for i in range(min(sm.num_iterations, capacity))]) for sm, it in zip(sms, its)]
 
+    def autotune(self, options: abi.Options, repeats: int = 5) -> abi.Schedule:
+        """mlba_autotune: time every dispatch schedule of the Schur block
+        kernel at the current state and keep the fastest (results unchanged)."""
+        sc = abi.Schedule()
+        self._check(self.lib.mlba_autotune(self.handle, ctypes.byref(options), repeats, ctypes.byref(sc)))
+        return sc
+
+    def set_schedule(self, band: int, waves: int, xcd: int) -> None:
+        sc = abi.Schedule(item_band_rows=band, item_waves=waves, xcd_mapping=xcd)
+        self._check(self.lib.mlba_set_schedule(self.handle, ctypes.byref(sc)))
+
+    def schedule(self) -> abi.Schedule:
+        sc = abi.Schedule()
+        self._check(self.lib.mlba_get_schedule(self.handle, ctypes.byref(sc)))
+        return sc
+
     def time_kernels(self, options: abi.Options, repeats: int = 10) -> abi.KernelTimes:
         """Average device time of each kernel of one LM iteration (HIP events
         on the library's stream)."""

@@ -66,6 +66,7 @@ def test_struct_layouts_match_header():
     (abi.IterationSummary, "mlba_iteration_summary", "maplab_ba.h"),
     (abi.Summary, "mlba_summary", "maplab_ba.h"),
     (abi.KernelTimes, "mlba_kernel_times", "maplab_ba.h"),
+    (abi.Schedule, "mlba_schedule", "maplab_ba.h"),
     (abi.SynthConfig, "mlba_synth_config", "maplab_ba_synth.h"),
     (abi.QualityOptions, "mlba_quality_options", "maplab_ba.h"),
     (abi.ViProblemOptions, "mlba_vi_problem_options", "maplab_ba_optimizer.h"),
@@ -180,3 +181,12 @@ def test_solve_many_rejects_bad_arguments_without_touching_the_device():
     o = abi.default_options()
     opts = (ctypes.POINTER(abi.Options) * 2)(ctypes.pointer(o), ctypes.pointer(o))
     assert lib.mlba_solve_many(handles, opts, None, None, 2, 0, 0) == abi.MLBA_EINVAL
+
+
+def test_schedule_calls_reject_null_arguments_without_touching_the_device():
+    lib = abi.load_library()
+    o = abi.default_options()
+    sc = abi.Schedule(item_band_rows=4, item_waves=4, xcd_mapping=1)
+    assert lib.mlba_autotune(None, ctypes.byref(o), 5, ctypes.byref(sc)) == abi.MLBA_EINVAL
+    assert lib.mlba_set_schedule(None, ctypes.byref(sc)) == abi.MLBA_EINVAL
+    assert lib.mlba_get_schedule(None, ctypes.byref(sc)) == abi.MLBA_EINVAL
