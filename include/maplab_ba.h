@@ -58,6 +58,13 @@ extern "C" {
                                       0.5 <= w <= 1.5 (distortion-fisheye.cc:145-156); its
                                       distortion Jacobian block has w in column 0, zeros after */
 
+/* Camera models (aslam::Camera::Type subset used by the visual error term,
+ * visual-error-term-factory-inl.h:33-95). */
+#define MLBA_CAMERA_PINHOLE 0             /* camera-pinhole.cc:65-164 */
+#define MLBA_CAMERA_UNIFIED_PROJECTION 1  /* camera-unified-projection.cc:74-196; mirror
+                                             parameter xi from camera_xi, fu fv cu cv from
+                                             camera_intrinsics */
+
 /* ceres_error_terms::LandmarkErrorType (common.h:14). */
 #define MLBA_LOCAL_KEYFRAME 0
 #define MLBA_LOCAL_MISSION 1
@@ -83,6 +90,10 @@ extern "C" {
 /* Evaluation flags for mlba_evaluate. */
 #define MLBA_EVAL_AMBIENT 0 /* Jacobians as VisualReprojectionError::Evaluate writes them */
 #define MLBA_EVAL_TANGENT 1 /* quaternion columns lifted by the JPL local parameterization */
+#define MLBA_EVAL_WITH_XI 2 /* or-ed in: `jacobians` has a 10th entry (nullable) receiving the
+                               2 x 1 Jacobian of the unified-projection xi per observation
+                               (zero for pinhole cameras); the reference's 5-wide intrinsics
+                               block of such a camera is [xi | fu fv cu cv] */
 
 /*
  * A map in structure-of-arrays form. Replaces the VIMap + OptimizationStateBuffer
@@ -118,6 +129,8 @@ typedef struct mlba_problem_desc {
   const double* obs_sigma;                /* num_observations, pixels, > 0 */
   const uint8_t* vertex_constant;         /* num_vertices, nullable (all variable) */
   const uint8_t* landmark_constant;       /* num_landmarks, nullable (all variable) */
+  const int32_t* camera_model;            /* num_cameras, MLBA_CAMERA_*, nullable (all pinhole) */
+  const double* camera_xi;                /* num_cameras, unified-projection xi >= 0, nullable */
 } mlba_problem_desc;
 
 /*

@@ -25,6 +25,8 @@ MLBA_EDEVICE = -3
 MLBA_ESTATE = -4
 
 DIST_NONE, DIST_RADTAN, DIST_EQUIDISTANT, DIST_FISHEYE = 0, 1, 2, 3
+CAMERA_PINHOLE, CAMERA_UNIFIED_PROJECTION = 0, 1
+EVAL_WITH_XI = 2
 LOCAL_KEYFRAME, LOCAL_MISSION, GLOBAL = 0, 1, 2
 LOSS_MAPLAB_HUBER, LOSS_CAUCHY, LOSS_TRIVIAL = 0, 1, 2
 SOLVER_SCHUR_PCG = 0
@@ -47,7 +49,7 @@ class ProblemDesc(ctypes.Structure):
         ("camera_distortion_type", c_vp), ("camera_resolution", c_vp), ("vertex_mission", c_vp),
         ("landmark_store_vertex", c_vp), ("obs_vertex", c_vp), ("obs_landmark", c_vp),
         ("obs_camera", c_vp), ("obs_keypoint", c_vp), ("obs_sigma", c_vp),
-        ("vertex_constant", c_vp), ("landmark_constant", c_vp),
+        ("vertex_constant", c_vp), ("landmark_constant", c_vp), ("camera_model", c_vp), ("camera_xi", c_vp),
     ]
 
 

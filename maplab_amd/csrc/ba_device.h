@@ -32,8 +32,9 @@ struct DevCamera {
   double intr[4];
   double dist[4];
   double width, height;
+  double xi;          // unified-projection mirror parameter (0 for pinhole)
   int32_t dist_type;
-  int32_t pad;
+  int32_t model;      // kCameraPinhole / kCameraUnifiedProjection
 };
 
 // Read-only problem topology (observation order = landmark-sorted).

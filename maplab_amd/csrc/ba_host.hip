@@ -1161,6 +1161,16 @@ int32_t mlba_create(const mlba_problem_desc* d, mlba_handle** out) {
         return set_err(h, MLBA_EINVAL, "fisheye distortion w outside [0.5, 1.5]");
     }
     if (!valid_quaternion(d->camera_q_CI__C_p_CI + 7 * c)) return set_err(h, MLBA_ESTATE, "camera quaternion invalid");
+    const int model = d->camera_model ? d->camera_model[c] : MLBA_CAMERA_PINHOLE;
+    if (model != MLBA_CAMERA_PINHOLE && model != MLBA_CAMERA_UNIFIED_PROJECTION)
+      return set_err(h, MLBA_EINVAL, "unknown camera model");
+    if (model == MLBA_CAMERA_UNIFIED_PROJECTION) {
+      // UnifiedProjectionCamera::areParametersValid (camera-unified-projection.cc:294-301).
+      const double xi = d->camera_xi ? d->camera_xi[c] : 0.0;
+      const double* in = d->camera_intrinsics + 4 * c;
+      if (!(std::isfinite(xi) && xi >= 0.0 && in[0] > 0.0 && in[1] > 0.0 && in[2] > 0.0 && in[3] > 0.0))
+        return set_err(h, MLBA_EINVAL, "unified-projection camera needs xi >= 0 and positive fu fv cu cv");
+    }
   }
   for (int m = 0; m < M; ++m)
     if (!valid_quaternion(d->baseframe_q_GM__G_p_GM + 7 * m)) return set_err(h, MLBA_ESTATE, "baseframe quaternion invalid");
@@ -1220,6 +1230,8 @@ int32_t mlba_create(const mlba_problem_desc* d, mlba_handle** out) {
     k.width = d->camera_resolution[2 * c];
     k.height = d->camera_resolution[2 * c + 1];
     k.dist_type = d->camera_distortion_type[c];
+    k.model = d->camera_model ? d->camera_model[c] : MLBA_CAMERA_PINHOLE;
+    k.xi = (k.model == MLBA_CAMERA_UNIFIED_PROJECTION && d->camera_xi) ? d->camera_xi[c] : 0.0;
   }
   std::vector<double> mT(12 * (size_t)M);
   for (int m = 0; m < M; ++m) {
@@ -1332,16 +1344,17 @@ int32_t mlba_landmark_quality(mlba_handle* h, const mlba_quality_options* o, uin
 int32_t mlba_evaluate(mlba_handle* h, int32_t flags, const mlba_options* options, double* residuals,
                       double* const* jacobians, double* cost) {
   if (!h) return MLBA_EINVAL;
-  if (flags != MLBA_EVAL_AMBIENT && flags != MLBA_EVAL_TANGENT) return set_err(h, MLBA_EINVAL, "unknown flags");
-  const bool tangent = flags == MLBA_EVAL_TANGENT;
-  static const int amb[9] = {3, 7, 7, 7, 7, 4, 3, 4, 4};
-  static const int tng[9] = {3, 6, 6, 6, 6, 3, 3, 4, 4};
+  if (flags & ~(MLBA_EVAL_TANGENT | MLBA_EVAL_WITH_XI)) return set_err(h, MLBA_EINVAL, "unknown flags");
+  const bool tangent = (flags & MLBA_EVAL_TANGENT) != 0;
+  const int nblocks = (flags & MLBA_EVAL_WITH_XI) ? 10 : 9;
+  static const int amb[10] = {3, 7, 7, 7, 7, 4, 3, 4, 4, 1};
+  static const int tng[10] = {3, 6, 6, 6, 6, 3, 3, 4, 4, 1};
   const int* sz = tangent ? tng : amb;
   int stride = 0;
   uint32_t mask = 0;
-  for (int b = 0; b < 9; ++b) {
-    stride += 2 * sz[b];
-    if (jacobians && jacobians[b]) mask |= 1u << b;
+  for (int b = 0; b < 10; ++b) {
+    stride += 2 * sz[b];  // the device record always has room for block 9
+    if (b < nblocks && jacobians && jacobians[b]) mask |= 1u << b;
   }
   hipStream_t s = h->stream;
   launch_vertex_prep(s, h->P, h->vert.p, h->vR.p);
@@ -1375,7 +1388,7 @@ int32_t mlba_evaluate(mlba_handle* h, int32_t flags, const mlba_options* options
   HIP_OK(hipStreamSynchronize(s));
   if (mask) {
     int off = 0;
-    for (int b = 0; b < 9; ++b) {
+    for (int b = 0; b < nblocks; ++b) {
       const int n = 2 * sz[b];
       if (jacobians[b])
         for (int o = 0; o < h->N; ++o)
@@ -1450,7 +1463,7 @@ static int reject_outliers(mlba_handle* h, const mlba_options* o, int* removed) 
     if (pC[2] <= 0.0) { bad[l] = 1; continue; }
     if (o->use_reprojection_error) {
       double px[2], jp[6];
-      const int st = project(cam.dist_type, cam.intr, cam.dist, cam.width, cam.height, pC, px, jp);
+      const int st = project_camera(cam.model, cam.xi, cam.dist_type, cam.intr, cam.dist, cam.width, cam.height, pC, px, jp);
       double e2 = 1.7976931348623157e308;
       if (st == kVisible || st == kOutsideImage) {
         const double du = px[0] - kp[2 * k], dv = px[1] - kp[2 * k + 1];

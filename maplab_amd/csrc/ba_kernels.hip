@@ -154,7 +154,7 @@ __device__ inline void eval_observation(const DevTopology& t, int o, const doubl
   mat3_vec(cam.R_C_I, pI, pC);
   pC[0] += cam.p_C_I[0]; pC[1] += cam.p_C_I[1]; pC[2] += cam.p_C_I[2];
   double kp[2], jp[6];
-  const int32_t status = project(cam.dist_type, cam.intr, cam.dist, cam.width, cam.height, pC, kp, jp);
+  const int32_t status = project_camera(cam.model, cam.xi, cam.dist_type, cam.intr, cam.dist, cam.width, cam.height, pC, kp, jp);
   e.failed = projection_failed(status, pC, kp);
   const double sinv = t.obs_sinv[o];
   const double* meas = t.obs_kp + 2 * o;
@@ -747,7 +747,7 @@ __global__ __launch_bounds__(64) void k_landmark_quality(DevTopology t, const do
     if (q.max_reproj2 > 0.0) {
       const DevCamera& cam = t.cams[obs_camera(t.obs_meta[o])];
       double kp[2], jp[6];
-      const int st = project(cam.dist_type, cam.intr, cam.dist, cam.width, cam.height, pC, kp, jp);
+      const int st = project_camera(cam.model, cam.xi, cam.dist_type, cam.intr, cam.dist, cam.width, cam.height, pC, kp, jp);
       double e2 = 1.7976931348623157e308;
       if (st == kVisible || st == kOutsideImage) {
         const double du = kp[0] - t.obs_kp[2 * o], dv = kp[1] - t.obs_kp[2 * o + 1];
@@ -1230,7 +1230,7 @@ __global__ __launch_bounds__(256) void k_evaluate_full(DevTopology t, const doub
   mat3_vec(cam.R_C_I, pI, pC);
   for (int k = 0; k < 3; ++k) pC[k] += cam.p_C_I[k];
   double kp[2], jp[6];
-  const int32_t status = project(cam.dist_type, cam.intr, cam.dist, cam.width, cam.height, pC, kp, jp);
+  const int32_t status = project_camera(cam.model, cam.xi, cam.dist_type, cam.intr, cam.dist, cam.width, cam.height, pC, kp, jp);
   const bool failed = projection_failed(status, pC, kp);
   const double sinv = t.obs_sinv[o];
   const int32_t oo = perm[o];
@@ -1246,12 +1246,14 @@ __global__ __launch_bounds__(256) void k_evaluate_full(DevTopology t, const doub
     for (int c = 0; c < 3; ++c)
       K[r * 3 + c] = Kc[r * 3] * cam.R_C_I[c] + Kc[r * 3 + 1] * cam.R_C_I[3 + c] + Kc[r * 3 + 2] * cam.R_C_I[6 + c];
   // Block offsets inside the output record of this observation.
-  const int amb[9] = {3, 7, 7, 7, 7, 4, 3, 4, 4};
-  const int tng[9] = {3, 6, 6, 6, 6, 3, 3, 4, 4};
+  // Block 9 (requested only with the xi flag) is d r / d xi of a
+  // unified-projection camera.
+  const int amb[10] = {3, 7, 7, 7, 7, 4, 3, 4, 4, 1};
+  const int tng[10] = {3, 6, 6, 6, 6, 3, 3, 4, 4, 1};
   double* rec = jout + (size_t)oo * jstride;
-  int offs[9];
+  int offs[10];
   int acc_off = 0;
-  for (int b = 0; b < 9; ++b) { offs[b] = acc_off; acc_off += 2 * (tangent ? tng[b] : amb[b]); }
+  for (int b = 0; b < 10; ++b) { offs[b] = acc_off; acc_off += 2 * (tangent ? tng[b] : amb[b]); }
   auto zero_block = [&](int b) {
     const int n = 2 * (tangent ? tng[b] : amb[b]);
     for (int k = 0; k < n; ++k) rec[offs[b] + k] = 0.0;
@@ -1291,11 +1293,12 @@ __global__ __launch_bounds__(256) void k_evaluate_full(DevTopology t, const doub
   const double* qCI = cam_q + 7 * ci;
   const int rot0 = tangent ? 0 : 0;
   (void)rot0;
-  for (int b = 0; b < 9; ++b) {
+  for (int b = 0; b < 10; ++b) {
     if (!((block_mask >> b) & 1u)) continue;
     zero_block(b);
     if (failed) continue;
     const bool used = (b == 0) || (b == 5) || (b == 6) || (b == 7) || (b == 8) ||
+                      (b == 9 && cam.model == kCameraUnifiedProjection) ||
                       ((b == 1 || b == 4) && type != kTypeKeyframe) || ((b == 2 || b == 3) && type == kTypeGlobal);
     if (!used) continue;
     const int pc0 = tangent ? 3 : 4;  // column of the position part of a pose block
@@ -1328,15 +1331,27 @@ __global__ __launch_bounds__(256) void k_evaluate_full(DevTopology t, const doub
         break;
       }
       case 8: {
-        // d kp / d distortion at the undistorted normalized point, rows x f.
+        // d kp / d distortion at the undistorted normalized point (the
+        // model's projection before distortion), rows x f.
         double* J = rec + offs[8];
-        const double x = pC[0] / pC[2], y = pC[1] / pC[2];
+        double den = pC[2];
+        if (cam.model == kCameraUnifiedProjection)
+          den += cam.xi * sqrt(pC[0] * pC[0] + pC[1] * pC[1] + pC[2] * pC[2]);
+        const double x = pC[0] / den, y = pC[1] / den;
         double d[8];
         distortion_parameter_jacobian(cam.dist_type, cam.dist, x, y, d);
         for (int k = 0; k < 4; ++k) {
           J[k] = d[k] * cam.intr[0] * sinv;
           J[4 + k] = d[4 + k] * cam.intr[1] * sinv;
         }
+        break;
+      }
+      case 9: {
+        // d kp / d xi (camera-unified-projection.cc:176-189).
+        double kx[2], jx[6], jxi[2];
+        project_unified(cam.xi, cam.dist_type, cam.intr, cam.dist, cam.width, cam.height, pC, kx, jx, jxi);
+        rec[offs[9]] = jxi[0] * sinv;
+        rec[offs[9] + 1] = jxi[1] * sinv;
         break;
       }
     }

@@ -31,6 +31,7 @@
 namespace mlba {
 
 enum : int32_t { kDistNone = 0, kDistRadtan = 1, kDistEquidistant = 2, kDistFisheye = 3 };
+enum : int32_t { kCameraPinhole = 0, kCameraUnifiedProjection = 1 };
 enum : int32_t { kTypeKeyframe = 0, kTypeMission = 1, kTypeGlobal = 2 };
 enum : int32_t { kVisible = 0, kOutsideImage = 1, kBehind = 2, kInvalid = 3 };
 
@@ -218,6 +219,64 @@ MLBA_HD int32_t project(int32_t dist_type, const double* intr, const double* dis
   const bool inside = kp[0] >= 0.0 && kp[1] >= 0.0 && kp[0] < width && kp[1] < height;
   if (p[2] > 1e-10) return inside ? kVisible : kOutsideImage;
   return p[2] < 0.0 ? kBehind : kInvalid;
+}
+
+// Unified projection (camera-unified-projection.cc:74-196): the point is
+// projected through the unit sphere with mirror parameter xi,
+// (x, y) / (z + xi |p|), then distorted and scaled like the pinhole. Valid
+// only for z > -fov(xi) |p| with fov(xi) = xi (xi <= 1) or 1 / xi; else the
+// status is invalid and kp = 0. Visible / outside by the image box when
+// |p|^2 > 1e-20, else invalid (no behind-camera status in this model).
+// jxi (nullable) = d kp / d xi.
+MLBA_HD int32_t project_unified(double xi, int32_t dist_type, const double* intr, const double* dist, double width,
+                                double height, const double* p, double* kp, double* jp, double* jxi) {
+  const double x = p[0], y = p[1], z = p[2];
+  const double d2 = x * x + y * y + z * z;
+  const double d = sqrt(d2);
+  const double fov = (xi <= 1.0) ? xi : 1.0 / xi;
+  if (!(z > -(fov * d))) {
+    kp[0] = kp[1] = 0.0;
+    for (int k = 0; k < 6; ++k) jp[k] = 0.0;
+    if (jxi) jxi[0] = jxi[1] = 0.0;
+    return kInvalid;
+  }
+  const double rz = 1.0 / (z + xi * d);
+  double u = x * rz, v = y * rz;
+  double jd[4];
+  distort(dist_type, dist, u, v, jd);
+  // d(x rz, y rz) / dp, 2 x 3 (row-major a).
+  double a[6];
+  const double s = rz * rz / d;
+  a[0] = s * (d * z + xi * (y * y + z * z));
+  a[1] = -s * xi * x * y;
+  a[3] = a[1];
+  a[4] = s * (d * z + xi * (x * x + z * z));
+  const double s2 = s * (-xi * z - d);
+  a[2] = x * s2;
+  a[5] = y * s2;
+  const double fu = intr[0], fv = intr[1];
+  for (int c = 0; c < 3; ++c) {
+    jp[c] = fu * (jd[0] * a[c] + jd[1] * a[3 + c]);
+    jp[3 + c] = fv * (jd[2] * a[c] + jd[3] * a[3 + c]);
+  }
+  if (jxi) {
+    const double gx = -x * rz * d * rz, gy = -y * rz * d * rz;
+    jxi[0] = fu * (jd[0] * gx + jd[1] * gy);
+    jxi[1] = fv * (jd[2] * gx + jd[3] * gy);
+  }
+  kp[0] = fu * u + intr[2];
+  kp[1] = fv * v + intr[3];
+  const bool inside = kp[0] >= 0.0 && kp[1] >= 0.0 && kp[0] < width && kp[1] < height;
+  if (d2 > 1e-20) return inside ? kVisible : kOutsideImage;
+  return kInvalid;
+}
+
+// Projection by camera model.
+MLBA_HD int32_t project_camera(int32_t model, double xi, int32_t dist_type, const double* intr, const double* dist,
+                               double width, double height, const double* p, double* kp, double* jp) {
+  if (model == kCameraUnifiedProjection)
+    return project_unified(xi, dist_type, intr, dist, width, height, p, kp, jp, nullptr);
+  return project(dist_type, intr, dist, width, height, p, kp, jp);
 }
 
 // The projection-failure rule of visual-error-term-inl.h:159-166.

@@ -38,6 +38,8 @@ class VisualMap:
     landmark_constant: np.ndarray   # (L,) uint8
     true_vertices: np.ndarray | None = None
     true_landmarks: np.ndarray | None = None
+    camera_model: np.ndarray | None = None  # (C,) int32 MLBA_CAMERA_*, None: all pinhole
+    camera_xi: np.ndarray | None = None     # (C,) unified-projection xi, None: 0
     _desc: abi.ProblemDesc | None = field(default=None, repr=False)
 
     @property
@@ -72,6 +74,10 @@ class VisualMap:
         self.obs_sigma = f(self.obs_sigma, np.float64)
         self.vertex_constant = f(self.vertex_constant, np.uint8)
         self.landmark_constant = f(self.landmark_constant, np.uint8)
+        if self.camera_model is not None:
+            self.camera_model = f(self.camera_model, np.int32)
+        if self.camera_xi is not None:
+            self.camera_xi = f(self.camera_xi, np.float64)
         return self
 
     def desc(self) -> abi.ProblemDesc:
@@ -100,6 +106,8 @@ class VisualMap:
         d.obs_sigma = _ptr(self.obs_sigma)
         d.vertex_constant = _ptr(self.vertex_constant)
         d.landmark_constant = _ptr(self.landmark_constant)
+        d.camera_model = _ptr(self.camera_model)
+        d.camera_xi = _ptr(self.camera_xi)
         self._desc = d
         return d
 

@@ -74,18 +74,20 @@ class DeviceProblem:
         return good.astype(bool)
 
     def evaluate(self, tangent: bool = False, blocks=range(9), options: abi.Options | None = None,
-                 want_cost: bool = True):
+                 want_cost: bool = True, with_xi: bool = False):
         """Batched VisualReprojectionError::Evaluate: residuals (N, 2), a dict
-        block -> (N, 2, size) Jacobians, and the robust cost."""
+        block -> (N, 2, size) Jacobians, and the robust cost. With with_xi,
+        block 9 (N, 2, 1) is the unified-projection xi column."""
         n = self.map.num_observations
-        sizes = abi.TANGENT_BLOCK_SIZES if tangent else abi.AMBIENT_BLOCK_SIZES
+        sizes = (abi.TANGENT_BLOCK_SIZES if tangent else abi.AMBIENT_BLOCK_SIZES) + (1,)
+        nb = 10 if with_xi else 9
         res = np.empty((n, 2))
-        jac = {b: np.empty((n, 2, sizes[b])) for b in blocks}
-        ptrs = (ctypes.c_void_p * 9)(*[jac[b].ctypes.data if b in jac else None for b in range(9)])
+        jac = {b: np.empty((n, 2, sizes[b])) for b in blocks if b < nb}
+        ptrs = (ctypes.c_void_p * nb)(*[jac[b].ctypes.data if b in jac else None for b in range(nb)])
         cost = ctypes.c_double(0.0)
         opt = options if options is not None else abi.default_options()
-        self._check(self.lib.mlba_evaluate(self.handle, abi.EVAL_TANGENT if tangent else abi.EVAL_AMBIENT,
-                                           ctypes.byref(opt), res.ctypes.data, ptrs,
+        flags = (abi.EVAL_TANGENT if tangent else abi.EVAL_AMBIENT) | (abi.EVAL_WITH_XI if with_xi else 0)
+        self._check(self.lib.mlba_evaluate(self.handle, flags, ctypes.byref(opt), res.ctypes.data, ptrs,
                                            ctypes.byref(cost) if want_cost else None))
         return res, jac, cost.value
 

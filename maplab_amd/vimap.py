@@ -220,8 +220,10 @@ def _read_proto(path: str, cls):
 
 
 def _cameras_of(sensors: dict) -> dict:
-    """NCAMERA sensors of sensors.yaml: ncamera id -> [(intrinsics, distortion
-    type, distortion (4), resolution, T_B_C)]."""
+    """NCAMERA sensors of sensors.yaml: ncamera id -> [(intrinsics (fu fv cu
+    cv), distortion type, distortion (4), resolution, T_B_C, camera model,
+    xi)]. Camera types pinhole and unified-projection (camera.cc:201-215;
+    the latter's intrinsics are xi fu fv cu cv)."""
     out = {}
     for s in sensors.get("sensors") or []:
         if str(s.get("sensor_type", "")).upper() != "NCAMERA":
@@ -229,10 +231,14 @@ def _cameras_of(sensors: dict) -> dict:
         cams = []
         for c in s.get("cameras") or []:
             cam = c["camera"]
-            if str(cam.get("type", "pinhole")) != "pinhole":
-                raise ValueError(f"camera type {cam.get('type')!r} is not supported (pinhole only)")
+            ctype = str(cam.get("type", "pinhole"))
+            if ctype not in ("pinhole", "unified-projection"):
+                raise ValueError(f"camera type {ctype!r} is not supported (pinhole, unified-projection)")
             intr = _yaml_matrix(cam["intrinsics"]).reshape(-1) if isinstance(cam["intrinsics"], dict) \
                 else np.asarray(cam["intrinsics"], dtype=np.float64)
+            model, xi = abi.CAMERA_PINHOLE, 0.0
+            if ctype == "unified-projection":
+                model, xi, intr = abi.CAMERA_UNIFIED_PROJECTION, float(intr[0]), intr[1:]
             dist = cam.get("distortion") or {"type": "none"}
             dname = str(dist.get("type", "none"))
             if dname not in _DIST_NAMES:
@@ -247,7 +253,7 @@ def _cameras_of(sensors: dict) -> dict:
             else:
                 T_B_C = np.linalg.inv(_yaml_matrix(c["T_C_B"]))
             cams.append((intr[:4], _DIST_NAMES[dname], dpar, (int(cam["image_width"]), int(cam["image_height"])),
-                         T_B_C))
+                         T_B_C, model, xi))
         out[hex_key(str(s["id"]))] = cams
     return out
 
@@ -345,7 +351,7 @@ def load_vimap(folder: str):
         bfs[i, :4] = _positive([-t[0], -t[1], -t[2], t[3]])  # JPL q_GM = Hamilton q_G_M inverse
         bfs[i, 4:] = t[4:]
     cq = np.zeros((len(cams), 7))
-    for i, (_, _, _, _, T_B_C) in enumerate(cams):
+    for i, (_, _, _, _, T_B_C, _, _) in enumerate(cams):
         R_B_C, p_B_C = T_B_C[:3, :3], T_B_C[:3, 3]
         cq[i, :4] = _matrix_to_quat(R_B_C)       # JPL q_CI = Hamilton q_C_I inverse = q(R_B_C)
         cq[i, 4:] = -R_B_C.T @ p_B_C            # C_p_CI
@@ -364,6 +370,9 @@ def load_vimap(folder: str):
         obs_sigma=np.asarray(sg, dtype=np.float64),
         vertex_constant=np.eye(1, len(order), dtype=np.uint8).reshape(-1),
         landmark_constant=np.zeros(L, dtype=np.uint8))
+    if any(c[5] != abi.CAMERA_PINHOLE for c in cams):
+        vmap.camera_model = np.asarray([c[5] for c in cams], dtype=np.int32)
+        vmap.camera_xi = np.asarray([c[6] for c in cams], dtype=np.float64)
     return vmap.normalized(), rec
 
 
@@ -504,9 +513,14 @@ def write_vimap(vmap: VisualMap, folder: str, compress: bool = True, seed: int =
         T = np.eye(4)
         T[:3, :3] = R_C_I.T
         T[:3, 3] = -R_C_I.T @ vmap.camera_q_CI__C_p_CI[c, 4:]
-        cam = {"id": id_hex(sensor_ids[c]), "sensor_type": "CAMERA", "type": "pinhole",
+        upm = vmap.camera_model is not None and int(vmap.camera_model[c]) == abi.CAMERA_UNIFIED_PROJECTION
+        intr = vmap.camera_intrinsics[c]
+        if upm:
+            intr = np.concatenate([[vmap.camera_xi[c] if vmap.camera_xi is not None else 0.0], intr])
+        cam = {"id": id_hex(sensor_ids[c]), "sensor_type": "CAMERA",
+               "type": "unified-projection" if upm else "pinhole",
                "image_width": int(vmap.camera_resolution[c, 0]), "image_height": int(vmap.camera_resolution[c, 1]),
-               "intrinsics": _matrix_yaml(vmap.camera_intrinsics[c].reshape(4, 1))}
+               "intrinsics": _matrix_yaml(intr.reshape(-1, 1))}
         dt = int(vmap.camera_distortion_type[c])
         cam["distortion"] = {"type": names[dt]}
         if dt != abi.DIST_NONE:

@@ -267,10 +267,90 @@ void distort(int type, const double* k, double* pt, double* J) {
 
 enum { kVisible = 0, kOutside = 1, kBehind = 2, kInvalid = 3 };
 
-// camera-pinhole.cc:65-164 + camera-pinhole-inl.h:48-65 + camera-inl.h:17-25
+// Camera model of one camera (pinhole, or unified projection with its xi).
+struct CamModel {
+  int model = MLBA_CAMERA_PINHOLE;
+  double xi = 0.0;
+};
+CamModel camModel(const mlba_problem_desc* d, int c) {
+  CamModel m;
+  if (d->camera_model) m.model = d->camera_model[c];
+  if (m.model == MLBA_CAMERA_UNIFIED_PROJECTION && d->camera_xi) m.xi = d->camera_xi[c];
+  return m;
+}
+
+// camera-unified-projection.cc:74-196 (+ evaluateProjectionResult). The
+// intrinsics Jacobian is written for (fu, fv, cu, cv) in Jint (2x4) and for
+// xi in Jxi (2x1).
+int projectUnified(double xi, int dist_type, const double* intr, const double* dist, int width, int height,
+                   const double* p, double* kp, double* Jp, double* Jint, double* Jdist, double* Jxi) {
+  const double fu = intr[0], fv = intr[1], cu = intr[2], cv = intr[3];
+  const double x = p[0], y = p[1], z = p[2];
+  const double d = std::sqrt(x * x + y * y + z * z);
+  const double rz = 1.0 / (z + xi * d);
+  const double fov_parameter = (xi <= 1.0) ? xi : (1 / xi);
+  if (!(z > -(fov_parameter * d))) {
+    if (Jint) for (int i = 0; i < 8; ++i) Jint[i] = 0.0;
+    if (Jdist) for (int i = 0; i < 8; ++i) Jdist[i] = 0.0;
+    if (Jxi) Jxi[0] = Jxi[1] = 0.0;
+    if (Jp) for (int i = 0; i < 6; ++i) Jp[i] = 0.0;
+    kp[0] = kp[1] = 0.0;
+    return kInvalid;
+  }
+  kp[0] = x * rz;
+  kp[1] = y * rz;
+  double Jd[4] = {1.0, 0.0, 0.0, 1.0};
+  if (Jdist) {
+    if (dist_type == MLBA_DISTORTION_RADTAN) radtanParamJac(kp, Jdist);
+    else if (dist_type == MLBA_DISTORTION_EQUIDISTANT) equidistantParamJac(kp, Jdist);
+    else if (dist_type == MLBA_DISTORTION_FISHEYE) fisheyeParamJac(dist, kp, Jdist);
+    else for (int i = 0; i < 8; ++i) Jdist[i] = 0.0;
+    for (int i = 0; i < 4; ++i) { Jdist[i] *= fu; Jdist[4 + i] *= fv; }
+  }
+  distort(dist_type, dist, kp, (Jp || Jint || Jxi) ? Jd : nullptr);
+  if (Jp) {
+    // J (2x3, row-major) as the reference fills it, then through distortion.
+    double J[6];
+    double rz2 = rz * rz / d;
+    J[0] = rz2 * (d * z + xi * (y * y + z * z));
+    J[3] = -rz2 * xi * x * y;
+    J[1] = J[3];
+    J[4] = rz2 * (d * z + xi * (x * x + z * z));
+    rz2 = rz2 * (-xi * z - d);
+    J[2] = x * rz2;
+    J[5] = y * rz2;
+    for (int c = 0; c < 3; ++c) {
+      Jp[c] = fu * (J[c] * Jd[0] + J[3 + c] * Jd[1]);
+      Jp[3 + c] = fv * (J[c] * Jd[2] + J[3 + c] * Jd[3]);
+    }
+  }
+  if (Jxi) {
+    const double jx = -x * rz * d * rz, jy = -y * rz * d * rz;
+    Jxi[0] = fu * (Jd[0] * jx + Jd[1] * jy);
+    Jxi[1] = fv * (Jd[2] * jx + Jd[3] * jy);
+  }
+  if (Jint) {
+    Jint[0] = kp[0]; Jint[1] = 0.0; Jint[2] = 1.0; Jint[3] = 0.0;
+    Jint[4] = 0.0; Jint[5] = kp[1]; Jint[6] = 0.0; Jint[7] = 1.0;
+  }
+  kp[0] = fu * kp[0] + cu;
+  kp[1] = fv * kp[1] + cv;
+  const bool visible = kp[0] >= 0.0 && kp[1] >= 0.0 && kp[0] < static_cast<double>(width) &&
+                       kp[1] < static_cast<double>(height);
+  const double d2 = x * x + y * y + z * z, min_depth2 = 1e-10 * 1e-10;  // kMinimumDepth
+  if (visible && d2 > min_depth2) return kVisible;
+  if (!visible && d2 > min_depth2) return kOutside;
+  return kInvalid;
+}
+
+// camera-pinhole.cc:65-164 + camera-pinhole-inl.h:48-65 + camera-inl.h:17-25;
+// a unified-projection camera goes to projectUnified.
 int project3Functional(int dist_type, const double* intr, const double* dist, int width,
                        int height, const double* p, double* kp, double* Jp, double* Jint,
-                       double* Jdist) {
+                       double* Jdist, const CamModel& cm = CamModel(), double* Jxi = nullptr) {
+  if (cm.model == MLBA_CAMERA_UNIFIED_PROJECTION)
+    return projectUnified(cm.xi, dist_type, intr, dist, width, height, p, kp, Jp, Jint, Jdist, Jxi);
+  if (Jxi) Jxi[0] = Jxi[1] = 0.0;
   const double fu = intr[0], fv = intr[1], cu = intr[2], cv = intr[3];
   const double x = p[0], y = p[1], z = p[2];
   const double rz = 1.0 / z;
@@ -314,7 +394,8 @@ int project3Functional(int dist_type, const double* intr, const double* dist, in
 // visual-error-term-inl.h:13-379. params in the reference block order;
 // jac entries nullable (ambient, row-major). Returns true on projection failure.
 bool evaluateVisual(const double* meas, double sigma_inv, int type, int dist_type, int width,
-                    int height, const double* const* params, double* res, double* const* jac) {
+                    int height, const double* const* params, double* res, double* const* jac,
+                    const CamModel& cm = CamModel(), double* jac_xi = nullptr) {
   const double* p_B = params[0];
   const double* q_B_LM = params[1];
   const double* p_LM_B = params[1] + 4;
@@ -362,12 +443,13 @@ bool evaluateVisual(const double* meas, double sigma_inv, int type, int dist_typ
   for (int i = 0; i < 3; ++i) p_C[i] += p_C_I[i];
 
   const bool want_j = jac != nullptr;
-  double kp[2], Jkp[6], Jint[8], Jdist[8];
+  double kp[2], Jkp[6], Jint[8], Jdist[8], Jxi[2];
   const int status = project3Functional(dist_type, intr, dist, width, height, p_C, kp,
                                         want_j ? Jkp : nullptr,
                                         (want_j && jac[7]) ? Jint : nullptr,
                                         (want_j && jac[8] && dist_type != MLBA_DISTORTION_NONE)
-                                            ? Jdist : nullptr);
+                                            ? Jdist : nullptr,
+                                        cm, jac_xi ? Jxi : nullptr);
   const bool failed = status == kBehind || status == kInvalid || p_C[2] < 0.05 ||
                       (kp[0] * kp[0] + kp[1] * kp[1]) > 1.0e5 * 1.0e5;
 
@@ -435,6 +517,10 @@ bool evaluateVisual(const double* meas, double sigma_inv, int type, int dist_typ
       for (int i = 0; i < 8; ++i)
         jac[8][i] = (failed || dist_type == MLBA_DISTORTION_NONE) ? 0.0 : Jdist[i] * sigma_inv;
     }
+  }
+  if (jac_xi) {
+    jac_xi[0] = failed ? 0.0 : Jxi[0] * sigma_inv;
+    jac_xi[1] = failed ? 0.0 : Jxi[1] * sigma_inv;
   }
   if (!failed) {
     res[0] = (kp[0] - meas[0]) * sigma_inv;
@@ -648,7 +734,7 @@ void linearize(const Problem& pb, const mlba_options& o, int nthreads, bool with
       const double sigma = d->obs_sigma[i];
       evaluateVisual(d->obs_keypoint + 2 * i, 1.0 / sigma, type, d->camera_distortion_type[c],
                      d->camera_resolution[2 * c], d->camera_resolution[2 * c + 1], params, r,
-                     with_jacobian ? jac : nullptr);
+                     with_jacobian ? jac : nullptr, camModel(d, c));
       const double sq = r[0] * r[0] + r[1] * r[1];
       double rho[3];
       lossEvaluate(o.loss_type, lossScale(o, type, sigma), sq, rho);
@@ -1377,7 +1463,7 @@ std::vector<int> findOutlierLandmarks(const Problem& pb, const mlba_options& o) 
       double kp[2];
       const int st = project3Functional(d->camera_distortion_type[c], params[7], params[8],
                                         d->camera_resolution[2 * c], d->camera_resolution[2 * c + 1],
-                                        p_C, kp, nullptr, nullptr, nullptr);
+                                        p_C, kp, nullptr, nullptr, nullptr, camModel(d, c));
       double e2 = std::numeric_limits<double>::max();
       if (st == kVisible || st == kOutside) {
         const double du = kp[0] - d->obs_keypoint[2 * i], dv = kp[1] - d->obs_keypoint[2 * i + 1];
@@ -1457,6 +1543,15 @@ int32_t oracle_project(int32_t dist_type, const double* intrinsics, const double
   return project3Functional(dist_type, intrinsics, dist, width, height, p_C, kp, jp, jint, jdist);
 }
 
+int32_t oracle_project_camera(int32_t model, double xi, int32_t dist_type, const double* intrinsics,
+                              const double* dist, int32_t width, int32_t height, const double* p_C, double* kp,
+                              double* jp, double* jint, double* jdist, double* jxi) {
+  CamModel cm;
+  cm.model = model;
+  cm.xi = xi;
+  return project3Functional(dist_type, intrinsics, dist, width, height, p_C, kp, jp, jint, jdist, cm, jxi);
+}
+
 void oracle_distort(int32_t dist_type, const double* dist, double* point, double* jd) {
   distort(dist_type, dist, point, jd);
 }
@@ -1482,7 +1577,8 @@ int32_t oracle_problem_evaluate(const mlba_problem_desc* desc, const uint8_t* la
   Problem pb(desc, landmark_active);
   static const int amb[9] = {3, 7, 7, 7, 7, 4, 3, 4, 4};
   static const int tan[9] = {3, 6, 6, 6, 6, 3, 3, 4, 4};
-  const int* sz = flags == MLBA_EVAL_TANGENT ? tan : amb;
+  const int* sz = (flags & MLBA_EVAL_TANGENT) ? tan : amb;
+  const bool with_xi = (flags & MLBA_EVAL_WITH_XI) != 0;
   double total = 0.0;
   for (int i = 0; i < pb.N; ++i) {
     const double* params[9];
@@ -1494,10 +1590,11 @@ int32_t oracle_problem_evaluate(const mlba_problem_desc* desc, const uint8_t* la
     for (int b = 0; b < 9; ++b) jp[b] = J[b];
     if (type == MLBA_LOCAL_KEYFRAME) jp[1] = jp[2] = jp[3] = jp[4] = nullptr;
     if (type == MLBA_LOCAL_MISSION) jp[2] = jp[3] = nullptr;
-    double r[2];
+    double r[2], jxi[2];
     evaluateVisual(desc->obs_keypoint + 2 * i, 1.0 / desc->obs_sigma[i], type, desc->camera_distortion_type[c],
                    desc->camera_resolution[2 * c], desc->camera_resolution[2 * c + 1], params, r,
-                   jacobians ? jp : nullptr);
+                   jacobians ? jp : nullptr, camModel(desc, c), (jacobians && with_xi) ? jxi : nullptr);
+    if (jacobians && with_xi && jacobians[9]) { jacobians[9][2 * i] = jxi[0]; jacobians[9][2 * i + 1] = jxi[1]; }
     if (residuals) { residuals[2 * i] = r[0]; residuals[2 * i + 1] = r[1]; }
     const int l = desc->obs_landmark[i];
     if (pb.lm_active[l]) {
@@ -1510,9 +1607,9 @@ int32_t oracle_problem_evaluate(const mlba_problem_desc* desc, const uint8_t* la
       if (!jacobians[b]) continue;
       double* out = jacobians[b] + (size_t)i * 2 * sz[b];
       if (!jp[b]) { for (int t = 0; t < 2 * sz[b]; ++t) out[t] = 0.0; continue; }
-      if (flags == MLBA_EVAL_TANGENT && (b >= 1 && b <= 4)) {
+      if ((flags & MLBA_EVAL_TANGENT) && (b >= 1 && b <= 4)) {
         liftPose(J[b], params[b], out);
-      } else if (flags == MLBA_EVAL_TANGENT && b == 5) {
+      } else if ((flags & MLBA_EVAL_TANGENT) && b == 5) {
         double Jq[12];
         quatJacobian(params[5], Jq);
         for (int rr = 0; rr < 2; ++rr)
@@ -1643,7 +1740,8 @@ int32_t oracle_landmark_quality(const mlba_problem_desc* d, const mlba_quality_o
         double kp[2];
         const int st = project3Functional(d->camera_distortion_type[c], d->camera_intrinsics + 4 * c,
                                           d->camera_distortion + 4 * c, d->camera_resolution[2 * c],
-                                          d->camera_resolution[2 * c + 1], p_C, kp, nullptr, nullptr, nullptr);
+                                          d->camera_resolution[2 * c + 1], p_C, kp, nullptr, nullptr, nullptr,
+                                          camModel(d, c));
         double e2 = std::numeric_limits<double>::max();
         if (st == kVisible || st == kOutside) {
           const double du = kp[0] - d->obs_keypoint[2 * i], dv = kp[1] - d->obs_keypoint[2 * i + 1];

@@ -34,6 +34,13 @@ int32_t oracle_project(int32_t dist_type, const double* intrinsics, const double
                        int32_t width, int32_t height, const double* p_C, double* kp,
                        double* jp, double* jint, double* jdist);
 
+/* The same for a camera model (MLBA_CAMERA_*; xi of a unified-projection
+ * camera): UnifiedProjectionCamera::project3Functional
+ * (camera-unified-projection.cc:74-196). jxi (2x1, nullable) = d kp / d xi. */
+int32_t oracle_project_camera(int32_t model, double xi, int32_t dist_type, const double* intrinsics,
+                              const double* dist, int32_t width, int32_t height, const double* p_C, double* kp,
+                              double* jp, double* jint, double* jdist, double* jxi);
+
 /* Distortion::distortUsingExternalCoefficients: in/out point, jd (2x2) nullable. */
 void oracle_distort(int32_t dist_type, const double* dist, double* point, double* jd);
 
@@ -53,7 +60,8 @@ int32_t oracle_evaluate_visual(const double* meas, double sigma, int32_t error_t
 /* ceres::LossFunction::Evaluate for the supported losses: rho[3]. */
 void oracle_loss(int32_t loss_type, double a, double s, double* rho);
 
-/* Batched evaluate over a problem, same contract as mlba_evaluate. */
+/* Batched evaluate over a problem, same contract as mlba_evaluate (flags
+ * MLBA_EVAL_TANGENT and MLBA_EVAL_WITH_XI). */
 int32_t oracle_problem_evaluate(const mlba_problem_desc* desc, const uint8_t* landmark_active,
                                 int32_t flags, const mlba_options* options,
                                 double* residuals, double* const* jacobians, double* cost);

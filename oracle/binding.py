@@ -34,6 +34,7 @@ def load():
     P = ctypes.POINTER
     sig = {
         "oracle_project": (i32, [i32, vp, vp, i32, i32, vp, vp, vp, vp, vp]),
+        "oracle_project_camera": (i32, [i32, f64, i32, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp]),
         "oracle_distort": (None, [i32, vp, vp, vp]),
         "oracle_quat_plus": (None, [vp, vp, vp]),
         "oracle_quat_jacobian": (None, [vp, vp]),
@@ -70,6 +71,21 @@ def project(dist_type, intrinsics, dist, width, height, p_C, want_jac=True):
                             _p(jp) if want_jac else None, _p(ji) if want_jac else None,
                             _p(jd) if want_jac else None)
     return st, kp, jp, ji, jd
+
+
+def project_camera(model, xi, dist_type, intrinsics, dist, width, height, p_C, want_jac=True):
+    """(status, kp, d kp/d p_C, d kp/d (fu fv cu cv), d kp/d distortion, d kp/d xi)."""
+    lib = load()
+    intr = np.ascontiguousarray(intrinsics, dtype=np.float64)
+    d = np.ascontiguousarray(dist if dist is not None else np.zeros(4), dtype=np.float64)
+    p = np.ascontiguousarray(p_C, dtype=np.float64)
+    kp = np.zeros(2)
+    jp, ji, jd, jx = np.zeros((2, 3)), np.zeros((2, 4)), np.zeros((2, 4)), np.zeros(2)
+    w = want_jac
+    st = lib.oracle_project_camera(model, xi, dist_type, _p(intr), _p(d), width, height, _p(p), _p(kp),
+                                   _p(jp) if w else None, _p(ji) if w else None, _p(jd) if w else None,
+                                   _p(jx) if w else None)
+    return st, kp, jp, ji, jd, jx
 
 
 def distort(dist_type, dist, point):
@@ -126,18 +142,20 @@ def loss(loss_type, a, s):
     return rho
 
 
-def problem_evaluate(vmap, tangent=False, options=None, active=None):
+def problem_evaluate(vmap, tangent=False, options=None, active=None, with_xi=False):
     from maplab_amd import abi
     lib = load()
     d = vmap.desc()
     n = vmap.num_observations
-    sizes = abi.TANGENT_BLOCK_SIZES if tangent else abi.AMBIENT_BLOCK_SIZES
+    sizes = (abi.TANGENT_BLOCK_SIZES if tangent else abi.AMBIENT_BLOCK_SIZES) + (1,)
+    nb = 10 if with_xi else 9
     res = np.zeros((n, 2))
-    jac = {b: np.zeros((n, 2, sizes[b])) for b in range(9)}
-    jp = (ctypes.c_void_p * 9)(*[jac[b].ctypes.data for b in range(9)])
+    jac = {b: np.zeros((n, 2, sizes[b])) for b in range(nb)}
+    jp = (ctypes.c_void_p * nb)(*[jac[b].ctypes.data for b in range(nb)])
     cost = ctypes.c_double(0.0)
     act = None if active is None else np.ascontiguousarray(active, dtype=np.uint8)
-    rc = lib.oracle_problem_evaluate(ctypes.byref(d), _p(act), 1 if tangent else 0,
+    flags = (1 if tangent else 0) | (abi.EVAL_WITH_XI if with_xi else 0)
+    rc = lib.oracle_problem_evaluate(ctypes.byref(d), _p(act), flags,
                                      ctypes.byref(options) if options is not None else None,
                                      _p(res), jp, ctypes.byref(cost))
     assert rc == 0, rc

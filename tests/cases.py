@@ -64,9 +64,10 @@ def observation_case(rng, error_type, dist_type, bf_translation=True):
     return meas, sigma, params
 
 
-def tiny_map(rng, dist_type=1, n_vertices=6, n_landmarks=40, global_missions=False, noise=0.5):
+def tiny_map(rng, dist_type=1, n_vertices=6, n_landmarks=40, global_missions=False, noise=0.5, xi=None):
     """A small map whose landmarks are stored in their first observer; with
-    global_missions, odd vertices live in a second mission (global error type)."""
+    global_missions, odd vertices live in a second mission (global error type);
+    with xi, the camera is a unified-projection camera of that xi."""
     verts = []
     for k in range(n_vertices):
         ang = 2 * np.pi * k / n_vertices
@@ -104,7 +105,7 @@ def tiny_map(rng, dist_type=1, n_vertices=6, n_landmarks=40, global_missions=Fal
             p_C = p_I + cam[4:]
             if p_C[2] < 0.5:
                 continue
-            kp = ind.project(dist_type, INTR, CAMERAS[dist_type], p_C)
+            kp = ind.project(dist_type, INTR, CAMERAS[dist_type], p_C, xi=xi)
             if not (0 <= kp[0] < WIDTH and 0 <= kp[1] < HEIGHT):
                 continue
             obs_v.append(k); obs_l.append(l); kps.append(kp + rng.normal(size=2) * noise)
@@ -129,7 +130,9 @@ def tiny_map(rng, dist_type=1, n_vertices=6, n_landmarks=40, global_missions=Fal
         landmark_store_vertex=store, obs_vertex=np.array(obs_v, dtype=np.int32),
         obs_landmark=np.array(obs_l, dtype=np.int32), obs_camera=np.zeros(len(obs_v), dtype=np.int32),
         obs_keypoint=np.array(kps), obs_sigma=np.full(len(obs_v), 0.5), vertex_constant=vconst,
-        landmark_constant=lconst, true_vertices=verts.copy(), true_landmarks=lms.copy())
+        landmark_constant=lconst, true_vertices=verts.copy(), true_landmarks=lms.copy(),
+        camera_model=None if xi is None else np.array([1], dtype=np.int32),
+        camera_xi=None if xi is None else np.array([xi]))
     # corrupt
     m.vertices = m.vertices.copy()
     for k in range(1, n_vertices):

@@ -77,8 +77,15 @@ def distort_fov(k, x, y):
     return x * f, y * f
 
 
-def project(dist_type, intr, dist, p_C):
-    x, y = p_C[0] / p_C[2], p_C[1] / p_C[2]
+def project(dist_type, intr, dist, p_C, xi=None):
+    """Pinhole, or with xi the unified (sphere) model of Geyer & Daniilidis /
+    Mei & Rives: the point is projected through the unit sphere, (x, y) /
+    (z + xi |p|)."""
+    if xi is None:
+        x, y = p_C[0] / p_C[2], p_C[1] / p_C[2]
+    else:
+        den = p_C[2] + xi * np.linalg.norm(p_C)
+        x, y = p_C[0] / den, p_C[1] / den
     if dist_type == 1:
         x, y = distort_radtan(dist, x, y)
     elif dist_type == 2:

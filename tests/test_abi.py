@@ -54,7 +54,7 @@ def test_synth_library_exports_every_declared_symbol():
 
 def test_struct_layouts_match_header():
     # Field offsets are what the C compiler produces for the declarations.
-    assert ctypes.sizeof(abi.ProblemDesc) == 24 + 17 * 8
+    assert ctypes.sizeof(abi.ProblemDesc) == 24 + 19 * 8
     assert ctypes.sizeof(abi.IterationSummary) == 16 + 8 * 8
     assert abi.Summary.kernel_launches.offset == ctypes.sizeof(abi.Summary) - 4 * abi.NUM_KERNELS
     assert abi.Summary.kernel_ms.offset == abi.Summary.iterations.offset + 8
@@ -137,6 +137,11 @@ def _create(m):
     (lambda m: m.vertices.__setitem__((1, 3), -0.5), abi.MLBA_ESTATE, "quaternion"),
     (lambda m: m.landmark_store_vertex.__setitem__(0, 99), abi.MLBA_EINVAL, "store_vertex"),
     (lambda m: m.camera_distortion_type.__setitem__(0, 7), abi.MLBA_EINVAL, "distortion"),
+    (lambda m: (m.camera_distortion_type.__setitem__(0, 3), m.camera_distortion.__setitem__((0, 0), 0.3)),
+     abi.MLBA_EINVAL, "fisheye"),
+    (lambda m: setattr(m, "camera_model", np.array([5], np.int32)), abi.MLBA_EINVAL, "camera model"),
+    (lambda m: (setattr(m, "camera_model", np.array([1], np.int32)), setattr(m, "camera_xi", np.array([-0.1]))),
+     abi.MLBA_EINVAL, "unified-projection"),
 ])
 def test_invalid_problem_is_rejected(mutate, code, text):
     # The reference CHECK-fails on these (visual-error-term.h:38,

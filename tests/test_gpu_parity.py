@@ -46,6 +46,36 @@ def test_evaluate_matches_oracle(oracle, dist_type, global_missions, tangent):
     assert abs(cost - ref_cost) <= 1e-12 * abs(ref_cost)
 
 
+@pytest.mark.parametrize("dist_type, xi", [(0, 0.9), (1, 1.3), (3, 0.6)])
+def test_unified_projection_evaluate_matches_oracle(oracle, dist_type, xi):
+    rng = np.random.default_rng(60 + dist_type)
+    m = tiny_map(rng, dist_type=dist_type, n_vertices=6, n_landmarks=60, global_missions=True, xi=xi)
+    print(f"unified projection evaluate: dist {dist_type} xi {xi} N {m.num_observations}", flush=True)
+    for tangent in (False, True):
+        with DeviceProblem(m) as dev:
+            res, jac, cost = dev.evaluate(tangent=tangent, blocks=range(10), with_xi=True)
+        ref_res, ref_jac, ref_cost = oracle.problem_evaluate(m, tangent=tangent, options=abi.default_options(),
+                                                             with_xi=True)
+        assert rel_err(res, ref_res) < 1e-12
+        for b in range(10):
+            if np.abs(ref_jac[b]).max() == 0:
+                assert np.abs(jac[b]).max() == 0, b
+                continue
+            assert rel_err(jac[b], ref_jac[b]) < 1e-9, (b, rel_err(jac[b], ref_jac[b]))
+        assert abs(cost - ref_cost) <= 1e-12 * abs(ref_cost)
+        assert np.abs(jac[9]).max() > 0
+
+
+def test_pinhole_xi_column_is_zero():
+    rng = np.random.default_rng(64)
+    m = tiny_map(rng, dist_type=1, n_vertices=5, n_landmarks=30)
+    with DeviceProblem(m) as dev:
+        _, jac, _ = dev.evaluate(blocks=(7, 9), with_xi=True)
+        _, jac9, _ = dev.evaluate(blocks=(7,))
+    assert np.abs(jac[9]).max() == 0.0
+    np.testing.assert_array_equal(jac[7], jac9[7])
+
+
 def test_evaluate_projection_failures_are_zero(oracle):
     rng = np.random.default_rng(31)
     m = tiny_map(rng, dist_type=1, n_vertices=5, n_landmarks=30)
@@ -74,6 +104,23 @@ def test_solve_iteration_log_matches_oracle(oracle, dist_type, global_missions, 
     ref, ref_rows, rv, rl, _ = oracle.solve(m, o, linear_solver=oracle.SCHUR_DIRECT, threads=1)
     assert sm.num_iterations == ref.num_iterations
     assert sm.termination_type == ref.termination_type
+    np.testing.assert_allclose([r.cost for r in rows], [r.cost for r in ref_rows], rtol=1e-9)
+    np.testing.assert_array_equal([r.step_is_successful for r in rows], [r.step_is_successful for r in ref_rows])
+    np.testing.assert_allclose(v, rv, atol=1e-8)
+    np.testing.assert_allclose(l, rl, atol=1e-8)
+
+
+@pytest.mark.parametrize("dist_type, xi", [(1, 0.9), (3, 1.2)])
+def test_unified_projection_solve_matches_oracle(oracle, dist_type, xi):
+    rng = np.random.default_rng(70 + dist_type)
+    m = tiny_map(rng, dist_type=dist_type, n_vertices=7, n_landmarks=80, xi=xi)
+    o = opts(max_num_iterations=10, pcg_relative_tolerance=1e-12)
+    print(f"unified projection solve: dist {dist_type} xi {xi} N {m.num_observations}", flush=True)
+    with DeviceProblem(m) as dev:
+        sm, rows = dev.solve(o)
+        v, l = dev.download_state()
+    ref, ref_rows, rv, rl, _ = oracle.solve(m, o, linear_solver=oracle.SCHUR_DIRECT, threads=1)
+    assert sm.num_iterations == ref.num_iterations
     np.testing.assert_allclose([r.cost for r in rows], [r.cost for r in ref_rows], rtol=1e-9)
     np.testing.assert_array_equal([r.step_is_successful for r in rows], [r.step_is_successful for r in ref_rows])
     np.testing.assert_allclose(v, rv, atol=1e-8)

@@ -6,7 +6,8 @@ Reference tests restated here:
   :491-520 (projection statuses), :88-93 / :136-146 / :190-199 (Jacobians vs
   numdiff); test-distortions.cc:145-150, :191-197 (distortion Jacobians);
   algorithms/ceres-error-terms/test/test_quaternion_parameterization_test.cc:13-31
-  (JPL local-parameterization Jacobian vs numdiff of Plus).
+  (JPL local-parameterization Jacobian vs numdiff of Plus); test-cameras.cc:427-487
+  (unified-projection manual projection and statuses).
 Test fixtures of the reference: test camera (400, 300, 320, 240, 640x480)
 (camera-pinhole.h:251); radtan (-0.28, 0.08, -0.00026, -0.00024)
 (distortion-radtan.h:97); equidistant (0.2, 0.01, 0.3, 0.05)
@@ -126,6 +127,75 @@ def test_fisheye_limits_follow_the_reference(oracle):
     _, _, _, _, jdist = oracle.project(FOV, TEST_INTR, [w, 0, 0, 0], 640, 480, [1e-3, 1e-3, 1.0])
     df = ((1 + np.tan(w / 2) ** 2) * w - 2 * np.tan(w / 2)) / (w * w)
     np.testing.assert_allclose(jdist[:, 0], [TEST_INTR[0] * df, TEST_INTR[1] * df], rtol=1e-12)
+
+
+UPM = 1
+UPM_TEST = np.array([400.0, 300.0, 320.0, 240.0])  # createTestCamera: xi 0.9, (400, 300, 320, 240), 640x480
+UPM_XI = 0.9
+
+
+def test_unified_projection_manual_projection_and_statuses(oracle):
+    # test-cameras.cc:427-466: keypoint of (1, 1.2, 10) to 1e-15.
+    p = np.array([1.0, 1.2, 10.0])
+    rz = 1.0 / (p[2] + UPM_XI * np.linalg.norm(p))
+    st, kp, *_ = oracle.project_camera(UPM, UPM_XI, NONE, UPM_TEST, None, 640, 480, p)
+    assert st == 0
+    np.testing.assert_allclose(kp, [400 * p[0] * rz + 320, 300 * p[1] * rz + 240], atol=1e-15, rtol=0)
+    # :468-487: visible, outside the image box, invalid (z <= -fov(xi) |p|).
+    for q, status in [((1, 1, 10), 0), ((0.0, 10, -10), 1), ((0, 0, -10), 3)]:
+        st, kp, *_ = oracle.project_camera(UPM, UPM_XI, NONE, UPM_TEST, None, 640, 480, q)
+        assert st == status, q
+    st, kp, *_ = oracle.project_camera(UPM, UPM_XI, NONE, UPM_TEST, None, 640, 480, [0, 0, -10])
+    np.testing.assert_array_equal(kp, [0.0, 0.0])
+
+
+def test_unified_projection_closed_forms(oracle):
+    # xi = 0 is the pinhole; the optical axis maps to (cu, cv) for every xi;
+    # with xi = 1 a ray at 90 degrees lands at unit normalized radius; the
+    # fov limit uses 1 / xi above xi = 1.
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        p = np.array([rng.uniform(-2, 2), rng.uniform(-2, 2), rng.uniform(3, 10)])
+        for dt, dist in [(NONE, np.zeros(4)), (RAD, RADTAN), (FOV, FISHEYE)]:
+            a = oracle.project_camera(UPM, 0.0, dt, UPM_TEST, dist, 640, 480, p)
+            b = oracle.project(dt, UPM_TEST, dist, 640, 480, p)
+            np.testing.assert_allclose(a[1], b[1], rtol=1e-14)
+            np.testing.assert_allclose(a[2], b[2], rtol=1e-12, atol=1e-12)
+            np.testing.assert_allclose(a[3], b[3], rtol=1e-14, atol=1e-15)
+    for xi in (0.3, 0.9, 1.7):
+        _, kp, *_ = oracle.project_camera(UPM, xi, NONE, UPM_TEST, None, 640, 480, [0, 0, 4.0])
+        np.testing.assert_allclose(kp, [320, 240], atol=1e-13)
+    _, kp, *_ = oracle.project_camera(UPM, 1.0, NONE, UPM_TEST, None, 640, 480, [2.5, 0, 0])
+    np.testing.assert_allclose(kp, [400 + 320, 240], rtol=1e-15)
+    # xi = 1.6: fov parameter 1/1.6; z = -0.6 |p| is valid, z = -0.65 |p| is not.
+    for zf, status in [(-0.6, 1), (-0.65, 3)]:
+        q = np.array([np.sqrt(1 - zf * zf), 0.0, zf])
+        st, *_ = oracle.project_camera(UPM, 1.6, NONE, UPM_TEST, None, 640, 480, q)
+        assert st == status, zf
+    # The independent sphere model.
+    for _ in range(20):
+        p = rng.normal(size=3) + [0, 0, 3]
+        for dt, dist in [(NONE, np.zeros(4)), (RAD, RADTAN), (EQD, EQUI), (FOV, FISHEYE)]:
+            _, kp, *_ = oracle.project_camera(UPM, 1.3, dt, UPM_TEST, dist, 640, 480, p, False)
+            np.testing.assert_allclose(kp, ind.project(dt, UPM_TEST, dist, p, xi=1.3), rtol=1e-13)
+
+
+@pytest.mark.parametrize("dist_type, dist", [(NONE, np.zeros(4)), (RAD, RADTAN), (EQD, EQUI), (FOV, FISHEYE)])
+@pytest.mark.parametrize("xi", [0.3, 0.9, 1.4])
+def test_unified_projection_jacobians_numdiff(oracle, dist_type, dist, xi):
+    rng = np.random.default_rng(11)
+    for _ in range(8):
+        p = np.array([rng.uniform(-3, 3), rng.uniform(-2, 2), rng.uniform(2, 10)])
+        st, kp, jp, ji, jd, jx = oracle.project_camera(UPM, xi, dist_type, UPM_TEST, dist, 640, 480, p)
+        f_p = lambda d: oracle.project_camera(UPM, xi, dist_type, UPM_TEST, dist, 640, 480, p + d, False)[1]
+        f_i = lambda d: oracle.project_camera(UPM, xi, dist_type, UPM_TEST + d, dist, 640, 480, p, False)[1]
+        f_x = lambda d: oracle.project_camera(UPM, xi + d[0], dist_type, UPM_TEST, dist, 640, 480, p, False)[1]
+        np.testing.assert_allclose(jp, ind.numdiff(f_p, 3, 1e-4), rtol=1e-7, atol=1e-8)
+        np.testing.assert_allclose(ji, ind.numdiff(f_i, 4, 1e-4), rtol=1e-7, atol=1e-8)
+        np.testing.assert_allclose(jx, ind.numdiff(f_x, 1, 1e-6)[:, 0], rtol=1e-7, atol=1e-7)
+        if dist_type != NONE:
+            f_d = lambda d: oracle.project_camera(UPM, xi, dist_type, UPM_TEST, dist + d, 640, 480, p, False)[1]
+            np.testing.assert_allclose(jd, ind.numdiff(f_d, 4, 1e-5), rtol=1e-6, atol=1e-7)
 
 
 def test_rotation_is_transposed_hamilton(oracle):

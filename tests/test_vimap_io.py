@@ -143,3 +143,27 @@ def test_solving_the_read_map_matches_the_original(written):
     b, brows, _, _, _ = oracle.solve(r, o, threads=4)
     assert [x.step_is_successful for x in arows] == [x.step_is_successful for x in brows]
     np.testing.assert_allclose([x.cost for x in brows], [x.cost for x in arows], rtol=1e-9)
+
+
+def test_unified_projection_camera_round_trip(tmp_path):
+    # sensors.yaml camera type "unified-projection" with intrinsics
+    # (xi, fu, fv, cu, cv) (camera.cc:201-215).
+    from cases import tiny_map
+    m = tiny_map(np.random.default_rng(9), dist_type=1, n_vertices=5, n_landmarks=30, xi=0.8)
+    folder = str(tmp_path / "upm")
+    vimap.write_vimap(m, folder)
+    import yaml
+    with open(os.path.join(folder, vimap.FOLDER, vimap.SENSORS)) as f:
+        cam = yaml.safe_load(f)["sensors"][0]["cameras"][0]["camera"]
+    assert cam["type"] == "unified-projection"
+    assert cam["intrinsics"]["rows"] == 5 and cam["intrinsics"]["data"][0] == 0.8
+    r, _ = vimap.load_vimap(folder)
+    np.testing.assert_array_equal(r.camera_model, [abi.CAMERA_UNIFIED_PROJECTION])
+    np.testing.assert_array_equal(r.camera_xi, [0.8])
+    np.testing.assert_array_equal(r.camera_intrinsics, m.camera_intrinsics)
+    perm = landmark_permutation(r, m)
+    for a, b in zip(sorted_obs(r, perm), sorted_obs(m)):
+        np.testing.assert_array_equal(a, b)
+    c_read = oracle.problem_evaluate(r)[2]
+    c_orig = oracle.problem_evaluate(m)[2]
+    assert abs(c_read - c_orig) <= 1e-12 * c_orig
