@@ -272,6 +272,34 @@ int32_t mlba_evaluate(mlba_handle* handle, int32_t flags, const mlba_options* op
 /* ceres::Solve on the visual problem (map_optimization::solve, solver.cc:14-44). */
 int32_t mlba_solve(mlba_handle* handle, const mlba_options* options, mlba_summary* summary);
 
+/*
+ * Camera calibration refinement with the map held constant: the blocks the
+ * visual error term takes for each camera (q_CI, C_p_CI, intrinsics with xi,
+ * distortion; visual-error-term-inl.h:321-367) are optimized with the Ceres
+ * LM rules of mlba_solve, on dense normal equations of at most 15 columns
+ * per camera, while every vertex and landmark in the problem is constant
+ * (MLBA_EINVAL otherwise; refining calibration jointly with the map is not
+ * supported). The fix flags mirror --ba_fix_ncamera_intrinsics (here split
+ * into intrinsics and distortion), --ba_fix_ncamera_extrinsics_rotation and
+ * --ba_fix_ncamera_extrinsics_translation (vi-optimization-builder.cc:24-33,
+ * optimization-terms-addition.cc:222-234). The refined cameras are used by
+ * every later call on the handle and read back with mlba_download_cameras.
+ */
+typedef struct mlba_calibration_options {
+  int32_t fix_intrinsics;             /* fu fv cu cv (and xi of a unified-projection camera) */
+  int32_t fix_distortion;
+  int32_t fix_extrinsics_rotation;    /* q_CI */
+  int32_t fix_extrinsics_translation; /* C_p_CI */
+} mlba_calibration_options;
+
+void mlba_calibration_options_init(mlba_calibration_options* options); /* all fixed (maplab's defaults) */
+int32_t mlba_solve_calibration(mlba_handle* handle, const mlba_options* options,
+                               const mlba_calibration_options* calibration, mlba_summary* summary);
+/* Current camera blocks: 7 (q_CI | C_p_CI), 4 intrinsics, 4 distortion and
+ * xi per camera; every pointer nullable. */
+int32_t mlba_download_cameras(const mlba_handle* handle, double* q_CI__C_p_CI, double* intrinsics,
+                              double* distortion, double* xi);
+
 /* Several independent problems solved concurrently: handle i runs
  * mlba_solve (or, with outlier_rejection, mlba_solve_with_outlier_rejection)
  * with options[i] and summaries[i] (nullable), each on its own stream; up to

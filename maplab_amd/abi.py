@@ -124,6 +124,11 @@ class KernelTimes(ctypes.Structure):
             "pcg_window_columns")]
 
 
+class CalibrationOptions(ctypes.Structure):
+    _fields_ = [("fix_intrinsics", c_i32), ("fix_distortion", c_i32), ("fix_extrinsics_rotation", c_i32),
+                ("fix_extrinsics_translation", c_i32)]
+
+
 class Schedule(ctypes.Structure):
     _fields_ = [("item_band_rows", c_i32), ("item_waves", c_i32), ("xcd_mapping", c_i32),
                 ("schedules_tried", c_i32), ("default_ms", c_f64), ("tuned_ms", c_f64)]
@@ -186,6 +191,10 @@ LIB_SYMBOLS = {
                                 ctypes.POINTER(ctypes.POINTER(Summary)), ctypes.POINTER(c_i32), c_i32, c_i32,
                                 c_i32]),
     "mlba_time_kernels": (c_i32, [c_vp, ctypes.POINTER(Options), c_i32, ctypes.POINTER(KernelTimes)]),
+    "mlba_calibration_options_init": (None, [ctypes.POINTER(CalibrationOptions)]),
+    "mlba_solve_calibration": (c_i32, [c_vp, ctypes.POINTER(Options), ctypes.POINTER(CalibrationOptions),
+                                       ctypes.POINTER(Summary)]),
+    "mlba_download_cameras": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     "mlba_autotune": (c_i32, [c_vp, ctypes.POINTER(Options), c_i32, ctypes.POINTER(Schedule)]),
     "mlba_set_schedule": (c_i32, [c_vp, ctypes.POINTER(Schedule)]),
     "mlba_get_schedule": (c_i32, [c_vp, ctypes.POINTER(Schedule)]),

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -56,6 +57,10 @@ void launch_schur_items(hipStream_t s, const DevTopology& t, int32_t n_items, co
                         const double* inc_q, const double* inc_w, const double* lm_a, double* partial,
                         int32_t waves, int32_t xcd_map);
 bool schur_item_waves_supported(int32_t waves);
+int32_t calib_entries();
+int32_t calib_record_stride();
+void launch_calib_normal(hipStream_t s, const DevTopology& t, const int32_t* perm, const double* res,
+                         const double* rec, DevLoss loss, bool cost_only, double* partial, double* out);
 uint32_t schur_pair_anchor_bit();
 void launch_landmark_quality(hipStream_t s, const DevTopology& t, const double* vert, const double* vR,
                              const double* lm, int32_t min_observers, double min_angle_deg, double max_dist,
@@ -146,6 +151,7 @@ struct mlba_handle {
   std::vector<int32_t> lm_begin, lm_anchor, vertex_mission;
   std::vector<uint8_t> lm_active, lm_const, vertex_const;
   std::vector<double> h_baseframes, h_cam_q;
+  std::vector<DevCamera> h_cams;        // host mirror of the device cameras
   // Derived per topology build.
   int32_t Pvar = 0, tiles = 0, nnzb = 0, n_inc = 0, n_split = 0, max_row_blocks = 0;
   int64_t schur_pairs = 0;
@@ -158,7 +164,8 @@ struct mlba_handle {
   DBuf<double> lm, lm_cand, lm_acc, lm_scale, lm_z, lm_u;
   DBuf<double> vert, vert_cand, vR, vR_cand;
   DBuf<int32_t> vertex_mission_d, vertex_var, var_vertex;
-  DBuf<DevCamera> cams;
+  DBuf<DevCamera> cams, cams_cand;
+  DBuf<double> cam_q_cand, calib_res, calib_rec, calib_part, calib_out;
   DBuf<double> mission_T, baseframes, cam_q;
   DBuf<int32_t> tile_begin;
   DBuf<double> tile_cost, tile_cost2;
@@ -1246,7 +1253,8 @@ int32_t mlba_create(const mlba_problem_desc* d, mlba_handle** out) {
       (rc = upload(h, h->lm_begin_d, h->lm_begin.data(), h->lm_begin.size())) ||
       (rc = upload(h, h->lm_anchor_d, h->lm_anchor.data(), (size_t)L)) ||
       (rc = upload(h, h->vertex_mission_d, h->vertex_mission.data(), (size_t)P)) ||
-      (rc = upload(h, h->cams, cams.data(), cams.size())) || (rc = upload(h, h->mission_T, mT.data(), mT.size())) ||
+      (rc = upload(h, h->cams, (h->h_cams = cams).data(), cams.size())) ||
+      (rc = upload(h, h->mission_T, mT.data(), mT.size())) ||
       (rc = upload(h, h->baseframes, h->h_baseframes.data(), h->h_baseframes.size())) ||
       (rc = upload(h, h->cam_q, h->h_cam_q.data(), h->h_cam_q.size())) ||
       (rc = upload(h, h->vert, d->vertex_q_IM__M_p_MI, 7 * (size_t)P)) ||
@@ -1550,6 +1558,369 @@ int32_t mlba_solve_with_outlier_rejection(mlba_handle* h, const mlba_options* o,
     sm->termination_type = term;
     sm->initial_cost = initial_cost;
     sm->num_residual_blocks = count_active_obs(h);
+  }
+  return MLBA_OK;
+}
+
+void mlba_calibration_options_init(mlba_calibration_options* c) {
+  if (!c) return;
+  c->fix_intrinsics = 1;
+  c->fix_distortion = 1;
+  c->fix_extrinsics_rotation = 1;
+  c->fix_extrinsics_translation = 1;
+}
+
+int32_t mlba_download_cameras(const mlba_handle* h, double* q_CI__C_p_CI, double* intrinsics, double* distortion,
+                              double* xi) {
+  if (!h) return MLBA_EINVAL;
+  for (int c = 0; c < h->C; ++c) {
+    const DevCamera& k = h->h_cams[c];
+    if (q_CI__C_p_CI)
+      for (int i = 0; i < 7; ++i) q_CI__C_p_CI[7 * c + i] = h->h_cam_q[7 * c + i];
+    if (intrinsics)
+      for (int i = 0; i < 4; ++i) intrinsics[4 * c + i] = k.intr[i];
+    if (distortion)
+      for (int i = 0; i < 4; ++i) distortion[4 * c + i] = k.dist[i];
+    if (xi) xi[c] = k.xi;
+  }
+  return MLBA_OK;
+}
+
+namespace {
+
+// Free tangent columns of one camera: q_CI (0..2), C_p_CI (3..5), fu fv cu
+// cv (6..9), distortion (10..13, as many as the model has), xi (14).
+struct CalibLayout {
+  std::vector<std::array<int32_t, 15>> col;  // per camera: global column or -1
+  int32_t n = 0;
+};
+
+int distortion_parameter_count(int32_t type) {
+  return type == kDistRadtan || type == kDistEquidistant ? 4 : (type == kDistFisheye ? 1 : 0);
+}
+
+// Host LM of the camera calibration blocks with the map held constant:
+// dense normal equations over the free columns (at most 15 per camera),
+// the Ceres rules of minimize() (Jacobi scaling, LM diagonal, gain ratio,
+// radius update, tolerances).
+class CalibSolver {
+ public:
+  CalibSolver(mlba_handle* h, const mlba_options* o, const CalibLayout& lay) : h_(h), o_(o), lay_(lay) {}
+
+  // Cost (cameras with free columns), fixed cost, and with `jac` the normal
+  // equations A, g at the cameras `cams` (device) / `cam_q`.
+  int evaluate(const DevCamera* cams, const double* cam_q, bool jac, double* cost, double* fixed,
+               std::vector<double>* A, std::vector<double>* g) {
+    mlba_handle* h = h_;
+    hipStream_t s = h->stream;
+    DevTopology tt = h->topo;
+    tt.cams = cams;
+    const int E = calib_entries();
+    launch_evaluate_full(s, tt, h->vert.p, h->vR.p, h->lm.p, h->baseframes.p, cam_q, true, h->perm_d.p,
+                         h->calib_res.p, jac ? h->calib_rec.p : nullptr, calib_record_stride(),
+                         jac ? 0x3E0u : 0u);
+    launch_calib_normal(s, tt, h->perm_d.p, h->calib_res.p, h->calib_rec.p, make_loss(o_), !jac, h->calib_part.p,
+                        h->calib_out.p);
+    HIP_OK(hipGetLastError());
+    std::vector<double> out((size_t)E * h->C);
+    HIP_OK(hipMemcpyAsync(out.data(), h->calib_out.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    *cost = *fixed = 0.0;
+    const int n = lay_.n;
+    if (jac) { A->assign((size_t)n * n, 0.0); g->assign(n, 0.0); }
+    for (int c = 0; c < h->C; ++c) {
+      const double* oc = out.data() + (size_t)E * c;
+      bool any = false;
+      for (int k = 0; k < 15; ++k) any = any || lay_.col[c][k] >= 0;
+      (any ? *cost : *fixed) += oc[135];
+      if (!jac || !any) continue;
+      for (int a = 0; a < 15; ++a) {
+        const int ga = lay_.col[c][a];
+        if (ga < 0) continue;
+        (*g)[ga] = oc[120 + a];
+        for (int b = a; b < 15; ++b) {
+          const int gb = lay_.col[c][b];
+          if (gb < 0) continue;
+          const double v = oc[a * 15 - a * (a - 1) / 2 + (b - a)];
+          (*A)[(size_t)ga * n + gb] = v;
+          (*A)[(size_t)gb * n + ga] = v;
+        }
+      }
+    }
+    return MLBA_OK;
+  }
+
+  // max |x - Plus(x, -g)| over the free parameters.
+  double gradient_max(const std::vector<double>& g) const {
+    double m = 0.0;
+    for (int c = 0; c < h_->C; ++c)
+      for (int k = 0; k < 15; ++k) {
+        const int j = lay_.col[c][k];
+        if (j < 0) continue;
+        if (k < 3) {
+          if (k > 0) continue;
+          const double* q = &h_->h_cam_q[7 * c];
+          const double d[3] = {-g[j], -g[lay_.col[c][1]], -g[lay_.col[c][2]]};
+          double qq[4];
+          quat_plus_jpl(q, d, qq);
+          for (int i = 0; i < 4; ++i) m = std::max(m, std::fabs(q[i] - qq[i]));
+        } else {
+          m = std::max(m, std::fabs(g[j]));
+        }
+      }
+    return m;
+  }
+
+  // Candidate cameras x [+] (-w) into cand / cand_q, and the ambient norm of x.
+  double candidate(const std::vector<double>& w, std::vector<DevCamera>* cand, std::vector<double>* cand_q) const {
+    *cand = h_->h_cams;
+    *cand_q = h_->h_cam_q;
+    double xx = 0.0;
+    for (int c = 0; c < h_->C; ++c) {
+      const auto& col = lay_.col[c];
+      DevCamera& k = (*cand)[c];
+      double* q = &(*cand_q)[7 * c];
+      const double* q0 = &h_->h_cam_q[7 * c];
+      const DevCamera& k0 = h_->h_cams[c];
+      if (col[0] >= 0) {
+        const double d[3] = {-w[col[0]], -w[col[1]], -w[col[2]]};
+        quat_plus_jpl(q0, d, q);
+        rot_jpl(q, k.R_C_I);
+        for (int i = 0; i < 4; ++i) xx += q0[i] * q0[i];
+      }
+      if (col[3] >= 0)
+        for (int i = 0; i < 3; ++i) {
+          q[4 + i] = q0[4 + i] - w[col[3 + i]];
+          k.p_C_I[i] = q[4 + i];
+          xx += q0[4 + i] * q0[4 + i];
+        }
+      if (col[6] >= 0)
+        for (int i = 0; i < 4; ++i) {
+          k.intr[i] = k0.intr[i] - w[col[6 + i]];
+          xx += k0.intr[i] * k0.intr[i];
+        }
+      for (int i = 0; i < 4; ++i)
+        if (col[10 + i] >= 0) {
+          k.dist[i] = k0.dist[i] - w[col[10 + i]];
+          xx += k0.dist[i] * k0.dist[i];
+        }
+      if (col[14] >= 0) {
+        k.xi = k0.xi - w[col[14]];
+        xx += k0.xi * k0.xi;
+      }
+    }
+    return std::sqrt(xx);
+  }
+
+ private:
+  mlba_handle* h_;
+  const mlba_options* o_;
+  const CalibLayout& lay_;
+};
+
+// Cholesky solve of the symmetric positive definite n x n system; false
+// when not positive definite.
+bool cholesky_solve(int n, std::vector<double> A, const std::vector<double>& b, std::vector<double>* x) {
+  for (int j = 0; j < n; ++j) {
+    double d = A[(size_t)j * n + j];
+    for (int k = 0; k < j; ++k) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+    if (!(d > 0.0) || !std::isfinite(d)) return false;
+    const double l = std::sqrt(d);
+    A[(size_t)j * n + j] = l;
+    for (int i = j + 1; i < n; ++i) {
+      double v = A[(size_t)i * n + j];
+      for (int k = 0; k < j; ++k) v -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+      A[(size_t)i * n + j] = v / l;
+    }
+  }
+  x->assign(b.begin(), b.end());
+  for (int i = 0; i < n; ++i) {
+    double v = (*x)[i];
+    for (int k = 0; k < i; ++k) v -= A[(size_t)i * n + k] * (*x)[k];
+    (*x)[i] = v / A[(size_t)i * n + i];
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double v = (*x)[i];
+    for (int k = i + 1; k < n; ++k) v -= A[(size_t)k * n + i] * (*x)[k];
+    (*x)[i] = v / A[(size_t)i * n + i];
+  }
+  return true;
+}
+
+}  // namespace
+
+int32_t mlba_solve_calibration(mlba_handle* h, const mlba_options* o, const mlba_calibration_options* co,
+                               mlba_summary* sm) {
+  if (!h) return MLBA_EINVAL;
+  int rc = validate_options(h, o);
+  if (rc) return rc;
+  if (!co) return set_err(h, MLBA_EINVAL, "calibration options are NULL");
+  reset_summary(sm);
+  // The map is held constant: every active observation's landmark, observer
+  // and (for the mission and global types) landmark store vertex.
+  std::vector<uint8_t> cam_used(h->C, 0);
+  int n_active = 0;
+  for (int k = 0; k < h->N; ++k) {
+    const int l = h->s_landmark[k];
+    if (!h->lm_active[l]) continue;
+    ++n_active;
+    cam_used[h->s_camera[k]] = 1;
+    if (!h->lm_const[l] || !h->vertex_const[h->s_vertex[k]] ||
+        (h->s_type[k] != kTypeKeyframe && !h->vertex_const[h->lm_anchor[l]]))
+      return set_err(h, MLBA_EINVAL,
+                     "calibration is solved with the map's vertices and landmarks held constant; "
+                     "a variable vertex or landmark is in the problem");
+  }
+  CalibLayout lay;
+  lay.col.assign(h->C, {});
+  for (int c = 0; c < h->C; ++c) {
+    auto& col = lay.col[c];
+    col.fill(-1);
+    if (!cam_used[c]) continue;
+    const DevCamera& k = h->h_cams[c];
+    auto take = [&](int first, int count) {
+      for (int i = 0; i < count; ++i) col[first + i] = lay.n++;
+    };
+    if (!co->fix_extrinsics_rotation) take(0, 3);
+    if (!co->fix_extrinsics_translation) take(3, 3);
+    if (!co->fix_intrinsics) take(6, 4);
+    if (!co->fix_distortion) take(10, distortion_parameter_count(k.dist_type));
+    if (!co->fix_intrinsics && k.model == kCameraUnifiedProjection) take(14, 1);
+  }
+  if (lay.n == 0) return set_err(h, MLBA_EINVAL, "no calibration block is free");
+  if (sm) sm->num_residual_blocks = n_active;
+  const int n = lay.n;
+  const int E = calib_entries();
+  const int tiles = (h->N + 255) / 256;
+  if (!h->calib_res.alloc(2 * (size_t)std::max(h->N, 1)) ||
+      !h->calib_rec.alloc((size_t)calib_record_stride() * std::max(h->N, 1)) ||
+      !h->calib_part.alloc((size_t)E * h->C * std::max(tiles, 1)) || !h->calib_out.alloc((size_t)E * h->C) ||
+      !h->cams_cand.alloc(h->C) || !h->cam_q_cand.alloc(7 * (size_t)h->C))
+    return set_err(h, MLBA_ENOMEM, "device allocation failed");
+  hipStream_t s = h->stream;
+  launch_vertex_prep(s, h->P, h->vert.p, h->vR.p);
+  CalibSolver cs(h, o, lay);
+  const double t_start = now_s();
+  std::vector<double> A, g;
+  double x_cost = 0.0, fixed_cost = 0.0;
+  if ((rc = cs.evaluate(h->cams.p, h->cam_q.p, true, &x_cost, &fixed_cost, &A, &g))) return rc;
+  double grad_max = cs.gradient_max(g);
+  if (sm) { sm->initial_cost = x_cost + fixed_cost; sm->fixed_cost = fixed_cost; }
+  mlba_iteration_summary it{};
+  it.step_is_valid = it.step_is_successful = 1;
+  it.cost = x_cost + fixed_cost;
+  it.gradient_max_norm = grad_max;
+  double radius = o->initial_trust_region_radius, decrease_factor = 2.0;
+  it.trust_region_radius = radius;
+  it.cumulative_time_in_seconds = it.iteration_time_in_seconds = now_s() - t_start;
+  push_iteration(sm, it);
+  int termination = MLBA_NO_CONVERGENCE;
+  if (grad_max <= o->gradient_tolerance) termination = MLBA_CONVERGENCE;
+  int consecutive_invalid = 0, iteration = 0;
+  std::vector<DevCamera> cand;
+  std::vector<double> cand_q, w, Ad, gd, scale(n, 1.0);
+  while (termination == MLBA_NO_CONVERGENCE && o->max_num_iterations > 0) {
+    const double t_iter = now_s();
+    ++iteration;
+    mlba_iteration_summary cur{};
+    cur.iteration = iteration;
+    // Jacobi scaling and the LM diagonal on the scaled system.
+    for (int j = 0; j < n; ++j)
+      scale[j] = o->use_jacobi_scaling ? 1.0 / (1.0 + std::sqrt(A[(size_t)j * n + j])) : 1.0;
+    Ad.assign((size_t)n * n, 0.0);
+    gd.assign(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+      gd[i] = scale[i] * g[i];
+      for (int j = 0; j < n; ++j) Ad[(size_t)i * n + j] = scale[i] * A[(size_t)i * n + j] * scale[j];
+    }
+    for (int j = 0; j < n; ++j)
+      Ad[(size_t)j * n + j] += std::min(std::max(Ad[(size_t)j * n + j], o->min_lm_diagonal), o->max_lm_diagonal) / radius;
+    const double t_ls = now_s();
+    bool step_valid = cholesky_solve(n, Ad, gd, &w);
+    double mcc = 0.0;
+    if (step_valid) {
+      for (int j = 0; j < n; ++j) w[j] *= scale[j];
+      // Model cost change of the step -w: g'w - w'Aw / 2.
+      double gw = 0.0, wAw = 0.0;
+      for (int i = 0; i < n; ++i) {
+        gw += g[i] * w[i];
+        double aw = 0.0;
+        for (int j = 0; j < n; ++j) aw += A[(size_t)i * n + j] * w[j];
+        wAw += w[i] * aw;
+      }
+      mcc = gw - 0.5 * wAw;
+      step_valid = std::isfinite(mcc) && mcc > 0.0;
+    }
+    if (sm) sm->linear_solver_seconds += now_s() - t_ls;
+    cur.linear_solver_iterations = 1;
+    cur.step_is_valid = step_valid;
+    if (!step_valid) {
+      if (++consecutive_invalid >= o->max_num_consecutive_invalid_steps) { termination = MLBA_FAILURE; break; }
+      radius /= decrease_factor;
+      decrease_factor *= 2.0;
+      cur.cost = x_cost + fixed_cost;
+      cur.gradient_max_norm = grad_max;
+      cur.trust_region_radius = radius;
+      if (sm) sm->num_unsuccessful_steps++;
+    } else {
+      consecutive_invalid = 0;
+      double step_norm = 0.0;
+      for (int j = 0; j < n; ++j) step_norm += w[j] * w[j];
+      step_norm = std::sqrt(step_norm);
+      const double x_norm = cs.candidate(w, &cand, &cand_q);
+      cur.step_norm = step_norm;
+      if (step_norm <= o->parameter_tolerance * (x_norm + o->parameter_tolerance)) {
+        termination = MLBA_CONVERGENCE;
+        break;
+      }
+      HIP_OK(hipMemcpyAsync(h->cams_cand.p, cand.data(), sizeof(DevCamera) * h->C, hipMemcpyHostToDevice, s));
+      HIP_OK(hipMemcpyAsync(h->cam_q_cand.p, cand_q.data(), sizeof(double) * 7 * h->C, hipMemcpyHostToDevice, s));
+      double cand_cost = 0.0, cand_fixed = 0.0;
+      if ((rc = cs.evaluate(h->cams_cand.p, h->cam_q_cand.p, false, &cand_cost, &cand_fixed, nullptr, nullptr)))
+        return rc;
+      cur.cost_change = x_cost - cand_cost;
+      if (std::fabs(cur.cost_change) <= o->function_tolerance * x_cost) {
+        termination = MLBA_CONVERGENCE;
+        break;
+      }
+      cur.relative_decrease = (x_cost - cand_cost) / mcc;
+      if (cur.relative_decrease > o->min_relative_decrease) {
+        h->h_cams = cand;
+        h->h_cam_q = cand_q;
+        HIP_OK(hipMemcpyAsync(h->cams.p, cand.data(), sizeof(DevCamera) * h->C, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(h->cam_q.p, cand_q.data(), sizeof(double) * 7 * h->C, hipMemcpyHostToDevice, s));
+        const double t_ev = now_s();
+        if ((rc = cs.evaluate(h->cams.p, h->cam_q.p, true, &x_cost, &fixed_cost, &A, &g))) return rc;
+        if (sm) sm->evaluate_seconds += now_s() - t_ev;
+        grad_max = cs.gradient_max(g);
+        cur.step_is_successful = 1;
+        cur.cost = x_cost + fixed_cost;
+        cur.gradient_max_norm = grad_max;
+        const double q = 2.0 * cur.relative_decrease - 1.0;
+        radius = std::min(o->max_trust_region_radius, radius / std::max(1.0 / 3.0, 1.0 - q * q * q));
+        decrease_factor = 2.0;
+        if (sm) sm->num_successful_steps++;
+      } else {
+        radius /= decrease_factor;
+        decrease_factor *= 2.0;
+        cur.cost = cand_cost + fixed_cost;
+        cur.gradient_max_norm = grad_max;
+        if (sm) sm->num_unsuccessful_steps++;
+      }
+      cur.trust_region_radius = radius;
+    }
+    cur.iteration_time_in_seconds = now_s() - t_iter;
+    cur.cumulative_time_in_seconds = now_s() - t_start;
+    push_iteration(sm, cur);
+    if (now_s() - t_start > o->max_solver_time_in_seconds) { termination = MLBA_NO_CONVERGENCE; break; }
+    if (iteration >= o->max_num_iterations) break;
+    if (cur.step_is_successful && cur.gradient_max_norm <= o->gradient_tolerance) { termination = MLBA_CONVERGENCE; break; }
+    if (radius <= o->min_trust_region_radius) { termination = MLBA_CONVERGENCE; break; }
+  }
+  if (sm) {
+    sm->termination_type = termination;
+    sm->final_cost = x_cost + fixed_cost;
+    sm->lm_loop_seconds += now_s() - t_start;
   }
   return MLBA_OK;
 }

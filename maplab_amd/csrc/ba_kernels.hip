@@ -1489,6 +1489,111 @@ void launch_reduce(hipStream_t s, int32_t n, int32_t stride, int32_t offset, boo
                    double* out) {
   k_reduce<<<1, 256, 0, s>>>(n, stride, offset, is_max ? 1 : 0, parts, out);
 }
+// ------------------------------------------------------------ calibration
+
+// Normal equations of the camera calibration blocks when vertices and
+// landmarks are constant. Per camera, over the 15 tangent columns
+// [q_CI (3) | C_p_CI (3) | fu fv cu cv | distortion (4) | xi], of the
+// robust-corrected residuals (ceres Corrector, as k_linearize):
+//   entries 0..119  J'J upper triangle (row-major over a <= b),
+//   entries 120..134 J'r, entry 135 the cost 0.5 rho.
+// One work-group per tile of 256 observations (landmark-sorted order) writes
+// its partial sums per camera; k_calib_sum adds the tiles in order.
+// `rec` is the tangent k_evaluate_full record of every observation (caller
+// order, stride kCalibStride) and `res` its residuals (caller order).
+constexpr int kCalibEntries = 136;
+constexpr int kCalibStride = 84;  // 2 x (3 + 6 + 6 + 6 + 6 + 3 + 3 + 4 + 4 + 1)
+
+__global__ __launch_bounds__(256) void k_calib_partials(DevTopology t, const int32_t* __restrict__ perm,
+                                                        const double* __restrict__ res,
+                                                        const double* __restrict__ rec, DevLoss loss,
+                                                        int32_t cost_only, double* __restrict__ partial) {
+  __shared__ double lds4[4];
+  const int o = blockIdx.x * 256 + threadIdx.x;
+  int cam = -1;
+  double J[30], r[2] = {0.0, 0.0}, cost = 0.0;
+  for (int k = 0; k < 30; ++k) J[k] = 0.0;
+  if (o < t.N) {
+    const int32_t meta = t.obs_meta[o];
+    const int32_t l = t.obs_landmark[o];
+    if (t.lm_flags[l] & kLmActive) {
+      cam = obs_camera(meta);
+      const int32_t oo = perm[o];
+      const double r0 = res[2 * oo], r1 = res[2 * oo + 1];
+      const double s2 = r0 * r0 + r1 * r1;
+      double rho[3];
+      robust_loss(loss.type, loss_scale(loss, obs_type(meta), t.obs_sinv[o]), s2, rho);
+      cost = 0.5 * rho[0];
+      if (!cost_only) {
+        // Tangent blocks 5 (q_CI), 6 (p_CI), 7 (intrinsics), 8 (distortion), 9 (xi).
+        const double* R = rec + (size_t)oo * kCalibStride;
+        const int off[5] = {54, 60, 66, 74, 82}, wid[5] = {3, 3, 4, 4, 1}, col[5] = {0, 3, 6, 10, 14};
+        for (int b = 0; b < 5; ++b)
+          for (int c = 0; c < wid[b]; ++c) {
+            J[col[b] + c] = R[off[b] + c];
+            J[15 + col[b] + c] = R[off[b] + wid[b] + c];
+          }
+        const double sq1 = sqrt(rho[1]);
+        double rs = sq1, asq = 0.0;
+        if (s2 > 0.0 && rho[2] > 0.0) {
+          const double alpha = 1.0 - sqrt(1.0 + 2.0 * s2 * rho[2] / rho[1]);
+          rs = sq1 / (1.0 - alpha);
+          asq = alpha / s2;
+        }
+        for (int c = 0; c < 15; ++c) {
+          const double rtj = J[c] * r0 + J[15 + c] * r1;
+          J[c] = sq1 * (J[c] - asq * r0 * rtj);
+          J[15 + c] = sq1 * (J[15 + c] - asq * r1 * rtj);
+        }
+        r[0] = r0 * rs;
+        r[1] = r1 * rs;
+      }
+    }
+  }
+  for (int c = 0; c < t.C; ++c) {
+    double* out = partial + ((size_t)blockIdx.x * t.C + c) * kCalibEntries;
+    const bool mine = cam == c;
+    if (!cost_only) {
+      // Fully unrolled so that J stays in registers.
+#pragma unroll
+      for (int a = 0; a < 15; ++a)
+#pragma unroll
+        for (int b = a; b < 15; ++b) {
+          const double v = block_sum256(mine ? J[a] * J[b] + J[15 + a] * J[15 + b] : 0.0, lds4);
+          if (threadIdx.x == 0) out[a * 15 - a * (a - 1) / 2 + (b - a)] = v;
+        }
+#pragma unroll
+      for (int a = 0; a < 15; ++a) {
+        const double v = block_sum256(mine ? J[a] * r[0] + J[15 + a] * r[1] : 0.0, lds4);
+        if (threadIdx.x == 0) out[120 + a] = v;
+      }
+    }
+    const double v = block_sum256(mine ? cost : 0.0, lds4);
+    if (threadIdx.x == 0) out[135] = v;
+  }
+}
+
+// out[c][e] = sum over tiles of the partials, in tile order.
+__global__ void k_calib_sum(int32_t tiles, int32_t C, const double* __restrict__ partial, double* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= C * kCalibEntries) return;
+  const int c = k / kCalibEntries, e = k % kCalibEntries;
+  double v = 0.0;
+  for (int tile = 0; tile < tiles; ++tile) v += partial[((size_t)tile * C + c) * kCalibEntries + e];
+  out[k] = v;
+}
+
+int32_t calib_entries() { return kCalibEntries; }
+int32_t calib_record_stride() { return kCalibStride; }
+void launch_calib_normal(hipStream_t s, const DevTopology& t, const int32_t* perm, const double* res,
+                         const double* rec, DevLoss loss, bool cost_only, double* partial, double* out) {
+  if (t.N <= 0 || t.C <= 0) return;
+  const int tiles = (t.N + 255) / 256;
+  k_calib_partials<<<tiles, 256, 0, s>>>(t, perm, res, rec, loss, cost_only ? 1 : 0, partial);
+  const int n = t.C * kCalibEntries;
+  k_calib_sum<<<(n + 255) / 256, 256, 0, s>>>(tiles, t.C, partial, out);
+}
+
 void launch_evaluate_full(hipStream_t s, const DevTopology& t, const double* vert, const double* vR, const double* lm,
                           const double* baseframes, const double* cam_q, bool tangent, const int32_t* perm,
                           double* res, double* jout, int32_t jstride, uint32_t mask) {

@@ -127,6 +127,28 @@ class DeviceProblem:
             raise RuntimeError(f"mlba_solve_many failed: {rc}")
         return [(sm, [it[i] for i in range(min(sm.num_iterations, capacity))]) for sm, it in zip(sms, its)]
 
+    def solve_calibration(self, options: abi.Options, fix_intrinsics: bool = True, fix_distortion: bool = True,
+                          fix_rotation: bool = True, fix_translation: bool = True, capacity: int = 256):
+        """mlba_solve_calibration: the free camera blocks refined with the map
+        held constant. Returns (summary, iteration rows)."""
+        co = abi.CalibrationOptions(int(fix_intrinsics), int(fix_distortion), int(fix_rotation),
+                                    int(fix_translation))
+        its = (abi.IterationSummary * capacity)()
+        sm = abi.Summary()
+        sm.iterations_capacity = capacity
+        sm.iterations = ctypes.cast(its, ctypes.POINTER(abi.IterationSummary))
+        self._check(self.lib.mlba_solve_calibration(self.handle, ctypes.byref(options), ctypes.byref(co),
+                                                    ctypes.byref(sm)))
+        return sm, [its[i] for i in range(min(sm.num_iterations, capacity))]
+
+    def download_cameras(self):
+        """(q_CI | C_p_CI (C, 7), intrinsics (C, 4), distortion (C, 4), xi (C,))."""
+        C = self.map.camera_intrinsics.shape[0]
+        q, i, d, x = np.zeros((C, 7)), np.zeros((C, 4)), np.zeros((C, 4)), np.zeros(C)
+        self._check(self.lib.mlba_download_cameras(self.handle, q.ctypes.data, i.ctypes.data, d.ctypes.data,
+                                                   x.ctypes.data))
+        return q, i, d, x
+
     def autotune(self, options: abi.Options, repeats: int = 5) -> abi.Schedule:
         """mlba_autotune: time every dispatch schedule of the Schur block
         kernel at the current state and keep the fastest (results unchanged)."""

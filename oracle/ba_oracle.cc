@@ -1650,6 +1650,277 @@ int32_t oracle_solve(const mlba_problem_desc* desc, const mlba_options* options,
   return MLBA_OK;
 }
 
+// Calibration refinement with the map held constant: the camera blocks
+// q_CI (JplQuaternionParameterization), C_p_CI, intrinsics (with xi for a
+// unified-projection camera) and distortion of visual-error-term-inl.h:321-367
+// are the only variables (fix flags: optimization-terms-addition.cc:222-234);
+// ceres: LM on dense normal equations (DENSE_QR / DENSE_SCHUR give the same
+// step here), with the rules of minimize() above. fix = {intrinsics,
+// distortion, extrinsics rotation, extrinsics translation}.
+int32_t oracle_solve_calibration(const mlba_problem_desc* desc, const mlba_options* options, const int32_t* fix,
+                                 const uint8_t* landmark_active, double* cam_q_out, double* intrinsics_out,
+                                 double* distortion_out, double* xi_out, mlba_summary* summary) {
+  if (!validateDesc(desc) || !options || !fix) return MLBA_EINVAL;
+  const mlba_options& o = *options;
+  Problem pb(desc, landmark_active);
+  const int C = desc->num_cameras;
+  std::vector<double> cq(desc->camera_q_CI__C_p_CI, desc->camera_q_CI__C_p_CI + 7 * C);
+  std::vector<double> ci(desc->camera_intrinsics, desc->camera_intrinsics + 4 * C);
+  std::vector<double> cd(desc->camera_distortion, desc->camera_distortion + 4 * C);
+  std::vector<double> cx(C, 0.0);
+  for (int c = 0; c < C; ++c) cx[c] = camModel(desc, c).xi;
+  // Every active residual's map blocks must be constant.
+  std::vector<uint8_t> used(C, 0);
+  for (int i : pb.active_obs) {
+    const int l = desc->obs_landmark[i], v = desc->obs_vertex[i], st = desc->landmark_store_vertex[l];
+    if (!pb.lmConst(l) || !pb.poseConst(v) || (pb.obs_type[i] != MLBA_LOCAL_KEYFRAME && !pb.poseConst(st)))
+      return MLBA_EINVAL;
+    used[desc->obs_camera[i]] = 1;
+  }
+  // Columns: per camera [rot 3 | trans 3 | fu fv cu cv | distortion | xi].
+  std::vector<std::array<int, 15>> col(C);
+  int n = 0;
+  for (int c = 0; c < C; ++c) {
+    col[c].fill(-1);
+    if (!used[c]) continue;
+    const int dt = desc->camera_distortion_type[c];
+    const int nd = (dt == MLBA_DISTORTION_RADTAN || dt == MLBA_DISTORTION_EQUIDISTANT) ? 4
+                   : (dt == MLBA_DISTORTION_FISHEYE ? 1 : 0);
+    if (!fix[2]) for (int k = 0; k < 3; ++k) col[c][k] = n++;
+    if (!fix[3]) for (int k = 3; k < 6; ++k) col[c][k] = n++;
+    if (!fix[0]) for (int k = 6; k < 10; ++k) col[c][k] = n++;
+    if (!fix[1]) for (int k = 0; k < nd; ++k) col[c][10 + k] = n++;
+    if (!fix[0] && camModel(desc, c).model == MLBA_CAMERA_UNIFIED_PROJECTION) col[c][14] = n++;
+  }
+  if (n == 0) return MLBA_EINVAL;
+  auto camFree = [&](int c) {
+    for (int k = 0; k < 15; ++k) if (col[c][k] >= 0) return true;
+    return false;
+  };
+  // Cost, fixed cost and (with A) J'J and J'r of the free columns.
+  auto linearize = [&](const std::vector<double>& q, const std::vector<double>& in, const std::vector<double>& di,
+                       const std::vector<double>& xi, double* cost, double* fixed, std::vector<double>* A,
+                       std::vector<double>* g) {
+    *cost = *fixed = 0.0;
+    if (A) { A->assign((size_t)n * n, 0.0); g->assign(n, 0.0); }
+    for (int i : pb.active_obs) {
+      const int c = desc->obs_camera[i];
+      const double* params[9];
+      pb.params(i, pb.vertices.data(), pb.landmarks.data(), params);
+      params[5] = &q[7 * c];
+      params[6] = &q[7 * c + 4];
+      params[7] = &in[4 * c];
+      params[8] = &di[4 * c];
+      CamModel cm = camModel(desc, c);
+      cm.xi = xi[c];
+      double J5[8], J6[6], J7[8], J8[8], Jx[2], r[2];
+      double* jac[9] = {nullptr, nullptr, nullptr, nullptr, nullptr, J5, J6, J7, J8};
+      const int type = pb.obs_type[i];
+      const double sigma = desc->obs_sigma[i];
+      evaluateVisual(desc->obs_keypoint + 2 * i, 1.0 / sigma, type, desc->camera_distortion_type[c],
+                     desc->camera_resolution[2 * c], desc->camera_resolution[2 * c + 1], params, r,
+                     A ? jac : nullptr, cm, A ? Jx : nullptr);
+      const double sq = r[0] * r[0] + r[1] * r[1];
+      double rho[3];
+      lossEvaluate(o.loss_type, lossScale(o, type, sigma), sq, rho);
+      (camFree(c) ? *cost : *fixed) += 0.5 * rho[0];
+      if (!A || !camFree(c)) continue;
+      // Tangent columns: q_CI through the JPL parameterization Jacobian.
+      double Jq[12], Jt[30];
+      quatJacobian(params[5], Jq);
+      for (int k = 0; k < 2; ++k) {
+        for (int a = 0; a < 3; ++a) {
+          double v = 0.0;
+          for (int m = 0; m < 4; ++m) v += J5[4 * k + m] * Jq[3 * m + a];
+          Jt[15 * k + a] = v;
+          Jt[15 * k + 3 + a] = J6[3 * k + a];
+        }
+        for (int a = 0; a < 4; ++a) { Jt[15 * k + 6 + a] = J7[4 * k + a]; Jt[15 * k + 10 + a] = J8[4 * k + a]; }
+        Jt[15 * k + 14] = Jx[k];
+      }
+      Corrector corr(sq, rho);
+      corr.correctJacobian(r, Jt, 15);
+      const double rr[2] = {r[0] * corr.residual_scaling, r[1] * corr.residual_scaling};
+      for (int a = 0; a < 15; ++a) {
+        const int ga = col[c][a];
+        if (ga < 0) continue;
+        (*g)[ga] += Jt[a] * rr[0] + Jt[15 + a] * rr[1];
+        for (int b = 0; b < 15; ++b) {
+          const int gb = col[c][b];
+          if (gb >= 0) (*A)[(size_t)ga * n + gb] += Jt[a] * Jt[b] + Jt[15 + a] * Jt[15 + b];
+        }
+      }
+    }
+  };
+  auto gradMax = [&](const std::vector<double>& q, const std::vector<double>& g) {
+    double m = 0.0;
+    for (int c = 0; c < C; ++c)
+      for (int k = 0; k < 15; ++k) {
+        const int j = col[c][k];
+        if (j < 0) continue;
+        if (k == 0) {
+          const double d[3] = {-g[j], -g[col[c][1]], -g[col[c][2]]};
+          double qq[4];
+          quatPlus(&q[7 * c], d, qq);
+          for (int a = 0; a < 4; ++a) m = std::max(m, std::fabs(q[7 * c + a] - qq[a]));
+        } else if (k >= 3) {
+          m = std::max(m, std::fabs(g[j]));
+        }
+      }
+    return m;
+  };
+  if (summary) {
+    summary->num_iterations = 0;
+    summary->num_successful_steps = summary->num_unsuccessful_steps = 0;
+    summary->total_linear_solver_iterations = 0;
+    summary->num_residual_blocks = (int)pb.active_obs.size();
+    summary->num_removed_landmarks = 0;
+  }
+  auto push = [&](const mlba_iteration_summary& it) {
+    if (!summary) return;
+    if (summary->iterations && summary->num_iterations < summary->iterations_capacity)
+      summary->iterations[summary->num_iterations] = it;
+    summary->num_iterations++;
+  };
+  std::vector<double> A, g;
+  double cost, fixed;
+  linearize(cq, ci, cd, cx, &cost, &fixed, &A, &g);
+  double gmax = gradMax(cq, g);
+  if (summary) { summary->initial_cost = cost + fixed; summary->fixed_cost = fixed; }
+  mlba_iteration_summary it0{};
+  it0.step_is_valid = it0.step_is_successful = 1;
+  it0.cost = cost + fixed;
+  it0.gradient_max_norm = gmax;
+  double radius = o.initial_trust_region_radius, decrease = 2.0;
+  it0.trust_region_radius = radius;
+  push(it0);
+  int term = gmax <= o.gradient_tolerance ? MLBA_CONVERGENCE : MLBA_NO_CONVERGENCE;
+  int invalid = 0, iteration = 0;
+  while (term == MLBA_NO_CONVERGENCE && o.max_num_iterations > 0) {
+    ++iteration;
+    mlba_iteration_summary cur{};
+    cur.iteration = iteration;
+    // ceres: Jacobi scaling 1 / (1 + |column|), LM diagonal clamp(diag) / radius.
+    std::vector<double> sc(n), M((size_t)n * n), b(n), w(n);
+    for (int j = 0; j < n; ++j) sc[j] = o.use_jacobi_scaling ? 1.0 / (1.0 + std::sqrt(A[(size_t)j * n + j])) : 1.0;
+    for (int a = 0; a < n; ++a) {
+      b[a] = sc[a] * g[a];
+      for (int c2 = 0; c2 < n; ++c2) M[(size_t)a * n + c2] = sc[a] * A[(size_t)a * n + c2] * sc[c2];
+    }
+    for (int j = 0; j < n; ++j)
+      M[(size_t)j * n + j] += std::min(std::max(M[(size_t)j * n + j], o.min_lm_diagonal), o.max_lm_diagonal) / radius;
+    // Dense LDL'-free Cholesky solve.
+    bool ok = true;
+    std::vector<double> L(M);
+    for (int j = 0; j < n && ok; ++j) {
+      double dj = L[(size_t)j * n + j];
+      for (int k = 0; k < j; ++k) dj -= L[(size_t)j * n + k] * L[(size_t)j * n + k];
+      if (!(dj > 0.0)) { ok = false; break; }
+      const double lj = std::sqrt(dj);
+      L[(size_t)j * n + j] = lj;
+      for (int i2 = j + 1; i2 < n; ++i2) {
+        double v = L[(size_t)i2 * n + j];
+        for (int k = 0; k < j; ++k) v -= L[(size_t)i2 * n + k] * L[(size_t)j * n + k];
+        L[(size_t)i2 * n + j] = v / lj;
+      }
+    }
+    double mcc = 0.0;
+    if (ok) {
+      for (int i2 = 0; i2 < n; ++i2) {
+        double v = b[i2];
+        for (int k = 0; k < i2; ++k) v -= L[(size_t)i2 * n + k] * w[k];
+        w[i2] = v / L[(size_t)i2 * n + i2];
+      }
+      for (int i2 = n - 1; i2 >= 0; --i2) {
+        double v = w[i2];
+        for (int k = i2 + 1; k < n; ++k) v -= L[(size_t)k * n + i2] * w[k];
+        w[i2] = v / L[(size_t)i2 * n + i2];
+      }
+      for (int j = 0; j < n; ++j) w[j] *= sc[j];
+      // ceres: model cost change of the step -w, -(J d)'(r + J d / 2).
+      double gw = 0.0, wAw = 0.0;
+      for (int a = 0; a < n; ++a) {
+        gw += g[a] * w[a];
+        for (int c2 = 0; c2 < n; ++c2) wAw += w[a] * A[(size_t)a * n + c2] * w[c2];
+      }
+      mcc = gw - 0.5 * wAw;
+      ok = std::isfinite(mcc) && mcc > 0.0;
+    }
+    cur.step_is_valid = ok;
+    cur.linear_solver_iterations = 1;
+    if (!ok) {
+      if (++invalid >= o.max_num_consecutive_invalid_steps) { term = MLBA_FAILURE; break; }
+      radius /= decrease;
+      decrease *= 2.0;
+      cur.cost = cost + fixed;
+      cur.gradient_max_norm = gmax;
+      cur.trust_region_radius = radius;
+      if (summary) summary->num_unsuccessful_steps++;
+    } else {
+      invalid = 0;
+      // Candidate x [+] (-w) and the ambient norm of the variable blocks.
+      std::vector<double> q2(cq), in2(ci), di2(cd), xi2(cx);
+      double xx = 0.0, ss = 0.0;
+      for (int j = 0; j < n; ++j) ss += w[j] * w[j];
+      for (int c = 0; c < C; ++c) {
+        const auto& cc = col[c];
+        if (cc[0] >= 0) {
+          const double d[3] = {-w[cc[0]], -w[cc[1]], -w[cc[2]]};
+          quatPlus(&cq[7 * c], d, &q2[7 * c]);
+          for (int a = 0; a < 4; ++a) xx += cq[7 * c + a] * cq[7 * c + a];
+        }
+        if (cc[3] >= 0)
+          for (int a = 0; a < 3; ++a) { q2[7 * c + 4 + a] -= w[cc[3 + a]]; xx += cq[7 * c + 4 + a] * cq[7 * c + 4 + a]; }
+        if (cc[6] >= 0)
+          for (int a = 0; a < 4; ++a) { in2[4 * c + a] -= w[cc[6 + a]]; xx += ci[4 * c + a] * ci[4 * c + a]; }
+        for (int a = 0; a < 4; ++a)
+          if (cc[10 + a] >= 0) { di2[4 * c + a] -= w[cc[10 + a]]; xx += cd[4 * c + a] * cd[4 * c + a]; }
+        if (cc[14] >= 0) { xi2[c] -= w[cc[14]]; xx += cx[c] * cx[c]; }
+      }
+      const double step_norm = std::sqrt(ss), x_norm = std::sqrt(xx);
+      cur.step_norm = step_norm;
+      if (step_norm <= o.parameter_tolerance * (x_norm + o.parameter_tolerance)) { term = MLBA_CONVERGENCE; break; }
+      double ccost, cfixed;
+      linearize(q2, in2, di2, xi2, &ccost, &cfixed, nullptr, nullptr);
+      cur.cost_change = cost - ccost;
+      if (std::fabs(cur.cost_change) <= o.function_tolerance * cost) { term = MLBA_CONVERGENCE; break; }
+      cur.relative_decrease = (cost - ccost) / mcc;
+      if (cur.relative_decrease > o.min_relative_decrease) {
+        cq = q2; ci = in2; cd = di2; cx = xi2;
+        linearize(cq, ci, cd, cx, &cost, &fixed, &A, &g);
+        gmax = gradMax(cq, g);
+        cur.step_is_successful = 1;
+        cur.cost = cost + fixed;
+        cur.gradient_max_norm = gmax;
+        const double qq = 2.0 * cur.relative_decrease - 1.0;
+        radius = std::min(o.max_trust_region_radius, radius / std::max(1.0 / 3.0, 1.0 - qq * qq * qq));
+        decrease = 2.0;
+        if (summary) summary->num_successful_steps++;
+      } else {
+        radius /= decrease;
+        decrease *= 2.0;
+        cur.cost = ccost + fixed;
+        cur.gradient_max_norm = gmax;
+        if (summary) summary->num_unsuccessful_steps++;
+      }
+      cur.trust_region_radius = radius;
+    }
+    push(cur);
+    if (iteration >= o.max_num_iterations) break;
+    if (cur.step_is_successful && cur.gradient_max_norm <= o.gradient_tolerance) { term = MLBA_CONVERGENCE; break; }
+    if (radius <= o.min_trust_region_radius) { term = MLBA_CONVERGENCE; break; }
+  }
+  if (summary) {
+    summary->termination_type = term;
+    summary->final_cost = cost + fixed;
+  }
+  if (cam_q_out) std::memcpy(cam_q_out, cq.data(), sizeof(double) * 7 * C);
+  if (intrinsics_out) std::memcpy(intrinsics_out, ci.data(), sizeof(double) * 4 * C);
+  if (distortion_out) std::memcpy(distortion_out, cd.data(), sizeof(double) * 4 * C);
+  if (xi_out) std::memcpy(xi_out, cx.data(), sizeof(double) * C);
+  return MLBA_OK;
+}
+
 int32_t oracle_solve_with_outlier_rejection(const mlba_problem_desc* desc, const mlba_options* options,
                                             int32_t linear_solver, int32_t num_threads, double* vertices_out,
                                             double* landmarks_out, uint8_t* landmark_active_out,

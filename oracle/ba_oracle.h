@@ -74,6 +74,14 @@ int32_t oracle_solve(const mlba_problem_desc* desc, const mlba_options* options,
                      const uint8_t* landmark_active, double* vertices_out,
                      double* landmarks_out, mlba_summary* summary);
 
+/* Camera calibration refinement with every vertex and landmark constant:
+ * fix = {intrinsics (with xi), distortion, extrinsics rotation, extrinsics
+ * translation}; refined cameras into the nullable outputs (7C, 4C, 4C, C).
+ * MLBA_EINVAL when a map block is variable or nothing is free. */
+int32_t oracle_solve_calibration(const mlba_problem_desc* desc, const mlba_options* options, const int32_t* fix,
+                                 const uint8_t* landmark_active, double* cam_q_out, double* intrinsics_out,
+                                 double* distortion_out, double* xi_out, mlba_summary* summary);
+
 /* solveWithOutlierRejection; landmark_active_out (L, nullable) receives the
  * final active mask. */
 int32_t oracle_solve_with_outlier_rejection(const mlba_problem_desc* desc,

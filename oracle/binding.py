@@ -45,6 +45,8 @@ def load():
         "oracle_solve": (i32, [P(abi.ProblemDesc), P(abi.Options), i32, i32, vp, vp, vp, P(abi.Summary)]),
         "oracle_solve_with_outlier_rejection": (i32, [P(abi.ProblemDesc), P(abi.Options), i32, i32, vp, vp, vp,
                                                       P(abi.Summary)]),
+        "oracle_solve_calibration": (i32, [P(abi.ProblemDesc), P(abi.Options), vp, vp, vp, vp, vp, vp,
+                                           P(abi.Summary)]),
         "oracle_reduced_camera_system": (i32, [P(abi.ProblemDesc), P(abi.Options), f64, i32, vp, vp, vp]),
         "oracle_landmark_quality": (i32, [P(abi.ProblemDesc), P(abi.QualityOptions), vp]),
     }
@@ -185,6 +187,27 @@ def solve(vmap, options, linear_solver=SCHUR_DIRECT, threads=0, outlier_rejectio
     assert rc == 0, rc
     rows = [its[i] for i in range(min(sm.num_iterations, capacity))]
     return sm, rows, v, l, act_out
+
+
+def solve_calibration(vmap, options, fix_intrinsics=True, fix_distortion=True, fix_rotation=True,
+                      fix_translation=True, capacity=256):
+    """Returns (summary, rows, (q_CI | C_p_CI, intrinsics, distortion, xi)) or
+    None when the oracle rejects the problem."""
+    from maplab_amd import abi
+    lib = load()
+    d = vmap.desc()
+    C = vmap.camera_intrinsics.shape[0]
+    q, i, dd, x = np.zeros((C, 7)), np.zeros((C, 4)), np.zeros((C, 4)), np.zeros(C)
+    fix = np.array([fix_intrinsics, fix_distortion, fix_rotation, fix_translation], dtype=np.int32)
+    its = (abi.IterationSummary * capacity)()
+    sm = abi.Summary()
+    sm.iterations_capacity = capacity
+    sm.iterations = ctypes.cast(its, ctypes.POINTER(abi.IterationSummary))
+    rc = lib.oracle_solve_calibration(ctypes.byref(d), ctypes.byref(options), _p(fix), None, _p(q), _p(i), _p(dd),
+                                      _p(x), ctypes.byref(sm))
+    if rc != 0:
+        return None
+    return sm, [its[k] for k in range(min(sm.num_iterations, capacity))], (q, i, dd, x)
 
 
 def reduced_camera_system(vmap, options, mu):

@@ -67,6 +67,7 @@ def test_struct_layouts_match_header():
     (abi.Summary, "mlba_summary", "maplab_ba.h"),
     (abi.KernelTimes, "mlba_kernel_times", "maplab_ba.h"),
     (abi.Schedule, "mlba_schedule", "maplab_ba.h"),
+    (abi.CalibrationOptions, "mlba_calibration_options", "maplab_ba.h"),
     (abi.SynthConfig, "mlba_synth_config", "maplab_ba_synth.h"),
     (abi.QualityOptions, "mlba_quality_options", "maplab_ba.h"),
     (abi.ViProblemOptions, "mlba_vi_problem_options", "maplab_ba_optimizer.h"),
