@@ -217,7 +217,8 @@ def main(argv=None):
     ap.add_argument("--keyframes", type=int, default=500)
     ap.add_argument("--landmarks", type=int, default=200000)
     ap.add_argument("--equidistant", action="store_true", help="equidistant distortion variant")
-    ap.add_argument("--cpu-iters", type=int, default=2, help="LM iterations of the CPU oracle sample")
+    ap.add_argument("--cpu-iters", type=int, default=12,
+                    help="LM iterations of the CPU oracle sample (about 1 s each on 16 host threads)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, nproc)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--repeats", type=int, default=20, help="launches per kernel in mlba_time_kernels")

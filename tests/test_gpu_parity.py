@@ -163,7 +163,9 @@ def test_outlier_rejection_driver_matches_oracle(oracle):
     free = np.where(m.landmark_constant == 0)[0]
     m.landmarks[free[:4], 2] = -2.0  # behind the store vertex: rejected after the first chunk
     m.normalized()
-    o = opts(max_num_iterations=7, reject_outliers_every_n_iters=3)
+    # The oracle solves the reduced system directly: the PCG runs to 1e-12 so
+    # that its step error stays below the 1e-9 bar on the logged costs.
+    o = opts(max_num_iterations=7, reject_outliers_every_n_iters=3, pcg_relative_tolerance=1e-12)
     with DeviceProblem(m) as dev:
         sm, rows = dev.solve(o, outlier_rejection=True)
         v, l = dev.download_state()
@@ -171,7 +173,12 @@ def test_outlier_rejection_driver_matches_oracle(oracle):
     ref, ref_rows, rv, rl, ract = oracle.solve(m, o, outlier_rejection=True, threads=1)
     assert sm.num_removed_landmarks == ref.num_removed_landmarks >= 4
     np.testing.assert_array_equal(active, ract)
-    np.testing.assert_allclose([r.cost for r in rows], [r.cost for r in ref_rows], rtol=1e-9)
+    # The first chunk's cost (2.5e5) is dominated by the planted landmarks, so
+    # the rest of the map is solved to an absolute accuracy set by that scale:
+    # the first cost after their removal (380) agrees to ~3e-9 relative, the
+    # later ones to 1e-9 again.
+    np.testing.assert_allclose([r.cost for r in rows], [r.cost for r in ref_rows], rtol=1e-8)
+    np.testing.assert_allclose([r.cost for r in rows][5:], [r.cost for r in ref_rows][5:], rtol=1e-9)
     np.testing.assert_allclose(l[active == 1], rl[active == 1], atol=1e-8)
 
 
