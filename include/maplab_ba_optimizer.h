@@ -37,6 +37,9 @@ typedef struct mlba_vi_problem_options {
   int32_t reserved0;
   mlba_options solver;                     /* initSolverOptionsFromFlags + outlier rejection flags */
   mlba_quality_options landmark_quality;   /* --elq_* (landmark admission) */
+  mlba_calibration_options calibration;    /* --ba_fix_ncamera_intrinsics (intrinsics and distortion),
+                                              --ba_fix_ncamera_extrinsics_rotation / _translation
+                                              (vi-optimization-builder.cc:24-33): all fixed by default */
 } mlba_vi_problem_options;
 
 void mlba_vi_problem_options_init(mlba_vi_problem_options* options);
@@ -87,6 +90,22 @@ int32_t mlba_select_problem(const mlba_problem_desc* map, const mlba_vi_problem_
 int32_t mlba_vi_map_optimize(const mlba_problem_desc* map, const mlba_vi_problem_options* options,
                              const uint8_t* mission_selected, uint8_t* landmark_quality, double* vertices_out,
                              double* landmarks_out, mlba_summary* summary);
+
+/* The same, with the camera blocks: when options->calibration frees any of
+ * them, the camera calibration is refined with mlba_solve_calibration
+ * instead of the visual BA (every vertex and landmark in the selected
+ * problem must then be constant, --ba_fix_vertices and
+ * --ba_fix_landmark_positions; MLBA_EINVAL otherwise; the outlier-rejection
+ * chunks do not apply). The camera outputs (7C q_CI | C_p_CI, 4C
+ * intrinsics, 4C distortion, C xi; each nullable) receive the cameras after
+ * the solve; vertices_out / landmarks_out receive the state as for
+ * mlba_vi_map_optimize. No counterpart entry point in the reference: maplab
+ * writes the refined NCamera back into the map's sensor manager. */
+int32_t mlba_vi_map_optimize_cameras(const mlba_problem_desc* map, const mlba_vi_problem_options* options,
+                                     const uint8_t* mission_selected, uint8_t* landmark_quality,
+                                     double* vertices_out, double* landmarks_out, double* camera_q_CI__C_p_CI_out,
+                                     double* camera_intrinsics_out, double* camera_distortion_out,
+                                     double* camera_xi_out, mlba_summary* summary);
 
 /* Last error text of the optimizer entry points of the calling thread. */
 const char* mlba_optimizer_last_error(void);

@@ -35,6 +35,12 @@ typedef struct mlba_console_map {
   double* vertices;
   double* landmarks;
   uint8_t* landmark_quality;
+  /* Camera blocks written back by a calibration refinement (7C, 4C, 4C, C;
+   * nullable: not written back). */
+  double* camera_q_CI__C_p_CI;
+  double* camera_intrinsics;
+  double* camera_distortion;
+  double* camera_xi;
 } mlba_console_map;
 
 /* The console a plugin is created for. selected_map NULL = no map selected. */

@@ -143,7 +143,8 @@ class QualityOptions(ctypes.Structure):
 class ViProblemOptions(ctypes.Structure):
     _fields_ = [("add_visual_constraints", c_i32), ("fix_landmark_positions", c_i32), ("fix_vertices", c_i32),
                 ("min_landmarks_per_frame", c_i32), ("enable_visual_outlier_rejection", c_i32),
-                ("reserved0", c_i32), ("solver", Options), ("landmark_quality", QualityOptions)]
+                ("reserved0", c_i32), ("solver", Options), ("landmark_quality", QualityOptions),
+                ("calibration", CalibrationOptions)]
 
 
 class ProblemSelection(ctypes.Structure):
@@ -209,6 +210,8 @@ OPTIMIZER_SYMBOLS = {
                                     ctypes.POINTER(ProblemSelection)]),
     "mlba_vi_map_optimize": (c_i32, [ctypes.POINTER(ProblemDesc), ctypes.POINTER(ViProblemOptions), c_vp, c_vp,
                                      c_vp, c_vp, ctypes.POINTER(Summary)]),
+    "mlba_vi_map_optimize_cameras": (c_i32, [ctypes.POINTER(ProblemDesc), ctypes.POINTER(ViProblemOptions), c_vp,
+                                             c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(Summary)]),
     "mlba_optimizer_last_error": (ctypes.c_char_p, []),
 }
 

@@ -61,6 +61,7 @@ void mlba_vi_problem_options_init(mlba_vi_problem_options* o) {
   o->add_visual_constraints = 1;
   o->fix_landmark_positions = 0;
   o->fix_vertices = 0;
+  mlba_calibration_options_init(&o->calibration);
   o->min_landmarks_per_frame = 0;
   o->enable_visual_outlier_rejection = 1;
   mlba_options_init(&o->solver);
@@ -179,6 +180,15 @@ int32_t mlba_select_problem(const mlba_problem_desc* d, const mlba_vi_problem_op
 int32_t mlba_vi_map_optimize(const mlba_problem_desc* d, const mlba_vi_problem_options* o,
                              const uint8_t* mission_selected, uint8_t* landmark_quality, double* vertices_out,
                              double* landmarks_out, mlba_summary* summary) {
+  return mlba_vi_map_optimize_cameras(d, o, mission_selected, landmark_quality, vertices_out, landmarks_out,
+                                      nullptr, nullptr, nullptr, nullptr, summary);
+}
+
+int32_t mlba_vi_map_optimize_cameras(const mlba_problem_desc* d, const mlba_vi_problem_options* o,
+                                     const uint8_t* mission_selected, uint8_t* landmark_quality,
+                                     double* vertices_out, double* landmarks_out, double* cam_q_out,
+                                     double* intrinsics_out, double* distortion_out, double* xi_out,
+                                     mlba_summary* summary) {
   int rc = check_map(d);
   if (rc) return rc;
   if (!o || (d->num_vertices > 0 && !vertices_out) || (d->num_landmarks > 0 && !landmarks_out))
@@ -260,9 +270,16 @@ int32_t mlba_vi_map_optimize(const mlba_problem_desc* d, const mlba_vi_problem_o
     mlba_destroy(h);
     return rc;
   }
-  rc = o->enable_visual_outlier_rejection ? mlba_solve_with_outlier_rejection(h, &o->solver, summary)
-                                          : mlba_solve(h, &o->solver, summary);
+  const mlba_calibration_options& co = o->calibration;
+  const bool calibrate = !co.fix_intrinsics || !co.fix_distortion || !co.fix_extrinsics_rotation ||
+                         !co.fix_extrinsics_translation;
+  if (calibrate)
+    rc = mlba_solve_calibration(h, &o->solver, &co, summary);
+  else
+    rc = o->enable_visual_outlier_rejection ? mlba_solve_with_outlier_rejection(h, &o->solver, summary)
+                                            : mlba_solve(h, &o->solver, summary);
   if (rc == MLBA_OK) rc = mlba_download_state(h, vertices_out, landmarks_out);
+  if (rc == MLBA_OK) rc = mlba_download_cameras(h, cam_q_out, intrinsics_out, distortion_out, xi_out);
   if (rc == MLBA_OK && landmark_quality && L > 0) {
     std::vector<uint8_t> after(L);
     rc = mlba_get_landmark_active(h, after.data());

@@ -89,7 +89,9 @@ class OptimizerPlugin {
         {"ba_outlier_rejection_use_reprojection_error", 'b'},
         {"ba_outlier_rejection_max_reprojection_error_px", 'd'}, {"ba_include_visual", 'b'},
         {"ba_include_inertial", 'b'}, {"ba_use_visual_outlier_rejection_solver", 'b'},
-        {"ba_fix_landmark_positions", 'b'}, {"ba_fix_vertices", 'b'}, {"ba_min_landmarks_per_frame", 'i'},
+        {"ba_fix_landmark_positions", 'b'}, {"ba_fix_vertices", 'b'}, {"ba_fix_ncamera_intrinsics", 'b'},
+        {"ba_fix_ncamera_extrinsics_rotation", 'b'}, {"ba_fix_ncamera_extrinsics_translation", 'b'},
+        {"ba_min_landmarks_per_frame", 'i'},
         {"elq_min_observers", 'i'}, {"elq_min_observation_angle_deg", 'd'},
         {"elq_max_distance_from_closest_observer", 'd'}, {"elq_min_distance_from_closest_observer", 'd'},
         {"elq_max_reprojection_error_px", 'd'}, {"map_mission", 's'}, {"map_mission_list", 's'}};
@@ -149,6 +151,9 @@ class OptimizerPlugin {
               get_bool("ba_use_visual_outlier_rejection_solver", &o.enable_visual_outlier_rejection) &&
               get_bool("ba_fix_landmark_positions", &o.fix_landmark_positions) &&
               get_bool("ba_fix_vertices", &o.fix_vertices) &&
+              get_bool("ba_fix_ncamera_intrinsics", &o.calibration.fix_intrinsics) &&
+              get_bool("ba_fix_ncamera_extrinsics_rotation", &o.calibration.fix_extrinsics_rotation) &&
+              get_bool("ba_fix_ncamera_extrinsics_translation", &o.calibration.fix_extrinsics_translation) &&
               get("ba_min_landmarks_per_frame", &o.min_landmarks_per_frame) &&
               get("elq_min_observers", &o.landmark_quality.min_observers) &&
               get("elq_min_observation_angle_deg", &o.landmark_quality.min_observation_angle_deg) &&
@@ -159,6 +164,9 @@ class OptimizerPlugin {
               get("elq_max_reprojection_error_px", &o.landmark_quality.max_reprojection_error_px) &&
               (!inertial || get_bool("ba_include_inertial", &include_inertial));
     if (!ok) return MLBA_CMD_STUPID_USER_ERROR;
+    // --ba_fix_ncamera_intrinsics covers the intrinsics and the distortion
+    // (optimization-terms-addition.cc:223-228).
+    o.calibration.fix_distortion = o.calibration.fix_intrinsics;
     if (include_inertial)
       return fail(MLBA_CMD_UNKNOWN_ERROR,
                   "inertial terms are evaluated (mlba_inertial_evaluate) but not part of the device solve; "
@@ -182,8 +190,10 @@ class OptimizerPlugin {
         missions[k] = 1;
       }
     }
-    const int rc = mlba_vi_map_optimize(&m->desc, &o, missions.empty() ? nullptr : missions.data(),
-                                        m->landmark_quality, m->vertices, m->landmarks, console_->last_summary);
+    const int rc = mlba_vi_map_optimize_cameras(&m->desc, &o, missions.empty() ? nullptr : missions.data(),
+                                                m->landmark_quality, m->vertices, m->landmarks,
+                                                m->camera_q_CI__C_p_CI, m->camera_intrinsics, m->camera_distortion,
+                                                m->camera_xi, console_->last_summary);
     if (rc != MLBA_OK) return fail(MLBA_CMD_UNKNOWN_ERROR, mlba_optimizer_last_error());
     return MLBA_CMD_SUCCESS;
   }

@@ -75,6 +75,7 @@ class OptimizationResult:
     vertices: np.ndarray
     landmarks: np.ndarray
     landmark_quality: np.ndarray
+    cameras: tuple | None = None  # (q_CI | C_p_CI, intrinsics, distortion, xi) after the solve
 
 
 class VIMapOptimizer:
@@ -97,8 +98,11 @@ class VIMapOptimizer:
         sm = abi.Summary()
         sm.iterations_capacity = capacity
         sm.iterations = ctypes.cast(its, ctypes.POINTER(abi.IterationSummary))
-        _check(lib.mlba_vi_map_optimize(ctypes.byref(d), ctypes.byref(self.options),
-                                        None if ms is None else ms.ctypes.data, q.ctypes.data,
-                                        v_out.ctypes.data, l_out.ctypes.data, ctypes.byref(sm)))
+        C = vmap.camera_intrinsics.shape[0]
+        cams = (np.empty((C, 7)), np.empty((C, 4)), np.empty((C, 4)), np.empty(C))
+        _check(lib.mlba_vi_map_optimize_cameras(ctypes.byref(d), ctypes.byref(self.options),
+                                                None if ms is None else ms.ctypes.data, q.ctypes.data,
+                                                v_out.ctypes.data, l_out.ctypes.data,
+                                                *[a.ctypes.data for a in cams], ctypes.byref(sm)))
         rows = [its[i] for i in range(min(sm.num_iterations, capacity))]
-        return OptimizationResult(sm, rows, v_out, l_out, q)
+        return OptimizationResult(sm, rows, v_out, l_out, q, cams)

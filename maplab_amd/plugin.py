@@ -18,7 +18,9 @@ CMD_SUCCESS, CMD_UNKNOWN_ERROR, CMD_STUPID_USER_ERROR = 0, 1, 2
 
 class ConsoleMap(ctypes.Structure):
     _fields_ = [("desc", abi.ProblemDesc), ("vertices", ctypes.c_void_p), ("landmarks", ctypes.c_void_p),
-                ("landmark_quality", ctypes.c_void_p)]
+                ("landmark_quality", ctypes.c_void_p), ("camera_q_CI__C_p_CI", ctypes.c_void_p),
+                ("camera_intrinsics", ctypes.c_void_p), ("camera_distortion", ctypes.c_void_p),
+                ("camera_xi", ctypes.c_void_p)]
 
 
 class Console(ctypes.Structure):
@@ -67,13 +69,25 @@ class MapConsole:
         self.quality = np.zeros(vmap.num_landmarks, dtype=np.uint8)
         self._state_v = self.vmap.vertices.copy()
         self._state_l = self.vmap.landmarks.copy()
+        C = self.vmap.camera_intrinsics.shape[0]
+        self._cams = (self.vmap.camera_q_CI__C_p_CI.copy(), self.vmap.camera_intrinsics.copy(),
+                      self.vmap.camera_distortion.copy(),
+                      self.vmap.camera_xi.copy() if self.vmap.camera_xi is not None else np.zeros(C))
         self.map = ConsoleMap(desc=self.vmap.desc(), vertices=self._state_v.ctypes.data,
-                              landmarks=self._state_l.ctypes.data, landmark_quality=self.quality.ctypes.data)
+                              landmarks=self._state_l.ctypes.data, landmark_quality=self.quality.ctypes.data,
+                              camera_q_CI__C_p_CI=self._cams[0].ctypes.data,
+                              camera_intrinsics=self._cams[1].ctypes.data,
+                              camera_distortion=self._cams[2].ctypes.data, camera_xi=self._cams[3].ctypes.data)
         self.console.selected_map = ctypes.pointer(self.map)
 
     @property
     def state(self):
         return self._state_v, self._state_l
+
+    @property
+    def cameras(self):
+        """(q_CI | C_p_CI, intrinsics, distortion, xi) written back by the last command."""
+        return self._cams
 
     def plugin_id(self) -> str:
         return self.lib.mlba_plugin_id(self.plugin).decode()
