@@ -285,6 +285,15 @@ def main(argv=None):
     jsm, jrows = dev.solve(jo)
     n_pcg_jacobi = jsm.total_linear_solver_iterations
     same_log = [r.step_is_successful for r in rows] == [r.step_is_successful for r in jrows]
+    # The same LM iterations with FP32 Jacobian storage (timed the same way).
+    dev.upload_state(init_v, init_l)
+    fo = abi.default_options()
+    ctypes.pointer(fo)[0] = opts
+    fo.jacobian_fp32 = 1
+    t0 = time.perf_counter()
+    fsm, frows = dev.solve(fo)
+    f_wall = time.perf_counter() - t0
+    f_iters = fsm.num_iterations - 1
     table = kernel_table(sm, kt, n_pcg)
     lin = table.get("k_linearize", {})
     obs_j_per_s = vmap.num_observations / (lin["avg_ms"] / 1e3) if lin else None
@@ -329,6 +338,11 @@ def main(argv=None):
             "deflated": int(n_pcg), "block_jacobi": int(n_pcg_jacobi),
             "ratio": round(n_pcg_jacobi / max(1, n_pcg), 2), "same_lm_log": bool(same_log),
             "tolerance": opts.pcg_relative_tolerance},
+        "fp32_jacobians": {
+            "lm_iterations_per_s": round(f_iters / f_wall, 3), "ms_per_step": round(1e3 * f_wall / max(1, f_iters), 3),
+            "final_cost": fsm.final_cost,
+            "final_cost_rel_to_fp64": abs(fsm.final_cost - sm.final_cost) / abs(sm.final_cost),
+            "note": "mlba_options.jacobian_fp32: Jacobian records in FP32, all products and sums in FP64"},
         "lm_accepted": int(sm.num_successful_steps), "lm_rejected": int(sm.num_unsuccessful_steps),
         "initial_cost": sm.initial_cost, "final_cost": sm.final_cost,
         "kernels": table,

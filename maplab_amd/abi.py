@@ -66,6 +66,7 @@ class Options(ctypes.Structure):
         ("pcg_relative_tolerance", c_f64), ("reject_outliers_every_n_iters", c_i32),
         ("use_reprojection_error", c_i32), ("max_reprojection_error_px", c_f64),
         ("obs_tile", c_i32), ("schur_row_chunk", c_i32), ("pcg_coarse_nodes", c_i32), ("pcg_ritz_vectors", c_i32),
+        ("jacobian_fp32", c_i32),
     ]
 
 

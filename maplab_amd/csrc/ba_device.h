@@ -11,6 +11,29 @@ namespace mlba {
 
 constexpr int kObsTile = 256;        // observations per linearization tile
 constexpr int kObsRecord = 20;       // doubles per observation record: r(2) Jl(2x3) Jo(2x6)
+
+// The record with FP32 Jacobians (residual kept in FP64): 88 bytes.
+struct Rec32 {
+  double r[2];
+  float J[18];  // Jl (2x3) then Jo (2x6)
+};
+
+// Read access to observation record x: entry k as in the FP64 layout
+// (0..1 residual, 2..7 Jl, 8..19 Jo).
+template <bool kF32>
+struct RecView;
+template <>
+struct RecView<false> {
+  const double* p;
+  __host__ __device__ RecView(const void* base, size_t x) : p(static_cast<const double*>(base) + kObsRecord * x) {}
+  __host__ __device__ double operator[](int k) const { return p[k]; }
+};
+template <>
+struct RecView<true> {
+  const Rec32* p;
+  __host__ __device__ RecView(const void* base, size_t x) : p(static_cast<const Rec32*>(base) + x) {}
+  __host__ __device__ double operator[](int k) const { return k < 2 ? p->r[k] : static_cast<double>(p->J[k - 2]); }
+};
 constexpr int kLmAcc = 18;           // V(6) V_mission(6) g(3) g_mission(3)
 constexpr int kRowThreads = 256;     // threads of a reduced-camera-system row block
 

@@ -33,18 +33,18 @@ namespace mlba {
 void launch_vertex_prep(hipStream_t s, int32_t P, const double* vert, double* vR);
 void launch_linearize(hipStream_t s, bool jac, int32_t tiles, const DevTopology& t, const double* vert,
                       const double* vR, const double* lm, DevLoss loss, const int32_t* tile_begin, double* obs_rec,
-                      double* lm_acc, double* tile_cost);
+                      double* lm_acc, double* tile_cost, bool f32 = false);
 void launch_zero_split(hipStream_t s, int32_t n, const int32_t* ids, double* lm_acc);
 void launch_pose_rows(hipStream_t s, const DevTopology& t, const double* vert, const double* vR, const double* lm,
                       const double* obs_rec, const double* lm_acc, double* pose_grad, double* pose_diag,
-                      double* pose_jtj);
+                      double* pose_jtj, bool f32 = false);
 void launch_jacobi_scaling(hipStream_t s, int32_t Pvar, int32_t L, int32_t use, const double* pose_diag,
                            const double* lm_acc, double* pose_scale, double* lm_scale);
 void launch_landmark_prep(hipStream_t s, const DevTopology& t, const double* lm_acc, const double* lm_scale,
                           double inv_mu, double min_diag, double max_diag, double* lm_z, int32_t* fail);
 void launch_schur_factors(hipStream_t s, const DevTopology& t, int32_t n_inc, const int32_t* inc_row,
                           const double* vR, const double* lm, const double* obs_rec, const double* lm_acc,
-                          const double* pose_scale, double* inc_w, double* inc_r0, double* lm_a);
+                          const double* pose_scale, double* inc_w, double* inc_r0, double* lm_a, bool f32 = false);
 void launch_schur_incidences(hipStream_t s, const DevTopology& t, int32_t n_inc, const double* inc_w,
                              const double* inc_r0, const double* lm_a, const double* lm_acc, const double* lm_z,
                              double* inc_q, double* inc_rhs);
@@ -72,7 +72,7 @@ int32_t pcg_max_lanczos();
 int32_t launch_back_substitute(hipStream_t s, const DevTopology& t, int32_t tiles, const int32_t* tile_begin,
                                int32_t n_split, const int32_t* split_ids, const double* vR, const double* lm,
                                const double* obs_rec, const double* lm_acc, const double* lm_z, const double* w,
-                               double* u, double* part);
+                               double* u, double* part, bool f32 = false);
 void launch_unscale_poses(hipStream_t s, int32_t Pvar, const double* y, const double* pose_scale, double* w,
                           double* part);
 void launch_candidate(hipStream_t s, const DevTopology& t, const double* vert, const double* w, double* cand_v,
@@ -184,6 +184,9 @@ struct mlba_handle {
   // of the dispatch order (0: block order), items (waves) per work-group,
   // XCD-contiguous work-group mapping. It changes no result.
   int32_t sched_band = kItemBandRows, sched_waves = 4, sched_xcd = 1;
+  // Jacobian storage of the observation records for the current solve
+  // (mlba_options.jacobian_fp32): FP32 Jacobians, FP64 residuals.
+  bool f32_jac = false;
   DBuf<int32_t> hat_node, node_range, grp_lo, blk_hat, blk_rng;
   std::vector<int32_t> h_row_ptr, h_col_var;  // host copy of the S block structure
   int32_t grp_width = 0;                // hat columns of the per-group SW window
@@ -832,7 +835,7 @@ DevLoss make_loss(const mlba_options* o) {
 void schur_factors(mlba_handle* h) {
   KernelTimer kt(h, MLBA_KERNEL_SCHUR_FACTORS);
   launch_schur_factors(h->stream, h->topo, h->n_inc, h->inc_row.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p,
-                       h->pose_scale.p, h->inc_w.p, h->inc_r0.p, h->lm_a.p);
+                       h->pose_scale.p, h->inc_w.p, h->inc_r0.p, h->lm_a.p, h->f32_jac);
 }
 
 int linearize(mlba_handle* h, const mlba_options* o) {
@@ -842,12 +845,12 @@ int linearize(mlba_handle* h, const mlba_options* o) {
   {
     KernelTimer kt(h, MLBA_KERNEL_LINEARIZE);
     launch_linearize(s, true, h->tiles, h->topo, h->vert.p, h->vR.p, h->lm.p, make_loss(o), h->tile_begin.p,
-                     h->obs_rec.p, h->lm_acc.p, h->tile_cost.p);
+                     h->obs_rec.p, h->lm_acc.p, h->tile_cost.p, h->f32_jac);
   }
   {
     KernelTimer kt(h, MLBA_KERNEL_POSE_ROWS);
     launch_pose_rows(s, h->topo, h->vert.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p, h->pose_grad.p,
-                     h->pose_diag.p, h->pose_jtj.p);
+                     h->pose_diag.p, h->pose_jtj.p, h->f32_jac);
   }
   launch_reduce(s, h->tiles, 2, 0, false, h->tile_cost.p, h->scal.p + kCost);
   launch_reduce(s, h->tiles, 2, 1, false, h->tile_cost.p, h->scal.p + kFixed);
@@ -878,6 +881,7 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
   const double t_start = now_s();
   double radius = initial_radius, decrease_factor = 2.0;
   int rc;
+  h->f32_jac = o->jacobian_fp32 != 0;
   if ((rc = ensure_coarse_basis(h, o->pcg_coarse_nodes))) return rc;
   const double t_ev0 = now_s();
   if ((rc = linearize(h, o))) return rc;
@@ -955,7 +959,7 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
         KernelTimer kt(h, MLBA_KERNEL_BACK_SUBSTITUTE);
         nbs = launch_back_substitute(s, h->topo, h->tiles, h->tile_begin.p, h->n_split, h->split_ids.p, h->vR.p,
                                      h->lm.p, h->obs_rec.p, h->lm_acc.p, h->lm_z.p, h->w.p, h->lm_u.p,
-                                     h->parts_l.p);
+                                     h->parts_l.p, h->f32_jac);
       }
       launch_reduce(s, nbs, 2, 0, false, h->parts_l.p, h->scal.p + kMcc);
       launch_reduce(s, nbs, 2, 1, false, h->parts_l.p, h->scal.p + kUU);
@@ -1081,6 +1085,7 @@ int validate_options(mlba_handle* h, const mlba_options* o) {
   if (!o) return set_err(h, MLBA_EINVAL, "options is NULL");
   if (o->loss_type < 0 || o->loss_type > 2) return set_err(h, MLBA_EINVAL, "unknown loss_type");
   if (o->linear_solver_type != MLBA_SOLVER_SCHUR_PCG) return set_err(h, MLBA_EINVAL, "unknown linear_solver_type");
+  if (o->jacobian_fp32 != 0 && o->jacobian_fp32 != 1) return set_err(h, MLBA_EINVAL, "jacobian_fp32 must be 0 or 1");
   if (!(o->initial_trust_region_radius > 0.0)) return set_err(h, MLBA_EINVAL, "initial_trust_region_radius <= 0");
   if (o->loss_type == MLBA_LOSS_CAUCHY && !(o->loss_scale_px > 0.0)) return set_err(h, MLBA_EINVAL, "loss_scale_px <= 0");
   return MLBA_OK;
@@ -2042,6 +2047,7 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
   };
   const DevLoss loss = make_loss(o);
   const double inv_mu = 1.0 / o->initial_trust_region_radius;
+  h->f32_jac = o->jacobian_fp32 != 0;
   if ((rc = linearize(h, o))) return rc;
   launch_jacobi_scaling(s, h->Pvar, h->L, o->use_jacobi_scaling, h->pose_diag.p, h->lm_acc.p, h->pose_scale.p,
                         h->lm_scale.p);
@@ -2050,7 +2056,7 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
   t->linearize_ms = timed([&] {
     launch_zero_split(s, h->n_split, h->split_ids.p, h->lm_acc.p);
     launch_linearize(s, true, h->tiles, h->topo, h->vert.p, h->vR.p, h->lm.p, loss, h->tile_begin.p, h->obs_rec.p,
-                     h->lm_acc.p, h->tile_cost.p);
+                     h->lm_acc.p, h->tile_cost.p, h->f32_jac);
   });
   t->cost_ms = timed([&] {
     launch_linearize(s, false, h->tiles, h->topo, h->vert.p, h->vR.p, h->lm.p, loss, h->tile_begin.p, nullptr,
@@ -2058,7 +2064,7 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
   });
   t->pose_rows_ms = timed([&] {
     launch_pose_rows(s, h->topo, h->vert.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p, h->pose_grad.p,
-                     h->pose_diag.p, h->pose_jtj.p);
+                     h->pose_diag.p, h->pose_jtj.p, h->f32_jac);
   });
   HIP_OK(hipMemsetAsync(h->fail.p, 0, sizeof(int32_t), s));
   t->landmark_prep_ms = timed([&] {
@@ -2109,7 +2115,8 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
   launch_unscale_poses(s, h->Pvar, h->px.p, h->pose_scale.p, h->w.p, h->parts_w.p);
   t->back_substitute_ms = timed([&] {
     (void)launch_back_substitute(s, h->topo, h->tiles, h->tile_begin.p, h->n_split, h->split_ids.p, h->vR.p,
-                                 h->lm.p, h->obs_rec.p, h->lm_acc.p, h->lm_z.p, h->w.p, h->lm_u.p, h->parts_l.p);
+                                 h->lm.p, h->obs_rec.p, h->lm_acc.p, h->lm_z.p, h->w.p, h->lm_u.p, h->parts_l.p,
+                                 h->f32_jac);
   });
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(s));

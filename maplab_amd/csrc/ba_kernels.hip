@@ -204,12 +204,17 @@ __device__ inline double loss_scale(const DevLoss& loss, int32_t type, double si
 // Per observation: robust-corrected residual and Jacobians into obs_rec;
 // per landmark: V = sum Jl'Jl, V_mission (pose-coupled observations only),
 // g = sum Jl'r, g_mission; per tile: cost and fixed cost.
-template <bool kJac>
+//
+// With kF32 the Jacobians are rounded to FP32 right after the correction and
+// every later product (V, g here; J'J, W, the model cost in the readers)
+// uses the rounded values, so S stays the Schur complement of J'J of one
+// Jacobian; records are Rec32 (residual in FP64).
+template <bool kJac, bool kF32 = false>
 __global__ __launch_bounds__(kObsTile) void k_linearize(DevTopology t, const double* __restrict__ vert,
                                                          const double* __restrict__ vR,
                                                          const double* __restrict__ lm, DevLoss loss,
                                                          const int32_t* __restrict__ tile_begin,
-                                                         double* __restrict__ obs_rec,
+                                                         void* __restrict__ obs_rec,
                                                          double* __restrict__ lm_acc,
                                                          double* __restrict__ tile_cost) {
   __shared__ double red[kJac ? kObsTile * kLmAcc : 1];
@@ -256,10 +261,19 @@ __global__ __launch_bounds__(kObsTile) void k_linearize(DevTopology t, const dou
         };
         correct(ev.Jl, 3);
         if (pose_coupled) correct(ev.Jo, 6);
-        double* rec = obs_rec + (size_t)kObsRecord * o;
-        rec[0] = rr[0]; rec[1] = rr[1];
-        for (int i = 0; i < 6; ++i) rec[2 + i] = ev.Jl[i];
-        for (int i = 0; i < 12; ++i) rec[8 + i] = ev.Jo[i];
+        if (kF32) {
+          for (int i = 0; i < 6; ++i) ev.Jl[i] = (double)(float)ev.Jl[i];
+          for (int i = 0; i < 12; ++i) ev.Jo[i] = (double)(float)ev.Jo[i];
+          Rec32* rec = static_cast<Rec32*>(obs_rec) + o;
+          rec->r[0] = rr[0]; rec->r[1] = rr[1];
+          for (int i = 0; i < 6; ++i) rec->J[i] = (float)ev.Jl[i];
+          for (int i = 0; i < 12; ++i) rec->J[6 + i] = (float)ev.Jo[i];
+        } else {
+          double* rec = static_cast<double*>(obs_rec) + (size_t)kObsRecord * o;
+          rec[0] = rr[0]; rec[1] = rr[1];
+          for (int i = 0; i < 6; ++i) rec[2 + i] = ev.Jl[i];
+          for (int i = 0; i < 12; ++i) rec[8 + i] = ev.Jo[i];
+        }
         const double* J = ev.Jl;
         double V[6], g[3];
         V[0] = J[0] * J[0] + J[3] * J[3];
@@ -277,8 +291,14 @@ __global__ __launch_bounds__(kObsTile) void k_linearize(DevTopology t, const dou
         }
       }
     } else if (kJac) {
-      double* rec = obs_rec + (size_t)kObsRecord * o;
-      for (int i = 0; i < kObsRecord; ++i) rec[i] = 0.0;
+      if (kF32) {
+        Rec32* rec = static_cast<Rec32*>(obs_rec) + o;
+        rec->r[0] = rec->r[1] = 0.0;
+        for (int i = 0; i < 18; ++i) rec->J[i] = 0.0f;
+      } else {
+        double* rec = static_cast<double*>(obs_rec) + (size_t)kObsRecord * o;
+        for (int i = 0; i < kObsRecord; ++i) rec[i] = 0.0;
+      }
     }
   }
   if (kJac) {
@@ -319,10 +339,11 @@ __global__ void k_zero_split(int32_t n, const int32_t* __restrict__ ids, double*
 // Per variable vertex: unscaled gradient g = J'r and diag(J'J) of its six
 // tangent columns, over its incidences (observations it makes and the
 // landmarks anchored in it).
+template <bool kF32>
 __global__ __launch_bounds__(kRowThreads) void k_pose_rows(DevTopology t, const double* __restrict__ vert,
                                                             const double* __restrict__ vR,
                                                             const double* __restrict__ lm,
-                                                            const double* __restrict__ obs_rec,
+                                                            const void* __restrict__ obs_rec,
                                                             const double* __restrict__ lm_acc,
                                                             double* __restrict__ pose_grad,
                                                             double* __restrict__ pose_diag,
@@ -336,8 +357,9 @@ __global__ __launch_bounds__(kRowThreads) void k_pose_rows(DevTopology t, const 
   for (int k = ib + threadIdx.x; k < ie; k += kRowThreads) {
     const int32_t x = t.inc[k];
     if (x >= 0) {
-      const double* rec = obs_rec + (size_t)kObsRecord * x;
-      const double* Jo = rec + 8;
+      const RecView<kF32> rec(obs_rec, x);
+      double Jo[12];
+      for (int k2 = 0; k2 < 12; ++k2) Jo[k2] = rec[8 + k2];
       for (int c = 0; c < 6; ++c) {
         g[c] += Jo[c] * rec[0] + Jo[6 + c] * rec[1];
         d[c] += Jo[c] * Jo[c] + Jo[6 + c] * Jo[6 + c];
@@ -434,11 +456,10 @@ __global__ void k_landmark_prep(DevTopology t, const double* __restrict__ lm_acc
 // ------------------------------------------------------------ Schur complement
 
 // W' = Jl^T (Jo C) (3x6): landmark side unscaled, pose side scaled.
-__device__ inline void wprime_obs(const double* rec, const double* cs, double* W) {
-  const double* Jl = rec + 2;
-  const double* Jo = rec + 8;
+template <class Rec>
+__device__ inline void wprime_obs(const Rec& rec, const double* cs, double* W) {
   for (int a = 0; a < 3; ++a)
-    for (int c = 0; c < 6; ++c) W[a * 6 + c] = (Jl[a] * Jo[c] + Jl[3 + a] * Jo[6 + c]) * cs[c];
+    for (int c = 0; c < 6; ++c) W[a * 6 + c] = (rec[2 + a] * rec[8 + c] + rec[5 + a] * rec[14 + c]) * cs[c];
 }
 
 // Incidence factors of the reduced camera system
@@ -455,9 +476,10 @@ __device__ inline void wprime_obs(const double* rec, const double* cs, double* W
 // anchor-lifted blocks). W and r0 change only with the linearization and
 // are stored in incidence order (rows of S, then landmarks), so the block
 // kernel streams them; A_l is stored per landmark.
+template <bool kF32>
 __global__ __launch_bounds__(256) void k_schur_factors(
     DevTopology t, int32_t n_inc, const int32_t* __restrict__ inc_row, const double* __restrict__ vR,
-    const double* __restrict__ lm, const double* __restrict__ obs_rec, const double* __restrict__ lm_acc,
+    const double* __restrict__ lm, const void* __restrict__ obs_rec, const double* __restrict__ lm_acc,
     const double* __restrict__ pose_scale, double* __restrict__ inc_w, double* __restrict__ inc_r0,
     double* __restrict__ lm_a) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -467,10 +489,9 @@ __global__ __launch_bounds__(256) void k_schur_factors(
   const double* cs = pose_scale + 6 * i;
   double W[18], r0[6];
   if (x >= 0) {
-    const double* rec = obs_rec + (size_t)kObsRecord * x;
+    const RecView<kF32> rec(obs_rec, x);
     wprime_obs(rec, cs, W);
-    const double* Jo = rec + 8;
-    for (int c = 0; c < 6; ++c) r0[c] = cs[c] * (Jo[c] * rec[0] + Jo[6 + c] * rec[1]);
+    for (int c = 0; c < 6; ++c) r0[c] = cs[c] * (rec[8 + c] * rec[0] + rec[14 + c] * rec[1]);
   } else {
     const int32_t l = -1 - x;
     const int v = t.var_vertex[i];
@@ -888,10 +909,11 @@ __device__ inline void landmark_step(const DevTopology& t, int l, const double* 
   }
 }
 
+template <bool kF32>
 __global__ __launch_bounds__(kObsTile) void k_back_substitute(DevTopology t, const int32_t* __restrict__ tile_begin,
                                                               const double* __restrict__ vR,
                                                               const double* __restrict__ lm,
-                                                              const double* __restrict__ obs_rec,
+                                                              const void* __restrict__ obs_rec,
                                                               const double* __restrict__ lm_acc,
                                                               const double* __restrict__ lm_z,
                                                               const double* __restrict__ w, double* __restrict__ u_out,
@@ -917,7 +939,7 @@ __global__ __launch_bounds__(kObsTile) void k_back_substitute(DevTopology t, con
   }
   const bool active = o < e && (flags & kLmActive);
   if (active) {
-    const double* rec = obs_rec + (size_t)kObsRecord * o;
+    const RecView<kF32> rec(obs_rec, o);
     const int32_t meta = t.obs_meta[o];
     coupled = obs_type(meta) != kTypeKeyframe;
     ovar = coupled && obs_observer_var(meta);
@@ -971,11 +993,12 @@ __global__ __launch_bounds__(kObsTile) void k_back_substitute(DevTopology t, con
 }
 
 // The same for the landmarks split over several tiles, one thread each.
+template <bool kF32>
 __global__ __launch_bounds__(256) void k_back_substitute_split(DevTopology t, int32_t n_split,
                                                                const int32_t* __restrict__ split_ids,
                                                                const double* __restrict__ vR,
                                                                const double* __restrict__ lm,
-                                                               const double* __restrict__ obs_rec,
+                                                               const void* __restrict__ obs_rec,
                                                                const double* __restrict__ lm_acc,
                                                                const double* __restrict__ lm_z,
                                                                const double* __restrict__ w,
@@ -991,7 +1014,7 @@ __global__ __launch_bounds__(256) void k_back_substitute_split(DevTopology t, in
     for (int o = lb; o < le; ++o) {
       const int32_t meta = t.obs_meta[o];
       if (obs_type(meta) == kTypeKeyframe || !obs_observer_var(meta)) continue;
-      const double* rec = obs_rec + (size_t)kObsRecord * o;
+      const RecView<kF32> rec(obs_rec, o);
       const double* wo = w + 6 * t.vertex_var[t.obs_vertex[o]];
       double jw[2];
       for (int rr = 0; rr < 2; ++rr) {
@@ -1007,7 +1030,7 @@ __global__ __launch_bounds__(256) void k_back_substitute_split(DevTopology t, in
     for (int a = 0; a < 3; ++a) u_out[3 * l + a] = u[a];
     for (int o = lb; o < le; ++o) {
       const int32_t meta = t.obs_meta[o];
-      const double* rec = obs_rec + (size_t)kObsRecord * o;
+      const RecView<kF32> rec(obs_rec, o);
       const bool coupled = obs_type(meta) != kTypeKeyframe;
       double lu[3];
       for (int a = 0; a < 3; ++a) lu[a] = u[a] + (coupled ? Mwa[a] : 0.0);
@@ -1365,9 +1388,11 @@ void launch_vertex_prep(hipStream_t s, int32_t P, const double* vert, double* vR
 }
 void launch_linearize(hipStream_t s, bool jac, int32_t tiles, const DevTopology& t, const double* vert,
                       const double* vR, const double* lm, DevLoss loss, const int32_t* tile_begin, double* obs_rec,
-                      double* lm_acc, double* tile_cost) {
+                      double* lm_acc, double* tile_cost, bool f32) {
   if (tiles <= 0) return;
-  if (jac)
+  if (jac && f32)
+    k_linearize<true, true><<<tiles, kObsTile, 0, s>>>(t, vert, vR, lm, loss, tile_begin, obs_rec, lm_acc, tile_cost);
+  else if (jac)
     k_linearize<true><<<tiles, kObsTile, 0, s>>>(t, vert, vR, lm, loss, tile_begin, obs_rec, lm_acc, tile_cost);
   else
     k_linearize<false><<<tiles, kObsTile, 0, s>>>(t, vert, vR, lm, loss, tile_begin, obs_rec, lm_acc, tile_cost);
@@ -1377,9 +1402,14 @@ void launch_zero_split(hipStream_t s, int32_t n, const int32_t* ids, double* lm_
 }
 void launch_pose_rows(hipStream_t s, const DevTopology& t, const double* vert, const double* vR, const double* lm,
                       const double* obs_rec, const double* lm_acc, double* pose_grad, double* pose_diag,
-                      double* pose_jtj) {
-  if (t.Pvar > 0)
-    k_pose_rows<<<t.Pvar, kRowThreads, 0, s>>>(t, vert, vR, lm, obs_rec, lm_acc, pose_grad, pose_diag, pose_jtj);
+                      double* pose_jtj, bool f32) {
+  if (t.Pvar <= 0) return;
+  if (f32)
+    k_pose_rows<true><<<t.Pvar, kRowThreads, 0, s>>>(t, vert, vR, lm, obs_rec, lm_acc, pose_grad, pose_diag,
+                                                     pose_jtj);
+  else
+    k_pose_rows<false><<<t.Pvar, kRowThreads, 0, s>>>(t, vert, vR, lm, obs_rec, lm_acc, pose_grad, pose_diag,
+                                                      pose_jtj);
 }
 void launch_jacobi_scaling(hipStream_t s, int32_t Pvar, int32_t L, int32_t use, const double* pose_diag,
                            const double* lm_acc, double* pose_scale, double* lm_scale) {
@@ -1393,10 +1423,14 @@ void launch_landmark_prep(hipStream_t s, const DevTopology& t, const double* lm_
 }
 void launch_schur_factors(hipStream_t s, const DevTopology& t, int32_t n_inc, const int32_t* inc_row,
                           const double* vR, const double* lm, const double* obs_rec, const double* lm_acc,
-                          const double* pose_scale, double* inc_w, double* inc_r0, double* lm_a) {
-  if (t.Pvar > 0 && n_inc > 0)
-    k_schur_factors<<<(n_inc + 255) / 256, 256, 0, s>>>(t, n_inc, inc_row, vR, lm, obs_rec, lm_acc, pose_scale,
-                                                        inc_w, inc_r0, lm_a);
+                          const double* pose_scale, double* inc_w, double* inc_r0, double* lm_a, bool f32) {
+  if (t.Pvar <= 0 || n_inc <= 0) return;
+  if (f32)
+    k_schur_factors<true><<<(n_inc + 255) / 256, 256, 0, s>>>(t, n_inc, inc_row, vR, lm, obs_rec, lm_acc,
+                                                              pose_scale, inc_w, inc_r0, lm_a);
+  else
+    k_schur_factors<false><<<(n_inc + 255) / 256, 256, 0, s>>>(t, n_inc, inc_row, vR, lm, obs_rec, lm_acc,
+                                                               pose_scale, inc_w, inc_r0, lm_a);
 }
 void launch_schur_incidences(hipStream_t s, const DevTopology& t, int32_t n_inc, const double* inc_w,
                              const double* inc_r0, const double* lm_a, const double* lm_acc, const double* lm_z,
@@ -1462,13 +1496,23 @@ void launch_block_inverse(hipStream_t s, int32_t Pvar, const int32_t* diag_pos, 
 int32_t launch_back_substitute(hipStream_t s, const DevTopology& t, int32_t tiles, const int32_t* tile_begin,
                                int32_t n_split, const int32_t* split_ids, const double* vR, const double* lm,
                                const double* obs_rec, const double* lm_acc, const double* lm_z, const double* w,
-                               double* u, double* part) {
-  if (tiles > 0)
-    k_back_substitute<<<tiles, kObsTile, 0, s>>>(t, tile_begin, vR, lm, obs_rec, lm_acc, lm_z, w, u, part);
+                               double* u, double* part, bool f32) {
+  if (tiles > 0) {
+    if (f32)
+      k_back_substitute<true><<<tiles, kObsTile, 0, s>>>(t, tile_begin, vR, lm, obs_rec, lm_acc, lm_z, w, u, part);
+    else
+      k_back_substitute<false><<<tiles, kObsTile, 0, s>>>(t, tile_begin, vR, lm, obs_rec, lm_acc, lm_z, w, u, part);
+  }
   const int sb = (n_split + 255) / 256;
-  if (sb > 0)
-    k_back_substitute_split<<<sb, 256, 0, s>>>(t, n_split, split_ids, vR, lm, obs_rec, lm_acc, lm_z, w, u,
-                                               part + 2 * (size_t)std::max(tiles, 0));
+  double* part_split = part + 2 * (size_t)std::max(tiles, 0);
+  if (sb > 0) {
+    if (f32)
+      k_back_substitute_split<true><<<sb, 256, 0, s>>>(t, n_split, split_ids, vR, lm, obs_rec, lm_acc, lm_z, w, u,
+                                                       part_split);
+    else
+      k_back_substitute_split<false><<<sb, 256, 0, s>>>(t, n_split, split_ids, vR, lm, obs_rec, lm_acc, lm_z, w,
+                                                        u, part_split);
+  }
   return std::max(tiles, 0) + sb;
 }
 void launch_unscale_poses(hipStream_t s, int32_t Pvar, const double* y, const double* pose_scale, double* w,
