@@ -231,7 +231,7 @@ def main(argv=None):
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding of the path is described in DESIGN.md, not run)", file=sys.stderr)
+              "(how the path would shard over several GPUs is documented as a design and not run)", file=sys.stderr)
         return 2
 
     from maplab_amd import abi, problem
