@@ -64,10 +64,13 @@ def observation_case(rng, error_type, dist_type, bf_translation=True):
     return meas, sigma, params
 
 
-def tiny_map(rng, dist_type=1, n_vertices=6, n_landmarks=40, global_missions=False, noise=0.5, xi=None):
+def tiny_map(rng, dist_type=1, n_vertices=6, n_landmarks=40, global_missions=False, noise=0.5, xi=None,
+             n_cameras=1):
     """A small map whose landmarks are stored in their first observer; with
     global_missions, odd vertices live in a second mission (global error type);
-    with xi, the camera is a unified-projection camera of that xi."""
+    with xi, the camera is a unified-projection camera of that xi; with
+    n_cameras = 2, a second camera of the rig (rotated 5 degrees about y,
+    other intrinsics) observes every landmark its vertex sees as well."""
     verts = []
     for k in range(n_vertices):
         ang = 2 * np.pi * k / n_vertices
@@ -90,6 +93,10 @@ def tiny_map(rng, dist_type=1, n_vertices=6, n_landmarks=40, global_missions=Fal
     obs_v, obs_l, kps = [], [], []
     store = np.zeros(n_landmarks, dtype=np.int32)
     cam = np.array([0, 0, 0, 1.0, 0.05, 0, 0])
+    th = np.deg2rad(5.0)
+    cams = [cam, np.array([0, np.sin(th / 2), 0, np.cos(th / 2), -0.05, 0.01, 0.0])][:n_cameras]
+    intrs = [INTR, INTR * np.array([1.02, 0.98, 1.0, 1.0])][:n_cameras]
+    obs_c = []
 
     def to_mission(p_W, m):  # world == mission 0 frame; mission 1 via its baseframe
         if m == 0:
@@ -102,14 +109,17 @@ def tiny_map(rng, dist_type=1, n_vertices=6, n_landmarks=40, global_missions=Fal
         for k in range(n_vertices):
             p_M = to_mission(world[l], missions[k])
             p_I = ind.R_jpl(verts[k, :4]) @ (p_M - verts[k, 4:])
-            p_C = p_I + cam[4:]
-            if p_C[2] < 0.5:
-                continue
-            kp = ind.project(dist_type, INTR, CAMERAS[dist_type], p_C, xi=xi)
-            if not (0 <= kp[0] < WIDTH and 0 <= kp[1] < HEIGHT):
-                continue
-            obs_v.append(k); obs_l.append(l); kps.append(kp + rng.normal(size=2) * noise)
-            if first is None:
+            seen = False
+            for ci, (cq, ki) in enumerate(zip(cams, intrs)):
+                p_C = ind.R_jpl(cq[:4]) @ p_I + cq[4:]
+                if p_C[2] < 0.5:
+                    continue
+                kp = ind.project(dist_type, ki, CAMERAS[dist_type], p_C, xi=xi)
+                if not (0 <= kp[0] < WIDTH and 0 <= kp[1] < HEIGHT):
+                    continue
+                obs_v.append(k); obs_l.append(l); obs_c.append(ci); kps.append(kp + rng.normal(size=2) * noise)
+                seen = True
+            if seen and first is None:
                 first = k
         store[l] = first if first is not None else 0
     lms = np.zeros((n_landmarks, 3))
@@ -122,17 +132,18 @@ def tiny_map(rng, dist_type=1, n_vertices=6, n_landmarks=40, global_missions=Fal
     first0 = np.where(store == 0)[0]
     if len(first0):
         lconst[first0[0]] = 1
+    C = len(cams)
     m = VisualMap(
-        vertices=verts, baseframes=bfs, landmarks=lms, camera_q_CI__C_p_CI=cam[None, :],
-        camera_intrinsics=INTR[None, :], camera_distortion=CAMERAS[dist_type][None, :],
-        camera_distortion_type=np.array([dist_type], dtype=np.int32),
-        camera_resolution=np.array([[WIDTH, HEIGHT]], dtype=np.int32), vertex_mission=missions,
+        vertices=verts, baseframes=bfs, landmarks=lms, camera_q_CI__C_p_CI=np.array(cams),
+        camera_intrinsics=np.array(intrs), camera_distortion=np.tile(CAMERAS[dist_type], (C, 1)),
+        camera_distortion_type=np.full(C, dist_type, dtype=np.int32),
+        camera_resolution=np.tile([WIDTH, HEIGHT], (C, 1)).astype(np.int32), vertex_mission=missions,
         landmark_store_vertex=store, obs_vertex=np.array(obs_v, dtype=np.int32),
-        obs_landmark=np.array(obs_l, dtype=np.int32), obs_camera=np.zeros(len(obs_v), dtype=np.int32),
+        obs_landmark=np.array(obs_l, dtype=np.int32), obs_camera=np.array(obs_c, dtype=np.int32),
         obs_keypoint=np.array(kps), obs_sigma=np.full(len(obs_v), 0.5), vertex_constant=vconst,
         landmark_constant=lconst, true_vertices=verts.copy(), true_landmarks=lms.copy(),
-        camera_model=None if xi is None else np.array([1], dtype=np.int32),
-        camera_xi=None if xi is None else np.array([xi]))
+        camera_model=None if xi is None else np.full(C, 1, dtype=np.int32),
+        camera_xi=None if xi is None else np.full(C, xi))
     # corrupt
     m.vertices = m.vertices.copy()
     for k in range(1, n_vertices):

@@ -127,6 +127,49 @@ def test_unified_projection_solve_matches_oracle(oracle, dist_type, xi):
     np.testing.assert_allclose(l, rl, atol=1e-8)
 
 
+def test_two_camera_rig_matches_oracle(oracle):
+    # Two cameras of one rig with different extrinsics and intrinsics; every
+    # landmark a vertex sees is observed by both (two observations of one
+    # landmark from one vertex), across two missions.
+    m = tiny_map(np.random.default_rng(78), dist_type=2, n_vertices=6, n_landmarks=60, global_missions=True,
+                 n_cameras=2)
+    print(f"two-camera rig: N {m.num_observations}", flush=True)
+    with DeviceProblem(m) as dev:
+        res, jac, cost = dev.evaluate(tangent=True)
+    ref_res, ref_jac, ref_cost = oracle.problem_evaluate(m, tangent=True, options=abi.default_options())
+    assert rel_err(res, ref_res) < 1e-12
+    for b in range(9):
+        if np.abs(ref_jac[b]).max() > 0:
+            assert rel_err(jac[b], ref_jac[b]) < 1e-9, b
+    o = opts(max_num_iterations=10, pcg_relative_tolerance=1e-12)
+    with DeviceProblem(m) as dev:
+        sm, rows = dev.solve(o)
+        v, l = dev.download_state()
+    ref, ref_rows, rv, rl, _ = oracle.solve(m, o, linear_solver=oracle.SCHUR_DIRECT, threads=1)
+    assert sm.num_iterations == ref.num_iterations
+    np.testing.assert_allclose([r.cost for r in rows], [r.cost for r in ref_rows], rtol=1e-9)
+    np.testing.assert_allclose(v, rv, atol=1e-8)
+    np.testing.assert_allclose(l, rl, atol=1e-8)
+
+
+def test_landmarks_split_over_tiles_match_oracle(oracle):
+    # 300 vertices around 12 landmarks: every landmark has 300 observations,
+    # more than one 256-observation linearization tile, so its sums and its
+    # back-substitution take the split path.
+    m = tiny_map(np.random.default_rng(77), dist_type=1, n_vertices=300, n_landmarks=12)
+    assert np.bincount(m.obs_landmark).min() > 256
+    o = opts(max_num_iterations=6, pcg_relative_tolerance=1e-12)
+    print(f"split landmarks: N {m.num_observations}", flush=True)
+    with DeviceProblem(m) as dev:
+        sm, rows = dev.solve(o)
+        v, l = dev.download_state()
+    ref, ref_rows, rv, rl, _ = oracle.solve(m, o, linear_solver=oracle.SCHUR_DIRECT, threads=8)
+    assert sm.num_iterations == ref.num_iterations
+    np.testing.assert_allclose([r.cost for r in rows], [r.cost for r in ref_rows], rtol=1e-9)
+    np.testing.assert_allclose(v, rv, atol=1e-8)
+    np.testing.assert_allclose(l, rl, atol=1e-8)
+
+
 def test_workload1_final_cost_matches_oracle(oracle):
     # 10 keyframes on a 1 m circle, 2000 landmarks, radtan, Cauchy(1 px), 10 iterations.
     m = problem.workload_map(1, seed=1)
