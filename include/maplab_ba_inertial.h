@@ -49,6 +49,28 @@ int32_t mlba_inertial_evaluate(const mlba_inertial_problem* problem, double* res
 
 const char* mlba_inertial_last_error(void);
 
+/* optvi's speed and bias blocks with the keyframe poses held constant (as
+ * with --ba_fix_vertices): the velocity and IMU biases of every vertex of
+ * the edges are optimized over the inertial terms alone, with the Ceres LM
+ * rules of mlba_solve on the block-tridiagonal normal equations of a
+ * keyframe chain (every edge must join vertex i to vertex i + 1). The fix
+ * flags are --ba_fix_gyro_bias, --ba_fix_accel_bias, --ba_fix_velocity
+ * (vi-optimization-builder.cc:42-50, optimization-terms-addition.cc:
+ * 468-485). The summary's costs are those of the inertial terms; the
+ * optimized states go to the nullable outputs (3 P each). Refining poses
+ * with the inertial terms (the full optvi problem) is not supported. */
+typedef struct mlba_inertial_options {
+  int32_t fix_gyro_bias;
+  int32_t fix_accel_bias;
+  int32_t fix_velocity;
+  int32_t reserved0;
+} mlba_inertial_options;
+
+void mlba_inertial_options_init(mlba_inertial_options* options);  /* nothing fixed (maplab's defaults) */
+int32_t mlba_solve_inertial_states(const mlba_inertial_problem* problem, const mlba_options* options,
+                                   const mlba_inertial_options* inertial, double* gyro_bias_out,
+                                   double* velocity_out, double* accel_bias_out, mlba_summary* summary);
+
 #ifdef __cplusplus
 }
 #endif

@@ -217,8 +217,16 @@ OPTIMIZER_SYMBOLS = {
 }
 
 # Every symbol include/maplab_ba_inertial.h declares (exported by the same library).
+class InertialOptions(ctypes.Structure):
+    _fields_ = [("fix_gyro_bias", c_i32), ("fix_accel_bias", c_i32), ("fix_velocity", c_i32), ("reserved0", c_i32)]
+
+
 INERTIAL_SYMBOLS = {
     "mlba_inertial_evaluate": (c_i32, [ctypes.POINTER(InertialProblem), c_vp, c_vp]),
+    "mlba_inertial_options_init": (None, [ctypes.POINTER(InertialOptions)]),
+    "mlba_solve_inertial_states": (c_i32, [ctypes.POINTER(InertialProblem), ctypes.POINTER(Options),
+                                           ctypes.POINTER(InertialOptions), c_vp, c_vp, c_vp,
+                                           ctypes.POINTER(Summary)]),
     "mlba_inertial_last_error": (ctypes.c_char_p, []),
 }
 

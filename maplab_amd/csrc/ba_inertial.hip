@@ -13,6 +13,9 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
 #include <new>
 #include <string>
 #include <vector>
@@ -386,6 +389,306 @@ int32_t mlba_inertial_evaluate(const mlba_inertial_problem* p, double* residuals
                         hipSuccess)
       return fail(MLBA_EDEVICE, "copy failed");
   if (nfail) return fail(MLBA_ESTATE, "accumulated IMU covariance is not positive definite");
+  return MLBA_OK;
+}
+
+void mlba_inertial_options_init(mlba_inertial_options* o) {
+  if (!o) return;
+  o->fix_gyro_bias = 0;
+  o->fix_accel_bias = 0;
+  o->fix_velocity = 0;
+  o->reserved0 = 0;
+}
+
+// Speed and bias states of a keyframe chain with the poses constant: the
+// inertial residuals of every edge (device, mlba_inertial_evaluate) and an
+// LM with the Ceres rules of mlba_solve on the block-tridiagonal normal
+// equations of the free [b_g | v_M | b_a] columns of each vertex.
+int32_t mlba_solve_inertial_states(const mlba_inertial_problem* p, const mlba_options* o,
+                                   const mlba_inertial_options* io, double* gyro_bias_out, double* velocity_out,
+                                   double* accel_bias_out, mlba_summary* sm) {
+  using namespace mlba;
+  if (!p || !o || !io) return fail(MLBA_EINVAL, "NULL argument");
+  const int P = p->num_vertices, E = p->num_edges;
+  if (P <= 0 || E <= 0) return fail(MLBA_EINVAL, "no inertial edges");
+  for (int e = 0; e < E; ++e)
+    if (p->edge_from[e] < 0 || p->edge_from[e] >= P || p->edge_to[e] != p->edge_from[e] + 1)
+      return fail(MLBA_EINVAL, "inertial edges must join consecutive vertices (a keyframe chain)");
+  // Free columns of a vertex: b_g (0..2), v_M (3..5), b_a (6..8).
+  int cols[9], nd = 0;
+  for (int g = 0; g < 3; ++g) {
+    const bool fixed = g == 0 ? io->fix_gyro_bias : (g == 1 ? io->fix_velocity : io->fix_accel_bias);
+    if (!fixed)
+      for (int k = 0; k < 3; ++k) cols[nd++] = 3 * g + k;
+  }
+  if (nd == 0) return fail(MLBA_EINVAL, "no speed or bias block is free");
+  std::vector<uint8_t> in(P, 0);
+  for (int e = 0; e < E; ++e) in[p->edge_from[e]] = in[p->edge_to[e]] = 1;
+  std::vector<double> bg(p->vertex_gyro_bias, p->vertex_gyro_bias + 3 * P);
+  std::vector<double> vel(p->vertex_velocity_M, p->vertex_velocity_M + 3 * P);
+  std::vector<double> ba(p->vertex_accel_bias, p->vertex_accel_bias + 3 * P);
+  // Evaluate: cost, and with A the blocks D (P x nd x nd), C (coupling of v
+  // and v + 1) and g.
+  std::vector<double> res(15 * (size_t)E), J[8];
+  const int sizes[8] = {7, 3, 3, 3, 7, 3, 3, 3};
+  auto evaluate = [&](const std::vector<double>& gb, const std::vector<double>& vv, const std::vector<double>& ab,
+                      bool jac, double* cost, std::vector<double>* D, std::vector<double>* C,
+                      std::vector<double>* g) -> int {
+    mlba_inertial_problem q = *p;
+    q.vertex_gyro_bias = gb.data();
+    q.vertex_velocity_M = vv.data();
+    q.vertex_accel_bias = ab.data();
+    double* jp[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (jac)
+      for (int b : {1, 2, 3, 5, 6, 7}) {
+        J[b].resize(15 * (size_t)sizes[b] * E);
+        jp[b] = J[b].data();
+      }
+    const int rc = mlba_inertial_evaluate(&q, res.data(), jac ? jp : nullptr);
+    if (rc) return rc;
+    double c = 0.0;
+    for (double r : res) c += r * r;
+    *cost = 0.5 * c;
+    if (!jac) return MLBA_OK;
+    D->assign((size_t)P * nd * nd, 0.0);
+    C->assign((size_t)P * nd * nd, 0.0);
+    g->assign((size_t)P * nd, 0.0);
+    double Ja[15 * 9], Jb[15 * 9];
+    for (int e = 0; e < E; ++e) {
+      const int a = p->edge_from[e], b = a + 1;
+      for (int r = 0; r < 15; ++r)
+        for (int k = 0; k < nd; ++k) {
+          const int grp = cols[k] / 3, cc = cols[k] % 3;  // parameter blocks 1..3 (from), 5..7 (to)
+          Ja[r * nd + k] = J[1 + grp][(size_t)45 * e + 3 * r + cc];
+          Jb[r * nd + k] = J[5 + grp][(size_t)45 * e + 3 * r + cc];
+        }
+      const double* re = &res[15 * (size_t)e];
+      for (int i = 0; i < nd; ++i) {
+        double ga = 0.0, gbv = 0.0;
+        for (int r = 0; r < 15; ++r) { ga += Ja[r * nd + i] * re[r]; gbv += Jb[r * nd + i] * re[r]; }
+        (*g)[(size_t)a * nd + i] += ga;
+        (*g)[(size_t)b * nd + i] += gbv;
+        for (int j = 0; j < nd; ++j) {
+          double daa = 0.0, dbb = 0.0, dab = 0.0;
+          for (int r = 0; r < 15; ++r) {
+            daa += Ja[r * nd + i] * Ja[r * nd + j];
+            dbb += Jb[r * nd + i] * Jb[r * nd + j];
+            dab += Ja[r * nd + i] * Jb[r * nd + j];
+          }
+          (*D)[((size_t)a * nd + i) * nd + j] += daa;
+          (*D)[((size_t)b * nd + i) * nd + j] += dbb;
+          (*C)[((size_t)a * nd + i) * nd + j] += dab;
+        }
+      }
+    }
+    return MLBA_OK;
+  };
+  if (sm) {
+    sm->num_iterations = sm->num_successful_steps = sm->num_unsuccessful_steps = 0;
+    sm->num_residual_blocks = E;
+    sm->num_removed_landmarks = 0;
+    sm->total_linear_solver_iterations = 0;
+    sm->fixed_cost = 0.0;
+  }
+  auto push = [&](const mlba_iteration_summary& it) {
+    if (!sm) return;
+    if (sm->iterations && sm->num_iterations < sm->iterations_capacity) sm->iterations[sm->num_iterations] = it;
+    sm->num_iterations++;
+  };
+  const auto t0 = std::chrono::steady_clock::now();
+  auto elapsed = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+  std::vector<double> D, C, g;
+  double cost = 0.0;
+  int rc = evaluate(bg, vel, ba, true, &cost, &D, &C, &g);
+  if (rc) return rc;
+  auto gmax = [&](const std::vector<double>& gv) {
+    double m = 0.0;
+    for (int v = 0; v < P; ++v)
+      if (in[v])
+        for (int k = 0; k < nd; ++k) m = std::max(m, std::fabs(gv[(size_t)v * nd + k]));
+    return m;
+  };
+  double grad = gmax(g);
+  if (sm) sm->initial_cost = cost;
+  mlba_iteration_summary it0{};
+  it0.step_is_valid = it0.step_is_successful = 1;
+  it0.cost = cost;
+  it0.gradient_max_norm = grad;
+  double radius = o->initial_trust_region_radius, decrease = 2.0;
+  it0.trust_region_radius = radius;
+  push(it0);
+  int term = grad <= o->gradient_tolerance ? MLBA_CONVERGENCE : MLBA_NO_CONVERGENCE;
+  int invalid = 0, iteration = 0;
+  std::vector<double> sc((size_t)P * nd), w((size_t)P * nd), L((size_t)P * nd * nd), B((size_t)P * nd * nd);
+  while (term == MLBA_NO_CONVERGENCE && o->max_num_iterations > 0) {
+    ++iteration;
+    mlba_iteration_summary cur{};
+    cur.iteration = iteration;
+    for (int v = 0; v < P; ++v)
+      for (int k = 0; k < nd; ++k) {
+        const double d = D[((size_t)v * nd + k) * nd + k];
+        sc[(size_t)v * nd + k] = o->use_jacobi_scaling ? 1.0 / (1.0 + std::sqrt(d)) : 1.0;
+      }
+    // Block-tridiagonal Cholesky of the scaled, damped system:
+    // L_v L_v' = D'_v - B_{v-1}' B_{v-1}, B_v = L_v^-1 C'_v.
+    bool ok = true;
+    std::vector<double> y((size_t)P * nd, 0.0);
+    for (int v = 0; v < P && ok; ++v) {
+      if (!in[v]) continue;
+      double M[81];
+      for (int i = 0; i < nd; ++i)
+        for (int j = 0; j < nd; ++j) {
+          double m = sc[(size_t)v * nd + i] * D[((size_t)v * nd + i) * nd + j] * sc[(size_t)v * nd + j];
+          if (i == j) m += std::min(std::max(m, o->min_lm_diagonal), o->max_lm_diagonal) / radius;
+          M[i * nd + j] = m;
+        }
+      const bool prev = v > 0 && in[v - 1];
+      if (prev)
+        for (int i = 0; i < nd; ++i)
+          for (int j = 0; j < nd; ++j) {
+            double s2 = 0.0;
+            for (int k = 0; k < nd; ++k) s2 += B[((size_t)(v - 1) * nd + k) * nd + i] * B[((size_t)(v - 1) * nd + k) * nd + j];
+            M[i * nd + j] -= s2;
+          }
+      double* Lv = &L[(size_t)v * nd * nd];
+      for (int j = 0; j < nd && ok; ++j) {
+        double dj = M[j * nd + j];
+        for (int k = 0; k < j; ++k) dj -= Lv[j * nd + k] * Lv[j * nd + k];
+        if (!(dj > 0.0) || !std::isfinite(dj)) { ok = false; break; }
+        Lv[j * nd + j] = std::sqrt(dj);
+        for (int i = j + 1; i < nd; ++i) {
+          double t = M[i * nd + j];
+          for (int k = 0; k < j; ++k) t -= Lv[i * nd + k] * Lv[j * nd + k];
+          Lv[i * nd + j] = t / Lv[j * nd + j];
+        }
+        for (int i = 0; i < j; ++i) Lv[i * nd + j] = 0.0;
+      }
+      if (!ok) break;
+      // Forward substitution of the right-hand side.
+      double rhs[9];
+      for (int i = 0; i < nd; ++i) rhs[i] = sc[(size_t)v * nd + i] * g[(size_t)v * nd + i];
+      if (prev)
+        for (int i = 0; i < nd; ++i)
+          for (int k = 0; k < nd; ++k) rhs[i] -= B[((size_t)(v - 1) * nd + k) * nd + i] * y[(size_t)(v - 1) * nd + k];
+      for (int i = 0; i < nd; ++i) {
+        double t = rhs[i];
+        for (int k = 0; k < i; ++k) t -= Lv[i * nd + k] * y[(size_t)v * nd + k];
+        y[(size_t)v * nd + i] = t / Lv[i * nd + i];
+      }
+      // B_v = L_v^-1 (S_v C_v S_{v+1}) when v + 1 is in the chain.
+      if (v + 1 < P && in[v + 1])
+        for (int j = 0; j < nd; ++j)
+          for (int i = 0; i < nd; ++i) {
+            double t = sc[(size_t)v * nd + i] * C[((size_t)v * nd + i) * nd + j] * sc[(size_t)(v + 1) * nd + j];
+            for (int k = 0; k < i; ++k) t -= Lv[i * nd + k] * B[((size_t)v * nd + k) * nd + j];
+            B[((size_t)v * nd + i) * nd + j] = t / Lv[i * nd + i];
+          }
+    }
+    double mcc = 0.0;
+    if (ok) {
+      for (int v = P - 1; v >= 0; --v) {
+        if (!in[v]) continue;
+        const double* Lv = &L[(size_t)v * nd * nd];
+        double rhs[9];
+        for (int i = 0; i < nd; ++i) rhs[i] = y[(size_t)v * nd + i];
+        if (v + 1 < P && in[v + 1])
+          for (int i = 0; i < nd; ++i)
+            for (int j = 0; j < nd; ++j) rhs[i] -= B[((size_t)v * nd + i) * nd + j] * w[(size_t)(v + 1) * nd + j];
+        for (int i = nd - 1; i >= 0; --i) {
+          double t = rhs[i];
+          for (int k = i + 1; k < nd; ++k) t -= Lv[k * nd + i] * w[(size_t)v * nd + k];
+          w[(size_t)v * nd + i] = t / Lv[i * nd + i];
+        }
+      }
+      for (size_t k = 0; k < w.size(); ++k) w[k] *= sc[k];
+      // Model cost change of the step -w: g'w - w'Aw / 2.
+      double gw = 0.0, wAw = 0.0;
+      for (int v = 0; v < P; ++v) {
+        if (!in[v]) continue;
+        for (int i = 0; i < nd; ++i) {
+          gw += g[(size_t)v * nd + i] * w[(size_t)v * nd + i];
+          double aw = 0.0;
+          for (int j = 0; j < nd; ++j) {
+            aw += D[((size_t)v * nd + i) * nd + j] * w[(size_t)v * nd + j];
+            if (v + 1 < P && in[v + 1]) aw += C[((size_t)v * nd + i) * nd + j] * w[(size_t)(v + 1) * nd + j];
+            if (v > 0 && in[v - 1]) aw += C[((size_t)(v - 1) * nd + j) * nd + i] * w[(size_t)(v - 1) * nd + j];
+          }
+          wAw += w[(size_t)v * nd + i] * aw;
+        }
+      }
+      mcc = gw - 0.5 * wAw;
+      ok = std::isfinite(mcc) && mcc > 0.0;
+    }
+    cur.step_is_valid = ok;
+    cur.linear_solver_iterations = 1;
+    if (!ok) {
+      if (++invalid >= o->max_num_consecutive_invalid_steps) { term = MLBA_FAILURE; break; }
+      radius /= decrease;
+      decrease *= 2.0;
+      cur.cost = cost;
+      cur.gradient_max_norm = grad;
+      cur.trust_region_radius = radius;
+      if (sm) sm->num_unsuccessful_steps++;
+    } else {
+      invalid = 0;
+      std::vector<double> s3[3] = {bg, vel, ba};
+      const std::vector<double>* cur3[3] = {&bg, &vel, &ba};
+      double ss = 0.0, xx = 0.0;
+      for (int v = 0; v < P; ++v) {
+        if (!in[v]) continue;
+        for (int k = 0; k < nd; ++k) {
+          const int c = cols[k];
+          const double x = (*cur3[c / 3])[3 * v + c % 3];
+          s3[c / 3][3 * v + c % 3] = x - w[(size_t)v * nd + k];
+          ss += w[(size_t)v * nd + k] * w[(size_t)v * nd + k];
+          xx += x * x;
+        }
+      }
+      cur.step_norm = std::sqrt(ss);
+      if (cur.step_norm <= o->parameter_tolerance * (std::sqrt(xx) + o->parameter_tolerance)) {
+        term = MLBA_CONVERGENCE;
+        break;
+      }
+      double ccost = 0.0;
+      if ((rc = evaluate(s3[0], s3[1], s3[2], false, &ccost, nullptr, nullptr, nullptr))) return rc;
+      cur.cost_change = cost - ccost;
+      if (std::fabs(cur.cost_change) <= o->function_tolerance * cost) { term = MLBA_CONVERGENCE; break; }
+      cur.relative_decrease = (cost - ccost) / mcc;
+      if (cur.relative_decrease > o->min_relative_decrease) {
+        bg = s3[0]; vel = s3[1]; ba = s3[2];
+        if ((rc = evaluate(bg, vel, ba, true, &cost, &D, &C, &g))) return rc;
+        grad = gmax(g);
+        cur.step_is_successful = 1;
+        cur.cost = cost;
+        cur.gradient_max_norm = grad;
+        const double qq = 2.0 * cur.relative_decrease - 1.0;
+        radius = std::min(o->max_trust_region_radius, radius / std::max(1.0 / 3.0, 1.0 - qq * qq * qq));
+        decrease = 2.0;
+        if (sm) sm->num_successful_steps++;
+      } else {
+        radius /= decrease;
+        decrease *= 2.0;
+        cur.cost = ccost;
+        cur.gradient_max_norm = grad;
+        if (sm) sm->num_unsuccessful_steps++;
+      }
+      cur.trust_region_radius = radius;
+    }
+    cur.cumulative_time_in_seconds = elapsed();
+    push(cur);
+    if (elapsed() > o->max_solver_time_in_seconds) break;
+    if (iteration >= o->max_num_iterations) break;
+    if (cur.step_is_successful && cur.gradient_max_norm <= o->gradient_tolerance) { term = MLBA_CONVERGENCE; break; }
+    if (radius <= o->min_trust_region_radius) { term = MLBA_CONVERGENCE; break; }
+  }
+  if (sm) {
+    sm->termination_type = term;
+    sm->final_cost = cost;
+    sm->lm_loop_seconds = elapsed();
+  }
+  if (gyro_bias_out) std::memcpy(gyro_bias_out, bg.data(), sizeof(double) * 3 * P);
+  if (velocity_out) std::memcpy(velocity_out, vel.data(), sizeof(double) * 3 * P);
+  if (accel_bias_out) std::memcpy(accel_bias_out, ba.data(), sizeof(double) * 3 * P);
   return MLBA_OK;
 }
 

@@ -108,6 +108,16 @@ int32_t oracle_inertial_evaluate(int32_t n, const double* imu, const int64_t* t_
                                  const double* const* params, double* residual, double* const* jacobians,
                                  double* phi_out, double* Q_out, double* end_state_out);
 
+/* Speed and bias states of the edges' vertices with the poses constant:
+ * dense LM over the free [b_g | v_M | b_a] columns; fix = {gyro bias, accel
+ * bias, velocity}; noise as oracle_inertial_evaluate. Outputs nullable. */
+int32_t oracle_solve_inertial_states(int32_t P, int32_t E, const double* vertices, const double* gyro_bias,
+                                     const double* velocity, const double* accel_bias, const int32_t* edge_from,
+                                     const int32_t* edge_to, const int32_t* sample_begin, const double* imu,
+                                     const int64_t* t_ns, const double* noise, const int32_t* fix,
+                                     const mlba_options* options, double* gyro_bias_out, double* velocity_out,
+                                     double* accel_bias_out, mlba_summary* summary);
+
 /* The damped, Jacobi-scaled reduced camera system at the given state:
  * S (n x n dense, n = 6 * number of variable vertices that have residuals,
  * ordered by vertex index) and rhs (n), for LM radius `mu`. Used to check the

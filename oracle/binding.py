@@ -233,6 +233,33 @@ def landmark_quality(vmap, options):
     return good.astype(bool)
 
 
+def solve_inertial_states(edges, options, fix_gyro_bias=False, fix_accel_bias=False, fix_velocity=False,
+                          capacity=256):
+    """Dense-LM restatement of the speed / bias solve with constant poses.
+    edges: an InertialEdges. Returns (summary, rows, gyro_bias, velocity,
+    accel_bias)."""
+    from maplab_amd import abi
+    lib = load()
+    fn = lib.oracle_solve_inertial_states
+    fn.restype = ctypes.c_int32
+    vp = ctypes.c_void_p
+    fn.argtypes = [ctypes.c_int32, ctypes.c_int32] + [vp] * 11 + [ctypes.POINTER(abi.Options), vp, vp, vp,
+                                                                  ctypes.POINTER(abi.Summary)]
+    a = edges._arrays()
+    P, E = a[0].shape[0], a[4].shape[0]
+    noise = np.ascontiguousarray(list(edges.noise) + [edges.gravity], dtype=np.float64)
+    fix = np.array([fix_gyro_bias, fix_accel_bias, fix_velocity], dtype=np.int32)
+    bg, v, ba = np.zeros((P, 3)), np.zeros((P, 3)), np.zeros((P, 3))
+    its = (abi.IterationSummary * capacity)()
+    sm = abi.Summary()
+    sm.iterations_capacity = capacity
+    sm.iterations = ctypes.cast(its, ctypes.POINTER(abi.IterationSummary))
+    rc = fn(P, E, *[x.ctypes.data for x in a], noise.ctypes.data, fix.ctypes.data, ctypes.byref(options),
+            bg.ctypes.data, v.ctypes.data, ba.ctypes.data, ctypes.byref(sm))
+    assert rc == 0, rc
+    return sm, [its[i] for i in range(min(sm.num_iterations, capacity))], bg, v, ba
+
+
 def inertial_evaluate(imu, t_ns, noise, params, want_jac=True):
     """InertialErrorTerm::Evaluate for one edge: returns (residual (15),
     jacobians [8] ambient 15 x {7,3,3,3,7,3,3,3}, phi_accum, Q_accum,

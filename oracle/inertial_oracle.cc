@@ -9,9 +9,11 @@
 // Matrices are row-major; the 16-D state is [q (JPL xyzw) | b_g | v_M | b_a |
 // p_M_I] and the 15-D error state [dtheta | b_g | v | b_a | p]
 // (imu-integrator/common.h:6-30).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <initializer_list>
+#include <vector>
 
 #include "ba_oracle.h"
 
@@ -318,6 +320,211 @@ int32_t oracle_inertial_evaluate(int32_t n, const double* imu, const int64_t* t_
       }
     }
   }
+  return MLBA_OK;
+}
+
+
+// Speed and bias states with the poses constant (addInertialTermsForEdges,
+// optimization-terms-addition.cc:420-490, fix flags :468-485): dense LM over
+// the free [b_g | v_M | b_a] columns of the edges' vertices, with the ceres
+// rules of the oracle's minimize(). edge_from / edge_to index vertices;
+// samples of edge e are [sample_begin[e], sample_begin[e + 1]).
+int32_t oracle_solve_inertial_states(int32_t P, int32_t E, const double* vertices, const double* gyro_bias,
+                                     const double* velocity, const double* accel_bias, const int32_t* edge_from,
+                                     const int32_t* edge_to, const int32_t* sample_begin, const double* imu,
+                                     const int64_t* t_ns, const double* noise, const int32_t* fix,
+                                     const mlba_options* options, double* gyro_bias_out, double* velocity_out,
+                                     double* accel_bias_out, mlba_summary* summary) {
+  if (P <= 0 || E <= 0 || !options || !fix) return MLBA_EINVAL;
+  const mlba_options& o = *options;
+  // Columns: vertex v, group k (0 b_g, 1 v_M, 2 b_a) -> index or -1.
+  std::vector<int> col((size_t)P * 9, -1);
+  std::vector<uint8_t> in(P, 0);
+  for (int e = 0; e < E; ++e) in[edge_from[e]] = in[edge_to[e]] = 1;
+  const bool fixed_group[3] = {fix[0] != 0, fix[2] != 0, fix[1] != 0};  // fix = {gyro, accel, velocity}
+  int n = 0;
+  for (int v = 0; v < P; ++v)
+    if (in[v])
+      for (int k = 0; k < 9; ++k)
+        if (!fixed_group[k / 3]) col[(size_t)v * 9 + k] = n++;
+  if (n == 0) return MLBA_EINVAL;
+  std::vector<double> st[3] = {std::vector<double>(gyro_bias, gyro_bias + 3 * P),
+                               std::vector<double>(velocity, velocity + 3 * P),
+                               std::vector<double>(accel_bias, accel_bias + 3 * P)};
+  auto linearize = [&](const std::vector<double>* s, double* cost, std::vector<double>* A, std::vector<double>* g) {
+    *cost = 0.0;
+    if (A) { A->assign((size_t)n * n, 0.0); g->assign(n, 0.0); }
+    for (int e = 0; e < E; ++e) {
+      const int a = edge_from[e], b = edge_to[e];
+      const double* params[8] = {vertices + 7 * a, &s[0][3 * a], &s[1][3 * a], &s[2][3 * a],
+                                 vertices + 7 * b, &s[0][3 * b], &s[1][3 * b], &s[2][3 * b]};
+      double r[15], Jb[8][45];
+      double* jac[8] = {nullptr, A ? Jb[1] : nullptr, A ? Jb[2] : nullptr, A ? Jb[3] : nullptr,
+                        nullptr, A ? Jb[5] : nullptr, A ? Jb[6] : nullptr, A ? Jb[7] : nullptr};
+      const int rc = oracle_inertial_evaluate(sample_begin[e + 1] - sample_begin[e], imu + 6 * sample_begin[e],
+                                              t_ns + sample_begin[e], noise, params, r, jac, nullptr, nullptr,
+                                              nullptr);
+      if (rc) return rc;
+      for (int k = 0; k < 15; ++k) *cost += 0.5 * r[k] * r[k];
+      if (!A) continue;
+      // Free columns of this edge: (vertex, group, component) -> (block, column).
+      int gc[18], blk[18], cc[18], m = 0;
+      for (int side = 0; side < 2; ++side)
+        for (int k = 0; k < 9; ++k) {
+          const int v = side ? b : a;
+          const int c = col[(size_t)v * 9 + k];
+          if (c < 0) continue;
+          gc[m] = c;
+          blk[m] = (side ? 5 : 1) + k / 3;
+          cc[m] = k % 3;
+          ++m;
+        }
+      for (int i = 0; i < m; ++i) {
+        for (int rr = 0; rr < 15; ++rr) (*g)[gc[i]] += Jb[blk[i]][3 * rr + cc[i]] * r[rr];
+        for (int j = 0; j < m; ++j) {
+          double v = 0.0;
+          for (int rr = 0; rr < 15; ++rr) v += Jb[blk[i]][3 * rr + cc[i]] * Jb[blk[j]][3 * rr + cc[j]];
+          (*A)[(size_t)gc[i] * n + gc[j]] += v;
+        }
+      }
+    }
+    return (int)MLBA_OK;
+  };
+  auto push = [&](const mlba_iteration_summary& it) {
+    if (!summary) return;
+    if (summary->iterations && summary->num_iterations < summary->iterations_capacity)
+      summary->iterations[summary->num_iterations] = it;
+    summary->num_iterations++;
+  };
+  if (summary) {
+    summary->num_iterations = summary->num_successful_steps = summary->num_unsuccessful_steps = 0;
+    summary->num_residual_blocks = E;
+  }
+  std::vector<double> A, g;
+  double cost;
+  if (linearize(st, &cost, &A, &g)) return MLBA_ESTATE;
+  auto gmax = [&] {
+    double m = 0.0;
+    for (double x : g) m = std::max(m, std::fabs(x));
+    return m;
+  };
+  double gm = gmax();
+  if (summary) { summary->initial_cost = cost; summary->fixed_cost = 0.0; }
+  mlba_iteration_summary it0{};
+  it0.step_is_valid = it0.step_is_successful = 1;
+  it0.cost = cost;
+  it0.gradient_max_norm = gm;
+  double radius = o.initial_trust_region_radius, decrease = 2.0;
+  it0.trust_region_radius = radius;
+  push(it0);
+  int term = gm <= o.gradient_tolerance ? MLBA_CONVERGENCE : MLBA_NO_CONVERGENCE;
+  int invalid = 0, iteration = 0;
+  while (term == MLBA_NO_CONVERGENCE && o.max_num_iterations > 0) {
+    ++iteration;
+    mlba_iteration_summary cur{};
+    cur.iteration = iteration;
+    std::vector<double> sc(n), M((size_t)n * n), w(n);
+    for (int j = 0; j < n; ++j) sc[j] = o.use_jacobi_scaling ? 1.0 / (1.0 + std::sqrt(A[(size_t)j * n + j])) : 1.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) M[(size_t)i * n + j] = sc[i] * A[(size_t)i * n + j] * sc[j];
+    for (int j = 0; j < n; ++j)
+      M[(size_t)j * n + j] += std::min(std::max(M[(size_t)j * n + j], o.min_lm_diagonal), o.max_lm_diagonal) / radius;
+    bool ok = true;
+    for (int j = 0; j < n && ok; ++j) {  // dense Cholesky in place (lower)
+      double d = M[(size_t)j * n + j];
+      for (int k = 0; k < j; ++k) d -= M[(size_t)j * n + k] * M[(size_t)j * n + k];
+      if (!(d > 0.0)) { ok = false; break; }
+      d = std::sqrt(d);
+      M[(size_t)j * n + j] = d;
+      for (int i = j + 1; i < n; ++i) {
+        double t = M[(size_t)i * n + j];
+        for (int k = 0; k < j; ++k) t -= M[(size_t)i * n + k] * M[(size_t)j * n + k];
+        M[(size_t)i * n + j] = t / d;
+      }
+    }
+    double mcc = 0.0;
+    if (ok) {
+      for (int i = 0; i < n; ++i) {
+        double t = sc[i] * g[i];
+        for (int k = 0; k < i; ++k) t -= M[(size_t)i * n + k] * w[k];
+        w[i] = t / M[(size_t)i * n + i];
+      }
+      for (int i = n - 1; i >= 0; --i) {
+        double t = w[i];
+        for (int k = i + 1; k < n; ++k) t -= M[(size_t)k * n + i] * w[k];
+        w[i] = t / M[(size_t)i * n + i];
+      }
+      for (int j = 0; j < n; ++j) w[j] *= sc[j];
+      double gw = 0.0, wAw = 0.0;
+      for (int i = 0; i < n; ++i) {
+        gw += g[i] * w[i];
+        for (int j = 0; j < n; ++j) wAw += w[i] * A[(size_t)i * n + j] * w[j];
+      }
+      mcc = gw - 0.5 * wAw;
+      ok = std::isfinite(mcc) && mcc > 0.0;
+    }
+    cur.step_is_valid = ok;
+    cur.linear_solver_iterations = 1;
+    if (!ok) {
+      if (++invalid >= o.max_num_consecutive_invalid_steps) { term = MLBA_FAILURE; break; }
+      radius /= decrease;
+      decrease *= 2.0;
+      cur.cost = cost;
+      cur.gradient_max_norm = gm;
+      cur.trust_region_radius = radius;
+      if (summary) summary->num_unsuccessful_steps++;
+    } else {
+      invalid = 0;
+      std::vector<double> s2[3] = {st[0], st[1], st[2]};
+      double ss = 0.0, xx = 0.0;
+      for (int v = 0; v < P; ++v)
+        for (int k = 0; k < 9; ++k) {
+          const int c = col[(size_t)v * 9 + k];
+          if (c < 0) continue;
+          const double x = st[k / 3][3 * v + k % 3];
+          s2[k / 3][3 * v + k % 3] = x - w[c];
+          ss += w[c] * w[c];
+          xx += x * x;
+        }
+      cur.step_norm = std::sqrt(ss);
+      if (cur.step_norm <= o.parameter_tolerance * (std::sqrt(xx) + o.parameter_tolerance)) {
+        term = MLBA_CONVERGENCE;
+        break;
+      }
+      double ccost;
+      if (linearize(s2, &ccost, nullptr, nullptr)) return MLBA_ESTATE;
+      cur.cost_change = cost - ccost;
+      if (std::fabs(cur.cost_change) <= o.function_tolerance * cost) { term = MLBA_CONVERGENCE; break; }
+      cur.relative_decrease = (cost - ccost) / mcc;
+      if (cur.relative_decrease > o.min_relative_decrease) {
+        for (int k = 0; k < 3; ++k) st[k] = s2[k];
+        if (linearize(st, &cost, &A, &g)) return MLBA_ESTATE;
+        gm = gmax();
+        cur.step_is_successful = 1;
+        cur.cost = cost;
+        cur.gradient_max_norm = gm;
+        const double qq = 2.0 * cur.relative_decrease - 1.0;
+        radius = std::min(o.max_trust_region_radius, radius / std::max(1.0 / 3.0, 1.0 - qq * qq * qq));
+        decrease = 2.0;
+        if (summary) summary->num_successful_steps++;
+      } else {
+        radius /= decrease;
+        decrease *= 2.0;
+        cur.cost = ccost;
+        cur.gradient_max_norm = gm;
+        if (summary) summary->num_unsuccessful_steps++;
+      }
+      cur.trust_region_radius = radius;
+    }
+    push(cur);
+    if (iteration >= o.max_num_iterations) break;
+    if (cur.step_is_successful && cur.gradient_max_norm <= o.gradient_tolerance) { term = MLBA_CONVERGENCE; break; }
+    if (radius <= o.min_trust_region_radius) { term = MLBA_CONVERGENCE; break; }
+  }
+  if (summary) { summary->termination_type = term; summary->final_cost = cost; }
+  if (gyro_bias_out) std::memcpy(gyro_bias_out, st[0].data(), sizeof(double) * 3 * P);
+  if (velocity_out) std::memcpy(velocity_out, st[1].data(), sizeof(double) * 3 * P);
+  if (accel_bias_out) std::memcpy(accel_bias_out, st[2].data(), sizeof(double) * 3 * P);
   return MLBA_OK;
 }
 
