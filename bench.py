@@ -228,6 +228,9 @@ def main(argv=None):
                          "and concurrently (mlba_solve_many)")
     ap.add_argument("--multimap-threads", type=int, default=8)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default Schur block schedule")
+    ap.add_argument("--pcg-partials", type=int, default=0, choices=(0, 1, 2),
+                    help="PCG per-group partials summed by every SpMV work-group (1), once per update (2), "
+                         "or by group count (0)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -265,6 +268,9 @@ def main(argv=None):
         print(f"bench: autotuned schedule (band {sched.item_band_rows}, waves {sched.item_waves}, "
               f"xcd {sched.xcd_mapping}): {sched.tuned_ms:.4f} ms vs default {sched.default_ms:.4f} ms",
               file=sys.stderr, flush=True)
+    if args.pcg_partials:
+        sc = dev.schedule()
+        dev.set_schedule(sc.item_band_rows, sc.item_waves, sc.xcd_mapping, args.pcg_partials)
     kt = dev.time_kernels(opts, repeats=args.repeats)
     dev.upload_state(init_v, init_l)
     # Timed region: exactly `steps` LM iterations (fewer only if Ceres'
@@ -360,6 +366,7 @@ def main(argv=None):
         "initial_cost": sm.initial_cost, "final_cost": sm.final_cost,
         "kernels": table,
         "roofline": roof,
+        "pcg_partials": args.pcg_partials,
     }
     if sched is not None:
         result["schur_schedule"] = {

@@ -388,14 +388,18 @@ int32_t mlba_time_kernels(mlba_handle* handle, const mlba_options* options, int3
  * chosen is only the order the items are dispatched in (bands of
  * item_band_rows block rows swept column by column, 0 = block order), how
  * many items a work-group runs (item_waves: 1, 2, 4, 8) and whether
- * work-groups are mapped XCD-contiguously. None of these changes a result
- * bit: a solve after mlba_autotune or mlba_set_schedule repeats the same
- * iteration log and state.
+ * work-groups are mapped XCD-contiguously. The PCG sums the per-group
+ * partials of each iteration (r'z, r'r and the coarse-space product of every
+ * 42-row group) in group order, either in every SpMV work-group
+ * (pcg_partials 1) or once, in the update work-group that finishes last
+ * (2); 0 picks 2 when the partials exceed one 16 KB LDS chunk. None of these
+ * changes a result bit: a solve after mlba_autotune or mlba_set_schedule
+ * repeats the same iteration log and state.
  *
  * mlba_autotune times every schedule (item_band_rows in {0,1,2,4,8,16,32} x
  * item_waves x mapping, `repeats` launches each) on the Schur factors of
- * the current state and keeps the fastest for this handle; it leaves the
- * state unchanged. The schedule survives topology rebuilds (outlier
+ * the current state and keeps the fastest for this handle (pcg_partials
+ * is kept as set); it leaves the state unchanged. The schedule survives topology rebuilds (outlier
  * rejection, landmark masks).
  */
 typedef struct mlba_schedule {
@@ -405,6 +409,8 @@ typedef struct mlba_schedule {
   int32_t schedules_tried;    /* mlba_autotune: schedules timed */
   double default_ms;          /* mlba_autotune: block kernel with the library default (4, 4, 1) */
   double tuned_ms;            /* mlba_autotune: block kernel with the chosen schedule */
+  int32_t pcg_partials;       /* 0 automatic, 1 per SpMV work-group, 2 once per PCG update */
+  int32_t reserved0;
 } mlba_schedule;
 
 int32_t mlba_autotune(mlba_handle* handle, const mlba_options* options, int32_t repeats,

@@ -132,7 +132,8 @@ class CalibrationOptions(ctypes.Structure):
 
 class Schedule(ctypes.Structure):
     _fields_ = [("item_band_rows", c_i32), ("item_waves", c_i32), ("xcd_mapping", c_i32),
-                ("schedules_tried", c_i32), ("default_ms", c_f64), ("tuned_ms", c_f64)]
+                ("schedules_tried", c_i32), ("default_ms", c_f64), ("tuned_ms", c_f64), ("pcg_partials", c_i32),
+                ("reserved0", c_i32)]
 
 
 class QualityOptions(ctypes.Structure):

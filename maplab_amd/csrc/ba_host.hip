@@ -183,7 +183,7 @@ struct mlba_handle {
   // Dispatch schedule of k_schur_items (mlba_autotune): block rows per band
   // of the dispatch order (0: block order), items (waves) per work-group,
   // XCD-contiguous work-group mapping. It changes no result.
-  int32_t sched_band = kItemBandRows, sched_waves = 4, sched_xcd = 1;
+  int32_t sched_band = kItemBandRows, sched_waves = 4, sched_xcd = 1, sched_pcg = 0;
   // Jacobian storage of the observation records for the current solve
   // (mlba_options.jacobian_fp32): FP32 Jacobians, FP64 residuals.
   bool f32_jac = false;
@@ -658,7 +658,7 @@ int ensure_coarse_basis(mlba_handle* h, int32_t requested) {
   if (!h->defl_SR.alloc(groups * gw * (size_t)(6 * gr)) ||
       !h->defl_E.alloc(std::max<size_t>(ne * ne, 1)) || !h->defl_Einv.alloc(std::max<size_t>(ne * ne + groups * gw * ne, 1)) ||
       !h->defl_y.alloc(std::max<size_t>(ne, 1)) || !h->defl_cden.alloc(std::max<size_t>(nr, 1)) ||
-      !h->pcg_part2.alloc(groups * (3 + ne)))
+      !h->pcg_part2.alloc((groups + 1) * (3 + ne)))
     return set_err(h, MLBA_ENOMEM, "device allocation failed");
   if (nr > 0) {
     const size_t n6 = 6 * (size_t)Pvar, zc = (size_t)pcg_max_lanczos();
@@ -699,6 +699,7 @@ PcgBuffers pcg_buffers(mlba_handle* h, int32_t nr, bool keep_lanczos) {
   B.nr = B.K > 0 ? nr : 0;
   B.gwh = h->grp_width;
   B.zcap = keep_lanczos && B.K > 0 ? h->ritz_zcap : 0;
+  B.partials = h->sched_pcg;
   B.row_ptr = h->row_ptr.p; B.col_var = h->col_var.p; B.S = h->S.p; B.Minv = h->Minv.p; B.b = h->rhs.p;
   B.hat_node = h->hat_node.p; B.hat_w = h->hat_w.p; B.node_range = h->node_range.p; B.grp_lo = h->grp_lo.p;
   B.R = h->ritz[h->ritz_cur].p;
@@ -1936,6 +1937,7 @@ int32_t mlba_get_schedule(const mlba_handle* h, mlba_schedule* sc) {
   sc->item_band_rows = h->sched_band;
   sc->item_waves = h->sched_waves;
   sc->xcd_mapping = h->sched_xcd;
+  sc->pcg_partials = h->sched_pcg;
   return MLBA_OK;
 }
 
@@ -1945,10 +1947,12 @@ int32_t mlba_set_schedule(mlba_handle* h, const mlba_schedule* sc) {
   if (sc->item_band_rows < 0 || sc->item_band_rows > 4096) return set_err(h, MLBA_EINVAL, "item_band_rows out of range");
   if (!schur_item_waves_supported(sc->item_waves)) return set_err(h, MLBA_EINVAL, "item_waves must be 1, 2, 4 or 8");
   if (sc->xcd_mapping != 0 && sc->xcd_mapping != 1) return set_err(h, MLBA_EINVAL, "xcd_mapping must be 0 or 1");
+  if (sc->pcg_partials < 0 || sc->pcg_partials > 2) return set_err(h, MLBA_EINVAL, "pcg_partials must be 0, 1 or 2");
   const bool reorder = sc->item_band_rows != h->sched_band;
   h->sched_band = sc->item_band_rows;
   h->sched_waves = sc->item_waves;
   h->sched_xcd = sc->xcd_mapping;
+  h->sched_pcg = sc->pcg_partials;
   return reorder ? upload_item_order(h) : MLBA_OK;
 }
 
@@ -1991,7 +1995,7 @@ int32_t mlba_autotune(mlba_handle* h, const mlba_options* o, int32_t repeats, ml
       return (double)ms / repeats;
     };
     static const int32_t kBands[] = {0, 1, 2, 4, 8, 16, 32}, kWaves[] = {1, 2, 4, 8};
-    mlba_schedule best{kItemBandRows, 4, 1, 0, 0.0, 0.0};
+    mlba_schedule best{kItemBandRows, 4, 1, 0, 0.0, 0.0, h->sched_pcg, 0};
     double best_ms = -1.0, default_ms = -1.0;
     for (int32_t band : kBands) {
       h->sched_band = band;
@@ -2023,6 +2027,7 @@ int32_t mlba_autotune(mlba_handle* h, const mlba_options* o, int32_t repeats, ml
   out->item_band_rows = h->sched_band;
   out->item_waves = h->sched_waves;
   out->xcd_mapping = h->sched_xcd;
+  out->pcg_partials = h->sched_pcg;
   return MLBA_OK;
 }
 

@@ -17,7 +17,8 @@ namespace mlba {
 // dense columns.
 struct PcgBuffers {
   int32_t Pvar, K, nr, gwh;
-  int32_t zcap, pad;           // Lanczos vectors kept (iterations), 0: none
+  int32_t zcap;                // Lanczos vectors kept (iterations), 0: none
+  int32_t partials;            // sum of the group partials: 0 automatic, 1 per SpMV work-group, 2 once
   const int32_t* row_ptr;      // block rows of S
   const int32_t* col_var;
   const double* S;
@@ -43,7 +44,7 @@ struct PcgBuffers {
   double* p1;
   double* q;
   double* part;                // Pvar
-  double* part2;               // groups x (3 + ne)
+  double* part2;               // (groups + 1) x (3 + ne): per-group partials, then their sum
   double* Zh;                  // zcap x 6 Pvar: deflated preconditioned residual of iteration k
   double* coef;                // 2 zcap: r'z and alpha of iteration k
   void* state_dev;             // pcg_state_bytes()

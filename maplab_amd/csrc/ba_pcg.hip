@@ -27,9 +27,12 @@
 // r0, z0 and the first per-group partials). Per PCG iteration: k_pcg_spmv (sums the previous
 // per-group partials; p = z - W mu + beta p_prev and
 // q = S z + beta S p_prev - (SW) mu) and k_pcg_update (x, r, z and the
-// per-group partials r'z, r'r, E^-1 (SW)'z). Every work-group sums partials
-// in the same fixed order: the set-up and the iteration are deterministic and
-// use no atomics.
+// per-group partials r'z, r'r, E^-1 (SW)'z). Every sum of partials runs in
+// the same fixed order, so the set-up and the iteration are deterministic.
+// With more groups than one LDS chunk of k_pcg_spmv holds, the work-group of
+// k_pcg_update (or k_defl_init) that finishes last sums the groups' partials
+// once, into part1, and k_pcg_spmv reads that one vector; the only atomic is
+// the integer arrival counter that picks this work-group.
 #include "ba_pcg.h"
 
 #include <algorithm>
@@ -56,6 +59,8 @@ struct PcgState {
   int32_t done;
   int32_t max_iters;
   int32_t defl_ok;  // E was positive definite: the deflation is applied
+  uint32_t arrive;  // work-groups that wrote their partials in the current launch
+  int32_t pad;
 };
 
 struct PcgArgs {
@@ -90,6 +95,8 @@ struct PcgArgs {
   double* q;
   double* part;                // Pvar: p'q of each row
   double* part2;               // groups x (3 + ne): r'z, r'r, b'b and E^-1 (SW)'z of the group's rows
+  double* part1;               // 3 + ne: part2 summed over the groups (with reduce1)
+  int32_t reduce1, pad4;       // the groups' partials exceed one LDS chunk of k_pcg_spmv
   double* y;                   // ne: E^-1 W'b
   double* cden;                // nr: R'b
   double* Zh;                  // zcap x 6 Pvar: deflated preconditioned residuals
@@ -284,6 +291,7 @@ __global__ __launch_bounds__(kPrepThreads) void k_defl_prep(PcgArgs a, double to
     st->done = 0;
     st->max_iters = max_iters;
     st->defl_ok = 0;
+    st->arrive = 0;
   }
   if (ne == 0) return;
   // c = W'b: thread (column, g) sums the support rows lo + g, lo + g + G, ...
@@ -481,6 +489,52 @@ __device__ inline void group_partials(const PcgArgs& a, int g, const GroupLoads&
   }
 }
 
+// With a.reduce1: the work-group that writes the last group's partials sums
+// every group's, in group order (the order k_pcg_spmv sums them in without
+// reduce1), into part1 and re-arms the counter. Which work-group arrives last
+// does not change the sum.
+// The partials are written by the first two waves (thread 0 and threads
+// below ne <= 102); only they release their stores at device scope (an L2
+// write-back, since each XCD has its own L2), and only the last work-group
+// acquires (an L2 invalidate): a full fence in every wave of every
+// work-group would invalidate the L2 under the other work-groups' loads.
+// The summing work-group stages whole groups' partials through LDS (red,
+// kRedChunk doubles), every load of a pass in flight at once.
+constexpr int kRedChunk = 4096;
+constexpr int kRedPer = kRedChunk / kGroupBlock;
+
+__device__ inline void sum_partials_if_last(const PcgArgs& a, bool* last, double* red) {
+  const int tid = threadIdx.x, stride = 3 + a.ne;
+  if (tid < 128) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (tid == 0) *last = atomicAdd(&a.st->arrive, 1u) == (uint32_t)a.groups - 1u;
+  __syncthreads();
+  if (!*last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every group's partials are visible here
+  const int per = kRedChunk / stride;
+  double acc = 0.0;
+  for (int g0 = 0; g0 < a.groups; g0 += per) {
+    const int ng = min(per, a.groups - g0), n = ng * stride;
+    const double* src = a.part2 + (size_t)g0 * stride;
+    double v[kRedPer];
+#pragma unroll
+    for (int m = 0; m < kRedPer; ++m) {
+      const int q = tid + kGroupBlock * m;
+      v[m] = q < n ? src[q] : 0.0;
+    }
+    __syncthreads();  // the previous pass is summed
+#pragma unroll
+    for (int m = 0; m < kRedPer; ++m) red[tid + kGroupBlock * m] = v[m];
+    __syncthreads();
+    if (tid < stride) {
+#pragma unroll 8
+      for (int gg = 0; gg < ng; ++gg) acc += red[gg * stride + tid];
+    }
+  }
+  if (tid < stride) a.part1[tid] = acc;
+  if (tid == 0) a.st->arrive = 0u;
+}
+
 // x0 = W y, r0 = b - (SW) y, z0 = M^-1 r0, p = 0 on one group of rows per
 // work-group, and the group's partials.
 __global__ __launch_bounds__(kGroupBlock) void k_defl_init(PcgArgs a) {
@@ -539,6 +593,9 @@ __global__ __launch_bounds__(kGroupBlock) void k_defl_init(PcgArgs a) {
   double rz = rv * zv, rr = rv * rv, bb = bv * bv;
   block_sum3(rz, rr, bb, lds);
   group_partials(a, g, L, zloc, rz, rr, bb, cs, mpart, a.part2 + (size_t)g * (3 + ne));
+  __shared__ bool last;
+  __shared__ double red[kRedChunk];
+  if (a.reduce1) sum_partials_if_last(a, &last, red);
 }
 
 // Iteration k: stops when |r|^2 <= tol2 or k = max_iters, else
@@ -547,10 +604,12 @@ __global__ __launch_bounds__(kGroupBlock) void k_defl_init(PcgArgs a) {
 // of the loads) do not wait for the sums of the previous update's per-group
 // partials (r'z, r'r and mu); the own row's p_k is stored and
 // part[i] = p_k,i . q_i. Thread t < 252 handles component t % 6 of every 42nd
-// block of the row and window columns t / 6 + 42 m of (SW)_i.
+// block of the row and window columns t / 6 + 42 m of (SW)_i. kReduced reads
+// the summed partials (part1) instead of summing every group's.
+template <bool kReduced>
 __global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a, int32_t k, const double* __restrict__ p_prev,
                                                   double* __restrict__ p_cur) {
-  __shared__ double chunk[kPartChunk];
+  __shared__ double chunk[kReduced ? 1 : kPartChunk];
   __shared__ double sums[3 + kMaxCoarse];
   __shared__ double ysz[kGroupThreads], ysp[kGroupThreads], ysw[kGroupThreads];
   PcgState* st = a.st;
@@ -562,13 +621,17 @@ __global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a, int32_t k, const do
   // dependent round of global loads costs microseconds): the state, the first
   // chunk of group partials, the row's own z, p_{k-1} and hat weights, (SW)_i
   // and the block products.
-  constexpr int kChunkPer = kPartChunk / 256;
+  constexpr int kChunkPer = kReduced ? 1 : kPartChunk / 256;
   const int per_chunk = kPartChunk / stride, ng0 = min(per_chunk, a.groups);
   double pre[kChunkPer];
+  if (kReduced) {
+    pre[0] = tid < stride ? a.part1[tid] : 0.0;
+  } else {
 #pragma unroll
-  for (int m = 0; m < kChunkPer; ++m) {
-    const int q = tid + 256 * m;
-    pre[m] = q < ng0 * stride ? a.part2[q] : 0.0;
+    for (int m = 0; m < kChunkPer; ++m) {
+      const int q = tid + 256 * m;
+      pre[m] = q < ng0 * stride ? a.part2[q] : 0.0;
+    }
   }
   double z_own = 0.0, p_own = 0.0, hw0 = 0.0, hw1 = 0.0, r_own[kMaxRitz];
   int hn0 = 0;
@@ -611,22 +674,30 @@ __global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a, int32_t k, const do
     }
   }
   if (done) return;  // the same value in every thread of the work-group
-  // Fixed-order sum of the group partials, a chunk of whole groups per pass.
+  if (kReduced) {
+    if (tid < stride) sums[tid] = pre[0];
+  } else {
+    // Fixed-order sum of the group partials, a chunk of whole groups per pass.
 #pragma unroll
-  for (int m = 0; m < kChunkPer; ++m) chunk[tid + 256 * m] = pre[m];
-  __syncthreads();
-  double acc = 0.0;
-  if (tid < stride)
-    for (int gg = 0; gg < ng0; ++gg) acc += chunk[gg * stride + tid];
-  for (int g0 = ng0; g0 < a.groups; g0 += per_chunk) {
-    const int ng = min(per_chunk, a.groups - g0);
+    for (int m = 0; m < kChunkPer; ++m) chunk[tid + 256 * m] = pre[m];
     __syncthreads();
-    for (int q = tid; q < ng * stride; q += blockDim.x) chunk[q] = a.part2[(size_t)g0 * stride + q];
-    __syncthreads();
-    if (tid < stride)
-      for (int gg = 0; gg < ng; ++gg) acc += chunk[gg * stride + tid];
+    double acc = 0.0;
+    if (tid < stride) {
+#pragma unroll 8
+      for (int gg = 0; gg < ng0; ++gg) acc += chunk[gg * stride + tid];
+    }
+    for (int g0 = ng0; g0 < a.groups; g0 += per_chunk) {
+      const int ng = min(per_chunk, a.groups - g0);
+      __syncthreads();
+      for (int q = tid; q < ng * stride; q += blockDim.x) chunk[q] = a.part2[(size_t)g0 * stride + q];
+      __syncthreads();
+      if (tid < stride) {
+#pragma unroll 8
+        for (int gg = 0; gg < ng; ++gg) acc += chunk[gg * stride + tid];
+      }
+    }
+    if (tid < stride) sums[tid] = acc;
   }
-  if (tid < stride) sums[tid] = acc;
   __syncthreads();
   const double* mu_s = sums + 3;
   const double rz = sums[0], rr = sums[1];
@@ -752,6 +823,9 @@ __global__ __launch_bounds__(kGroupBlock) void k_pcg_update(PcgArgs a, int32_t k
   double rzg = rv * zv, rr = rv * rv, zero = 0.0;
   block_sum3(rzg, rr, zero, lds);
   group_partials(a, g, L, zloc, rzg, rr, 0.0, cs, mpart, a.part2 + (size_t)g * (3 + ne));
+  __shared__ bool last;
+  __shared__ double red[kRedChunk];
+  if (a.reduce1) sum_partials_if_last(a, &last, red);
 }
 
 // Clears the convergence flag and lifts the iteration limit so that one more
@@ -872,8 +946,19 @@ static PcgArgs make_args(const PcgBuffers& B) {
   a.SR = B.SR; a.E = B.E; a.Einv = B.Einv; a.EinvW = B.Einv + (size_t)a.ne * a.ne;
   a.x = B.x; a.r = B.r; a.z = B.z; a.p0 = B.p0; a.p1 = B.p1; a.q = B.q;
   a.part = B.part; a.part2 = B.part2; a.y = B.y; a.cden = B.cden; a.Zh = B.Zh; a.coef = B.coef;
+  // part2 holds groups + 1 rows of partials; the last one is part1.
+  a.part1 = B.part2 + (size_t)a.groups * (3 + a.ne);
+  a.reduce1 = B.partials == 0 ? (a.groups > kPartChunk / (3 + a.ne) ? 1 : 0) : (B.partials == 2 ? 1 : 0);
+  a.pad4 = 0;
   a.st = static_cast<PcgState*>(B.state_dev);
   return a;
+}
+
+static void launch_spmv(hipStream_t s, const PcgArgs& a, int32_t k, const double* p_prev, double* p_cur) {
+  if (a.reduce1)
+    k_pcg_spmv<true><<<a.Pvar, 256, 0, s>>>(a, k, p_prev, p_cur);
+  else
+    k_pcg_spmv<false><<<a.Pvar, 256, 0, s>>>(a, k, p_prev, p_cur);
 }
 
 // Deflated PCG solve of S y = b. The host polls the state after `first`
@@ -905,11 +990,11 @@ int launch_pcg_deflated(hipStream_t s, const PcgBuffers& B, double tol, int32_t 
   while (true) {
     const int n = std::min(k == 0 ? std::max(first, 1) : poll, max_iters - k);
     for (int j = 0; j < n; ++j, ++k) {
-      k_pcg_spmv<<<Pvar, 256, 0, s>>>(a, k, (k & 1) ? p1 : p0, (k & 1) ? p0 : p1);
+      launch_spmv(s, a, k, (k & 1) ? p1 : p0, (k & 1) ? p0 : p1);
       k_pcg_update<<<groups, kGroupBlock, 0, s>>>(a, k, (k & 1) ? p0 : p1);
     }
     // At k = max_iters this launch only records the final state.
-    if (k >= max_iters) k_pcg_spmv<<<Pvar, 256, 0, s>>>(a, k, p0, p1);
+    if (k >= max_iters) launch_spmv(s, a, k, p0, p1);
     if (hipMemcpyAsync(hs, a.st, sizeof(PcgState), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
     if (hipStreamSynchronize(s) != hipSuccess) return -1;
     if (hs->done || k >= max_iters) break;
@@ -921,7 +1006,7 @@ int launch_pcg_deflated(hipStream_t s, const PcgBuffers& B, double tol, int32_t 
     k_pcg_reopen<<<1, 1, 0, s>>>(a.st);
     const int kk = iters;
     if (hipEventRecord(probe[0], s) != hipSuccess) return -1;
-    for (int j = 0; j < probe_reps; ++j) k_pcg_spmv<<<Pvar, 256, 0, s>>>(a, kk, (kk & 1) ? p1 : p0, (kk & 1) ? p0 : p1);
+    for (int j = 0; j < probe_reps; ++j) launch_spmv(s, a, kk, (kk & 1) ? p1 : p0, (kk & 1) ? p0 : p1);
     if (hipEventRecord(probe[1], s) != hipSuccess) return -1;
     for (int j = 0; j < probe_reps; ++j) k_pcg_update<<<groups, kGroupBlock, 0, s>>>(a, kk, (kk & 1) ? p0 : p1);
     if (hipEventRecord(probe[2], s) != hipSuccess) return -1;

@@ -156,8 +156,8 @@ class DeviceProblem:
         self._check(self.lib.mlba_autotune(self.handle, ctypes.byref(options), repeats, ctypes.byref(sc)))
         return sc
 
-    def set_schedule(self, band: int, waves: int, xcd: int) -> None:
-        sc = abi.Schedule(item_band_rows=band, item_waves=waves, xcd_mapping=xcd)
+    def set_schedule(self, band: int, waves: int, xcd: int, pcg_partials: int = 0) -> None:
+        sc = abi.Schedule(item_band_rows=band, item_waves=waves, xcd_mapping=xcd, pcg_partials=pcg_partials)
         self._check(self.lib.mlba_set_schedule(self.handle, ctypes.byref(sc)))
 
     def schedule(self) -> abi.Schedule:

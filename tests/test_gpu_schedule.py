@@ -1,7 +1,7 @@
 """The Schur block kernel's launch schedule (dispatch order, items per
-work-group, XCD mapping) and its autotuner change no result: solves under
-any schedule repeat the default schedule's iteration log and state bit for
-bit."""
+work-group, XCD mapping), the PCG's way of summing its per-group partials
+and the autotuner change no result: solves under any schedule repeat the
+default schedule's iteration log and state bit for bit."""
 import numpy as np
 import pytest
 
@@ -36,11 +36,12 @@ def reference(walk):
     return solve(walk)
 
 
-@pytest.mark.parametrize("schedule", [(0, 1, 0), (1, 2, 1), (32, 8, 0), (7, 4, 1)])
+@pytest.mark.parametrize("schedule", [(0, 1, 0), (1, 2, 1), (32, 8, 0), (7, 4, 1), (4, 4, 1, 1), (4, 4, 1, 2)])
 def test_explicit_schedules_repeat_the_default_solve(walk, reference, schedule):
     sm, rows, v, l, sc, _ = solve(walk, schedule=schedule)
     print(f"schedule {schedule}: {sm.num_iterations} rows, final {sm.final_cost:.12e}", flush=True)
-    assert (sc.item_band_rows, sc.item_waves, sc.xcd_mapping) == schedule
+    assert (sc.item_band_rows, sc.item_waves, sc.xcd_mapping) == schedule[:3]
+    assert sc.pcg_partials == (schedule[3] if len(schedule) > 3 else 0)
     assert sm.final_cost == reference[0].final_cost
     assert rows == reference[1]
     np.testing.assert_array_equal(v, reference[2])
@@ -63,8 +64,8 @@ def test_autotune_keeps_the_fastest_schedule_and_the_results(walk, reference):
 
 def test_invalid_schedules_are_rejected(walk):
     with DeviceProblem(walk) as dev:
-        for bad in [(-1, 4, 1), (4, 3, 1), (4, 16, 1), (4, 4, 2), (5000, 4, 0)]:
+        for bad in [(-1, 4, 1), (4, 3, 1), (4, 16, 1), (4, 4, 2), (5000, 4, 0), (4, 4, 1, 3), (4, 4, 1, -1)]:
             with pytest.raises(RuntimeError, match="maplab_ba error -1"):
                 dev.set_schedule(*bad)
         sc = dev.schedule()
-        assert (sc.item_band_rows, sc.item_waves, sc.xcd_mapping) == (4, 4, 1)
+        assert (sc.item_band_rows, sc.item_waves, sc.xcd_mapping, sc.pcg_partials) == (4, 4, 1, 0)
