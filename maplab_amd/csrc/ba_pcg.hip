@@ -279,7 +279,7 @@ __global__ __launch_bounds__(kPrepThreads) void k_defl_prep(PcgArgs a, double to
   __shared__ double Ms[kMaxCoarse * kMaxCoarse];
   __shared__ double cpart[kPrepThreads];
   __shared__ double cvec[kMaxCoarse];
-  __shared__ double colk[kMaxCoarse];
+  __shared__ double colb[2][kMaxCoarse];  // column k of the swept matrix in colb[k & 1]
   const int ne = a.ne, tid = threadIdx.x;
   if (tid == 0) {
     PcgState* st = a.st;
@@ -333,16 +333,20 @@ __global__ __launch_bounds__(kPrepThreads) void k_defl_prep(PcgArgs a, double to
     v[m] = e < nt ? a.E[(size_t)i * ne + j] : 0.0;
   }
   bool ok = true;
+  // One barrier per pivot: step k reads column k from colb[k & 1] and writes
+  // column k + 1 of its result into the other buffer, which nobody reads
+  // until the next step's barrier (its last readers were in step k - 1).
+#pragma unroll
+  for (int m = 0; m < kInvPerThread; ++m) {
+    if (er[m] == 0) colb[0][ec[m]] = v[m];
+    else if (ec[m] == 0) colb[0][er[m]] = v[m];
+  }
   for (int k = 0; k < ne; ++k) {
     __syncthreads();
-#pragma unroll
-    for (int m = 0; m < kInvPerThread; ++m) {
-      if (er[m] == k) colk[ec[m]] = v[m];       // (k, j <= k)
-      else if (ec[m] == k) colk[er[m]] = v[m];  // (i > k, k)
-    }
-    __syncthreads();
+    const double* colk = colb[k & 1];
+    double* next = colb[(k + 1) & 1];
     const double p = colk[k];
-    if (!(p > 0.0)) {
+    if (!(p > 0.0)) {  // the same value in every thread
       ok = false;
       break;
     }
@@ -354,6 +358,8 @@ __global__ __launch_bounds__(kPrepThreads) void k_defl_prep(PcgArgs a, double to
       double nv = fma(-colk[i] * ip, colk[j], v[m]);
       nv = (ri || rj) ? v[m] * ip : nv;
       v[m] = (ri && rj) ? -ip : nv;
+      if (er[m] == k + 1) next[ec[m]] = v[m];       // (k + 1, j <= k + 1)
+      else if (ec[m] == k + 1) next[er[m]] = v[m];  // (i > k + 1, k + 1)
     }
   }
   __syncthreads();
