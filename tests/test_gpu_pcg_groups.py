@@ -8,8 +8,10 @@ recycled vectors are added) the work-group that finishes the update last sums
 them once for all (an integer arrival counter picks it). The schedule field
 pcg_partials forces either way.
 
-Bar: on a 1000-keyframe random walk (24 groups) both ways give the same
-iteration log and state bit for bit, and runs repeat bit for bit; the LM log
+Bar: on a 1000-keyframe random walk (24 groups) and an 850-keyframe one
+(21 groups: the automatic choice switches after the first solve) both ways
+give the same iteration log and state bit for bit, and runs repeat bit for
+bit; on the 1000-keyframe walk the LM log
 agrees with the oracle's direct Schur solve to 1e-8 relative on the first
 steps (the PCG solves to 1e-10).
 """
@@ -45,13 +47,24 @@ def run(m, partials=0, **kw):
     return sm, [(r.cost, r.step_is_successful, r.linear_solver_iterations) for r in rows], v, l
 
 
-def test_both_partial_sums_give_the_same_solve(walk1000):
-    auto = run(walk1000, max_num_iterations=6)
-    per_spmv = run(walk1000, partials=1, max_num_iterations=6)
-    once = run(walk1000, partials=2, max_num_iterations=6)
-    print(f"1000 keyframes: LM log {auto[1]}", flush=True)
+@pytest.fixture(scope="module")
+def walk850():
+    # 21 groups: the first solve (15-node coarse space only) sums per SpMV,
+    # the later ones (12 recycled vectors added) once per update.
+    m = problem.workload_map(2, seed=5, num_keyframes=850, num_landmarks=15000)
+    assert 19 < (m.num_vertices + 41) // 42 <= 22
+    return m
+
+
+@pytest.mark.parametrize("size", [1000, 850])
+def test_both_partial_sums_give_the_same_solve(request, size):
+    m = request.getfixturevalue(f"walk{size}")
+    auto = run(m, max_num_iterations=6)
+    per_spmv = run(m, partials=1, max_num_iterations=6)
+    once = run(m, partials=2, max_num_iterations=6)
+    print(f"{size} keyframes: LM log {auto[1]}", flush=True)
     assert max(r[2] for r in auto[1]) < 500
-    for other in (per_spmv, once, run(walk1000, max_num_iterations=6)):
+    for other in (per_spmv, once, run(m, max_num_iterations=6)):
         assert other[1] == auto[1]
         assert other[0].final_cost == auto[0].final_cost
         np.testing.assert_array_equal(other[2], auto[2])
