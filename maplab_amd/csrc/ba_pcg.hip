@@ -506,6 +506,7 @@ __device__ inline void group_partials(const PcgArgs& a, int g, const GroupLoads&
 // work-group would invalidate the L2 under the other work-groups' loads.
 // The summing work-group stages whole groups' partials through LDS (red,
 // kRedChunk doubles), every load of a pass in flight at once.
+static_assert(3 + kMaxCoarse <= 128, "the group partials are written by the first two waves");
 constexpr int kRedChunk = 4096;
 constexpr int kRedPer = kRedChunk / kGroupBlock;
 
