@@ -349,7 +349,7 @@ def main(argv=None):
         },
         "obs_jacobians_per_s": round(obs_j_per_s, 1) if obs_j_per_s else None,
         "pcg_iterations": int(n_pcg),
-        "pcg_preconditioner": ("block-Jacobi + deflation by piecewise-linear hat functions and up to 12 Ritz "
+        "pcg_preconditioner": ("block-Jacobi + deflation by piecewise-linear hat functions and up to 16 Ritz "
                                "vectors recycled from the previous solves"),
         "pcg_iterations_vs_block_jacobi": {
             "deflated": int(n_pcg), "block_jacobi": int(n_pcg_jacobi),

@@ -167,7 +167,7 @@ typedef struct mlba_options {
   int32_t schur_row_chunk;                /* reserved */
   int32_t pcg_coarse_nodes;               /* deflation hat nodes: 0 automatic, -1 none, else K (<= 15) */
   int32_t pcg_ritz_vectors;               /* recycled Ritz vectors added to the deflation space:
-                                             0 automatic (12), -1 none, else r (<= 12) */
+                                             0 automatic (16), -1 none, else r (<= 16) */
   int32_t jacobian_fp32;                  /* 1: the robust-corrected Jacobians are rounded to FP32 and
                                              stored so (88 instead of 160 bytes per observation);
                                              every product of them, S, the PCG, residuals, costs and

@@ -102,11 +102,12 @@ constexpr int32_t kPcgPoll = 16, kPcgPollAfter = 4;
 // Block rows per band of the Schur item dispatch order.
 constexpr int kItemBandRows = 4;
 
-// Pinned host buffer of the recycling: the solve's coefficients (2 x 64),
-// the tridiagonal's eigenvectors (64 x 12), the Gram matrices (2 x 24 x 24)
-// and the merge coefficients (24 x 12).
-constexpr size_t kRitzCoefOff = 2 * 64, kRitzGramOff = kRitzCoefOff + 64 * 12;
-constexpr size_t kRitzMergeOff = kRitzGramOff + 2 * 24 * 24, kRitzPinned = kRitzMergeOff + 24 * 12;
+// Pinned host buffer of the recycling (R = kPcgMaxRitz): the solve's
+// coefficients (2 x 64), the tridiagonal's eigenvectors (64 x R), the Gram
+// matrices (2 x 2R x 2R) and the merge coefficients (2R x R).
+constexpr size_t kRitzCoefOff = 2 * 64, kRitzGramOff = kRitzCoefOff + 64 * kPcgMaxRitz;
+constexpr size_t kRitzMergeOff = kRitzGramOff + 2 * (2 * kPcgMaxRitz) * (2 * kPcgMaxRitz);
+constexpr size_t kRitzPinned = kRitzMergeOff + 2 * kPcgMaxRitz * kPcgMaxRitz;
 
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();

@@ -53,6 +53,8 @@ struct PcgBuffers {
 
 size_t pcg_state_bytes();
 int32_t pcg_max_hat_nodes();
+// Recycled dense columns of the deflation space, at most.
+constexpr int kPcgMaxRitz = 16;
 int32_t pcg_max_ritz();
 int32_t pcg_group_rows();
 int32_t pcg_update_groups(int32_t Pvar);

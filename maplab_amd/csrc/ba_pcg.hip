@@ -7,7 +7,7 @@
 // Erhel, Guyomarc'h, "A deflated version of the conjugate gradient
 // algorithm", SIAM J. Sci. Comput. 21, 2000) with a coarse space
 // W = [H | R]: H the piecewise-linear "hat" functions over the vertex order
-// (K nodes, 6 K columns), R up to 12 dense columns recycled from earlier
+// (K nodes, 6 K columns), R up to 16 dense columns recycled from earlier
 // solves (Ritz vectors of the preconditioned operator for its smallest
 // eigenvalues, from each solve's Lanczos coefficients; see the ritz kernels
 // below). x0 = W E^-1 W'b, E = W'SW, and every search direction is made
@@ -43,8 +43,8 @@
 namespace mlba {
 
 constexpr int kMaxHatNodes = 15;
-constexpr int kMaxRitz = 12;
-constexpr int kMaxCoarse = 6 * kMaxHatNodes + kMaxRitz;  // 102 columns
+constexpr int kMaxRitz = kPcgMaxRitz;
+constexpr int kMaxCoarse = 6 * kMaxHatNodes + kMaxRitz;  // 106 columns
 constexpr int kRowsPerGroup = 42;
 constexpr int kGroupThreads = 6 * kRowsPerGroup;  // 252 entries (row, component) per group
 constexpr int kGroupBlock = 1024;                 // threads of the per-group kernels
@@ -500,7 +500,7 @@ __device__ inline void group_partials(const PcgArgs& a, int g, const GroupLoads&
 // reduce1), into part1 and re-arms the counter. Which work-group arrives last
 // does not change the sum.
 // The partials are written by the first two waves (thread 0 and threads
-// below ne <= 102); only they release their stores at device scope (an L2
+// below ne <= 106); only they release their stores at device scope (an L2
 // write-back, since each XCD has its own L2), and only the last work-group
 // acquires (an L2 invalidate): a full fence in every wave of every
 // work-group would invalidate the L2 under the other work-groups' loads.

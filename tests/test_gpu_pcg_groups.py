@@ -3,7 +3,7 @@
 Each PCG iteration sums per-group partials (r'z, r'r and the coarse-space
 product of each group of 42 vertex rows) in group order. While they fit one
 LDS chunk of the SpMV kernel, each of its work-groups sums them itself; past
-that (more than 22 groups with the 15-node coarse space, more than 19 once 12
+that (more than 22 groups with the 15-node coarse space, more than 18 once 16
 recycled vectors are added) the work-group that finishes the update last sums
 them once for all (an integer arrival counter picks it). The schedule field
 pcg_partials forces either way.
@@ -50,9 +50,9 @@ def run(m, partials=0, **kw):
 @pytest.fixture(scope="module")
 def walk850():
     # 21 groups: the first solve (15-node coarse space only) sums per SpMV,
-    # the later ones (12 recycled vectors added) once per update.
+    # the later ones (16 recycled vectors added) once per update.
     m = problem.workload_map(2, seed=5, num_keyframes=850, num_landmarks=15000)
-    assert 19 < (m.num_vertices + 41) // 42 <= 22
+    assert 18 < (m.num_vertices + 41) // 42 <= 22
     return m
 
 
