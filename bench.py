@@ -106,10 +106,10 @@ TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "prof
                                "lm_loop_random_walk_500kf_hbm_traffic.md")
 # Bench kernel row -> the kernel whose rocprofv3 FETCH_SIZE / WRITE_SIZE row
 # gives its HBM traffic (the Schur block build is dominated by its item pass).
-TRAFFIC_KERNEL = {"k_schur_blocks": "mlba::k_schur_items", "k_pcg_spmv": "mlba::k_pcg_spmv",
+TRAFFIC_KERNEL = {"k_schur_blocks": "mlba::k_schur_items<4>", "k_pcg_spmv": "mlba::k_pcg_spmv<false>",
                   "k_pcg_update": "mlba::k_pcg_update", "k_schur_incidences": "mlba::k_schur_incidences",
-                  "k_schur_factors": "mlba::k_schur_factors", "k_linearize": "mlba::k_linearize<true>",
-                  "k_back_substitute": "mlba::k_back_substitute"}
+                  "k_schur_factors": "mlba::k_schur_factors<false>", "k_linearize": "mlba::k_linearize<true, false>",
+                  "k_back_substitute": "mlba::k_back_substitute<false>"}
 
 
 def measured_traffic(kernel):
