@@ -71,6 +71,16 @@ def test_both_partial_sums_give_the_same_solve(request, size):
         np.testing.assert_array_equal(other[3], auto[3])
 
 
+def test_both_partial_sums_without_deflation(walk1000):
+    # Block-Jacobi PCG: 3 partials per group (the per-SpMV sum by default).
+    kw = dict(max_num_iterations=4, pcg_coarse_nodes=-1, pcg_ritz_vectors=-1, pcg_max_iterations=20000)
+    per_spmv = run(walk1000, partials=1, **kw)
+    once = run(walk1000, partials=2, **kw)
+    assert once[1] == per_spmv[1]
+    np.testing.assert_array_equal(once[2], per_spmv[2])
+    np.testing.assert_array_equal(once[3], per_spmv[3])
+
+
 def test_many_groups_match_oracle(oracle, walk1000):
     o = opts(max_num_iterations=2)
     sm, rows, v, _ = run(walk1000, max_num_iterations=2)
