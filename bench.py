@@ -79,7 +79,7 @@ def kernel_table(sm, kt, n_pcg_total):
         r["achieved_tflops"] = round(pairs * 216 / r["avg_ms"] / 1e9, 3)
     add("k_back_substitute", abi.KERNEL_BACK_SUBSTITUTE, N * 160 + L * 216)
     add("k_pose_rows", abi.KERNEL_POSE_ROWS)
-    add("k_landmark_prep", abi.KERNEL_LANDMARK_PREP, L * (144 + 24 + 48))
+    add("k_landmark_prep", abi.KERNEL_LANDMARK_PREP, L * (144 + 24 + 72))
     launches = sm.kernel_launches[abi.KERNEL_PCG]
     if launches and n_pcg_total > 0:
         it_ms = sm.kernel_ms[abi.KERNEL_PCG] / n_pcg_total

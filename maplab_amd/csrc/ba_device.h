@@ -35,6 +35,7 @@ struct RecView<true> {
   __host__ __device__ double operator[](int k) const { return k < 2 ? p->r[k] : static_cast<double>(p->J[k - 2]); }
 };
 constexpr int kLmAcc = 18;           // V(6) V_mission(6) g(3) g_mission(3)
+constexpr int kLmZ = 9;              // Z(6, upper triangle) and a copy of g(3), read together per incidence
 constexpr int kRowThreads = 256;     // threads of a reduced-camera-system row block
 
 // Landmark flags.

@@ -46,7 +46,7 @@ void launch_schur_factors(hipStream_t s, const DevTopology& t, int32_t n_inc, co
                           const double* vR, const double* lm, const double* obs_rec, const double* lm_acc,
                           const double* pose_scale, double* inc_w, double* inc_r0, double* lm_a, bool f32 = false);
 void launch_schur_incidences(hipStream_t s, const DevTopology& t, int32_t n_inc, const double* inc_w,
-                             const double* inc_r0, const double* lm_a, const double* lm_acc, const double* lm_z,
+                             const double* inc_r0, const double* lm_a, const double* lm_z,
                              double* inc_q, double* inc_rhs);
 void launch_schur_blocks(hipStream_t s, const DevTopology& t, int32_t nub, const int4* ublk, int32_t n_items,
                          const int4* items, const int32_t* item_ptr, const uint2* pairs, const double* inc_q,
@@ -929,7 +929,7 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
       }
       {
         KernelTimer kt(h, MLBA_KERNEL_SCHUR_INCIDENCES);
-        launch_schur_incidences(s, h->topo, h->n_inc, h->inc_w.p, h->inc_r0.p, h->lm_a.p, h->lm_acc.p, h->lm_z.p,
+        launch_schur_incidences(s, h->topo, h->n_inc, h->inc_w.p, h->inc_r0.p, h->lm_a.p, h->lm_z.p,
                                 h->inc_q.p, h->inc_rhs.p);
       }
       {
@@ -1270,7 +1270,7 @@ int32_t mlba_create(const mlba_problem_desc* d, mlba_handle** out) {
   const bool ok = h->obs_rec.alloc((size_t)kObsRecord * N) && h->vert_cand.alloc(7 * (size_t)P) &&
                   h->vR.alloc(12 * (size_t)P) && h->vR_cand.alloc(12 * (size_t)P) && h->lm_cand.alloc(3 * (size_t)L) &&
                   h->lm_acc.alloc((size_t)kLmAcc * L) && h->lm_scale.alloc(3 * (size_t)L) &&
-                  h->lm_z.alloc(6 * (size_t)L) && h->lm_u.alloc(3 * (size_t)L) && h->scal.alloc(kNumSlots) &&
+                  h->lm_z.alloc(kLmZ * (size_t)L) && h->lm_u.alloc(3 * (size_t)L) && h->scal.alloc(kNumSlots) &&
                   h->fail.alloc(1) && h->pcg_state.alloc(pcg_state_bytes());
   if (!ok) { h->err = "device allocation failed"; return fail_out(MLBA_ENOMEM); }
   if (hipHostMalloc(&h->pcg_host, pcg_state_bytes(), hipHostMallocDefault) != hipSuccess ||
@@ -1973,7 +1973,7 @@ int32_t mlba_autotune(mlba_handle* h, const mlba_options* o, int32_t repeats, ml
     HIP_OK(hipMemsetAsync(h->fail.p, 0, sizeof(int32_t), s));
     launch_landmark_prep(s, h->topo, h->lm_acc.p, h->lm_scale.p, 1.0 / o->initial_trust_region_radius,
                          o->min_lm_diagonal, o->max_lm_diagonal, h->lm_z.p, h->fail.p);
-    launch_schur_incidences(s, h->topo, h->n_inc, h->inc_w.p, h->inc_r0.p, h->lm_a.p, h->lm_acc.p, h->lm_z.p,
+    launch_schur_incidences(s, h->topo, h->n_inc, h->inc_w.p, h->inc_r0.p, h->lm_a.p, h->lm_z.p,
                             h->inc_q.p, h->inc_rhs.p);
     HIP_OK(hipGetLastError());
     hipEvent_t e0, e1;
@@ -2078,7 +2078,7 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
                          h->lm_z.p, h->fail.p);
   });
   t->schur_incidences_ms = timed([&] {
-    launch_schur_incidences(s, h->topo, h->n_inc, h->inc_w.p, h->inc_r0.p, h->lm_a.p, h->lm_acc.p, h->lm_z.p,
+    launch_schur_incidences(s, h->topo, h->n_inc, h->inc_w.p, h->inc_r0.p, h->lm_a.p, h->lm_z.p,
                             h->inc_q.p, h->inc_rhs.p);
   });
   t->schur_blocks_ms = timed([&] {

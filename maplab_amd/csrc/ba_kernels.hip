@@ -424,16 +424,19 @@ __global__ void k_jacobi_scaling(int32_t Pvar, int32_t L, int32_t use, const dou
 // ------------------------------------------------------------ landmark pivot
 
 // Z_l = C (C V C + D^2)^-1 C, D^2 = clamp(diag(C V C)) / mu (ceres
-// LevenbergMarquardtStrategy diagonal). Zero for non-variable landmarks.
+// LevenbergMarquardtStrategy diagonal), followed by a copy of the landmark
+// gradient g_l, so that the per-incidence Schur factors read Z_l and g_l of
+// their (scattered) landmark from one 72-byte record. Zero for non-variable
+// landmarks.
 __global__ void k_landmark_prep(DevTopology t, const double* __restrict__ lm_acc,
                                 const double* __restrict__ lm_scale, double inv_mu, double min_diag,
                                 double max_diag, double* __restrict__ lm_z, int32_t* __restrict__ fail) {
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= t.L) return;
-  double* z = lm_z + 6 * l;
+  double* z = lm_z + kLmZ * l;
   const uint32_t flags = t.lm_flags[l];
   if (!(flags & kLmActive) || (flags & kLmConstant)) {
-    for (int i = 0; i < 6; ++i) z[i] = 0.0;
+    for (int i = 0; i < kLmZ; ++i) z[i] = 0.0;
     return;
   }
   const double* V = lm_acc + (size_t)kLmAcc * l;
@@ -446,11 +449,12 @@ __global__ void k_landmark_prep(DevTopology t, const double* __restrict__ lm_acc
   double inv[6];
   if (!spd3_inverse(A, inv)) {
     atomicAdd(fail, 1);
-    for (int i = 0; i < 6; ++i) z[i] = 0.0;
+    for (int i = 0; i < kLmZ; ++i) z[i] = 0.0;
     return;
   }
   z[0] = c[0] * c[0] * inv[0]; z[1] = c[0] * c[1] * inv[1]; z[2] = c[0] * c[2] * inv[2];
   z[3] = c[1] * c[1] * inv[3]; z[4] = c[1] * c[2] * inv[4]; z[5] = c[2] * c[2] * inv[5];
+  z[6] = V[12]; z[7] = V[13]; z[8] = V[14];
 }
 
 // ------------------------------------------------------------ Schur complement
@@ -518,15 +522,15 @@ __global__ __launch_bounds__(256) void k_schur_factors(
 // Per LM iteration: Q_x and rhs_x of every incidence (one thread each).
 __global__ __launch_bounds__(256) void k_schur_incidences(
     DevTopology t, int32_t n_inc, const double* __restrict__ inc_w, const double* __restrict__ inc_r0,
-    const double* __restrict__ lm_a, const double* __restrict__ lm_acc, const double* __restrict__ lm_z,
-    double* __restrict__ inc_q, double* __restrict__ inc_rhs) {
+    const double* __restrict__ lm_a, const double* __restrict__ lm_z, double* __restrict__ inc_q,
+    double* __restrict__ inc_rhs) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n_inc) return;
   const int32_t x = t.inc[p];
   const int32_t l = x >= 0 ? t.obs_landmark[x] : -1 - x;
   const double* W = inc_w + (size_t)18 * p;
   double Z[9], Qx[18], rx[6];
-  sym3_full(lm_z + 6 * l, Z);
+  sym3_full(lm_z + kLmZ * l, Z);
   for (int c = 0; c < 6; ++c) rx[c] = inc_r0[(size_t)6 * p + c];
   if (x >= 0) {
     for (int k = 0; k < 18; ++k) Qx[k] = 0.0;
@@ -535,7 +539,7 @@ __global__ __launch_bounds__(256) void k_schur_incidences(
     for (int c = 0; c < 6; ++c)
       for (int a = 0; a < 3; ++a) Qx[c * 3 + a] = A[a * 6 + c];
   }
-  const double* gl = lm_acc + (size_t)kLmAcc * l + 12;
+  const double* gl = lm_z + kLmZ * l + 6;
   for (int c = 0; c < 6; ++c) {
     double wz[3];
     for (int a = 0; a < 3; ++a) wz[a] = W[c] * Z[a] + W[6 + c] * Z[3 + a] + W[12 + c] * Z[6 + a];
@@ -904,7 +908,7 @@ __device__ inline void landmark_step(const DevTopology& t, int l, const double* 
       for (int a = 0; a < 3; ++a) tv[a] -= VLM[a * 3] * Mwa[0] + VLM[a * 3 + 1] * Mwa[1] + VLM[a * 3 + 2] * Mwa[2];
     }
     double Z[9];
-    sym3_full(lm_z + 6 * l, Z);
+    sym3_full(lm_z + kLmZ * l, Z);
     for (int a = 0; a < 3; ++a) u[a] = Z[a * 3] * tv[0] + Z[a * 3 + 1] * tv[1] + Z[a * 3 + 2] * tv[2];
   }
 }
@@ -1433,11 +1437,10 @@ void launch_schur_factors(hipStream_t s, const DevTopology& t, int32_t n_inc, co
                                                                pose_scale, inc_w, inc_r0, lm_a);
 }
 void launch_schur_incidences(hipStream_t s, const DevTopology& t, int32_t n_inc, const double* inc_w,
-                             const double* inc_r0, const double* lm_a, const double* lm_acc, const double* lm_z,
-                             double* inc_q, double* inc_rhs) {
+                             const double* inc_r0, const double* lm_a, const double* lm_z, double* inc_q,
+                             double* inc_rhs) {
   if (t.Pvar > 0 && n_inc > 0)
-    k_schur_incidences<<<(n_inc + 255) / 256, 256, 0, s>>>(t, n_inc, inc_w, inc_r0, lm_a, lm_acc, lm_z, inc_q,
-                                                           inc_rhs);
+    k_schur_incidences<<<(n_inc + 255) / 256, 256, 0, s>>>(t, n_inc, inc_w, inc_r0, lm_a, lm_z, inc_q, inc_rhs);
 }
 template <int kWaves>
 void launch_items_w(hipStream_t s, const DevTopology& t, int32_t n_items, int32_t xcd_map, const int4* items,
