@@ -341,15 +341,11 @@ __global__ __launch_bounds__(kPrepThreads) void k_defl_prep(PcgArgs a, double to
     if (er[m] == 0) colb[0][ec[m]] = v[m];
     else if (ec[m] == 0) colb[0][er[m]] = v[m];
   }
-  for (int k = 0; k < ne; ++k) {
-    __syncthreads();
-    const double* colk = colb[k & 1];
-    double* next = colb[(k + 1) & 1];
+  // Pivot k reads column k from colk and writes column k + 1 into next; the
+  // loop takes two pivots per pass so that each buffer is a fixed LDS address.
+  auto pivot = [&](int k, const double* colk, double* next) -> bool {
     const double p = colk[k];
-    if (!(p > 0.0)) {  // the same value in every thread
-      ok = false;
-      break;
-    }
+    if (!(p > 0.0)) return false;  // the same value in every thread
     const double ip = 1.0 / p;
 #pragma unroll
     for (int m = 0; m < kInvPerThread; ++m) {
@@ -360,6 +356,20 @@ __global__ __launch_bounds__(kPrepThreads) void k_defl_prep(PcgArgs a, double to
       v[m] = (ri && rj) ? -ip : nv;
       if (er[m] == k + 1) next[ec[m]] = v[m];       // (k + 1, j <= k + 1)
       else if (ec[m] == k + 1) next[er[m]] = v[m];  // (i > k + 1, k + 1)
+    }
+    return true;
+  };
+  for (int k = 0; k < ne; k += 2) {
+    __syncthreads();
+    if (!pivot(k, colb[0], colb[1])) {
+      ok = false;
+      break;
+    }
+    if (k + 1 == ne) break;
+    __syncthreads();
+    if (!pivot(k + 1, colb[1], colb[0])) {
+      ok = false;
+      break;
     }
   }
   __syncthreads();
