@@ -79,6 +79,7 @@ void launch_candidate(hipStream_t s, const DevTopology& t, const double* vert, c
                       double* part_v, const double* lm, const double* u, double* cand_l, double* part_l);
 void launch_gradient_norm(hipStream_t s, const DevTopology& t, const double* vert, const double* pose_grad,
                           const double* lm_acc, double* part);
+void launch_reduce_many(hipStream_t s, const ReduceBatch& batch);
 void launch_reduce(hipStream_t s, int32_t n, int32_t stride, int32_t offset, bool is_max, const double* parts,
                    double* out);
 void launch_evaluate_full(hipStream_t s, const DevTopology& t, const double* vert, const double* vR, const double* lm,
@@ -854,11 +855,14 @@ int linearize(mlba_handle* h, const mlba_options* o) {
     launch_pose_rows(s, h->topo, h->vert.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p, h->pose_grad.p,
                      h->pose_diag.p, h->pose_jtj.p, h->f32_jac);
   }
-  launch_reduce(s, h->tiles, 2, 0, false, h->tile_cost.p, h->scal.p + kCost);
-  launch_reduce(s, h->tiles, 2, 1, false, h->tile_cost.p, h->scal.p + kFixed);
   const int gb = (h->Pvar + h->L + 255) / 256;
   launch_gradient_norm(s, h->topo, h->vert.p, h->pose_grad.p, h->lm_acc.p, h->parts_g.p);
-  launch_reduce(s, gb, 1, 0, true, h->parts_g.p, h->scal.p + kGradMax);
+  ReduceBatch rb{};
+  rb.d[0] = {h->tiles, 2, 0, 0, h->tile_cost.p, h->scal.p + kCost};
+  rb.d[1] = {h->tiles, 2, 1, 0, h->tile_cost.p, h->scal.p + kFixed};
+  rb.d[2] = {gb, 1, 0, 1, h->parts_g.p, h->scal.p + kGradMax};
+  rb.count = 3;
+  launch_reduce_many(s, rb);
   HIP_OK(hipGetLastError());
   return MLBA_OK;
 }
@@ -963,14 +967,18 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
                                      h->lm.p, h->obs_rec.p, h->lm_acc.p, h->lm_z.p, h->w.p, h->lm_u.p,
                                      h->parts_l.p, h->f32_jac);
       }
-      launch_reduce(s, nbs, 2, 0, false, h->parts_l.p, h->scal.p + kMcc);
-      launch_reduce(s, nbs, 2, 1, false, h->parts_l.p, h->scal.p + kUU);
-      launch_reduce(s, wb, 1, 0, false, h->parts_w.p, h->scal.p + kWW);
-      // Candidate point and its cost.
+      // Candidate point and its cost; the step's scalars (model cost change,
+      // step and state norms) in one batch of reductions.
       launch_candidate(s, h->topo, h->vert.p, h->w.p, h->vert_cand.p, h->parts_v.p, h->lm.p, h->lm_u.p,
                        h->lm_cand.p, h->parts_c.p);
-      launch_reduce(s, pb, 1, 0, false, h->parts_v.p, h->scal.p + kXV);
-      launch_reduce(s, lb, 1, 0, false, h->parts_c.p, h->scal.p + kXL);
+      ReduceBatch rb{};
+      rb.d[0] = {nbs, 2, 0, 0, h->parts_l.p, h->scal.p + kMcc};
+      rb.d[1] = {nbs, 2, 1, 0, h->parts_l.p, h->scal.p + kUU};
+      rb.d[2] = {wb, 1, 0, 0, h->parts_w.p, h->scal.p + kWW};
+      rb.d[3] = {pb, 1, 0, 0, h->parts_v.p, h->scal.p + kXV};
+      rb.d[4] = {lb, 1, 0, 0, h->parts_c.p, h->scal.p + kXL};
+      rb.count = 5;
+      launch_reduce_many(s, rb);
       launch_vertex_prep(s, h->P, h->vert_cand.p, h->vR_cand.p);
       {
         KernelTimer kt(h, MLBA_KERNEL_COST);

@@ -1148,8 +1148,8 @@ __global__ __launch_bounds__(256) void k_gradient_norm(DevTopology t, const doub
 
 // Fixed-order final reduction of partial arrays: out[k] = sum (or max) of
 // parts[k] (stride `stride`, n entries). One block per output.
-__global__ void k_reduce(int32_t n, int32_t stride, int32_t offset, int32_t is_max, const double* __restrict__ parts,
-                         double* __restrict__ out) {
+__device__ inline void reduce_parts(int32_t n, int32_t stride, int32_t offset, int32_t is_max,
+                                    const double* __restrict__ parts, double* __restrict__ out) {
   __shared__ double lds4[4];
   double s = 0.0;
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
@@ -1166,6 +1166,18 @@ __global__ void k_reduce(int32_t n, int32_t stride, int32_t offset, int32_t is_m
     s = block_sum256(s, lds4);
     if (threadIdx.x == 0) *out = s;
   }
+}
+
+__global__ void k_reduce(int32_t n, int32_t stride, int32_t offset, int32_t is_max, const double* __restrict__ parts,
+                         double* __restrict__ out) {
+  reduce_parts(n, stride, offset, is_max, parts, out);
+}
+
+// Several independent reductions in one launch, work-group b computing
+// reduction b exactly as k_reduce does.
+__global__ void k_reduce_many(ReduceBatch batch) {
+  const ReduceDesc& d = batch.d[blockIdx.x];
+  reduce_parts(d.n, d.stride, d.offset, d.is_max, d.parts, d.out);
 }
 
 // ------------------------------------------------------------ Evaluate API
@@ -1535,6 +1547,9 @@ void launch_gradient_norm(hipStream_t s, const DevTopology& t, const double* ver
 void launch_reduce(hipStream_t s, int32_t n, int32_t stride, int32_t offset, bool is_max, const double* parts,
                    double* out) {
   k_reduce<<<1, 256, 0, s>>>(n, stride, offset, is_max ? 1 : 0, parts, out);
+}
+void launch_reduce_many(hipStream_t s, const ReduceBatch& batch) {
+  if (batch.count > 0 && batch.count <= kMaxReduceBatch) k_reduce_many<<<batch.count, 256, 0, s>>>(batch);
 }
 // ------------------------------------------------------------ calibration
 
