@@ -1,15 +1,20 @@
 /*
  * maplab_ba_plugin.h -- console-plugin boundary of the map-optimization
  * plugin (console-plugins/map-optimization-plugin/src/optimizer-plugin.cc:
- * 27-55, 190-235). libmaplab_ba_plugin.so exports the two symbols maplab's
- * console resolves with dlsym (console-plugin-base.h:54-58,
- * console-plugin-base-with-plotter.h:43-56), createConsolePlugin and
- * destroyConsolePlugin, under the plugin id "map-optimization", with the
- * commands optimize_visual / optv and optimize_visual_inertial / optvi run
- * synchronously. Their options come from the reference's flag names
- * (--ba_*, --elq_*, --map_mission, --map_mission_list) given on the command
- * line. The console is the small host structure below: it holds the selected
- * map (the VIMapManager map under write access in the reference).
+ * 27-55, 190-235): plugin id "map-optimization", commands optimize_visual /
+ * optv and optimize_visual_inertial / optvi run synchronously, options from
+ * the reference's flag names (--ba_*, --elq_*, --map_mission,
+ * --map_mission_list) given on the command line. The console is the small
+ * host structure below: it holds the selected map (the VIMapManager map
+ * under write access in the reference).
+ *
+ * The entry points are C functions with C types. maplab's console resolves
+ * createConsolePlugin / destroyConsolePlugin with C++ types
+ * (console-plugin-base-with-plotter.h:43-56, maplab-console.cc:129-138); this
+ * library does not export those names, so it cannot be mistaken for a
+ * plugin of that ABI. A C++ shim compiled against maplab exports them and
+ * forwards to mlba_plugin_create / mlba_plugin_destroy (shown in the
+ * integration notes).
  */
 #ifndef MAPLAB_BA_PLUGIN_H_
 #define MAPLAB_BA_PLUGIN_H_
@@ -49,11 +54,14 @@ typedef struct mlba_console {
   mlba_summary* last_summary;     /* nullable: receives the summary of the last optimization */
 } mlba_console;
 
-/* Plugin lifetime (the console's dlsym names; plotter unused, may be NULL). */
-void* createConsolePlugin(mlba_console* console, void* plotter);
-void destroyConsolePlugin(void* plugin);
+/* Plugin lifetime: createConsolePlugin / destroyConsolePlugin
+ * (console-plugin-base-with-plotter.h:43-56) with C types; plotter unused,
+ * may be NULL. */
+void* mlba_plugin_create(mlba_console* console, void* plotter);
+void mlba_plugin_destroy(void* plugin);
 
-/* The plugin id ("map-optimization") and its command names. */
+/* The plugin id ("map-optimization") and its command names; the returned
+ * strings stay valid until mlba_plugin_destroy. */
 const char* mlba_plugin_id(const void* plugin);
 int32_t mlba_plugin_num_commands(const void* plugin);
 const char* mlba_plugin_command_name(const void* plugin, int32_t index);

@@ -35,9 +35,10 @@ struct RecView<true> {
   __host__ __device__ double operator[](int k) const { return k < 2 ? p->r[k] : static_cast<double>(p->J[k - 2]); }
 };
 constexpr int kLmAcc = 18;           // V(6) V_mission(6) g(3) g_mission(3)
-constexpr int kLmZ = 9;
+constexpr int kLmZ = 9;              // Z(6, upper triangle) and a copy of g(3), read together per incidence
 
-// A batch of independent scalar reductions (k_reduce_many).
+// A batch of independent scalar reductions (k_reduce_many), 1..kMaxReduceBatch
+// of them.
 struct ReduceDesc {
   int32_t n, stride, offset, is_max;
   const double* parts;
@@ -47,7 +48,7 @@ constexpr int kMaxReduceBatch = 8;
 struct ReduceBatch {
   ReduceDesc d[kMaxReduceBatch];
   int32_t count;
-};              // Z(6, upper triangle) and a copy of g(3), read together per incidence
+};
 constexpr int kRowThreads = 256;     // threads of a reduced-camera-system row block
 
 // Landmark flags.

@@ -33,8 +33,9 @@ namespace mlba {
 void launch_vertex_prep(hipStream_t s, int32_t P, const double* vert, double* vR);
 void launch_linearize(hipStream_t s, bool jac, int32_t tiles, const DevTopology& t, const double* vert,
                       const double* vR, const double* lm, DevLoss loss, const int32_t* tile_begin, double* obs_rec,
-                      double* lm_acc, double* tile_cost, bool f32 = false);
-void launch_zero_split(hipStream_t s, int32_t n, const int32_t* ids, double* lm_acc);
+                      double* lm_acc, double* split_part, double* tile_cost, bool f32 = false);
+void launch_split_sum(hipStream_t s, int32_t n, const int32_t* ids, const int32_t* split_tiles,
+                      const double* split_part, double* lm_acc);
 void launch_pose_rows(hipStream_t s, const DevTopology& t, const double* vert, const double* vR, const double* lm,
                       const double* obs_rec, const double* lm_acc, double* pose_grad, double* pose_diag,
                       double* pose_jtj, bool f32 = false);
@@ -79,7 +80,7 @@ void launch_candidate(hipStream_t s, const DevTopology& t, const double* vert, c
                       double* part_v, const double* lm, const double* u, double* cand_l, double* part_l);
 void launch_gradient_norm(hipStream_t s, const DevTopology& t, const double* vert, const double* pose_grad,
                           const double* lm_acc, double* part);
-void launch_reduce_many(hipStream_t s, const ReduceBatch& batch);
+bool launch_reduce_many(hipStream_t s, const ReduceBatch& batch);
 void launch_reduce(hipStream_t s, int32_t n, int32_t stride, int32_t offset, bool is_max, const double* parts,
                    double* out);
 void launch_evaluate_full(hipStream_t s, const DevTopology& t, const double* vert, const double* vR, const double* lm,
@@ -161,7 +162,8 @@ struct mlba_handle {
   // Device memory.
   DBuf<double> obs_kp, obs_sinv, obs_rec;
   DBuf<int32_t> obs_vertex, obs_meta, obs_landmark, perm_d;
-  DBuf<int32_t> lm_begin_d, lm_anchor_d, split_ids;
+  DBuf<int32_t> lm_begin_d, lm_anchor_d, split_ids, split_tiles;  // split landmarks: id; first tile, tile count
+  DBuf<double> split_part;                                         // per-tile sums of split landmarks
   DBuf<uint32_t> lm_flags;
   DBuf<double> lm, lm_cand, lm_acc, lm_scale, lm_z, lm_u;
   DBuf<double> vert, vert_cand, vR, vR_cand;
@@ -330,6 +332,15 @@ int build_topology(mlba_handle* h) {
     // Remove empty tiles.
     tile_begin.erase(std::unique(tile_begin.begin(), tile_begin.end()), tile_begin.end());
   }
+  // The tiles of a split landmark: consecutive, the first starting at its
+  // first observation.
+  std::vector<int32_t> split_tiles;
+  for (int l : split) {
+    const auto it = std::lower_bound(tile_begin.begin(), tile_begin.end(), h->lm_begin[l]);
+    const int n = h->lm_begin[l + 1] - h->lm_begin[l];
+    split_tiles.push_back((int32_t)(it - tile_begin.begin()));
+    split_tiles.push_back((n + kObsTile - 1) / kObsTile);
+  }
   h->tiles = (int32_t)tile_begin.size() - 1;
   h->n_split = (int32_t)split.size();
   for (int l = 0; l < L; ++l) {
@@ -494,6 +505,7 @@ int build_topology(mlba_handle* h) {
   if ((rc = upload(h, h->var_vertex, var_vertex.data(), var_vertex.size()))) return rc;
   if ((rc = upload(h, h->tile_begin, tile_begin.data(), tile_begin.size()))) return rc;
   if ((rc = upload(h, h->split_ids, split.data(), split.size()))) return rc;
+  if ((rc = upload(h, h->split_tiles, split_tiles.data(), split_tiles.size()))) return rc;
   if ((rc = upload(h, h->inc_ptr, inc_ptr.data(), inc_ptr.size()))) return rc;
   if ((rc = upload(h, h->inc, inc.data(), inc.size()))) return rc;
   if ((rc = upload(h, h->row_ptr, row_ptr.data(), row_ptr.size()))) return rc;
@@ -508,6 +520,7 @@ int build_topology(mlba_handle* h) {
   const size_t T = h->tiles, V6 = 6 * (size_t)std::max(Pvar, 1);
   const size_t vb = (Pvar + 255) / 256 + 1;
   bool ok = h->tile_cost.alloc(2 * T) && h->tile_cost2.alloc(2 * T) && h->S.alloc(36 * (size_t)h->nnzb) &&
+            h->split_part.alloc(h->n_split ? kLmAcc * T : 1) &&
             h->inc_q.alloc(18 * std::max<size_t>(inc.size(), 1)) && h->inc_rhs.alloc(6 * std::max<size_t>(inc.size(), 1)) &&
             h->inc_w.alloc(18 * std::max<size_t>(inc.size(), 1)) && h->inc_r0.alloc(6 * std::max<size_t>(inc.size(), 1)) &&
             h->lm_a.alloc(18 * (size_t)std::max(L, 1)) &&
@@ -844,12 +857,12 @@ void schur_factors(mlba_handle* h) {
 int linearize(mlba_handle* h, const mlba_options* o) {
   hipStream_t s = h->stream;
   launch_vertex_prep(s, h->P, h->vert.p, h->vR.p);
-  launch_zero_split(s, h->n_split, h->split_ids.p, h->lm_acc.p);
   {
     KernelTimer kt(h, MLBA_KERNEL_LINEARIZE);
     launch_linearize(s, true, h->tiles, h->topo, h->vert.p, h->vR.p, h->lm.p, make_loss(o), h->tile_begin.p,
-                     h->obs_rec.p, h->lm_acc.p, h->tile_cost.p, h->f32_jac);
+                     h->obs_rec.p, h->lm_acc.p, h->split_part.p, h->tile_cost.p, h->f32_jac);
   }
+  launch_split_sum(s, h->n_split, h->split_ids.p, h->split_tiles.p, h->split_part.p, h->lm_acc.p);
   {
     KernelTimer kt(h, MLBA_KERNEL_POSE_ROWS);
     launch_pose_rows(s, h->topo, h->vert.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p, h->pose_grad.p,
@@ -862,7 +875,7 @@ int linearize(mlba_handle* h, const mlba_options* o) {
   rb.d[1] = {h->tiles, 2, 1, 0, h->tile_cost.p, h->scal.p + kFixed};
   rb.d[2] = {gb, 1, 0, 1, h->parts_g.p, h->scal.p + kGradMax};
   rb.count = 3;
-  launch_reduce_many(s, rb);
+  if (!launch_reduce_many(s, rb)) return set_err(h, MLBA_EINVAL, "internal: scalar reduction batch size");
   HIP_OK(hipGetLastError());
   return MLBA_OK;
 }
@@ -978,12 +991,12 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
       rb.d[3] = {pb, 1, 0, 0, h->parts_v.p, h->scal.p + kXV};
       rb.d[4] = {lb, 1, 0, 0, h->parts_c.p, h->scal.p + kXL};
       rb.count = 5;
-      launch_reduce_many(s, rb);
+      if (!launch_reduce_many(s, rb)) return set_err(h, MLBA_EINVAL, "internal: scalar reduction batch size");
       launch_vertex_prep(s, h->P, h->vert_cand.p, h->vR_cand.p);
       {
         KernelTimer kt(h, MLBA_KERNEL_COST);
         launch_linearize(s, false, h->tiles, h->topo, h->vert_cand.p, h->vR_cand.p, h->lm_cand.p, make_loss(o),
-                         h->tile_begin.p, nullptr, nullptr, h->tile_cost2.p);
+                         h->tile_begin.p, nullptr, nullptr, nullptr, h->tile_cost2.p);
       }
       launch_reduce(s, h->tiles, 2, 0, false, h->tile_cost2.p, h->scal.p + kCandCost);
       HIP_OK(hipGetLastError());
@@ -1391,7 +1404,7 @@ int32_t mlba_evaluate(mlba_handle* h, int32_t flags, const mlba_options* options
     mlba_options_init(&defaults);
     const mlba_options* o = options ? options : &defaults;
     launch_linearize(s, false, h->tiles, h->topo, h->vert.p, h->vR.p, h->lm.p, make_loss(o), h->tile_begin.p,
-                     nullptr, nullptr, h->tile_cost2.p);
+                     nullptr, nullptr, nullptr, h->tile_cost2.p);
     launch_reduce(s, h->tiles, 2, 0, false, h->tile_cost2.p, h->scal.p + kCandCost);
     launch_reduce(s, h->tiles, 2, 1, false, h->tile_cost2.p, h->scal.p + kCandFixed);
   }
@@ -2068,13 +2081,13 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
   t->schur_factors_ms = timed([&] { schur_factors(h); });
   launch_vertex_prep(s, h->P, h->vert.p, h->vR.p);
   t->linearize_ms = timed([&] {
-    launch_zero_split(s, h->n_split, h->split_ids.p, h->lm_acc.p);
     launch_linearize(s, true, h->tiles, h->topo, h->vert.p, h->vR.p, h->lm.p, loss, h->tile_begin.p, h->obs_rec.p,
-                     h->lm_acc.p, h->tile_cost.p, h->f32_jac);
+                     h->lm_acc.p, h->split_part.p, h->tile_cost.p, h->f32_jac);
+    launch_split_sum(s, h->n_split, h->split_ids.p, h->split_tiles.p, h->split_part.p, h->lm_acc.p);
   });
   t->cost_ms = timed([&] {
     launch_linearize(s, false, h->tiles, h->topo, h->vert.p, h->vR.p, h->lm.p, loss, h->tile_begin.p, nullptr,
-                     nullptr, h->tile_cost2.p);
+                     nullptr, nullptr, h->tile_cost2.p);
   });
   t->pose_rows_ms = timed([&] {
     launch_pose_rows(s, h->topo, h->vert.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p, h->pose_grad.p,

@@ -23,8 +23,9 @@
 //   k_gradient_norm   max |x - Plus(x, -g)|
 //   k_evaluate_full   every reference Jacobian block (Evaluate parity API)
 // All reductions are fixed-order (per-tile partials summed by one block,
-// S blocks by one wave each), so results repeat bit for bit run to run,
-// except landmarks split over linearization tiles (FP64 atomics).
+// S blocks by one wave each, the tile sums of a landmark split over several
+// linearization tiles added in tile order), so results repeat bit for bit
+// run to run.
 #include "ba_device.h"
 #include "ba_math.h"
 #include "ba_reduce.h"
@@ -216,6 +217,7 @@ __global__ __launch_bounds__(kObsTile) void k_linearize(DevTopology t, const dou
                                                          const int32_t* __restrict__ tile_begin,
                                                          void* __restrict__ obs_rec,
                                                          double* __restrict__ lm_acc,
+                                                         double* __restrict__ split_part,
                                                          double* __restrict__ tile_cost) {
   __shared__ double red[kJac ? kObsTile * kLmAcc : 1];
   __shared__ int32_t seg[kJac ? kObsTile : 1];
@@ -311,12 +313,11 @@ __global__ __launch_bounds__(kObsTile) void k_linearize(DevTopology t, const dou
       for (int i = 0; i < kLmAcc; ++i) sum[i] = 0.0;
       for (int k = tid; k < kObsTile && seg[k] == l; ++k)
         for (int i = 0; i < kLmAcc; ++i) sum[i] += red[k * kLmAcc + i];
-      double* dst = lm_acc + (size_t)kLmAcc * l;
-      if (t.lm_flags[l] & kLmSplit) {
-        for (int i = 0; i < kLmAcc; ++i) atomicAdd(dst + i, sum[i]);
-      } else {
-        for (int i = 0; i < kLmAcc; ++i) dst[i] = sum[i];
-      }
+      // A split landmark owns its tiles whole: the tile's sum goes to the
+      // tile's slot, and k_split_sum adds the slots in tile order.
+      double* dst = (t.lm_flags[l] & kLmSplit) ? split_part + (size_t)kLmAcc * blockIdx.x
+                                               : lm_acc + (size_t)kLmAcc * l;
+      for (int i = 0; i < kLmAcc; ++i) dst[i] = sum[i];
     }
   }
   const double c_sum = block_sum256(cost, lds4);
@@ -327,11 +328,17 @@ __global__ __launch_bounds__(kObsTile) void k_linearize(DevTopology t, const dou
   }
 }
 
-// Zero the accumulators of landmarks split over several tiles.
-__global__ void k_zero_split(int32_t n, const int32_t* __restrict__ ids, double* __restrict__ lm_acc) {
+// Accumulators of the landmarks split over several tiles: the tile sums of
+// each, added in tile order. split_tiles[2k], [2k+1] = first tile, tile count.
+__global__ void k_split_sum(int32_t n, const int32_t* __restrict__ ids, const int32_t* __restrict__ split_tiles,
+                            const double* __restrict__ split_part, double* __restrict__ lm_acc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n * kLmAcc) return;
-  lm_acc[(size_t)kLmAcc * ids[i / kLmAcc] + i % kLmAcc] = 0.0;
+  const int k = i / kLmAcc, c = i % kLmAcc;
+  const int t0 = split_tiles[2 * k], nt = split_tiles[2 * k + 1];
+  double s = 0.0;
+  for (int j = 0; j < nt; ++j) s += split_part[(size_t)kLmAcc * (t0 + j) + c];
+  lm_acc[(size_t)kLmAcc * ids[k] + c] = s;
 }
 
 // ------------------------------------------------------------ vertex rows
@@ -1404,17 +1411,21 @@ void launch_vertex_prep(hipStream_t s, int32_t P, const double* vert, double* vR
 }
 void launch_linearize(hipStream_t s, bool jac, int32_t tiles, const DevTopology& t, const double* vert,
                       const double* vR, const double* lm, DevLoss loss, const int32_t* tile_begin, double* obs_rec,
-                      double* lm_acc, double* tile_cost, bool f32) {
+                      double* lm_acc, double* split_part, double* tile_cost, bool f32) {
   if (tiles <= 0) return;
   if (jac && f32)
-    k_linearize<true, true><<<tiles, kObsTile, 0, s>>>(t, vert, vR, lm, loss, tile_begin, obs_rec, lm_acc, tile_cost);
+    k_linearize<true, true><<<tiles, kObsTile, 0, s>>>(t, vert, vR, lm, loss, tile_begin, obs_rec, lm_acc,
+                                                       split_part, tile_cost);
   else if (jac)
-    k_linearize<true><<<tiles, kObsTile, 0, s>>>(t, vert, vR, lm, loss, tile_begin, obs_rec, lm_acc, tile_cost);
+    k_linearize<true><<<tiles, kObsTile, 0, s>>>(t, vert, vR, lm, loss, tile_begin, obs_rec, lm_acc, split_part,
+                                                 tile_cost);
   else
-    k_linearize<false><<<tiles, kObsTile, 0, s>>>(t, vert, vR, lm, loss, tile_begin, obs_rec, lm_acc, tile_cost);
+    k_linearize<false><<<tiles, kObsTile, 0, s>>>(t, vert, vR, lm, loss, tile_begin, obs_rec, lm_acc, split_part,
+                                                  tile_cost);
 }
-void launch_zero_split(hipStream_t s, int32_t n, const int32_t* ids, double* lm_acc) {
-  if (n > 0) k_zero_split<<<(n * kLmAcc + 255) / 256, 256, 0, s>>>(n, ids, lm_acc);
+void launch_split_sum(hipStream_t s, int32_t n, const int32_t* ids, const int32_t* split_tiles,
+                      const double* split_part, double* lm_acc) {
+  if (n > 0) k_split_sum<<<(n * kLmAcc + 255) / 256, 256, 0, s>>>(n, ids, split_tiles, split_part, lm_acc);
 }
 void launch_pose_rows(hipStream_t s, const DevTopology& t, const double* vert, const double* vR, const double* lm,
                       const double* obs_rec, const double* lm_acc, double* pose_grad, double* pose_diag,
@@ -1548,8 +1559,10 @@ void launch_reduce(hipStream_t s, int32_t n, int32_t stride, int32_t offset, boo
                    double* out) {
   k_reduce<<<1, 256, 0, s>>>(n, stride, offset, is_max ? 1 : 0, parts, out);
 }
-void launch_reduce_many(hipStream_t s, const ReduceBatch& batch) {
-  if (batch.count > 0 && batch.count <= kMaxReduceBatch) k_reduce_many<<<batch.count, 256, 0, s>>>(batch);
+bool launch_reduce_many(hipStream_t s, const ReduceBatch& batch) {
+  if (batch.count < 1 || batch.count > kMaxReduceBatch) return false;  // caller reports it; nothing launched
+  k_reduce_many<<<batch.count, 256, 0, s>>>(batch);
+  return true;
 }
 // ------------------------------------------------------------ calibration
 

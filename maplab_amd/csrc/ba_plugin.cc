@@ -30,6 +30,7 @@ class OptimizerPlugin {
                "Visual optimization over the selected missions (per default all).");
     addCommand({"optimize_visual_inertial", "optvi"}, [this] { return optimize(/*inertial=*/true); },
                "Visual-inertial optimization over the selected missions (per default all).");
+    name_list_ = names();
   }
 
   const char* id() const { return "map-optimization"; }
@@ -202,31 +203,31 @@ class OptimizerPlugin {
   std::vector<Command> commands_;
   std::map<std::string, std::string> flags_;
   std::string message_;
-  mutable std::vector<std::string> name_cache_;
+  std::vector<std::string> name_list_;  // built once: returned pointers stay valid while the plugin lives
   friend const char* ::mlba_plugin_command_name(const void*, int32_t);
+  friend int32_t(::mlba_plugin_num_commands)(const void*);
 };
 
 }  // namespace
 
 extern "C" {
 
-void* createConsolePlugin(mlba_console* console, void* /*plotter*/) { return new OptimizerPlugin(console); }
+void* mlba_plugin_create(mlba_console* console, void* /*plotter*/) { return new OptimizerPlugin(console); }
 
-void destroyConsolePlugin(void* plugin) { delete static_cast<OptimizerPlugin*>(plugin); }
+void mlba_plugin_destroy(void* plugin) { delete static_cast<OptimizerPlugin*>(plugin); }
 
 const char* mlba_plugin_id(const void* plugin) {
   return plugin ? static_cast<const OptimizerPlugin*>(plugin)->id() : "";
 }
 
 int32_t mlba_plugin_num_commands(const void* plugin) {
-  return plugin ? (int32_t)static_cast<const OptimizerPlugin*>(plugin)->names().size() : 0;
+  return plugin ? (int32_t)static_cast<const OptimizerPlugin*>(plugin)->name_list_.size() : 0;
 }
 
 const char* mlba_plugin_command_name(const void* plugin, int32_t index) {
   if (!plugin) return "";
   const OptimizerPlugin* p = static_cast<const OptimizerPlugin*>(plugin);
-  p->name_cache_ = p->names();
-  return (index >= 0 && index < (int32_t)p->name_cache_.size()) ? p->name_cache_[index].c_str() : "";
+  return (index >= 0 && index < (int32_t)p->name_list_.size()) ? p->name_list_[index].c_str() : "";
 }
 
 int32_t mlba_plugin_process_command(void* plugin, const char* line) {

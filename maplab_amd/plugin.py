@@ -1,7 +1,8 @@
 """ctypes host of the map-optimization console plugin
 (libmaplab_ba_plugin.so, include/maplab_ba_plugin.h): what maplab's console
-does with a plugin -- dlopen, createConsolePlugin, run a command line on the
-selected map, destroyConsolePlugin."""
+does with a plugin -- dlopen, create the plugin (mlba_plugin_create, the C
+counterpart of createConsolePlugin), run a command line on the selected map,
+destroy it (mlba_plugin_destroy)."""
 from __future__ import annotations
 
 import ctypes
@@ -28,8 +29,8 @@ class Console(ctypes.Structure):
 
 
 PLUGIN_SYMBOLS = {
-    "createConsolePlugin": (ctypes.c_void_p, [ctypes.POINTER(Console), ctypes.c_void_p]),
-    "destroyConsolePlugin": (None, [ctypes.c_void_p]),
+    "mlba_plugin_create": (ctypes.c_void_p, [ctypes.POINTER(Console), ctypes.c_void_p]),
+    "mlba_plugin_destroy": (None, [ctypes.c_void_p]),
     "mlba_plugin_id": (ctypes.c_char_p, [ctypes.c_void_p]),
     "mlba_plugin_num_commands": (ctypes.c_int32, [ctypes.c_void_p]),
     "mlba_plugin_command_name": (ctypes.c_char_p, [ctypes.c_void_p, ctypes.c_int32]),
@@ -62,7 +63,7 @@ class MapConsole:
         self.map = None
         if vmap is not None:
             self.select(vmap)
-        self.plugin = self.lib.createConsolePlugin(ctypes.byref(self.console), None)
+        self.plugin = self.lib.mlba_plugin_create(ctypes.byref(self.console), None)
 
     def select(self, vmap: VisualMap) -> None:
         self.vmap = vmap.normalized()
@@ -102,7 +103,7 @@ class MapConsole:
 
     def close(self) -> None:
         if self.plugin:
-            self.lib.destroyConsolePlugin(self.plugin)
+            self.lib.mlba_plugin_destroy(self.plugin)
             self.plugin = None
 
     def __del__(self):

@@ -170,6 +170,28 @@ def test_landmarks_split_over_tiles_match_oracle(oracle):
     np.testing.assert_allclose(l, rl, atol=1e-8)
 
 
+def test_landmarks_over_three_tiles_repeat_bit_for_bit(oracle):
+    # 600 vertices around 6 landmarks: every landmark spans three 256-observation
+    # tiles, whose sums are added in tile order, so two solves are identical to
+    # the last bit (an order-dependent sum of three partials would not be).
+    m = tiny_map(np.random.default_rng(78), dist_type=1, n_vertices=600, n_landmarks=6)
+    assert np.bincount(m.obs_landmark).min() > 512
+    o = opts(max_num_iterations=4, pcg_relative_tolerance=1e-12)
+    print(f"landmarks over 3 tiles: N {m.num_observations}", flush=True)
+    runs = []
+    for _ in range(2):
+        with DeviceProblem(m) as dev:
+            sm, rows = dev.solve(o)
+            runs.append(([(r.cost, r.step_is_successful, r.trust_region_radius) for r in rows],
+                         *dev.download_state()))
+    assert runs[0][0] == runs[1][0]
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])
+    np.testing.assert_array_equal(runs[0][2], runs[1][2])
+    ref, ref_rows, rv, rl, _ = oracle.solve(m, o, linear_solver=oracle.SCHUR_DIRECT, threads=8)
+    np.testing.assert_allclose([c for c, _, _ in runs[0][0]], [r.cost for r in ref_rows], rtol=1e-9)
+    np.testing.assert_allclose(runs[0][2], rl, atol=1e-8)
+
+
 def test_workload1_final_cost_matches_oracle(oracle):
     # 10 keyframes on a 1 m circle, 2000 landmarks, radtan, Cauchy(1 px), 10 iterations.
     m = problem.workload_map(1, seed=1)
@@ -223,6 +245,30 @@ def test_outlier_rejection_driver_matches_oracle(oracle):
     np.testing.assert_allclose([r.cost for r in rows], [r.cost for r in ref_rows], rtol=1e-8)
     np.testing.assert_allclose([r.cost for r in rows][5:], [r.cost for r in ref_rows][5:], rtol=1e-9)
     np.testing.assert_allclose(l[active == 1], rl[active == 1], atol=1e-8)
+
+
+@pytest.mark.parametrize("dist_type", [1, 2])
+def test_outlier_rejection_reprojection_branch_matches_oracle(oracle, dist_type):
+    # use_reprojection_error: one keypoint planted 60 px off (above the 50 px
+    # threshold) is removed with its landmark, one 20 px off is kept.
+    rng = np.random.default_rng(7 + dist_type)
+    m = tiny_map(rng, dist_type=dist_type, n_vertices=5, n_landmarks=30)
+    by_lm = np.bincount(m.obs_landmark, minlength=m.num_landmarks)
+    cand = [l for l in np.where((m.landmark_constant == 0) & (by_lm >= 3))[0]]
+    far, near = cand[0], cand[1]
+    m.obs_keypoint[np.where(m.obs_landmark == far)[0][1]] += [60.0, 0.0]
+    m.obs_keypoint[np.where(m.obs_landmark == near)[0][1]] += [0.0, 20.0]
+    o = opts(max_num_iterations=6, reject_outliers_every_n_iters=3, use_reprojection_error=1,
+             max_reprojection_error_px=50.0, pcg_relative_tolerance=1e-12)
+    print(f"outlier rejection, reprojection branch: dist {dist_type} N {m.num_observations}", flush=True)
+    with DeviceProblem(m) as dev:
+        sm, rows = dev.solve(o, outlier_rejection=True)
+        active = dev.landmark_active()
+    ref, ref_rows, rv, rl, ract = oracle.solve(m, o, outlier_rejection=True, threads=1)
+    np.testing.assert_array_equal(active, ract)
+    assert active[far] == 0 and active[near] == 1
+    assert sm.num_removed_landmarks == ref.num_removed_landmarks
+    np.testing.assert_allclose([r.cost for r in rows], [r.cost for r in ref_rows], rtol=1e-9)
 
 
 def test_full_size_random_walk_properties():

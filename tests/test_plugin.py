@@ -14,13 +14,37 @@ from maplab_amd import plugin
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def test_exports_console_entry_points():
+def test_exports_plugin_entry_points():
     lib = ctypes.CDLL(plugin.PLUGIN_PATH)
     text = open(os.path.join(ROOT, "include", "maplab_ba_plugin.h")).read()
     declared = set(re.findall(r"^(?![/ *#])[^\n(]*?\b(\w+)\s*\(", text, re.M))
     assert declared == set(plugin.PLUGIN_SYMBOLS), declared ^ set(plugin.PLUGIN_SYMBOLS)
-    for name in ("createConsolePlugin", "destroyConsolePlugin"):
+    for name in ("mlba_plugin_create", "mlba_plugin_destroy"):
         assert hasattr(lib, name)
+
+
+def test_does_not_export_the_cpp_console_names():
+    # maplab's console dlsyms createConsolePlugin / destroyConsolePlugin and
+    # calls them with C++ types; the C library must not answer to those names
+    # (a shim against maplab's headers exports them and forwards).
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--defined-only", plugin.PLUGIN_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    names = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    assert "createConsolePlugin" not in names and "destroyConsolePlugin" not in names
+    assert {"mlba_plugin_create", "mlba_plugin_destroy"} <= names
+
+
+def test_command_names_stay_valid():
+    # A returned name stays valid after later calls (same library, raw pointers).
+    c = plugin.MapConsole()
+    f = ctypes.CDLL(plugin.PLUGIN_PATH).mlba_plugin_command_name
+    f.restype, f.argtypes = ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_int32]
+    first = f(c.plugin, 0)
+    later = [f(c.plugin, i) for i in range(4)]
+    assert ctypes.cast(first, ctypes.c_char_p).value == b"optimize_visual"
+    assert first == later[0]
+    c.close()
 
 
 def test_plugin_id_and_commands():
