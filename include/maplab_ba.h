@@ -275,6 +275,25 @@ int32_t mlba_get_landmark_active(const mlba_handle* handle, uint8_t* mask);
 int32_t mlba_evaluate(mlba_handle* handle, int32_t flags, const mlba_options* options,
                       double* residuals, double* const* jacobians, double* cost);
 
+/*
+ * mlba_evaluate without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory during a call (tight serving
+ * loops, harnesses that forbid hipMalloc). Same arguments, outputs and
+ * results as mlba_evaluate (bit for bit). The residuals and Jacobian records
+ * are staged in one fixed array of MLBA_NOALLOC_SCRATCH_BYTES per device,
+ * held from the first call until the process ends; a call that would need
+ * more (2N doubles for residuals plus, when any Jacobian is requested, N
+ * records of 2 x (sum of the block sizes incl. the xi column) doubles: 84
+ * tangent, 94 ambient) fails with MLBA_EINVAL and its reason in
+ * mlba_last_error. Calls on one device share the array: the library
+ * serializes them with a per-device lock held until the call's stream work
+ * has finished, so overlapping calls from several threads wait for each
+ * other. The handle's device is made current for the call and restored.
+ */
+#define MLBA_NOALLOC_SCRATCH_BYTES (64u << 20)
+int32_t mlba_evaluate_noalloc(mlba_handle* handle, int32_t flags, const mlba_options* options,
+                              double* residuals, double* const* jacobians, double* cost);
+
 /* ceres::Solve on the visual problem (map_optimization::solve, solver.cc:14-44). */
 int32_t mlba_solve(mlba_handle* handle, const mlba_options* options, mlba_summary* summary);
 

@@ -19,6 +19,7 @@
 #include <cstring>
 #include <new>
 #include <atomic>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -147,6 +148,7 @@ enum Slot { kCost = 0, kFixed, kMcc, kUU, kWW, kXV, kXL, kCandCost, kCandFixed, 
 struct mlba_handle {
   std::string err;
   hipStream_t stream = nullptr;
+  int device = 0;                       // device current at mlba_create (the stream's)
   int N = 0, L = 0, P = 0, C = 0, M = 0;
   // Host topology.
   std::vector<int32_t> perm;            // sorted -> caller observation index
@@ -1217,6 +1219,7 @@ int32_t mlba_create(const mlba_problem_desc* d, mlba_handle** out) {
     delete h;
     return code;
   };
+  if (hipGetDevice(&h->device) != hipSuccess) h->device = 0;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     h->err = "hipStreamCreate failed";
     return fail_out(MLBA_EDEVICE);
@@ -1432,6 +1435,97 @@ int32_t mlba_evaluate(mlba_handle* h, int32_t flags, const mlba_options* options
       off += n;
     }
   }
+  return MLBA_OK;
+}
+
+// ------------------------------------------------------------ evaluate without allocation
+
+// Staging of mlba_evaluate_noalloc: one array per device for the whole
+// process (residuals, then the Jacobian records of every observation).
+// Host code never uses its name as an address: the device address comes
+// from hipGetSymbolAddress, once per device, with that device current.
+static __device__ __align__(256) double scratch[MLBA_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu[kMaxDevices];   // one call at a time per device holds its array
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+int32_t mlba_evaluate_noalloc(mlba_handle* h, int32_t flags, const mlba_options* options, double* residuals,
+                              double* const* jacobians, double* cost) {
+  if (!h) return MLBA_EINVAL;
+  if (flags & ~(MLBA_EVAL_TANGENT | MLBA_EVAL_WITH_XI)) return set_err(h, MLBA_EINVAL, "unknown flags");
+  if (h->device < 0 || h->device >= kMaxDevices) return set_err(h, MLBA_EINVAL, "device index out of range");
+  const bool tangent = (flags & MLBA_EVAL_TANGENT) != 0;
+  const int nblocks = (flags & MLBA_EVAL_WITH_XI) ? 10 : 9;
+  static const int amb[10] = {3, 7, 7, 7, 7, 4, 3, 4, 4, 1};
+  static const int tng[10] = {3, 6, 6, 6, 6, 3, 3, 4, 4, 1};
+  const int* sz = tangent ? tng : amb;
+  int stride = 0;
+  uint32_t mask = 0;
+  for (int b = 0; b < 10; ++b) {
+    stride += 2 * sz[b];
+    if (b < nblocks && jacobians && jacobians[b]) mask |= 1u << b;
+  }
+  const size_t res_n = residuals ? 2 * (size_t)h->N : 0;
+  const size_t need = sizeof(double) * (res_n + (mask ? (size_t)stride * h->N : 0));
+  if (need > MLBA_NOALLOC_SCRATCH_BYTES)
+    return set_err(h, MLBA_EINVAL, "mlba_evaluate_noalloc: " + std::to_string(need) +
+                                       " bytes of staging exceed the fixed " +
+                                       std::to_string((size_t)MLBA_NOALLOC_SCRATCH_BYTES) + "-byte array");
+  std::lock_guard<std::mutex> lock(g_scratch_mu[h->device]);
+  int prev = 0;
+  HIP_OK(hipGetDevice(&prev));
+  if (prev != h->device) HIP_OK(hipSetDevice(h->device));
+  // On every return the stream's work on the array has finished before the
+  // lock is released, and the caller's device is current again.
+  struct Restore {
+    int dev, prev;
+    hipStream_t s;
+    ~Restore() {
+      (void)hipStreamSynchronize(s);
+      if (prev != dev) (void)hipSetDevice(prev);
+    }
+  } restore{h->device, prev, h->stream};
+  if (!g_scratch_ptr[h->device]) {
+    void* ptr = nullptr;
+    HIP_OK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[h->device] = static_cast<double*>(ptr);
+  }
+  double* res_d = g_scratch_ptr[h->device];
+  double* jac_d = res_d + res_n;
+  hipStream_t s = h->stream;
+  launch_vertex_prep(s, h->P, h->vert.p, h->vR.p);
+  launch_evaluate_full(s, h->topo, h->vert.p, h->vR.p, h->lm.p, h->baseframes.p, h->cam_q.p, tangent, h->perm_d.p,
+                       residuals ? res_d : nullptr, mask ? jac_d : nullptr, stride, mask);
+  if (cost) {
+    mlba_options defaults;
+    mlba_options_init(&defaults);
+    const mlba_options* o = options ? options : &defaults;
+    launch_linearize(s, false, h->tiles, h->topo, h->vert.p, h->vR.p, h->lm.p, make_loss(o), h->tile_begin.p,
+                     nullptr, nullptr, nullptr, h->tile_cost2.p);
+    launch_reduce(s, h->tiles, 2, 0, false, h->tile_cost2.p, h->scal.p + kCandCost);
+    launch_reduce(s, h->tiles, 2, 1, false, h->tile_cost2.p, h->scal.p + kCandFixed);
+  }
+  HIP_OK(hipGetLastError());
+  if (residuals && h->N)
+    HIP_OK(hipMemcpyAsync(residuals, res_d, 2 * sizeof(double) * h->N, hipMemcpyDeviceToHost, s));
+  if (mask && h->N) {
+    // Block b of every record straight into the caller's row-major array.
+    int off = 0;
+    for (int b = 0; b < nblocks; ++b) {
+      const size_t n = 2 * (size_t)sz[b];
+      if (jacobians[b])
+        HIP_OK(hipMemcpy2DAsync(jacobians[b], n * sizeof(double), jac_d + off, (size_t)stride * sizeof(double),
+                                n * sizeof(double), (size_t)h->N, hipMemcpyDeviceToHost, s));
+      off += (int)n;
+    }
+  }
+  if (cost) {
+    int rc = read_scalars(h);
+    if (rc) return rc;
+    *cost = h->scal_host[kCandCost] + h->scal_host[kCandFixed];
+  }
+  HIP_OK(hipStreamSynchronize(s));
   return MLBA_OK;
 }
 

@@ -74,10 +74,11 @@ class DeviceProblem:
         return good.astype(bool)
 
     def evaluate(self, tangent: bool = False, blocks=range(9), options: abi.Options | None = None,
-                 want_cost: bool = True, with_xi: bool = False):
+                 want_cost: bool = True, with_xi: bool = False, noalloc: bool = False):
         """Batched VisualReprojectionError::Evaluate: residuals (N, 2), a dict
         block -> (N, 2, size) Jacobians, and the robust cost. With with_xi,
-        block 9 (N, 2, 1) is the unified-projection xi column."""
+        block 9 (N, 2, 1) is the unified-projection xi column. With noalloc,
+        through mlba_evaluate_noalloc (fixed staging array, no allocation)."""
         n = self.map.num_observations
         sizes = (abi.TANGENT_BLOCK_SIZES if tangent else abi.AMBIENT_BLOCK_SIZES) + (1,)
         nb = 10 if with_xi else 9
@@ -87,8 +88,9 @@ class DeviceProblem:
         cost = ctypes.c_double(0.0)
         opt = options if options is not None else abi.default_options()
         flags = (abi.EVAL_TANGENT if tangent else abi.EVAL_AMBIENT) | (abi.EVAL_WITH_XI if with_xi else 0)
-        self._check(self.lib.mlba_evaluate(self.handle, flags, ctypes.byref(opt), res.ctypes.data, ptrs,
-                                           ctypes.byref(cost) if want_cost else None))
+        fn = self.lib.mlba_evaluate_noalloc if noalloc else self.lib.mlba_evaluate
+        self._check(fn(self.handle, flags, ctypes.byref(opt), res.ctypes.data, ptrs,
+                       ctypes.byref(cost) if want_cost else None))
         return res, jac, cost.value
 
     def solve(self, options: abi.Options, outlier_rejection: bool = False, capacity: int = 256):
