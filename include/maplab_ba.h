@@ -174,6 +174,8 @@ typedef struct mlba_options {
                                              LM scalars stay FP64, and S remains the Schur complement
                                              of J'J of that one rounded Jacobian (positive
                                              semi-definite). 0 (default): FP64 Jacobians */
+  int32_t pcg_preconditioner;             /* MLBA_PRECOND_* of maplab_ba_learned.h: 0 deflated
+                                             block-Jacobi (default), 1 block-Jacobi, 2 learned */
 } mlba_options;
 
 /* One row of the Ceres iteration log (ceres::IterationSummary subset). */

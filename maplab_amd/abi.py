@@ -66,7 +66,7 @@ class Options(ctypes.Structure):
         ("pcg_relative_tolerance", c_f64), ("reject_outliers_every_n_iters", c_i32),
         ("use_reprojection_error", c_i32), ("max_reprojection_error_px", c_f64),
         ("obs_tile", c_i32), ("schur_row_chunk", c_i32), ("pcg_coarse_nodes", c_i32), ("pcg_ritz_vectors", c_i32),
-        ("jacobian_fp32", c_i32),
+        ("jacobian_fp32", c_i32), ("pcg_preconditioner", c_i32),
     ]
 
 
@@ -232,6 +232,35 @@ INERTIAL_SYMBOLS = {
     "mlba_inertial_last_error": (ctypes.c_char_p, []),
 }
 
+# Every symbol include/maplab_ba_learned.h declares (exported by the same library).
+PRECOND_DEFLATED, PRECOND_BLOCK_JACOBI, PRECOND_LEARNED = 0, 1, 2
+GNN_NODE_FEATURES, GNN_EDGE_FEATURES, GNN_NORM_FLOATS = 9, 3, 24
+
+
+class PreconditionerStats(ctypes.Structure):
+    _fields_ = [("solves", ctypes.c_int64), ("iterations", ctypes.c_int64), ("fallbacks", ctypes.c_int64),
+                ("fallback_iterations", ctypes.c_int64)]
+
+
+class GnnDataset(ctypes.Structure):
+    _fields_ = [("nodes", c_i32), ("slots", c_i32), ("row_ptr", c_vp), ("col", c_vp), ("rev", c_vp),
+                ("node_features", c_vp), ("edge_features", c_vp), ("qc", c_vp)]
+
+
+LEARNED_SYMBOLS = {
+    "mlba_gnn_num_params": (c_i32, []),
+    "mlba_set_preconditioner_model": (c_i32, [c_vp, c_vp, c_vp]),
+    "mlba_get_preconditioner_stats": (c_i32, [c_vp, ctypes.POINTER(PreconditionerStats)]),
+    "mlba_reset_preconditioner_stats": (c_i32, [c_vp]),
+    "mlba_rcs_size": (c_i32, [c_vp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32)]),
+    "mlba_rcs_records": (c_i32, [c_vp, ctypes.POINTER(Options), c_f64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "mlba_rcs_pcg_iterations": (c_i32, [c_vp, ctypes.POINTER(Options), c_f64, c_i32, c_f64, c_i32,
+                                        ctypes.POINTER(c_i32)]),
+    "mlba_gnn_evaluate": (c_i32, [ctypes.POINTER(GnnDataset), c_vp, c_vp, c_vp, c_vp]),
+    "mlba_gnn_train": (c_i32, [ctypes.POINTER(GnnDataset), c_i32, ctypes.c_float, c_vp, c_vp]),
+    "mlba_gnn_last_error": (ctypes.c_char_p, []),
+}
+
 SYNTH_SYMBOLS = {
     "mlba_synth_config_init": (c_i32, [ctypes.POINTER(SynthConfig), c_i32]),
     "mlba_synth_generate": (c_i32, [ctypes.POINTER(SynthConfig), ctypes.POINTER(c_vp)]),
@@ -253,7 +282,7 @@ def load_library(path: str = LIB_PATH):
             raise OSError(f"{path} is missing: build it with `make -C maplab_amd/csrc` "
                           "(the visual BA path has no CPU fallback)")
         lib = ctypes.CDLL(path)
-        for name, (res, args) in {**LIB_SYMBOLS, **OPTIMIZER_SYMBOLS, **INERTIAL_SYMBOLS}.items():
+        for name, (res, args) in {**LIB_SYMBOLS, **OPTIMIZER_SYMBOLS, **INERTIAL_SYMBOLS, **LEARNED_SYMBOLS}.items():
             _declare(lib, name, res, args)
         _lib = lib
     return _lib

@@ -26,7 +26,9 @@
 
 #include "../../include/maplab_ba.h"
 #include "ba_pcg.h"
+#include "ba_learned.h"
 #include "ba_ritz.h"
+#include "../../include/maplab_ba_learned.h"
 #include "ba_device.h"
 #include "ba_math.h"
 
@@ -166,6 +168,20 @@ struct mlba_handle {
   DBuf<int32_t> obs_vertex, obs_meta, obs_landmark, perm_d;
   DBuf<int32_t> lm_begin_d, lm_anchor_d, split_ids, split_tiles;  // split landmarks: id; first tile, tile count
   DBuf<double> split_part;                                         // per-tile sums of split landmarks
+  // Learned preconditioner (ba_learned.h): per block position of S its row,
+  // transposed position and shared observation pairs; set-up, network and
+  // PCG buffers (allocated on first use).
+  DBuf<int32_t> lp_pos_row, lp_tpos, lp_shared;
+  int32_t lp_max_lower_deg = 0;
+  DBuf<double> lp_linv, lp_t, lp_u, lp_part;
+  DBuf<float> lp_shat, lp_snorm, lp_kmat, lp_t2, lp_trace, lp_nf, lp_ef, lp_theta;
+  DBuf<char> lp_state;
+  void* lp_state_host = nullptr;
+  DBuf<float> lp_params, lp_norm;       // network parameters and feature (mean, std)
+  bool lp_model = false;
+  GnnWorkspace* lp_ws = nullptr;
+  int32_t lp_ws_nodes = -1, lp_ws_slots = -1;
+  mlba_preconditioner_stats lp_stats{};
   DBuf<uint32_t> lm_flags;
   DBuf<double> lm, lm_cand, lm_acc, lm_scale, lm_z, lm_u;
   DBuf<double> vert, vert_cand, vR, vR_cand;
@@ -235,6 +251,8 @@ struct mlba_handle {
     if (ritz_ev_ab) (void)hipEventDestroy(ritz_ev_ab);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     if (pcg_host) (void)hipHostFree(pcg_host);
+    if (lp_state_host) (void)hipHostFree(lp_state_host);
+    if (lp_ws) gnn_workspace_destroy(lp_ws);
     if (scal_host) (void)hipHostFree(scal_host);
     if (scal_ev) (void)hipEventDestroy(scal_ev);
     if (stream) (void)hipStreamDestroy(stream);
@@ -493,6 +511,20 @@ int build_topology(mlba_handle* h) {
       item_ptr.push_back((int32_t)items.size());
     }
   }
+  // Learned preconditioner structure: row and transposed position of every
+  // block of S, and the landmark pairs summed into it (shared observations).
+  std::vector<int32_t> lp_pos_row(h->nnzb), lp_tpos(h->nnzb), lp_shared(h->nnzb, 0);
+  h->lp_max_lower_deg = 0;
+  for (int i = 0; i < Pvar; ++i) {
+    for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) lp_pos_row[k] = i;
+    h->lp_max_lower_deg = std::max(h->lp_max_lower_deg, diag_pos[i] - row_ptr[i]);
+  }
+  for (int u = 0; u < h->n_upper; ++u) {
+    const int4 b = ublk[u];
+    lp_tpos[b.x] = b.y;
+    lp_tpos[b.y] = b.x;
+    lp_shared[b.x] = lp_shared[b.y] = pair_ptr[u + 1] - pair_ptr[u];
+  }
   h->n_items = (int32_t)items.size();
   h->h_item_rc.resize(items.size());
   for (size_t k = 0; k < items.size(); ++k) h->h_item_rc[k] = make_int2(ublk[items[k].x].z, ublk[items[k].x].w);
@@ -508,6 +540,9 @@ int build_topology(mlba_handle* h) {
   if ((rc = upload(h, h->tile_begin, tile_begin.data(), tile_begin.size()))) return rc;
   if ((rc = upload(h, h->split_ids, split.data(), split.size()))) return rc;
   if ((rc = upload(h, h->split_tiles, split_tiles.data(), split_tiles.size()))) return rc;
+  if ((rc = upload(h, h->lp_pos_row, lp_pos_row.data(), lp_pos_row.size()))) return rc;
+  if ((rc = upload(h, h->lp_tpos, lp_tpos.data(), lp_tpos.size()))) return rc;
+  if ((rc = upload(h, h->lp_shared, lp_shared.data(), lp_shared.size()))) return rc;
   if ((rc = upload(h, h->inc_ptr, inc_ptr.data(), inc_ptr.size()))) return rc;
   if ((rc = upload(h, h->inc, inc.data(), inc.size()))) return rc;
   if ((rc = upload(h, h->row_ptr, row_ptr.data(), row_ptr.size()))) return rc;
@@ -896,6 +931,119 @@ void push_iteration(mlba_summary* sm, const mlba_iteration_summary& it) {
 }
 
 // One ceres::Solve (TrustRegionMinimizer + LevenbergMarquardtStrategy).
+// ------------------------------------------------------------ learned preconditioner
+
+LpSystem lp_system(const mlba_handle* h) {
+  LpSystem sys{};
+  sys.Pvar = h->Pvar;
+  sys.nnzb = h->nnzb;
+  sys.row_ptr = h->row_ptr.p;
+  sys.col_var = h->col_var.p;
+  sys.diag_pos = h->diag_pos.p;
+  sys.pos_row = h->lp_pos_row.p;
+  sys.tpos = h->lp_tpos.p;
+  sys.S = h->S.p;
+  return sys;
+}
+
+// Buffers of the preconditioner (network ones with `network`, T2 with `t2`).
+int lp_ensure(mlba_handle* h, bool network, bool t2) {
+  const size_t P = std::max(h->Pvar, 1), Z = std::max(h->nnzb, 1);
+  bool ok = h->lp_linv.alloc(36 * P) && h->lp_t.alloc(6 * P) && h->lp_u.alloc(6 * P) && h->lp_part.alloc(2 * P) &&
+            h->lp_shat.alloc(36 * Z) && h->lp_snorm.alloc(Z) && h->lp_kmat.alloc(36 * Z) &&
+            h->lp_state.alloc(lp_pcg_state_bytes());
+  if (network) ok = ok && h->lp_nf.alloc(kLpNodeFeatures * P) && h->lp_ef.alloc(kLpEdgeFeatures * Z) &&
+                    h->lp_theta.alloc(kLpTerms * P);
+  if (t2) ok = ok && h->lp_t2.alloc(36 * Z) && h->lp_trace.alloc(4 * P);
+  if (!ok) return set_err(h, MLBA_ENOMEM, "device allocation failed");
+  if (!h->lp_state_host && hipHostMalloc(&h->lp_state_host, lp_pcg_state_bytes(), hipHostMallocDefault) != hipSuccess) {
+    h->lp_state_host = nullptr;
+    return set_err(h, MLBA_ENOMEM, "pinned allocation failed");
+  }
+  if (network && h->lp_model && (h->lp_ws_nodes != h->Pvar || h->lp_ws_slots != h->nnzb)) {
+    if (h->lp_ws) gnn_workspace_destroy(h->lp_ws);
+    h->lp_ws = gnn_workspace_create(h->Pvar, h->nnzb, false);
+    if (!h->lp_ws) { h->lp_ws_nodes = -1; return set_err(h, MLBA_ENOMEM, "network workspace allocation failed"); }
+    h->lp_ws_nodes = h->Pvar;
+    h->lp_ws_slots = h->nnzb;
+  }
+  return MLBA_OK;
+}
+
+LpPcgBuffers lp_buffers(mlba_handle* h, bool use_k) {
+  LpPcgBuffers B{};
+  B.sys = lp_system(h);
+  B.linv = h->lp_linv.p;
+  B.kmat = use_k ? h->lp_kmat.p : nullptr;
+  B.b = h->rhs.p;
+  B.x = h->px.p; B.r = h->pr.p; B.t = h->lp_t.p; B.u = h->lp_u.p; B.z = h->pz.p;
+  B.p0 = h->pp.p; B.p1 = h->pp2.p; B.q = h->pq.p; B.part = h->lp_part.p;
+  B.state_dev = h->lp_state.p;
+  B.state_host = h->lp_state_host;
+  return B;
+}
+
+GnnGraph lp_graph(const mlba_handle* h) {
+  GnnGraph g{};
+  g.nodes = h->Pvar;
+  g.slots = h->nnzb;
+  g.row_ptr = h->row_ptr.p;
+  g.col = h->col_var.p;
+  g.rev = h->lp_tpos.p;
+  g.node_feat = h->lp_nf.p;
+  g.edge_feat = h->lp_ef.p;
+  return g;
+}
+
+// Set-up of the preconditioner of the current S (factors into the step's
+// failure counter). With `learned`: normalized blocks, features, the
+// network's theta and K; *use_k = false when a row does not fit the
+// neighbourhood kernel (block-Jacobi is used then).
+int lp_setup(mlba_handle* h, bool learned, bool* use_k) {
+  hipStream_t s = h->stream;
+  const LpSystem sys = lp_system(h);
+  launch_lp_factor(s, sys, h->lp_linv.p, h->fail.p);
+  *use_k = false;
+  if (learned) {
+    launch_lp_normalize(s, sys, h->lp_linv.p, h->lp_shat.p, h->lp_snorm.p);
+    launch_lp_features(s, sys, h->pose_jtj.p, h->pose_scale.p, h->lp_snorm.p, h->lp_shared.p, h->lp_norm.p,
+                       h->lp_nf.p, h->lp_ef.p);
+    if (!gnn_forward(s, h->lp_ws, lp_graph(h), h->lp_params.p, h->lp_theta.p))
+      return set_err(h, MLBA_EDEVICE, "network launch failed");
+    *use_k = launch_lp_terms(s, sys, h->lp_max_lower_deg, kLpAssemble, h->lp_shat.p, h->lp_theta.p, h->lp_kmat.p,
+                             nullptr, nullptr);
+  }
+  HIP_OK(hipGetLastError());
+  return MLBA_OK;
+}
+
+// PCG on the current S with the learned preconditioner (block-Jacobi when not
+// `learned`), the block-Jacobi repeat when it does not converge.
+int lp_solve(mlba_handle* h, const mlba_options* o, bool learned, int* iters) {
+  int rc = lp_ensure(h, learned, false);
+  if (rc) return rc;
+  bool use_k = false;
+  if ((rc = lp_setup(h, learned, &use_k))) return rc;
+  int32_t conv = 0;
+  int it = launch_lp_pcg(h->stream, lp_buffers(h, use_k), o->pcg_relative_tolerance, o->pcg_max_iterations,
+                         kPcgPoll, &conv);
+  if (it < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
+  if (use_k) {
+    h->lp_stats.solves++;
+    h->lp_stats.iterations += it;
+    if (!conv) {
+      const int it2 = launch_lp_pcg(h->stream, lp_buffers(h, false), o->pcg_relative_tolerance,
+                                    o->pcg_max_iterations, kPcgPoll, &conv);
+      if (it2 < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
+      h->lp_stats.fallbacks++;
+      h->lp_stats.fallback_iterations += it2;
+      it += it2;
+    }
+  }
+  *iters = it;
+  return MLBA_OK;
+}
+
 int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_summary* sm, double* last_radius,
              int* termination) {
   hipStream_t s = h->stream;
@@ -961,7 +1109,10 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
       launch_block_inverse(s, h->Pvar, h->diag_pos.p, h->S.p, h->Minv.p, h->fail.p);
       HIP_OK(hipGetLastError());
       int pcg_iters = 0;
-      if (h->Pvar > 0) {
+      if (h->Pvar > 0 && o->pcg_preconditioner != MLBA_PRECOND_DEFLATED) {
+        KernelTimer kt(h, MLBA_KERNEL_PCG);
+        if ((rc = lp_solve(h, o, o->pcg_preconditioner == MLBA_PRECOND_LEARNED, &pcg_iters))) return rc;
+      } else if (h->Pvar > 0) {
         KernelTimer kt(h, MLBA_KERNEL_PCG);
         const int32_t nritz = ritz_count(o);
         if ((rc = ritz_merge(h))) return rc;
@@ -1111,6 +1262,10 @@ int validate_options(mlba_handle* h, const mlba_options* o) {
   if (o->loss_type < 0 || o->loss_type > 2) return set_err(h, MLBA_EINVAL, "unknown loss_type");
   if (o->linear_solver_type != MLBA_SOLVER_SCHUR_PCG) return set_err(h, MLBA_EINVAL, "unknown linear_solver_type");
   if (o->jacobian_fp32 != 0 && o->jacobian_fp32 != 1) return set_err(h, MLBA_EINVAL, "jacobian_fp32 must be 0 or 1");
+  if (o->pcg_preconditioner < MLBA_PRECOND_DEFLATED || o->pcg_preconditioner > MLBA_PRECOND_LEARNED)
+    return set_err(h, MLBA_EINVAL, "unknown pcg_preconditioner");
+  if (o->pcg_preconditioner == MLBA_PRECOND_LEARNED && !h->lp_model)
+    return set_err(h, MLBA_ESTATE, "learned preconditioner without a model (mlba_set_preconditioner_model)");
   if (!(o->initial_trust_region_radius > 0.0)) return set_err(h, MLBA_EINVAL, "initial_trust_region_radius <= 0");
   if (o->loss_type == MLBA_LOSS_CAUCHY && !(o->loss_scale_px > 0.0)) return set_err(h, MLBA_EINVAL, "loss_scale_px <= 0");
   return MLBA_OK;
@@ -1526,6 +1681,164 @@ int32_t mlba_evaluate_noalloc(mlba_handle* h, int32_t flags, const mlba_options*
     *cost = h->scal_host[kCandCost] + h->scal_host[kCandFixed];
   }
   HIP_OK(hipStreamSynchronize(s));
+  return MLBA_OK;
+}
+
+// ------------------------------------------------------------ learned preconditioner (C ABI)
+
+int32_t mlba_set_preconditioner_model(mlba_handle* h, const float* params, const float* norm) {
+  if (!h || !params || !norm) return MLBA_EINVAL;
+  for (int k = 0; k < kGnnNormFloats; ++k)
+    if (!std::isfinite(norm[k]) || (((k / kLpNodeFeatures == 1 && k < 2 * kLpNodeFeatures) ||
+                                     k >= 2 * kLpNodeFeatures + kLpEdgeFeatures) && !(norm[k] > 0.0f)))
+      return set_err(h, MLBA_EINVAL, "feature normalization: non-finite value or std <= 0");
+  for (int k = 0; k < kGnnParams; ++k)
+    if (!std::isfinite(params[k])) return set_err(h, MLBA_EINVAL, "non-finite network parameter");
+  int rc;
+  if ((rc = upload(h, h->lp_params, params, kGnnParams))) return rc;
+  if ((rc = upload(h, h->lp_norm, norm, kGnnNormFloats))) return rc;
+  HIP_OK(hipStreamSynchronize(h->stream));
+  h->lp_model = true;
+  return MLBA_OK;
+}
+
+int32_t mlba_get_preconditioner_stats(const mlba_handle* h, mlba_preconditioner_stats* st) {
+  if (!h || !st) return MLBA_EINVAL;
+  *st = h->lp_stats;
+  return MLBA_OK;
+}
+
+int32_t mlba_reset_preconditioner_stats(mlba_handle* h) {
+  if (!h) return MLBA_EINVAL;
+  h->lp_stats = mlba_preconditioner_stats{};
+  return MLBA_OK;
+}
+
+int32_t mlba_rcs_size(const mlba_handle* h, int32_t* rows, int32_t* slots) {
+  if (!h || !rows || !slots) return MLBA_EINVAL;
+  *rows = h->Pvar;
+  *slots = h->nnzb;
+  return MLBA_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// S and its right-hand side at the current state with damping diag / radius
+// (the linear system of an LM iteration), the state unchanged.
+int build_rcs(mlba_handle* h, const mlba_options* o, double radius) {
+  hipStream_t s = h->stream;
+  int rc;
+  if (!(radius > 0.0)) return set_err(h, MLBA_EINVAL, "radius <= 0");
+  h->f32_jac = o->jacobian_fp32 != 0;
+  if ((rc = linearize(h, o))) return rc;
+  launch_jacobi_scaling(s, h->Pvar, h->L, o->use_jacobi_scaling, h->pose_diag.p, h->lm_acc.p, h->pose_scale.p,
+                        h->lm_scale.p);
+  schur_factors(h);
+  HIP_OK(hipMemsetAsync(h->fail.p, 0, sizeof(int32_t), s));
+  const double inv_mu = 1.0 / radius;
+  launch_landmark_prep(s, h->topo, h->lm_acc.p, h->lm_scale.p, inv_mu, o->min_lm_diagonal, o->max_lm_diagonal,
+                       h->lm_z.p, h->fail.p);
+  launch_schur_incidences(s, h->topo, h->n_inc, h->inc_w.p, h->inc_r0.p, h->lm_a.p, h->lm_z.p, h->inc_q.p,
+                          h->inc_rhs.p);
+  launch_schur_blocks(s, h->topo, h->n_upper, h->ublk.p, h->n_items, h->items.p, h->item_ptr.p, h->pairs.p,
+                      h->inc_q.p, h->inc_w.p, h->inc_rhs.p, h->lm_a.p, h->pose_scale.p, h->pose_diag.p,
+                      h->pose_jtj.p, inv_mu, o->min_lm_diagonal, o->max_lm_diagonal, h->schur_partial.p, h->S.p,
+                      h->rhs.p, h->sched_waves, h->sched_xcd);
+  HIP_OK(hipGetLastError());
+  return MLBA_OK;
+}
+
+int read_fail(mlba_handle* h, int32_t* fail) {
+  HIP_OK(hipMemcpyAsync(fail, h->fail.p, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  return MLBA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t mlba_rcs_records(mlba_handle* h, const mlba_options* o, double radius, int32_t* row_ptr, int32_t* col,
+                         int32_t* rev, float* node_features, float* edge_features, float* qc) {
+  if (!h) return MLBA_EINVAL;
+  if (!row_ptr || !col || !rev || !node_features || !edge_features || !qc)
+    return set_err(h, MLBA_EINVAL, "NULL output");
+  int rc = validate_options(h, o);
+  if (rc) return rc;
+  if ((rc = build_rcs(h, o, radius))) return rc;
+  if ((rc = lp_ensure(h, true, true))) return rc;
+  hipStream_t s = h->stream;
+  const LpSystem sys = lp_system(h);
+  launch_lp_factor(s, sys, h->lp_linv.p, h->fail.p);
+  int32_t fail = 0;
+  if ((rc = read_fail(h, &fail))) return rc;
+  if (fail) return set_err(h, MLBA_ESTATE, "a landmark or vertex block of the system is not positive definite");
+  launch_lp_normalize(s, sys, h->lp_linv.p, h->lp_shat.p, h->lp_snorm.p);
+  std::vector<float> ident(kGnnNormFloats, 0.0f);
+  for (int k = kLpNodeFeatures; k < 2 * kLpNodeFeatures; ++k) ident[k] = 1.0f;
+  for (int k = 2 * kLpNodeFeatures + kLpEdgeFeatures; k < kGnnNormFloats; ++k) ident[k] = 1.0f;
+  DBuf<float> norm;
+  if ((rc = upload(h, norm, ident.data(), ident.size()))) return rc;
+  launch_lp_features(s, sys, h->pose_jtj.p, h->pose_scale.p, h->lp_snorm.p, h->lp_shared.p, norm.p, h->lp_nf.p,
+                     h->lp_ef.p);
+  if (!launch_lp_terms(s, sys, h->lp_max_lower_deg, kLpStoreT2, h->lp_shat.p, nullptr, nullptr, h->lp_t2.p,
+                       h->lp_trace.p) ||
+      !launch_lp_terms(s, sys, h->lp_max_lower_deg, kLpTraceT3, h->lp_shat.p, nullptr, nullptr, h->lp_t2.p,
+                       h->lp_trace.p))
+    return set_err(h, MLBA_EINVAL, "a row's lower neighbourhood does not fit the neighbourhood kernel");
+  HIP_OK(hipGetLastError());
+  const int P = h->Pvar, Z = h->nnzb;
+  std::vector<float> tr(4 * (size_t)P);
+  if (P) {
+    HIP_OK(hipMemcpyAsync(node_features, h->lp_nf.p, sizeof(float) * kLpNodeFeatures * P, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(tr.data(), h->lp_trace.p, sizeof(float) * 4 * P, hipMemcpyDeviceToHost, s));
+  }
+  if (Z) HIP_OK(hipMemcpyAsync(edge_features, h->lp_ef.p, sizeof(float) * kLpEdgeFeatures * Z, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  std::memcpy(row_ptr, h->h_row_ptr.data(), sizeof(int32_t) * (P + 1));
+  std::memcpy(col, h->h_col_var.data(), sizeof(int32_t) * Z);
+  HIP_OK(hipMemcpy(rev, h->lp_tpos.p, sizeof(int32_t) * Z, hipMemcpyDeviceToHost));
+  for (int i = 0; i < P; ++i) {
+    const bool has = h->h_row_ptr[i] < Z && h->h_col_var[h->h_row_ptr[i]] < i;
+    for (int k = 0; k < 4; ++k) qc[5 * (size_t)i + k] = has ? tr[4 * (size_t)i + k] : 0.0f;
+    qc[5 * (size_t)i + 4] = has ? 1.0f : 0.0f;
+  }
+  return MLBA_OK;
+}
+
+int32_t mlba_rcs_pcg_iterations(mlba_handle* h, const mlba_options* o, double radius, int32_t preconditioner,
+                                double tolerance, int32_t max_iterations, int32_t* iterations) {
+  if (!h || !iterations) return MLBA_EINVAL;
+  int rc = validate_options(h, o);
+  if (rc) return rc;
+  if (preconditioner < MLBA_PRECOND_DEFLATED || preconditioner > MLBA_PRECOND_LEARNED || !(tolerance > 0.0) ||
+      max_iterations <= 0)
+    return set_err(h, MLBA_EINVAL, "preconditioner, tolerance or iteration cap");
+  if (preconditioner == MLBA_PRECOND_LEARNED && !h->lp_model) return set_err(h, MLBA_ESTATE, "no preconditioner model");
+  if ((rc = build_rcs(h, o, radius))) return rc;
+  hipStream_t s = h->stream;
+  *iterations = -1;
+  if (h->Pvar == 0) { *iterations = 0; return MLBA_OK; }
+  if (preconditioner == MLBA_PRECOND_DEFLATED) {
+    launch_block_inverse(s, h->Pvar, h->diag_pos.p, h->S.p, h->Minv.p, h->fail.p);
+    if ((rc = ensure_coarse_basis(h, o->pcg_coarse_nodes))) return rc;
+    const int it = launch_pcg_deflated(s, pcg_buffers(h, 0, false), tolerance, max_iterations, kPcgPoll, kPcgPoll,
+                                       nullptr, 0);
+    if (it < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
+    *iterations = it < max_iterations ? it : -1;
+    return MLBA_OK;
+  }
+  const bool learned = preconditioner == MLBA_PRECOND_LEARNED;
+  if ((rc = lp_ensure(h, learned, false))) return rc;
+  bool use_k = false;
+  if ((rc = lp_setup(h, learned, &use_k))) return rc;
+  if (learned && !use_k) return set_err(h, MLBA_EINVAL, "a row's lower neighbourhood does not fit the neighbourhood kernel");
+  int32_t conv = 0;
+  const int it = launch_lp_pcg(s, lp_buffers(h, use_k), tolerance, max_iterations, kPcgPoll, &conv);
+  if (it < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
+  *iterations = conv ? it : -1;
   return MLBA_OK;
 }
 

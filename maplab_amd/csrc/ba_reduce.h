@@ -43,6 +43,22 @@ __device__ inline double wave_max(double v) {
   return wave_reduce(v, -__builtin_inf(), [](double a, double b) { return fmax(a, b); });
 }
 
+// FP32 wave sum over the same DPP sequence (fixed order); result in every lane.
+template <int kCtrl, int kRowMask = 0xF>
+__device__ inline float dpp_move_f(float v, float old) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), kCtrl, kRowMask, 0xF, false));
+}
+__device__ inline float wave_sum_f(float v) {
+  v += dpp_move_f<0xB1>(v, 0.0f);
+  v += dpp_move_f<0x4E>(v, 0.0f);
+  v += dpp_move_f<0x141>(v, 0.0f);
+  v += dpp_move_f<0x140>(v, 0.0f);
+  v += dpp_move_f<0x142, 0xA>(v, 0.0f);
+  v += dpp_move_f<0x143, 0xC>(v, 0.0f);
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 // Block-wide sum of one value for a 256-thread block; result valid in thread 0.
 __device__ inline double block_sum256(double v, double* lds4) {
   v = wave_sum(v);

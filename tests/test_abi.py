@@ -44,6 +44,15 @@ def test_library_exports_every_inertial_symbol():
         assert hasattr(lib, s), s
 
 
+def test_library_exports_every_learned_preconditioner_symbol():
+    lib = abi.load_library()
+    syms = declared("maplab_ba_learned.h")
+    assert syms == set(abi.LEARNED_SYMBOLS), syms ^ set(abi.LEARNED_SYMBOLS)
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert lib.mlba_gnn_num_params() == 9 * 64 + 64 + 3 * (64 * 64 + 64 + 3 + 1 + 128 * 64 + 64) + 64 * 2 + 2
+
+
 def test_synth_library_exports_every_declared_symbol():
     lib = abi.load_synth()
     syms = declared("maplab_ba_synth.h")
