@@ -195,6 +195,49 @@ def multimap(args):
     return 0
 
 
+def cpu_model():
+    """Host CPU model name (first 'model name' of /proc/cpuinfo)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def learned_model():
+    """The committed preconditioner model, or None when it is absent."""
+    from maplab_amd import learned
+    return learned.Model.load() if os.path.exists(learned.DEFAULT_MODEL) else None
+
+
+def learned_eval_2000(args, model):
+    """PCG iterations to |r|/|b| <= 1e-6 on the reduced camera systems of the
+    2000-keyframe map (1M landmarks) at three trust-region radii, with the
+    learned preconditioner, plain block-Jacobi and the deflated solver."""
+    from maplab_amd import abi, learned, problem
+    from maplab_amd.solver import DeviceProblem
+    t0 = time.perf_counter()
+    m = problem.workload_map(3, seed=args.seed)
+    print(f"bench: 2000-keyframe map for the preconditioner comparison ({m.num_observations} observations, "
+          f"{time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
+    o = abi.default_options()
+    rows = {}
+    with DeviceProblem(m) as dev:
+        model.install(dev)
+        for radius in (1e4, 1e6, 1e8):
+            rows[f"{radius:g}"] = {
+                name: learned.rcs_pcg_iterations(dev, o, radius, pc, 1e-6, 20000)
+                for name, pc in (("learned", abi.PRECOND_LEARNED), ("block_jacobi", abi.PRECOND_BLOCK_JACOBI),
+                                 ("deflated", abi.PRECOND_DEFLATED))}
+    tot = {k: sum(r[k] for r in rows.values()) for k in ("learned", "block_jacobi", "deflated")}
+    return {"workload": f"random walk, 2000 keyframes, {m.num_landmarks} landmarks, {m.num_observations} observations",
+            "tolerance": 1e-6, "per_radius": rows, "total": tot,
+            "learned_vs_block_jacobi": round(tot["block_jacobi"] / max(1, tot["learned"]), 2),
+            "learned_vs_deflated": round(tot["deflated"] / max(1, tot["learned"]), 2)}
+
+
 def cpu_baseline(vmap, opts, iters, threads):
     from oracle import binding as oracle
     import ctypes
@@ -219,7 +262,10 @@ def main(argv=None):
     ap.add_argument("--equidistant", action="store_true", help="equidistant distortion variant")
     ap.add_argument("--cpu-iters", type=int, default=12,
                     help="LM iterations of the CPU oracle sample (about 1 s each on 16 host threads)")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, nproc)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0: the reference's policy max(1, hw/2) with hw = min(16, nproc), and hw beside it")
+    ap.add_argument("--no-learned-2000", action="store_true",
+                    help="skip the preconditioner comparison on the 2000-keyframe map")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--repeats", type=int, default=20, help="launches per kernel in mlba_time_kernels")
     ap.add_argument("--multimap", type=int, default=0,
@@ -302,6 +348,40 @@ def main(argv=None):
         lo.pcg_coarse_nodes, lo.pcg_ritz_vectors = nodes, ritz
         lsm, _ = dev.solve(lo)
         loose[name] = int(lsm.total_linear_solver_iterations)
+    # The learned preconditioner on the same LM iterations at 1e-6 (timed), its
+    # LM log against block-Jacobi's at the same accuracy.
+    model = learned_model()
+    learned_lm = None
+    if model is not None:
+        from maplab_amd import learned
+        dev.upload_state(init_v, init_l)
+        model.install(dev)
+        dev.lib.mlba_reset_preconditioner_stats(dev.handle)
+        ro = abi.default_options()
+        ctypes.pointer(ro)[0] = opts
+        ro.pcg_relative_tolerance = 1e-6
+        ro.pcg_preconditioner = abi.PRECOND_LEARNED
+        t0 = time.perf_counter()
+        rsm, rrows = dev.solve(ro)
+        r_wall = time.perf_counter() - t0
+        st = learned.stats(dev)
+        dev.upload_state(init_v, init_l)
+        bo = abi.default_options()
+        ctypes.pointer(bo)[0] = opts
+        bo.pcg_relative_tolerance = 1e-6
+        bo.pcg_preconditioner = abi.PRECOND_BLOCK_JACOBI
+        bsm, brows = dev.solve(bo)
+        loose["learned"] = int(rsm.total_linear_solver_iterations)
+        learned_lm = {
+            "pcg_iterations": int(rsm.total_linear_solver_iterations),
+            "block_jacobi_pcg_iterations": int(bsm.total_linear_solver_iterations),
+            "ratio": round(bsm.total_linear_solver_iterations / max(1, rsm.total_linear_solver_iterations), 2),
+            "lm_iterations_per_s": round((rsm.num_iterations - 1) / r_wall, 3),
+            "same_lm_log_as_block_jacobi": [r.step_is_successful for r in rrows] ==
+                                           [r.step_is_successful for r in brows],
+            "final_cost_rel_to_block_jacobi": abs(rsm.final_cost - bsm.final_cost) / abs(bsm.final_cost),
+            "solves": int(st.solves), "block_jacobi_fallbacks": int(st.fallbacks), "tolerance": 1e-6,
+            "model": model.meta.get("generator")}
     # The same LM iterations with FP32 Jacobian storage (timed the same way).
     dev.upload_state(init_v, init_l)
     fo = abi.default_options()
@@ -332,6 +412,7 @@ def main(argv=None):
         "steps": n_iters,
         "warmup": args.warmup,
         "ms_per_step": round(1e3 * wall / max(1, n_iters), 3),
+        "ms_per_accepted_step": round(1e3 * wall / max(1, sm.num_successful_steps), 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -357,6 +438,7 @@ def main(argv=None):
             "tolerance": opts.pcg_relative_tolerance},
         "pcg_iterations_vs_block_jacobi_at_1e-6": {
             **loose, "ratio": round(loose["block_jacobi"] / max(1, loose["deflated"]), 2), "tolerance": 1e-6},
+        "learned_preconditioner": learned_lm,
         "fp32_jacobians": {
             "lm_iterations_per_s": round(f_iters / f_wall, 3), "ms_per_step": round(1e3 * f_wall / max(1, f_iters), 3),
             "final_cost": fsm.final_cost,
@@ -374,12 +456,23 @@ def main(argv=None):
             "xcd_mapping": sched.xcd_mapping, "schedules_tried": sched.schedules_tried,
             "k_schur_items_default_ms": round(sched.default_ms, 4), "k_schur_items_tuned_ms": round(sched.tuned_ms, 4)}
 
+    if model is not None and not args.no_learned_2000:
+        result["learned_preconditioner_2000kf"] = learned_eval_2000(args, model)
+    copy_ms = ctypes.c_double(0.0)
+    if dev.lib.mlba_measure_copy(1 << 30, 20, ctypes.byref(copy_ms)) == 0 and copy_ms.value > 0:
+        result["hbm_copy_gbs_measured"] = round(2 * (1 << 30) / copy_ms.value / 1e6, 1)
+
     if not args.no_cpu_baseline:
         nproc = os.cpu_count() or 1
-        threads = args.cpu_threads or min(16, nproc)
+        hw = min(16, nproc)
+        policy = args.cpu_threads or max(1, hw // 2)   # solver-options.h:37-38: max(1, hw / 2)
         vm_cpu = vmap.copy().normalized()
         vm_cpu.vertices, vm_cpu.landmarks = init_v.copy(), init_l.copy()
-        csm, crows, cn, cwall, cloop = cpu_baseline(vm_cpu, opts, args.cpu_iters, threads)
+        csm, crows, cn, cwall, cloop = cpu_baseline(vm_cpu, opts, args.cpu_iters, policy)
+        all_rate = None
+        if not args.cpu_threads and hw != policy:
+            _, _, an, _, aloop = cpu_baseline(vm_cpu, opts, args.cpu_iters, hw)
+            all_rate = round(an / aloop, 4) if aloop > 0 else None
         # GPU on the same sample for the cost parity after the same iterations.
         dev.upload_state(init_v, init_l)
         opts.max_num_iterations = args.cpu_iters
@@ -388,11 +481,15 @@ def main(argv=None):
         result["cpu_baseline"] = {
             "value": round(cn / cloop, 4) if cloop > 0 else None,
             "unit": "LM iterations/s",
-            "cores": threads,
+            "cores": policy,
             "kind": "port",
-            "sample": (f"FP64 C++ oracle (Schur + dense Cholesky of the reduced camera system), "
-                       f"{cn} LM iterations of the same map from the same start, {threads} threads of "
-                       f"{nproc} host cores, {cloop:.1f}s LM loop"),
+            "sample": (f"FP64 C++ oracle (Schur + dense Cholesky of the reduced camera system, the direct "
+                       f"SPARSE_SCHUR-equivalent solve), {cn} LM iterations of the same map from the same start, "
+                       f"{policy} threads (maplab's max(1, hw/2) policy with hw = {hw} of {nproc} visible cores), "
+                       f"{cloop:.1f}s LM loop; CPU: {cpu_model()}"),
+            "value_at_hw_threads": all_rate,
+            "hw_threads": hw,
+            "cpu_model": cpu_model(),
         }
         result["cost_rel_err_vs_oracle"] = {"iterations": cn, "gpu": gsm.final_cost, "oracle": csm.final_cost,
                                             "rel_err": rel}

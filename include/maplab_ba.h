@@ -403,6 +403,16 @@ int32_t mlba_time_kernels(mlba_handle* handle, const mlba_options* options, int3
                           mlba_kernel_times* times);
 
 /*
+ * Measurement hook (no counterpart in the reference): a plain device copy of
+ * `bytes` bytes (16-byte vector loads and stores, grid-stride), launched
+ * `repeats` times back to back on its own stream and timed with HIP events
+ * around the batch; ms_per_copy receives the average. The reference point of
+ * the HBM bandwidth and of the rocprofv3 FETCH_SIZE / WRITE_SIZE counters
+ * (a copy moves 2 x bytes). Allocates and frees its two buffers.
+ */
+int32_t mlba_measure_copy(int64_t bytes, int32_t repeats, double* ms_per_copy);
+
+/*
  * Launch-schedule autotuner (no counterpart in the reference: Ceres has no
  * device schedules). The reduced-camera-system block kernel sums fixed
  * incidence-pair ranges (work items) into fixed partial slots; what can be

@@ -189,6 +189,7 @@ LIB_SYMBOLS = {
     "mlba_get_landmark_active": (c_i32, [c_vp, c_vp]),
     "mlba_evaluate": (c_i32, [c_vp, c_i32, ctypes.POINTER(Options), c_vp, c_vp, c_vp]),
     "mlba_evaluate_noalloc": (c_i32, [c_vp, c_i32, ctypes.POINTER(Options), c_vp, c_vp, c_vp]),
+    "mlba_measure_copy": (c_i32, [ctypes.c_int64, c_i32, ctypes.POINTER(c_f64)]),
     "mlba_solve": (c_i32, [c_vp, ctypes.POINTER(Options), ctypes.POINTER(Summary)]),
     "mlba_solve_with_outlier_rejection": (c_i32, [c_vp, ctypes.POINTER(Options), ctypes.POINTER(Summary)]),
     "mlba_solve_many": (c_i32, [ctypes.POINTER(c_vp), ctypes.POINTER(ctypes.POINTER(Options)),

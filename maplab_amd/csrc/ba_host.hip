@@ -84,6 +84,9 @@ void launch_candidate(hipStream_t s, const DevTopology& t, const double* vert, c
 void launch_gradient_norm(hipStream_t s, const DevTopology& t, const double* vert, const double* pose_grad,
                           const double* lm_acc, double* part);
 bool launch_reduce_many(hipStream_t s, const ReduceBatch& batch);
+void launch_stream_copy(hipStream_t s, int64_t words, const void* src, void* dst);
+void launch_find_outliers(hipStream_t s, const DevTopology& t, const double* vert, const double* vR, const double* lm,
+                          int32_t use_reproj, double max_reproj2, uint32_t* lm_flags, uint8_t* bad);
 void launch_reduce(hipStream_t s, int32_t n, int32_t stride, int32_t offset, bool is_max, const double* parts,
                    double* out);
 void launch_evaluate_full(hipStream_t s, const DevTopology& t, const double* vert, const double* vR, const double* lm,
@@ -183,6 +186,7 @@ struct mlba_handle {
   int32_t lp_ws_nodes = -1, lp_ws_slots = -1;
   mlba_preconditioner_stats lp_stats{};
   DBuf<uint32_t> lm_flags;
+  DBuf<uint8_t> lm_bad;                 // outlier flags of the last rejection pass
   DBuf<double> lm, lm_cand, lm_acc, lm_scale, lm_z, lm_u;
   DBuf<double> vert, vert_cand, vR, vR_cand;
   DBuf<int32_t> vertex_mission_d, vertex_var, var_vertex;
@@ -1859,67 +1863,34 @@ int32_t mlba_solve(mlba_handle* h, const mlba_options* o, mlba_summary* sm) {
   return MLBA_OK;
 }
 
-// outlier-rejection-solver.cc:28-103 on the host: p_C.z <= 0 for any
-// observation of an active landmark (optionally a pixel reprojection error
-// above the threshold) removes the landmark.
+// outlier-rejection-solver.cc:28-103 and :142-182 on the device: every active
+// landmark with an observation at p_C.z <= 0 (optionally a pixel
+// reprojection error above the threshold) is deactivated in the device
+// flags, which removes its observations from the cost, the landmark sums, S
+// and the step without rebuilding the topology; the host reads back which
+// landmarks were removed.
 static int reject_outliers(mlba_handle* h, const mlba_options* o, int* removed) {
-  std::vector<double> vert(7 * (size_t)h->P), lmv(3 * (size_t)h->L);
-  int rc = mlba_download_state(h, vert.data(), lmv.data());
-  if (rc) return rc;
-  std::vector<DevCamera> cams(h->C);
-  HIP_OK(hipMemcpy(cams.data(), h->cams.p, sizeof(DevCamera) * h->C, hipMemcpyDeviceToHost));
-  std::vector<double> mT(12 * (size_t)h->M);
-  HIP_OK(hipMemcpy(mT.data(), h->mission_T.p, sizeof(double) * mT.size(), hipMemcpyDeviceToHost));
-  std::vector<double> kp(2 * (size_t)h->N);
-  HIP_OK(hipMemcpy(kp.data(), h->obs_kp.p, sizeof(double) * kp.size(), hipMemcpyDeviceToHost));
-  std::vector<uint8_t> bad(h->L, 0);
-  const double max_sq = o->max_reprojection_error_px * o->max_reprojection_error_px;
-  for (int k = 0; k < h->N; ++k) {
-    const int l = h->s_landmark[k];
-    if (!h->lm_active[l] || bad[l]) continue;
-    const int v = h->s_vertex[k], a = h->lm_anchor[l];
-    const double* pB = &lmv[3 * l];
-    double pI[3];
-    if (h->s_type[k] == kTypeKeyframe) {
-      for (int i = 0; i < 3; ++i) pI[i] = pB[i];
-    } else {
-      double Ra[9], Rv[9], pM[3], dm[3];
-      rot_jpl(&vert[7 * a], Ra);
-      rot_jpl(&vert[7 * v], Rv);
-      mat3t_vec(Ra, pB, pM);
-      for (int i = 0; i < 3; ++i) pM[i] += vert[7 * a + 4 + i];
-      if (h->s_type[k] == kTypeGlobal) {
-        const double* TL = &mT[12 * h->vertex_mission[a]];
-        const double* TM = &mT[12 * h->vertex_mission[v]];
-        double pG[3], d[3];
-        mat3_vec(TL, pM, pG);
-        for (int i = 0; i < 3; ++i) d[i] = pG[i] + TL[9 + i] - TM[9 + i];
-        mat3t_vec(TM, d, pM);
-      }
-      for (int i = 0; i < 3; ++i) dm[i] = pM[i] - vert[7 * v + 4 + i];
-      mat3_vec(Rv, dm, pI);
-    }
-    const DevCamera& cam = cams[h->s_camera[k]];
-    double pC[3];
-    mat3_vec(cam.R_C_I, pI, pC);
-    for (int i = 0; i < 3; ++i) pC[i] += cam.p_C_I[i];
-    if (pC[2] <= 0.0) { bad[l] = 1; continue; }
-    if (o->use_reprojection_error) {
-      double px[2], jp[6];
-      const int st = project_camera(cam.model, cam.xi, cam.dist_type, cam.intr, cam.dist, cam.width, cam.height, pC, px, jp);
-      double e2 = 1.7976931348623157e308;
-      if (st == kVisible || st == kOutsideImage) {
-        const double du = px[0] - kp[2 * k], dv = px[1] - kp[2 * k + 1];
-        e2 = du * du + dv * dv;
-      }
-      if (e2 > max_sq) bad[l] = 1;
-    }
-  }
+  hipStream_t s = h->stream;
+  *removed = 0;
+  if (h->L == 0) return MLBA_OK;
+  if (!h->lm_bad.alloc(h->L)) return set_err(h, MLBA_ENOMEM, "device allocation failed");
+  launch_vertex_prep(s, h->P, h->vert.p, h->vR.p);
+  const double m = o->max_reprojection_error_px;
+  launch_find_outliers(s, h->topo, h->vert.p, h->vR.p, h->lm.p, o->use_reprojection_error ? 1 : 0, m * m,
+                       h->lm_flags.p, h->lm_bad.p);
+  HIP_OK(hipGetLastError());
+  std::vector<uint8_t> bad(h->L);
+  HIP_OK(hipMemcpyAsync(bad.data(), h->lm_bad.p, h->L, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
   int n = 0;
   for (int l = 0; l < h->L; ++l)
     if (bad[l]) { h->lm_active[l] = 0; ++n; }
   *removed = n;
-  if (n) return build_topology(h);
+  if (n) {
+    // The system changed: recycled deflation vectors start again.
+    h->ritz_nr = 0;
+    h->ritz_stage = 0;
+  }
   return MLBA_OK;
 }
 
@@ -2458,6 +2429,34 @@ int32_t mlba_autotune(mlba_handle* h, const mlba_options* o, int32_t repeats, ml
   out->xcd_mapping = h->sched_xcd;
   out->pcg_partials = h->sched_pcg;
   return MLBA_OK;
+}
+
+int32_t mlba_measure_copy(int64_t bytes, int32_t repeats, double* ms_per_copy) {
+  if (bytes < 16 || bytes % 16 != 0 || repeats <= 0 || !ms_per_copy) return MLBA_EINVAL;
+  void *a = nullptr, *b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int32_t rc = MLBA_EDEVICE;
+  if (hipMalloc(&a, bytes) == hipSuccess && hipMalloc(&b, bytes) == hipSuccess &&
+      hipMemset(a, 1, bytes) == hipSuccess && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess &&
+      hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+    launch_stream_copy(s, bytes / 16, a, b);  // warm-up
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < repeats; ++r) launch_stream_copy(s, bytes / 16, a, b);
+    (void)hipEventRecord(e1, s);
+    float ms = 0.f;
+    if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&
+        hipGetLastError() == hipSuccess) {
+      *ms_per_copy = (double)ms / repeats;
+      rc = MLBA_OK;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
 }
 
 int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats, mlba_kernel_times* t) {

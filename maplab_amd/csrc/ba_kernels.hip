@@ -1677,4 +1677,96 @@ void launch_evaluate_full(hipStream_t s, const DevTopology& t, const double* ver
                                                       jout, jstride, mask);
 }
 
+// ------------------------------------------------------------ outlier rejection
+
+// p_C of observation o at the current state: the frame chain of
+// eval_observation without the Jacobians.
+__device__ inline void obs_point_camera(const DevTopology& t, int o, const double* __restrict__ vert,
+                                        const double* __restrict__ vR, const double* __restrict__ lm,
+                                        double* pC) {
+  const int32_t meta = t.obs_meta[o];
+  const int32_t type = obs_type(meta);
+  const DevCamera& cam = t.cams[obs_camera(meta)];
+  const int32_t l = t.obs_landmark[o];
+  const int32_t v = t.obs_vertex[o];
+  const double* pB = lm + 3 * l;
+  double pI[3];
+  if (type == kTypeKeyframe) {
+    pI[0] = pB[0]; pI[1] = pB[1]; pI[2] = pB[2];
+  } else {
+    const int32_t a = t.lm_anchor[l];
+    double pM[3];
+    mat3t_vec(vR + 12 * a, pB, pM);
+    for (int i = 0; i < 3; ++i) pM[i] += vert[7 * a + 4 + i];
+    if (type == kTypeGlobal) {
+      const double* TL = t.mission_T + 12 * t.vertex_mission[a];
+      const double* TM = t.mission_T + 12 * t.vertex_mission[v];
+      double pG[3], d[3];
+      mat3_vec(TL, pM, pG);
+      for (int i = 0; i < 3; ++i) d[i] = pG[i] + TL[9 + i] - TM[9 + i];
+      mat3t_vec(TM, d, pM);
+    }
+    const double dm[3] = {pM[0] - vert[7 * v + 4], pM[1] - vert[7 * v + 5], pM[2] - vert[7 * v + 6]};
+    mat3_vec(vR + 12 * v, dm, pI);
+  }
+  mat3_vec(cam.R_C_I, pI, pC);
+  for (int i = 0; i < 3; ++i) pC[i] += cam.p_C_I[i];
+}
+
+// findOutlierLandmarks (outlier-rejection-solver.cc:28-103) over the active
+// landmarks, one wave per landmark: outlier when any observation has
+// p_C.z <= 0, or (use_reproj) a squared pixel error above max_reproj2, statuses
+// other than visible / outside the image counting as infinite
+// (landmark-quality-metrics.cc:57-73). An outlier is deactivated in place
+// (kLmActive cleared: its observations then contribute nothing to the cost,
+// the landmark sums, S or the step) and bad[l] = 1.
+__global__ __launch_bounds__(64) void k_find_outliers(DevTopology t, const double* __restrict__ vert,
+                                                      const double* __restrict__ vR, const double* __restrict__ lm,
+                                                      int32_t use_reproj, double max_reproj2,
+                                                      uint32_t* __restrict__ lm_flags, uint8_t* __restrict__ bad) {
+  const int l = blockIdx.x, lane = threadIdx.x;
+  const uint32_t flags = lm_flags[l];
+  int out = 0;
+  if (flags & kLmActive) {
+    for (int o = t.lm_begin[l] + lane; o < t.lm_begin[l + 1]; o += 64) {
+      double pC[3];
+      obs_point_camera(t, o, vert, vR, lm, pC);
+      if (pC[2] <= 0.0) { out = 1; continue; }
+      if (use_reproj) {
+        const DevCamera& cam = t.cams[obs_camera(t.obs_meta[o])];
+        double kp[2], jp[6];
+        const int st = project_camera(cam.model, cam.xi, cam.dist_type, cam.intr, cam.dist, cam.width, cam.height,
+                                      pC, kp, jp);
+        double e2 = 1.7976931348623157e308;
+        if (st == kVisible || st == kOutsideImage) {
+          const double du = kp[0] - t.obs_kp[2 * o], dv = kp[1] - t.obs_kp[2 * o + 1];
+          e2 = du * du + dv * dv;
+        }
+        if (e2 > max_reproj2) out = 1;
+      }
+    }
+  }
+  out = __any(out);
+  if (lane == 0) {
+    bad[l] = (uint8_t)out;
+    if (out) lm_flags[l] = flags & ~kLmActive;
+  }
+}
+void launch_find_outliers(hipStream_t s, const DevTopology& t, const double* vert, const double* vR, const double* lm,
+                          int32_t use_reproj, double max_reproj2, uint32_t* lm_flags, uint8_t* bad) {
+  if (t.L > 0) k_find_outliers<<<t.L, 64, 0, s>>>(t, vert, vR, lm, use_reproj, max_reproj2, lm_flags, bad);
+}
+
+// ------------------------------------------------------------ measurement
+
+// Grid-stride copy of n 16-byte words (the bandwidth reference).
+__global__ __launch_bounds__(256) void k_stream_copy(int64_t n, const int4* __restrict__ src, int4* __restrict__ dst) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+void launch_stream_copy(hipStream_t s, int64_t words, const void* src, void* dst) {
+  if (words > 0)
+    k_stream_copy<<<4096, 256, 0, s>>>(words, static_cast<const int4*>(src), static_cast<int4*>(dst));
+}
+
 }  // namespace mlba

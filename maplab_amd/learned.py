@@ -10,9 +10,9 @@ G S G' row by row on reduced camera systems the BA produces. No counterpart
 in the reference (maplab's Ceres ITERATIVE_SCHUR uses the fixed SCHUR_JACOBI
 preconditioner, solver-options.h:57-68).
 
-python -m maplab_amd.learned train [--maps N] [--records R] [--epochs E] [--out FILE]
-    generates records on maps of the circle workload generator with varied
-    seeds and sizes, trains, and writes the model file.
+python -m maplab_amd.learned train [--maps N] [--walk-maps W] [--epochs E] [--out FILE]
+    generates records on maps of the circle and random-walk workload
+    generators with varied seeds and sizes, trains, and writes the model file.
 """
 from __future__ import annotations
 
@@ -195,28 +195,37 @@ def stats(dev: DeviceProblem) -> abi.PreconditionerStats:
     return st
 
 
-def training_records(n_maps: int, records_per_map: int, seed: int = 1000, log=print):
-    """Records of reduced camera systems from maps of the circle workload
-    generator (10-120 keyframes, 500-8000 landmarks, varied seeds): at each
-    map's start and along its LM iterations, radius log-uniform in
-    [1e3, 1e9]."""
+def training_records(n_maps: int, records_per_map: int, seed: int = 1000, walk_maps: int = 0,
+                     walk_records: int = 8, log=print):
+    """Records of reduced camera systems the BA produces: n_maps maps of the
+    circle workload generator (10-120 keyframes, 500-8000 landmarks) and
+    walk_maps maps of the random-walk generator (30-300 keyframes, 150-400
+    landmarks per keyframe), varied seeds; at each map's start and along its
+    LM iterations (two iterations between every third record), radius
+    log-uniform in [1e3, 1e9]."""
     rng = np.random.default_rng(seed)
     out = []
     t0 = time.time()
-    for k in range(n_maps):
-        kf = int(rng.integers(10, 121))
-        lm = int(rng.integers(500, 8001))
-        m = problem.workload_map(1, seed=seed + k, num_keyframes=kf, num_landmarks=lm)
+    total = n_maps + walk_maps
+    for k in range(total):
+        if k < n_maps:
+            kf = int(rng.integers(10, 121))
+            m = problem.workload_map(1, seed=seed + k, num_keyframes=kf, num_landmarks=int(rng.integers(500, 8001)))
+            per = records_per_map
+        else:
+            kf = int(rng.integers(30, 301))
+            m = problem.workload_map(2, seed=seed + k, num_keyframes=kf, num_landmarks=kf * int(rng.integers(150, 401)))
+            per = walk_records
         o = abi.default_options()
         with DeviceProblem(m) as dev:
-            for r in range(records_per_map):
+            for r in range(per):
                 if r and r % 3 == 0:
                     step = abi.default_options()
                     step.max_num_iterations = 2
                     dev.solve(step)
                 out.append(rcs_records(dev, o, float(10 ** rng.uniform(3, 9))))
-        if log and (k + 1) % max(1, n_maps // 10) == 0:
-            log(f"records: {k + 1}/{n_maps} maps, {len(out)} systems, {time.time() - t0:.1f}s")
+        if log and (k + 1) % max(1, total // 10) == 0:
+            log(f"records: {k + 1}/{total} maps, {len(out)} systems, {time.time() - t0:.1f}s")
     return out
 
 
@@ -224,14 +233,17 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description="train the covisibility preconditioner network on the device")
     sub = ap.add_subparsers(dest="cmd", required=True)
     t = sub.add_parser("train")
-    t.add_argument("--maps", type=int, default=500)
-    t.add_argument("--records", type=int, default=10)
+    t.add_argument("--maps", type=int, default=300, help="circle-generator maps")
+    t.add_argument("--records", type=int, default=10, help="systems per circle map")
+    t.add_argument("--walk-maps", type=int, default=250, help="random-walk-generator maps")
+    t.add_argument("--walk-records", type=int, default=8, help="systems per random-walk map")
     t.add_argument("--epochs", type=int, default=400)
     t.add_argument("--lr", type=float, default=3e-3)
     t.add_argument("--seed", type=int, default=1000)
     t.add_argument("--out", default=DEFAULT_MODEL)
     a = ap.parse_args(argv)
-    recs = training_records(a.maps, a.records, a.seed, log=lambda s: print(s, flush=True))
+    recs = training_records(a.maps, a.records, a.seed, a.walk_maps, a.walk_records,
+                            log=lambda s: print(s, flush=True))
     norm = normalization(recs)
     data = Dataset(recs, norm)
     print(f"dataset: {len(recs)} systems, {data.nodes} rows, {data.slots} slots", flush=True)
@@ -242,7 +254,9 @@ def main(argv=None):
     Model(params, norm, {"systems": len(recs), "rows": int(data.nodes), "slots": int(data.slots),
                          "epochs": a.epochs, "learning_rate": a.lr, "seed": a.seed,
                          "loss_first": float(log[0]), "loss_last": float(log[-1]),
-                         "generator": "circle workload, 10-120 keyframes, 500-8000 landmarks"}).save(a.out)
+                         "generator": (f"{a.maps} circle maps (10-120 keyframes, 500-8000 landmarks) x {a.records} "
+                                       f"systems, {a.walk_maps} random-walk maps (30-300 keyframes) x "
+                                       f"{a.walk_records} systems")}).save(a.out)
     print(f"wrote {a.out}", flush=True)
 
 

@@ -1,0 +1,93 @@
+"""The two largest workloads on the device, through the C ABI, with
+size-independent checks (the FP64 oracle does not finish these sizes in
+seconds):
+
+* 2000 keyframes, 1M landmarks, ~12.6M observations (the mixed-precision
+  workload's map): accepted costs decrease, quaternions stay unit with
+  w >= 0, Evaluate's cost equals the solve's, the deflated, block-Jacobi and
+  learned-preconditioner solves follow the same LM log, and the FP32-Jacobian
+  solve stays close to the FP64 one.
+* 64 independent maps (50-800 keyframes, 10k-150k landmarks log-uniform,
+  radtan and equidistant, detection probability 0.5: ~60M observations)
+  solved concurrently, each bit for bit equal to its solve alone.
+"""
+import numpy as np
+import pytest
+
+from maplab_amd import abi, learned, problem
+from maplab_amd.solver import DeviceProblem
+
+pytestmark = pytest.mark.gpu
+
+
+def opts(**kw):
+    o = abi.default_options()
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def test_2000_keyframe_map_properties():
+    m = problem.workload_map(3, seed=1)
+    print(f"2000 keyframes: N {m.num_observations}", flush=True)
+    assert m.num_observations > 10_000_000
+    base = dict(max_num_iterations=3, pcg_max_iterations=3000)
+    with DeviceProblem(m) as dev:
+        _, _, c0 = dev.evaluate(blocks=(), options=opts(**base))
+        sm, rows = dev.solve(opts(**base))
+        _, _, c1 = dev.evaluate(blocks=(), options=opts(**base))
+        v, l = dev.download_state()
+        logs = {"deflated": [r.cost for r in rows]}
+        model = learned.Model.load() if learned.os.path.exists(learned.DEFAULT_MODEL) else None
+        for name, pc in (("block_jacobi", abi.PRECOND_BLOCK_JACOBI), ("learned", abi.PRECOND_LEARNED)):
+            if pc == abi.PRECOND_LEARNED:
+                if model is None:
+                    continue
+                model.install(dev)
+            dev.upload_state(m.vertices, m.landmarks)
+            _, r2 = dev.solve(opts(pcg_preconditioner=pc, **base))
+            logs[name] = [r.cost for r in r2]
+        dev.upload_state(m.vertices, m.landmarks)
+        fsm, _ = dev.solve(opts(jacobian_fp32=1, **base))
+    assert abs(c0 - sm.initial_cost) <= 1e-12 * c0
+    assert abs(c1 - sm.final_cost) <= 1e-12 * c1
+    accepted = [r.cost for r in rows if r.step_is_successful]
+    assert all(b <= a for a, b in zip(accepted, accepted[1:]))
+    assert sm.final_cost < 0.5 * sm.initial_cost
+    assert np.isfinite(v).all() and np.isfinite(l).all()
+    np.testing.assert_allclose(np.linalg.norm(v[:, :4], axis=1), 1.0, atol=1e-12)
+    assert (v[:, 3] >= 0).all()
+    for name, log in logs.items():
+        print(f"{name}: {log}", flush=True)
+        np.testing.assert_allclose(log, logs["deflated"], rtol=1e-8)
+    assert abs(fsm.final_cost - sm.final_cost) <= 1e-3 * sm.final_cost
+
+
+def test_64_maps_concurrently_equal_each_alone():
+    rng = np.random.default_rng(64)
+    maps = []
+    for i in range(64):
+        kf = int(round(np.exp(rng.uniform(np.log(50), np.log(800)))))
+        lm = int(round(np.exp(rng.uniform(np.log(10000), np.log(150000)))))
+        kw = {} if i % 2 == 0 else dict(distortion_type=2, distortion=(-0.01, 0.02, -0.005, 0.001))
+        maps.append(problem.workload_map(2, seed=100 + i, num_keyframes=kf, num_landmarks=lm,
+                                         detection_probability=0.5, **kw))
+    n_obs = sum(m.num_observations for m in maps)
+    print(f"64 maps: {n_obs} observations", flush=True)
+    assert n_obs > 40_000_000
+    o = opts(max_num_iterations=3)
+    devs = [DeviceProblem(m) for m in maps]
+    try:
+        together = DeviceProblem.solve_many(devs, o, threads=8)
+        states = [d.download_state() for d in devs]
+        for d, m, (sm, rows), (v, l) in zip(devs, maps, together, states):
+            d.upload_state(m.vertices, m.landmarks)
+            sm1, rows1 = d.solve(o)
+            v1, l1 = d.download_state()
+            assert [r.cost for r in rows] == [r.cost for r in rows1]
+            np.testing.assert_array_equal(v, v1)
+            np.testing.assert_array_equal(l, l1)
+            assert sm.final_cost < sm.initial_cost
+    finally:
+        for d in devs:
+            d.close()

@@ -161,3 +161,27 @@ def test_training_on_the_device_lowers_the_loss():
     assert np.isfinite(log).all() and log[-1] < log[0]
     _, loss, _ = learned.evaluate(data, params, want_grad=False)
     assert loss < log[0]
+
+
+def test_committed_model_halves_the_block_jacobi_iterations():
+    # The trained network (maplab_amd/models/covis_precond.json) against the
+    # method without it, plain block-Jacobi, on a 300-keyframe random walk
+    # (PCG to |r|/|b| <= 1e-6 at two trust-region radii), and the LM log of
+    # a solve with it against block-Jacobi's.
+    model = learned.Model.load()
+    m = walk(300, 60000, 11)
+    o = opts()
+    with DeviceProblem(m) as dev:
+        model.install(dev)
+        lj = [learned.rcs_pcg_iterations(dev, o, r, abi.PRECOND_LEARNED, 1e-6) for r in (1e6, 1e8)]
+        bj = [learned.rcs_pcg_iterations(dev, o, r, abi.PRECOND_BLOCK_JACOBI, 1e-6) for r in (1e6, 1e8)]
+        print(f"committed model: learned {lj} block-Jacobi {bj}", flush=True)
+        assert min(lj) > 0 and sum(lj) <= 0.6 * sum(bj)
+        logs = {}
+        for pc in (abi.PRECOND_BLOCK_JACOBI, abi.PRECOND_LEARNED):
+            dev.upload_state(m.vertices, m.landmarks)
+            _, rows = dev.solve(opts(max_num_iterations=5, pcg_relative_tolerance=1e-10, pcg_max_iterations=3000,
+                                     pcg_preconditioner=pc))
+            logs[pc] = [r.cost for r in rows]
+        np.testing.assert_allclose(logs[abi.PRECOND_LEARNED], logs[abi.PRECOND_BLOCK_JACOBI], rtol=1e-8)
+        assert learned.stats(dev).fallbacks == 0
