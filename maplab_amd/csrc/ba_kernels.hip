@@ -1759,14 +1759,23 @@ void launch_find_outliers(hipStream_t s, const DevTopology& t, const double* ver
 
 // ------------------------------------------------------------ measurement
 
-// Grid-stride copy of n 16-byte words (the bandwidth reference).
+// Copy of n 16-byte words (the bandwidth reference): each thread moves four
+// words of one 4 x 256-word chunk per step (four loads in flight), chunks
+// assigned grid-stride; the tail (n % 1024) word by word.
 __global__ __launch_bounds__(256) void k_stream_copy(int64_t n, const int4* __restrict__ src, int4* __restrict__ dst) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+  const int64_t chunks = n / 1024;
+  for (int64_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+    const int64_t b = c * 1024 + threadIdx.x;
+    const int4 v0 = src[b], v1 = src[b + 256], v2 = src[b + 512], v3 = src[b + 768];
+    dst[b] = v0; dst[b + 256] = v1; dst[b + 512] = v2; dst[b + 768] = v3;
+  }
+  for (int64_t i = chunks * 1024 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
     dst[i] = src[i];
 }
 void launch_stream_copy(hipStream_t s, int64_t words, const void* src, void* dst) {
   if (words > 0)
-    k_stream_copy<<<4096, 256, 0, s>>>(words, static_cast<const int4*>(src), static_cast<int4*>(dst));
+    k_stream_copy<<<2048, 256, 0, s>>>(words, static_cast<const int4*>(src), static_cast<int4*>(dst));
 }
 
 }  // namespace mlba
