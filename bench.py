@@ -283,6 +283,7 @@ def main(argv=None):
               "(how the path would shard over several GPUs is documented as a design and not run)", file=sys.stderr)
         return 2
 
+    import numpy as np
     from maplab_amd import abi, problem
     from maplab_amd.solver import DeviceProblem
 
@@ -382,6 +383,21 @@ def main(argv=None):
             "final_cost_rel_to_block_jacobi": abs(rsm.final_cost - bsm.final_cost) / abs(bsm.final_cost),
             "solves": int(st.solves), "block_jacobi_fallbacks": int(st.fallbacks), "tolerance": 1e-6,
             "model": model.meta.get("generator")}
+    # maplab's default driver (--ba_use_visual_outlier_rejection_solver):
+    # chunks of 3 iterations with device outlier rejection after each,
+    # timed over the same iteration count.
+    dev.upload_state(init_v, init_l)
+    oo = abi.default_options()
+    ctypes.pointer(oo)[0] = opts
+    t0 = time.perf_counter()
+    osm, orows = dev.solve(oo, outlier_rejection=True)
+    o_wall = time.perf_counter() - t0
+    o_iters = sum(1 for r in orows if r.iteration > 0)
+    outlier_run = {"lm_iterations_per_s": round(o_iters / o_wall, 3), "lm_iterations": o_iters,
+                   "ms_per_step": round(1e3 * o_wall / max(1, o_iters), 3),
+                   "removed_landmarks": int(osm.num_removed_landmarks), "final_cost": osm.final_cost,
+                   "chunk_iterations": int(oo.reject_outliers_every_n_iters)}
+    dev.set_landmark_active(np.ones(vmap.num_landmarks, dtype=np.uint8))
     # The same LM iterations with FP32 Jacobian storage (timed the same way).
     dev.upload_state(init_v, init_l)
     fo = abi.default_options()
@@ -439,6 +455,7 @@ def main(argv=None):
         "pcg_iterations_vs_block_jacobi_at_1e-6": {
             **loose, "ratio": round(loose["block_jacobi"] / max(1, loose["deflated"]), 2), "tolerance": 1e-6},
         "learned_preconditioner": learned_lm,
+        "outlier_rejection_solver": outlier_run,
         "fp32_jacobians": {
             "lm_iterations_per_s": round(f_iters / f_wall, 3), "ms_per_step": round(1e3 * f_wall / max(1, f_iters), 3),
             "final_cost": fsm.final_cost,

@@ -262,6 +262,16 @@ LEARNED_SYMBOLS = {
     "mlba_gnn_last_error": (ctypes.c_char_p, []),
 }
 
+# Every symbol include/maplab_ba_vimap.h declares (exported by the same library).
+VIMAP_SYMBOLS = {
+    "mlba_vimap_load": (c_i32, [ctypes.c_char_p, ctypes.POINTER(c_vp)]),
+    "mlba_vimap_free": (None, [c_vp]),
+    "mlba_vimap_describe": (c_i32, [c_vp, ctypes.POINTER(ProblemDesc)]),
+    "mlba_vimap_landmark_quality": (c_i32, [c_vp, c_vp]),
+    "mlba_vimap_save": (c_i32, [c_vp, ctypes.c_char_p, c_vp, c_vp, c_vp]),
+    "mlba_vimap_last_error": (ctypes.c_char_p, []),
+}
+
 SYNTH_SYMBOLS = {
     "mlba_synth_config_init": (c_i32, [ctypes.POINTER(SynthConfig), c_i32]),
     "mlba_synth_generate": (c_i32, [ctypes.POINTER(SynthConfig), ctypes.POINTER(c_vp)]),
@@ -283,7 +293,8 @@ def load_library(path: str = LIB_PATH):
             raise OSError(f"{path} is missing: build it with `make -C maplab_amd/csrc` "
                           "(the visual BA path has no CPU fallback)")
         lib = ctypes.CDLL(path)
-        for name, (res, args) in {**LIB_SYMBOLS, **OPTIMIZER_SYMBOLS, **INERTIAL_SYMBOLS, **LEARNED_SYMBOLS}.items():
+        for name, (res, args) in {**LIB_SYMBOLS, **OPTIMIZER_SYMBOLS, **INERTIAL_SYMBOLS, **LEARNED_SYMBOLS,
+                                  **VIMAP_SYMBOLS}.items():
             _declare(lib, name, res, args)
         _lib = lib
     return _lib

@@ -29,6 +29,7 @@ fields and written back unchanged.
 """
 from __future__ import annotations
 
+import ctypes
 import gzip
 import os
 import shutil
@@ -531,6 +532,80 @@ def write_vimap(vmap: VisualMap, folder: str, compress: bool = True, seed: int =
     with open(os.path.join(base, SENSORS), "w") as f:
         yaml.safe_dump(sensors, f, sort_keys=False)
     return load_vimap(folder)[1]
+
+
+# ------------------------------------------------------------------ native reader
+
+class NativeMap:
+    """A map folder read by the library's C++ reader (include/maplab_ba_vimap.h),
+    the entry a C / C++ host uses; same problem as `load_vimap`."""
+
+    def __init__(self, folder: str):
+        self.lib = abi.load_library()
+        h = ctypes.c_void_p()
+        rc = self.lib.mlba_vimap_load(folder.encode(), ctypes.byref(h))
+        if rc != abi.MLBA_OK:
+            raise RuntimeError(f"mlba_vimap_load: {self.lib.mlba_vimap_last_error().decode()}")
+        self.handle = h
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.mlba_vimap_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def problem(self) -> VisualMap:
+        d = abi.ProblemDesc()
+        abi.check(self.lib.mlba_vimap_describe(self.handle, ctypes.byref(d)))
+        P, L, N, C, M = d.num_vertices, d.num_landmarks, d.num_observations, d.num_cameras, d.num_missions
+
+        def arr(ptr, n, dtype, shape=None):
+            ct = {np.float64: ctypes.c_double, np.int32: ctypes.c_int32, np.uint8: ctypes.c_uint8}[dtype]
+            out = np.zeros(n, dtype=dtype) if n == 0 or not ptr else \
+                np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ct)), (n,)).copy()
+            return out.reshape(shape) if shape else out
+
+        m = VisualMap(
+            vertices=arr(d.vertex_q_IM__M_p_MI, 7 * P, np.float64, (P, 7)),
+            baseframes=arr(d.baseframe_q_GM__G_p_GM, 7 * M, np.float64, (M, 7)),
+            landmarks=arr(d.landmark_p_B, 3 * L, np.float64, (L, 3)),
+            camera_q_CI__C_p_CI=arr(d.camera_q_CI__C_p_CI, 7 * C, np.float64, (C, 7)),
+            camera_intrinsics=arr(d.camera_intrinsics, 4 * C, np.float64, (C, 4)),
+            camera_distortion=arr(d.camera_distortion, 4 * C, np.float64, (C, 4)),
+            camera_distortion_type=arr(d.camera_distortion_type, C, np.int32),
+            camera_resolution=arr(d.camera_resolution, 2 * C, np.int32, (C, 2)),
+            vertex_mission=arr(d.vertex_mission, P, np.int32),
+            landmark_store_vertex=arr(d.landmark_store_vertex, L, np.int32),
+            obs_vertex=arr(d.obs_vertex, N, np.int32), obs_landmark=arr(d.obs_landmark, N, np.int32),
+            obs_camera=arr(d.obs_camera, N, np.int32), obs_keypoint=arr(d.obs_keypoint, 2 * N, np.float64, (N, 2)),
+            obs_sigma=arr(d.obs_sigma, N, np.float64), vertex_constant=arr(d.vertex_constant, P, np.uint8),
+            landmark_constant=arr(d.landmark_constant, L, np.uint8))
+        if d.camera_model:
+            m.camera_model = arr(d.camera_model, C, np.int32)
+            m.camera_xi = arr(d.camera_xi, C, np.float64)
+        return m.normalized()
+
+    def landmark_quality(self) -> np.ndarray:
+        d = abi.ProblemDesc()
+        abi.check(self.lib.mlba_vimap_describe(self.handle, ctypes.byref(d)))
+        q = np.zeros(d.num_landmarks, dtype=np.uint8)
+        abi.check(self.lib.mlba_vimap_landmark_quality(self.handle, q.ctypes.data))
+        return q
+
+    def save(self, folder: str, vertices, landmarks, landmark_quality=None) -> None:
+        """Writes the states into `folder` (which must hold vi_map/, e.g. a copy)."""
+        v = np.ascontiguousarray(vertices, dtype=np.float64)
+        l = np.ascontiguousarray(landmarks, dtype=np.float64)
+        q = None if landmark_quality is None else np.ascontiguousarray(landmark_quality, dtype=np.uint8)
+        rc = self.lib.mlba_vimap_save(self.handle, folder.encode(), v.ctypes.data, l.ctypes.data,
+                                      None if q is None else q.ctypes.data)
+        if rc != abi.MLBA_OK:
+            raise RuntimeError(f"mlba_vimap_save: {self.lib.mlba_vimap_last_error().decode()}")
 
 
 def optimize_map_folder(folder: str, out_folder: str | None = None, options=None, missions=None):

@@ -53,6 +53,14 @@ def test_library_exports_every_learned_preconditioner_symbol():
     assert lib.mlba_gnn_num_params() == 9 * 64 + 64 + 3 * (64 * 64 + 64 + 3 + 1 + 128 * 64 + 64) + 64 * 2 + 2
 
 
+def test_library_exports_every_vimap_symbol():
+    lib = abi.load_library()
+    syms = declared("maplab_ba_vimap.h")
+    assert syms == set(abi.VIMAP_SYMBOLS), syms ^ set(abi.VIMAP_SYMBOLS)
+    for s in syms:
+        assert hasattr(lib, s), s
+
+
 def test_synth_library_exports_every_declared_symbol():
     lib = abi.load_synth()
     syms = declared("maplab_ba_synth.h")
