@@ -36,10 +36,16 @@ def kernel_table(sm, kt, n_pcg_total):
       cost:               N * 36 + L * 32
       schur_factors:      n_inc * 352 (160 record read, 144 W + 48 r0 written)
                           (per linearization)
-      schur_incidences:   n_inc * 388 (144 W + 48 r0 read, 144 Q + 48 rhs
-                          written, 4 index) + L * 72 (pivot Z and g)
-      schur_blocks:       pairs * 8 (pair list) + n_inc * 336 (Q, W, rhs)
+      schur_blocks:       n_g * 192 (W or A_l, r0 of every factor block, read
+                          once) + L * 120 (Z, g; V_LM of anchors) + staged
+                          slots * 8 (slot words) + tile pairs * 2 (pair list)
+                          + batches * 530 (batch word, pair offsets)
+                          + segments * (1040 + 2 * 86016) (segment word,
+                          thread parts, partials written and read)
                           + upper blocks * 584 (S_ij and S_ji written, index)
+                          + P * 48 (rhs)
+                          (Q is formed in LDS: the former per-iteration
+                          incidence pass has no launches)
       pcg_spmv:           nnzb * 292 (6x6 block + column index) + P * 200 (z, p
                           read; q, p written; p'q) + P * 6 * gw * 8 (row of SW)
       pcg_update:         P * 344 (x, r read and written, p, q read, z
@@ -48,8 +54,9 @@ def kernel_table(sm, kt, n_pcg_total):
                           + groups * (3 + ne) * 8 (partials written)
     with gw the coarse columns a row group couples to, ne the coarse columns
     and groups = ceil(P / 42).
-    The Schur block kernel also does pairs * 216 FP64 flops (6x3 . 3x6),
-    below its byte time at MI355X's FP64 peak."""
+    with n_g the (landmark, row) factor blocks. The Schur block kernel also
+    does tile pairs * 216 FP64 flops (3x6' . 3x6), below its byte time at
+    MI355X's FP64 peak."""
     from maplab_amd import abi
     N, L, P = kt.num_observations, kt.num_landmarks, kt.num_variable_vertices
     nnzb, pairs, n_inc, nub = kt.schur_blocks, kt.schur_pairs, kt.incidences, kt.schur_upper_blocks
@@ -71,12 +78,15 @@ def kernel_table(sm, kt, n_pcg_total):
     add("k_linearize", abi.KERNEL_LINEARIZE, N * 196 + L * 176)
     add("k_linearize_cost", abi.KERNEL_COST, N * 36 + L * 32)
     add("k_schur_factors", abi.KERNEL_SCHUR_FACTORS, n_inc * 352)
-    add("k_schur_incidences", abi.KERNEL_SCHUR_INCIDENCES, n_inc * 388 + L * 72)
-    add("k_schur_blocks", abi.KERNEL_SCHUR_BLOCKS, pairs * 8 + n_inc * 336 + nub * 584)
+    tp = kt.schur_tile_pairs
+    add("k_schur_blocks", abi.KERNEL_SCHUR_BLOCKS,
+        kt.schur_factor_blocks * 192 + L * 120 + kt.schur_staged_blocks * 8 + tp * 2 + kt.schur_batches * 530
+        + kt.schur_segments * (1040 + 2 * 86016) + nub * 584 + P * 48)
     if "k_schur_blocks" in rows:
         r = rows["k_schur_blocks"]
-        r["flops"] = int(pairs * 216)
-        r["achieved_tflops"] = round(pairs * 216 / r["avg_ms"] / 1e9, 3)
+        r["flops"] = int(tp * 216)
+        r["achieved_tflops"] = round(tp * 216 / r["avg_ms"] / 1e9, 3)
+        r["staged_factor_blocks"] = int(kt.schur_staged_blocks)
     add("k_back_substitute", abi.KERNEL_BACK_SUBSTITUTE, N * 160 + L * 216)
     add("k_pose_rows", abi.KERNEL_POSE_ROWS)
     add("k_landmark_prep", abi.KERNEL_LANDMARK_PREP, L * (144 + 24 + 72))
@@ -105,9 +115,9 @@ def kernel_table(sm, kt, n_pcg_total):
 TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                                "lm_loop_random_walk_500kf_hbm_traffic.md")
 # Bench kernel row -> the kernel whose rocprofv3 FETCH_SIZE / WRITE_SIZE row
-# gives its HBM traffic (the Schur block build is dominated by its item pass).
-TRAFFIC_KERNEL = {"k_schur_blocks": "mlba::k_schur_items<4>", "k_pcg_spmv": "mlba::k_pcg_spmv<false>",
-                  "k_pcg_update": "mlba::k_pcg_update", "k_schur_incidences": "mlba::k_schur_incidences",
+# gives its HBM traffic (the Schur block build is dominated by its tile pass).
+TRAFFIC_KERNEL = {"k_schur_blocks": "mlba::k_schur_tiles", "k_pcg_spmv": "mlba::k_pcg_spmv<false>",
+                  "k_pcg_update": "mlba::k_pcg_update",
                   "k_schur_factors": "mlba::k_schur_factors<false>", "k_linearize": "mlba::k_linearize<true, false>",
                   "k_back_substitute": "mlba::k_back_substitute<false>"}
 
@@ -471,7 +481,7 @@ def main(argv=None):
         result["schur_schedule"] = {
             "item_band_rows": sched.item_band_rows, "item_waves": sched.item_waves,
             "xcd_mapping": sched.xcd_mapping, "schedules_tried": sched.schedules_tried,
-            "k_schur_items_default_ms": round(sched.default_ms, 4), "k_schur_items_tuned_ms": round(sched.tuned_ms, 4)}
+            "k_schur_tiles_default_ms": round(sched.default_ms, 4), "k_schur_tiles_tuned_ms": round(sched.tuned_ms, 4)}
 
     if model is not None and not args.no_learned_2000:
         result["learned_preconditioner_2000kf"] = learned_eval_2000(args, model)

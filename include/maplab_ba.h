@@ -397,6 +397,16 @@ typedef struct mlba_kernel_times {
   int64_t pcg_iterations;     /* iterations of the timed PCG solve */
   int64_t pcg_coarse_columns; /* columns of the deflation space (6 x hat nodes) */
   int64_t pcg_window_columns; /* coarse columns a PCG row group couples to */
+  /* The block kernel's work decomposition: (landmark, row) factor blocks,
+   * factor blocks staged over all tile visits (each visit stages its
+   * landmark's blocks in the tile's two bands), landmark tile visits,
+   * LDS batches, work items (tile segments) and factor-block products. */
+  int64_t schur_factor_blocks;
+  int64_t schur_staged_blocks;
+  int64_t schur_visits;
+  int64_t schur_batches;
+  int64_t schur_segments;
+  int64_t schur_tile_pairs;
 } mlba_kernel_times;
 
 int32_t mlba_time_kernels(mlba_handle* handle, const mlba_options* options, int32_t repeats,
@@ -415,11 +425,13 @@ int32_t mlba_measure_copy(int64_t bytes, int32_t repeats, double* ms_per_copy);
 /*
  * Launch-schedule autotuner (no counterpart in the reference: Ceres has no
  * device schedules). The reduced-camera-system block kernel sums fixed
- * incidence-pair ranges (work items) into fixed partial slots; what can be
- * chosen is only the order the items are dispatched in (bands of
- * item_band_rows block rows swept column by column, 0 = block order), how
- * many items a work-group runs (item_waves: 1, 2, 4, 8) and whether
- * work-groups are mapped XCD-contiguously. The PCG sums the per-group
+ * work items (segments of 16 x 16-block tiles of S, each summing its
+ * landmarks' factor-block products in a fixed order) into fixed partial
+ * slots; what can be chosen is only the order the items are dispatched in
+ * (bands of item_band_rows tile rows swept tile column by tile column; 0 =
+ * the items with the most work first), how many items a work-group runs
+ * (item_waves: 1, 2, 4, 8) and whether work-groups are mapped
+ * XCD-contiguously. The PCG sums the per-group
  * partials of each iteration (r'z, r'r and the coarse-space product of every
  * 42-row group) in group order, either in every SpMV work-group
  * (pcg_partials 1) or once, in the update work-group that finishes last
@@ -438,7 +450,7 @@ typedef struct mlba_schedule {
   int32_t item_waves;         /* 1, 2, 4 or 8 */
   int32_t xcd_mapping;        /* 0 or 1 */
   int32_t schedules_tried;    /* mlba_autotune: schedules timed */
-  double default_ms;          /* mlba_autotune: block kernel with the library default (4, 4, 1) */
+  double default_ms;          /* mlba_autotune: block kernel with the library default (0, 1, 0) */
   double tuned_ms;            /* mlba_autotune: block kernel with the chosen schedule */
   int32_t pcg_partials;       /* 0 automatic, 1 per SpMV work-group, 2 once per PCG update */
   int32_t reserved0;

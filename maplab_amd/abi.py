@@ -122,7 +122,8 @@ class KernelTimes(ctypes.Structure):
         (n, ctypes.c_int64) for n in (
             "num_observations", "num_landmarks", "num_variable_vertices", "schur_blocks", "schur_upper_blocks",
             "schur_pairs", "incidences", "linearize_tiles", "pcg_iterations", "pcg_coarse_columns",
-            "pcg_window_columns")]
+            "pcg_window_columns", "schur_factor_blocks", "schur_staged_blocks", "schur_visits", "schur_batches",
+            "schur_segments", "schur_tile_pairs")]
 
 
 class CalibrationOptions(ctypes.Structure):

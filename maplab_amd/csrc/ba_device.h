@@ -100,6 +100,23 @@ struct DevTopology {
   int32_t max_row_blocks;
 };
 
+// Device copy of the reduced camera system's work decomposition (ba_tiles.h):
+// segments (in dispatch order through seg_order) with their thread parts,
+// batches with their slot and pair lists, and the segment range of every
+// tile.
+struct SchurTilesDev {
+  int32_t n_segs, nbands;
+  const int32_t* seg_order;
+  const int4* segs;
+  const uint32_t* seg_thread;
+  const uint16_t* seg_block;
+  const int4* batches;
+  const int2* slots;
+  const uint16_t* batch_off;
+  const uint16_t* pairs;
+  const int2* tile_seg;
+};
+
 // Loss of the whitened residual.
 struct DevLoss {
   int32_t type;      // 0 Huber (maplab), 1 Cauchy, 2 trivial

@@ -28,6 +28,7 @@
 #include "ba_pcg.h"
 #include "ba_learned.h"
 #include "ba_ritz.h"
+#include "ba_tiles.h"
 #include "../../include/maplab_ba_learned.h"
 #include "ba_device.h"
 #include "ba_math.h"
@@ -46,30 +47,26 @@ void launch_jacobi_scaling(hipStream_t s, int32_t Pvar, int32_t L, int32_t use, 
                            const double* lm_acc, double* pose_scale, double* lm_scale);
 void launch_landmark_prep(hipStream_t s, const DevTopology& t, const double* lm_acc, const double* lm_scale,
                           double inv_mu, double min_diag, double max_diag, double* lm_z, int32_t* fail);
-void launch_schur_factors(hipStream_t s, const DevTopology& t, int32_t n_inc, const int32_t* inc_row,
-                          const double* vR, const double* lm, const double* obs_rec, const double* lm_acc,
-                          const double* pose_scale, double* inc_w, double* inc_r0, double* lm_a, bool f32 = false);
-void launch_schur_incidences(hipStream_t s, const DevTopology& t, int32_t n_inc, const double* inc_w,
-                             const double* inc_r0, const double* lm_a, const double* lm_z,
-                             double* inc_q, double* inc_rhs);
-void launch_schur_blocks(hipStream_t s, const DevTopology& t, int32_t nub, const int4* ublk, int32_t n_items,
-                         const int4* items, const int32_t* item_ptr, const uint2* pairs, const double* inc_q,
-                         const double* inc_w, const double* inc_rhs, const double* lm_a, const double* pose_scale,
-                         const double* pose_diag, const double* pose_jtj, double inv_mu, double min_diag,
-                         double max_diag, double* partial, double* S, double* rhs, int32_t waves, int32_t xcd_map);
-void launch_schur_items(hipStream_t s, const DevTopology& t, int32_t n_items, const int4* items, const uint2* pairs,
-                        const double* inc_q, const double* inc_w, const double* lm_a, double* partial,
-                        int32_t waves, int32_t xcd_map);
-bool schur_item_waves_supported(int32_t waves);
+void launch_schur_factors(hipStream_t s, const DevTopology& t, int32_t n_g, const int32_t* g_row,
+                          const int32_t* g_inc_ptr, const int32_t* g_inc, const double* vR, const double* lm,
+                          const double* obs_rec, const double* lm_acc, const double* pose_scale, double* gw,
+                          double* gr0, bool f32 = false);
+void launch_schur_blocks(hipStream_t s, const DevTopology& t, const SchurTilesDev& d, int32_t nub, const int4* ublk,
+                         const double* gw, const double* gr0, const double* lm_acc, const double* lm_z,
+                         const double* pose_scale, const double* pose_diag, const double* pose_jtj, double inv_mu,
+                         double min_diag, double max_diag, double* partial, double* S, double* rhs,
+                         int32_t items_per_wg, int32_t xcd_map);
+void launch_schur_tiles(hipStream_t s, const SchurTilesDev& d, const double* gw, const double* gr0,
+                        const double* lm_acc, const double* lm_z, double* partial, int32_t items_per_wg,
+                        int32_t xcd_map);
+bool schur_items_per_wg_supported(int32_t n);
 int32_t calib_entries();
 int32_t calib_record_stride();
 void launch_calib_normal(hipStream_t s, const DevTopology& t, const int32_t* perm, const double* res,
                          const double* rec, DevLoss loss, bool cost_only, double* partial, double* out);
-uint32_t schur_pair_anchor_bit();
 void launch_landmark_quality(hipStream_t s, const DevTopology& t, const double* vert, const double* vR,
                              const double* lm, int32_t min_observers, double min_angle_deg, double max_dist,
                              double min_dist, double max_reproj_px, uint8_t* good);
-int32_t schur_item_pairs();
 void launch_block_inverse(hipStream_t s, int32_t Pvar, const int32_t* diag_pos, const double* S, double* Minv,
                           int32_t* fail);
 int32_t pcg_max_lanczos();
@@ -107,8 +104,9 @@ thread_local std::string g_create_error;
 // past convergence two early-exit launches (≈10 µs).
 constexpr int32_t kPcgPoll = 16, kPcgPollAfter = 4;
 
-// Block rows per band of the Schur item dispatch order.
-constexpr int kItemBandRows = 4;
+// Default Schur dispatch schedule: largest segments first, one segment per
+// work-group, no XCD remapping.
+constexpr int kItemBandRows = 0, kItemsPerWg = 1, kItemXcd = 0;
 
 // Pinned host buffer of the recycling (R = kPcgMaxRitz): the solve's
 // coefficients (2 x 64), the tridiagonal's eigenvectors (64 x R), the Gram
@@ -164,8 +162,10 @@ struct mlba_handle {
   std::vector<DevCamera> h_cams;        // host mirror of the device cameras
   // Derived per topology build.
   int32_t Pvar = 0, tiles = 0, nnzb = 0, n_inc = 0, n_split = 0, max_row_blocks = 0;
-  int64_t schur_pairs = 0;
-  int32_t n_upper = 0, n_items = 0;     // upper-triangle blocks of S; Schur work items
+  int64_t schur_pairs = 0;              // incidence pairs summed into the upper blocks
+  int32_t n_upper = 0;                  // upper-triangle blocks of S
+  int32_t n_g = 0, n_segs = 0, nbands = 0;  // factor blocks, Schur work items (tile segments), tile bands
+  int64_t n_visits = 0, n_batches = 0, n_tpairs = 0, n_staged = 0;
   // Device memory.
   DBuf<double> obs_kp, obs_sinv, obs_rec;
   DBuf<int32_t> obs_vertex, obs_meta, obs_landmark, perm_d;
@@ -196,20 +196,21 @@ struct mlba_handle {
   DBuf<int32_t> tile_begin;
   DBuf<double> tile_cost, tile_cost2;
   DBuf<int32_t> row_ptr, col_var, diag_pos, inc_ptr, inc, obs_col;
-  DBuf<int32_t> inc_row, item_ptr;      // row of each incidence; work-item range of each upper block
-  DBuf<int4> ublk, items;               // upper block (position of S_ij, of S_ji, i, j); item (block, pairs)
-  DBuf<uint2> pairs;                    // incidence pairs (x in row i, y in row j) of each upper block
-  DBuf<double> inc_w, inc_r0, lm_a;     // per-linearization Schur factors (incidence order) and anchor maps
-  DBuf<double> inc_q, inc_rhs;          // per-iteration Schur factors
-  DBuf<double> schur_partial;           // 36 per work item
-  // Schur work items in block order (item.w = partial slot) with the row and
-  // column of each item's block; the device copy is in dispatch order.
-  std::vector<int4> h_items;
-  std::vector<int2> h_item_rc;
-  // Dispatch schedule of k_schur_items (mlba_autotune): block rows per band
-  // of the dispatch order (0: block order), items (waves) per work-group,
-  // XCD-contiguous work-group mapping. It changes no result.
-  int32_t sched_band = kItemBandRows, sched_waves = 4, sched_xcd = 1, sched_pcg = 0;
+  DBuf<int4> ublk;                      // upper block (position of S_ij, of S_ji, i, j)
+  DBuf<int32_t> g_row, g_inc_ptr, g_inc;  // factor blocks (ba_tiles.h)
+  DBuf<int4> t_batches, t_segs;
+  DBuf<int2> t_slots;
+  DBuf<uint32_t> t_seg_thread;
+  DBuf<uint16_t> t_batch_off, t_pairs, t_seg_block;
+  DBuf<int2> t_tile_seg;
+  DBuf<int32_t> t_seg_order;            // segments in dispatch order
+  DBuf<double> gw, gr0;                 // per-linearization factor blocks (W or A_l, r0)
+  DBuf<double> schur_partial;           // kTilePartial per segment
+  std::vector<int4> h_segs;             // host copy of the segments (dispatch order)
+  // Dispatch schedule of k_schur_tiles (mlba_autotune): tile rows per band of
+  // the dispatch order (0: tile order), segments (work items) per
+  // work-group, XCD-contiguous work-group mapping. It changes no result.
+  int32_t sched_band = kItemBandRows, sched_waves = kItemsPerWg, sched_xcd = kItemXcd, sched_pcg = 0;
   // Jacobian storage of the observation records for the current solve
   // (mlba_options.jacobian_fp32): FP32 Jacobians, FP64 residuals.
   bool f32_jac = false;
@@ -243,6 +244,7 @@ struct mlba_handle {
   double* scal_host = nullptr;  // pinned: kNumSlots scalars, then the step's failure flag
   hipEvent_t scal_ev = nullptr;
   DevTopology topo{};
+  SchurTilesDev tdev{};
   // Event pairs bracketing kernels inside the LM loop (kernel id, start, end).
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
@@ -289,25 +291,29 @@ bool valid_quaternion(const double* q) {
   return std::isfinite(n2) && std::fabs(n2 - 1.0) < 2e-5 && q[3] >= 0.0;
 }
 
-// Upload the Schur work items in the dispatch order of the current schedule:
-// bands of sched_band block rows, each band swept by column j, so that the
-// band's Q factors and the current column's W factors are reused from L2
-// (sched_band 0: block order). Partial slots keep the block order.
+// Upload the Schur work items (tile segments) in the dispatch order of the
+// current schedule: bands of sched_band tile rows, each band swept by tile
+// column J, so that the band's factor blocks (staged again by every tile of
+// the band) are reused from L2; sched_band 0: the segments with the most
+// batches first (the short ones fill the end of the launch). Partial slots
+// are the segment indices; the order changes no result.
 int upload_item_order(mlba_handle* h) {
-  std::vector<int32_t> ord(h->h_items.size());
+  std::vector<int32_t> ord(h->h_segs.size());
   for (size_t k = 0; k < ord.size(); ++k) ord[k] = (int32_t)k;
   const int band = h->sched_band;
+  if (band == 0)
+    std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) {
+      return h->h_segs[a].y - h->h_segs[a].x > h->h_segs[b].y - h->h_segs[b].x;
+    });
   if (band > 0)
     std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) {
-      const int2 ra = h->h_item_rc[a], rb = h->h_item_rc[b];
-      const int ba = ra.x / band, bb = rb.x / band;
+      const int4 ra = h->h_segs[a], rb = h->h_segs[b];
+      const int ba = ra.z / band, bb = rb.z / band;
       if (ba != bb) return ba < bb;
-      if (ra.y != rb.y) return ra.y < rb.y;
-      return ra.x < rb.x;
+      if (ra.w != rb.w) return ra.w < rb.w;
+      return ra.z < rb.z;
     });
-  std::vector<int4> items(ord.size());
-  for (size_t k = 0; k < ord.size(); ++k) items[k] = h->h_items[ord[k]];
-  int rc = upload(h, h->items, items.data(), items.size());
+  int rc = upload(h, h->t_seg_order, ord.data(), ord.size());
   if (rc) return rc;
   // The host vector is released when the call returns: wait for the copy.
   HIP_OK(hipStreamSynchronize(h->stream));
@@ -438,83 +444,61 @@ int build_topology(mlba_handle* h) {
   h->ritz_stage = 0;
   h->h_row_ptr = row_ptr;
   h->h_col_var = col_var;
-  // Upper blocks of S and, for each, the incidence pairs (x in row i, y in
-  // row j >= i, same landmark) whose products sum into it, ordered by x.
-  std::vector<int32_t> inc_row(inc.size());
-  for (int i = 0; i < Pvar; ++i)
-    for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) inc_row[k] = i;
-  auto inc_landmark = [&](int32_t x) { return x >= 0 ? h->s_landmark[x] : -1 - x; };
-  std::vector<int32_t> lm_inc_ptr(L + 1, 0), lm_inc(inc.size());
-  for (size_t k = 0; k < inc.size(); ++k) lm_inc_ptr[inc_landmark(inc[k]) + 1]++;
-  for (int l = 0; l < L; ++l) lm_inc_ptr[l + 1] += lm_inc_ptr[l];
-  {
-    std::vector<int32_t> fill(lm_inc_ptr.begin(), lm_inc_ptr.end() - 1);
-    for (size_t k = 0; k < inc.size(); ++k) lm_inc[fill[inc_landmark(inc[k])]++] = (int32_t)k;
-  }
+  // Upper blocks of S (row i, column j >= i, in row order) and the work
+  // decomposition of their sums (ba_tiles.h).
   std::vector<int4> ublk;
-  std::vector<int32_t> pair_ptr{0};
-  std::vector<uint2> pairs;
-  {
-    const uint32_t abit = schur_pair_anchor_bit();
-    std::vector<int32_t> upos(Pvar, -1), cnt;
-    std::vector<int64_t> fill;
-    int64_t total = 0;
-    for (int i = 0; i < Pvar; ++i) {
-      const int base = (int)ublk.size();
-      for (int k = diag_pos[i]; k < row_ptr[i + 1]; ++k) {
-        const int j = col_var[k];
-        int kt = k;
-        if (j != i) {
-          const int32_t* b = col_var.data() + row_ptr[j];
-          const int32_t* e = col_var.data() + row_ptr[j + 1];
-          kt = (int32_t)(std::lower_bound(b, e, i) - col_var.data());
-        }
-        upos[j] = (int32_t)ublk.size();
-        ublk.push_back(make_int4(k, kt, i, j));
+  std::vector<int32_t> ubase(Pvar + 1, 0);
+  for (int i = 0; i < Pvar; ++i) {
+    ubase[i] = (int32_t)ublk.size();
+    for (int k = diag_pos[i]; k < row_ptr[i + 1]; ++k) {
+      const int j = col_var[k];
+      int kt = k;
+      if (j != i) {
+        const int32_t* b = col_var.data() + row_ptr[j];
+        const int32_t* e = col_var.data() + row_ptr[j + 1];
+        kt = (int32_t)(std::lower_bound(b, e, i) - col_var.data());
       }
-      const int nb = (int)ublk.size() - base;
-      cnt.assign(nb, 0);
-      for (int x = inc_ptr[i]; x < inc_ptr[i + 1]; ++x) {
-        const int l = inc_landmark(inc[x]);
-        for (int m = lm_inc_ptr[l]; m < lm_inc_ptr[l + 1]; ++m) {
-          const int j = inc_row[lm_inc[m]];
-          if (j < i) continue;
-          if (upos[j] < base) return set_err(h, MLBA_EINVAL, "internal: S block structure misses a landmark pair");
-          cnt[upos[j] - base]++;
-        }
-      }
-      fill.assign(nb, 0);
-      for (int b = 0; b < nb; ++b) {
-        fill[b] = total;
-        total += cnt[b];
-        if (total > INT32_MAX) return set_err(h, MLBA_ENOMEM, "reduced camera system has too many landmark pairs");
-        pair_ptr.push_back((int32_t)total);
-      }
-      pairs.resize((size_t)total);
-      for (int x = inc_ptr[i]; x < inc_ptr[i + 1]; ++x) {
-        const int l = inc_landmark(inc[x]);
-        const uint32_t xe = (uint32_t)x | (inc[x] < 0 ? abit : 0u);
-        for (int m = lm_inc_ptr[l]; m < lm_inc_ptr[l + 1]; ++m) {
-          const int y = lm_inc[m];
-          if (inc_row[y] < i) continue;
-          pairs[(size_t)fill[upos[inc_row[y]] - base]++] = make_uint2(xe, (uint32_t)y | (inc[y] < 0 ? abit : 0u));
-        }
-      }
+      ublk.push_back(make_int4(k, kt, i, j));
     }
   }
+  ubase[Pvar] = (int32_t)ublk.size();
   h->n_upper = (int32_t)ublk.size();
-  h->schur_pairs = (int64_t)pairs.size();
-  // Work items: each upper block's pairs in ranges of at most schur_item_pairs().
-  std::vector<int4> items;
-  std::vector<int32_t> item_ptr{0};
+  SchurTiles st;
   {
-    const int ip = schur_item_pairs();
-    for (int u = 0; u < h->n_upper; ++u) {
-      for (int pb = pair_ptr[u]; pb < pair_ptr[u + 1]; pb += ip)
-        items.push_back(make_int4(u, pb, std::min(pb + ip, pair_ptr[u + 1]), (int32_t)items.size()));
-      item_ptr.push_back((int32_t)items.size());
-    }
+    std::string terr;
+    if (!build_schur_tiles(Pvar, L, inc_ptr.data(), inc.data(), h->s_landmark.data(), &st, &terr))
+      return set_err(h, MLBA_ENOMEM, terr);
   }
+  // Incidence pairs (x in row i, y in row j >= i, same landmark) summed into
+  // each upper block: two factor blocks of a landmark multiply their
+  // incidence counts.
+  std::vector<int32_t> upair(ublk.size(), 0);
+  int64_t total_pairs = 0;
+  for (int l = 0; l < L; ++l)
+    for (int a = st.lm_g_ptr[l]; a < st.lm_g_ptr[l + 1]; ++a) {
+      const int i = st.g_row[a];
+      const int ma = st.g_inc_ptr[a + 1] - st.g_inc_ptr[a];
+      const int32_t* cb = col_var.data() + diag_pos[i];
+      const int32_t* ce = col_var.data() + row_ptr[i + 1];
+      for (int b = a; b < st.lm_g_ptr[l + 1]; ++b) {
+        const int j = st.g_row[b];
+        const int32_t* f = std::lower_bound(cb, ce, j);
+        if (f == ce || *f != j) return set_err(h, MLBA_EINVAL, "internal: S block structure misses a landmark pair");
+        const int64_t m = (int64_t)ma * (st.g_inc_ptr[b + 1] - st.g_inc_ptr[b]);
+        upair[ubase[i] + (f - cb)] += (int32_t)m;
+        total_pairs += m;
+      }
+    }
+  h->schur_pairs = total_pairs;
+  h->n_g = (int32_t)st.g_row.size();
+  h->nbands = st.nbands;
+  h->n_segs = (int32_t)(st.segs.size() / 4);
+  h->n_visits = st.visits;
+  h->n_batches = (int64_t)(st.batches.size() / 4);
+  h->n_tpairs = (int64_t)st.pairs.size();
+  h->n_staged = st.staged;
+  h->h_segs.assign(reinterpret_cast<const int4*>(st.segs.data()),
+                   reinterpret_cast<const int4*>(st.segs.data()) + h->n_segs);
   // Learned preconditioner structure: row and transposed position of every
   // block of S, and the landmark pairs summed into it (shared observations).
   std::vector<int32_t> lp_pos_row(h->nnzb), lp_tpos(h->nnzb), lp_shared(h->nnzb, 0);
@@ -527,12 +511,8 @@ int build_topology(mlba_handle* h) {
     const int4 b = ublk[u];
     lp_tpos[b.x] = b.y;
     lp_tpos[b.y] = b.x;
-    lp_shared[b.x] = lp_shared[b.y] = pair_ptr[u + 1] - pair_ptr[u];
+    lp_shared[b.x] = lp_shared[b.y] = upair[u];
   }
-  h->n_items = (int32_t)items.size();
-  h->h_item_rc.resize(items.size());
-  for (size_t k = 0; k < items.size(); ++k) h->h_item_rc[k] = make_int2(ublk[items[k].x].z, ublk[items[k].x].w);
-  h->h_items = std::move(items);
   std::vector<int32_t> obs_col(N, -1);
   for (int o = 0; o < N; ++o)
     if ((meta[o] & 4) && h->lm_active[h->s_landmark[o]]) obs_col[o] = vertex_var[h->s_vertex[o]];
@@ -552,20 +532,29 @@ int build_topology(mlba_handle* h) {
   if ((rc = upload(h, h->row_ptr, row_ptr.data(), row_ptr.size()))) return rc;
   if ((rc = upload(h, h->col_var, col_var.data(), col_var.size()))) return rc;
   if ((rc = upload(h, h->diag_pos, diag_pos.data(), diag_pos.size()))) return rc;
-  if ((rc = upload(h, h->inc_row, inc_row.data(), inc_row.size()))) return rc;
   if ((rc = upload(h, h->ublk, ublk.data(), ublk.size()))) return rc;
+  if ((rc = upload(h, h->g_row, st.g_row.data(), st.g_row.size()))) return rc;
+  if ((rc = upload(h, h->g_inc_ptr, st.g_inc_ptr.data(), st.g_inc_ptr.size()))) return rc;
+  if ((rc = upload(h, h->g_inc, st.g_inc.data(), st.g_inc.size()))) return rc;
+  if ((rc = upload(h, h->t_slots, reinterpret_cast<const int2*>(st.slots.data()), st.slots.size() / 2))) return rc;
+  if ((rc = upload(h, h->t_seg_thread, st.seg_thread.data(), st.seg_thread.size()))) return rc;
+  if ((rc = upload(h, h->t_seg_block, st.seg_block.data(), st.seg_block.size()))) return rc;
+  if ((rc = upload(h, h->t_batches, reinterpret_cast<const int4*>(st.batches.data()), st.batches.size() / 4)))
+    return rc;
+  if ((rc = upload(h, h->t_segs, reinterpret_cast<const int4*>(st.segs.data()), st.segs.size() / 4))) return rc;
+  if ((rc = upload(h, h->t_batch_off, st.batch_off.data(), st.batch_off.size()))) return rc;
+  if ((rc = upload(h, h->t_pairs, st.pairs.data(), st.pairs.size()))) return rc;
+  if ((rc = upload(h, h->t_tile_seg, reinterpret_cast<const int2*>(st.tile_seg.data()), st.tile_seg.size() / 2)))
+    return rc;
   if ((rc = upload_item_order(h))) return rc;
-  if ((rc = upload(h, h->item_ptr, item_ptr.data(), item_ptr.size()))) return rc;
-  if ((rc = upload(h, h->pairs, pairs.data(), pairs.size()))) return rc;
   if ((rc = upload(h, h->obs_col, obs_col.data(), obs_col.size()))) return rc;
   const size_t T = h->tiles, V6 = 6 * (size_t)std::max(Pvar, 1);
   const size_t vb = (Pvar + 255) / 256 + 1;
   bool ok = h->tile_cost.alloc(2 * T) && h->tile_cost2.alloc(2 * T) && h->S.alloc(36 * (size_t)h->nnzb) &&
             h->split_part.alloc(h->n_split ? kLmAcc * T : 1) &&
-            h->inc_q.alloc(18 * std::max<size_t>(inc.size(), 1)) && h->inc_rhs.alloc(6 * std::max<size_t>(inc.size(), 1)) &&
-            h->inc_w.alloc(18 * std::max<size_t>(inc.size(), 1)) && h->inc_r0.alloc(6 * std::max<size_t>(inc.size(), 1)) &&
-            h->lm_a.alloc(18 * (size_t)std::max(L, 1)) &&
-            h->schur_partial.alloc(36 * std::max<size_t>(h->h_items.size(), 1)) &&
+            h->gw.alloc(18 * std::max<size_t>(st.g_row.size(), 1)) &&
+            h->gr0.alloc(6 * std::max<size_t>(st.g_row.size(), 1)) &&
+            h->schur_partial.alloc((size_t)kTilePartial * std::max(h->n_segs, 1)) &&
             h->pose_jtj.alloc(36 * (size_t)std::max(Pvar, 1)) &&
             h->rhs.alloc(V6) && h->pose_grad.alloc(V6) && h->pose_diag.alloc(V6) && h->pose_scale.alloc(V6) &&
             h->w.alloc(V6) && h->Minv.alloc(36 * (size_t)std::max(Pvar, 1)) && h->px.alloc(V6) &&
@@ -587,6 +576,10 @@ int build_topology(mlba_handle* h) {
   t.vertex_mission = h->vertex_mission_d.p; t.vertex_var = h->vertex_var.p; t.var_vertex = h->var_vertex.p;
   t.row_ptr = h->row_ptr.p; t.col_var = h->col_var.p; t.diag_pos = h->diag_pos.p; t.inc_ptr = h->inc_ptr.p;
   t.inc = h->inc.p; t.max_row_blocks = h->max_row_blocks;
+  SchurTilesDev& d = h->tdev;
+  d.n_segs = h->n_segs; d.nbands = h->nbands; d.seg_order = h->t_seg_order.p; d.segs = h->t_segs.p;
+  d.seg_thread = h->t_seg_thread.p; d.seg_block = h->t_seg_block.p; d.batches = h->t_batches.p;
+  d.slots = h->t_slots.p; d.batch_off = h->t_batch_off.p; d.pairs = h->t_pairs.p; d.tile_seg = h->t_tile_seg.p;
   HIP_OK(hipStreamSynchronize(h->stream));
   return MLBA_OK;
 }
@@ -887,12 +880,12 @@ DevLoss make_loss(const mlba_options* o) {
 // Linearize at the current state: residual records, landmark reductions,
 // vertex-row gradient and diag(F'F). Leaves cost / fixed cost / gradient
 // max norm in the scalar slots (device).
-// Per-linearization Schur factors (W, r0 of every incidence, anchor maps);
+// Per-linearization Schur factors (W, r0 of every factor block, anchor maps);
 // needs the Jacobi scaling of the solve.
 void schur_factors(mlba_handle* h) {
   KernelTimer kt(h, MLBA_KERNEL_SCHUR_FACTORS);
-  launch_schur_factors(h->stream, h->topo, h->n_inc, h->inc_row.p, h->vR.p, h->lm.p, h->obs_rec.p, h->lm_acc.p,
-                       h->pose_scale.p, h->inc_w.p, h->inc_r0.p, h->lm_a.p, h->f32_jac);
+  launch_schur_factors(h->stream, h->topo, h->n_g, h->g_row.p, h->g_inc_ptr.p, h->g_inc.p, h->vR.p, h->lm.p,
+                       h->obs_rec.p, h->lm_acc.p, h->pose_scale.p, h->gw.p, h->gr0.p, h->f32_jac);
 }
 
 int linearize(mlba_handle* h, const mlba_options* o) {
@@ -1099,16 +1092,11 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
                              o->max_lm_diagonal, h->lm_z.p, h->fail.p);
       }
       {
-        KernelTimer kt(h, MLBA_KERNEL_SCHUR_INCIDENCES);
-        launch_schur_incidences(s, h->topo, h->n_inc, h->inc_w.p, h->inc_r0.p, h->lm_a.p, h->lm_z.p,
-                                h->inc_q.p, h->inc_rhs.p);
-      }
-      {
         KernelTimer kt(h, MLBA_KERNEL_SCHUR_BLOCKS);
-        launch_schur_blocks(s, h->topo, h->n_upper, h->ublk.p, h->n_items, h->items.p, h->item_ptr.p, h->pairs.p,
-                            h->inc_q.p, h->inc_w.p, h->inc_rhs.p, h->lm_a.p, h->pose_scale.p, h->pose_diag.p,
-                            h->pose_jtj.p, inv_mu, o->min_lm_diagonal, o->max_lm_diagonal, h->schur_partial.p,
-                            h->S.p, h->rhs.p, h->sched_waves, h->sched_xcd);
+        launch_schur_blocks(s, h->topo, h->tdev, h->n_upper, h->ublk.p, h->gw.p, h->gr0.p, h->lm_acc.p, h->lm_z.p,
+                            h->pose_scale.p, h->pose_diag.p, h->pose_jtj.p, inv_mu, o->min_lm_diagonal,
+                            o->max_lm_diagonal, h->schur_partial.p, h->S.p, h->rhs.p, h->sched_waves,
+                            h->sched_xcd);
       }
       launch_block_inverse(s, h->Pvar, h->diag_pos.p, h->S.p, h->Minv.p, h->fail.p);
       HIP_OK(hipGetLastError());
@@ -1744,12 +1732,9 @@ int build_rcs(mlba_handle* h, const mlba_options* o, double radius) {
   const double inv_mu = 1.0 / radius;
   launch_landmark_prep(s, h->topo, h->lm_acc.p, h->lm_scale.p, inv_mu, o->min_lm_diagonal, o->max_lm_diagonal,
                        h->lm_z.p, h->fail.p);
-  launch_schur_incidences(s, h->topo, h->n_inc, h->inc_w.p, h->inc_r0.p, h->lm_a.p, h->lm_z.p, h->inc_q.p,
-                          h->inc_rhs.p);
-  launch_schur_blocks(s, h->topo, h->n_upper, h->ublk.p, h->n_items, h->items.p, h->item_ptr.p, h->pairs.p,
-                      h->inc_q.p, h->inc_w.p, h->inc_rhs.p, h->lm_a.p, h->pose_scale.p, h->pose_diag.p,
-                      h->pose_jtj.p, inv_mu, o->min_lm_diagonal, o->max_lm_diagonal, h->schur_partial.p, h->S.p,
-                      h->rhs.p, h->sched_waves, h->sched_xcd);
+  launch_schur_blocks(s, h->topo, h->tdev, h->n_upper, h->ublk.p, h->gw.p, h->gr0.p, h->lm_acc.p, h->lm_z.p,
+                      h->pose_scale.p, h->pose_diag.p, h->pose_jtj.p, inv_mu, o->min_lm_diagonal,
+                      o->max_lm_diagonal, h->schur_partial.p, h->S.p, h->rhs.p, h->sched_waves, h->sched_xcd);
   HIP_OK(hipGetLastError());
   return MLBA_OK;
 }
@@ -2345,7 +2330,7 @@ int32_t mlba_set_schedule(mlba_handle* h, const mlba_schedule* sc) {
   if (!h) return MLBA_EINVAL;
   if (!sc) return set_err(h, MLBA_EINVAL, "schedule is NULL");
   if (sc->item_band_rows < 0 || sc->item_band_rows > 4096) return set_err(h, MLBA_EINVAL, "item_band_rows out of range");
-  if (!schur_item_waves_supported(sc->item_waves)) return set_err(h, MLBA_EINVAL, "item_waves must be 1, 2, 4 or 8");
+  if (!schur_items_per_wg_supported(sc->item_waves)) return set_err(h, MLBA_EINVAL, "item_waves must be 1, 2, 4 or 8");
   if (sc->xcd_mapping != 0 && sc->xcd_mapping != 1) return set_err(h, MLBA_EINVAL, "xcd_mapping must be 0 or 1");
   if (sc->pcg_partials < 0 || sc->pcg_partials > 2) return set_err(h, MLBA_EINVAL, "pcg_partials must be 0, 1 or 2");
   const bool reorder = sc->item_band_rows != h->sched_band;
@@ -2363,7 +2348,7 @@ int32_t mlba_autotune(mlba_handle* h, const mlba_options* o, int32_t repeats, ml
   if (rc) return rc;
   std::memset(out, 0, sizeof(*out));
   hipStream_t s = h->stream;
-  if (h->n_items > 0) {
+  if (h->n_segs > 0) {
     // The Schur factors of the first LM iteration at the current state.
     if ((rc = linearize(h, o))) return rc;
     launch_jacobi_scaling(s, h->Pvar, h->L, o->use_jacobi_scaling, h->pose_diag.p, h->lm_acc.p, h->pose_scale.p,
@@ -2372,8 +2357,6 @@ int32_t mlba_autotune(mlba_handle* h, const mlba_options* o, int32_t repeats, ml
     HIP_OK(hipMemsetAsync(h->fail.p, 0, sizeof(int32_t), s));
     launch_landmark_prep(s, h->topo, h->lm_acc.p, h->lm_scale.p, 1.0 / o->initial_trust_region_radius,
                          o->min_lm_diagonal, o->max_lm_diagonal, h->lm_z.p, h->fail.p);
-    launch_schur_incidences(s, h->topo, h->n_inc, h->inc_w.p, h->inc_r0.p, h->lm_a.p, h->lm_z.p,
-                            h->inc_q.p, h->inc_rhs.p);
     HIP_OK(hipGetLastError());
     hipEvent_t e0, e1;
     HIP_OK(hipEventCreate(&e0));
@@ -2382,12 +2365,12 @@ int32_t mlba_autotune(mlba_handle* h, const mlba_options* o, int32_t repeats, ml
       return set_err(h, MLBA_EDEVICE, "hipEventCreate failed");
     }
     auto time_items = [&]() -> double {
-      launch_schur_items(s, h->topo, h->n_items, h->items.p, h->pairs.p, h->inc_q.p, h->inc_w.p, h->lm_a.p,
-                         h->schur_partial.p, h->sched_waves, h->sched_xcd);  // warm
+      launch_schur_tiles(s, h->tdev, h->gw.p, h->gr0.p, h->lm_acc.p, h->lm_z.p, h->schur_partial.p, h->sched_waves,
+                         h->sched_xcd);  // warm
       (void)hipEventRecord(e0, s);
       for (int k = 0; k < repeats; ++k)
-        launch_schur_items(s, h->topo, h->n_items, h->items.p, h->pairs.p, h->inc_q.p, h->inc_w.p, h->lm_a.p,
-                           h->schur_partial.p, h->sched_waves, h->sched_xcd);
+        launch_schur_tiles(s, h->tdev, h->gw.p, h->gr0.p, h->lm_acc.p, h->lm_z.p, h->schur_partial.p,
+                           h->sched_waves, h->sched_xcd);
       (void)hipEventRecord(e1, s);
       (void)hipEventSynchronize(e1);
       float ms = 0.f;
@@ -2395,7 +2378,7 @@ int32_t mlba_autotune(mlba_handle* h, const mlba_options* o, int32_t repeats, ml
       return (double)ms / repeats;
     };
     static const int32_t kBands[] = {0, 1, 2, 4, 8, 16, 32}, kWaves[] = {1, 2, 4, 8};
-    mlba_schedule best{kItemBandRows, 4, 1, 0, 0.0, 0.0, h->sched_pcg, 0};
+    mlba_schedule best{kItemBandRows, kItemsPerWg, kItemXcd, 0, 0.0, 0.0, h->sched_pcg, 0};
     double best_ms = -1.0, default_ms = -1.0;
     for (int32_t band : kBands) {
       h->sched_band = band;
@@ -2406,7 +2389,7 @@ int32_t mlba_autotune(mlba_handle* h, const mlba_options* o, int32_t repeats, ml
           h->sched_xcd = xcd;
           const double ms = time_items();
           ++out->schedules_tried;
-          if (band == kItemBandRows && waves == 4 && xcd == 1) default_ms = ms;
+          if (band == kItemBandRows && waves == kItemsPerWg && xcd == kItemXcd) default_ms = ms;
           if (best_ms < 0.0 || ms < best_ms) {
             best_ms = ms;
             best.item_band_rows = band;
@@ -2504,15 +2487,10 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
     launch_landmark_prep(s, h->topo, h->lm_acc.p, h->lm_scale.p, inv_mu, o->min_lm_diagonal, o->max_lm_diagonal,
                          h->lm_z.p, h->fail.p);
   });
-  t->schur_incidences_ms = timed([&] {
-    launch_schur_incidences(s, h->topo, h->n_inc, h->inc_w.p, h->inc_r0.p, h->lm_a.p, h->lm_z.p,
-                            h->inc_q.p, h->inc_rhs.p);
-  });
   t->schur_blocks_ms = timed([&] {
-    launch_schur_blocks(s, h->topo, h->n_upper, h->ublk.p, h->n_items, h->items.p, h->item_ptr.p, h->pairs.p,
-                        h->inc_q.p, h->inc_w.p, h->inc_rhs.p, h->lm_a.p, h->pose_scale.p, h->pose_diag.p,
-                        h->pose_jtj.p, inv_mu, o->min_lm_diagonal, o->max_lm_diagonal, h->schur_partial.p, h->S.p,
-                        h->rhs.p, h->sched_waves, h->sched_xcd);
+    launch_schur_blocks(s, h->topo, h->tdev, h->n_upper, h->ublk.p, h->gw.p, h->gr0.p, h->lm_acc.p, h->lm_z.p,
+                        h->pose_scale.p, h->pose_diag.p, h->pose_jtj.p, inv_mu, o->min_lm_diagonal,
+                        o->max_lm_diagonal, h->schur_partial.p, h->S.p, h->rhs.p, h->sched_waves, h->sched_xcd);
   });
   t->block_inverse_ms = timed([&] { launch_block_inverse(s, h->Pvar, h->diag_pos.p, h->S.p, h->Minv.p, h->fail.p); });
   if (h->Pvar > 0) {
@@ -2563,6 +2541,12 @@ int32_t mlba_time_kernels(mlba_handle* h, const mlba_options* o, int32_t repeats
   t->schur_pairs = h->schur_pairs;
   t->incidences = h->n_inc;
   t->linearize_tiles = h->tiles;
+  t->schur_factor_blocks = h->n_g;
+  t->schur_staged_blocks = h->n_staged;
+  t->schur_visits = h->n_visits;
+  t->schur_batches = h->n_batches;
+  t->schur_segments = h->n_segs;
+  t->schur_tile_pairs = h->n_tpairs;
   return MLBA_OK;
 }
 

@@ -29,6 +29,7 @@
 #include "ba_device.h"
 #include "ba_math.h"
 #include "ba_reduce.h"
+#include "ba_tiles.h"
 
 namespace mlba {
 
@@ -473,7 +474,7 @@ __device__ inline void wprime_obs(const Rec& rec, const double* cs, double* W) {
     for (int c = 0; c < 6; ++c) W[a * 6 + c] = (rec[2 + a] * rec[8 + c] + rec[5 + a] * rec[14 + c]) * cs[c];
 }
 
-// Incidence factors of the reduced camera system
+// Factor blocks of the reduced camera system
 //   S = F'F + D^2 - sum_l W_l' Z_l W_l,  rhs = F'r - sum_l W_l' Z_l g_l,
 // with Z_l = C_l (C_l V_l C_l + D_l^2)^-1 C_l from k_landmark_prep. An
 // incidence x of variable vertex i is an observation by i whose residual
@@ -484,186 +485,290 @@ __device__ inline void wprime_obs(const Rec& rec, const double* cs, double* W) {
 // and per LM iteration (Z_l depends on the damping)
 //   Q_x = [anchor: A_l^T] - W_x^T Z_l,  rhs_x = r0_x - W_x^T Z_l g_l
 // (ceres' Schur eliminator, schur_eliminator_impl.h, restated on the
-// anchor-lifted blocks). W and r0 change only with the linearization and
-// are stored in incidence order (rows of S, then landmarks), so the block
-// kernel streams them; A_l is stored per landmark.
+// anchor-lifted blocks). S_ij sums, over the landmarks l seen from rows i and
+// j >= i, Q_x W_y + [x observation, y anchor] W_x^T A_l over l's incidences
+// x in row i and y in row j. Both terms are linear in the incidences of one
+// row, so the incidences of a landmark in one row are summed first into a
+// factor block (W, r0); an anchor is alone in its row and its block stores
+// A_l in place of W (the Schur kernel forms W = V_LM,l A_l next to it).
+// Factor blocks are stored landmark-major (rows ascending), per
+// linearization.
 template <bool kF32>
 __global__ __launch_bounds__(256) void k_schur_factors(
-    DevTopology t, int32_t n_inc, const int32_t* __restrict__ inc_row, const double* __restrict__ vR,
-    const double* __restrict__ lm, const void* __restrict__ obs_rec, const double* __restrict__ lm_acc,
-    const double* __restrict__ pose_scale, double* __restrict__ inc_w, double* __restrict__ inc_r0,
-    double* __restrict__ lm_a) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_inc) return;
-  const int i = inc_row[p];
-  const int32_t x = t.inc[p];
+    DevTopology t, int32_t n_g, const int32_t* __restrict__ g_row, const int32_t* __restrict__ g_inc_ptr,
+    const int32_t* __restrict__ g_inc, const double* __restrict__ vR, const double* __restrict__ lm,
+    const void* __restrict__ obs_rec, const double* __restrict__ lm_acc, const double* __restrict__ pose_scale,
+    double* __restrict__ gw, double* __restrict__ gr0) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_g) return;
+  const int i = g_row[g];
   const double* cs = pose_scale + 6 * i;
   double W[18], r0[6];
-  if (x >= 0) {
-    const RecView<kF32> rec(obs_rec, x);
-    wprime_obs(rec, cs, W);
-    for (int c = 0; c < 6; ++c) r0[c] = cs[c] * (rec[8 + c] * rec[0] + rec[14 + c] * rec[1]);
-  } else {
-    const int32_t l = -1 - x;
-    const int v = t.var_vertex[i];
-    const double* acc_l = lm_acc + (size_t)kLmAcc * l;
-    double M[18], A[18], VLM[9];
-    anchor_map(lm + 3 * l, vR + 12 * v, vR[12 * v + 9], M);
-    for (int a = 0; a < 3; ++a)
-      for (int c = 0; c < 6; ++c) A[a * 6 + c] = M[a * 6 + c] * cs[c];
-    sym3_full(acc_l + 6, VLM);
-    for (int a = 0; a < 3; ++a)
-      for (int c = 0; c < 6; ++c)
-        W[a * 6 + c] = VLM[a * 3] * A[c] + VLM[a * 3 + 1] * A[6 + c] + VLM[a * 3 + 2] * A[12 + c];
-    const double* gLM = acc_l + 15;
-    for (int c = 0; c < 6; ++c) r0[c] = A[c] * gLM[0] + A[6 + c] * gLM[1] + A[12 + c] * gLM[2];
-    double* out = lm_a + (size_t)18 * l;
-    for (int k = 0; k < 18; ++k) out[k] = A[k];
+  for (int k = 0; k < 18; ++k) W[k] = 0.0;
+  for (int c = 0; c < 6; ++c) r0[c] = 0.0;
+  for (int m = g_inc_ptr[g]; m < g_inc_ptr[g + 1]; ++m) {
+    const int32_t x = g_inc[m];
+    double Wx[18], rx[6];
+    if (x >= 0) {
+      const RecView<kF32> rec(obs_rec, x);
+      wprime_obs(rec, cs, Wx);
+      for (int c = 0; c < 6; ++c) rx[c] = cs[c] * (rec[8 + c] * rec[0] + rec[14 + c] * rec[1]);
+    } else {
+      const int32_t l = -1 - x;
+      const int v = t.var_vertex[i];
+      const double* acc_l = lm_acc + (size_t)kLmAcc * l;
+      double M[18], A[18];
+      anchor_map(lm + 3 * l, vR + 12 * v, vR[12 * v + 9], M);
+      for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 6; ++c) A[a * 6 + c] = M[a * 6 + c] * cs[c];
+      const double* gLM = acc_l + 15;
+      for (int c = 0; c < 6; ++c) rx[c] = A[c] * gLM[0] + A[6 + c] * gLM[1] + A[12 + c] * gLM[2];
+      // The anchor's block keeps A_l; the Schur kernel forms W_x = V_LM,l A_l.
+      for (int k = 0; k < 18; ++k) Wx[k] = A[k];
+    }
+    for (int k = 0; k < 18; ++k) W[k] += Wx[k];
+    for (int c = 0; c < 6; ++c) r0[c] += rx[c];
   }
-  double* w = inc_w + (size_t)18 * p;
+  double* w = gw + (size_t)18 * g;
   for (int k = 0; k < 18; ++k) w[k] = W[k];
-  double* rr = inc_r0 + (size_t)6 * p;
+  double* rr = gr0 + (size_t)6 * g;
   for (int c = 0; c < 6; ++c) rr[c] = r0[c];
 }
 
-// Per LM iteration: Q_x and rhs_x of every incidence (one thread each).
-__global__ __launch_bounds__(256) void k_schur_incidences(
-    DevTopology t, int32_t n_inc, const double* __restrict__ inc_w, const double* __restrict__ inc_r0,
-    const double* __restrict__ lm_a, const double* __restrict__ lm_z, double* __restrict__ inc_q,
-    double* __restrict__ inc_rhs) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_inc) return;
-  const int32_t x = t.inc[p];
-  const int32_t l = x >= 0 ? t.obs_landmark[x] : -1 - x;
-  const double* W = inc_w + (size_t)18 * p;
-  double Z[9], Qx[18], rx[6];
-  sym3_full(lm_z + kLmZ * l, Z);
-  for (int c = 0; c < 6; ++c) rx[c] = inc_r0[(size_t)6 * p + c];
-  if (x >= 0) {
-    for (int k = 0; k < 18; ++k) Qx[k] = 0.0;
-  } else {
-    const double* A = lm_a + (size_t)18 * l;
-    for (int c = 0; c < 6; ++c)
-      for (int a = 0; a < 3; ++a) Qx[c * 3 + a] = A[a * 6 + c];
+// LDS of one batch of a tile: the staged factor blocks (W and Q^T, both 3x6
+// row-major, and the rhs contribution r0 - W^T Z g on diagonal tiles), the
+// batch's pair list and the segment's batch words. Slot entries are
+// slot-minor (entry e of slot s at [e][s]), so the staging writes of
+// consecutive threads hit consecutive banks.
+struct TileLds {
+  double f[2][18][kBatchSlots];  // [0] W, [1] Q^T = [anchor: A_l] - Z_l W
+  double r[6][kBatchSlots];
+  uint32_t pairs[kBatchPairs / 2];
+  int4 bt[kSegmentBatches];
+  uint8_t anchor[kBatchSlots];
+};
+
+struct TileArgs {
+  const int4* batches;
+  const int2* slots;
+  const uint16_t* batch_off;
+  const uint32_t* pairs;  // the pair list as 32-bit words (batches start on a word)
+  const double* gw;
+  const double* gr0;
+  const double* lm_acc;
+  const double* lm_z;
+};
+
+// A thread's index loads of one batch (two batches ahead): its slot's word,
+// its part's pair range and its share of the pair list.
+struct TileIdx {
+  int2 slot;
+  int32_t pb, pe;
+  uint32_t pw[kPairWords];
+};
+
+// A thread's data loads of one batch (one batch ahead): its slot's factor
+// block (W, or A_l for an anchor's block) and r0, its landmark's pivot (Z,
+// g) and, for an anchor, V_LM; and the index words.
+struct TileStage {
+  double w[18], z[9], v[6], r0[6];
+  int32_t pb, pe;
+  uint32_t pw[kPairWords];
+  int2 slot;
+};
+
+__device__ inline void tile_idx(TileIdx& q, const TileArgs& ta, const int4& bt, int b, int tid, int blk, int part,
+                                int np) {
+  q.slot = make_int2(-1, 0);
+  if (tid < bt.y) q.slot = ta.slots[bt.x + tid];
+  q.pb = q.pe = 0;
+  if (np) {
+    const uint16_t* off = ta.batch_off + (size_t)b * (kTileBlocks + 1);
+    q.pb = off[blk] + part;
+    q.pe = off[blk + 1];
   }
-  const double* gl = lm_z + kLmZ * l + 6;
-  for (int c = 0; c < 6; ++c) {
-    double wz[3];
-    for (int a = 0; a < 3; ++a) wz[a] = W[c] * Z[a] + W[6 + c] * Z[3 + a] + W[12 + c] * Z[6 + a];
-    for (int a = 0; a < 3; ++a) Qx[c * 3 + a] -= wz[a];
-    rx[c] -= wz[0] * gl[0] + wz[1] * gl[1] + wz[2] * gl[2];
+  const int nw = (bt.w + 1) >> 1;
+  const uint32_t* pw = ta.pairs + (bt.z >> 1);
+#pragma unroll
+  for (int k = 0; k < kPairWords; ++k) {
+    const int j = tid + k * kTileBlocks;
+    q.pw[k] = j < nw ? pw[j] : 0u;
   }
-  double* q = inc_q + (size_t)18 * p;
-  for (int k = 0; k < 18; ++k) q[k] = Qx[k];
-  double* rr = inc_rhs + (size_t)6 * p;
-  for (int c = 0; c < 6; ++c) rr[c] = rx[c];
 }
 
-constexpr int kItemPairs = 512;    // incidence pairs per work item (32 per wave pass)
-constexpr uint32_t kPairAnchor = 0x80000000u;  // the incidence is the anchor of its landmark
+__device__ inline void tile_data(TileStage& st, const TileIdx& q, const TileArgs& ta, bool diag) {
+  st.slot = q.slot;
+  st.pb = q.pb;
+  st.pe = q.pe;
+#pragma unroll
+  for (int k = 0; k < kPairWords; ++k) st.pw[k] = q.pw[k];
+  if (q.slot.x < 0) return;
+  const uint32_t lw = (uint32_t)q.slot.y;
+  const int l = (int)(lw & ~kSlotAnchor);
+  const double* wp = ta.gw + (size_t)18 * q.slot.x;
+#pragma unroll
+  for (int e = 0; e < 18; ++e) st.w[e] = wp[e];
+  const double* zp = ta.lm_z + (size_t)kLmZ * l;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) st.z[e] = zp[e];
+  if (lw & kSlotAnchor) {
+    const double* vp = ta.lm_acc + (size_t)kLmAcc * l + 6;
+#pragma unroll
+    for (int e = 0; e < 6; ++e) st.v[e] = vp[e];
+  }
+  if (diag) {
+    const double* rp = ta.gr0 + (size_t)6 * q.slot.x;
+#pragma unroll
+    for (int e = 0; e < 6; ++e) st.r0[e] = rp[e];
+  }
+}
 
-// Partial sums of S blocks over work items = (upper block u, a range of at
-// most kItemPairs of its incidence pairs), one wave per item:
-//   sum over pairs (x in row i, y in row j, same landmark) of
-//     Q_x W_y + [x observation, y anchor] W_x^T A_l.
-// Four lanes share a pair, lane quadrant q owning rows 3 (q >> 1) .. +2 and
-// columns 3 (q & 1) .. +2 of the 6x6 block; the 16 lanes of a quadrant are
-// summed in a fixed butterfly order. Pair entries are incidence indices
-// (| kPairAnchor). Items come in dispatch order (bands of a few block rows
-// swept together column by column, so that the band's Q factors and one
-// column's W factors are reused from L2) and carry the slot of their partial
-// sum; with xcd_map, work-groups are mapped so that each XCD walks a
-// contiguous range of the dispatch order (work-group b runs on XCD b mod 8).
-// The dispatch order, kWaves (items per work-group) and xcd_map are a
-// schedule only: every item's partial is the same whichever is chosen.
-template <int kWaves>
-__global__ __launch_bounds__(64 * kWaves) void k_schur_items(
-    DevTopology t, int32_t n_items, int32_t xcd_map, const int4* __restrict__ items,
-    const uint2* __restrict__ pairs, const double* __restrict__ inc_q, const double* __restrict__ inc_w,
-    const double* __restrict__ lm_a, double* __restrict__ partial) {
+// One work item = one segment of a tile (I, J). Per batch: every thread
+// stages its slot's factor block in LDS (Q^T formed from W, Z and A) and its
+// share of the pair list, then each thread adds the products of its part of
+// its block's pairs in list order,
+//   Q_x W_y              (y not the anchor, or x the anchor too)
+//   W_x^T (Q^T_y)        (y the anchor of an observation x: Q_x W_y + W_x^T A_l)
+// both as sum_a Lt(a, r) Rt(a, c) over the rows of two 3x6 blocks; a part of
+// a diagonal block of a diagonal tile also adds the rhs of its pairs' block.
+// Loads are software-pipelined: while a batch's products run, the next
+// batch's data loads and the batch after that's index loads are in flight.
+// Each part's sums go to its own slot of the segment's partial. Work-groups
+// take items_per_wg consecutive items of the dispatch order; with xcd_map
+// each XCD walks a contiguous range of work-groups (work-group b runs on XCD
+// b mod 8). Neither changes a partial.
+__global__ __launch_bounds__(kTileBlocks) __attribute__((amdgpu_waves_per_eu(2))) void k_schur_tiles(
+    int32_t n_segs, int32_t items_per_wg, int32_t xcd_map, const int32_t* __restrict__ seg_order,
+    const int4* __restrict__ segs, const uint32_t* __restrict__ seg_thread, TileArgs ta,
+    double* __restrict__ partial) {
+  __shared__ TileLds sh;
+  const int wgs = (n_segs + items_per_wg - 1) / items_per_wg;
   const int per_xcd = gridDim.x / 8;
-  const int wg = xcd_map ? (blockIdx.x % 8) * per_xcd + blockIdx.x / 8 : (int)blockIdx.x;
-  const int it = wg * kWaves + (threadIdx.x >> 6);
-  if (it >= n_items) return;
-  const int lane = threadIdx.x & 63, slot = lane >> 2, quad = lane & 3;
-  const int r0 = 3 * (quad >> 1), c0 = 3 * (quad & 1);
-  const int4 item = items[it];  // (u, first pair, end pair, partial slot)
-  double acc[9];
-  for (int k = 0; k < 9; ++k) acc[k] = 0.0;
-  // Two pairs per lane and pass (slots p and p + 16), and the pair indices of
-  // the next pass requested before this pass's products: each pass waits for
-  // one round of loads instead of four dependent ones.
-  const uint2 none = make_uint2(0u, 0u);
-  int p = item.y + slot;
-  uint2 n0 = p < item.z ? pairs[p] : none, n1 = p + 16 < item.z ? pairs[p + 16] : none;
-  for (; p < item.z; p += 32) {
-    const uint2 pr0 = n0, pr1 = n1;
-    const bool has1 = p + 16 < item.z;
-    const int32_t x0 = (int32_t)(pr0.x & ~kPairAnchor), y0 = (int32_t)(pr0.y & ~kPairAnchor);
-    const int32_t x1 = (int32_t)(pr1.x & ~kPairAnchor), y1 = (int32_t)(pr1.y & ~kPairAnchor);
-    const double* Q0 = inc_q + (size_t)18 * x0 + 3 * r0;
-    const double* W0 = inc_w + (size_t)18 * y0 + c0;
-    const double* Q1 = inc_q + (size_t)18 * x1 + 3 * r0;
-    const double* W1 = inc_w + (size_t)18 * y1 + c0;
-    double q[18], W[18];
+  const int wg = xcd_map ? (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8) : (int)blockIdx.x;
+  if (wg >= wgs) return;
+  const int tid = threadIdx.x;
+  for (int it = 0; it < items_per_wg; ++it) {
+    const int si = wg * items_per_wg + it;
+    if (si >= n_segs) break;
+    const int sid = seg_order[si];
+    const int4 sg = segs[sid];
+    const bool diag = sg.z == sg.w;
+    const int nb = sg.y - sg.x;
+    const uint32_t tw = seg_thread[(size_t)sid * kTileBlocks + tid];
+    const int np = seg_thread_parts(tw), blk = seg_thread_block(tw), part = seg_thread_part(tw);
+    const bool rhs_part = diag && np && blk / kTileBand == blk % kTileBand;
+    __syncthreads();
+    if (tid < nb) sh.bt[tid] = ta.batches[sg.x + tid];
+    __syncthreads();
+    double acc[36], racc[6];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) q[k] = Q0[k];
+    for (int e = 0; e < 36; ++e) acc[e] = 0.0;
 #pragma unroll
-    for (int aa = 0; aa < 3; ++aa)
+    for (int c = 0; c < 6; ++c) racc[c] = 0.0;
+    TileIdx q;
+    TileStage st;
+    tile_idx(q, ta, sh.bt[0], sg.x, tid, blk, part, np);
+    tile_data(st, q, ta, diag);
+    if (nb > 1) tile_idx(q, ta, sh.bt[1], sg.x + 1, tid, blk, part, np);
+    for (int k = 0; k < nb; ++k) {
+      __syncthreads();
+      if (st.slot.x >= 0) {
+        // Column by column: W (= V_LM A_l for an anchor, whose block holds
+        // A_l), Q^T = [anchor: A_l] - Z W and r0 - W^T Z g.
+        const double* z = st.z;
+        const double Z[9] = {z[0], z[1], z[2], z[1], z[3], z[4], z[2], z[4], z[5]};
+        const bool anc = ((uint32_t)st.slot.y & kSlotAnchor) != 0;
+        const double* v = st.v;
+        const double V[9] = {v[0], v[1], v[2], v[1], v[3], v[4], v[2], v[4], v[5]};
+        double zg[3];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) W[aa * 3 + d] = W0[aa * 6 + d];
-    if (has1) {
+        for (int a = 0; a < 3; ++a) zg[a] = Z[a * 3] * z[6] + Z[a * 3 + 1] * z[7] + Z[a * 3 + 2] * z[8];
 #pragma unroll
-      for (int k = 0; k < 9; ++k) q[9 + k] = Q1[k];
+        for (int c = 0; c < 6; ++c) {
+          const double b0 = st.w[c], b1 = st.w[6 + c], b2 = st.w[12 + c];
+          double w0 = b0, w1 = b1, w2 = b2;
+          if (anc) {
+            w0 = V[0] * b0 + V[1] * b1 + V[2] * b2;
+            w1 = V[3] * b0 + V[4] * b1 + V[5] * b2;
+            w2 = V[6] * b0 + V[7] * b1 + V[8] * b2;
+          }
+          const double q0 = Z[0] * w0 + Z[1] * w1 + Z[2] * w2;
+          const double q1 = Z[3] * w0 + Z[4] * w1 + Z[5] * w2;
+          const double q2 = Z[6] * w0 + Z[7] * w1 + Z[8] * w2;
+          sh.f[0][c][tid] = w0;
+          sh.f[0][6 + c][tid] = w1;
+          sh.f[0][12 + c][tid] = w2;
+          sh.f[1][c][tid] = anc ? b0 - q0 : -q0;
+          sh.f[1][6 + c][tid] = anc ? b1 - q1 : -q1;
+          sh.f[1][12 + c][tid] = anc ? b2 - q2 : -q2;
+          if (diag) sh.r[c][tid] = st.r0[c] - (w0 * zg[0] + w1 * zg[1] + w2 * zg[2]);
+        }
+        sh.anchor[tid] = anc ? 1 : 0;
+      }
 #pragma unroll
-      for (int aa = 0; aa < 3; ++aa)
+      for (int j = 0; j < kPairWords; ++j) sh.pairs[tid + j * kTileBlocks] = st.pw[j];
+      const int pb = st.pb, pe = st.pe;
+      __syncthreads();
+      if (k + 1 < nb) tile_data(st, q, ta, diag);
+      if (k + 2 < nb) tile_idx(q, ta, sh.bt[k + 2], sg.x + k + 2, tid, blk, part, np);
+      const uint16_t* pl = reinterpret_cast<const uint16_t*>(sh.pairs);
+      for (int p = pb; p < pe; p += np) {
+        const uint32_t pr = pl[p];
+        const int x = (int)(pr & 255u), y = (int)(pr >> 8);
+        const int sel = (sh.anchor[y] && !sh.anchor[x]) ? 1 : 0;
+        const double* Lt = &sh.f[1 - sel][0][x];
+        const double* Rt = &sh.f[sel][0][y];
+#pragma unroll 1
+        for (int a = 0; a < 3; ++a) {
+          double l6[6], r6[6];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) W[9 + aa * 3 + d] = W1[aa * 6 + d];
-    } else {
+          for (int c = 0; c < 6; ++c) {
+            l6[c] = Lt[(a * 6 + c) * kBatchSlots];
+            r6[c] = Rt[(a * 6 + c) * kBatchSlots];
+          }
 #pragma unroll
-      for (int k = 0; k < 9; ++k) q[9 + k] = W[9 + k] = 0.0;
-    }
-    n0 = p + 32 < item.z ? pairs[p + 32] : none;
-    n1 = p + 48 < item.z ? pairs[p + 48] : none;
+          for (int r = 0; r < 6; ++r)
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
+            for (int c = 0; c < 6; ++c) acc[r * 6 + c] += l6[r] * r6[c];
+        }
+        if (rhs_part)
 #pragma unroll
-      for (int rr = 0; rr < 3; ++rr)
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-          acc[rr * 3 + d] += q[9 * h + rr * 3] * W[9 * h + d] + q[9 * h + rr * 3 + 1] * W[9 * h + 3 + d] +
-                             q[9 * h + rr * 3 + 2] * W[9 * h + 6 + d];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const uint2 pr = h ? pr1 : pr0;
-      if ((h == 0 || has1) && (pr.y & kPairAnchor) && !(pr.x & kPairAnchor)) {
-        // direct observer-anchor coupling W_x^T A_l
-        const int32_t x = h ? x1 : x0, y = h ? y1 : y0;
-        const double* A = lm_a + (size_t)18 * (-1 - t.inc[y]) + c0;
-        const double* Wx = inc_w + (size_t)18 * x + r0;
-        for (int rr = 0; rr < 3; ++rr)
-          for (int d = 0; d < 3; ++d)
-            acc[rr * 3 + d] += Wx[rr] * A[d] + Wx[6 + rr] * A[6 + d] + Wx[12 + rr] * A[12 + d];
+          for (int c = 0; c < 6; ++c) racc[c] += sh.r[c][x];
       }
     }
-  }
-  for (int k = 0; k < 9; ++k) {
-    double v = acc[k];
-    for (int off = 4; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
-    acc[k] = v;
-  }
-  if (lane < 4) {
-    double* out = partial + (size_t)36 * item.w;
-    for (int rr = 0; rr < 3; ++rr)
-      for (int d = 0; d < 3; ++d) out[(r0 + rr) * 6 + c0 + d] = acc[rr * 3 + d];
+    if (np) {
+      double* out = partial + (size_t)sid * kTilePartial + (size_t)tid * kPartEntries;
+#pragma unroll
+      for (int e = 0; e < 36; ++e) out[e] = acc[e];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) out[36 + c] = racc[c];
+    }
   }
 }
 
-// S_ij = sum of the block's item partials in item order, plus on S_ii the
-// direct observation term C_i (sum Jo^T Jo) C_i and the LM diagonal
-// D^2 / radius; written to S_ij and S_ji = S_ij^T. One thread per entry.
+// Sum of block (i, j) entry e over its parts (in part order) and its tile's
+// segments (in segment order).
+__device__ inline double tile_block_sum(const int2* __restrict__ tile_seg, int32_t nbands,
+                                        const uint16_t* __restrict__ seg_block, const double* __restrict__ partial,
+                                        int i, int j, int e) {
+  const int2 sr = tile_seg[(i / kTileBand) * nbands + j / kTileBand];
+  const int blk = (i % kTileBand) * kTileBand + j % kTileBand;
+  double v = 0.0;
+  for (int sg = sr.x; sg < sr.y; ++sg) {
+    const uint32_t bw = seg_block[(size_t)sg * kTileBlocks + blk];
+    const double* p = partial + (size_t)sg * kTilePartial + (size_t)(bw & 255u) * kPartEntries + e;
+    for (int q = 0; q < (int)(bw >> 8); ++q) v += p[(size_t)q * kPartEntries];
+  }
+  return v;
+}
+
+// S_ij = sum of the block's tile partials (parts in part order, segments in
+// segment order), plus on S_ii the direct observation term C_i (sum Jo^T Jo)
+// C_i and the LM diagonal D^2 / radius; written to S_ij and S_ji = S_ij^T.
+// One thread per entry.
 __global__ __launch_bounds__(252) void k_schur_finalize(int32_t nub, const int4* __restrict__ ublk,
-                                                        const int32_t* __restrict__ item_ptr,
+                                                        int32_t nbands, const int2* __restrict__ tile_seg,
+                                                        const uint16_t* __restrict__ seg_block,
                                                         const double* __restrict__ partial,
                                                         const double* __restrict__ pose_scale,
                                                         const double* __restrict__ pose_diag,
@@ -672,8 +777,7 @@ __global__ __launch_bounds__(252) void k_schur_finalize(int32_t nub, const int4*
   const int u = blockIdx.x * 7 + threadIdx.x / 36, e = threadIdx.x % 36;
   if (u >= nub) return;
   const int4 ub = ublk[u];
-  double v = 0.0;
-  for (int it = item_ptr[u]; it < item_ptr[u + 1]; ++it) v += partial[(size_t)36 * it + e];
+  double v = tile_block_sum(tile_seg, nbands, seg_block, partial, ub.z, ub.w, e);
   const int r = e / 6, c = e % 6;
   if (ub.z == ub.w) {
     const int i = ub.z;
@@ -687,21 +791,14 @@ __global__ __launch_bounds__(252) void k_schur_finalize(int32_t nub, const int4*
   if (ub.y != ub.x) S[(size_t)36 * ub.y + c * 6 + r] = v;
 }
 
-// rhs_i = sum of rhs_x over the row's incidences, one work-group per row.
-__global__ __launch_bounds__(256) void k_schur_rhs(DevTopology t, const double* __restrict__ inc_rhs,
-                                                   double* __restrict__ rhs) {
-  __shared__ double red[4][6];
-  const int i = blockIdx.x;
-  double g[6] = {0, 0, 0, 0, 0, 0};
-  for (int k = t.inc_ptr[i] + threadIdx.x; k < t.inc_ptr[i + 1]; k += 256)
-    for (int c = 0; c < 6; ++c) g[c] += inc_rhs[(size_t)6 * k + c];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int c = 0; c < 6; ++c) g[c] = wave_sum(g[c]);
-  if (lane == 0)
-    for (int c = 0; c < 6; ++c) red[wave][c] = g[c];
-  __syncthreads();
-  if (threadIdx.x < 6)
-    rhs[6 * i + threadIdx.x] = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+// rhs_i = the rhs sums of the diagonal block (i, i)'s tile partials.
+__global__ __launch_bounds__(256) void k_schur_rhs(int32_t Pvar, int32_t nbands, const int2* __restrict__ tile_seg,
+                                                   const uint16_t* __restrict__ seg_block,
+                                                   const double* __restrict__ partial, double* __restrict__ rhs) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= 6 * Pvar) return;
+  const int i = k / 6;
+  rhs[k] = tile_block_sum(tile_seg, nbands, seg_block, partial, i, i, 36 + k % 6);
 }
 
 // ------------------------------------------------------------ landmark admission
@@ -1448,57 +1545,41 @@ void launch_landmark_prep(hipStream_t s, const DevTopology& t, const double* lm_
   if (t.L > 0)
     k_landmark_prep<<<(t.L + 255) / 256, 256, 0, s>>>(t, lm_acc, lm_scale, inv_mu, min_diag, max_diag, lm_z, fail);
 }
-void launch_schur_factors(hipStream_t s, const DevTopology& t, int32_t n_inc, const int32_t* inc_row,
-                          const double* vR, const double* lm, const double* obs_rec, const double* lm_acc,
-                          const double* pose_scale, double* inc_w, double* inc_r0, double* lm_a, bool f32) {
-  if (t.Pvar <= 0 || n_inc <= 0) return;
+void launch_schur_factors(hipStream_t s, const DevTopology& t, int32_t n_g, const int32_t* g_row,
+                          const int32_t* g_inc_ptr, const int32_t* g_inc, const double* vR, const double* lm,
+                          const double* obs_rec, const double* lm_acc, const double* pose_scale, double* gw,
+                          double* gr0, bool f32) {
+  if (t.Pvar <= 0 || n_g <= 0) return;
   if (f32)
-    k_schur_factors<true><<<(n_inc + 255) / 256, 256, 0, s>>>(t, n_inc, inc_row, vR, lm, obs_rec, lm_acc,
-                                                              pose_scale, inc_w, inc_r0, lm_a);
+    k_schur_factors<true><<<(n_g + 255) / 256, 256, 0, s>>>(t, n_g, g_row, g_inc_ptr, g_inc, vR, lm, obs_rec,
+                                                            lm_acc, pose_scale, gw, gr0);
   else
-    k_schur_factors<false><<<(n_inc + 255) / 256, 256, 0, s>>>(t, n_inc, inc_row, vR, lm, obs_rec, lm_acc,
-                                                               pose_scale, inc_w, inc_r0, lm_a);
+    k_schur_factors<false><<<(n_g + 255) / 256, 256, 0, s>>>(t, n_g, g_row, g_inc_ptr, g_inc, vR, lm, obs_rec,
+                                                             lm_acc, pose_scale, gw, gr0);
 }
-void launch_schur_incidences(hipStream_t s, const DevTopology& t, int32_t n_inc, const double* inc_w,
-                             const double* inc_r0, const double* lm_a, const double* lm_z, double* inc_q,
-                             double* inc_rhs) {
-  if (t.Pvar > 0 && n_inc > 0)
-    k_schur_incidences<<<(n_inc + 255) / 256, 256, 0, s>>>(t, n_inc, inc_w, inc_r0, lm_a, lm_z, inc_q, inc_rhs);
-}
-template <int kWaves>
-void launch_items_w(hipStream_t s, const DevTopology& t, int32_t n_items, int32_t xcd_map, const int4* items,
-                    const uint2* pairs, const double* inc_q, const double* inc_w, const double* lm_a,
-                    double* partial) {
-  const int wgs = (n_items + kWaves - 1) / kWaves;
+bool schur_items_per_wg_supported(int32_t n) { return n == 1 || n == 2 || n == 4 || n == 8; }
+void launch_schur_tiles(hipStream_t s, const SchurTilesDev& d, const double* gw, const double* gr0,
+                        const double* lm_acc, const double* lm_z, double* partial, int32_t items_per_wg,
+                        int32_t xcd_map) {
+  if (d.n_segs <= 0) return;
+  const int wgs = (d.n_segs + items_per_wg - 1) / items_per_wg;
   const int per_xcd = (wgs + 7) / 8;
-  k_schur_items<kWaves><<<8 * per_xcd, 64 * kWaves, 0, s>>>(t, n_items, xcd_map, items, pairs, inc_q, inc_w, lm_a,
-                                                           partial);
+  const TileArgs ta{d.batches, d.slots, d.batch_off, reinterpret_cast<const uint32_t*>(d.pairs), gw, gr0, lm_acc,
+                    lm_z};
+  k_schur_tiles<<<8 * per_xcd, kTileBlocks, 0, s>>>(d.n_segs, items_per_wg, xcd_map, d.seg_order, d.segs,
+                                                     d.seg_thread, ta, partial);
 }
-bool schur_item_waves_supported(int32_t waves) { return waves == 1 || waves == 2 || waves == 4 || waves == 8; }
-void launch_schur_items(hipStream_t s, const DevTopology& t, int32_t n_items, const int4* items, const uint2* pairs,
-                        const double* inc_q, const double* inc_w, const double* lm_a, double* partial,
-                        int32_t waves, int32_t xcd_map) {
-  if (n_items <= 0) return;
-  switch (waves) {
-    case 1: launch_items_w<1>(s, t, n_items, xcd_map, items, pairs, inc_q, inc_w, lm_a, partial); break;
-    case 2: launch_items_w<2>(s, t, n_items, xcd_map, items, pairs, inc_q, inc_w, lm_a, partial); break;
-    case 8: launch_items_w<8>(s, t, n_items, xcd_map, items, pairs, inc_q, inc_w, lm_a, partial); break;
-    default: launch_items_w<4>(s, t, n_items, xcd_map, items, pairs, inc_q, inc_w, lm_a, partial); break;
-  }
-}
-void launch_schur_blocks(hipStream_t s, const DevTopology& t, int32_t nub, const int4* ublk, int32_t n_items,
-                         const int4* items, const int32_t* item_ptr, const uint2* pairs, const double* inc_q,
-                         const double* inc_w, const double* inc_rhs, const double* lm_a, const double* pose_scale,
-                         const double* pose_diag, const double* pose_jtj, double inv_mu, double min_diag,
-                         double max_diag, double* partial, double* S, double* rhs, int32_t waves, int32_t xcd_map) {
+void launch_schur_blocks(hipStream_t s, const DevTopology& t, const SchurTilesDev& d, int32_t nub, const int4* ublk,
+                         const double* gw, const double* gr0, const double* lm_acc, const double* lm_z,
+                         const double* pose_scale, const double* pose_diag, const double* pose_jtj, double inv_mu,
+                         double min_diag, double max_diag, double* partial, double* S, double* rhs,
+                         int32_t items_per_wg, int32_t xcd_map) {
   if (t.Pvar <= 0 || nub <= 0) return;
-  k_schur_rhs<<<t.Pvar, 256, 0, s>>>(t, inc_rhs, rhs);
-  launch_schur_items(s, t, n_items, items, pairs, inc_q, inc_w, lm_a, partial, waves, xcd_map);
-  k_schur_finalize<<<(nub + 6) / 7, 252, 0, s>>>(nub, ublk, item_ptr, partial, pose_scale, pose_diag, pose_jtj,
-                                                 inv_mu, min_diag, max_diag, S);
+  launch_schur_tiles(s, d, gw, gr0, lm_acc, lm_z, partial, items_per_wg, xcd_map);
+  k_schur_finalize<<<(nub + 6) / 7, 252, 0, s>>>(nub, ublk, d.nbands, d.tile_seg, d.seg_block, partial, pose_scale,
+                                                 pose_diag, pose_jtj, inv_mu, min_diag, max_diag, S);
+  k_schur_rhs<<<(6 * t.Pvar + 255) / 256, 256, 0, s>>>(t.Pvar, d.nbands, d.tile_seg, d.seg_block, partial, rhs);
 }
-int32_t schur_item_pairs() { return kItemPairs; }
-uint32_t schur_pair_anchor_bit() { return kPairAnchor; }
 void launch_landmark_quality(hipStream_t s, const DevTopology& t, const double* vert, const double* vR,
                              const double* lm, int32_t min_observers, double min_angle_deg, double max_dist,
                              double min_dist, double max_reproj_px, uint8_t* good) {
