@@ -102,19 +102,19 @@ struct DevTopology {
 
 // Device copy of the reduced camera system's work decomposition (ba_tiles.h):
 // segments (in dispatch order through seg_order) with their thread parts,
-// batches with their slot and pair lists, and the segment range of every
-// tile.
+// batches with their slot and pair lists, and for every upper block of S
+// the partial slots (segment * kTileBlocks + thread) summed into it, in order.
 struct SchurTilesDev {
-  int32_t n_segs, nbands;
+  int32_t n_segs;
   const int32_t* seg_order;
   const int4* segs;
   const uint32_t* seg_thread;
-  const uint16_t* seg_block;
   const int4* batches;
   const int2* slots;
   const uint16_t* batch_off;
   const uint16_t* pairs;
-  const int2* tile_seg;
+  const int32_t* uslot_ptr;
+  const int32_t* uslot;
 };
 
 // Loss of the whitened residual.

@@ -201,8 +201,8 @@ struct mlba_handle {
   DBuf<int4> t_batches, t_segs;
   DBuf<int2> t_slots;
   DBuf<uint32_t> t_seg_thread;
-  DBuf<uint16_t> t_batch_off, t_pairs, t_seg_block;
-  DBuf<int2> t_tile_seg;
+  DBuf<uint16_t> t_batch_off, t_pairs;
+  DBuf<int32_t> t_uslot_ptr, t_uslot;   // partial slots of every upper block
   DBuf<int32_t> t_seg_order;            // segments in dispatch order
   DBuf<double> gw, gr0;                 // per-linearization factor blocks (W or A_l, r0)
   DBuf<double> schur_partial;           // kTilePartial per segment
@@ -499,6 +499,19 @@ int build_topology(mlba_handle* h) {
   h->n_staged = st.staged;
   h->h_segs.assign(reinterpret_cast<const int4*>(st.segs.data()),
                    reinterpret_cast<const int4*>(st.segs.data()) + h->n_segs);
+  // Partial slots of each upper block: its tile's segments in order, each
+  // segment's parts of the block in order.
+  std::vector<int32_t> uslot_ptr(ublk.size() + 1, 0), uslot;
+  for (size_t u = 0; u < ublk.size(); ++u) {
+    const int i = ublk[u].z, j = ublk[u].w;
+    const int64_t t = (int64_t)(i / kTileBand) * st.nbands + j / kTileBand;
+    const int blk = (i % kTileBand) * kTileBand + j % kTileBand;
+    for (int sg = st.tile_seg[2 * t]; sg < st.tile_seg[2 * t + 1]; ++sg) {
+      const uint16_t bw = st.seg_block[(size_t)sg * kTileBlocks + blk];
+      for (int q = 0; q < (bw >> 8); ++q) uslot.push_back(sg * kTileBlocks + (bw & 255) + q);
+    }
+    uslot_ptr[u + 1] = (int32_t)uslot.size();
+  }
   // Learned preconditioner structure: row and transposed position of every
   // block of S, and the landmark pairs summed into it (shared observations).
   std::vector<int32_t> lp_pos_row(h->nnzb), lp_tpos(h->nnzb), lp_shared(h->nnzb, 0);
@@ -538,14 +551,13 @@ int build_topology(mlba_handle* h) {
   if ((rc = upload(h, h->g_inc, st.g_inc.data(), st.g_inc.size()))) return rc;
   if ((rc = upload(h, h->t_slots, reinterpret_cast<const int2*>(st.slots.data()), st.slots.size() / 2))) return rc;
   if ((rc = upload(h, h->t_seg_thread, st.seg_thread.data(), st.seg_thread.size()))) return rc;
-  if ((rc = upload(h, h->t_seg_block, st.seg_block.data(), st.seg_block.size()))) return rc;
+  if ((rc = upload(h, h->t_uslot_ptr, uslot_ptr.data(), uslot_ptr.size()))) return rc;
+  if ((rc = upload(h, h->t_uslot, uslot.data(), uslot.size()))) return rc;
   if ((rc = upload(h, h->t_batches, reinterpret_cast<const int4*>(st.batches.data()), st.batches.size() / 4)))
     return rc;
   if ((rc = upload(h, h->t_segs, reinterpret_cast<const int4*>(st.segs.data()), st.segs.size() / 4))) return rc;
   if ((rc = upload(h, h->t_batch_off, st.batch_off.data(), st.batch_off.size()))) return rc;
   if ((rc = upload(h, h->t_pairs, st.pairs.data(), st.pairs.size()))) return rc;
-  if ((rc = upload(h, h->t_tile_seg, reinterpret_cast<const int2*>(st.tile_seg.data()), st.tile_seg.size() / 2)))
-    return rc;
   if ((rc = upload_item_order(h))) return rc;
   if ((rc = upload(h, h->obs_col, obs_col.data(), obs_col.size()))) return rc;
   const size_t T = h->tiles, V6 = 6 * (size_t)std::max(Pvar, 1);
@@ -577,9 +589,9 @@ int build_topology(mlba_handle* h) {
   t.row_ptr = h->row_ptr.p; t.col_var = h->col_var.p; t.diag_pos = h->diag_pos.p; t.inc_ptr = h->inc_ptr.p;
   t.inc = h->inc.p; t.max_row_blocks = h->max_row_blocks;
   SchurTilesDev& d = h->tdev;
-  d.n_segs = h->n_segs; d.nbands = h->nbands; d.seg_order = h->t_seg_order.p; d.segs = h->t_segs.p;
-  d.seg_thread = h->t_seg_thread.p; d.seg_block = h->t_seg_block.p; d.batches = h->t_batches.p;
-  d.slots = h->t_slots.p; d.batch_off = h->t_batch_off.p; d.pairs = h->t_pairs.p; d.tile_seg = h->t_tile_seg.p;
+  d.n_segs = h->n_segs; d.seg_order = h->t_seg_order.p; d.segs = h->t_segs.p; d.seg_thread = h->t_seg_thread.p;
+  d.batches = h->t_batches.p; d.slots = h->t_slots.p; d.batch_off = h->t_batch_off.p; d.pairs = h->t_pairs.p;
+  d.uslot_ptr = h->t_uslot_ptr.p; d.uslot = h->t_uslot.p;
   HIP_OK(hipStreamSynchronize(h->stream));
   return MLBA_OK;
 }

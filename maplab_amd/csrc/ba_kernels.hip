@@ -746,38 +746,31 @@ __global__ __launch_bounds__(kTileBlocks) __attribute__((amdgpu_waves_per_eu(2))
   }
 }
 
-// Sum of block (i, j) entry e over its parts (in part order) and its tile's
-// segments (in segment order).
-__device__ inline double tile_block_sum(const int2* __restrict__ tile_seg, int32_t nbands,
-                                        const uint16_t* __restrict__ seg_block, const double* __restrict__ partial,
-                                        int i, int j, int e) {
-  const int2 sr = tile_seg[(i / kTileBand) * nbands + j / kTileBand];
-  const int blk = (i % kTileBand) * kTileBand + j % kTileBand;
-  double v = 0.0;
-  for (int sg = sr.x; sg < sr.y; ++sg) {
-    const uint32_t bw = seg_block[(size_t)sg * kTileBlocks + blk];
-    const double* p = partial + (size_t)sg * kTilePartial + (size_t)(bw & 255u) * kPartEntries + e;
-    for (int q = 0; q < (int)(bw >> 8); ++q) v += p[(size_t)q * kPartEntries];
-  }
-  return v;
-}
-
-// S_ij = sum of the block's tile partials (parts in part order, segments in
-// segment order), plus on S_ii the direct observation term C_i (sum Jo^T Jo)
-// C_i and the LM diagonal D^2 / radius; written to S_ij and S_ji = S_ij^T.
-// One thread per entry.
+// S_ij = sum of the block's part slots (segments in segment order, parts in
+// part order, listed per upper block), plus on S_ii the direct observation
+// term C_i (sum Jo^T Jo) C_i and the LM diagonal D^2 / radius; written to
+// S_ij and S_ji = S_ij^T. One thread per entry; the first six threads of a
+// diagonal block also sum the rhs of its row.
 __global__ __launch_bounds__(252) void k_schur_finalize(int32_t nub, const int4* __restrict__ ublk,
-                                                        int32_t nbands, const int2* __restrict__ tile_seg,
-                                                        const uint16_t* __restrict__ seg_block,
+                                                        const int32_t* __restrict__ uslot_ptr,
+                                                        const int32_t* __restrict__ uslot,
                                                         const double* __restrict__ partial,
                                                         const double* __restrict__ pose_scale,
                                                         const double* __restrict__ pose_diag,
                                                         const double* __restrict__ pose_jtj, double inv_mu,
-                                                        double min_diag, double max_diag, double* __restrict__ S) {
+                                                        double min_diag, double max_diag, double* __restrict__ S,
+                                                        double* __restrict__ rhs) {
   const int u = blockIdx.x * 7 + threadIdx.x / 36, e = threadIdx.x % 36;
   if (u >= nub) return;
   const int4 ub = ublk[u];
-  double v = tile_block_sum(tile_seg, nbands, seg_block, partial, ub.z, ub.w, e);
+  const int kb = uslot_ptr[u], ke = uslot_ptr[u + 1];
+  const bool rhs_entry = ub.z == ub.w && e < 6;
+  double v = 0.0, g = 0.0;
+  for (int k = kb; k < ke; ++k) {
+    const double* p = partial + (size_t)uslot[k] * kPartEntries;
+    v += p[e];
+    if (rhs_entry) g += p[36 + e];
+  }
   const int r = e / 6, c = e % 6;
   if (ub.z == ub.w) {
     const int i = ub.z;
@@ -786,19 +779,10 @@ __global__ __launch_bounds__(252) void k_schur_finalize(int32_t nub, const int4*
       const double sc = pose_scale[6 * i + c];
       v += fmin(fmax(sc * sc * pose_diag[6 * i + c], min_diag), max_diag) * inv_mu;
     }
+    if (rhs_entry) rhs[6 * i + e] = g;
   }
   S[(size_t)36 * ub.x + e] = v;
   if (ub.y != ub.x) S[(size_t)36 * ub.y + c * 6 + r] = v;
-}
-
-// rhs_i = the rhs sums of the diagonal block (i, i)'s tile partials.
-__global__ __launch_bounds__(256) void k_schur_rhs(int32_t Pvar, int32_t nbands, const int2* __restrict__ tile_seg,
-                                                   const uint16_t* __restrict__ seg_block,
-                                                   const double* __restrict__ partial, double* __restrict__ rhs) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= 6 * Pvar) return;
-  const int i = k / 6;
-  rhs[k] = tile_block_sum(tile_seg, nbands, seg_block, partial, i, i, 36 + k % 6);
 }
 
 // ------------------------------------------------------------ landmark admission
@@ -1576,9 +1560,8 @@ void launch_schur_blocks(hipStream_t s, const DevTopology& t, const SchurTilesDe
                          int32_t items_per_wg, int32_t xcd_map) {
   if (t.Pvar <= 0 || nub <= 0) return;
   launch_schur_tiles(s, d, gw, gr0, lm_acc, lm_z, partial, items_per_wg, xcd_map);
-  k_schur_finalize<<<(nub + 6) / 7, 252, 0, s>>>(nub, ublk, d.nbands, d.tile_seg, d.seg_block, partial, pose_scale,
-                                                 pose_diag, pose_jtj, inv_mu, min_diag, max_diag, S);
-  k_schur_rhs<<<(6 * t.Pvar + 255) / 256, 256, 0, s>>>(t.Pvar, d.nbands, d.tile_seg, d.seg_block, partial, rhs);
+  k_schur_finalize<<<(nub + 6) / 7, 252, 0, s>>>(nub, ublk, d.uslot_ptr, d.uslot, partial, pose_scale, pose_diag,
+                                                 pose_jtj, inv_mu, min_diag, max_diag, S, rhs);
 }
 void launch_landmark_quality(hipStream_t s, const DevTopology& t, const double* vert, const double* vR,
                              const double* lm, int32_t min_observers, double min_angle_deg, double max_dist,
