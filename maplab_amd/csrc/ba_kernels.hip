@@ -766,7 +766,27 @@ __global__ __launch_bounds__(252) void k_schur_finalize(int32_t nub, const int4*
   const int kb = uslot_ptr[u], ke = uslot_ptr[u + 1];
   const bool rhs_entry = ub.z == ub.w && e < 6;
   double v = 0.0, g = 0.0;
-  for (int k = kb; k < ke; ++k) {
+  int k = kb;
+  // Four slots' loads in flight at a time; the sums keep slot order.
+  for (; k + 4 <= ke; k += 4) {
+    const double* p0 = partial + (size_t)uslot[k] * kPartEntries;
+    const double* p1 = partial + (size_t)uslot[k + 1] * kPartEntries;
+    const double* p2 = partial + (size_t)uslot[k + 2] * kPartEntries;
+    const double* p3 = partial + (size_t)uslot[k + 3] * kPartEntries;
+    const double a0 = p0[e], a1 = p1[e], a2 = p2[e], a3 = p3[e];
+    v += a0;
+    v += a1;
+    v += a2;
+    v += a3;
+    if (rhs_entry) {
+      const double b0 = p0[36 + e], b1 = p1[36 + e], b2 = p2[36 + e], b3 = p3[36 + e];
+      g += b0;
+      g += b1;
+      g += b2;
+      g += b3;
+    }
+  }
+  for (; k < ke; ++k) {
     const double* p = partial + (size_t)uslot[k] * kPartEntries;
     v += p[e];
     if (rhs_entry) g += p[36 + e];
