@@ -82,6 +82,17 @@ int32_t mlba_rcs_records(mlba_handle* handle, const mlba_options* options, doubl
                          int32_t* col, int32_t* rev, float* node_features, float* edge_features, float* qc);
 
 /*
+ * The RCS itself at the current state with trust-region radius `radius`:
+ * S in block CSR by row (row_ptr rows+1, col slots, blocks 36 slots, each
+ * 6x6 row-major; full symmetric storage) and its right-hand side (6 rows), in
+ * the Jacobi-scaled variables of an LM iteration (no counterpart in the
+ * reference: the system Ceres' Schur eliminator forms internally,
+ * schur_eliminator_impl.h). The state is not changed.
+ */
+int32_t mlba_rcs_system(mlba_handle* handle, const mlba_options* options, double radius, int32_t* row_ptr,
+                        int32_t* col, double* blocks, double* rhs);
+
+/*
  * PCG iterations to solve the RCS at the current state and radius, with
  * b = its right-hand side, to ||r|| <= tolerance ||b|| with `preconditioner`
  * (MLBA_PRECOND_*; LEARNED needs a model). iterations: -1 when not converged

@@ -255,6 +255,7 @@ LEARNED_SYMBOLS = {
     "mlba_get_preconditioner_stats": (c_i32, [c_vp, ctypes.POINTER(PreconditionerStats)]),
     "mlba_reset_preconditioner_stats": (c_i32, [c_vp]),
     "mlba_rcs_size": (c_i32, [c_vp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32)]),
+    "mlba_rcs_system": (c_i32, [c_vp, ctypes.POINTER(Options), c_f64, c_vp, c_vp, c_vp, c_vp]),
     "mlba_rcs_records": (c_i32, [c_vp, ctypes.POINTER(Options), c_f64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "mlba_rcs_pcg_iterations": (c_i32, [c_vp, ctypes.POINTER(Options), c_f64, c_i32, c_f64, c_i32,
                                         ctypes.POINTER(c_i32)]),

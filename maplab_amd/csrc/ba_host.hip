@@ -1761,6 +1761,23 @@ int read_fail(mlba_handle* h, int32_t* fail) {
 
 extern "C" {
 
+int32_t mlba_rcs_system(mlba_handle* h, const mlba_options* o, double radius, int32_t* row_ptr, int32_t* col,
+                        double* blocks, double* rhs) {
+  if (!h) return MLBA_EINVAL;
+  if (!row_ptr || !col || !blocks || !rhs) return set_err(h, MLBA_EINVAL, "NULL output");
+  int rc = validate_options(h, o);
+  if (rc) return rc;
+  if ((rc = build_rcs(h, o, radius))) return rc;
+  hipStream_t s = h->stream;
+  if (h->nnzb > 0)
+    HIP_OK(hipMemcpyAsync(blocks, h->S.p, sizeof(double) * 36 * (size_t)h->nnzb, hipMemcpyDeviceToHost, s));
+  if (h->Pvar > 0) HIP_OK(hipMemcpyAsync(rhs, h->rhs.p, sizeof(double) * 6 * (size_t)h->Pvar, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  std::memcpy(row_ptr, h->h_row_ptr.data(), sizeof(int32_t) * h->h_row_ptr.size());
+  std::memcpy(col, h->h_col_var.data(), sizeof(int32_t) * h->h_col_var.size());
+  return MLBA_OK;
+}
+
 int32_t mlba_rcs_records(mlba_handle* h, const mlba_options* o, double radius, int32_t* row_ptr, int32_t* col,
                          int32_t* rev, float* node_features, float* edge_features, float* qc) {
   if (!h) return MLBA_EINVAL;

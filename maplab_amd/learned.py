@@ -68,6 +68,27 @@ def rcs_records(dev: DeviceProblem, options: abi.Options, radius: float) -> Reco
     return Records(row_ptr, col, rev, nf, ef, qc)
 
 
+def rcs_system(dev: DeviceProblem, options: abi.Options, radius: float):
+    """The RCS at the current state and radius as a dense matrix (6 rows per
+    variable vertex, Jacobi-scaled variables) and its right-hand side."""
+    lib = dev.lib
+    rows, slots = ctypes.c_int32(), ctypes.c_int32()
+    abi.check(lib.mlba_rcs_size(dev.handle, ctypes.byref(rows), ctypes.byref(slots)))
+    n, z = rows.value, slots.value
+    row_ptr = np.zeros(n + 1, np.int32)
+    col = np.zeros(z, np.int32)
+    blocks = np.zeros((z, 6, 6))
+    rhs = np.zeros(6 * n)
+    dev._check(lib.mlba_rcs_system(dev.handle, ctypes.byref(options), radius, _p(row_ptr), _p(col), _p(blocks),
+                                   _p(rhs)))
+    S = np.zeros((6 * n, 6 * n))
+    for i in range(n):
+        for k in range(row_ptr[i], row_ptr[i + 1]):
+            j = col[k]
+            S[6 * i:6 * i + 6, 6 * j:6 * j + 6] = blocks[k]
+    return S, rhs
+
+
 def rcs_pcg_iterations(dev: DeviceProblem, options: abi.Options, radius: float, preconditioner: int,
                        tolerance: float = 1e-6, max_iterations: int = 5000) -> int:
     it = ctypes.c_int32()
