@@ -504,7 +504,13 @@ int build_topology(mlba_handle* h) {
   std::vector<int32_t> uslot_ptr(ublk.size() + 1, 0), uslot;
   for (size_t u = 0; u < ublk.size(); ++u) {
     const int i = ublk[u].z, j = ublk[u].w;
-    const int64_t t = (int64_t)(i / kTileBand) * st.nbands + j / kTileBand;
+    const int64_t key = (int64_t)(i / kTileBand) * st.nbands + j / kTileBand;
+    const auto it = std::lower_bound(st.tile_key.begin(), st.tile_key.end(), key);
+    if (it == st.tile_key.end() || *it != key) {
+      uslot_ptr[u + 1] = (int32_t)uslot.size();  // a block no landmark couples (diagonal only)
+      continue;
+    }
+    const size_t t = (size_t)(it - st.tile_key.begin());
     const int blk = (i % kTileBand) * kTileBand + j % kTileBand;
     for (int sg = st.tile_seg[2 * t]; sg < st.tile_seg[2 * t + 1]; ++sg) {
       const uint16_t bw = st.seg_block[(size_t)sg * kTileBlocks + blk];

@@ -57,7 +57,8 @@ struct SchurTiles {
   std::vector<int32_t> segs;        // 4 per segment: first batch, end batch, I, J
   std::vector<uint32_t> seg_thread; // kTileBlocks per segment: the thread's part
   std::vector<uint16_t> seg_block;  // kTileBlocks per segment: the block's parts
-  std::vector<int32_t> tile_seg;    // 2 per tile (I * nbands + J): segment range
+  std::vector<int64_t> tile_key;    // visited tiles, ascending I * nbands + J
+  std::vector<int32_t> tile_seg;    // 2 per visited tile: its segment range
   int64_t visits = 0;               // landmark tile visits
   int64_t staged = 0;               // factor blocks staged over all batches
 };

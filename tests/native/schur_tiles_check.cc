@@ -5,6 +5,7 @@
 // checks on every slot, pair and part, and S and its rhs are compared with a
 // brute-force sum over incidence pairs (Q_x W_y + [x observation, y anchor]
 // W_x^T A_l, rhs_x = r0_x - W_x^T Z g). Usage: schur_tiles_check Pvar L seed.
+#include <algorithm>
 #include <cassert>
 #include <cmath>
 #include <cstdio>
@@ -166,7 +167,10 @@ int main(int argc, char** argv) {
   double maxerr = 0, maxv = 0, rerr = 0;
   const int nb = T.nbands;
   auto block_sum = [&](int i, int j, int e) {
-    const int I = i / kTileBand, J = j / kTileBand, t = I * nb + J;
+    const int64_t key = (int64_t)(i / kTileBand) * nb + j / kTileBand;
+    const auto it = std::lower_bound(T.tile_key.begin(), T.tile_key.end(), key);
+    if (it == T.tile_key.end() || *it != key) return 0.0;
+    const size_t t = (size_t)(it - T.tile_key.begin());
     const int blk = (i % kTileBand) * kTileBand + (j % kTileBand);
     double v = 0;
     for (int sg = T.tile_seg[2 * t]; sg < T.tile_seg[2 * t + 1]; ++sg) {
