@@ -240,12 +240,15 @@ def learned_eval_2000(args, model):
             rows[f"{radius:g}"] = {
                 name: learned.rcs_pcg_iterations(dev, o, radius, pc, 1e-6, 20000)
                 for name, pc in (("learned", abi.PRECOND_LEARNED), ("block_jacobi", abi.PRECOND_BLOCK_JACOBI),
-                                 ("deflated", abi.PRECOND_DEFLATED))}
-    tot = {k: sum(r[k] for r in rows.values()) for k in ("learned", "block_jacobi", "deflated")}
+                                 ("deflated", abi.PRECOND_DEFLATED),
+                                 ("learned_deflated", abi.PRECOND_LEARNED_DEFLATED))}
+    tot = {k: sum(r[k] for r in rows.values()) for k in ("learned", "block_jacobi", "deflated", "learned_deflated")}
     return {"workload": f"random walk, 2000 keyframes, {m.num_landmarks} landmarks, {m.num_observations} observations",
             "tolerance": 1e-6, "per_radius": rows, "total": tot,
             "learned_vs_block_jacobi": round(tot["block_jacobi"] / max(1, tot["learned"]), 2),
-            "learned_vs_deflated": round(tot["deflated"] / max(1, tot["learned"]), 2)}
+            "learned_vs_deflated": round(tot["deflated"] / max(1, tot["learned"]), 2),
+            "learned_deflated_vs_block_jacobi": round(tot["block_jacobi"] / max(1, tot["learned_deflated"]), 2),
+            "learned_deflated_vs_deflated": round(tot["deflated"] / max(1, tot["learned_deflated"]), 2)}
 
 
 def cpu_baseline(vmap, opts, iters, threads):
@@ -384,7 +387,19 @@ def main(argv=None):
         bo.pcg_relative_tolerance = 1e-6
         bo.pcg_preconditioner = abi.PRECOND_BLOCK_JACOBI
         bsm, brows = dev.solve(bo)
+        # G'G inside the hat deflation, same iterations and tolerance.
+        dev.upload_state(init_v, init_l)
+        dev.lib.mlba_reset_preconditioner_stats(dev.handle)
+        co = abi.default_options()
+        ctypes.pointer(co)[0] = opts
+        co.pcg_relative_tolerance = 1e-6
+        co.pcg_preconditioner = abi.PRECOND_LEARNED_DEFLATED
+        t0 = time.perf_counter()
+        csm, crows = dev.solve(co)
+        c_wall = time.perf_counter() - t0
+        cst = learned.stats(dev)
         loose["learned"] = int(rsm.total_linear_solver_iterations)
+        loose["learned_deflated"] = int(csm.total_linear_solver_iterations)
         learned_lm = {
             "pcg_iterations": int(rsm.total_linear_solver_iterations),
             "block_jacobi_pcg_iterations": int(bsm.total_linear_solver_iterations),
@@ -394,7 +409,14 @@ def main(argv=None):
                                            [r.step_is_successful for r in brows],
             "final_cost_rel_to_block_jacobi": abs(rsm.final_cost - bsm.final_cost) / abs(bsm.final_cost),
             "solves": int(st.solves), "block_jacobi_fallbacks": int(st.fallbacks), "tolerance": 1e-6,
-            "model": model.meta.get("generator")}
+            "model": model.meta.get("generator"),
+            "with_hat_deflation": {
+                "pcg_iterations": int(csm.total_linear_solver_iterations),
+                "lm_iterations_per_s": round((csm.num_iterations - 1) / c_wall, 3),
+                "same_lm_log_as_block_jacobi": [r.step_is_successful for r in crows] ==
+                                               [r.step_is_successful for r in brows],
+                "final_cost_rel_to_block_jacobi": abs(csm.final_cost - bsm.final_cost) / abs(bsm.final_cost),
+                "solves": int(cst.solves), "fallbacks": int(cst.fallbacks)}}
     # maplab's default driver (--ba_use_visual_outlier_rejection_solver):
     # chunks of 3 iterations with device outlier rejection after each,
     # timed over the same iteration count.

@@ -175,7 +175,8 @@ typedef struct mlba_options {
                                              of J'J of that one rounded Jacobian (positive
                                              semi-definite). 0 (default): FP64 Jacobians */
   int32_t pcg_preconditioner;             /* MLBA_PRECOND_* of maplab_ba_learned.h: 0 deflated
-                                             block-Jacobi (default), 1 block-Jacobi, 2 learned */
+                                             block-Jacobi (default), 1 block-Jacobi, 2 learned,
+                                             3 learned with the hat deflation */
 } mlba_options;
 
 /* One row of the Ceres iteration log (ceres::IterationSummary subset). */

@@ -26,7 +26,9 @@
  * pcg_preconditioner. With MLBA_PRECOND_LEARNED a solve whose PCG does not
  * reach pcg_relative_tolerance within pcg_max_iterations (or breaks down) is
  * repeated with block-Jacobi; the LM step is accepted by the same cost test
- * as every other step.
+ * as every other step. MLBA_PRECOND_LEARNED_DEFLATED applies G'G inside the
+ * hat-deflated PCG of MLBA_PRECOND_DEFLATED (without its recycled Ritz
+ * columns); its fallback is that deflated PCG with block-Jacobi.
  */
 #ifndef MAPLAB_BA_LEARNED_H_
 #define MAPLAB_BA_LEARNED_H_
@@ -42,6 +44,7 @@ extern "C" {
 #define MLBA_PRECOND_DEFLATED 0      /* block-Jacobi with hat and recycled Ritz deflation (default) */
 #define MLBA_PRECOND_BLOCK_JACOBI 1  /* plain block-Jacobi */
 #define MLBA_PRECOND_LEARNED 2       /* block-Jacobi with the network's correction G'G */
+#define MLBA_PRECOND_LEARNED_DEFLATED 3 /* G'G with the hat deflation of MLBA_PRECOND_DEFLATED */
 
 #define MLBA_GNN_NODE_FEATURES 9
 #define MLBA_GNN_EDGE_FEATURES 3
@@ -95,7 +98,8 @@ int32_t mlba_rcs_system(mlba_handle* handle, const mlba_options* options, double
 /*
  * PCG iterations to solve the RCS at the current state and radius, with
  * b = its right-hand side, to ||r|| <= tolerance ||b|| with `preconditioner`
- * (MLBA_PRECOND_*; LEARNED needs a model). iterations: -1 when not converged
+ * (MLBA_PRECOND_*; LEARNED and LEARNED_DEFLATED need a model; no fallback
+ * and no recycled columns here). iterations: -1 when not converged
  * within max_iterations. The state is not changed.
  */
 int32_t mlba_rcs_pcg_iterations(mlba_handle* handle, const mlba_options* options, double radius,

@@ -235,7 +235,7 @@ INERTIAL_SYMBOLS = {
 }
 
 # Every symbol include/maplab_ba_learned.h declares (exported by the same library).
-PRECOND_DEFLATED, PRECOND_BLOCK_JACOBI, PRECOND_LEARNED = 0, 1, 2
+PRECOND_DEFLATED, PRECOND_BLOCK_JACOBI, PRECOND_LEARNED, PRECOND_LEARNED_DEFLATED = 0, 1, 2, 3
 GNN_NODE_FEATURES, GNN_EDGE_FEATURES, GNN_NORM_FLOATS = 9, 3, 24
 
 

@@ -1059,6 +1059,55 @@ int lp_solve(mlba_handle* h, const mlba_options* o, bool learned, int* iters) {
   return MLBA_OK;
 }
 
+// The hat-deflated PCG with the learned preconditioner G'G (set up by
+// lp_setup with `learned`); no recycled columns.
+PcgBuffers lp_deflated_buffers(mlba_handle* h) {
+  PcgBuffers B = pcg_buffers(h, 0, false);
+  B.linv = h->lp_linv.p;
+  B.kmat = h->lp_kmat.p;
+  B.diag_pos = h->diag_pos.p;
+  B.tpos = h->lp_tpos.p;
+  B.t = h->lp_t.p;
+  B.u = h->lp_u.p;
+  return B;
+}
+
+// PCG on the current S with G'G inside the hat deflation; a solve that does
+// not converge (or a row that does not fit the neighbourhood kernel) is
+// repeated (run) with the deflated block-Jacobi PCG, which needs Minv.
+int lp_deflated_solve(mlba_handle* h, const mlba_options* o, int* iters) {
+  int rc = lp_ensure(h, true, false);
+  if (rc) return rc;
+  bool use_k = false;
+  if ((rc = lp_setup(h, true, &use_k))) return rc;
+  // The recycled columns come from block-Jacobi solves; they start again
+  // when that solver is selected next.
+  h->ritz_nr = 0;
+  h->ritz_stage = 0;
+  const int32_t maxit = o->pcg_max_iterations;
+  int it = 0;
+  if (use_k) {
+    it = launch_pcg_deflated(h->stream, lp_deflated_buffers(h), o->pcg_relative_tolerance, maxit, kPcgPoll, kPcgPoll,
+                             nullptr, 0);
+    if (it < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
+    h->lp_stats.solves++;
+    h->lp_stats.iterations += it;
+    if (it < maxit) {
+      *iters = it;
+      return MLBA_OK;
+    }
+  }
+  const int it2 = launch_pcg_deflated(h->stream, pcg_buffers(h, 0, false), o->pcg_relative_tolerance, maxit, kPcgPoll,
+                                      kPcgPoll, nullptr, 0);
+  if (it2 < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
+  if (use_k) {
+    h->lp_stats.fallbacks++;
+    h->lp_stats.fallback_iterations += it2;
+  }
+  *iters = it + it2;
+  return MLBA_OK;
+}
+
 int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_summary* sm, double* last_radius,
              int* termination) {
   hipStream_t s = h->stream;
@@ -1119,9 +1168,13 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
       launch_block_inverse(s, h->Pvar, h->diag_pos.p, h->S.p, h->Minv.p, h->fail.p);
       HIP_OK(hipGetLastError());
       int pcg_iters = 0;
-      if (h->Pvar > 0 && o->pcg_preconditioner != MLBA_PRECOND_DEFLATED) {
+      if (h->Pvar > 0 && (o->pcg_preconditioner == MLBA_PRECOND_BLOCK_JACOBI ||
+                          o->pcg_preconditioner == MLBA_PRECOND_LEARNED)) {
         KernelTimer kt(h, MLBA_KERNEL_PCG);
         if ((rc = lp_solve(h, o, o->pcg_preconditioner == MLBA_PRECOND_LEARNED, &pcg_iters))) return rc;
+      } else if (h->Pvar > 0 && o->pcg_preconditioner == MLBA_PRECOND_LEARNED_DEFLATED) {
+        KernelTimer kt(h, MLBA_KERNEL_PCG);
+        if ((rc = lp_deflated_solve(h, o, &pcg_iters))) return rc;
       } else if (h->Pvar > 0) {
         KernelTimer kt(h, MLBA_KERNEL_PCG);
         const int32_t nritz = ritz_count(o);
@@ -1272,9 +1325,10 @@ int validate_options(mlba_handle* h, const mlba_options* o) {
   if (o->loss_type < 0 || o->loss_type > 2) return set_err(h, MLBA_EINVAL, "unknown loss_type");
   if (o->linear_solver_type != MLBA_SOLVER_SCHUR_PCG) return set_err(h, MLBA_EINVAL, "unknown linear_solver_type");
   if (o->jacobian_fp32 != 0 && o->jacobian_fp32 != 1) return set_err(h, MLBA_EINVAL, "jacobian_fp32 must be 0 or 1");
-  if (o->pcg_preconditioner < MLBA_PRECOND_DEFLATED || o->pcg_preconditioner > MLBA_PRECOND_LEARNED)
+  if (o->pcg_preconditioner < MLBA_PRECOND_DEFLATED || o->pcg_preconditioner > MLBA_PRECOND_LEARNED_DEFLATED)
     return set_err(h, MLBA_EINVAL, "unknown pcg_preconditioner");
-  if (o->pcg_preconditioner == MLBA_PRECOND_LEARNED && !h->lp_model)
+  if ((o->pcg_preconditioner == MLBA_PRECOND_LEARNED || o->pcg_preconditioner == MLBA_PRECOND_LEARNED_DEFLATED) &&
+      !h->lp_model)
     return set_err(h, MLBA_ESTATE, "learned preconditioner without a model (mlba_set_preconditioner_model)");
   if (!(o->initial_trust_region_radius > 0.0)) return set_err(h, MLBA_EINVAL, "initial_trust_region_radius <= 0");
   if (o->loss_type == MLBA_LOSS_CAUCHY && !(o->loss_scale_px > 0.0)) return set_err(h, MLBA_EINVAL, "loss_scale_px <= 0");
@@ -1837,10 +1891,11 @@ int32_t mlba_rcs_pcg_iterations(mlba_handle* h, const mlba_options* o, double ra
   if (!h || !iterations) return MLBA_EINVAL;
   int rc = validate_options(h, o);
   if (rc) return rc;
-  if (preconditioner < MLBA_PRECOND_DEFLATED || preconditioner > MLBA_PRECOND_LEARNED || !(tolerance > 0.0) ||
-      max_iterations <= 0)
+  if (preconditioner < MLBA_PRECOND_DEFLATED || preconditioner > MLBA_PRECOND_LEARNED_DEFLATED ||
+      !(tolerance > 0.0) || max_iterations <= 0)
     return set_err(h, MLBA_EINVAL, "preconditioner, tolerance or iteration cap");
-  if (preconditioner == MLBA_PRECOND_LEARNED && !h->lp_model) return set_err(h, MLBA_ESTATE, "no preconditioner model");
+  if ((preconditioner == MLBA_PRECOND_LEARNED || preconditioner == MLBA_PRECOND_LEARNED_DEFLATED) && !h->lp_model)
+    return set_err(h, MLBA_ESTATE, "no preconditioner model");
   if ((rc = build_rcs(h, o, radius))) return rc;
   hipStream_t s = h->stream;
   *iterations = -1;
@@ -1849,6 +1904,20 @@ int32_t mlba_rcs_pcg_iterations(mlba_handle* h, const mlba_options* o, double ra
     launch_block_inverse(s, h->Pvar, h->diag_pos.p, h->S.p, h->Minv.p, h->fail.p);
     if ((rc = ensure_coarse_basis(h, o->pcg_coarse_nodes))) return rc;
     const int it = launch_pcg_deflated(s, pcg_buffers(h, 0, false), tolerance, max_iterations, kPcgPoll, kPcgPoll,
+                                       nullptr, 0);
+    if (it < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
+    *iterations = it < max_iterations ? it : -1;
+    return MLBA_OK;
+  }
+  if (preconditioner == MLBA_PRECOND_LEARNED_DEFLATED) {
+    if ((rc = lp_ensure(h, true, false))) return rc;
+    bool use_k = false;
+    if ((rc = lp_setup(h, true, &use_k))) return rc;
+    if (!use_k) return set_err(h, MLBA_EINVAL, "a row's lower neighbourhood does not fit the neighbourhood kernel");
+    if ((rc = ensure_coarse_basis(h, o->pcg_coarse_nodes))) return rc;
+    h->ritz_nr = 0;
+    h->ritz_stage = 0;
+    const int it = launch_pcg_deflated(s, lp_deflated_buffers(h), tolerance, max_iterations, kPcgPoll, kPcgPoll,
                                        nullptr, 0);
     if (it < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
     *iterations = it < max_iterations ? it : -1;

@@ -49,6 +49,14 @@ struct PcgBuffers {
   double* coef;                // 2 zcap: r'z and alpha of iteration k
   void* state_dev;             // pcg_state_bytes()
   void* state_host;            // pinned, pcg_state_bytes()
+  // Learned preconditioner z = L^-T (I - K)' (I - K) L^-1 r in place of
+  // block-Jacobi (Minv) when kmat is set (no recycled columns, nr = zcap = 0).
+  const double* linv;          // 36 Pvar: L_i^-1 of S_ii = L_i L_i' (lower triangle, row-major)
+  const float* kmat;           // 36 nnzb: K_ij on the strictly lower blocks
+  const int32_t* diag_pos;     // Pvar: position of the diagonal block of each row
+  const int32_t* tpos;         // nnzb: position of the transposed block
+  double* t;                   // 6 Pvar: L^-1 r
+  double* u;                   // 6 Pvar: (I - K) L^-1 r
 };
 
 size_t pcg_state_bytes();
@@ -65,8 +73,9 @@ bool pcg_deflation_applied(const void* state_host);
 // Deflated PCG solve. The host polls the state after `first` iterations, then
 // every `poll` iterations. With `probe` (three events), k_pcg_spmv and then
 // k_pcg_update of one more iteration are each launched `probe_reps` times
-// back to back after the solve and timed as two batches. Returns the
-// iteration count or -1 on a launch error.
+// back to back after the solve and timed as two batches (block-Jacobi only).
+// Returns the iteration count (max_iters: not converged) or -1 on a launch
+// error or an unsupported combination.
 int launch_pcg_deflated(hipStream_t s, const PcgBuffers& B, double tol, int32_t max_iters, int32_t first,
                         int32_t poll, const hipEvent_t* probe, int32_t probe_reps);
 

@@ -33,6 +33,13 @@
 // k_pcg_update (or k_defl_init) that finishes last sums the groups' partials
 // once, into part1, and k_pcg_spmv reads that one vector; the only atomic is
 // the integer arrival counter that picks this work-group.
+//
+// The deflation holds for any SPD preconditioner: with the learned one
+// (z = L^-T (I - K)' (I - K) L^-1 r, K strictly block-lower-triangular),
+// k_defl_init and k_pcg_update stop at t = L^-1 r, and k_pcg_lower,
+// k_pcg_upper (the two row sweeps) and k_defl_partials (the same group
+// partials) follow; the hat space is kept, the recycled columns are not
+// (their Rayleigh-Ritz step is taken with respect to block-Jacobi's M).
 #include "ba_pcg.h"
 
 #include <algorithm>
@@ -102,6 +109,12 @@ struct PcgArgs {
   double* Zh;                  // zcap x 6 Pvar: deflated preconditioned residuals
   double* coef;                // 2 zcap: r'z and alpha per iteration
   PcgState* st;
+  const double* linv;          // learned preconditioner (with kmat): L_i^-1, 36 Pvar
+  const float* kmat;           // K_ij, 36 nnzb
+  const int32_t* diag_pos;
+  const int32_t* tpos;
+  double* t;                   // L^-1 r
+  double* u;                   // (I - K) L^-1 r
 };
 
 __device__ inline double hat_weight(const PcgArgs& a, int i, int k) {
@@ -552,8 +565,19 @@ __device__ inline void sum_partials_if_last(const PcgArgs& a, bool* last, double
   if (tid == 0) a.st->arrive = 0u;
 }
 
-// x0 = W y, r0 = b - (SW) y, z0 = M^-1 r0, p = 0 on one group of rows per
-// work-group, and the group's partials.
+// t_e = (L_i^-1 r_i)_comp for entry e = 6 i + comp; rloc holds the group's r.
+__device__ inline double lower_solve_entry(const PcgArgs& a, int row, int comp, const double* rloc, int base) {
+  const double* Li = a.linv + (size_t)36 * row + 6 * comp;
+  double s = 0.0;
+  for (int m = 0; m <= comp; ++m) s += Li[m] * rloc[base + m];
+  return s;
+}
+
+// x0 = W y, r0 = b - (SW) y, p = 0 on one group of rows per work-group; with
+// block-Jacobi z0 = M^-1 r0 and the group's partials, with the learned
+// preconditioner t0 = L^-1 r0 (k_pcg_lower, k_pcg_upper and
+// k_defl_partials follow).
+template <bool kLearned>
 __global__ __launch_bounds__(kGroupBlock) void k_defl_init(PcgArgs a) {
   __shared__ double rloc[kGroupThreads];
   __shared__ double zloc[kGroupThreads];
@@ -563,7 +587,7 @@ __global__ __launch_bounds__(kGroupBlock) void k_defl_init(PcgArgs a) {
   __shared__ double ys[kMaxCoarse];
   const int ne = a.ne, tid = threadIdx.x, P = a.Pvar, g = blockIdx.x;
   GroupLoads L;
-  load_group<true>(a, g, L);
+  if (!kLearned) load_group<true>(a, g, L);
   for (int c = tid; c < ne; c += blockDim.x) ys[c] = a.y[c];
   if (ne) {
     // The group's window rows of E^-1 for k_pcg_update.
@@ -599,6 +623,10 @@ __global__ __launch_bounds__(kGroupBlock) void k_defl_init(PcgArgs a) {
   }
   if (tid < kGroupThreads) rloc[tid] = rv;
   __syncthreads();
+  if (kLearned) {
+    if (active) a.t[6 * row + comp] = lower_solve_entry(a, row, comp, rloc, tid - comp);
+    return;
+  }
   double zv = 0.0;
   if (active) {
     const double* M = a.Minv + (size_t)36 * row + 6 * comp;
@@ -786,8 +814,11 @@ __global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a, int32_t k, const do
   if (tid == 0) a.part[i] = ((ysz[0] + ysz[1]) + (ysz[2] + ysz[3])) + (ysz[4] + ysz[5]);
 }
 
-// alpha = r'z / p'q; x += alpha p; r -= alpha q; z = M^-1 r on one group of
-// rows per work-group, and the group's partials r'z, r'r, E^-1 (SW)'z.
+// alpha = r'z / p'q; x += alpha p; r -= alpha q on one group of rows per
+// work-group; with block-Jacobi z = M^-1 r and the group's partials r'z, r'r,
+// E^-1 (SW)'z, with the learned preconditioner t = L^-1 r (k_pcg_lower,
+// k_pcg_upper and k_defl_partials follow).
+template <bool kLearned>
 __global__ __launch_bounds__(kGroupBlock) void k_pcg_update(PcgArgs a, int32_t k, const double* __restrict__ p_cur) {
   __shared__ double rloc[kGroupThreads];
   __shared__ double zloc[kGroupThreads];
@@ -801,8 +832,9 @@ __global__ __launch_bounds__(kGroupBlock) void k_pcg_update(PcgArgs a, int32_t k
   const int row = g * kRowsPerGroup + tid / 6, comp = tid % 6;
   const bool active = tid < kGroupThreads && row < P;
   GroupLoads L;
-  load_group<false>(a, g, L);
-  // The group's vectors and preconditioner rows, loaded before the reduction.
+  if (!kLearned) load_group<false>(a, g, L);
+  // The group's vectors and preconditioner rows (M^-1 or L^-1), loaded
+  // before the reduction.
   double xv = 0.0, pv = 0.0, rv = 0.0, qv = 0.0, M[6] = {};
   if (active) {
     const int e = 6 * row + comp;
@@ -810,9 +842,9 @@ __global__ __launch_bounds__(kGroupBlock) void k_pcg_update(PcgArgs a, int32_t k
     pv = p_cur[e];
     rv = a.r[e];
     qv = a.q[e];
-    const double* Mr = a.Minv + (size_t)36 * row + 6 * comp;
+    const double* Mr = (kLearned ? a.linv : a.Minv) + (size_t)36 * row + 6 * comp;
 #pragma unroll
-    for (int c = 0; c < 6; ++c) M[c] = Mr[c];
+    for (int c = 0; c < 6; ++c) M[c] = (!kLearned || c <= comp) ? Mr[c] : 0.0;
   }
   // p'q, summed by every group in the same order.
   double pq = 0.0, unused0 = 0.0, unused1 = 0.0;
@@ -834,12 +866,111 @@ __global__ __launch_bounds__(kGroupBlock) void k_pcg_update(PcgArgs a, int32_t k
     const int base = tid - comp;
 #pragma unroll
     for (int c = 0; c < 6; ++c) zv += M[c] * rloc[base + c];
-    a.z[6 * row + comp] = zv;
+    (kLearned ? a.t : a.z)[6 * row + comp] = zv;
   }
+  if (kLearned) return;
   if (tid < kGroupThreads) zloc[tid] = zv;
   double rzg = rv * zv, rr = rv * rv, zero = 0.0;
   block_sum3(rzg, rr, zero, lds);
   group_partials(a, g, L, zloc, rzg, rr, 0.0, cs, mpart, a.part2 + (size_t)g * (3 + ne));
+  __shared__ bool last;
+  __shared__ double red[kRedChunk];
+  if (a.reduce1) sum_partials_if_last(a, &last, red);
+}
+
+// Learned preconditioner, after t = L^-1 r: u_i = t_i - sum_{j < i} K_ij t_j,
+// one wave per row (lanes take every 64th lower block, then a fixed-order
+// wave sum).
+__global__ __launch_bounds__(64) void k_pcg_lower(PcgArgs a) {
+  if (a.st->done) return;
+  const int i = blockIdx.x, lane = threadIdx.x;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int pos = a.row_ptr[i] + lane; pos < a.diag_pos[i]; pos += 64) {
+    const int j = a.col_var[pos];
+    const float* K = a.kmat + (size_t)36 * pos;
+    const double* tj = a.t + 6 * (size_t)j;
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      double s = 0.0;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) s += (double)K[6 * r + c] * tj[c];
+      acc[r] += s;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 6; ++r) acc[r] = wave_sum(acc[r]);
+  if (lane < 6) {
+    double v = acc[0];
+#pragma unroll
+    for (int r = 1; r < 6; ++r)
+      if (lane == r) v = acc[r];
+    a.u[6 * i + lane] = a.t[6 * i + lane] - v;
+  }
+}
+
+// w_j = u_j - sum_{i > j} K_ij' u_i, z_j = L_j^-T w_j; one wave per row (K_ij
+// is stored in row i, at the transposed position of block (j, i)).
+__global__ __launch_bounds__(64) void k_pcg_upper(PcgArgs a) {
+  __shared__ double w[6];
+  if (a.st->done) return;
+  const int j = blockIdx.x, lane = threadIdx.x;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int pos = a.diag_pos[j] + 1 + lane; pos < a.row_ptr[j + 1]; pos += 64) {
+    const int i = a.col_var[pos];
+    const float* K = a.kmat + (size_t)36 * a.tpos[pos];
+    const double* ui = a.u + 6 * (size_t)i;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double s = 0.0;
+#pragma unroll
+      for (int r = 0; r < 6; ++r) s += (double)K[6 * r + c] * ui[r];
+      acc[c] += s;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) acc[c] = wave_sum(acc[c]);
+  if (lane < 6) {
+    double v = acc[0];
+#pragma unroll
+    for (int c = 1; c < 6; ++c)
+      if (lane == c) v = acc[c];
+    w[lane] = a.u[6 * j + lane] - v;
+  }
+  __syncthreads();
+  if (lane < 6) {
+    const double* Lj = a.linv + (size_t)36 * j;
+    double s = 0.0;
+    for (int m = lane; m < 6; ++m) s += Lj[6 * m + lane] * w[m];
+    a.z[6 * j + lane] = s;
+  }
+}
+
+// Learned preconditioner: the group's partials r'z, r'r (b'b with
+// `with_b`) and E^-1 (SW)'z from r and z, one group of rows per work-group,
+// as k_pcg_update and k_defl_init form them with block-Jacobi.
+__global__ __launch_bounds__(kGroupBlock) void k_defl_partials(PcgArgs a, int32_t with_b) {
+  __shared__ double zloc[kGroupThreads];
+  __shared__ double lds[48];
+  __shared__ double cs[kMaxCoarse];
+  __shared__ double mpart[kGroupBlock];
+  const int32_t done = a.st->done;
+  const int ne = a.ne, tid = threadIdx.x, P = a.Pvar, g = blockIdx.x;
+  const int row = g * kRowsPerGroup + tid / 6, comp = tid % 6;
+  const bool active = tid < kGroupThreads && row < P;
+  GroupLoads L;
+  load_group<false>(a, g, L);
+  double rv = 0.0, zv = 0.0, bv = 0.0;
+  if (active) {
+    const int e = 6 * row + comp;
+    rv = a.r[e];
+    zv = a.z[e];
+    if (with_b) bv = a.b[e];
+  }
+  if (done) return;  // the same value in every thread
+  if (tid < kGroupThreads) zloc[tid] = zv;
+  double rz = rv * zv, rr = rv * rv, bb = bv * bv;
+  block_sum3(rz, rr, bb, lds);
+  group_partials(a, g, L, zloc, rz, rr, bb, cs, mpart, a.part2 + (size_t)g * (3 + ne));
   __shared__ bool last;
   __shared__ double red[kRedChunk];
   if (a.reduce1) sum_partials_if_last(a, &last, red);
@@ -968,7 +1099,25 @@ static PcgArgs make_args(const PcgBuffers& B) {
   a.reduce1 = B.partials == 0 ? (a.groups > kPartChunk / (3 + a.ne) ? 1 : 0) : (B.partials == 2 ? 1 : 0);
   a.pad4 = 0;
   a.st = static_cast<PcgState*>(B.state_dev);
+  a.linv = B.linv; a.kmat = B.kmat; a.diag_pos = B.diag_pos; a.tpos = B.tpos; a.t = B.t; a.u = B.u;
   return a;
+}
+
+// With the learned preconditioner: z = L^-T (I - K)' (I - K) t after t = L^-1 r,
+// then the group partials.
+static void launch_learned_z(hipStream_t s, const PcgArgs& a, int32_t with_b) {
+  k_pcg_lower<<<a.Pvar, 64, 0, s>>>(a);
+  k_pcg_upper<<<a.Pvar, 64, 0, s>>>(a);
+  k_defl_partials<<<a.groups, kGroupBlock, 0, s>>>(a, with_b);
+}
+
+static void launch_update(hipStream_t s, const PcgArgs& a, int32_t k, const double* p_cur) {
+  if (a.kmat) {
+    k_pcg_update<true><<<a.groups, kGroupBlock, 0, s>>>(a, k, p_cur);
+    launch_learned_z(s, a, 0);
+  } else {
+    k_pcg_update<false><<<a.groups, kGroupBlock, 0, s>>>(a, k, p_cur);
+  }
 }
 
 static void launch_spmv(hipStream_t s, const PcgArgs& a, int32_t k, const double* p_prev, double* p_cur) {
@@ -989,6 +1138,8 @@ int launch_pcg_deflated(hipStream_t s, const PcgBuffers& B, double tol, int32_t 
   const int32_t Pvar = B.Pvar;
   if (Pvar <= 0) return 0;
   if (B.K > kMaxHatNodes || B.nr > kMaxRitz || B.nr < 0) return -1;
+  const bool learned = B.kmat != nullptr;
+  if (learned && (B.nr > 0 || B.zcap > 0 || probe || !B.linv || !B.diag_pos || !B.tpos || !B.t || !B.u)) return -1;
   const PcgArgs a = make_args(B);
   double* p0 = B.p0;
   double* p1 = B.p1;
@@ -1001,14 +1152,19 @@ int launch_pcg_deflated(hipStream_t s, const PcgBuffers& B, double tol, int32_t 
     if (a.nr > 0) k_defl_dense<<<dim3(a.nr, a.nr + 1), 256, 0, s>>>(a);
   }
   k_defl_prep<<<1, kPrepThreads, 0, s>>>(a, tol, max_iters);
-  k_defl_init<<<groups, kGroupBlock, 0, s>>>(a);
+  if (learned) {
+    k_defl_init<true><<<groups, kGroupBlock, 0, s>>>(a);
+    launch_learned_z(s, a, 1);
+  } else {
+    k_defl_init<false><<<groups, kGroupBlock, 0, s>>>(a);
+  }
   PcgState* hs = static_cast<PcgState*>(B.state_host);
   int k = 0;
   while (true) {
     const int n = std::min(k == 0 ? std::max(first, 1) : poll, max_iters - k);
     for (int j = 0; j < n; ++j, ++k) {
       launch_spmv(s, a, k, (k & 1) ? p1 : p0, (k & 1) ? p0 : p1);
-      k_pcg_update<<<groups, kGroupBlock, 0, s>>>(a, k, (k & 1) ? p0 : p1);
+      launch_update(s, a, k, (k & 1) ? p0 : p1);
     }
     // At k = max_iters this launch only records the final state.
     if (k >= max_iters) launch_spmv(s, a, k, p0, p1);
@@ -1025,7 +1181,7 @@ int launch_pcg_deflated(hipStream_t s, const PcgBuffers& B, double tol, int32_t 
     if (hipEventRecord(probe[0], s) != hipSuccess) return -1;
     for (int j = 0; j < probe_reps; ++j) launch_spmv(s, a, kk, (kk & 1) ? p1 : p0, (kk & 1) ? p0 : p1);
     if (hipEventRecord(probe[1], s) != hipSuccess) return -1;
-    for (int j = 0; j < probe_reps; ++j) k_pcg_update<<<groups, kGroupBlock, 0, s>>>(a, kk, (kk & 1) ? p0 : p1);
+    for (int j = 0; j < probe_reps; ++j) k_pcg_update<false><<<groups, kGroupBlock, 0, s>>>(a, kk, (kk & 1) ? p0 : p1);
     if (hipEventRecord(probe[2], s) != hipSuccess) return -1;
     if (hipGetLastError() != hipSuccess) return -1;
   }

@@ -19,9 +19,16 @@ mathematics directly:
   the points the device rounds them (bf16=True) or none (bf16=False, the
   exact function whose gradient finite differences check).
 
+* deflated PCG (Saad, Yeung, Erhel, Guyomarc'h, SIAM J. Sci. Comput. 21,
+  2000) over the hat-function coarse space of the vertex order, with any SPD
+  preconditioner: x0 = W E^-1 W'b, E = W'SW, and each search direction
+  p = z - W E^-1 (SW)'z + beta p made S-orthogonal to W.
+
 Pinned by (tests/test_oracle_gnn.py): finite differences of the loss for the
 gradient; t_i(theta) against tr[(G S G')_ii] formed densely; theta = 0
-giving block-Jacobi; M^-1 symmetric positive definite for random theta.
+giving block-Jacobi; M^-1 symmetric positive definite for random theta; the
+deflated PCG reducing to plain PCG without coarse columns, solving S x = b,
+and converging in m iterations when W spans all but m eigenvectors.
 """
 from __future__ import annotations
 
@@ -114,6 +121,62 @@ def pcg_iterations(S, b, Minv, tol=1e-6, maxit=5000):
         p = z + (rz2 / rz) * p
         rz = rz2
     return -1
+
+
+def hat_basis(n, K):
+    """Hat functions over the order of n vertices with K nodes (6 K columns,
+    column 6 k + c for node k and component c): vertex i at position
+    s = i (K - 1) / (n - 1) has weights (1 - t, t) on nodes (n0, n0 + 1),
+    n0 = min(floor(s), K - 2), t = s - n0."""
+    W = np.zeros((6 * n, 6 * K))
+    if K <= 0:
+        return W
+    for i in range(n):
+        s = i * (K - 1) / (n - 1)
+        n0 = min(int(s), K - 2)
+        t = s - n0
+        for c in range(6):
+            W[6 * i + c, 6 * n0 + c] = 1.0 - t
+            W[6 * i + c, 6 * (n0 + 1) + c] = t
+    return W
+
+
+def default_hat_nodes(n):
+    """Coarse nodes of the default deflation (min(15, n / 16); none below 2
+    or when n < 2 K)."""
+    K = min(15, n // 16)
+    return 0 if K < 2 or n < 2 * K else K
+
+
+def deflated_pcg(S, b, Minv, W, tol=1e-6, maxit=5000):
+    """Deflated PCG iterations to |r| <= tol |b| (-1: not within maxit) and
+    the solution."""
+    nb = np.linalg.norm(b)
+    if W.shape[1] == 0:
+        SW = np.zeros((S.shape[0], 0))
+        Einv = np.zeros((0, 0))
+    else:
+        SW = S @ W
+        Einv = np.linalg.inv(W.T @ SW)
+    x = W @ (Einv @ (W.T @ b))
+    r = b - SW @ (Einv @ (W.T @ b))
+    if np.linalg.norm(r) <= tol * nb:
+        return 0, x
+    z = Minv(r)
+    p = z - W @ (Einv @ (SW.T @ z))
+    rz = r @ z
+    for k in range(maxit):
+        q = S @ p
+        a = rz / (p @ q)
+        x = x + a * p
+        r = r - a * q
+        if np.linalg.norm(r) <= tol * nb:
+            return k + 1, x
+        z = Minv(r)
+        rz2 = r @ z
+        p = z - W @ (Einv @ (SW.T @ z)) + (rz2 / rz) * p
+        rz = rz2
+    return -1, x
 
 
 # ------------------------------------------------------------ network

@@ -115,29 +115,60 @@ def test_learned_pcg_matches_oracle_iterations(oracle):
     assert it_l < it_j
 
 
+def test_learned_deflated_pcg_matches_oracle_iterations(oracle):
+    # G'G inside the hat deflation, and the hat-deflated block-Jacobi PCG,
+    # against the numpy deflated PCG over the same coarse space.
+    m = walk(50, 8000, 7)
+    o = opts()
+    radius = 1e6
+    theta0 = (0.5, -4.0)
+    with DeviceProblem(m) as dev:
+        const_model(theta0).install(dev)
+        it_ld = learned.rcs_pcg_iterations(dev, o, radius, abi.PRECOND_LEARNED_DEFLATED, 1e-6)
+        it_d = learned.rcs_pcg_iterations(dev, o, radius, abi.PRECOND_DEFLATED, 1e-6)
+        it_l = learned.rcs_pcg_iterations(dev, o, radius, abi.PRECOND_LEARNED, 1e-6)
+    S, b, _ = oracle.reduced_camera_system(m, o, radius)
+    n = S.shape[0] // 6
+    W = go.hat_basis(n, go.default_hat_nodes(n))
+    assert W.shape[1] > 0
+    G = go.kmatrix(S, np.tile(theta0, (n, 1)))
+    Li = go.kmatrix(S, np.zeros((n, 2)))
+    ref_ld, _ = go.deflated_pcg(S, b, lambda r: G.T @ (G @ r), W)
+    ref_d, _ = go.deflated_pcg(S, b, lambda r: Li.T @ (Li @ r), W)
+    print(f"PCG to 1e-6: learned+deflation {it_ld} (oracle {ref_ld}), deflated {it_d} (oracle {ref_d}), "
+          f"learned {it_l}", flush=True)
+    assert abs(it_d - ref_d) <= 1 and abs(it_ld - ref_ld) <= 2
+    assert 0 < it_ld < it_d
+
+
 def test_lm_with_learned_preconditioner_keeps_the_log():
     m = walk(60, 12000, 2)
     base = dict(max_num_iterations=8, pcg_relative_tolerance=1e-10, pcg_max_iterations=2000)
     out = {}
-    for pc in (abi.PRECOND_DEFLATED, abi.PRECOND_BLOCK_JACOBI, abi.PRECOND_LEARNED):
+    pcs = (abi.PRECOND_DEFLATED, abi.PRECOND_BLOCK_JACOBI, abi.PRECOND_LEARNED, abi.PRECOND_LEARNED_DEFLATED)
+    for pc in pcs:
         with DeviceProblem(m) as dev:
             const_model((0.5, -4.0)).install(dev)
             sm, rows = dev.solve(opts(pcg_preconditioner=pc, **base))
             out[pc] = (sm, [r.cost for r in rows], sum(r.linear_solver_iterations for r in rows),
                        learned.stats(dev))
     ref = out[abi.PRECOND_DEFLATED][1]
-    for pc in (abi.PRECOND_BLOCK_JACOBI, abi.PRECOND_LEARNED):
+    for pc in pcs[1:]:
         np.testing.assert_allclose(out[pc][1], ref, rtol=1e-8)
-    st = out[abi.PRECOND_LEARNED][3]
     print(f"PCG iterations over the LM run: learned {out[abi.PRECOND_LEARNED][2]}, "
-          f"block-Jacobi {out[abi.PRECOND_BLOCK_JACOBI][2]}, solves {st.solves}, fallbacks {st.fallbacks}", flush=True)
-    assert st.solves >= out[abi.PRECOND_LEARNED][0].num_iterations - 1 and st.fallbacks == 0
+          f"block-Jacobi {out[abi.PRECOND_BLOCK_JACOBI][2]}, learned+deflation "
+          f"{out[abi.PRECOND_LEARNED_DEFLATED][2]}, deflated {out[abi.PRECOND_DEFLATED][2]}", flush=True)
+    for pc in (abi.PRECOND_LEARNED, abi.PRECOND_LEARNED_DEFLATED):
+        st = out[pc][3]
+        assert st.solves >= out[pc][0].num_iterations - 1 and st.fallbacks == 0
     assert out[abi.PRECOND_LEARNED][2] < out[abi.PRECOND_BLOCK_JACOBI][2]
+    assert out[abi.PRECOND_LEARNED_DEFLATED][2] < out[abi.PRECOND_LEARNED][2]
 
 
-def test_block_jacobi_fallback_when_the_learned_solve_does_not_converge():
-    m = walk(30, 4000, 4)
-    o = opts(max_num_iterations=3, pcg_max_iterations=3, pcg_preconditioner=abi.PRECOND_LEARNED)
+@pytest.mark.parametrize("pc", [abi.PRECOND_LEARNED, abi.PRECOND_LEARNED_DEFLATED])
+def test_block_jacobi_fallback_when_the_learned_solve_does_not_converge(pc):
+    m = walk(40, 4000, 4)
+    o = opts(max_num_iterations=3, pcg_max_iterations=3, pcg_preconditioner=pc)
     with DeviceProblem(m) as dev:
         const_model((0.5, -4.0)).install(dev)
         dev.solve(o)
@@ -145,10 +176,35 @@ def test_block_jacobi_fallback_when_the_learned_solve_does_not_converge():
     assert st.solves > 0 and st.fallbacks == st.solves and st.fallback_iterations > 0
 
 
-def test_learned_preconditioner_needs_a_model():
+@pytest.mark.parametrize("pc", [abi.PRECOND_LEARNED, abi.PRECOND_LEARNED_DEFLATED])
+def test_learned_preconditioner_needs_a_model(pc):
     with DeviceProblem(walk(20, 2000, 6)) as dev:
         with pytest.raises(RuntimeError, match="model"):
-            dev.solve(opts(pcg_preconditioner=abi.PRECOND_LEARNED, max_num_iterations=1))
+            dev.solve(opts(pcg_preconditioner=pc, max_num_iterations=1))
+
+
+def test_learned_deflated_with_summed_group_partials_keeps_the_log():
+    # 1000 keyframes: 24 row groups, more than one LDS chunk of partials, so
+    # the last work-group of k_defl_partials sums them once per iteration;
+    # the solve repeats bit for bit and follows the deflated solver's LM log
+    # (to 1e-7: both stop at |r| <= 1e-10 |b| on this long, ill-conditioned
+    # walk, with different errors in the step; measured 3e-8 on the last).
+    m = problem.workload_map(2, seed=3, num_keyframes=1000, num_landmarks=20000)
+    assert (m.num_vertices + 41) // 42 > 22
+    base = dict(max_num_iterations=4, pcg_relative_tolerance=1e-10, pcg_max_iterations=3000)
+    logs = []
+    with DeviceProblem(m) as dev:
+        const_model((0.5, -4.0)).install(dev)
+        init_v, init_l = m.vertices.copy(), m.landmarks.copy()
+        for pc in (abi.PRECOND_LEARNED_DEFLATED, abi.PRECOND_LEARNED_DEFLATED, abi.PRECOND_DEFLATED):
+            dev.upload_state(init_v, init_l)
+            _, rows = dev.solve(opts(pcg_preconditioner=pc, **base))
+            logs.append([(r.cost, r.linear_solver_iterations) for r in rows])
+        st = learned.stats(dev)
+    print(f"learned+deflation {[i for _, i in logs[0]]}, deflated {[i for _, i in logs[2]]}", flush=True)
+    assert logs[0] == logs[1]
+    np.testing.assert_allclose([c for c, _ in logs[0]], [c for c, _ in logs[2]], rtol=1e-7)
+    assert st.fallbacks == 0
 
 
 def test_training_on_the_device_lowers_the_loss():

@@ -85,6 +85,49 @@ def test_trace_identities():
             assert abs(np.sum(T1 * T3) - np.sum(T2 * T2)) <= 1e-10 * max(1.0, np.sum(T2 * T2))
 
 
+def test_hat_basis_interpolates_linear_functions():
+    # Partition of unity and exact interpolation of the node index: the
+    # weights of vertex i sum to 1 and place it at s = i (K - 1) / (n - 1).
+    n, K = 37, 5
+    W = go.hat_basis(n, K)
+    for c in range(6):
+        Wc = W[c::6, c::6]
+        np.testing.assert_allclose(Wc.sum(axis=1), 1.0, rtol=0, atol=1e-15)
+        np.testing.assert_allclose(Wc @ np.arange(K), np.arange(n) * (K - 1) / (n - 1), rtol=0, atol=1e-13)
+        assert np.abs(W[c::6, [k for k in range(6 * K) if k % 6 != c]]).max() == 0.0
+    assert go.default_hat_nodes(49) == 3 and go.default_hat_nodes(31) == 0 and go.default_hat_nodes(2000) == 15
+
+
+def test_deflated_pcg_reduces_to_pcg_and_solves():
+    rng = np.random.default_rng(5)
+    S = block_spd(rng, n=40)
+    b = rng.normal(size=S.shape[0])
+    n = S.shape[0] // 6
+    G = go.kmatrix(S, rng.normal(size=(n, 2)) * 0.5)
+    Minv = lambda r: G.T @ (G @ r)  # noqa: E731
+    it0, _ = go.deflated_pcg(S, b, Minv, np.zeros((S.shape[0], 0)), tol=1e-8)
+    assert it0 == go.pcg_iterations(S, b, Minv, tol=1e-8)
+    it, x = go.deflated_pcg(S, b, Minv, go.hat_basis(n, 4), tol=1e-10)
+    assert it > 0
+    np.testing.assert_allclose(x, np.linalg.solve(S, b), rtol=1e-7, atol=1e-9 * np.abs(np.linalg.solve(S, b)).max())
+
+
+def test_deflation_of_all_but_m_eigenvectors_takes_m_iterations():
+    # W spanning every eigenvector of S but m leaves an operator with m
+    # distinct eigenvalues on the complement: CG ends in m steps. A dropped
+    # projection, a wrong sign or a transposed SW breaks this.
+    rng = np.random.default_rng(6)
+    Q, _ = np.linalg.qr(rng.normal(size=(30, 30)))
+    lam = np.linspace(1.0, 50.0, 30)
+    S = (Q * lam) @ Q.T
+    b = rng.normal(size=30)
+    for m in (1, 3):
+        W = Q[:, m:] @ rng.normal(size=(30 - m, 30 - m))  # any basis of that span
+        it, x = go.deflated_pcg(S, b, lambda r: r.copy(), W, tol=1e-10)
+        assert 0 < it <= m
+        np.testing.assert_allclose(x, np.linalg.solve(S, b), rtol=1e-8)
+
+
 def test_gradient_matches_finite_differences():
     rng = np.random.default_rng(4)
     g, qc = random_graph(rng)
