@@ -832,6 +832,17 @@ void scaleColumns(const Problem& pb, const std::vector<double>& s, Linearization
 
 // ------------------------------------------------------------ dense algebra
 
+// Dot product of two panel rows with eight independent partial sums (the
+// chains vectorize and overlap; the sum order is fixed).
+static inline double panel_dot(const double* a, const double* b, int w) {
+  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int t = 0;
+  for (; t + 8 <= w; t += 8)
+    for (int u = 0; u < 8; ++u) s[u] += a[t + u] * b[t + u];
+  for (; t < w; ++t) s[0] += a[t] * b[t];
+  return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+}
+
 // Blocked right-looking Cholesky of an n x n row-major SPD matrix (lower
 // factor in place). Returns false when not positive definite.
 bool cholesky(std::vector<double>& A, int n, int nthreads) {
@@ -860,15 +871,21 @@ bool cholesky(std::vector<double>& A, int n, int nthreads) {
           A[(size_t)i * n + j] = v / A[(size_t)j * n + j];
         }
     });
-    // Trailing update of the lower triangle: A_ij -= L_ik L_jk^T.
-    parallel_for(n - k1, nthreads, [&](int b, int e, int) {
-      for (int i = k1 + b; i < k1 + e; ++i) {
-        const double* li = &A[(size_t)i * n + k0];
-        for (int j = k1; j <= i; ++j) {
+    // Trailing update of the lower triangle: A_ij -= L_ik L_jk^T, four rows
+    // i at a time (each panel row j is read once per quad). Quads are taken
+    // from both ends in turn so that every thread's share of the triangle
+    // is about the same.
+    const int w = k1 - k0, nq = (n - k1 + 3) / 4;
+    parallel_for(nq, nthreads, [&](int b, int e, int) {
+      for (int idx = b; idx < e; ++idx) {
+        const int q = (idx & 1) ? nq - 1 - idx / 2 : idx / 2;
+        const int i0 = k1 + 4 * q, ni = std::min(4, n - i0);
+        for (int j = k1; j < i0 + ni; ++j) {
           const double* lj = &A[(size_t)j * n + k0];
-          double s = 0.0;
-          for (int t = 0; t < k1 - k0; ++t) s += li[t] * lj[t];
-          A[(size_t)i * n + j] -= s;
+          for (int r = std::max(0, j - i0); r < ni; ++r) {
+            const int i = i0 + r;
+            A[(size_t)i * n + j] -= panel_dot(&A[(size_t)i * n + k0], lj, w);
+          }
         }
       }
     });
