@@ -165,6 +165,26 @@ def test_lm_with_learned_preconditioner_keeps_the_log():
     assert out[abi.PRECOND_LEARNED_DEFLATED][2] < out[abi.PRECOND_LEARNED][2]
 
 
+def test_learned_deflated_without_coarse_columns_is_the_learned_pcg():
+    # 19 variable vertices: too few for the hat space (no coarse columns), so
+    # the deflated PCG with G'G is the learned PCG with other reduction orders.
+    m = walk(20, 2000, 6)
+    assert go.default_hat_nodes(m.num_vertices - 1) == 0
+    with DeviceProblem(m) as dev:
+        const_model((0.5, -4.0)).install(dev)
+        its = {pc: [learned.rcs_pcg_iterations(dev, opts(), r, pc, 1e-8) for r in (1e4, 1e8)]
+               for pc in (abi.PRECOND_LEARNED, abi.PRECOND_LEARNED_DEFLATED)}
+        logs = {}
+        for pc in (abi.PRECOND_LEARNED, abi.PRECOND_LEARNED_DEFLATED):
+            dev.upload_state(m.vertices, m.landmarks)
+            _, rows = dev.solve(opts(pcg_preconditioner=pc, max_num_iterations=5, pcg_relative_tolerance=1e-10))
+            logs[pc] = [r.cost for r in rows]
+    print(f"iterations to 1e-8: {its}", flush=True)
+    for a, b in zip(its[abi.PRECOND_LEARNED], its[abi.PRECOND_LEARNED_DEFLATED]):
+        assert a > 0 and abs(a - b) <= 1
+    np.testing.assert_allclose(logs[abi.PRECOND_LEARNED_DEFLATED], logs[abi.PRECOND_LEARNED], rtol=1e-9)
+
+
 @pytest.mark.parametrize("pc", [abi.PRECOND_LEARNED, abi.PRECOND_LEARNED_DEFLATED])
 def test_block_jacobi_fallback_when_the_learned_solve_does_not_converge(pc):
     m = walk(40, 4000, 4)
