@@ -185,44 +185,62 @@ __global__ __launch_bounds__(256) void k_defl_sr(PcgArgs a) {
 }
 
 // SW for the dense columns (or S Rn for the recycling), one block row per
-// work-group of six waves: wave comp, lane l sums over the row's blocks
-// l, l + 64, ... the products S_ij[comp][:] . R_c[6 j ..] for every dense
-// column c, then a fixed-order wave sum per column. Writes the group window
-// (dense part) or a plain column-major buffer.
+// work-group of six waves: wave w takes the dense columns 3 w .. 3 w + 2 for
+// all six components (so R_j is read once per block, not once per
+// component), lane l sums over the row's blocks l, l + 64, ... the products
+// S_ij[comp][:] . R_c[6 j ..], then a fixed-order wave sum per (component,
+// column). Writes the group window (dense part) or a plain column-major
+// buffer.
 constexpr int kDenseThreads = 384;
+constexpr int kDenseColsPerWave = (kMaxRitz + kDenseThreads / 64 - 1) / (kDenseThreads / 64);  // 3
 template <bool kWindow>
 __global__ __launch_bounds__(kDenseThreads) void k_row_dense(PcgArgs a, const int32_t* __restrict__ row_ptr,
                                                              const int32_t* __restrict__ col_var,
                                                              const double* __restrict__ S,
                                                              const double* __restrict__ R, int32_t nr, int32_t n6,
                                                              double* __restrict__ out) {
-  const int i = blockIdx.x, comp = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blockIdx.x, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int c0 = wave * kDenseColsPerWave;
+  if (c0 >= nr) return;  // the whole wave: no barrier in this kernel
   const int rb = row_ptr[i], re = row_ptr[i + 1];
-  double acc[kMaxRitz];
+  double acc[6][kDenseColsPerWave];
 #pragma unroll
-  for (int c = 0; c < kMaxRitz; ++c) acc[c] = 0.0;
+  for (int comp = 0; comp < 6; ++comp)
+#pragma unroll
+    for (int m = 0; m < kDenseColsPerWave; ++m) acc[comp][m] = 0.0;
   for (int kb = rb + lane; kb < re; kb += 64) {
-    const double* sv = S + (size_t)36 * kb + 6 * comp;
-    const double s0 = sv[0], s1 = sv[1], s2 = sv[2], s3 = sv[3], s4 = sv[4], s5 = sv[5];
+    const double* sb = S + (size_t)36 * kb;
+    double sv[36];
+#pragma unroll
+    for (int e = 0; e < 36; ++e) sv[e] = sb[e];
     const double* rj = R + 6 * col_var[kb];
 #pragma unroll
-    for (int c = 0; c < kMaxRitz; ++c) {
-      if (c < nr) {
-        const double* r = rj + (size_t)c * n6;
-        acc[c] += ((s0 * r[0] + s1 * r[1]) + (s2 * r[2] + s3 * r[3])) + (s4 * r[4] + s5 * r[5]);
+    for (int m = 0; m < kDenseColsPerWave; ++m) {
+      if (c0 + m < nr) {
+        const double* r = rj + (size_t)(c0 + m) * n6;
+        const double r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5];
+#pragma unroll
+        for (int comp = 0; comp < 6; ++comp) {
+          const double* s6 = sv + 6 * comp;
+          acc[comp][m] += ((s6[0] * r0 + s6[1] * r1) + (s6[2] * r2 + s6[3] * r3)) + (s6[4] * r4 + s6[5] * r5);
+        }
       }
     }
   }
 #pragma unroll
-  for (int c = 0; c < kMaxRitz; ++c) {
+  for (int m = 0; m < kDenseColsPerWave; ++m) {
+    const int c = c0 + m;
     if (c >= nr) break;
-    const double v = wave_sum(acc[c]);
-    if (lane == 0) {
-      if (kWindow) {
-        const int g = i / kRowsPerGroup;
-        sr_col(a, g, a.gwh + c)[6 * (i - g * kRowsPerGroup) + comp] = v;
-      } else {
-        out[(size_t)c * n6 + 6 * i + comp] = v;
+#pragma unroll
+    for (int comp = 0; comp < 6; ++comp) {
+      const double v = wave_sum(acc[comp][m]);
+      if (lane == 0) {
+        if (kWindow) {
+          const int g = i / kRowsPerGroup;
+          sr_col(a, g, a.gwh + c)[6 * (i - g * kRowsPerGroup) + comp] = v;
+        } else {
+          out[(size_t)c * n6 + 6 * i + comp] = v;
+        }
       }
     }
   }
