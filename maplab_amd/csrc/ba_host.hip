@@ -1975,11 +1975,10 @@ static int reject_outliers(mlba_handle* h, const mlba_options* o, int* removed) 
   for (int l = 0; l < h->L; ++l)
     if (bad[l]) { h->lm_active[l] = 0; ++n; }
   *removed = n;
-  if (n) {
-    // The system changed: recycled deflation vectors start again.
-    h->ritz_nr = 0;
-    h->ritz_stage = 0;
-  }
+  // The recycled deflation columns are kept: the next solve rebuilds E from
+  // the new S (and drops the deflation if E is not positive definite), so
+  // any columns are valid, and the slow modes of a map with a few landmarks
+  // removed are still close to them.
   return MLBA_OK;
 }
 
