@@ -1915,8 +1915,8 @@ int32_t mlba_rcs_pcg_iterations(mlba_handle* h, const mlba_options* o, double ra
     if ((rc = lp_setup(h, true, &use_k))) return rc;
     if (!use_k) return set_err(h, MLBA_EINVAL, "a row's lower neighbourhood does not fit the neighbourhood kernel");
     if ((rc = ensure_coarse_basis(h, o->pcg_coarse_nodes))) return rc;
-    h->ritz_nr = 0;
-    h->ritz_stage = 0;
+    // No recycled columns and no Lanczos vectors: the recycling state is
+    // left as it is.
     const int it = launch_pcg_deflated(s, lp_deflated_buffers(h), tolerance, max_iterations, kPcgPoll, kPcgPoll,
                                        nullptr, 0);
     if (it < 0) return set_err(h, MLBA_EDEVICE, "PCG launch failed");
