@@ -165,6 +165,34 @@ def test_lm_with_learned_preconditioner_keeps_the_log():
     assert out[abi.PRECOND_LEARNED_DEFLATED][2] < out[abi.PRECOND_LEARNED][2]
 
 
+def test_every_preconditioner_with_fp32_jacobians_keeps_the_log():
+    # The four solvers with FP64 and with FP32 Jacobian storage: each run
+    # follows the deflated solver's log of the same precision to 1e-7 (all
+    # stop at |r| <= 1e-10 |b| with different errors in the step; the last,
+    # rejected candidate's cost moved 1.1e-8). FP32 against FP64: the large
+    # first step (cost 5.2e6 to 1.7e6) differs by 3.5e-6 and the logs meet
+    # again (2.2e-8 at the last iteration), so the log is held to 1e-5 and
+    # the final cost to the 1e-6 bar.
+    m = walk(60, 12000, 2)
+    base = dict(max_num_iterations=6, pcg_relative_tolerance=1e-10, pcg_max_iterations=2000)
+    pcs = (abi.PRECOND_DEFLATED, abi.PRECOND_BLOCK_JACOBI, abi.PRECOND_LEARNED, abi.PRECOND_LEARNED_DEFLATED)
+    logs = {}
+    with DeviceProblem(m) as dev:
+        const_model((0.5, -4.0)).install(dev)
+        for f32 in (0, 1):
+            for pc in pcs:
+                dev.upload_state(m.vertices, m.landmarks)
+                _, rows = dev.solve(opts(pcg_preconditioner=pc, jacobian_fp32=f32, **base))
+                logs[(pc, f32)] = [r.cost for r in rows]
+        assert learned.stats(dev).fallbacks == 0
+    for f32 in (0, 1):
+        for pc in pcs[1:]:
+            np.testing.assert_allclose(logs[(pc, f32)], logs[(abi.PRECOND_DEFLATED, f32)], rtol=1e-7)
+    f32, f64 = logs[(abi.PRECOND_DEFLATED, 1)], logs[(abi.PRECOND_DEFLATED, 0)]
+    np.testing.assert_allclose(f32, f64, rtol=1e-5)
+    assert abs(f32[-1] - f64[-1]) <= 1e-6 * f64[-1]
+
+
 def test_learned_deflated_without_coarse_columns_is_the_learned_pcg():
     # 19 variable vertices: too few for the hat space (no coarse columns), so
     # the deflated PCG with G'G is the learned PCG with other reduction orders.
