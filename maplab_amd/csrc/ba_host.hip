@@ -1197,24 +1197,25 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
                                      h->parts_l.p, h->f32_jac);
       }
       // Candidate point and its cost; the step's scalars (model cost change,
-      // step and state norms) in one batch of reductions.
+      // step and state norms) and the cost in one batch of reductions.
       launch_candidate(s, h->topo, h->vert.p, h->w.p, h->vert_cand.p, h->parts_v.p, h->lm.p, h->lm_u.p,
                        h->lm_cand.p, h->parts_c.p);
-      ReduceBatch rb{};
-      rb.d[0] = {nbs, 2, 0, 0, h->parts_l.p, h->scal.p + kMcc};
-      rb.d[1] = {nbs, 2, 1, 0, h->parts_l.p, h->scal.p + kUU};
-      rb.d[2] = {wb, 1, 0, 0, h->parts_w.p, h->scal.p + kWW};
-      rb.d[3] = {pb, 1, 0, 0, h->parts_v.p, h->scal.p + kXV};
-      rb.d[4] = {lb, 1, 0, 0, h->parts_c.p, h->scal.p + kXL};
-      rb.count = 5;
-      if (!launch_reduce_many(s, rb)) return set_err(h, MLBA_EINVAL, "internal: scalar reduction batch size");
       launch_vertex_prep(s, h->P, h->vert_cand.p, h->vR_cand.p);
       {
         KernelTimer kt(h, MLBA_KERNEL_COST);
         launch_linearize(s, false, h->tiles, h->topo, h->vert_cand.p, h->vR_cand.p, h->lm_cand.p, make_loss(o),
                          h->tile_begin.p, nullptr, nullptr, nullptr, h->tile_cost2.p);
       }
-      launch_reduce(s, h->tiles, 2, 0, false, h->tile_cost2.p, h->scal.p + kCandCost);
+      // The step's scalars and the candidate cost in one launch.
+      ReduceBatch rb{};
+      rb.d[0] = {nbs, 2, 0, 0, h->parts_l.p, h->scal.p + kMcc};
+      rb.d[1] = {nbs, 2, 1, 0, h->parts_l.p, h->scal.p + kUU};
+      rb.d[2] = {wb, 1, 0, 0, h->parts_w.p, h->scal.p + kWW};
+      rb.d[3] = {pb, 1, 0, 0, h->parts_v.p, h->scal.p + kXV};
+      rb.d[4] = {lb, 1, 0, 0, h->parts_c.p, h->scal.p + kXL};
+      rb.d[5] = {h->tiles, 2, 0, 0, h->tile_cost2.p, h->scal.p + kCandCost};
+      rb.count = 6;
+      if (!launch_reduce_many(s, rb)) return set_err(h, MLBA_EINVAL, "internal: scalar reduction batch size");
       HIP_OK(hipGetLastError());
       int32_t* fail_pinned = reinterpret_cast<int32_t*>(h->scal_host + kNumSlots);
       HIP_OK(hipMemcpyAsync(fail_pinned, h->fail.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
