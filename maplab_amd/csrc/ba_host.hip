@@ -77,7 +77,8 @@ int32_t launch_back_substitute(hipStream_t s, const DevTopology& t, int32_t tile
 void launch_unscale_poses(hipStream_t s, int32_t Pvar, const double* y, const double* pose_scale, double* w,
                           double* part);
 void launch_candidate(hipStream_t s, const DevTopology& t, const double* vert, const double* w, double* cand_v,
-                      double* part_v, const double* lm, const double* u, double* cand_l, double* part_l);
+                      double* cand_vR, double* part_v, const double* lm, const double* u, double* cand_l,
+                      double* part_l);
 void launch_gradient_norm(hipStream_t s, const DevTopology& t, const double* vert, const double* pose_grad,
                           const double* lm_acc, double* part);
 bool launch_reduce_many(hipStream_t s, const ReduceBatch& batch);
@@ -1198,9 +1199,8 @@ int minimize(mlba_handle* h, const mlba_options* o, double initial_radius, mlba_
       }
       // Candidate point and its cost; the step's scalars (model cost change,
       // step and state norms) and the cost in one batch of reductions.
-      launch_candidate(s, h->topo, h->vert.p, h->w.p, h->vert_cand.p, h->parts_v.p, h->lm.p, h->lm_u.p,
-                       h->lm_cand.p, h->parts_c.p);
-      launch_vertex_prep(s, h->P, h->vert_cand.p, h->vR_cand.p);
+      launch_candidate(s, h->topo, h->vert.p, h->w.p, h->vert_cand.p, h->vR_cand.p, h->parts_v.p, h->lm.p,
+                       h->lm_u.p, h->lm_cand.p, h->parts_c.p);
       {
         KernelTimer kt(h, MLBA_KERNEL_COST);
         launch_linearize(s, false, h->tiles, h->topo, h->vert_cand.p, h->vR_cand.p, h->lm_cand.p, make_loss(o),

@@ -82,17 +82,20 @@ __device__ inline void anchor_map(const double* pB, const double* Ra, double n2,
 
 // ------------------------------------------------------------ vertices
 
-__global__ void k_vertex_prep(int32_t P, const double* __restrict__ vert, double* __restrict__ vR) {
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= P) return;
-  const double* q = vert + 7 * v;
+// The 12-double rotation record of a vertex: R_I_M (JPL, 9), |q|^2, 0, 0.
+__device__ inline void vertex_rotation(const double* q, double* __restrict__ out) {
   double R[9];
   rot_jpl(q, R);
-  double* out = vR + 12 * v;
   for (int i = 0; i < 9; ++i) out[i] = R[i];
   out[9] = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
   out[10] = 0.0;
   out[11] = 0.0;
+}
+
+__global__ void k_vertex_prep(int32_t P, const double* __restrict__ vert, double* __restrict__ vR) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= P) return;
+  vertex_rotation(vert + 7 * v, vR + 12 * v);
 }
 
 // ------------------------------------------------------------ linearization
@@ -1179,27 +1182,32 @@ __global__ __launch_bounds__(256) void k_unscale_poses(int32_t Pvar, const doubl
 
 // Candidate state x (+) delta with delta = -w (vertices, JPL Plus) and
 // -u (landmarks); partials of |x|^2 over variable blocks.
+// Candidate vertex x [+] (-w) and its rotation record (as k_vertex_prep).
 __global__ __launch_bounds__(256) void k_candidate_vertices(DevTopology t, const double* __restrict__ vert,
                                                             const double* __restrict__ w,
-                                                            double* __restrict__ cand, double* __restrict__ part) {
+                                                            double* __restrict__ cand, double* __restrict__ cand_vR,
+                                                            double* __restrict__ part) {
   __shared__ double lds4[4];
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   double xx = 0.0;
   if (v < t.P) {
     const double* x = vert + 7 * v;
-    double* out = cand + 7 * v;
+    double c[7];
     const int32_t j = t.vertex_var[v];
     if (j >= 0) {
       const double* wv = w + 6 * j;
       const double d[3] = {-wv[0], -wv[1], -wv[2]};
-      quat_plus_jpl(x, d, out);
-      out[4] = x[4] - wv[3];
-      out[5] = x[5] - wv[4];
-      out[6] = x[6] - wv[5];
+      quat_plus_jpl(x, d, c);
+      c[4] = x[4] - wv[3];
+      c[5] = x[5] - wv[4];
+      c[6] = x[6] - wv[5];
       for (int k = 0; k < 7; ++k) xx += x[k] * x[k];
     } else {
-      for (int k = 0; k < 7; ++k) out[k] = x[k];
+      for (int k = 0; k < 7; ++k) c[k] = x[k];
     }
+    double* out = cand + 7 * v;
+    for (int k = 0; k < 7; ++k) out[k] = c[k];
+    vertex_rotation(c, cand_vR + 12 * v);
   }
   const double s = block_sum256(xx, lds4);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
@@ -1630,8 +1638,9 @@ void launch_unscale_poses(hipStream_t s, int32_t Pvar, const double* y, const do
   if (Pvar > 0) k_unscale_poses<<<(6 * Pvar + 255) / 256, 256, 0, s>>>(Pvar, y, pose_scale, w, part);
 }
 void launch_candidate(hipStream_t s, const DevTopology& t, const double* vert, const double* w, double* cand_v,
-                      double* part_v, const double* lm, const double* u, double* cand_l, double* part_l) {
-  if (t.P > 0) k_candidate_vertices<<<(t.P + 255) / 256, 256, 0, s>>>(t, vert, w, cand_v, part_v);
+                      double* cand_vR, double* part_v, const double* lm, const double* u, double* cand_l,
+                      double* part_l) {
+  if (t.P > 0) k_candidate_vertices<<<(t.P + 255) / 256, 256, 0, s>>>(t, vert, w, cand_v, cand_vR, part_v);
   if (t.L > 0) k_candidate_landmarks<<<(t.L + 255) / 256, 256, 0, s>>>(t, lm, u, cand_l, part_l);
 }
 void launch_gradient_norm(hipStream_t s, const DevTopology& t, const double* vert, const double* pose_grad,
