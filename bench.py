@@ -148,7 +148,8 @@ def multimap(args):
         kf = int(round(np.exp(rng.uniform(np.log(50), np.log(800)))))
         lm = int(round(np.exp(rng.uniform(np.log(10000), np.log(150000)))))
         kw = {} if i % 2 == 0 else dict(distortion_type=2, distortion=(-0.01, 0.02, -0.005, 0.001))
-        maps.append(problem.workload_map(2, seed=args.seed + 1 + i, num_keyframes=kf, num_landmarks=lm, **kw))
+        maps.append(problem.workload_map(2, seed=args.seed + 1 + i, num_keyframes=kf, num_landmarks=lm,
+                                         detection_probability=0.5, **kw))
     n_obs = sum(m.num_observations for m in maps)
     print(f"bench: {len(maps)} maps, {sum(m.num_vertices for m in maps)} keyframes, {n_obs} observations",
           file=sys.stderr, flush=True)
