@@ -29,7 +29,7 @@ namespace mlba {
 constexpr int kTileBand = 16;                        // block rows (and columns) of a tile
 constexpr int kTileBlocks = kTileBand * kTileBand;   // blocks of a tile = threads of a work-group
 constexpr int kBatchSlots = 192;                     // factor blocks staged in LDS per batch (<= 256)
-constexpr int kSegmentSlots = 8192;                  // staged factor blocks per segment (about)
+constexpr int kSegmentSlots = 5120;                  // staged factor blocks per segment (about)
 constexpr int kSegmentBatches = 64;                  // batches per segment, at most
 constexpr int kBatchPairs = 2048;                    // pairs per batch, at most
 constexpr int kPairWords = kBatchPairs / 2 / kTileBlocks;  // 32-bit pair words a thread stages per batch
