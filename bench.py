@@ -207,14 +207,24 @@ def multimap(args):
 
 
 def cpu_model():
-    """Host CPU model name (first 'model name' of /proc/cpuinfo)."""
+    """Host CPU model name (first 'model name' of /proc/cpuinfo) with its
+    family / model / stepping numbers (some hosts report only a generic
+    name)."""
+    fields = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" not in line:
+                if fields:
+                    break
+                continue
+            k, v = (x.strip() for x in line.split(":", 1))
+            if k in ("model name", "cpu family", "model", "stepping") and k not in fields:
+                fields[k] = v
     except OSError:
         pass
-    return "unknown"
+    name = fields.get("model name", "unknown")
+    ids = [f"{k} {fields[k]}" for k in ("cpu family", "model", "stepping") if k in fields]
+    return f"{name} ({', '.join(ids)})" if ids else name
 
 
 def learned_model():
